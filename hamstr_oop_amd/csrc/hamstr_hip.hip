@@ -1,0 +1,1708 @@
+// hamstr_hip.hip -- MI355X (gfx950) implementation of HAMSTR_OOP's
+// line-implicit steady Euler iteration behind the C ABI of include/hamstr_hip.h.
+//
+// Device data layout (per context):
+//   q, r, r0, dq      double [nCell][4]   AoS, the reference's layout
+//   sigma             double [nCell]
+//   per processed face p (chain order, closed chains without their repeated
+//   first face): flux [p][4], sr [p], lflux [p][4], normal [p][2],
+//   ff  T [32][nProc]  element-major (lmat row-major then rmat), T = double or
+//                      float (FP32-LHS)
+//   chain slot table  int2 [nSlot]: the source of every row the reference
+//                     gathers into f[] (computeRHS.cpp:57-260): a cell, a cell
+//                     reflected at a wall face, or the freestream state.
+//   cell CSR          the faces of every cell in the reference's
+//                     accumulation order, so residual sums are bit-identical.
+// Kernels per iteration: rhs_faces (reconstruction + Roe flux + Jacobians),
+// rhs_cells (residual/sigma sums, norms, ADI scaling), one ADI kernel per
+// chain phase (block-tridiagonal solves), accumulate dq, lin_faces
+// (linearised fluxes), lin_cells (linear residual, norms, q += dq).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/hamstr_hip.h"
+#include "device_math.hpp"
+
+using namespace hamstr;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(x)                                                                           \
+  do {                                                                                      \
+    hipError_t e_ = (x);                                                                    \
+    if (e_ != hipSuccess)                                                                   \
+      return fail(HAMSTR_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));         \
+  } while (0)
+
+constexpr int kBlock = 256;
+constexpr int kRedBlock = 256;
+
+inline int nblk(long n, int b = kBlock) { return (int)((n + b - 1) / b); }
+
+// --------------------------------------------------------------------------
+// Device tables
+// --------------------------------------------------------------------------
+struct Consts {
+  double cfl, gamma, qt, eps;
+  double qinf[4];
+  int nCell, nProc, order;
+};
+
+struct FaceTab {
+  const int* pslot;      // slot of row n (reference index), bit 30: last face of an open chain
+  const int2* src;       // per slot: (cell, wall face) ; cell < 0: freestream ; face < 0: copy
+  const double* nvec;    // [nProc][2]
+  const double* refmtx;  // [nFace][4]
+};
+
+struct ChainTab {
+  const int* pbase;   // first processed face of the chain
+  const int* nf;      // processed faces
+  const int* ncell;   // rows of the block system
+  const int* periodic;
+  const int* pleft;   // per processed face: left cell
+  const int* pright;  // per processed face: right cell (or boundary id)
+  const int* pface;   // per processed face: face id
+  const double* refmtx;
+};
+
+__device__ __forceinline__ double4 load_state(const int2 e, const double* __restrict__ q,
+                                              const double* __restrict__ refmtx, const double4 qinf) {
+  if (e.x < 0) return qinf;
+  double4 v = *reinterpret_cast<const double4*>(q + 4 * (size_t)e.x);
+  if (e.y >= 0) {
+    const double4 R = *reinterpret_cast<const double4*>(refmtx + 4 * (size_t)e.y);
+    const double a = v.y, b = v.z;
+    v.y = (a * R.x + b * R.y);
+    v.z = (a * R.z + b * R.w);
+  }
+  return v;
+}
+
+// same rows for dq: the freestream ghost has dq = 0 (computeLinearRHS.cpp:262-270)
+__device__ __forceinline__ double4 load_dstate(const int2 e, const double* __restrict__ dq,
+                                               const double* __restrict__ refmtx) {
+  return load_state(e, dq, refmtx, make_double4(0.0, 0.0, 0.0, 0.0));
+}
+
+__device__ __forceinline__ void put4(double* d, const double4 v) {
+  d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
+}
+
+// ql[n] and qr[n+1] along the chain for one face (reconstruction.cpp)
+template <int ORDER>
+__device__ __forceinline__ void reconstruct_face(const double (*f)[4], double eps, double qt, double* qln,
+                                                 double* qrn1) {
+  // f[] holds slots n-2 .. n+3 at indices 0..5
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    if (ORDER == 1) {
+      qln[k] = f[2][k];
+      qrn1[k] = f[3][k];
+    } else if (ORDER == 3) {
+      double a, b;
+      muscl_pt(f[1][k], f[2][k], f[3][k], qt, eps, &qln[k], &a);
+      muscl_pt(f[2][k], f[3][k], f[4][k], qt, eps, &b, &qrn1[k]);
+    } else {
+      qln[k] = weno5(f[0][k], f[1][k], f[2][k], f[3][k], f[4][k], 1e-15);
+      qrn1[k] = weno5(f[5][k], f[4][k], f[3][k], f[2][k], f[1][k], 1e-15);
+    }
+  }
+}
+
+template <int ORDER>
+__device__ __forceinline__ void reconstruct_dface(const double (*f)[4], const double (*d)[4], double eps,
+                                                  double qt, double* dln, double* drn1) {
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    if (ORDER == 1) {
+      dln[k] = d[2][k];
+      drn1[k] = d[3][k];
+    } else if (ORDER == 3) {
+      double a, b;
+      muscl_deriv_pt(f[1][k], f[2][k], f[3][k], d[1][k], d[2][k], d[3][k], qt, eps, &dln[k], &a);
+      muscl_deriv_pt(f[2][k], f[3][k], f[4][k], d[2][k], d[3][k], d[4][k], qt, eps, &b, &drn1[k]);
+    } else {
+      dln[k] = weno5_deriv(f[0][k], f[1][k], f[2][k], f[3][k], f[4][k], d[0][k], d[1][k], d[2][k], d[3][k],
+                           d[4][k], eps);
+      drn1[k] = weno5_deriv(f[5][k], f[4][k], f[3][k], f[2][k], f[1][k], d[5][k], d[4][k], d[3][k], d[2][k],
+                            d[1][k], eps);
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// rhs_faces: chain gather + reconstruction + Roe flux (+ Jacobians)
+// computeRHS.cpp:53-408
+// --------------------------------------------------------------------------
+template <int ORDER, typename TL, bool IMPL>
+__global__ __launch_bounds__(kBlock) void k_rhs_faces(FaceTab ft, Consts c, const double* __restrict__ q,
+                                                      double* __restrict__ flux, double* __restrict__ sr,
+                                                      TL* __restrict__ ff, int* __restrict__ nanflag) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= c.nProc) return;
+  const int code = ft.pslot[p];
+  const int s = code & 0x3fffffff;
+  const bool flip = (code >> 30) & 1;
+  constexpr int lo = (ORDER == 5) ? 0 : (ORDER == 3 ? 1 : 2);
+  constexpr int hi = (ORDER == 5) ? 5 : (ORDER == 3 ? 4 : 3);
+  const double4 qinf = make_double4(c.qinf[0], c.qinf[1], c.qinf[2], c.qinf[3]);
+  double f[6][4];
+#pragma unroll
+  for (int j = lo; j <= hi; j++) put4(f[j], load_state(ft.src[s - 2 + j], q, ft.refmtx, qinf));
+  double qln[4], qrn1[4];
+  reconstruct_face<ORDER>(f, c.eps, c.qt, qln, qrn1);
+  double left[4], right[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    left[k] = flip ? qln[k] : qrn1[k];
+    right[k] = flip ? qrn1[k] : qln[k];
+  }
+  const double2 n = *reinterpret_cast<const double2*>(ft.nvec + 2 * (size_t)p);
+  double fl[4], s_r;
+  flux_roe2d(n.x, n.y, left, right, c.gamma, fl, &s_r);
+  *reinterpret_cast<double4*>(flux + 4 * (size_t)p) = make_double4(fl[0], fl[1], fl[2], fl[3]);
+  sr[p] = s_r;
+  if (s_r != s_r) atomicOr(nanflag, 1);
+  if (IMPL) {
+    TL lm[16], rm[16];
+    jacobian_roe2d<TL>(n.x, n.y, left, right, c.gamma, lm, rm);
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+      ff[(size_t)e * c.nProc + p] = lm[e];
+      ff[(size_t)(16 + e) * c.nProc + p] = rm[e];
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// block reductions for the density-residual norms
+// --------------------------------------------------------------------------
+__device__ __forceinline__ void block_norm_partial(double sq, double mx, double* partial) {
+  __shared__ double ssq[kBlock];
+  __shared__ double smx[kBlock];
+  const int t = threadIdx.x;
+  ssq[t] = sq;
+  smx[t] = mx;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if (t < w) {
+      ssq[t] += ssq[t + w];
+      smx[t] = fmax(smx[t], smx[t + w]);
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    partial[2 * blockIdx.x] = ssq[0];
+    partial[2 * blockIdx.x + 1] = smx[0];
+  }
+}
+
+// L2 = sqrt(sum r_rho^2) / nCell, Linf = max |r_rho| (computeRHS.cpp:415-425)
+__global__ __launch_bounds__(kRedBlock) void k_norm_final(const double* __restrict__ partial, int nPart,
+                                                          int nCell, double* __restrict__ out) {
+  __shared__ double ssq[kRedBlock];
+  __shared__ double smx[kRedBlock];
+  const int t = threadIdx.x;
+  double a = 0.0, m = 0.0;
+  for (int i = t; i < nPart; i += kRedBlock) {
+    a += partial[2 * i];
+    m = fmax(m, partial[2 * i + 1]);
+  }
+  ssq[t] = a;
+  smx[t] = m;
+  __syncthreads();
+  for (int w = kRedBlock / 2; w > 0; w >>= 1) {
+    if (t < w) {
+      ssq[t] += ssq[t + w];
+      smx[t] = fmax(smx[t], smx[t + w]);
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    out[0] = sqrt(ssq[0]) / nCell;
+    out[1] = smx[0];
+  }
+}
+
+// --------------------------------------------------------------------------
+// rhs_cells: r and sigma summed in the reference's order, norms, and (in the
+// fused iteration) ADI's scaling r *= CFL/sigma, r0 = r (ADI.cpp:40-49)
+// --------------------------------------------------------------------------
+template <bool ADI_INIT>
+__global__ __launch_bounds__(kBlock) void k_rhs_cells(Consts c, const int* __restrict__ off,
+                                                      const int* __restrict__ ent,
+                                                      const double* __restrict__ flux,
+                                                      const double* __restrict__ sr, double* __restrict__ r,
+                                                      double* __restrict__ r0, double* __restrict__ sigma,
+                                                      double* __restrict__ partial) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double sq = 0.0, mx = 0.0;
+  if (i < c.nCell) {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, sg = 0.0;
+    const int e1 = off[i + 1];
+    for (int e = off[i]; e < e1; e++) {
+      const int code = ent[e];
+      const int p = code >> 1;
+      const double4 fx = *reinterpret_cast<const double4*>(flux + 4 * (size_t)p);
+      if (code & 1) {
+        a0 += fx.x; a1 += fx.y; a2 += fx.z; a3 += fx.w;
+      } else {
+        a0 -= fx.x; a1 -= fx.y; a2 -= fx.z; a3 -= fx.w;
+      }
+      sg += sr[p];
+    }
+    sq = a0 * a0;
+    mx = fabs(a0);
+    sigma[i] = sg;
+    if (ADI_INIT) {
+      const double dtfac = c.cfl / sg;
+      const double4 v = make_double4(a0 * dtfac, a1 * dtfac, a2 * dtfac, a3 * dtfac);
+      *reinterpret_cast<double4*>(r + 4 * (size_t)i) = v;
+      *reinterpret_cast<double4*>(r0 + 4 * (size_t)i) = v;
+    } else {
+      *reinterpret_cast<double4*>(r + 4 * (size_t)i) = make_double4(a0, a1, a2, a3);
+    }
+  }
+  block_norm_partial(sq, mx, partial);
+}
+
+// ADI.cpp:40-49 on its own (API path)
+__global__ void k_adi_init(Consts c, double* __restrict__ r, double* __restrict__ r0, double* __restrict__ dq,
+                           const double* __restrict__ sigma) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= c.nCell) return;
+  const double dtfac = c.cfl / sigma[i];
+  double4 v = *reinterpret_cast<double4*>(r + 4 * (size_t)i);
+  v.x *= dtfac; v.y *= dtfac; v.z *= dtfac; v.w *= dtfac;
+  *reinterpret_cast<double4*>(r + 4 * (size_t)i) = v;
+  *reinterpret_cast<double4*>(r0 + 4 * (size_t)i) = v;
+  *reinterpret_cast<double4*>(dq + 4 * (size_t)i) = make_double4(0.0, 0.0, 0.0, 0.0);
+}
+
+// dq += r (ADI.cpp:247-248); with FIRST the old dq is zero
+template <bool FIRST>
+__global__ void k_accum_dq(int n4, const double* __restrict__ r, double* __restrict__ dq) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n4) return;
+  if (FIRST)
+    dq[i] = 0.0 + r[i];
+  else
+    dq[i] += r[i];
+}
+
+// --------------------------------------------------------------------------
+// ADI block-tridiagonal row assembly (ADI.cpp:57-217) for row m of a chain
+// --------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void load_fmat(const T* __restrict__ ff, int nProc, int p, T* L, T* R) {
+#pragma unroll
+  for (int e = 0; e < 16; e++) {
+    L[e] = ff[(size_t)e * nProc + p];
+    R[e] = ff[(size_t)(16 + e) * nProc + p];
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void reflect_cols(T* R, const double* rm) {
+  const T a1 = T(rm[0]), b1 = T(rm[1]), a2 = T(rm[2]), b2 = T(rm[3]);
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const T r01 = R[4 * j + 1], r02 = R[4 * j + 2];
+    R[4 * j + 1] = a1 * r01 + b1 * r02;
+    R[4 * j + 2] = a2 * r01 + b2 * r02;
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void assemble_row(const ChainTab& ct, const T* __restrict__ ff, int nProc, int pb,
+                                             int nf, int N, bool periodic, int m, double cfl,
+                                             const double* __restrict__ sigma, const double* __restrict__ r,
+                                             T* A, T* B, T* C, T* F, int* cell_out) {
+  const int pm = pb + m;
+  T L[16], R[16];
+  load_fmat(ff, nProc, pm, L, R);
+  if (periodic) {
+    const int pn = pb + ((m + 1 == N) ? 0 : m + 1);
+    T Ln[16], Rn[16];
+    load_fmat(ff, nProc, pn, Ln, Rn);
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+      B[e] = L[e] - Rn[e];
+      A[e] = R[e];
+      C[e] = T(0) - Ln[e];
+    }
+  } else {
+    const int rc = ct.pright[pm];
+    if (rc == -2) reflect_cols(R, ct.refmtx + 4 * (size_t)ct.pface[pm]);
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+      B[e] = L[e];
+      A[e] = T(0);
+      C[e] = T(0);
+    }
+    if (m > 0 && rc > -1) {
+#pragma unroll
+      for (int e = 0; e < 16; e++) A[e] = R[e];
+    } else if (rc == -2) {
+#pragma unroll
+      for (int e = 0; e < 16; e++) B[e] += R[e];
+    }
+    if (m + 1 < nf) {
+      const int pn = pm + 1;
+      const int rn = ct.pright[pn];
+      load_fmat(ff, nProc, pn, L, R);
+      if (rn == -2) reflect_cols(R, ct.refmtx + 4 * (size_t)ct.pface[pn]);
+      if (m + 1 == nf - 1 && rn < 0) {
+#pragma unroll
+        for (int e = 0; e < 16; e++) B[e] += L[e];
+        if (rn == -2) {
+#pragma unroll
+          for (int e = 0; e < 16; e++) B[e] += R[e];
+        }
+      } else if (rn > -1) {
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+          B[e] -= R[e];
+          C[e] = T(0) - L[e];
+        }
+      }
+    }
+  }
+  const int cell = ct.pleft[pm];
+  *cell_out = cell;
+  const T dtfac = T(cfl / sigma[cell]);
+#pragma unroll
+  for (int e = 0; e < 16; e++) {
+    A[e] *= dtfac;
+    B[e] *= dtfac;
+    C[e] *= dtfac;
+  }
+  B[0] += T(1.0);
+  B[5] += T(1.0);
+  B[10] += T(1.0);
+  B[15] += T(1.0);
+  const double4 rv = *reinterpret_cast<const double4*>(r + 4 * (size_t)cell);
+  F[0] = T(rv.x);
+  F[1] = T(rv.y);
+  F[2] = T(rv.z);
+  F[3] = T(rv.w);
+}
+
+// --------------------------------------------------------------------------
+// ADI phase, exact order: one thread per chain runs the reference's block
+// Thomas (blockTridag4) or cyclic variant (blockTridagPeriodic4) and writes
+// the solution back into r (ADI.cpp:226-244).  Scratch rows hold C', F' and
+// the spike a' for the back sweep: element e of row i of chain t lives at
+// scratch[(i*36+e)*stride + t].
+// --------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(64) void k_adi_exact(ChainTab ct, const int* __restrict__ chains, int nch,
+                                                  const T* __restrict__ ff, int nProc, double cfl,
+                                                  const double* __restrict__ sigma, double* __restrict__ r,
+                                                  T* __restrict__ scratch, int stride) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nch) return;
+  const int ch = chains[t];
+  const int pb = ct.pbase[ch], nf = ct.nf[ch], N = ct.ncell[ch];
+  const bool periodic = ct.periodic[ch] != 0;
+  auto S = [&](int i, int e) -> T& { return scratch[((size_t)i * 36 + e) * stride + t]; };
+
+  T Cp[16], Fp[4], ap[16];
+  for (int i = 0; i < N; i++) {
+    T A[16], B[16], C[16], F[4], a[16];
+    int cell;
+    assemble_row<T>(ct, ff, nProc, pb, nf, N, periodic, i, cfl, sigma, r, A, B, C, F, &cell);
+    if (i != 0) {
+      mat_sub_mul(B, A, Cp);
+      vec_sub_mul(F, A, Fp);
+      if (periodic) {
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            T x = T(0);
+#pragma unroll
+            for (int l = 0; l < 4; l++) x -= A[4 * k + l] * ap[4 * l + j];
+            a[4 * k + j] = x;
+          }
+      }
+    } else if (periodic) {
+#pragma unroll
+      for (int e = 0; e < 16; e++) a[e] = A[e];
+    }
+    const LU4<T> L = lu4(B);
+    lu4_solve(L, F, 1);
+    if (!periodic) {
+      if (i != N - 1) {
+#pragma unroll
+        for (int j = 0; j < 4; j++) lu4_solve(L, C + j, 4);
+      }
+    } else if (i != N - 2) {
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        lu4_solve(L, C + j, 4);
+        lu4_solve(L, a + j, 4);
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < 16; e++) C[e] += a[e];
+#pragma unroll
+      for (int j = 0; j < 4; j++) lu4_solve(L, C + j, 4);
+#pragma unroll
+      for (int e = 0; e < 16; e++) a[e] = C[e];
+    }
+    if (i < N - 1) {
+#pragma unroll
+      for (int e = 0; e < 16; e++) S(i, e) = C[e];
+#pragma unroll
+      for (int e = 0; e < 4; e++) S(i, 16 + e) = F[e];
+      if (periodic) {
+#pragma unroll
+        for (int e = 0; e < 16; e++) S(i, 20 + e) = a[e];
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+      Cp[e] = C[e];
+      ap[e] = a[e];
+    }
+#pragma unroll
+    for (int e = 0; e < 4; e++) Fp[e] = F[e];
+  }
+
+  auto put = [&](int i, const T* x) {
+    const int cell = ct.pleft[pb + i];
+    *reinterpret_cast<double4*>(r + 4 * (size_t)cell) = make_double4(double(x[0]), double(x[1]), double(x[2]),
+                                                                     double(x[3]));
+  };
+  if (!periodic) {
+    T x[4] = {Fp[0], Fp[1], Fp[2], Fp[3]};
+    put(N - 1, x);
+    for (int i = N - 2; i >= 0; i--) {
+      T c[16], fv[4];
+#pragma unroll
+      for (int e = 0; e < 16; e++) c[e] = S(i, e);
+#pragma unroll
+      for (int e = 0; e < 4; e++) fv[e] = S(i, 16 + e);
+      vec_sub_mul(fv, c, x);
+#pragma unroll
+      for (int e = 0; e < 4; e++) x[e] = fv[e];
+      put(i, x);
+    }
+  } else {
+    // bordered last row (triSolvers.cpp:966-1029)
+    T Bl[16], Cl[16], Fl[4];
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+      Bl[e] = (e % 5 == 0) ? T(1.0) : T(0.0);
+      Cl[e] = Cp[e];
+    }
+#pragma unroll
+    for (int e = 0; e < 4; e++) Fl[e] = Fp[e];
+    for (int i = 0; i < N - 1; i++) {
+      T c[16], a[16], fv[4], tt[16];
+#pragma unroll
+      for (int e = 0; e < 16; e++) {
+        c[e] = S(i, e);
+        a[e] = S(i, 20 + e);
+      }
+#pragma unroll
+      for (int e = 0; e < 4; e++) fv[e] = S(i, 16 + e);
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          T x = T(0);
+          T b = Bl[4 * k + j];
+#pragma unroll
+          for (int l = 0; l < 4; l++) {
+            x -= Cl[4 * k + l] * c[4 * l + j];
+            b -= Cl[4 * k + l] * a[4 * l + j];
+          }
+          tt[4 * k + j] = x;
+          Bl[4 * k + j] = b;
+        }
+      vec_sub_mul(Fl, Cl, fv);
+#pragma unroll
+      for (int e = 0; e < 16; e++) Cl[e] = tt[e];
+    }
+    const LU4<T> L = lu4(Bl);
+    lu4_solve(L, Fl, 1);
+    put(N - 1, Fl);
+    T x[4];
+    {
+      T c[16], fv[4];
+#pragma unroll
+      for (int e = 0; e < 16; e++) c[e] = S(N - 2, e);
+#pragma unroll
+      for (int e = 0; e < 4; e++) fv[e] = S(N - 2, 16 + e);
+      vec_sub_mul(fv, c, Fl);
+#pragma unroll
+      for (int e = 0; e < 4; e++) x[e] = fv[e];
+      put(N - 2, x);
+    }
+    for (int i = N - 3; i >= 0; i--) {
+      T c[16], a[16], fv[4];
+#pragma unroll
+      for (int e = 0; e < 16; e++) {
+        c[e] = S(i, e);
+        a[e] = S(i, 20 + e);
+      }
+#pragma unroll
+      for (int e = 0; e < 4; e++) fv[e] = S(i, 16 + e);
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        T v = fv[k];
+#pragma unroll
+        for (int j = 0; j < 4; j++) v -= (c[4 * k + j] * x[j] + a[4 * k + j] * Fl[j]);
+        fv[k] = v;
+      }
+#pragma unroll
+      for (int e = 0; e < 4; e++) x[e] = fv[e];
+      put(i, x);
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
+// lin_faces: reconstruction derivatives + lmat*dqL + rmat*dqR per face
+// (computeLinearRHS.cpp:52-325)
+// --------------------------------------------------------------------------
+template <int ORDER, typename TL>
+__global__ __launch_bounds__(kBlock) void k_lin_faces(FaceTab ft, Consts c, const double* __restrict__ q,
+                                                      const double* __restrict__ dq, const TL* __restrict__ ff,
+                                                      double* __restrict__ lflux) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= c.nProc) return;
+  const int code = ft.pslot[p];
+  const int s = code & 0x3fffffff;
+  const bool flip = (code >> 30) & 1;
+  constexpr int lo = (ORDER == 5) ? 0 : (ORDER == 3 ? 1 : 2);
+  constexpr int hi = (ORDER == 5) ? 5 : (ORDER == 3 ? 4 : 3);
+  const double4 qinf = make_double4(c.qinf[0], c.qinf[1], c.qinf[2], c.qinf[3]);
+  double f[6][4], d[6][4];
+#pragma unroll
+  for (int j = lo; j <= hi; j++) {
+    const int2 e = ft.src[s - 2 + j];
+    if (ORDER != 1) put4(f[j], load_state(e, q, ft.refmtx, qinf));
+    put4(d[j], load_dstate(e, dq, ft.refmtx));
+  }
+  double dln[4], drn1[4];
+  reconstruct_dface<ORDER>(f, d, c.eps, c.qt, dln, drn1);
+  double dl[4], dr[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    dl[k] = flip ? dln[k] : drn1[k];
+    dr[k] = flip ? drn1[k] : dln[k];
+  }
+  double out[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    double acc = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const double lm = double(ff[(size_t)(4 * j + k) * c.nProc + p]);
+      const double rm = double(ff[(size_t)(16 + 4 * j + k) * c.nProc + p]);
+      acc += ((lm * dl[k]) + (rm * dr[k]));
+    }
+    out[j] = acc;
+  }
+  *reinterpret_cast<double4*>(lflux + 4 * (size_t)p) = make_double4(out[0], out[1], out[2], out[3]);
+}
+
+// lin_cells: r = r0 - dq - CFL/sigma * sum(+-lflux), norms, optional q += dq
+// (computeLinearRHS.cpp:45, 326-368; ADI.cpp:258-264)
+template <bool UPDATE>
+__global__ __launch_bounds__(kBlock) void k_lin_cells(Consts c, const int* __restrict__ off,
+                                                      const int* __restrict__ ent,
+                                                      const double* __restrict__ lflux,
+                                                      const double* __restrict__ r0, const double* __restrict__ dq,
+                                                      const double* __restrict__ sigma, double* __restrict__ r,
+                                                      double* __restrict__ q, double* __restrict__ partial) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double sq = 0.0, mx = 0.0;
+  if (i < c.nCell) {
+    const double4 a = *reinterpret_cast<const double4*>(r0 + 4 * (size_t)i);
+    const double4 b = *reinterpret_cast<const double4*>(dq + 4 * (size_t)i);
+    double a0 = (a.x - b.x), a1 = (a.y - b.y), a2 = (a.z - b.z), a3 = (a.w - b.w);
+    const double dtfac = c.cfl / sigma[i];
+    const int e1 = off[i + 1];
+    for (int e = off[i]; e < e1; e++) {
+      const int code = ent[e];
+      const int p = code >> 1;
+      const double4 fx = *reinterpret_cast<const double4*>(lflux + 4 * (size_t)p);
+      if (code & 1) {
+        a0 += (fx.x * dtfac); a1 += (fx.y * dtfac); a2 += (fx.z * dtfac); a3 += (fx.w * dtfac);
+      } else {
+        a0 -= (fx.x * dtfac); a1 -= (fx.y * dtfac); a2 -= (fx.z * dtfac); a3 -= (fx.w * dtfac);
+      }
+    }
+    *reinterpret_cast<double4*>(r + 4 * (size_t)i) = make_double4(a0, a1, a2, a3);
+    sq = a0 * a0;
+    mx = fabs(a0);
+    if (UPDATE) {
+      double4 qv = *reinterpret_cast<double4*>(q + 4 * (size_t)i);
+      qv.x += b.x; qv.y += b.y; qv.z += b.z; qv.w += b.w;
+      *reinterpret_cast<double4*>(q + 4 * (size_t)i) = qv;
+    }
+  }
+  block_norm_partial(sq, mx, partial);
+}
+
+__global__ void k_update(int n4, double* __restrict__ q, const double* __restrict__ dq) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n4) q[i] += dq[i];
+}
+
+__global__ void k_fill_state(int nCell, double4 v, double* __restrict__ q) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nCell) *reinterpret_cast<double4*>(q + 4 * (size_t)i) = v;
+}
+
+// --------------------------------------------------------------------------
+// Unit kernels (per-function parity)
+// --------------------------------------------------------------------------
+__global__ void k_unit_flux(int n, const double* nv, const double* ql, const double* qr, double gamma,
+                            double* flux, double* sr, int* nanflag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double f[4], s;
+  flux_roe2d(nv[2 * i], nv[2 * i + 1], ql + 4 * i, qr + 4 * i, gamma, f, &s);
+  for (int k = 0; k < 4; k++) flux[4 * i + k] = f[k];
+  sr[i] = s;
+  if (s != s) atomicOr(nanflag, 1);
+}
+
+template <typename T>
+__global__ void k_unit_jac(int n, const double* nv, const double* ql, const double* qr, double gamma,
+                           double* lm, double* rm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T L[16], R[16];
+  jacobian_roe2d<T>(nv[2 * i], nv[2 * i + 1], ql + 4 * i, qr + 4 * i, gamma, L, R);
+  for (int k = 0; k < 16; k++) {
+    lm[16 * i + k] = double(L[k]);
+    rm[16 * i + k] = double(R[k]);
+  }
+}
+
+// kind: 0 MUSCL, 1 WENO, 2 MUSCL_DERIV, 3 WENO_DERIV.  One thread per (chain, row).
+__global__ void k_unit_recon(int kind, int nchain, int imax, int is, int iend, double qt, double eps,
+                             const double* f, const double* dq, double* ql, double* qr) {
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nrow = iend - is;
+  if (id >= nchain * nrow * 4) return;
+  const int k = id % 4;
+  const int i = is + (id / 4) % nrow;
+  const int ch = id / (4 * nrow);
+  const double* F = f + (size_t)ch * imax * 4;
+  auto at = [&](const double* a, int row) { return a[(size_t)row * 4 + k]; };
+  double a = 0, b = 0;
+  if (kind == 0) {
+    if (qt == 0.0) {
+      a = b = at(F, i);
+    } else {
+      muscl_pt(at(F, i - 1), at(F, i), at(F, i + 1), qt, eps, &a, &b);
+    }
+  } else if (kind == 1) {
+    a = weno5(at(F, i - 2), at(F, i - 1), at(F, i), at(F, i + 1), at(F, i + 2), 1e-15);
+    b = weno5(at(F, i + 2), at(F, i + 1), at(F, i), at(F, i - 1), at(F, i - 2), 1e-15);
+  } else {
+    const double* D = dq + (size_t)ch * imax * 4;
+    if (kind == 2) {
+      if (qt == 0.0) {
+        a = b = at(D, i);
+      } else {
+        muscl_deriv_pt(at(F, i - 1), at(F, i), at(F, i + 1), at(D, i - 1), at(D, i), at(D, i + 1), qt, eps, &a,
+                       &b);
+      }
+    } else {
+      a = weno5_deriv(at(F, i - 2), at(F, i - 1), at(F, i), at(F, i + 1), at(F, i + 2), at(D, i - 2),
+                      at(D, i - 1), at(D, i), at(D, i + 1), at(D, i + 2), eps);
+      b = weno5_deriv(at(F, i + 2), at(F, i + 1), at(F, i), at(F, i - 1), at(F, i - 2), at(D, i + 2),
+                      at(D, i + 1), at(D, i), at(D, i - 1), at(D, i - 2), eps);
+    }
+  }
+  ql[((size_t)ch * imax + i) * 4 + k] = a;
+  qr[((size_t)ch * imax + i) * 4 + k] = b;
+}
+
+// blockTridag4 / blockTridagPeriodic4 on stored systems, thread per system
+__global__ void k_unit_tridag(int nsys, int N, int periodic, const double* a_, const double* b_,
+                              const double* c_, double* f_, double* scratch) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nsys) return;
+  const double* a0 = a_ + (size_t)t * N * 16;
+  const double* b0 = b_ + (size_t)t * N * 16;
+  const double* c0 = c_ + (size_t)t * N * 16;
+  double* f0 = f_ + (size_t)t * N * 4;
+  double* S = scratch + (size_t)t * N * 36;
+  double Cp[16], Fp[4], ap[16];
+  for (int i = 0; i < N; i++) {
+    double A[16], B[16], C[16], F[4], a[16];
+    for (int e = 0; e < 16; e++) {
+      A[e] = a0[16 * i + e];
+      B[e] = b0[16 * i + e];
+      C[e] = c0[16 * i + e];
+    }
+    for (int e = 0; e < 4; e++) F[e] = f0[4 * i + e];
+    if (i != 0) {
+      mat_sub_mul(B, A, Cp);
+      vec_sub_mul(F, A, Fp);
+      if (periodic) {
+        for (int k = 0; k < 4; k++)
+          for (int j = 0; j < 4; j++) {
+            double x = 0.0;
+            for (int l = 0; l < 4; l++) x -= A[4 * k + l] * ap[4 * l + j];
+            a[4 * k + j] = x;
+          }
+      }
+    } else if (periodic) {
+      for (int e = 0; e < 16; e++) a[e] = A[e];
+    }
+    const LU4<double> L = lu4(B);
+    lu4_solve(L, F, 1);
+    if (!periodic) {
+      if (i != N - 1)
+        for (int j = 0; j < 4; j++) lu4_solve(L, C + j, 4);
+    } else if (i != N - 2) {
+      for (int j = 0; j < 4; j++) {
+        lu4_solve(L, C + j, 4);
+        lu4_solve(L, a + j, 4);
+      }
+    } else {
+      for (int e = 0; e < 16; e++) C[e] += a[e];
+      for (int j = 0; j < 4; j++) lu4_solve(L, C + j, 4);
+      for (int e = 0; e < 16; e++) a[e] = C[e];
+    }
+    for (int e = 0; e < 16; e++) S[36 * i + e] = C[e];
+    for (int e = 0; e < 4; e++) S[36 * i + 16 + e] = F[e];
+    for (int e = 0; e < 16; e++) S[36 * i + 20 + e] = a[e];
+    for (int e = 0; e < 16; e++) {
+      Cp[e] = C[e];
+      ap[e] = a[e];
+    }
+    for (int e = 0; e < 4; e++) Fp[e] = F[e];
+  }
+  if (!periodic) {
+    double x[4] = {Fp[0], Fp[1], Fp[2], Fp[3]};
+    for (int e = 0; e < 4; e++) f0[4 * (N - 1) + e] = x[e];
+    for (int i = N - 2; i >= 0; i--) {
+      double fv[4];
+      for (int e = 0; e < 4; e++) fv[e] = S[36 * i + 16 + e];
+      vec_sub_mul(fv, &S[36 * i], x);
+      for (int e = 0; e < 4; e++) x[e] = f0[4 * i + e] = fv[e];
+    }
+    return;
+  }
+  double Bl[16], Cl[16], Fl[4];
+  for (int e = 0; e < 16; e++) {
+    Bl[e] = (e % 5 == 0) ? 1.0 : 0.0;
+    Cl[e] = Cp[e];
+  }
+  for (int e = 0; e < 4; e++) Fl[e] = Fp[e];
+  for (int i = 0; i < N - 1; i++) {
+    const double* c = &S[36 * i];
+    const double* a = &S[36 * i + 20];
+    double tt[16];
+    for (int k = 0; k < 4; k++)
+      for (int j = 0; j < 4; j++) {
+        double x = 0.0, b = Bl[4 * k + j];
+        for (int l = 0; l < 4; l++) {
+          x -= Cl[4 * k + l] * c[4 * l + j];
+          b -= Cl[4 * k + l] * a[4 * l + j];
+        }
+        tt[4 * k + j] = x;
+        Bl[4 * k + j] = b;
+      }
+    vec_sub_mul(Fl, Cl, &S[36 * i + 16]);
+    for (int e = 0; e < 16; e++) Cl[e] = tt[e];
+  }
+  const LU4<double> L = lu4(Bl);
+  lu4_solve(L, Fl, 1);
+  for (int e = 0; e < 4; e++) f0[4 * (N - 1) + e] = Fl[e];
+  double x[4];
+  {
+    double fv[4];
+    for (int e = 0; e < 4; e++) fv[e] = S[36 * (N - 2) + 16 + e];
+    vec_sub_mul(fv, &S[36 * (N - 2)], Fl);
+    for (int e = 0; e < 4; e++) x[e] = f0[4 * (N - 2) + e] = fv[e];
+  }
+  for (int i = N - 3; i >= 0; i--) {
+    const double* c = &S[36 * i];
+    const double* a = &S[36 * i + 20];
+    double fv[4];
+    for (int k = 0; k < 4; k++) {
+      double v = S[36 * i + 16 + k];
+      for (int j = 0; j < 4; j++) v -= (c[4 * k + j] * x[j] + a[4 * k + j] * Fl[j]);
+      fv[k] = v;
+    }
+    for (int e = 0; e < 4; e++) x[e] = f0[4 * i + e] = fv[e];
+  }
+}
+
+// --------------------------------------------------------------------------
+// host helpers
+// --------------------------------------------------------------------------
+template <typename V>
+int upload(V** dst, const std::vector<V>& src) {
+  *dst = nullptr;
+  if (src.empty()) return HAMSTR_OK;
+  HIPCHK(hipMalloc(dst, sizeof(V) * src.size()));
+  HIPCHK(hipMemcpy(*dst, src.data(), sizeof(V) * src.size(), hipMemcpyHostToDevice));
+  return HAMSTR_OK;
+}
+
+struct DevBuf {
+  void* p = nullptr;
+};
+
+}  // namespace
+
+// ==========================================================================
+// context
+// ==========================================================================
+struct hamstr_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hamstr_params prm{};
+  int nCell = 0, nFace = 0, nChain = 0, nProc = 0, nSlot = 0, nGhost = 2;
+  Consts c{};
+  double L2rhs = 0, Linfrhs = 0, L2lin = 0, Linflin = 0;
+  // device arrays
+  double *q = nullptr, *r = nullptr, *r0 = nullptr, *dq = nullptr, *sigma = nullptr;
+  double *flux = nullptr, *sr = nullptr, *lflux = nullptr;
+  void* ff = nullptr;
+  int* pslot = nullptr;
+  int2* src = nullptr;
+  double *nvec = nullptr, *refmtx = nullptr;
+  int *pleft = nullptr, *pright = nullptr, *pface = nullptr;
+  int *cell_off = nullptr, *cell_ent = nullptr;
+  int *ch_pbase = nullptr, *ch_nf = nullptr, *ch_ncell = nullptr, *ch_periodic = nullptr;
+  int* phase_chains = nullptr;
+  std::vector<int> phase_off;       // host: offsets into phase_chains
+  std::vector<int> phase_maxrows;
+  void* scratch = nullptr;
+  size_t scratch_stride = 0;
+  double* partial = nullptr;
+  int nPart = 0;
+  double* norms = nullptr;  // [4]: L2rhs, Linfrhs, L2lin, Linflin
+  double* hist = nullptr;   // [2*histCap]
+  int histCap = 0;
+  int* nanflag = nullptr;
+  std::vector<int> proc_face;  // host: face id per processed face
+  // timing
+  bool timing = false;
+  std::vector<hipEvent_t> ev;
+  std::vector<double> last_ms;
+
+  FaceTab ftab() const { return FaceTab{pslot, src, nvec, refmtx}; }
+  ChainTab ctab() const {
+    return ChainTab{ch_pbase, ch_nf, ch_ncell, ch_periodic, pleft, pright, pface, refmtx};
+  }
+  bool lhs32() const { return prm.lhs == HAMSTR_LHS_FP32; }
+};
+
+namespace {
+
+int launch_rhs_faces(hamstr_ctx* x, bool implicit) {
+  const FaceTab ft = x->ftab();
+  dim3 g(nblk(x->nProc)), b(kBlock);
+#define RHSF(O, T, I) \
+  hipLaunchKernelGGL((k_rhs_faces<O, T, I>), g, b, 0, x->stream, ft, x->c, x->q, x->flux, x->sr, (T*)x->ff, x->nanflag)
+  const int o = x->prm.order;
+  if (x->lhs32()) {
+    if (implicit) {
+      if (o == 1) RHSF(1, float, true); else if (o == 3) RHSF(3, float, true); else RHSF(5, float, true);
+    } else {
+      if (o == 1) RHSF(1, float, false); else if (o == 3) RHSF(3, float, false); else RHSF(5, float, false);
+    }
+  } else {
+    if (implicit) {
+      if (o == 1) RHSF(1, double, true); else if (o == 3) RHSF(3, double, true); else RHSF(5, double, true);
+    } else {
+      if (o == 1) RHSF(1, double, false); else if (o == 3) RHSF(3, double, false); else RHSF(5, double, false);
+    }
+  }
+#undef RHSF
+  HIPCHK(hipGetLastError());
+  return HAMSTR_OK;
+}
+
+int launch_lin_faces(hamstr_ctx* x) {
+  const FaceTab ft = x->ftab();
+  dim3 g(nblk(x->nProc)), b(kBlock);
+#define LINF(O, T) \
+  hipLaunchKernelGGL((k_lin_faces<O, T>), g, b, 0, x->stream, ft, x->c, x->q, x->dq, (const T*)x->ff, x->lflux)
+  const int o = x->prm.order;
+  if (x->lhs32()) {
+    if (o == 1) LINF(1, float); else if (o == 3) LINF(3, float); else LINF(5, float);
+  } else {
+    if (o == 1) LINF(1, double); else if (o == 3) LINF(3, double); else LINF(5, double);
+  }
+#undef LINF
+  HIPCHK(hipGetLastError());
+  return HAMSTR_OK;
+}
+
+int launch_norm(hamstr_ctx* x, double* out) {
+  hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(kRedBlock), 0, x->stream, x->partial, x->nPart, x->nCell, out);
+  HIPCHK(hipGetLastError());
+  return HAMSTR_OK;
+}
+
+int launch_adi_phases(hamstr_ctx* x) {
+  const ChainTab ct = x->ctab();
+  for (size_t ph = 0; ph + 1 < x->phase_off.size(); ph++) {
+    const int n = x->phase_off[ph + 1] - x->phase_off[ph];
+    const int* chains = x->phase_chains + x->phase_off[ph];
+    if (x->lhs32())
+      hipLaunchKernelGGL((k_adi_exact<float>), dim3(nblk(n, 64)), dim3(64), 0, x->stream, ct, chains, n,
+                         (const float*)x->ff, x->nProc, x->c.cfl, x->sigma, x->r, (float*)x->scratch,
+                         (int)x->scratch_stride);
+    else
+      hipLaunchKernelGGL((k_adi_exact<double>), dim3(nblk(n, 64)), dim3(64), 0, x->stream, ct, chains, n,
+                         (const double*)x->ff, x->nProc, x->c.cfl, x->sigma, x->r, (double*)x->scratch,
+                         (int)x->scratch_stride);
+    HIPCHK(hipGetLastError());
+  }
+  return HAMSTR_OK;
+}
+
+// computeLinearRHS; update: also q += dq
+int launch_linear(hamstr_ctx* x, bool update, double* normout) {
+  int rc = launch_lin_faces(x);
+  if (rc) return rc;
+  dim3 g(nblk(x->nCell)), b(kBlock);
+  if (update)
+    hipLaunchKernelGGL((k_lin_cells<true>), g, b, 0, x->stream, x->c, x->cell_off, x->cell_ent, x->lflux, x->r0,
+                       x->dq, x->sigma, x->r, x->q, x->partial);
+  else
+    hipLaunchKernelGGL((k_lin_cells<false>), g, b, 0, x->stream, x->c, x->cell_off, x->cell_ent, x->lflux,
+                       x->r0, x->dq, x->sigma, x->r, x->q, x->partial);
+  HIPCHK(hipGetLastError());
+  return launch_norm(x, normout);
+}
+
+// the ADI sweeps after r/r0 have been scaled (ADI.cpp:55-264)
+int launch_sweeps(hamstr_ctx* x, double* linnorm) {
+  const int n4 = 4 * x->nCell;
+  for (int s = 0; s < x->prm.msweep; s++) {
+    int rc = launch_adi_phases(x);
+    if (rc) return rc;
+    if (s == 0)
+      hipLaunchKernelGGL((k_accum_dq<true>), dim3(nblk(n4)), dim3(kBlock), 0, x->stream, n4, x->r, x->dq);
+    else
+      hipLaunchKernelGGL((k_accum_dq<false>), dim3(nblk(n4)), dim3(kBlock), 0, x->stream, n4, x->r, x->dq);
+    HIPCHK(hipGetLastError());
+    rc = launch_linear(x, s == x->prm.msweep - 1, linnorm);
+    if (rc) return rc;
+  }
+  return HAMSTR_OK;
+}
+
+int read_norms(hamstr_ctx* x) {
+  double h[4];
+  HIPCHK(hipMemcpyAsync(h, x->norms, sizeof(h), hipMemcpyDeviceToHost, x->stream));
+  int flag = 0;
+  HIPCHK(hipMemcpyAsync(&flag, x->nanflag, sizeof(int), hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  x->L2rhs = h[0];
+  x->Linfrhs = h[1];
+  x->L2lin = h[2];
+  x->Linflin = h[3];
+  if (flag) return fail(HAMSTR_ERR_NAN, "NaN spectral radius in the Roe flux");
+  return HAMSTR_OK;
+}
+
+void free_ctx(hamstr_ctx* x) {
+  void* ptrs[] = {x->q, x->r, x->r0, x->dq, x->sigma, x->flux, x->sr, x->lflux, x->ff, x->pslot, x->src,
+                  x->nvec, x->refmtx, x->pleft, x->pright, x->pface, x->cell_off, x->cell_ent, x->ch_pbase,
+                  x->ch_nf, x->ch_ncell, x->ch_periodic, x->phase_chains, x->scratch, x->partial, x->norms,
+                  x->hist, x->nanflag};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  for (auto e : x->ev) (void)hipEventDestroy(e);
+  if (x->stream) (void)hipStreamDestroy(x->stream);
+}
+
+}  // namespace
+
+// ==========================================================================
+// C ABI
+// ==========================================================================
+extern "C" {
+
+const char* hamstr_last_error(void) { return g_err.c_str(); }
+
+int hamstr_preprocess(int nNode, const double* x, int nFace, int* faces, double* normVec, double* refMtx,
+                      int sol_visc) {
+  if (nNode <= 0 || nFace <= 0 || !x || !faces || !normVec || !refMtx)
+    return fail(HAMSTR_ERR_ARG, "hamstr_preprocess: bad arguments");
+  for (int i = 0; i < nFace; i++) {
+    const int i1 = faces[6 * i], i2 = faces[6 * i + 1];
+    if (i1 < 0 || i1 >= nNode || i2 < 0 || i2 >= nNode) return fail(HAMSTR_ERR_ARG, "face node out of range");
+    const double x1 = x[2 * i1], y1 = x[2 * i1 + 1], x2 = x[2 * i2], y2 = x[2 * i2 + 1];
+    const double n0 = (y2 - y1), n1 = -(x2 - x1);
+    normVec[2 * i] = n0;
+    normVec[2 * i + 1] = n1;
+    double dsnorm = n0 * n0 + n1 * n1;
+    dsnorm = 1. / dsnorm;
+    refMtx[4 * i + 0] = 1. - 2. * n0 * n0 * dsnorm;
+    refMtx[4 * i + 1] = -2. * n0 * n1 * dsnorm;
+    refMtx[4 * i + 2] = -2. * n1 * n0 * dsnorm;
+    refMtx[4 * i + 3] = 1. - 2. * n1 * n1 * dsnorm;
+    if (faces[6 * i + 4] == -1) {
+      const double r1 = std::sqrt(x[2 * i1] * x[2 * i1] + x[2 * i1 + 1] * x[2 * i1 + 1]);
+      const double r2 = std::sqrt(x[2 * i2] * x[2 * i2] + x[2 * i2 + 1] * x[2 * i2 + 1]);
+      if (0.5 * (r1 + r2) < 3.0) faces[6 * i + 4] = -(sol_visc + 2);
+    }
+  }
+  return HAMSTR_OK;
+}
+
+int hamstr_create(const hamstr_mesh_desc* mesh, const hamstr_params* params, hamstr_ctx** out) {
+  if (!mesh || !params || !out) return fail(HAMSTR_ERR_ARG, "hamstr_create: null argument");
+  *out = nullptr;
+  const hamstr_params P = *params;
+  if (P.order != 1 && P.order != 3 && P.order != 5) return fail(HAMSTR_ERR_ARG, "order must be 1, 3 or 5");
+  if (P.msweep < 1) return fail(HAMSTR_ERR_ARG, "msweep must be >= 1");
+  if (P.lhs != HAMSTR_LHS_FP64 && P.lhs != HAMSTR_LHS_FP32) return fail(HAMSTR_ERR_ARG, "bad lhs precision");
+  if (P.adi_mode != HAMSTR_ADI_EXACT && P.adi_mode != HAMSTR_ADI_FAST) return fail(HAMSTR_ERR_ARG, "bad adi mode");
+  const int nCell = mesh->nCell, nFace = mesh->nFace, nChain = mesh->nChain;
+  if (nCell <= 0 || nFace <= 0 || nChain <= 0 || !mesh->faces || !mesh->chainStart || !mesh->chainConn ||
+      !mesh->normVec || !mesh->refMtx)
+    return fail(HAMSTR_ERR_ARG, "hamstr_create: bad mesh description");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(HAMSTR_ERR_NODEV, "no HIP device");
+
+  const int* faces = mesh->faces;
+  const int* fs = mesh->chainStart;
+  const int* cc = mesh->chainConn;
+  const int nGhost = (P.order == 5) ? 3 : 2;
+  const int nChainFace = fs[nChain];
+  if (fs[0] != 0 || nChainFace <= 0) return fail(HAMSTR_ERR_MESH, "chainStart must start at 0");
+  for (int i = 0; i < nChain; i++)
+    if (fs[i + 1] - fs[i] < 2) return fail(HAMSTR_ERR_MESH, "chain with fewer than 2 faces");
+  for (int f = 0; f < nChainFace; f++)
+    if (cc[f] < 0 || cc[f] >= nFace) return fail(HAMSTR_ERR_MESH, "chainConn entry out of range");
+  for (int i = 0; i < nFace; i++) {
+    const int L = faces[6 * i + 2], R = faces[6 * i + 4];
+    if (L < 0 || L >= nCell || R >= nCell) return fail(HAMSTR_ERR_MESH, "face cell out of range");
+    if (R == -5) return fail(HAMSTR_ERR_MESH, "inter-domain faces (-5) are not supported");
+  }
+
+  // ---- chains, slots and processed faces (computeRHS.cpp:53-298) ----
+  std::vector<int> ch_pbase(nChain), ch_nf(nChain), ch_ncell(nChain), ch_periodic(nChain);
+  std::vector<int2> src;
+  std::vector<int> pslot, pleft, pright, pface;
+  std::vector<double> nvec;
+  std::vector<int> proc_count(nFace, 0);
+  std::vector<int> cindx;
+  for (int i = 0; i < nChain; i++) {
+    const int f1 = fs[i], f2 = fs[i + 1];
+    const int periodic = (cc[f1] == cc[f2 - 1]);
+    const int nf = f2 - f1 - periodic;
+    const int ncell = f2 - f1 - 1;
+    if (periodic && ncell < 3) return fail(HAMSTR_ERR_MESH, "closed chain with fewer than 3 cells");
+    if (!periodic && ncell < 1) return fail(HAMSTR_ERR_MESH, "open chain without cells");
+    // cindx exactly as the reference builds it (cell 0 quirks included)
+    cindx.assign((f2 - f1) + 8, 0);
+    int m = nGhost;
+    for (int f = f1; f < f2; f++) cindx[m++] = faces[6 * cc[f] + 2];
+    int chainSize;
+    if (periodic) {
+      cindx[m++] = faces[6 * cc[f1 + 1] + 2];
+      chainSize = m;
+      m = 0;
+      for (int f = f2 - nGhost - 1; f < f2 - 1; f++) cindx[m++] = faces[6 * cc[f] + 2];
+    } else if (P.order == 5) {
+      m--;
+      cindx[m] = -cindx[m];
+      m++;
+      cindx[m] = -cindx[m - 3];
+      m++;
+      cindx[m] = -cindx[m - 5];
+      chainSize = m + 1;
+      cindx[0] = -cindx[5];
+      cindx[1] = -cindx[4];
+      cindx[2] = -cindx[3];
+    } else {
+      m--;
+      cindx[m] = -cindx[m];
+      m++;
+      cindx[m] = -cindx[m - 3];
+      chainSize = m + 1;
+      cindx[0] = -cindx[3];
+      cindx[1] = -cindx[2];
+    }
+    const int slotBase = (int)src.size();
+    for (int j = 0; j < chainSize; j++) {
+      int icell = cindx[j];
+      int2 e = make_int2(icell, -1);
+      const bool copy = icell >= 0 || (icell == 0 && j == nGhost) || (icell == 0 && j == chainSize - nGhost - 1);
+      const bool ghost = icell < 0 || (icell == 0 && j == nGhost - 1) || (icell == 0 && j == chainSize - nGhost);
+      if (!copy && !ghost) return fail(HAMSTR_ERR_MESH, "internal: slot without source");
+      if (ghost) {
+        const int iface = (j < nGhost) ? cc[f1] : cc[f2 - 1];
+        const int rightCell = faces[6 * iface + 4];
+        if (rightCell == -2)
+          e = make_int2(-icell, iface);
+        else
+          e = make_int2(-1, -1);  // freestream
+      }
+      src.push_back(e);
+    }
+    ch_pbase[i] = (int)pslot.size();
+    ch_nf[i] = nf;
+    ch_ncell[i] = ncell;
+    ch_periodic[i] = periodic;
+    const int is = nGhost - 1;
+    for (int k = 0; k < nf; k++) {
+      const int f = f1 + k;
+      const int iface = cc[f];
+      const int flip = (!periodic && f == f2 - 1) ? 1 : 0;
+      const int n = slotBase + is + k;
+      pslot.push_back(n | (flip << 30));
+      pleft.push_back(faces[6 * iface + 2]);
+      pright.push_back(faces[6 * iface + 4]);
+      pface.push_back(iface);
+      nvec.push_back(mesh->normVec[2 * iface]);
+      nvec.push_back(mesh->normVec[2 * iface + 1]);
+      if (++proc_count[iface] > 1) return fail(HAMSTR_ERR_MESH, "face processed by more than one chain");
+    }
+  }
+  const int nProc = (int)pslot.size();
+  const int nSlot = (int)src.size();
+  if (nSlot >= (1 << 30)) return fail(HAMSTR_ERR_MESH, "too many chain slots");
+
+  // ---- per-cell accumulation lists in processing order ----
+  std::vector<int> cnt(nCell + 1, 0);
+  for (int p = 0; p < nProc; p++) {
+    cnt[pleft[p] + 1]++;
+    if (pright[p] > -1) cnt[pright[p] + 1]++;
+  }
+  for (int i = 0; i < nCell; i++) cnt[i + 1] += cnt[i];
+  std::vector<int> cell_ent(cnt[nCell]);
+  std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+  for (int p = 0; p < nProc; p++) {
+    cell_ent[fill[pleft[p]]++] = (p << 1) | 0;
+    if (pright[p] > -1) cell_ent[fill[pright[p]]++] = (p << 1) | 1;
+  }
+
+  // ---- ADI phases: chain i runs after every earlier chain sharing a cell ----
+  std::vector<int> last(nCell, -1), phase(nChain, 0);
+  int nPhase = 0;
+  for (int i = 0; i < nChain; i++) {
+    int ph = 0;
+    for (int k = 0; k < ch_ncell[i]; k++) ph = std::max(ph, last[pleft[ch_pbase[i] + k]] + 1);
+    for (int k = 0; k < ch_ncell[i]; k++) last[pleft[ch_pbase[i] + k]] = ph;
+    phase[i] = ph;
+    nPhase = std::max(nPhase, ph + 1);
+  }
+  std::vector<int> phase_chains;
+  std::vector<int> phase_off(1, 0), phase_maxrows;
+  size_t maxPhaseChains = 0, maxRows = 0;
+  for (int ph = 0; ph < nPhase; ph++) {
+    int mr = 0;
+    for (int i = 0; i < nChain; i++)
+      if (phase[i] == ph) {
+        phase_chains.push_back(i);
+        mr = std::max(mr, ch_ncell[i]);
+      }
+    phase_off.push_back((int)phase_chains.size());
+    phase_maxrows.push_back(mr);
+    maxPhaseChains = std::max(maxPhaseChains, (size_t)(phase_off[ph + 1] - phase_off[ph]));
+    maxRows = std::max(maxRows, (size_t)mr);
+  }
+
+  hamstr_ctx* x = new hamstr_ctx();
+  x->prm = P;
+  x->nCell = nCell;
+  x->nFace = nFace;
+  x->nChain = nChain;
+  x->nProc = nProc;
+  x->nSlot = nSlot;
+  x->nGhost = nGhost;
+  x->phase_off = phase_off;
+  x->phase_maxrows = phase_maxrows;
+  x->proc_face = pface;
+  // freestream (initFlow.cpp:107-142)
+  const double gamma = 1.4, gm1 = gamma - 1., rinf = 1.0, pinf = 1.0 / gamma;
+  const double PI = 3.1415926535897932e+00;
+  const double uinf = P.mach * std::cos(P.alpha * (PI / 180.0));
+  const double vinf = P.mach * std::sin(P.alpha * (PI / 180.0));
+  const double einf = pinf / (gm1) + 0.5 * rinf * (uinf * uinf + vinf * vinf);
+  x->c.cfl = P.cfl;
+  x->c.gamma = gamma;
+  x->c.qt = (P.order == 1) ? 0.0 : 0.25;
+  x->c.eps = 1e-10;
+  x->c.qinf[0] = rinf;
+  x->c.qinf[1] = rinf * uinf;
+  x->c.qinf[2] = rinf * vinf;
+  x->c.qinf[3] = einf;
+  x->c.nCell = nCell;
+  x->c.nProc = nProc;
+  x->c.order = P.order;
+
+  int rc = HAMSTR_OK;
+  auto chk = [&](hipError_t e, const char* what) {
+    if (e != hipSuccess && rc == HAMSTR_OK) rc = fail(HAMSTR_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+  };
+  chk(hipGetDevice(&x->device), "hipGetDevice");
+  chk(hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking), "hipStreamCreate");
+  const size_t c4 = sizeof(double) * 4 * nCell;
+  chk(hipMalloc(&x->q, c4), "alloc q");
+  chk(hipMalloc(&x->r, c4), "alloc r");
+  chk(hipMalloc(&x->r0, c4), "alloc r0");
+  chk(hipMalloc(&x->dq, c4), "alloc dq");
+  chk(hipMalloc(&x->sigma, sizeof(double) * nCell), "alloc sigma");
+  chk(hipMalloc(&x->flux, sizeof(double) * 4 * nProc), "alloc flux");
+  chk(hipMalloc(&x->sr, sizeof(double) * nProc), "alloc sr");
+  chk(hipMalloc(&x->lflux, sizeof(double) * 4 * nProc), "alloc lflux");
+  const size_t tl = x->lhs32() ? sizeof(float) : sizeof(double);
+  chk(hipMalloc(&x->ff, tl * 32 * nProc), "alloc ff");
+  x->scratch_stride = maxPhaseChains;
+  chk(hipMalloc(&x->scratch, tl * 36 * std::max<size_t>(maxRows, 1) * std::max<size_t>(maxPhaseChains, 1)),
+      "alloc scratch");
+  x->nPart = nblk(nCell);
+  chk(hipMalloc(&x->partial, sizeof(double) * 2 * x->nPart), "alloc partial");
+  chk(hipMalloc(&x->norms, sizeof(double) * 4), "alloc norms");
+  chk(hipMalloc(&x->nanflag, sizeof(int)), "alloc nanflag");
+  if (rc) {
+    free_ctx(x);
+    delete x;
+    return rc;
+  }
+  std::vector<double> rm(mesh->refMtx, mesh->refMtx + 4 * (size_t)nFace);
+  if ((rc = upload(&x->pslot, pslot)) || (rc = upload(&x->src, src)) || (rc = upload(&x->nvec, nvec)) ||
+      (rc = upload(&x->refmtx, rm)) || (rc = upload(&x->pleft, pleft)) || (rc = upload(&x->pright, pright)) ||
+      (rc = upload(&x->pface, pface)) || (rc = upload(&x->cell_off, cnt)) ||
+      (rc = upload(&x->cell_ent, cell_ent)) || (rc = upload(&x->ch_pbase, ch_pbase)) ||
+      (rc = upload(&x->ch_nf, ch_nf)) || (rc = upload(&x->ch_ncell, ch_ncell)) ||
+      (rc = upload(&x->ch_periodic, ch_periodic)) || (rc = upload(&x->phase_chains, phase_chains))) {
+    free_ctx(x);
+    delete x;
+    return rc;
+  }
+  chk(hipMemset(x->nanflag, 0, sizeof(int)), "memset nanflag");
+  chk(hipMemset(x->norms, 0, sizeof(double) * 4), "memset norms");
+  chk(hipMemset(x->dq, 0, c4), "memset dq");
+  chk(hipMemset(x->r, 0, c4), "memset r");
+  chk(hipMemset(x->r0, 0, c4), "memset r0");
+  chk(hipMemset(x->sigma, 0, sizeof(double) * nCell), "memset sigma");
+  chk(hipMemset(x->ff, 0, tl * 32 * nProc), "memset ff");
+  if (rc) {
+    free_ctx(x);
+    delete x;
+    return rc;
+  }
+  *out = x;
+  return hamstr_init_flow(x);
+}
+
+int hamstr_destroy(hamstr_ctx* x) {
+  if (!x) return HAMSTR_OK;
+  (void)hipStreamSynchronize(x->stream);
+  free_ctx(x);
+  delete x;
+  return HAMSTR_OK;
+}
+
+int hamstr_freestream(const hamstr_ctx* x, double* out4) {
+  if (!x || !out4) return fail(HAMSTR_ERR_ARG, "null argument");
+  for (int k = 0; k < 4; k++) out4[k] = x->c.qinf[k];
+  return HAMSTR_OK;
+}
+
+int hamstr_init_flow(hamstr_ctx* x) {
+  if (!x) return fail(HAMSTR_ERR_ARG, "null context");
+  const double4 v = make_double4(x->c.qinf[0], x->c.qinf[1], x->c.qinf[2], x->c.qinf[3]);
+  hipLaunchKernelGGL(k_fill_state, dim3(nblk(x->nCell)), dim3(kBlock), 0, x->stream, x->nCell, v, x->q);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return HAMSTR_OK;
+}
+
+int hamstr_set_field(hamstr_ctx* x, int field, const double* host) {
+  if (!x || !host) return fail(HAMSTR_ERR_ARG, "null argument");
+  double* dst = nullptr;
+  size_t n = 4 * (size_t)x->nCell;
+  switch (field) {
+    case HAMSTR_FIELD_Q: dst = x->q; break;
+    case HAMSTR_FIELD_R: dst = x->r; break;
+    case HAMSTR_FIELD_R0: dst = x->r0; break;
+    case HAMSTR_FIELD_DQ: dst = x->dq; break;
+    case HAMSTR_FIELD_SIGMA: dst = x->sigma; n = x->nCell; break;
+    case HAMSTR_FIELD_FF: {
+      const size_t tl = x->lhs32() ? sizeof(float) : sizeof(double);
+      std::vector<char> buf(tl * 32 * (size_t)x->nProc);
+      for (int p = 0; p < x->nProc; p++)
+        for (int e = 0; e < 32; e++) {
+          const double v = host[32 * (size_t)x->proc_face[p] + e];
+          if (x->lhs32())
+            reinterpret_cast<float*>(buf.data())[(size_t)e * x->nProc + p] = float(v);
+          else
+            reinterpret_cast<double*>(buf.data())[(size_t)e * x->nProc + p] = v;
+        }
+      HIPCHK(hipMemcpyAsync(x->ff, buf.data(), buf.size(), hipMemcpyHostToDevice, x->stream));
+      HIPCHK(hipStreamSynchronize(x->stream));
+      return HAMSTR_OK;
+    }
+    default: return fail(HAMSTR_ERR_ARG, "unknown field id");
+  }
+  HIPCHK(hipMemcpyAsync(dst, host, sizeof(double) * n, hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return HAMSTR_OK;
+}
+
+int hamstr_get_field(hamstr_ctx* x, int field, double* host) {
+  if (!x || !host) return fail(HAMSTR_ERR_ARG, "null argument");
+  const double* srcp = nullptr;
+  size_t n = 4 * (size_t)x->nCell;
+  switch (field) {
+    case HAMSTR_FIELD_Q: srcp = x->q; break;
+    case HAMSTR_FIELD_R: srcp = x->r; break;
+    case HAMSTR_FIELD_R0: srcp = x->r0; break;
+    case HAMSTR_FIELD_DQ: srcp = x->dq; break;
+    case HAMSTR_FIELD_SIGMA: srcp = x->sigma; n = x->nCell; break;
+    case HAMSTR_FIELD_FF: {
+      const size_t tl = x->lhs32() ? sizeof(float) : sizeof(double);
+      std::vector<char> buf(tl * 32 * (size_t)x->nProc);
+      HIPCHK(hipMemcpyAsync(buf.data(), x->ff, buf.size(), hipMemcpyDeviceToHost, x->stream));
+      HIPCHK(hipStreamSynchronize(x->stream));
+      std::memset(host, 0, sizeof(double) * 32 * (size_t)x->nFace);
+      for (int p = 0; p < x->nProc; p++)
+        for (int e = 0; e < 32; e++) {
+          const size_t k = (size_t)e * x->nProc + p;
+          host[32 * (size_t)x->proc_face[p] + e] =
+              x->lhs32() ? double(reinterpret_cast<float*>(buf.data())[k]) : reinterpret_cast<double*>(buf.data())[k];
+        }
+      return HAMSTR_OK;
+    }
+    default: return fail(HAMSTR_ERR_ARG, "unknown field id");
+  }
+  HIPCHK(hipMemcpyAsync(host, srcp, sizeof(double) * n, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return HAMSTR_OK;
+}
+
+int hamstr_compute_rhs(hamstr_ctx* x, int implicit, double* L2, double* Linf) {
+  if (!x) return fail(HAMSTR_ERR_ARG, "null context");
+  int rc = launch_rhs_faces(x, implicit != 0);
+  if (rc) return rc;
+  hipLaunchKernelGGL((k_rhs_cells<false>), dim3(nblk(x->nCell)), dim3(kBlock), 0, x->stream, x->c, x->cell_off,
+                     x->cell_ent, x->flux, x->sr, x->r, x->r0, x->sigma, x->partial);
+  HIPCHK(hipGetLastError());
+  if ((rc = launch_norm(x, x->norms))) return rc;
+  rc = read_norms(x);
+  if (L2) *L2 = x->L2rhs;
+  if (Linf) *Linf = x->Linfrhs;
+  return rc;
+}
+
+int hamstr_adi(hamstr_ctx* x, double* L2lin, double* Linflin) {
+  if (!x) return fail(HAMSTR_ERR_ARG, "null context");
+  hipLaunchKernelGGL(k_adi_init, dim3(nblk(x->nCell)), dim3(kBlock), 0, x->stream, x->c, x->r, x->r0, x->dq,
+                     x->sigma);
+  HIPCHK(hipGetLastError());
+  int rc = launch_sweeps(x, x->norms + 2);
+  if (rc) return rc;
+  rc = read_norms(x);
+  if (L2lin) *L2lin = x->L2lin;
+  if (Linflin) *Linflin = x->Linflin;
+  return rc;
+}
+
+int hamstr_compute_linear_rhs(hamstr_ctx* x, double* L2lin, double* Linflin) {
+  if (!x) return fail(HAMSTR_ERR_ARG, "null context");
+  int rc = launch_linear(x, false, x->norms + 2);
+  if (rc) return rc;
+  rc = read_norms(x);
+  if (L2lin) *L2lin = x->L2lin;
+  if (Linflin) *Linflin = x->Linflin;
+  return rc;
+}
+
+int hamstr_step(hamstr_ctx* x, int nsteps, double tol, int check_every, double* l2rhs, double* l2lin, int* done) {
+  if (!x || nsteps < 0) return fail(HAMSTR_ERR_ARG, "bad arguments");
+  if (check_every < 1) check_every = 1;
+  if (x->histCap < nsteps) {
+    if (x->hist) (void)hipFree(x->hist);
+    x->hist = nullptr;
+    HIPCHK(hipMalloc(&x->hist, sizeof(double) * 4 * std::max(nsteps, 1)));
+    x->histCap = nsteps;
+  }
+  const int nk = 5 + 2 * (int)(x->phase_off.size() - 1) + 2;
+  if (x->timing && (int)x->ev.size() < nk + 1) {
+    for (auto e : x->ev) (void)hipEventDestroy(e);
+    x->ev.assign(nk + 1, nullptr);
+    for (auto& e : x->ev) HIPCHK(hipEventCreate(&e));
+  }
+  int ran = 0, rc = HAMSTR_OK;
+  int lastRead = 0;
+  for (int n = 0; n < nsteps; n++) {
+    double* hn = x->hist + 4 * n;
+    int ie = 0;
+    auto mark = [&]() {
+      if (x->timing && n == nsteps - 1 && ie < (int)x->ev.size()) (void)hipEventRecord(x->ev[ie++], x->stream);
+    };
+    mark();
+    if ((rc = launch_rhs_faces(x, true))) return rc;
+    mark();
+    hipLaunchKernelGGL((k_rhs_cells<true>), dim3(nblk(x->nCell)), dim3(kBlock), 0, x->stream, x->c, x->cell_off,
+                       x->cell_ent, x->flux, x->sr, x->r, x->r0, x->sigma, x->partial);
+    HIPCHK(hipGetLastError());
+    if ((rc = launch_norm(x, hn))) return rc;
+    mark();
+    for (int s = 0; s < x->prm.msweep; s++) {
+      if ((rc = launch_adi_phases(x))) return rc;
+      mark();
+      const int n4 = 4 * x->nCell;
+      if (s == 0)
+        hipLaunchKernelGGL((k_accum_dq<true>), dim3(nblk(n4)), dim3(kBlock), 0, x->stream, n4, x->r, x->dq);
+      else
+        hipLaunchKernelGGL((k_accum_dq<false>), dim3(nblk(n4)), dim3(kBlock), 0, x->stream, n4, x->r, x->dq);
+      HIPCHK(hipGetLastError());
+      if ((rc = launch_linear(x, s == x->prm.msweep - 1, hn + 2))) return rc;
+      mark();
+    }
+    ran = n + 1;
+    if (ran % check_every == 0 || ran == nsteps) {
+      std::vector<double> h(4 * (ran - lastRead));
+      HIPCHK(hipMemcpyAsync(h.data(), x->hist + 4 * lastRead, sizeof(double) * h.size(), hipMemcpyDeviceToHost,
+                            x->stream));
+      int flag = 0;
+      HIPCHK(hipMemcpyAsync(&flag, x->nanflag, sizeof(int), hipMemcpyDeviceToHost, x->stream));
+      HIPCHK(hipStreamSynchronize(x->stream));
+      for (int k = lastRead; k < ran; k++) {
+        if (l2rhs) l2rhs[k] = h[4 * (k - lastRead)];
+        if (l2lin) l2lin[k] = h[4 * (k - lastRead) + 2];
+      }
+      x->L2rhs = h[4 * (ran - 1 - lastRead)];
+      x->Linfrhs = h[4 * (ran - 1 - lastRead) + 1];
+      x->L2lin = h[4 * (ran - 1 - lastRead) + 2];
+      x->Linflin = h[4 * (ran - 1 - lastRead) + 3];
+      if (flag) {
+        if (done) *done = ran;
+        return fail(HAMSTR_ERR_NAN, "NaN spectral radius in the Roe flux");
+      }
+      bool conv = false;
+      if (tol > 0)
+        for (int k = lastRead; k < ran; k++)
+          if (h[4 * (k - lastRead)] <= tol) {
+            conv = true;
+            ran = k + 1;
+            break;
+          }
+      lastRead = n + 1;
+      if (conv) break;
+    }
+  }
+  if (x->timing) {
+    HIPCHK(hipStreamSynchronize(x->stream));
+    x->last_ms.clear();
+    for (size_t k = 0; k + 1 < x->ev.size() && x->ev[k + 1]; k++) {
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, x->ev[k], x->ev[k + 1]) != hipSuccess) break;
+      x->last_ms.push_back(ms);
+    }
+  }
+  if (done) *done = ran;
+  return rc;
+}
+
+int hamstr_sync(hamstr_ctx* x) {
+  if (!x) return fail(HAMSTR_ERR_ARG, "null context");
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return HAMSTR_OK;
+}
+
+int hamstr_set_timing(hamstr_ctx* x, int on) {
+  if (!x) return fail(HAMSTR_ERR_ARG, "null context");
+  x->timing = on != 0;
+  return HAMSTR_OK;
+}
+
+int hamstr_last_timings(hamstr_ctx* x, double* ms, int cap, int* n) {
+  if (!x || !ms || !n) return fail(HAMSTR_ERR_ARG, "null argument");
+  int k = 0;
+  for (; k < cap && k < (int)x->last_ms.size(); k++) ms[k] = x->last_ms[k];
+  *n = k;
+  return HAMSTR_OK;
+}
+
+}  // extern "C"
+
+// ---- unit entry points ------------------------------------------------------
+namespace {
+struct Scoped {
+  std::vector<void*> ptrs;
+  ~Scoped() {
+    for (void* p : ptrs) (void)hipFree(p);
+  }
+  template <typename V>
+  int up(V** d, const V* h, size_t n) {
+    HIPCHK(hipMalloc(d, sizeof(V) * std::max<size_t>(n, 1)));
+    ptrs.push_back(*d);
+    if (h) HIPCHK(hipMemcpy(*d, h, sizeof(V) * n, hipMemcpyHostToDevice));
+    return HAMSTR_OK;
+  }
+};
+}  // namespace
+
+extern "C" {
+
+int hamstr_flux_roe2d(int n, const double* nvec, const double* ql, const double* qr, double gamma, double* flux,
+                      double* sr) {
+  if (n <= 0 || !nvec || !ql || !qr || !flux || !sr) return fail(HAMSTR_ERR_ARG, "bad arguments");
+  Scoped s;
+  double *dn, *dl, *dr, *df, *ds;
+  int* dflag;
+  int rc;
+  const int zero = 0;
+  if ((rc = s.up(&dn, nvec, 2 * (size_t)n)) || (rc = s.up(&dl, ql, 4 * (size_t)n)) ||
+      (rc = s.up(&dr, qr, 4 * (size_t)n)) || (rc = s.up(&df, (const double*)nullptr, 4 * (size_t)n)) ||
+      (rc = s.up(&ds, (const double*)nullptr, (size_t)n)) || (rc = s.up(&dflag, &zero, 1)))
+    return rc;
+  hipLaunchKernelGGL(k_unit_flux, dim3(nblk(n)), dim3(kBlock), 0, 0, n, dn, dl, dr, gamma, df, ds, dflag);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(flux, df, sizeof(double) * 4 * n, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(sr, ds, sizeof(double) * n, hipMemcpyDeviceToHost));
+  int flag = 0;
+  HIPCHK(hipMemcpy(&flag, dflag, sizeof(int), hipMemcpyDeviceToHost));
+  return flag ? fail(HAMSTR_ERR_NAN, "NaN spectral radius") : HAMSTR_OK;
+}
+
+int hamstr_jacobian_roe2d(int n, const double* nvec, const double* ql, const double* qr, double gamma, int lhs,
+                          double* lmat, double* rmat) {
+  if (n <= 0 || !nvec || !ql || !qr || !lmat || !rmat) return fail(HAMSTR_ERR_ARG, "bad arguments");
+  Scoped s;
+  double *dn, *dl, *dr, *dlm, *drm;
+  int rc;
+  if ((rc = s.up(&dn, nvec, 2 * (size_t)n)) || (rc = s.up(&dl, ql, 4 * (size_t)n)) ||
+      (rc = s.up(&dr, qr, 4 * (size_t)n)) || (rc = s.up(&dlm, (const double*)nullptr, 16 * (size_t)n)) ||
+      (rc = s.up(&drm, (const double*)nullptr, 16 * (size_t)n)))
+    return rc;
+  if (lhs == HAMSTR_LHS_FP32)
+    hipLaunchKernelGGL(k_unit_jac<float>, dim3(nblk(n)), dim3(kBlock), 0, 0, n, dn, dl, dr, gamma, dlm, drm);
+  else
+    hipLaunchKernelGGL(k_unit_jac<double>, dim3(nblk(n)), dim3(kBlock), 0, 0, n, dn, dl, dr, gamma, dlm, drm);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(lmat, dlm, sizeof(double) * 16 * n, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(rmat, drm, sizeof(double) * 16 * n, hipMemcpyDeviceToHost));
+  return HAMSTR_OK;
+}
+
+}  // extern "C"
+static int unit_recon(int kind, int nchain, int imax, int is, int iend, double qt, double eps, const double* f,
+                      const double* dq, double* ql, double* qr) {
+  if (nchain <= 0 || imax <= 0 || !f || !ql || !qr) return fail(HAMSTR_ERR_ARG, "bad arguments");
+  const bool weno = (kind == 1 || kind == 3);
+  if (is < (weno ? 2 : 1) || iend > (weno ? imax - 2 : imax - 1) || iend <= is)
+    return fail(HAMSTR_ERR_ARG, "row range outside the chain");
+  if ((kind >= 2) && !dq) return fail(HAMSTR_ERR_ARG, "dq required");
+  const size_t tot = (size_t)nchain * imax * 4;
+  Scoped s;
+  double *df, *dd = nullptr, *dl, *dr;
+  int rc;
+  if ((rc = s.up(&df, f, tot)) || (rc = s.up(&dl, ql, tot)) || (rc = s.up(&dr, qr, tot))) return rc;
+  if (kind >= 2 && (rc = s.up(&dd, dq, tot))) return rc;
+  const int nth = nchain * (iend - is) * 4;
+  hipLaunchKernelGGL(k_unit_recon, dim3(nblk(nth)), dim3(kBlock), 0, 0, kind, nchain, imax, is, iend, qt, eps, df,
+                     dd, dl, dr);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(ql, dl, sizeof(double) * tot, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(qr, dr, sizeof(double) * tot, hipMemcpyDeviceToHost));
+  return HAMSTR_OK;
+}
+
+extern "C" {
+// MUSCL rows is..ie-1 (row ie uses a stale slope in the reference and is
+// never read by its callers); first order (qt == 0) copies rows is..ie.
+int hamstr_muscl(int nchain, int imax, int is, int ie, double th, double qt, double eps, const double* f,
+                 double* ql, double* qr) {
+  if (th != kThird) return fail(HAMSTR_ERR_ARG, "th must be 1/3 (computeRHS.cpp:268)");
+  if (qt == 0.0) {
+    if (is < 0 || ie >= imax || ie < is) return fail(HAMSTR_ERR_ARG, "row range outside the chain");
+    for (int c = 0; c < nchain; c++)
+      for (int i = is; i <= ie; i++)
+        for (int k = 0; k < 4; k++) {
+          const size_t o = ((size_t)c * imax + i) * 4 + k;
+          ql[o] = qr[o] = f[o];
+        }
+    return HAMSTR_OK;
+  }
+  return unit_recon(0, nchain, imax, is, ie, qt, eps, f, nullptr, ql, qr);
+}
+
+int hamstr_weno(int nchain, int imax, int is, int ie, double eps, const double* f, double* ql, double* qr) {
+  (void)ie;
+  (void)eps;
+  return unit_recon(1, nchain, imax, is, imax - 2, 0.0, 1e-15, f, nullptr, ql, qr);
+}
+
+int hamstr_muscl_deriv(int nchain, int imax, int is, int ie, double th, double qt, double eps, const double* f,
+                       const double* dq, double* ql, double* qr) {
+  if (th != kThird) return fail(HAMSTR_ERR_ARG, "th must be 1/3 (computeLinearRHS.cpp:283)");
+  if (qt == 0.0) {
+    if (is < 0 || ie >= imax || ie < is) return fail(HAMSTR_ERR_ARG, "row range outside the chain");
+    for (int c = 0; c < nchain; c++)
+      for (int i = is; i <= ie; i++)
+        for (int k = 0; k < 4; k++) {
+          const size_t o = ((size_t)c * imax + i) * 4 + k;
+          ql[o] = qr[o] = dq[o];
+        }
+    return HAMSTR_OK;
+  }
+  return unit_recon(2, nchain, imax, is, ie, qt, eps, f, dq, ql, qr);
+}
+
+int hamstr_weno_deriv(int nchain, int imax, int is, int ie, double eps, const double* f, const double* dq,
+                      double* ql, double* qr) {
+  (void)ie;
+  return unit_recon(3, nchain, imax, is, imax - 2, 0.0, eps, f, dq, ql, qr);
+}
+
+}  // extern "C"
+static int unit_tridag(int periodic, int nsys, int N, const double* a, const double* b, const double* c,
+                       double* f) {
+  if (nsys <= 0 || N < (periodic ? 3 : 1) || !a || !b || !c || !f) return fail(HAMSTR_ERR_ARG, "bad arguments");
+  Scoped s;
+  double *da, *db, *dc, *df, *ds;
+  int rc;
+  const size_t m = (size_t)nsys * N;
+  if ((rc = s.up(&da, a, 16 * m)) || (rc = s.up(&db, b, 16 * m)) || (rc = s.up(&dc, c, 16 * m)) ||
+      (rc = s.up(&df, f, 4 * m)) || (rc = s.up(&ds, (const double*)nullptr, 36 * m)))
+    return rc;
+  hipLaunchKernelGGL(k_unit_tridag, dim3(nblk(nsys, 64)), dim3(64), 0, 0, nsys, N, periodic, da, db, dc, df, ds);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(f, df, sizeof(double) * 4 * m, hipMemcpyDeviceToHost));
+  return HAMSTR_OK;
+}
+
+extern "C" {
+int hamstr_block_tridag4(int nsys, int N, const double* a, const double* b, const double* c, double* f) {
+  return unit_tridag(0, nsys, N, a, b, c, f);
+}
+
+int hamstr_block_tridag_periodic4(int nsys, int N, const double* a, const double* b, const double* c, double* f) {
+  return unit_tridag(1, nsys, N, a, b, c, f);
+}
+
+}  // extern "C"

@@ -1,0 +1,272 @@
+"""ctypes binding of the C ABI in include/hamstr_hip.h (libhamstr_hip.so).
+
+``Solver`` mirrors the reference's SOLVER call surface (solver.h:14-55):
+``initFlow``, ``computeRHS``, ``ADI``, ``computeLinearRHS``, ``stepSolution``,
+``marchInTime``, plus field access.  The library must be built
+(``python -m hamstr_oop_amd.build``); loading fails loudly otherwise -- there
+is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhamstr_hip.so")
+
+OK, ERR_ARG, ERR_HIP, ERR_NAN, ERR_MESH, ERR_NODEV = range(6)
+LHS_FP64, LHS_FP32 = 0, 1
+ADI_EXACT, ADI_FAST = 0, 1
+FIELDS = {"q": 0, "r": 1, "r0": 2, "dq": 3, "sigma": 4, "ff": 5}
+
+# every entry point declared in include/hamstr_hip.h
+EXPORTS = [
+    "hamstr_preprocess", "hamstr_create", "hamstr_destroy", "hamstr_init_flow", "hamstr_freestream",
+    "hamstr_set_field", "hamstr_get_field", "hamstr_compute_rhs", "hamstr_adi",
+    "hamstr_compute_linear_rhs", "hamstr_step", "hamstr_sync", "hamstr_set_timing",
+    "hamstr_last_timings", "hamstr_flux_roe2d", "hamstr_jacobian_roe2d", "hamstr_muscl", "hamstr_weno",
+    "hamstr_muscl_deriv", "hamstr_weno_deriv", "hamstr_block_tridag4", "hamstr_block_tridag_periodic4",
+    "hamstr_last_error",
+]
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class HamstrError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"hamstr error {code}: {msg}")
+        self.code = code
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("nCell", C.c_int), ("nFace", C.c_int), ("nChain", C.c_int),
+                ("faces", _ip), ("chainStart", _ip), ("chainConn", _ip),
+                ("normVec", _dp), ("refMtx", _dp)]
+
+
+class Params(C.Structure):
+    _fields_ = [("mach", C.c_double), ("alpha", C.c_double), ("cfl", C.c_double),
+                ("order", C.c_int), ("msweep", C.c_int), ("lhs", C.c_int), ("adi_mode", C.c_int)]
+
+
+_lib: Optional[C.CDLL] = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(f"{LIB_PATH} is not built: run python -m hamstr_oop_amd.build")
+        L = C.CDLL(LIB_PATH)
+        L.hamstr_last_error.restype = C.c_char_p
+        L.hamstr_create.argtypes = [C.POINTER(MeshDesc), C.POINTER(Params), C.POINTER(C.c_void_p)]
+        for name in EXPORTS:
+            getattr(L, name)  # raises if missing
+        _lib = L
+    return _lib
+
+
+def _chk(rc: int) -> None:
+    if rc != OK:
+        raise HamstrError(rc, lib().hamstr_last_error().decode())
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def preprocess(x: np.ndarray, faces: np.ndarray, sol_visc: int = 0):
+    """MESHBLOCK::preprocess: returns (faces with wall tags, normVec, refMtx)."""
+    x = np.ascontiguousarray(x, np.float64)
+    faces = np.array(faces, np.int32, copy=True, order="C")
+    nf = faces.shape[0]
+    nv = np.zeros((nf, 2))
+    rm = np.zeros((nf, 4))
+    _chk(lib().hamstr_preprocess(x.shape[0], _d(x), nf, _i(faces), _d(nv), _d(rm), sol_visc))
+    return faces, nv, rm
+
+
+class Solver:
+    """A device context for one mesh/flow case (SOLVER + SOLNBLOCK state)."""
+
+    def __init__(self, mesh, mach: float, alpha: float, order: int, cfl: float, msweep: int = 1,
+                 lhs: int = LHS_FP64, adi_mode: int = ADI_EXACT, sol_visc: int = 0):
+        self.mesh = mesh
+        faces, nv, rm = preprocess(mesh.x, mesh.faces, sol_visc)
+        self.faces, self.normvec, self.refmtx = faces, nv, rm
+        self.n_cell = mesh.n_cell
+        self.n_face = mesh.n_face
+        self._keep = [faces, nv, rm, np.ascontiguousarray(mesh.chain_start, np.int32),
+                      np.ascontiguousarray(mesh.chain_conn, np.int32)]
+        md = MeshDesc(mesh.n_cell, mesh.n_face, mesh.n_chain, _i(faces), _i(self._keep[3]),
+                      _i(self._keep[4]), _d(nv), _d(rm))
+        prm = Params(float(mach), float(alpha), float(cfl), int(order), int(msweep), int(lhs), int(adi_mode))
+        h = C.c_void_p()
+        _chk(lib().hamstr_create(C.byref(md), C.byref(prm), C.byref(h)))
+        self.h = h
+        self.order, self.cfl, self.msweep, self.lhs = order, cfl, msweep, lhs
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().hamstr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- fields -------------------------------------------------------------
+    def _size(self, name):
+        return {"sigma": self.n_cell, "ff": 32 * self.n_face}.get(name, 4 * self.n_cell)
+
+    def get(self, name: str) -> np.ndarray:
+        out = np.zeros(self._size(name))
+        _chk(lib().hamstr_get_field(self.h, FIELDS[name], _d(out)))
+        return out
+
+    def set(self, name: str, val) -> None:
+        val = np.ascontiguousarray(val, np.float64).reshape(-1)
+        if val.size != self._size(name):
+            raise ValueError(f"{name}: expected {self._size(name)} values, got {val.size}")
+        _chk(lib().hamstr_set_field(self.h, FIELDS[name], _d(val)))
+
+    def freestream(self) -> np.ndarray:
+        out = np.zeros(4)
+        _chk(lib().hamstr_freestream(self.h, _d(out)))
+        return out
+
+    # ---- SOLVER surface -----------------------------------------------------
+    def initFlow(self):
+        _chk(lib().hamstr_init_flow(self.h))
+
+    def computeRHS(self, option: str = "implicit"):
+        if option not in ("implicit", "explicit"):
+            raise ValueError("option must be 'implicit' or 'explicit'")
+        l2, li = C.c_double(), C.c_double()
+        _chk(lib().hamstr_compute_rhs(self.h, int(option == "implicit"), C.byref(l2), C.byref(li)))
+        return l2.value, li.value
+
+    def ADI(self):
+        l2, li = C.c_double(), C.c_double()
+        _chk(lib().hamstr_adi(self.h, C.byref(l2), C.byref(li)))
+        return l2.value, li.value
+
+    def computeLinearRHS(self):
+        l2, li = C.c_double(), C.c_double()
+        _chk(lib().hamstr_compute_linear_rhs(self.h, C.byref(l2), C.byref(li)))
+        return l2.value, li.value
+
+    def step(self, nsteps: int, tol: float = 0.0, check_every: int = 1):
+        """nsteps x stepSolution (ADI); returns (steps run, l2rhs[], l2lin[])."""
+        l2r = np.zeros(max(nsteps, 1))
+        l2l = np.zeros(max(nsteps, 1))
+        done = C.c_int(0)
+        rc = lib().hamstr_step(self.h, int(nsteps), C.c_double(tol), int(check_every), _d(l2r), _d(l2l),
+                               C.byref(done))
+        _chk(rc)
+        n = done.value
+        return n, l2r[:n], l2l[:n]
+
+    def stepSolution(self):
+        return self.step(1)
+
+    def marchInTime(self, nsteps: int):
+        return self.step(nsteps)
+
+    def sync(self):
+        _chk(lib().hamstr_sync(self.h))
+
+    def set_timing(self, on: bool):
+        _chk(lib().hamstr_set_timing(self.h, int(on)))
+
+    def last_timings(self):
+        buf = np.zeros(64)
+        n = C.c_int()
+        _chk(lib().hamstr_last_timings(self.h, _d(buf), 64, C.byref(n)))
+        return buf[: n.value]
+
+
+# ---- per-function entry points ---------------------------------------------
+
+def flux_roe2d(nvec, ql, qr, gamma=1.4):
+    nvec = np.ascontiguousarray(nvec, np.float64).reshape(-1, 2)
+    ql = np.ascontiguousarray(ql, np.float64).reshape(-1, 4)
+    qr = np.ascontiguousarray(qr, np.float64).reshape(-1, 4)
+    n = nvec.shape[0]
+    flux = np.zeros((n, 4))
+    sr = np.zeros(n)
+    _chk(lib().hamstr_flux_roe2d(n, _d(nvec), _d(ql), _d(qr), C.c_double(gamma), _d(flux), _d(sr)))
+    return flux, sr
+
+
+def jacobian_roe2d(nvec, ql, qr, gamma=1.4, lhs=LHS_FP64):
+    nvec = np.ascontiguousarray(nvec, np.float64).reshape(-1, 2)
+    ql = np.ascontiguousarray(ql, np.float64).reshape(-1, 4)
+    qr = np.ascontiguousarray(qr, np.float64).reshape(-1, 4)
+    n = nvec.shape[0]
+    lm = np.zeros((n, 4, 4))
+    rm = np.zeros((n, 4, 4))
+    _chk(lib().hamstr_jacobian_roe2d(n, _d(nvec), _d(ql), _d(qr), C.c_double(gamma), int(lhs), _d(lm), _d(rm)))
+    return lm, rm
+
+
+def _recon(name, f, dq, is_, ie, th=None, qt=None, eps=1e-10):
+    f = np.ascontiguousarray(f, np.float64)
+    if f.ndim == 2:
+        f = f[None]
+    nch, imax, _ = f.shape
+    ql = np.zeros_like(f)
+    qr = np.zeros_like(f)
+    L = lib()
+    if name == "muscl":
+        _chk(L.hamstr_muscl(nch, imax, is_, ie, C.c_double(th), C.c_double(qt), C.c_double(eps), _d(f), _d(ql), _d(qr)))
+    elif name == "weno":
+        _chk(L.hamstr_weno(nch, imax, is_, ie, C.c_double(eps), _d(f), _d(ql), _d(qr)))
+    else:
+        dq = np.ascontiguousarray(dq, np.float64).reshape(f.shape)
+        if name == "muscl_deriv":
+            _chk(L.hamstr_muscl_deriv(nch, imax, is_, ie, C.c_double(th), C.c_double(qt), C.c_double(eps), _d(f),
+                                      _d(dq), _d(ql), _d(qr)))
+        else:
+            _chk(L.hamstr_weno_deriv(nch, imax, is_, ie, C.c_double(eps), _d(f), _d(dq), _d(ql), _d(qr)))
+    return ql, qr
+
+
+def muscl(f, is_, ie, th, qt, eps=1e-10):
+    return _recon("muscl", f, None, is_, ie, th, qt, eps)
+
+
+def weno(f, is_, ie, eps=1e-10):
+    return _recon("weno", f, None, is_, ie, eps=eps)
+
+
+def muscl_deriv(f, dq, is_, ie, th, qt, eps=1e-10):
+    return _recon("muscl_deriv", f, dq, is_, ie, th, qt, eps)
+
+
+def weno_deriv(f, dq, is_, ie, eps=1e-10):
+    return _recon("weno_deriv", f, dq, is_, ie, eps=eps)
+
+
+def block_tridag4(a, b, c, f, periodic=False):
+    """a, b, c: (nsys, N, 4, 4); f: (nsys, N, 4).  Returns the solution."""
+    a = np.ascontiguousarray(a, np.float64)
+    if a.ndim == 3:
+        a, b, c, f = a[None], np.asarray(b)[None], np.asarray(c)[None], np.asarray(f)[None]
+    b = np.ascontiguousarray(b, np.float64)
+    c = np.ascontiguousarray(c, np.float64)
+    x = np.array(f, np.float64, copy=True, order="C")
+    nsys, N = a.shape[0], a.shape[1]
+    fn = lib().hamstr_block_tridag_periodic4 if periodic else lib().hamstr_block_tridag4
+    _chk(fn(nsys, N, _d(a), _d(b), _d(c), _d(x)))
+    return x

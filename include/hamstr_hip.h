@@ -1,0 +1,184 @@
+/*
+ * hamstr_hip.h -- C ABI of the MI355X implementation of HAMSTR_OOP's
+ * line-implicit steady Euler iteration (computeRHS -> ADI -> computeLinearRHS
+ * -> update).
+ *
+ * Every entry point returns an int status (HAMSTR_OK or one of the codes
+ * below).  Host pointers are read or written only during the call.  A context
+ * owns its device memory and one HIP stream; contexts are independent, and one
+ * host thread at a time may use a context.
+ *
+ * Array layouts follow the reference's memory layout after
+ * MESHBLOCK::readMeshData and MESHBLOCK::preprocess:
+ *   faces      int    [6*nFace]   n1, n2, leftCell, leftEdge, rightCell, rightEdge
+ *                                 (rightCell: >=0 cell, -1 far field, -2 inviscid wall)
+ *   chainStart int    [nChain+1]  faceStartPerChain (duplicates already removed)
+ *   chainConn  int    [nChainFace] face ids along each chain
+ *   normVec    double [2*nFace]   (y2-y1, -(x2-x1))
+ *   refMtx     double [4*nFace]   I - 2 n n^T / |n|^2, row-major
+ *   q, r, r0, dq     double [4*nCell]  AoS (rho, rho u, rho v, E) per cell
+ *   sigma            double [nCell]
+ *   ff               double [32*nFace] lmat[4][4] then rmat[4][4] per face
+ */
+#ifndef HAMSTR_HIP_H
+#define HAMSTR_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HAMSTR_OK 0
+#define HAMSTR_ERR_ARG 1      /* bad argument or size                          */
+#define HAMSTR_ERR_HIP 2      /* a HIP runtime call failed                     */
+#define HAMSTR_ERR_NAN 3      /* NaN spectral radius (reference: exit(1))      */
+#define HAMSTR_ERR_MESH 4     /* chain layout the device path does not accept  */
+#define HAMSTR_ERR_NODEV 5    /* no HIP device                                 */
+
+/* LHS precision: the Roe Jacobians and ADI block solves.  The residual, its
+ * norms and the solution update are always FP64. */
+#define HAMSTR_LHS_FP64 0
+#define HAMSTR_LHS_FP32 1
+
+/* ADI block-tridiagonal solve ordering.
+ * EXACT: the reference's sequential block Thomas order (bit-identical).
+ * FAST : the same systems solved with a segmented (partitioned) elimination,
+ *        equal to EXACT up to rounding. */
+#define HAMSTR_ADI_EXACT 0
+#define HAMSTR_ADI_FAST 1
+
+typedef struct hamstr_ctx hamstr_ctx;
+
+typedef struct hamstr_mesh_desc {
+  int nCell;
+  int nFace;
+  int nChain;
+  const int *faces;
+  const int *chainStart;
+  const int *chainConn;
+  const double *normVec;
+  const double *refMtx;
+} hamstr_mesh_desc;
+
+typedef struct hamstr_params {
+  double mach;      /* SOLNBLOCK::Mach   */
+  double alpha;     /* SOLNBLOCK::alpha, degrees */
+  double cfl;       /* SOLNBLOCK::CFL    */
+  int order;        /* 1 first order, 3 MUSCL, 5 WENO5 */
+  int msweep;       /* ADI sweeps per iteration */
+  int lhs;          /* HAMSTR_LHS_FP64 | HAMSTR_LHS_FP32 */
+  int adi_mode;     /* HAMSTR_ADI_EXACT | HAMSTR_ADI_FAST */
+} hamstr_params;
+
+/* Field ids for hamstr_get_field / hamstr_set_field. */
+#define HAMSTR_FIELD_Q 0
+#define HAMSTR_FIELD_R 1
+#define HAMSTR_FIELD_R0 2
+#define HAMSTR_FIELD_DQ 3
+#define HAMSTR_FIELD_SIGMA 4
+#define HAMSTR_FIELD_FF 5   /* converted to double when the LHS is FP32 */
+
+/* ---- set-up ------------------------------------------------------------- */
+
+/* MESHBLOCK::preprocess (preprocess.cpp:56-193): face normals, reflection
+ * matrices, and wall tagging of boundary faces (rightCell == -1) whose
+ * midpoint radius is below 3.0 (rightCell := -(sol_visc+2)).  Host only. */
+int hamstr_preprocess(int nNode, const double *x, int nFace, int *faces,
+                      double *normVec, double *refMtx, int sol_visc);
+
+/* SOLVER::SOLVER + SOLVER::initFlow (solver.cpp:13-81, initFlow.cpp:12-93):
+ * uploads the mesh, builds the device chain tables and sets q to freestream. */
+int hamstr_create(const hamstr_mesh_desc *mesh, const hamstr_params *params,
+                  hamstr_ctx **out);
+int hamstr_destroy(hamstr_ctx *ctx);
+
+/* SOLVER::initFlow (initFlow.cpp:12-93): q := freestream. */
+int hamstr_init_flow(hamstr_ctx *ctx);
+/* freestream (rinf, rinf*uinf, rinf*vinf, einf) as initFlow computes it */
+int hamstr_freestream(const hamstr_ctx *ctx, double *out4);
+
+int hamstr_set_field(hamstr_ctx *ctx, int field, const double *host);
+int hamstr_get_field(hamstr_ctx *ctx, int field, double *host);
+
+/* ---- the iteration (SOLVER methods) ------------------------------------- */
+
+/* SOLVER::computeRHS(char*) (computeRHS.cpp:17-427); implicit != 0 is
+ * "implicit" (also stores the face Jacobians ff).  L2/Linf of the density
+ * residual: L2 = sqrt(sum r_rho^2)/nCell.  Either pointer may be NULL. */
+int hamstr_compute_rhs(hamstr_ctx *ctx, int implicit, double *L2, double *Linf);
+
+/* SOLVER::ADI (ADI.cpp:15-266), including computeLinearRHS per sweep and the
+ * final q += dq.  Returns the linear-residual norms of the last sweep. */
+int hamstr_adi(hamstr_ctx *ctx, double *L2lin, double *Linflin);
+
+/* SOLVER::computeLinearRHS (computeLinearRHS.cpp:13-371) on the current
+ * q, dq, r0, sigma, ff. */
+int hamstr_compute_linear_rhs(hamstr_ctx *ctx, double *L2lin, double *Linflin);
+
+/* nsteps x SOLVER::stepSolution's ADI branch (march.cpp:112-117).  Per-step
+ * nonlinear (computeRHS) and linear (last computeLinearRHS) L2 norms are
+ * written to l2rhs[n] / l2lin[n] when the arrays are not NULL.  When tol > 0
+ * the loop stops after the first step whose nonlinear L2 <= tol; *done gets
+ * the number of steps run.  The norms are read back every check_every steps
+ * (>= 1); the steps in between run without host synchronisation. */
+int hamstr_step(hamstr_ctx *ctx, int nsteps, double tol, int check_every,
+                double *l2rhs, double *l2lin, int *done);
+
+/* Blocks until the context's stream is idle. */
+int hamstr_sync(hamstr_ctx *ctx);
+
+/* Turns per-kernel HIP-event timing of hamstr_step on (on != 0) or off. */
+int hamstr_set_timing(hamstr_ctx *ctx, int on);
+
+/* Per-launch duration (milliseconds, HIP events on the context's stream) of
+ * the kernels of the last hamstr_step call: rhs-faces, rhs-cells, adi
+ * phases, linear-faces, linear-cells+update.  *n gets the number written
+ * (at most cap). */
+int hamstr_last_timings(hamstr_ctx *ctx, double *ms, int cap, int *n);
+
+/* ---- per-function entry points, batched over independent problems -------
+ * Inputs and outputs are host arrays; each call copies to the device,
+ * launches, and copies back.  Used for per-function parity. */
+
+/* SOLVER::fluxRoe2D (interfaceFlux.cpp:24-169), n problems:
+ * nvec[2n], ql[4n], qr[4n] -> flux[4n], sr[n].  Returns HAMSTR_ERR_NAN if a
+ * spectral radius is NaN. */
+int hamstr_flux_roe2d(int n, const double *nvec, const double *ql, const double *qr,
+                      double gamma, double *flux, double *sr);
+
+/* SOLVER::jacobianRoe2D (jacobianRoe.cpp:9-70, IMODE 1):
+ * -> lmat[16n], rmat[16n] row-major.  lhs selects FP64 or FP32 arithmetic
+ * (FP32 results are returned widened to double). */
+int hamstr_jacobian_roe2d(int n, const double *nvec, const double *ql, const double *qr,
+                          double gamma, int lhs, double *lmat, double *rmat);
+
+/* RECONSTRUCTION::MUSCL / WENO / MUSCL_DERIV / WENO_DERIV
+ * (reconstruction.cpp:16-70, 133-160, 168-236, 353-386) on nchain chains
+ * of imax rows each: f[nchain*imax*4] (and dq for the derivatives) ->
+ * ql, qr [nchain*imax*4].  Rows outside the reference's computed range are
+ * left untouched.  is/ie/th/qt/eps as in the reference call sites. */
+int hamstr_muscl(int nchain, int imax, int is, int ie, double th, double qt, double eps,
+                 const double *f, double *ql, double *qr);
+int hamstr_weno(int nchain, int imax, int is, int ie, double eps,
+                const double *f, double *ql, double *qr);
+int hamstr_muscl_deriv(int nchain, int imax, int is, int ie, double th, double qt, double eps,
+                       const double *f, const double *dq, double *ql, double *qr);
+int hamstr_weno_deriv(int nchain, int imax, int is, int ie, double eps,
+                      const double *f, const double *dq, double *ql, double *qr);
+
+/* TRISOLVER::blockTridag4 / blockTridagPeriodic4 (triSolvers.cpp:769-847,
+ * 850-1046) on nsys independent systems of N block rows:
+ * a, b, c [nsys*N*16], f [nsys*N*4]; f is overwritten with the solution.
+ * a, b, c are not modified. */
+int hamstr_block_tridag4(int nsys, int N, const double *a, const double *b, const double *c,
+                         double *f);
+int hamstr_block_tridag_periodic4(int nsys, int N, const double *a, const double *b,
+                                  const double *c, double *f);
+
+/* Text of the last error on this thread. */
+const char *hamstr_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HAMSTR_HIP_H */

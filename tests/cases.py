@@ -1,0 +1,47 @@
+"""Small meshes and random inputs shared by the tests."""
+import numpy as np
+
+from hamstr_oop_amd import meshgen
+
+# (mesh factory, Mach, alpha)
+SMALL_CASES = {
+    "circle16x8": (lambda: meshgen.circle_ogrid(16, 8), 0.3, 0.0),
+    "naca32x12": (lambda: meshgen.naca_ogrid(32, 12), 0.8, 1.25),
+    "naca24x10_a2": (lambda: meshgen.naca_ogrid(24, 10, t=0.10), 0.5, 2.0),
+}
+
+
+def random_states(rng, n, rho=(0.5, 1.5), vel=0.8, p=(0.4, 1.6), gamma=1.4):
+    """n conserved states (rho, rho u, rho v, E) with positive pressure."""
+    r = rng.uniform(*rho, n)
+    u = rng.uniform(-vel, vel, n)
+    v = rng.uniform(-vel, vel, n)
+    pr = rng.uniform(*p, n)
+    E = pr / (gamma - 1.0) + 0.5 * r * (u * u + v * v)
+    return np.stack([r, r * u, r * v, E], axis=1)
+
+
+def random_normals(rng, n):
+    ang = rng.uniform(0, 2 * np.pi, n)
+    mag = rng.uniform(0.01, 2.0, n)
+    return np.stack([mag * np.cos(ang), mag * np.sin(ang)], axis=1)
+
+
+def random_block_system(rng, nsys, N, dominance=6.0):
+    a = rng.uniform(-1, 1, (nsys, N, 4, 4))
+    b = rng.uniform(-1, 1, (nsys, N, 4, 4)) + dominance * np.eye(4)
+    c = rng.uniform(-1, 1, (nsys, N, 4, 4))
+    f = rng.uniform(-1, 1, (nsys, N, 4))
+    return a, b, c, f
+
+
+def perturbed_state(solver_like, rng, amp=0.02):
+    """Freestream plus a smooth-ish random perturbation, positive pressure."""
+    q0 = solver_like.freestream()
+    n = solver_like.n_cell
+    q = np.tile(q0, (n, 1))
+    q[:, 0] *= 1.0 + amp * rng.uniform(-1, 1, n)
+    q[:, 1] += amp * rng.uniform(-1, 1, n)
+    q[:, 2] += amp * rng.uniform(-1, 1, n)
+    q[:, 3] *= 1.0 + amp * rng.uniform(-1, 1, n)
+    return q.reshape(-1)
