@@ -1,0 +1,157 @@
+"""Writes the golden fixtures in this directory from the reference's own code.
+
+Runs in the build container only (the reference tree is not on the GPU box):
+the reference harness in oracle/_ref/ is compiled from the reference's
+sources by ``make -C oracle ref``.  Two builds exist, -O2 and -O0; order-5
+(WENO) cases and the WENO unit vectors use -O0 because the reference's -O2 build crashes there (WENO
+writes one element past its stack array f0, reconstruction.cpp:143,150), and
+for orders 1 and 3 the script checks that -O0 and -O2 agree before writing.
+
+Fixtures (numpy .npz, inputs and outputs only):
+  unit.npz       per-function vectors: fluxRoe2D, jacobianRoe2D, MUSCL, WENO,
+                 MUSCL_DERIV, WENO_DERIV, blockTridag4, blockTridagPeriodic4
+  case_<name>.npz  a mesh (coordinates, faces, chains), its preprocess
+                 output, a perturbed initial state, and computeRHS / ADI /
+                 5-step results of the reference SOLVER
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+sys.path.insert(0, os.path.dirname(HERE))
+
+from hamstr_oop_amd import meshgen  # noqa: E402
+import oracle_py  # noqa: E402
+from cases import random_block_system, random_normals, random_states  # noqa: E402
+
+THIRD = 0.3333333333333333
+
+CASES = [
+    # name, mesh factory, Mach, alpha, order
+    ("circle12x6_o1", lambda: meshgen.circle_ogrid(12, 6), 0.3, 0.0, 1),
+    ("circle12x6_o3", lambda: meshgen.circle_ogrid(12, 6), 0.3, 0.0, 3),
+    ("circle12x6_o5", lambda: meshgen.circle_ogrid(12, 6), 0.3, 0.0, 5),
+    ("naca16x8_o3", lambda: meshgen.naca_ogrid(16, 8), 0.8, 1.25, 3),
+    ("naca16x8_o5", lambda: meshgen.naca_ogrid(16, 8), 0.5, 2.0, 5),
+]
+
+
+def unit_vectors(ref: oracle_py.Reference, ref_weno: oracle_py.Reference) -> dict:
+    rng = np.random.default_rng(20261203)
+    out = {}
+    n = 64
+    nv = random_normals(rng, n)
+    ql = random_states(rng, n)
+    qr = random_states(rng, n)
+    ql[:8] = qr[:8]  # equal states: the consistency case
+    flux = np.zeros((n, 4))
+    sr = np.zeros(n)
+    lm = np.zeros((n, 4, 4))
+    rm = np.zeros((n, 4, 4))
+    for i in range(n):
+        flux[i], sr[i] = ref.flux_roe2d(nv[i], ql[i], qr[i])
+        lm[i], rm[i] = ref.jacobian_roe2d(nv[i], ql[i], qr[i])
+    out.update(flux_n=nv, flux_ql=ql, flux_qr=qr, flux=flux, flux_sr=sr, jac_l=lm, jac_r=rm)
+
+    imax = 19
+    f = rng.uniform(0.5, 2.0, (4, imax, 4))
+    dq = rng.uniform(-0.1, 0.1, (4, imax, 4))
+    out["recon_f"] = f
+    out["recon_dq"] = dq
+    for qt in (0.0, 0.25):
+        tag = "1" if qt == 0.0 else "3"
+        L = np.zeros_like(f)
+        R = np.zeros_like(f)
+        DL = np.zeros_like(f)
+        DR = np.zeros_like(f)
+        for c in range(f.shape[0]):
+            L[c], R[c] = ref.muscl(f[c], 1, imax - 1, THIRD, qt, 1e-10)
+            DL[c], DR[c] = ref.muscl_deriv(f[c], dq[c], 1, imax - 1, THIRD, qt, 1e-10)
+        out[f"muscl{tag}_l"], out[f"muscl{tag}_r"] = L, R
+        out[f"muscld{tag}_l"], out[f"muscld{tag}_r"] = DL, DR
+    L = np.zeros_like(f)
+    R = np.zeros_like(f)
+    DL = np.zeros_like(f)
+    DR = np.zeros_like(f)
+    for c in range(f.shape[0]):
+        L[c], R[c] = ref_weno.weno(f[c], 2, imax - 1, 1e-10)
+        DL[c], DR[c] = ref_weno.weno_deriv(f[c], dq[c], 2, imax - 1, 1e-10)
+    out["weno_l"], out["weno_r"], out["wenod_l"], out["wenod_r"] = L, R, DL, DR
+
+    for periodic in (False, True):
+        tag = "per" if periodic else "tri"
+        for N in (3, 6, 11):
+            a, b, c, rhs = random_block_system(rng, 2, N)
+            x = np.stack([ref.block_tridag4(a[s], b[s], c[s], rhs[s], periodic=periodic) for s in range(2)])
+            out[f"{tag}{N}_a"], out[f"{tag}{N}_b"], out[f"{tag}{N}_c"] = a, b, c
+            out[f"{tag}{N}_f"], out[f"{tag}{N}_x"] = rhs, x
+    return out
+
+
+def run_case(ref: oracle_py.Reference, mesh, mach, alpha, order, workdir) -> dict:
+    d = tempfile.mkdtemp(dir=workdir)
+    meshgen.write_reference_case(mesh, d, mach, alpha, order, 5, 10.0)
+    rc = ref.load(d)
+    out = {
+        "x": mesh.x, "faces_in": mesh.faces, "chain_start": mesh.chain_start, "chain_conn": mesh.chain_conn,
+        "mach": mach, "alpha": alpha, "order": order, "cfl": 10.0,
+        "faces": rc.get_ints("faces").reshape(-1, 6), "normvec": rc.get("normvec").reshape(-1, 2),
+        "refmtx": rc.get("refmtx").reshape(-1, 4), "q_freestream": rc.get("q"),
+    }
+    rng = np.random.default_rng(order * 101 + mesh.n_cell)
+    q = rc.get("q").reshape(-1, 4).copy()
+    n = q.shape[0]
+    q[:, 0] *= 1 + 0.02 * rng.uniform(-1, 1, n)
+    q[:, 1] += 0.02 * rng.uniform(-1, 1, n)
+    q[:, 2] += 0.02 * rng.uniform(-1, 1, n)
+    q[:, 3] *= 1 + 0.02 * rng.uniform(-1, 1, n)
+    q = q.reshape(-1)
+    rc.set_q(q)
+    out["q0"] = q
+    l2, li = rc.compute_rhs(True)
+    out.update(rhs_r=rc.get("r"), rhs_sigma=rc.get("sigma"), rhs_ff=rc.get("ff"), rhs_l2=l2, rhs_linf=li)
+    l2, li = rc.adi()
+    out.update(adi_q=rc.get("q"), adi_dq=rc.get("dq"), adi_r=rc.get("r"), adi_r0=rc.get("r0"),
+               adi_l2=l2, adi_linf=li)
+    l2r, l2l, _ = rc.step(5)
+    out.update(step5_q=rc.get("q"), step5_l2rhs=l2r, step5_l2lin=l2l)
+    shutil.rmtree(d, ignore_errors=True)
+    return out
+
+
+def main():
+    if not (oracle_py.Reference.available("O2") and oracle_py.Reference.available("O0")):
+        oracle_py.build(reference=True)
+    ref2 = oracle_py.Reference("O2")
+    ref0 = oracle_py.Reference("O0")
+    work = tempfile.mkdtemp()
+    u2 = unit_vectors(ref2, ref0)
+    u0 = unit_vectors(ref0, ref0)
+    for k in u2:
+        if not np.array_equal(u2[k], u0[k]):
+            raise SystemExit(f"reference -O2 and -O0 disagree on {k}")
+    np.savez_compressed(os.path.join(HERE, "unit.npz"), **u2)
+    for name, make, mach, alpha, order in CASES:
+        mesh = make()
+        data0 = run_case(ref0, mesh, mach, alpha, order, work)
+        if order != 5:
+            data2 = run_case(ref2, mesh, mach, alpha, order, work)
+            for k in data0:
+                if not np.array_equal(data0[k], data2[k]):
+                    raise SystemExit(f"{name}: reference -O2 and -O0 disagree on {k}")
+        np.savez_compressed(os.path.join(HERE, f"case_{name}.npz"), **data0)
+        print("wrote", name, flush=True)
+    shutil.rmtree(work, ignore_errors=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,126 @@
+"""The CPU oracle against the golden vectors the reference itself wrote.
+
+tests/golden/*.npz come from the reference's own code (tests/golden/
+make_golden.py); every comparison here is bit for bit.  When the reference
+harness is built (in the build container), the oracle is also compared with
+it directly on a few more cases.
+"""
+import glob
+import os
+import tempfile
+
+import numpy as np
+import pytest
+
+from cases import SMALL_CASES
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden")
+THIRD = 0.3333333333333333
+
+
+@pytest.fixture(scope="module")
+def unit():
+    return np.load(os.path.join(GOLD, "unit.npz"))
+
+
+def test_flux_and_jacobian(oracle, unit):
+    for i in range(unit["flux_n"].shape[0]):
+        f, s = oracle.flux_roe2d(unit["flux_n"][i], unit["flux_ql"][i], unit["flux_qr"][i])
+        assert np.array_equal(f, unit["flux"][i]) and s == unit["flux_sr"][i], i
+        lm, rm = oracle.jacobian_roe2d(unit["flux_n"][i], unit["flux_ql"][i], unit["flux_qr"][i])
+        assert np.array_equal(lm, unit["jac_l"][i]) and np.array_equal(rm, unit["jac_r"][i]), i
+
+
+def test_reconstruction(oracle, unit):
+    f, dq = unit["recon_f"], unit["recon_dq"]
+    imax = f.shape[1]
+    for c in range(f.shape[0]):
+        for qt, tag in ((0.0, "1"), (0.25, "3")):
+            rows = slice(1, imax) if qt == 0.0 else slice(1, imax - 1)
+            L, R = oracle.muscl(f[c], 1, imax - 1, THIRD, qt, 1e-10)
+            assert np.array_equal(L[rows], unit[f"muscl{tag}_l"][c][rows])
+            assert np.array_equal(R[rows], unit[f"muscl{tag}_r"][c][rows])
+            L, R = oracle.muscl_deriv(f[c], dq[c], 1, imax - 1, THIRD, qt, 1e-10)
+            assert np.array_equal(L[rows], unit[f"muscld{tag}_l"][c][rows])
+            assert np.array_equal(R[rows], unit[f"muscld{tag}_r"][c][rows])
+        rows = slice(2, imax - 2)
+        L, R = oracle.weno(f[c], 2, imax - 1, 1e-10)
+        assert np.array_equal(L[rows], unit["weno_l"][c][rows])
+        assert np.array_equal(R[rows], unit["weno_r"][c][rows])
+        L, R = oracle.weno_deriv(f[c], dq[c], 2, imax - 1, 1e-10)
+        assert np.array_equal(L[rows], unit["wenod_l"][c][rows])
+        assert np.array_equal(R[rows], unit["wenod_r"][c][rows])
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+@pytest.mark.parametrize("N", [3, 6, 11])
+def test_block_tridag(oracle, unit, periodic, N):
+    tag = "per" if periodic else "tri"
+    a, b, c, f, x = (unit[f"{tag}{N}_{k}"] for k in "abcfx")
+    for s in range(a.shape[0]):
+        assert np.array_equal(oracle.block_tridag4(a[s], b[s], c[s], f[s], periodic=periodic), x[s])
+
+
+class _Mesh:
+    def __init__(self, d):
+        self.x = d["x"]
+        self.faces = d["faces_in"]
+        self.chain_start = d["chain_start"]
+        self.chain_conn = d["chain_conn"]
+        self.n_cell = int(d["q0"].size // 4)
+        self.n_face = int(self.faces.shape[0])
+        self.n_chain = int(self.chain_start.size - 1)
+
+
+CASE_FILES = sorted(glob.glob(os.path.join(GOLD, "case_*.npz")))
+
+
+@pytest.mark.parametrize("path", CASE_FILES, ids=[os.path.basename(p)[5:-4] for p in CASE_FILES])
+def test_solver_case(oracle, path):
+    d = np.load(path)
+    mesh = _Mesh(d)
+    o = oracle.case(mesh, float(d["mach"]), float(d["alpha"]), int(d["order"]), float(d["cfl"]))
+    assert np.array_equal(o.faces, d["faces"]), "preprocess wall tags"
+    assert np.array_equal(o.normvec, d["normvec"])
+    assert np.array_equal(o.refmtx, d["refmtx"])
+    assert np.array_equal(o.get("q"), d["q_freestream"]), "initFlow"
+    o.set("q", d["q0"])
+    _, l2, li = o.compute_rhs(True)
+    assert l2 == d["rhs_l2"] and li == d["rhs_linf"]
+    for k in ("r", "sigma", "ff"):
+        assert np.array_equal(o.get(k), d[f"rhs_{k}"]), k
+    l2, li = o.adi()
+    assert l2 == d["adi_l2"] and li == d["adi_linf"]
+    for k in ("q", "dq", "r", "r0"):
+        assert np.array_equal(o.get(k), d[f"adi_{k}"]), k
+    _, l2r, l2l, _ = o.step(5)
+    assert np.array_equal(l2r, d["step5_l2rhs"])
+    assert np.array_equal(l2l, d["step5_l2lin"])
+    assert np.array_equal(o.get("q"), d["step5_q"])
+
+
+def _reference(opt):
+    import oracle_py
+    if not oracle_py.Reference.available(opt):
+        pytest.skip("reference harness not built (needs the reference tree; make -C oracle ref)")
+    return oracle_py.Reference(opt)
+
+
+@pytest.mark.parametrize("order", [1, 3])
+@pytest.mark.parametrize("name", list(SMALL_CASES))
+def test_oracle_vs_reference_harness(oracle, name, order):
+    """Direct cross-check with the compiled reference, 10 iterations."""
+    from hamstr_oop_amd import meshgen
+    ref = _reference("O2")
+    make, mach, alpha = SMALL_CASES[name]
+    mesh = make()
+    with tempfile.TemporaryDirectory() as d:
+        meshgen.write_reference_case(mesh, d, mach, alpha, order, 10, 10.0)
+        rc = ref.load(d)
+        o = oracle.case(mesh, mach, alpha, order, 10.0)
+        l2r, l2l, _ = rc.step(10)
+        _, o2r, o2l, _ = o.step(10)
+        assert np.array_equal(l2r, o2r) and np.array_equal(l2l, o2l)
+        for k in ("q", "r", "r0", "dq", "sigma", "ff"):
+            assert np.array_equal(rc.get(k), o.get(k)), k
