@@ -910,6 +910,7 @@ struct hamstr_ctx {
   bool timing = false;
   std::vector<hipEvent_t> ev;
   std::vector<double> last_ms;
+  std::vector<std::string> stage_names;
 
   FaceTab ftab() const { return FaceTab{pslot, src, nvec, refmtx}; }
   ChainTab ctab() const {
@@ -966,28 +967,31 @@ int launch_norm(hamstr_ctx* x, double* out) {
   return HAMSTR_OK;
 }
 
-int launch_adi_phases(hamstr_ctx* x) {
+int launch_adi_phase(hamstr_ctx* x, int ph) {
   const ChainTab ct = x->ctab();
+  const int n = x->phase_off[ph + 1] - x->phase_off[ph];
+  const int* chains = x->phase_chains + x->phase_off[ph];
+  if (x->lhs32())
+    hipLaunchKernelGGL((k_adi_exact<float>), dim3(nblk(n, 64)), dim3(64), 0, x->stream, ct, chains, n,
+                       (const float*)x->ff, x->nProc, x->c.cfl, x->sigma, x->r, (float*)x->scratch,
+                       (int)x->scratch_stride);
+  else
+    hipLaunchKernelGGL((k_adi_exact<double>), dim3(nblk(n, 64)), dim3(64), 0, x->stream, ct, chains, n,
+                       (const double*)x->ff, x->nProc, x->c.cfl, x->sigma, x->r, (double*)x->scratch,
+                       (int)x->scratch_stride);
+  HIPCHK(hipGetLastError());
+  return HAMSTR_OK;
+}
+
+int launch_adi_phases(hamstr_ctx* x) {
   for (size_t ph = 0; ph + 1 < x->phase_off.size(); ph++) {
-    const int n = x->phase_off[ph + 1] - x->phase_off[ph];
-    const int* chains = x->phase_chains + x->phase_off[ph];
-    if (x->lhs32())
-      hipLaunchKernelGGL((k_adi_exact<float>), dim3(nblk(n, 64)), dim3(64), 0, x->stream, ct, chains, n,
-                         (const float*)x->ff, x->nProc, x->c.cfl, x->sigma, x->r, (float*)x->scratch,
-                         (int)x->scratch_stride);
-    else
-      hipLaunchKernelGGL((k_adi_exact<double>), dim3(nblk(n, 64)), dim3(64), 0, x->stream, ct, chains, n,
-                         (const double*)x->ff, x->nProc, x->c.cfl, x->sigma, x->r, (double*)x->scratch,
-                         (int)x->scratch_stride);
-    HIPCHK(hipGetLastError());
+    int rc = launch_adi_phase(x, (int)ph);
+    if (rc) return rc;
   }
   return HAMSTR_OK;
 }
 
-// computeLinearRHS; update: also q += dq
-int launch_linear(hamstr_ctx* x, bool update, double* normout) {
-  int rc = launch_lin_faces(x);
-  if (rc) return rc;
+int launch_lin_cells(hamstr_ctx* x, bool update, double* normout) {
   dim3 g(nblk(x->nCell)), b(kBlock);
   if (update)
     hipLaunchKernelGGL((k_lin_cells<true>), g, b, 0, x->stream, x->c, x->cell_off, x->cell_ent, x->lflux, x->r0,
@@ -997,6 +1001,13 @@ int launch_linear(hamstr_ctx* x, bool update, double* normout) {
                        x->r0, x->dq, x->sigma, x->r, x->q, x->partial);
   HIPCHK(hipGetLastError());
   return launch_norm(x, normout);
+}
+
+// computeLinearRHS; update: also q += dq
+int launch_linear(hamstr_ctx* x, bool update, double* normout) {
+  int rc = launch_lin_faces(x);
+  if (rc) return rc;
+  return launch_lin_cells(x, update, normout);
 }
 
 // the ADI sweeps after r/r0 have been scaled (ADI.cpp:55-264)
@@ -1448,40 +1459,58 @@ int hamstr_step(hamstr_ctx* x, int nsteps, double tol, int check_every, double* 
     HIPCHK(hipMalloc(&x->hist, sizeof(double) * 4 * std::max(nsteps, 1)));
     x->histCap = nsteps;
   }
-  const int nk = 5 + 2 * (int)(x->phase_off.size() - 1) + 2;
-  if (x->timing && (int)x->ev.size() < nk + 1) {
-    for (auto e : x->ev) (void)hipEventDestroy(e);
-    x->ev.assign(nk + 1, nullptr);
-    for (auto& e : x->ev) HIPCHK(hipEventCreate(&e));
+  int rc = HAMSTR_OK;
+  // stage timing: events at every stage boundary of every step
+  std::vector<hipEvent_t> evs;
+  int nstage = 0;
+  if (x->timing) {
+    x->stage_names.clear();
+    const int nb = (int)x->ev.size();
+    (void)nb;
   }
-  int ran = 0, rc = HAMSTR_OK;
-  int lastRead = 0;
+  auto mark = [&](const char* name, int n) -> int {
+    if (!x->timing) return HAMSTR_OK;
+    if (n == 0) x->stage_names.push_back(name);
+    const size_t need = evs.size() + 1;
+    if (x->ev.size() < need) {
+      hipEvent_t e;
+      HIPCHK(hipEventCreate(&e));
+      x->ev.push_back(e);
+    }
+    hipEvent_t e = x->ev[need - 1];
+    evs.push_back(e);
+    HIPCHK(hipEventRecord(e, x->stream));
+    return HAMSTR_OK;
+  };
+  int ran = 0, lastRead = 0;
   for (int n = 0; n < nsteps; n++) {
     double* hn = x->hist + 4 * n;
-    int ie = 0;
-    auto mark = [&]() {
-      if (x->timing && n == nsteps - 1 && ie < (int)x->ev.size()) (void)hipEventRecord(x->ev[ie++], x->stream);
-    };
-    mark();
+    if ((rc = mark("start", n))) return rc;
     if ((rc = launch_rhs_faces(x, true))) return rc;
-    mark();
+    if ((rc = mark("rhs_faces", n))) return rc;
     hipLaunchKernelGGL((k_rhs_cells<true>), dim3(nblk(x->nCell)), dim3(kBlock), 0, x->stream, x->c, x->cell_off,
                        x->cell_ent, x->flux, x->sr, x->r, x->r0, x->sigma, x->partial);
     HIPCHK(hipGetLastError());
     if ((rc = launch_norm(x, hn))) return rc;
-    mark();
-    for (int s = 0; s < x->prm.msweep; s++) {
-      if ((rc = launch_adi_phases(x))) return rc;
-      mark();
+    if ((rc = mark("rhs_cells", n))) return rc;
+    for (int sw = 0; sw < x->prm.msweep; sw++) {
+      for (size_t ph = 0; ph + 1 < x->phase_off.size(); ph++) {
+        if ((rc = launch_adi_phase(x, (int)ph))) return rc;
+        if ((rc = mark(ph == 0 ? "adi_phase0" : (ph == 1 ? "adi_phase1" : "adi_phase"), n))) return rc;
+      }
       const int n4 = 4 * x->nCell;
-      if (s == 0)
+      if (sw == 0)
         hipLaunchKernelGGL((k_accum_dq<true>), dim3(nblk(n4)), dim3(kBlock), 0, x->stream, n4, x->r, x->dq);
       else
         hipLaunchKernelGGL((k_accum_dq<false>), dim3(nblk(n4)), dim3(kBlock), 0, x->stream, n4, x->r, x->dq);
       HIPCHK(hipGetLastError());
-      if ((rc = launch_linear(x, s == x->prm.msweep - 1, hn + 2))) return rc;
-      mark();
+      if ((rc = mark("accum_dq", n))) return rc;
+      if ((rc = launch_lin_faces(x))) return rc;
+      if ((rc = mark("lin_faces", n))) return rc;
+      if ((rc = launch_lin_cells(x, sw == x->prm.msweep - 1, hn + 2))) return rc;
+      if ((rc = mark("lin_cells", n))) return rc;
     }
+    if (n == 0) nstage = (int)evs.size() - 1;
     ran = n + 1;
     if (ran % check_every == 0 || ran == nsteps) {
       std::vector<double> h(4 * (ran - lastRead));
@@ -1514,14 +1543,18 @@ int hamstr_step(hamstr_ctx* x, int nsteps, double tol, int check_every, double* 
       if (conv) break;
     }
   }
-  if (x->timing) {
+  if (x->timing && nstage > 0) {
     HIPCHK(hipStreamSynchronize(x->stream));
-    x->last_ms.clear();
-    for (size_t k = 0; k + 1 < x->ev.size() && x->ev[k + 1]; k++) {
-      float ms = 0;
-      if (hipEventElapsedTime(&ms, x->ev[k], x->ev[k + 1]) != hipSuccess) break;
-      x->last_ms.push_back(ms);
-    }
+    x->last_ms.assign(nstage, 0.0);
+    const int per = nstage + 1;
+    const int steps = (int)evs.size() / per;
+    for (int k = 0; k < steps; k++)
+      for (int st = 0; st < nstage; st++) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, evs[k * per + st], evs[k * per + st + 1]));
+        x->last_ms[st] += ms / steps;
+      }
+    x->stage_names.erase(x->stage_names.begin());  // "start"
   }
   if (done) *done = ran;
   return rc;
@@ -1536,6 +1569,17 @@ int hamstr_sync(hamstr_ctx* x) {
 int hamstr_set_timing(hamstr_ctx* x, int on) {
   if (!x) return fail(HAMSTR_ERR_ARG, "null context");
   x->timing = on != 0;
+  return HAMSTR_OK;
+}
+
+int hamstr_last_timing_names(hamstr_ctx* x, char* buf, int cap) {
+  if (!x || !buf || cap <= 0) return fail(HAMSTR_ERR_ARG, "null argument");
+  std::string all;
+  for (size_t k = 0; k < x->stage_names.size(); k++) {
+    if (k) all += ",";
+    all += x->stage_names[k];
+  }
+  std::snprintf(buf, cap, "%s", all.c_str());
   return HAMSTR_OK;
 }
 

@@ -27,7 +27,7 @@ EXPORTS = [
     "hamstr_preprocess", "hamstr_create", "hamstr_destroy", "hamstr_init_flow", "hamstr_freestream",
     "hamstr_set_field", "hamstr_get_field", "hamstr_compute_rhs", "hamstr_adi",
     "hamstr_compute_linear_rhs", "hamstr_step", "hamstr_sync", "hamstr_set_timing",
-    "hamstr_last_timings", "hamstr_flux_roe2d", "hamstr_jacobian_roe2d", "hamstr_muscl", "hamstr_weno",
+    "hamstr_last_timings", "hamstr_last_timing_names", "hamstr_flux_roe2d", "hamstr_jacobian_roe2d", "hamstr_muscl", "hamstr_weno",
     "hamstr_muscl_deriv", "hamstr_weno_deriv", "hamstr_block_tridag4", "hamstr_block_tridag_periodic4",
     "hamstr_last_error",
 ]
@@ -189,11 +189,18 @@ class Solver:
     def set_timing(self, on: bool):
         _chk(lib().hamstr_set_timing(self.h, int(on)))
 
-    def last_timings(self):
+    def last_timings(self) -> dict:
+        """Average per-step ms of each stage of the last step() with timing on."""
         buf = np.zeros(64)
         n = C.c_int()
         _chk(lib().hamstr_last_timings(self.h, _d(buf), 64, C.byref(n)))
-        return buf[: n.value]
+        names = C.create_string_buffer(1024)
+        _chk(lib().hamstr_last_timing_names(self.h, names, 1024))
+        keys = names.value.decode().split(",") if names.value else []
+        out = {}
+        for k, v in zip(keys, buf[: n.value]):
+            out[k] = out.get(k, 0.0) + float(v)
+        return out
 
 
 # ---- per-function entry points ---------------------------------------------

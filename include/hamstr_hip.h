@@ -129,11 +129,14 @@ int hamstr_sync(hamstr_ctx *ctx);
 /* Turns per-kernel HIP-event timing of hamstr_step on (on != 0) or off. */
 int hamstr_set_timing(hamstr_ctx *ctx, int on);
 
-/* Per-launch duration (milliseconds, HIP events on the context's stream) of
- * the kernels of the last hamstr_step call: rhs-faces, rhs-cells, adi
- * phases, linear-faces, linear-cells+update.  *n gets the number written
- * (at most cap). */
+/* Average per-step duration (milliseconds, HIP events recorded on the
+ * context's stream around every stage of every step) of the stages of the
+ * last hamstr_step call with timing on.  *n gets the number written (at most
+ * cap); hamstr_last_timing_names gives their comma-separated names
+ * (rhs_faces, rhs_cells, adi_phase0, adi_phase1, ..., accum_dq, lin_faces,
+ * lin_cells). */
 int hamstr_last_timings(hamstr_ctx *ctx, double *ms, int cap, int *n);
+int hamstr_last_timing_names(hamstr_ctx *ctx, char *buf, int cap);
 
 /* ---- per-function entry points, batched over independent problems -------
  * Inputs and outputs are host arrays; each call copies to the device,
