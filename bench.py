@@ -1,0 +1,205 @@
+#!/usr/bin/env python3
+"""Benchmark of the line-implicit steady Euler iteration on one MI355X.
+
+Workload (default): O-grid around a NACA 0012 thickness-formula section,
+512 x 256 quads (131,072 cells, all-quad variant), MUSCL (order 3) with the
+reference's smooth limiter, M = 0.5, alpha = 2 deg, CFL = 10, msweep = 1,
+synthetic freestream start.  A "step" is one stepSolution ADI iteration:
+computeRHS("implicit") + ADI (two chain phases) + computeLinearRHS + update.
+
+Prints one JSON line: cell-iterations/s (value), the roofline of the dominant
+kernel from HIP events over the timed region, the time to nonlinear
+L2 <= 1e-10 from freestream, the FP32-LHS variant, and the reference's own CPU
+code timed on this host on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def algorithmic_bytes(n_cell, n_proc, phase_faces, b, msweep=1):
+    """Bytes per cell-iteration of the iteration's dataflow when every
+    materialised array (q, r0, sigma, ff, r, dq, face fluxes) is read and
+    written once per phase and stencil data is reused on chip.
+
+    phi = processed faces per cell, b = bytes per LHS scalar (8 FP64, 4 FP32).
+    computeRHS + Jacobians: 72 + phi*(24 + 32b); ADI phase k: 72 + 32*b*phi_k;
+    computeLinearRHS + update: 168 + phi*(8 + 32b); extra sweeps:
+    312 + phi*(8 + 64b) each.
+    """
+    phi = n_proc / n_cell
+    stages = {"rhs": 72 + phi * (24 + 32 * b)}
+    for k, nf in enumerate(phase_faces):
+        stages[f"adi_phase{k}"] = 72 + 32 * b * nf / n_cell
+    stages["linear"] = 168 + phi * (8 + 32 * b)
+    total = sum(stages.values()) + (msweep - 1) * (312 + phi * (8 + 64 * b))
+    return total, stages
+
+
+def phase_face_counts(mesh):
+    """Processed faces per ADI phase for an O-grid (loops, then lines)."""
+    import numpy as np
+    start = np.asarray(mesh.chain_start)
+    conn = np.asarray(mesh.chain_conn)
+    per = []
+    for c in range(len(start) - 1):
+        f1, f2 = start[c], start[c + 1]
+        per.append((f2 - f1) - (1 if conn[f1] == conn[f2 - 1] else 0))
+    nl = mesh.chains_per_color[0]
+    return [int(sum(per[:nl])), int(sum(per[nl:]))], int(sum(per))
+
+
+def cpu_baseline(mesh, mach, alpha, order, cfl, budget_s=12.0, max_iters=30):
+    """The reference's own code (oracle/_ref, compiled from its sources) on a
+    bounded sample: `n` iterations of the same case from freestream, 1 core.
+    Falls back to the oracle (a leak-free C++ restatement) if the reference
+    build is absent."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py
+    from hamstr_oop_amd import meshgen
+    if oracle_py.Reference.available("O2"):
+        ref = oracle_py.Reference("O2")
+        with tempfile.TemporaryDirectory() as d:
+            meshgen.write_reference_case(mesh, d, mach, alpha, order, 1, cfl)
+            rc = ref.load(d)
+            _, _, t1 = rc.step(1)
+            n = int(max(1, min(max_iters, budget_s / max(t1, 1e-6))))
+            _, _, sec = rc.step(n)
+        kind = "reference"
+        what = "reference SOLVER (computeRHS+ADI) compiled -O2 from its sources"
+    else:
+        orc = oracle_py.Oracle()
+        oc = orc.case(mesh, mach, alpha, order, cfl)
+        _, _, _, t1 = oc.step(1)
+        n = int(max(1, min(max_iters, budget_s / max(t1, 1e-6))))
+        _, _, _, sec = oc.step(n)
+        kind = "port"
+        what = "oracle C++ restatement, -O2"
+    return {
+        "value": mesh.n_cell * n / sec, "unit": "cell-iters/s", "cores": 1, "kind": kind,
+        "sample": f"{n} iterations (after 1 warm-up) of the same case from freestream; {what}; "
+                  f"{sec:.2f} s wall on {platform.processor() or platform.machine()}, nproc={os.cpu_count()}",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--nc", type=int, default=512)
+    ap.add_argument("--nr", type=int, default=256)
+    ap.add_argument("--order", type=int, default=3)
+    ap.add_argument("--mach", type=float, default=0.5)
+    ap.add_argument("--alpha", type=float, default=2.0)
+    ap.add_argument("--cfl", type=float, default=10.0)
+    ap.add_argument("--adi", choices=["exact", "fast"], default="exact")
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--max-iters", type=int, default=20000)
+    ap.add_argument("--no-converge", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=12.0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU runs are not part of it "
+                                   "(cases would shard whole meshes per GPU)", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    from hamstr_oop_amd import hip, meshgen
+    mesh = meshgen.naca_ogrid(args.nc, args.nr, t=0.12)
+    phase_faces, n_proc = phase_face_counts(mesh)
+    adi_mode = hip.ADI_FAST if args.adi == "fast" else hip.ADI_EXACT
+
+    def run(lhs):
+        s = hip.Solver(mesh, args.mach, args.alpha, args.order, args.cfl, lhs=lhs, adi_mode=adi_mode)
+        s.step(args.warmup, check_every=max(args.warmup, 1))
+        s.set_timing(True)
+        s.sync()
+        t0 = time.perf_counter()
+        n, l2r, _ = s.step(args.steps, check_every=args.steps)
+        s.sync()
+        dt = time.perf_counter() - t0
+        stages = s.last_timings()
+        s.close()
+        return n, dt, stages, float(l2r[-1])
+
+    def converge(lhs):
+        s = hip.Solver(mesh, args.mach, args.alpha, args.order, args.cfl, lhs=lhs, adi_mode=adi_mode)
+        s.sync()
+        t0 = time.perf_counter()
+        n, l2r, _ = s.step(args.max_iters, tol=args.tol, check_every=50)
+        s.sync()
+        dt = time.perf_counter() - t0
+        q = s.get("q")
+        s.close()
+        return {"iterations": n, "seconds": dt, "final_l2": float(l2r[-1]), "converged": bool(l2r[-1] <= args.tol)}, q
+
+    n, dt, stages, l2 = run(hip.LHS_FP64)
+    value = mesh.n_cell * n / dt
+    total_b, stage_b = algorithmic_bytes(mesh.n_cell, n_proc, phase_faces, 8)
+    # dominant kernel by event time; its algorithmic bytes per launch (a phase's
+    # bytes are split over the kernels of that phase in proportion to time)
+    group = {"rhs_faces": "rhs", "rhs_cells": "rhs", "accum_dq": "linear", "lin_faces": "linear",
+             "lin_cells": "linear"}
+    dom = max(stages, key=stages.get)
+    g = group.get(dom, dom)
+    g_time = sum(v for k, v in stages.items() if group.get(k, k) == g)
+    dom_bytes = stage_b[g] * mesh.n_cell * stages[dom] / g_time
+    achieved = dom_bytes / (stages[dom] * 1e-3) / 1e9
+    out = {
+        "metric": "cell-iters/s (FP64 line-implicit ADI iteration)",
+        "value": value,
+        "unit": "cell-iters/s",
+        "n_gpus": 1,
+        "steps": n,
+        "warmup": args.warmup,
+        "ms_per_step": dt / n * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: generated O-grid, freestream initial state",
+        "config": {"workload": f"NACA0012 O-grid {args.nc}x{args.nr} all-quad, order {args.order} (MUSCL), "
+                               f"M={args.mach}, alpha={args.alpha}, CFL={args.cfl}, msweep=1",
+                   "cells": mesh.n_cell, "faces": n_proc, "adi": args.adi, "lhs": "fp64"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "iteration_roofline": {"bytes_per_cell_iter": total_b,
+                               "achieved_gbs": value * total_b / 1e9,
+                               "frac": value * total_b / 1e9 / HBM_PEAK_GBS},
+        "stage_ms": {k: round(v, 5) for k, v in stages.items()},
+        "l2_after_timed_steps": l2,
+    }
+    if not args.no_converge:
+        c64, q64 = converge(hip.LHS_FP64)
+        out["time_to_tol_fp64"] = c64
+    if not args.no_fp32:
+        n32, dt32, st32, _ = run(hip.LHS_FP32)
+        out["fp32_lhs"] = {"value": mesh.n_cell * n32 / dt32, "ms_per_step": dt32 / n32 * 1e3,
+                           "stage_ms": {k: round(v, 5) for k, v in st32.items()}}
+        if not args.no_converge:
+            import numpy as np
+            c32, q32 = converge(hip.LHS_FP32)
+            scale = np.abs(q64.reshape(-1, 4)).max(axis=0)
+            c32["max_field_diff_vs_fp64_rel"] = float((np.abs(q32 - q64).reshape(-1, 4) / scale).max())
+            out["fp32_lhs"]["time_to_tol"] = c32
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(mesh, args.mach, args.alpha, args.order, args.cfl, args.cpu_budget)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
