@@ -580,6 +580,313 @@ __global__ __launch_bounds__(64) void k_adi_exact(ChainTab ct, const int* __rest
 }
 
 // --------------------------------------------------------------------------
+// ADI phase, FAST order: one workgroup per chain, one thread per block row.
+// The rows are assembled exactly as in the reference (assemble_row), each
+// row is normalised by its diagonal block (A^ = B^-1 A, C^ = B^-1 C,
+// F^ = B^-1 F), and parallel cyclic reduction in LDS eliminates the
+// neighbours at strides 1, 2, 4, ... until every row is decoupled (open
+// chains), the cyclic subsystems have two rows (solved directly) or an odd
+// number of rows (solved with the cyclic block Thomas of
+// triSolvers.cpp:850-1046 on the normalised rows).  The solution equals the
+// EXACT ordering up to rounding.  LDS holds the 36 normalised entries of every
+// row, element-major: lds[e*N + i].
+// --------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void mm4_acc(T* X, const T* A, const T* B) {
+  // X -= A*B
+#pragma clang fp contract(fast)
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      T x = X[4 * k + j];
+#pragma unroll
+      for (int l = 0; l < 4; l++) x -= A[4 * k + l] * B[4 * l + j];
+      X[4 * k + j] = x;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void mv4_acc(T* x, const T* A, const T* y) {
+#pragma clang fp contract(fast)
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    T v = x[k];
+#pragma unroll
+    for (int j = 0; j < 4; j++) v -= A[4 * k + j] * y[j];
+    x[k] = v;
+  }
+}
+
+// normalise a row in registers: a := B^-1 a, c := B^-1 c, f := B^-1 f
+template <typename T>
+__device__ __forceinline__ void normalise_row(const T* B, T* a, T* c, T* f) {
+#pragma clang fp contract(fast)
+  const LU4<T> L = lu4(B);
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    lu4_solve(L, a + j, 4);
+    lu4_solve(L, c + j, 4);
+  }
+  lu4_solve(L, f, 1);
+}
+
+template <typename T>
+__device__ __forceinline__ void lds_load_row(const T* lds, int N, int i, T* a, T* c, T* f) {
+#pragma unroll
+  for (int e = 0; e < 16; e++) {
+    a[e] = lds[e * N + i];
+    c[e] = lds[(16 + e) * N + i];
+  }
+#pragma unroll
+  for (int e = 0; e < 4; e++) f[e] = lds[(32 + e) * N + i];
+}
+
+template <typename T>
+__device__ __forceinline__ void lds_store_row(T* lds, int N, int i, const T* a, const T* c, const T* f) {
+#pragma unroll
+  for (int e = 0; e < 16; e++) {
+    lds[e * N + i] = a[e];
+    lds[(16 + e) * N + i] = c[e];
+  }
+#pragma unroll
+  for (int e = 0; e < 4; e++) lds[(32 + e) * N + i] = f[e];
+}
+
+// X -= A * (LDS matrix at element offset e0, row i); A in registers
+template <typename T>
+__device__ __forceinline__ void mm4_acc_lds(T* X, const T* A, const T* lds, int N, int e0, int i) {
+#pragma clang fp contract(fast)
+#pragma unroll
+  for (int l = 0; l < 4; l++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const T b = lds[(e0 + 4 * l + j) * N + i];
+#pragma unroll
+      for (int k = 0; k < 4; k++) X[4 * k + j] -= A[4 * k + l] * b;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void mv4_acc_lds(T* x, const T* A, const T* lds, int N, int i) {
+#pragma clang fp contract(fast)
+#pragma unroll
+  for (int l = 0; l < 4; l++) {
+    const T b = lds[(32 + l) * N + i];
+#pragma unroll
+    for (int k = 0; k < 4; k++) x[k] -= A[4 * k + l] * b;
+  }
+}
+
+enum { kFinalOpen = 0, kFinalPair = 1, kFinalOdd = 2 };
+
+template <typename T, int FINAL, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_adi_fast(ChainTab ct, const int* __restrict__ chains,
+                                                   const T* __restrict__ ff, int nProc, double cfl,
+                                                   const double* __restrict__ sigma, double* __restrict__ r) {
+#pragma clang fp contract(fast)
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* lds = reinterpret_cast<T*>(smem_raw);
+  const int ch = chains[blockIdx.x];
+  const int pb = ct.pbase[ch], nf = ct.nf[ch], N = ct.ncell[ch];
+  const bool periodic = FINAL != kFinalOpen;
+  const int i = threadIdx.x;
+  const bool active = i < N;
+
+  if (active) {
+    T a[16], B[16], c[16], f[4];
+    int cell;
+    assemble_row<T>(ct, ff, nProc, pb, nf, N, periodic, i, cfl, sigma, r, a, B, c, f, &cell);
+    if (!periodic) {
+      if (i == 0) {
+#pragma unroll
+        for (int e = 0; e < 16; e++) a[e] = T(0);
+      }
+      if (i == N - 1) {
+#pragma unroll
+        for (int e = 0; e < 16; e++) c[e] = T(0);
+      }
+    }
+    normalise_row(B, a, c, f);
+    lds_store_row(lds, N, i, a, c, f);
+  }
+  __syncthreads();
+
+  // parallel cyclic reduction
+  int d = 1;
+  int n = N;  // cyclic subsystem size
+  while (periodic ? (n % 2 == 0 && n >= 4) : (d < N)) {
+    T M[16], na[16], nc[16], nfv[4];
+    if (active) {
+      const int iL = periodic ? ((i - d) % N + N) % N : i - d;
+      const int iR = periodic ? (i + d) % N : i + d;
+#pragma unroll
+      for (int e = 0; e < 16; e++) {
+        M[e] = (e % 5 == 0) ? T(1) : T(0);
+        na[e] = T(0);
+        nc[e] = T(0);
+      }
+#pragma unroll
+      for (int e = 0; e < 4; e++) nfv[e] = lds[(32 + e) * N + i];
+      if (periodic || iL >= 0) {
+        T own[16];
+#pragma unroll
+        for (int e = 0; e < 16; e++) own[e] = lds[e * N + i];
+        mm4_acc_lds(M, own, lds, N, 16, iL);   // M -= A^_i C^_L
+        mm4_acc_lds(na, own, lds, N, 0, iL);   // A^new = -A^_i A^_L
+        mv4_acc_lds(nfv, own, lds, N, iL);     // F -= A^_i F^_L
+      }
+      if (periodic || iR < N) {
+        T own[16];
+#pragma unroll
+        for (int e = 0; e < 16; e++) own[e] = lds[(16 + e) * N + i];
+        mm4_acc_lds(M, own, lds, N, 0, iR);    // M -= C^_i A^_R
+        mm4_acc_lds(nc, own, lds, N, 16, iR);  // C^new = -C^_i C^_R
+        mv4_acc_lds(nfv, own, lds, N, iR);     // F -= C^_i F^_R
+      }
+      normalise_row(M, na, nc, nfv);
+    }
+    __syncthreads();
+    if (active) lds_store_row(lds, N, i, na, nc, nfv);
+    __syncthreads();
+    d *= 2;
+    n /= 2;
+  }
+
+  T x[4] = {T(0), T(0), T(0), T(0)};
+  if (FINAL == kFinalOpen) {
+    if (active) {
+#pragma unroll
+      for (int e = 0; e < 4; e++) x[e] = lds[(32 + e) * N + i];
+    }
+  } else if (FINAL == kFinalPair) {
+    // rows i and j = i + N/2 couple only to each other:
+    // x_i = (I - P Q)^-1 (f_i - P f_j), P = A^_i + C^_i, Q = A^_j + C^_j
+    if (active) {
+      const int j = (i + N / 2) % N;
+      T P[16], Q[16], M[16];
+#pragma unroll
+      for (int e = 0; e < 16; e++) {
+        P[e] = lds[e * N + i] + lds[(16 + e) * N + i];
+        Q[e] = lds[e * N + j] + lds[(16 + e) * N + j];
+        M[e] = (e % 5 == 0) ? T(1) : T(0);
+      }
+      mm4_acc(M, P, Q);
+#pragma unroll
+      for (int e = 0; e < 4; e++) x[e] = lds[(32 + e) * N + i];
+      mv4_acc_lds(x, P, lds, N, j);
+      const LU4<T> L = lu4(M);
+      lu4_solve(L, x, 1);
+    }
+  } else {
+    // odd cyclic subsystems {r, r+d, r+2d, ...} of n rows: the leader r < d
+    // runs the cyclic block Thomas on the normalised rows (B = I) in place
+    if (i < d) {
+      T Cp[16], Fp[4], ap[16];
+      for (int k = 0; k < n; k++) {
+        const int rk = i + k * d;
+        T A[16], Bm[16], C[16], F[4], sp[16];
+        lds_load_row(lds, N, rk, A, C, F);
+#pragma unroll
+        for (int e = 0; e < 16; e++) Bm[e] = (e % 5 == 0) ? T(1) : T(0);
+        if (k != 0) {
+          mm4_acc(Bm, A, Cp);
+          mv4_acc(F, A, Fp);
+#pragma unroll
+          for (int e = 0; e < 16; e++) sp[e] = T(0);
+          mm4_acc(sp, A, ap);
+        } else {
+#pragma unroll
+          for (int e = 0; e < 16; e++) sp[e] = A[e];
+        }
+        const LU4<T> L = lu4(Bm);
+        lu4_solve(L, F, 1);
+        if (k != n - 2) {
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            lu4_solve(L, C + j, 4);
+            lu4_solve(L, sp + j, 4);
+          }
+        } else {
+#pragma unroll
+          for (int e = 0; e < 16; e++) C[e] += sp[e];
+#pragma unroll
+          for (int j = 0; j < 4; j++) lu4_solve(L, C + j, 4);
+#pragma unroll
+          for (int e = 0; e < 16; e++) sp[e] = C[e];
+        }
+        lds_store_row(lds, N, rk, sp, C, F);  // row k holds (spike a', C', F')
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+          Cp[e] = C[e];
+          ap[e] = sp[e];
+        }
+#pragma unroll
+        for (int e = 0; e < 4; e++) Fp[e] = F[e];
+      }
+      T Bl[16], Cl[16], Fl[4];
+#pragma unroll
+      for (int e = 0; e < 16; e++) {
+        Bl[e] = (e % 5 == 0) ? T(1) : T(0);
+        Cl[e] = Cp[e];
+      }
+#pragma unroll
+      for (int e = 0; e < 4; e++) Fl[e] = Fp[e];
+      for (int k = 0; k < n - 1; k++) {
+        const int rk = i + k * d;
+        T tt[16];
+#pragma unroll
+        for (int e = 0; e < 16; e++) tt[e] = T(0);
+        mm4_acc_lds(tt, Cl, lds, N, 16, rk);
+        mm4_acc_lds(Bl, Cl, lds, N, 0, rk);
+        mv4_acc_lds(Fl, Cl, lds, N, rk);
+#pragma unroll
+        for (int e = 0; e < 16; e++) Cl[e] = tt[e];
+      }
+      const LU4<T> L = lu4(Bl);
+      lu4_solve(L, Fl, 1);
+      T xn[4] = {Fl[0], Fl[1], Fl[2], Fl[3]};
+      // back sweep; the solution overwrites the F slots
+      for (int k = n - 1; k >= 0; k--) {
+        const int rk = i + k * d;
+        T sf[4];
+        if (k == n - 1) {
+#pragma unroll
+          for (int e = 0; e < 4; e++) sf[e] = Fl[e];
+        } else {
+          T sc[16];
+#pragma unroll
+          for (int e = 0; e < 4; e++) sf[e] = lds[(32 + e) * N + rk];
+#pragma unroll
+          for (int e = 0; e < 16; e++) sc[e] = lds[(16 + e) * N + rk];
+          mv4_acc(sf, sc, xn);
+          if (k != n - 2) {
+#pragma unroll
+            for (int e = 0; e < 16; e++) sc[e] = lds[e * N + rk];
+            mv4_acc(sf, sc, Fl);
+          }
+        }
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          xn[e] = sf[e];
+          lds[(32 + e) * N + rk] = sf[e];
+        }
+      }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+      for (int e = 0; e < 4; e++) x[e] = lds[(32 + e) * N + i];
+    }
+  }
+  if (active) {
+    const int cell = ct.pleft[pb + i];
+    *reinterpret_cast<double4*>(r + 4 * (size_t)cell) =
+        make_double4(double(x[0]), double(x[1]), double(x[2]), double(x[3]));
+  }
+}
+
+// --------------------------------------------------------------------------
 // lin_faces: reconstruction derivatives + lmat*dqL + rmat*dqR per face
 // (computeLinearRHS.cpp:52-325)
 // --------------------------------------------------------------------------
@@ -897,6 +1204,8 @@ struct hamstr_ctx {
   int* phase_chains = nullptr;
   std::vector<int> phase_off;       // host: offsets into phase_chains
   std::vector<int> phase_maxrows;
+  std::vector<int> phase_fast;   // phase can use the LDS-resident FAST kernel
+  std::vector<int> phase_final;  // its final solve: open, two-row pairs, odd cyclic
   void* scratch = nullptr;
   size_t scratch_stride = 0;
   double* partial = nullptr;
@@ -971,6 +1280,27 @@ int launch_adi_phase(hamstr_ctx* x, int ph) {
   const ChainTab ct = x->ctab();
   const int n = x->phase_off[ph + 1] - x->phase_off[ph];
   const int* chains = x->phase_chains + x->phase_off[ph];
+  if (x->prm.adi_mode == HAMSTR_ADI_FAST && x->phase_fast[ph]) {
+    const int rows = x->phase_maxrows[ph];
+    const int threads = ((rows + 63) / 64) * 64;
+    const int fin = x->phase_final[ph];
+    const dim3 g(n), b(threads);
+#define FASTL(T, F, M)                                                                                    \
+  hipLaunchKernelGGL((k_adi_fast<T, F, M>), g, b, sizeof(T) * 36 * rows, x->stream, ct, chains, (const T*)x->ff, \
+                     x->nProc, x->c.cfl, x->sigma, x->r)
+    if (x->lhs32()) {
+      if (fin == kFinalOpen) FASTL(float, kFinalOpen, 1024);
+      else if (fin == kFinalPair) FASTL(float, kFinalPair, 1024);
+      else FASTL(float, kFinalOdd, 1024);
+    } else {
+      if (fin == kFinalOpen) FASTL(double, kFinalOpen, 512);
+      else if (fin == kFinalPair) FASTL(double, kFinalPair, 512);
+      else FASTL(double, kFinalOdd, 512);
+    }
+#undef FASTL
+    HIPCHK(hipGetLastError());
+    return HAMSTR_OK;
+  }
   if (x->lhs32())
     hipLaunchKernelGGL((k_adi_exact<float>), dim3(nblk(n, 64)), dim3(64), 0, x->stream, ct, chains, n,
                        (const float*)x->ff, x->nProc, x->c.cfl, x->sigma, x->r, (float*)x->scratch,
@@ -1252,6 +1582,35 @@ int hamstr_create(const hamstr_mesh_desc* mesh, const hamstr_params* params, ham
   x->nGhost = nGhost;
   x->phase_off = phase_off;
   x->phase_maxrows = phase_maxrows;
+  {
+    const size_t tl = (P.lhs == HAMSTR_LHS_FP32) ? sizeof(float) : sizeof(double);
+    hipDeviceProp_t prop;
+    size_t lds_max = 160 * 1024;
+    if (hipGetDeviceProperties(&prop, 0) == hipSuccess && prop.sharedMemPerBlock > 0) lds_max = prop.sharedMemPerBlock;
+    for (size_t ph = 0; ph < phase_maxrows.size(); ph++) {
+      const size_t bytes = tl * 36 * (size_t)phase_maxrows[ph];
+      // every chain of the phase must use the same row count for one launch shape
+      bool uniform = true;
+      for (int k = phase_off[ph]; k < phase_off[ph + 1]; k++)
+        if (ch_ncell[phase_chains[k]] != phase_maxrows[ph]) uniform = false;
+      // every chain of the phase must share the periodicity too
+      int per = -1;
+      for (int k = phase_off[ph]; k < phase_off[ph + 1]; k++) {
+        const int pk = ch_periodic[phase_chains[k]];
+        if (per == -1) per = pk;
+        if (pk != per) uniform = false;
+      }
+      const int maxt = (P.lhs == HAMSTR_LHS_FP32) ? 1024 : 512;
+      x->phase_fast.push_back(uniform && phase_maxrows[ph] <= maxt && bytes <= lds_max ? 1 : 0);
+      int fin = kFinalOpen;
+      if (per == 1) {
+        int nn = phase_maxrows[ph];
+        while (nn % 2 == 0 && nn >= 4) nn /= 2;
+        fin = (nn == 2) ? kFinalPair : kFinalOdd;
+      }
+      x->phase_final.push_back(fin);
+    }
+  }
   x->proc_face = pface;
   // freestream (initFlow.cpp:107-142)
   const double gamma = 1.4, gm1 = gamma - 1., rinf = 1.0, pinf = 1.0 / gamma;

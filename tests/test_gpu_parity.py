@@ -197,3 +197,50 @@ def test_compute_linear_rhs_bitwise(hip, oracle):
     lo, _ = o.compute_linear_rhs()
     assert lg == pytest.approx(lo, rel=1e-12)
     _eq(g.get("r"), o.get("r"), "linear residual")
+
+
+# ---- FAST ADI ordering: equal to the exact ordering up to rounding ----------
+
+def _rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+@pytest.mark.parametrize("order", [1, 3, 5])
+@pytest.mark.parametrize("name", list(SMALL_CASES))
+def test_fast_adi_matches_exact(hip, oracle, name, order):
+    """One ADI call: FP64 FAST (parallel cyclic reduction) vs the exact order;
+    FP32-LHS FAST vs FP64.  naca24x10 has 24-row loops, whose reduction ends in
+    odd (3-row) cyclic subsystems; circle16x8 ends in two-row pairs."""
+    make, mach, alpha = SMALL_CASES[name]
+    mesh = make()
+    rng = np.random.default_rng(9)
+    o = oracle.case(mesh, mach, alpha, order, 10.0)
+    q = perturbed_state(o, rng)
+    out = {}
+    for key, lhs, mode in (("exact", hip.LHS_FP64, hip.ADI_EXACT), ("fast", hip.LHS_FP64, hip.ADI_FAST),
+                           ("fast32", hip.LHS_FP32, hip.ADI_FAST)):
+        g = hip.Solver(mesh, mach, alpha, order, 10.0, lhs=lhs, adi_mode=mode)
+        g.set("q", q)
+        g.computeRHS("implicit")
+        g.ADI()
+        out[key] = g.get("dq")
+        g.close()
+    assert _rel(out["fast"], out["exact"]) < 1e-11
+    assert _rel(out["fast32"], out["exact"]) < 1e-4
+
+
+def test_fast_adi_trajectory(hip):
+    """100 fused steps from freestream: FAST and EXACT states agree to 1e-10."""
+    from hamstr_oop_amd import meshgen
+    mesh = meshgen.naca_ogrid(48, 24)
+    res = {}
+    for mode in (hip.ADI_EXACT, hip.ADI_FAST):
+        g = hip.Solver(mesh, 0.5, 2.0, 3, 10.0, adi_mode=mode)
+        n, l2, _ = g.step(100, check_every=100)
+        res[mode] = (g.get("q"), l2)
+        g.close()
+    q0 = res[hip.ADI_EXACT][0].reshape(-1, 4)
+    q1 = res[hip.ADI_FAST][0].reshape(-1, 4)
+    scale = np.abs(q0).max(axis=0)
+    assert (np.abs(q1 - q0) / scale).max() < 1e-10
+    np.testing.assert_allclose(res[hip.ADI_FAST][1], res[hip.ADI_EXACT][1], rtol=1e-8)
