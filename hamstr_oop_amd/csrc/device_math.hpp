@@ -405,4 +405,55 @@ __device__ __forceinline__ void vec_sub_mul(T* x, const T* A, const T* y) {
   }
 }
 
+// --------------------------------------------------------------------------
+// The same 4x4 LU for the FAST (reordered) solves: contracted FMAs and the
+// hardware reciprocal refined by Newton steps (two in FP64, to full precision;
+// the FP32 reciprocal is used as is).
+// --------------------------------------------------------------------------
+__device__ __forceinline__ double fast_rcp(double x) {
+#pragma clang fp contract(fast)
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+template <typename T>
+__device__ __forceinline__ LU4<T> lu4_fast(const T* b) {
+#pragma clang fp contract(fast)
+  LU4<T> L;
+  L.b11 = fast_rcp(b[0]);
+  L.u12 = b[1] * L.b11;
+  L.u13 = b[2] * L.b11;
+  L.u14 = b[3] * L.b11;
+  L.b21 = b[4];
+  L.b22 = fast_rcp(b[5] - L.b21 * L.u12);
+  L.u23 = (b[6] - L.b21 * L.u13) * L.b22;
+  L.u24 = (b[7] - L.b21 * L.u14) * L.b22;
+  L.b31 = b[8];
+  L.b32 = b[9] - L.b31 * L.u12;
+  L.b33 = fast_rcp(b[10] - L.b31 * L.u13 - L.b32 * L.u23);
+  L.u34 = (b[11] - L.b31 * L.u14 - L.b32 * L.u24) * L.b33;
+  L.b41 = b[12];
+  L.b42 = b[13] - L.b41 * L.u12;
+  L.b43 = b[14] - L.b41 * L.u13 - L.b42 * L.u23;
+  L.b44 = fast_rcp(b[15] - L.b41 * L.u14 - L.b42 * L.u24 - L.b43 * L.u34);
+  return L;
+}
+
+template <typename T>
+__device__ __forceinline__ void lu4_solve_fast(const LU4<T>& L, T* v, int s) {
+#pragma clang fp contract(fast)
+  const T d1 = v[0] * L.b11;
+  const T d2 = (v[s] - L.b21 * d1) * L.b22;
+  const T d3 = (v[2 * s] - L.b31 * d1 - L.b32 * d2) * L.b33;
+  const T d4 = (v[3 * s] - L.b41 * d1 - L.b42 * d2 - L.b43 * d3) * L.b44;
+  v[3 * s] = d4;
+  v[2 * s] = d3 - L.u34 * d4;
+  v[s] = d2 - L.u23 * v[2 * s] - L.u24 * d4;
+  v[0] = d1 - L.u12 * v[s] - L.u13 * v[2 * s] - L.u14 * d4;
+}
+
 }  // namespace hamstr

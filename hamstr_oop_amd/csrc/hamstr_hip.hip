@@ -24,6 +24,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -192,9 +193,14 @@ __global__ __launch_bounds__(kBlock) void k_rhs_faces(FaceTab ft, Consts c, cons
 // --------------------------------------------------------------------------
 // block reductions for the density-residual norms
 // --------------------------------------------------------------------------
-__device__ __forceinline__ void block_norm_partial(double sq, double mx, double* partial) {
+__device__ __forceinline__ void block_norm(double sq, double mx, double* __restrict__ partial,
+                                           unsigned* __restrict__ counter, int nCell, double* __restrict__ out) {
+  // Per-block partial (sum of r_rho^2, max |r_rho|), then the last block to
+  // finish reduces the partials in block order, so the result does not depend
+  // on scheduling.  L2 = sqrt(sum)/nCell, Linf = max (computeRHS.cpp:415-425).
   __shared__ double ssq[kBlock];
   __shared__ double smx[kBlock];
+  __shared__ bool last;
   const int t = threadIdx.x;
   ssq[t] = sq;
   smx[t] = mx;
@@ -209,24 +215,22 @@ __device__ __forceinline__ void block_norm_partial(double sq, double mx, double*
   if (t == 0) {
     partial[2 * blockIdx.x] = ssq[0];
     partial[2 * blockIdx.x + 1] = smx[0];
+    __threadfence();
+    last = (atomicAdd(counter, 1u) == gridDim.x - 1);
   }
-}
-
-// L2 = sqrt(sum r_rho^2) / nCell, Linf = max |r_rho| (computeRHS.cpp:415-425)
-__global__ __launch_bounds__(kRedBlock) void k_norm_final(const double* __restrict__ partial, int nPart,
-                                                          int nCell, double* __restrict__ out) {
-  __shared__ double ssq[kRedBlock];
-  __shared__ double smx[kRedBlock];
-  const int t = threadIdx.x;
+  __syncthreads();
+  if (!last) return;
+  __threadfence();
+  const volatile double* vp = partial;
   double a = 0.0, m = 0.0;
-  for (int i = t; i < nPart; i += kRedBlock) {
-    a += partial[2 * i];
-    m = fmax(m, partial[2 * i + 1]);
+  for (int i = t; i < (int)gridDim.x; i += kBlock) {
+    a += vp[2 * i];
+    m = fmax(m, vp[2 * i + 1]);
   }
   ssq[t] = a;
   smx[t] = m;
   __syncthreads();
-  for (int w = kRedBlock / 2; w > 0; w >>= 1) {
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
     if (t < w) {
       ssq[t] += ssq[t + w];
       smx[t] = fmax(smx[t], smx[t + w]);
@@ -236,6 +240,7 @@ __global__ __launch_bounds__(kRedBlock) void k_norm_final(const double* __restri
   if (t == 0) {
     out[0] = sqrt(ssq[0]) / nCell;
     out[1] = smx[0];
+    *counter = 0u;
   }
 }
 
@@ -249,7 +254,8 @@ __global__ __launch_bounds__(kBlock) void k_rhs_cells(Consts c, const int* __res
                                                       const double* __restrict__ flux,
                                                       const double* __restrict__ sr, double* __restrict__ r,
                                                       double* __restrict__ r0, double* __restrict__ sigma,
-                                                      double* __restrict__ partial) {
+                                                      double* __restrict__ partial, unsigned* __restrict__ counter,
+                                                      double* __restrict__ normout) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double sq = 0.0, mx = 0.0;
   if (i < c.nCell) {
@@ -278,7 +284,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_cells(Consts c, const int* __res
       *reinterpret_cast<double4*>(r + 4 * (size_t)i) = make_double4(a0, a1, a2, a3);
     }
   }
-  block_norm_partial(sq, mx, partial);
+  block_norm(sq, mx, partial, counter, c.nCell, normout);
 }
 
 // ADI.cpp:40-49 on its own (API path)
@@ -403,6 +409,23 @@ __device__ __forceinline__ void assemble_row(const ChainTab& ct, const T* __rest
   F[3] = T(rv.w);
 }
 
+// r[cell] = x; with dqmode 1 also dq[cell] = 0 + x, with 2 dq[cell] += x
+// (ADI.cpp:240, 247-248, fused into the last phase of a sweep when that
+// phase covers every cell)
+template <typename T>
+__device__ __forceinline__ void store_solution(double* __restrict__ r, double* __restrict__ dq, int dqmode,
+                                               int cell, const T* x) {
+  const double4 v = make_double4(double(x[0]), double(x[1]), double(x[2]), double(x[3]));
+  *reinterpret_cast<double4*>(r + 4 * (size_t)cell) = v;
+  if (dqmode == 1) {
+    *reinterpret_cast<double4*>(dq + 4 * (size_t)cell) = make_double4(0.0 + v.x, 0.0 + v.y, 0.0 + v.z, 0.0 + v.w);
+  } else if (dqmode == 2) {
+    double4 d = *reinterpret_cast<double4*>(dq + 4 * (size_t)cell);
+    d.x += v.x; d.y += v.y; d.z += v.z; d.w += v.w;
+    *reinterpret_cast<double4*>(dq + 4 * (size_t)cell) = d;
+  }
+}
+
 // --------------------------------------------------------------------------
 // ADI phase, exact order: one thread per chain runs the reference's block
 // Thomas (blockTridag4) or cyclic variant (blockTridagPeriodic4) and writes
@@ -414,7 +437,8 @@ template <typename T>
 __global__ __launch_bounds__(64) void k_adi_exact(ChainTab ct, const int* __restrict__ chains, int nch,
                                                   const T* __restrict__ ff, int nProc, double cfl,
                                                   const double* __restrict__ sigma, double* __restrict__ r,
-                                                  T* __restrict__ scratch, int stride) {
+                                                  T* __restrict__ scratch, int stride, double* __restrict__ dq,
+                                                  int dqmode) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nch) return;
   const int ch = chains[t];
@@ -487,8 +511,7 @@ __global__ __launch_bounds__(64) void k_adi_exact(ChainTab ct, const int* __rest
 
   auto put = [&](int i, const T* x) {
     const int cell = ct.pleft[pb + i];
-    *reinterpret_cast<double4*>(r + 4 * (size_t)cell) = make_double4(double(x[0]), double(x[1]), double(x[2]),
-                                                                     double(x[3]));
+    store_solution(r, dq, dqmode, cell, x);
   };
   if (!periodic) {
     T x[4] = {Fp[0], Fp[1], Fp[2], Fp[3]};
@@ -622,13 +645,13 @@ __device__ __forceinline__ void mv4_acc(T* x, const T* A, const T* y) {
 template <typename T>
 __device__ __forceinline__ void normalise_row(const T* B, T* a, T* c, T* f) {
 #pragma clang fp contract(fast)
-  const LU4<T> L = lu4(B);
+  const LU4<T> L = lu4_fast(B);
 #pragma unroll
   for (int j = 0; j < 4; j++) {
-    lu4_solve(L, a + j, 4);
-    lu4_solve(L, c + j, 4);
+    lu4_solve_fast(L, a + j, 4);
+    lu4_solve_fast(L, c + j, 4);
   }
-  lu4_solve(L, f, 1);
+  lu4_solve_fast(L, f, 1);
 }
 
 template <typename T>
@@ -683,7 +706,8 @@ enum { kFinalOpen = 0, kFinalPair = 1, kFinalOdd = 2 };
 template <typename T, int FINAL, int MAXT>
 __global__ __launch_bounds__(MAXT) void k_adi_fast(ChainTab ct, const int* __restrict__ chains,
                                                    const T* __restrict__ ff, int nProc, double cfl,
-                                                   const double* __restrict__ sigma, double* __restrict__ r) {
+                                                   const double* __restrict__ sigma, double* __restrict__ r,
+                                                   double* __restrict__ dq, int dqmode) {
 #pragma clang fp contract(fast)
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* lds = reinterpret_cast<T*>(smem_raw);
@@ -775,8 +799,8 @@ __global__ __launch_bounds__(MAXT) void k_adi_fast(ChainTab ct, const int* __res
 #pragma unroll
       for (int e = 0; e < 4; e++) x[e] = lds[(32 + e) * N + i];
       mv4_acc_lds(x, P, lds, N, j);
-      const LU4<T> L = lu4(M);
-      lu4_solve(L, x, 1);
+      const LU4<T> L = lu4_fast(M);
+      lu4_solve_fast(L, x, 1);
     }
   } else {
     // odd cyclic subsystems {r, r+d, r+2d, ...} of n rows: the leader r < d
@@ -799,19 +823,19 @@ __global__ __launch_bounds__(MAXT) void k_adi_fast(ChainTab ct, const int* __res
 #pragma unroll
           for (int e = 0; e < 16; e++) sp[e] = A[e];
         }
-        const LU4<T> L = lu4(Bm);
-        lu4_solve(L, F, 1);
+        const LU4<T> L = lu4_fast(Bm);
+        lu4_solve_fast(L, F, 1);
         if (k != n - 2) {
 #pragma unroll
           for (int j = 0; j < 4; j++) {
-            lu4_solve(L, C + j, 4);
-            lu4_solve(L, sp + j, 4);
+            lu4_solve_fast(L, C + j, 4);
+            lu4_solve_fast(L, sp + j, 4);
           }
         } else {
 #pragma unroll
           for (int e = 0; e < 16; e++) C[e] += sp[e];
 #pragma unroll
-          for (int j = 0; j < 4; j++) lu4_solve(L, C + j, 4);
+          for (int j = 0; j < 4; j++) lu4_solve_fast(L, C + j, 4);
 #pragma unroll
           for (int e = 0; e < 16; e++) sp[e] = C[e];
         }
@@ -843,8 +867,8 @@ __global__ __launch_bounds__(MAXT) void k_adi_fast(ChainTab ct, const int* __res
 #pragma unroll
         for (int e = 0; e < 16; e++) Cl[e] = tt[e];
       }
-      const LU4<T> L = lu4(Bl);
-      lu4_solve(L, Fl, 1);
+      const LU4<T> L = lu4_fast(Bl);
+      lu4_solve_fast(L, Fl, 1);
       T xn[4] = {Fl[0], Fl[1], Fl[2], Fl[3]};
       // back sweep; the solution overwrites the F slots
       for (int k = n - 1; k >= 0; k--) {
@@ -879,11 +903,7 @@ __global__ __launch_bounds__(MAXT) void k_adi_fast(ChainTab ct, const int* __res
       for (int e = 0; e < 4; e++) x[e] = lds[(32 + e) * N + i];
     }
   }
-  if (active) {
-    const int cell = ct.pleft[pb + i];
-    *reinterpret_cast<double4*>(r + 4 * (size_t)cell) =
-        make_double4(double(x[0]), double(x[1]), double(x[2]), double(x[3]));
-  }
+  if (active) store_solution(r, dq, dqmode, ct.pleft[pb + i], x);
 }
 
 // --------------------------------------------------------------------------
@@ -940,7 +960,8 @@ __global__ __launch_bounds__(kBlock) void k_lin_cells(Consts c, const int* __res
                                                       const double* __restrict__ lflux,
                                                       const double* __restrict__ r0, const double* __restrict__ dq,
                                                       const double* __restrict__ sigma, double* __restrict__ r,
-                                                      double* __restrict__ q, double* __restrict__ partial) {
+                                                      double* __restrict__ q, double* __restrict__ partial,
+                                                      unsigned* __restrict__ counter, double* __restrict__ normout) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double sq = 0.0, mx = 0.0;
   if (i < c.nCell) {
@@ -968,7 +989,7 @@ __global__ __launch_bounds__(kBlock) void k_lin_cells(Consts c, const int* __res
       *reinterpret_cast<double4*>(q + 4 * (size_t)i) = qv;
     }
   }
-  block_norm_partial(sq, mx, partial);
+  block_norm(sq, mx, partial, counter, c.nCell, normout);
 }
 
 __global__ void k_update(int n4, double* __restrict__ q, const double* __restrict__ dq) {
@@ -1206,6 +1227,7 @@ struct hamstr_ctx {
   std::vector<int> phase_maxrows;
   std::vector<int> phase_fast;   // phase can use the LDS-resident FAST kernel
   std::vector<int> phase_final;  // its final solve: open, two-row pairs, odd cyclic
+  bool last_phase_all_cells = false;
   void* scratch = nullptr;
   size_t scratch_stride = 0;
   double* partial = nullptr;
@@ -1214,6 +1236,7 @@ struct hamstr_ctx {
   double* hist = nullptr;   // [2*histCap]
   int histCap = 0;
   int* nanflag = nullptr;
+  unsigned* counter = nullptr;  // last-block counter of the norm reductions
   std::vector<int> proc_face;  // host: face id per processed face
   // timing
   bool timing = false;
@@ -1270,13 +1293,19 @@ int launch_lin_faces(hamstr_ctx* x) {
   return HAMSTR_OK;
 }
 
-int launch_norm(hamstr_ctx* x, double* out) {
-  hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(kRedBlock), 0, x->stream, x->partial, x->nPart, x->nCell, out);
+int launch_rhs_cells(hamstr_ctx* x, bool adi_init, double* normout) {
+  if (adi_init)
+    hipLaunchKernelGGL((k_rhs_cells<true>), dim3(nblk(x->nCell)), dim3(kBlock), 0, x->stream, x->c, x->cell_off,
+                       x->cell_ent, x->flux, x->sr, x->r, x->r0, x->sigma, x->partial, x->counter, normout);
+  else
+    hipLaunchKernelGGL((k_rhs_cells<false>), dim3(nblk(x->nCell)), dim3(kBlock), 0, x->stream, x->c,
+                       x->cell_off, x->cell_ent, x->flux, x->sr, x->r, x->r0, x->sigma, x->partial, x->counter,
+                       normout);
   HIPCHK(hipGetLastError());
   return HAMSTR_OK;
 }
 
-int launch_adi_phase(hamstr_ctx* x, int ph) {
+int launch_adi_phase(hamstr_ctx* x, int ph, int dqmode) {
   const ChainTab ct = x->ctab();
   const int n = x->phase_off[ph + 1] - x->phase_off[ph];
   const int* chains = x->phase_chains + x->phase_off[ph];
@@ -1287,7 +1316,7 @@ int launch_adi_phase(hamstr_ctx* x, int ph) {
     const dim3 g(n), b(threads);
 #define FASTL(T, F, M)                                                                                    \
   hipLaunchKernelGGL((k_adi_fast<T, F, M>), g, b, sizeof(T) * 36 * rows, x->stream, ct, chains, (const T*)x->ff, \
-                     x->nProc, x->c.cfl, x->sigma, x->r)
+                     x->nProc, x->c.cfl, x->sigma, x->r, x->dq, dqmode)
     if (x->lhs32()) {
       if (fin == kFinalOpen) FASTL(float, kFinalOpen, 1024);
       else if (fin == kFinalPair) FASTL(float, kFinalPair, 1024);
@@ -1304,19 +1333,33 @@ int launch_adi_phase(hamstr_ctx* x, int ph) {
   if (x->lhs32())
     hipLaunchKernelGGL((k_adi_exact<float>), dim3(nblk(n, 64)), dim3(64), 0, x->stream, ct, chains, n,
                        (const float*)x->ff, x->nProc, x->c.cfl, x->sigma, x->r, (float*)x->scratch,
-                       (int)x->scratch_stride);
+                       (int)x->scratch_stride, x->dq, dqmode);
   else
     hipLaunchKernelGGL((k_adi_exact<double>), dim3(nblk(n, 64)), dim3(64), 0, x->stream, ct, chains, n,
                        (const double*)x->ff, x->nProc, x->c.cfl, x->sigma, x->r, (double*)x->scratch,
-                       (int)x->scratch_stride);
+                       (int)x->scratch_stride, x->dq, dqmode);
   HIPCHK(hipGetLastError());
   return HAMSTR_OK;
 }
 
-int launch_adi_phases(hamstr_ctx* x) {
-  for (size_t ph = 0; ph + 1 < x->phase_off.size(); ph++) {
-    int rc = launch_adi_phase(x, (int)ph);
+// all phases of one sweep; the last phase also accumulates dq when it covers
+// every cell, otherwise a separate kernel does (ADI.cpp:57-248)
+int launch_sweep_phases(hamstr_ctx* x, int sweep, const std::function<int(const char*)>& mark) {
+  const int nph = (int)x->phase_off.size() - 1;
+  for (int ph = 0; ph < nph; ph++) {
+    const bool fuse = (ph == nph - 1) && x->last_phase_all_cells;
+    const int rc = launch_adi_phase(x, ph, fuse ? (sweep == 0 ? 1 : 2) : 0);
     if (rc) return rc;
+    if (mark && (ph == 0 ? mark("adi_phase0") : (ph == 1 ? mark("adi_phase1") : mark("adi_phase")))) return HAMSTR_ERR_HIP;
+  }
+  if (!x->last_phase_all_cells) {
+    const int n4 = 4 * x->nCell;
+    if (sweep == 0)
+      hipLaunchKernelGGL((k_accum_dq<true>), dim3(nblk(n4)), dim3(kBlock), 0, x->stream, n4, x->r, x->dq);
+    else
+      hipLaunchKernelGGL((k_accum_dq<false>), dim3(nblk(n4)), dim3(kBlock), 0, x->stream, n4, x->r, x->dq);
+    HIPCHK(hipGetLastError());
+    if (mark && mark("accum_dq")) return HAMSTR_ERR_HIP;
   }
   return HAMSTR_OK;
 }
@@ -1325,12 +1368,12 @@ int launch_lin_cells(hamstr_ctx* x, bool update, double* normout) {
   dim3 g(nblk(x->nCell)), b(kBlock);
   if (update)
     hipLaunchKernelGGL((k_lin_cells<true>), g, b, 0, x->stream, x->c, x->cell_off, x->cell_ent, x->lflux, x->r0,
-                       x->dq, x->sigma, x->r, x->q, x->partial);
+                       x->dq, x->sigma, x->r, x->q, x->partial, x->counter, normout);
   else
     hipLaunchKernelGGL((k_lin_cells<false>), g, b, 0, x->stream, x->c, x->cell_off, x->cell_ent, x->lflux,
-                       x->r0, x->dq, x->sigma, x->r, x->q, x->partial);
+                       x->r0, x->dq, x->sigma, x->r, x->q, x->partial, x->counter, normout);
   HIPCHK(hipGetLastError());
-  return launch_norm(x, normout);
+  return HAMSTR_OK;
 }
 
 // computeLinearRHS; update: also q += dq
@@ -1342,15 +1385,9 @@ int launch_linear(hamstr_ctx* x, bool update, double* normout) {
 
 // the ADI sweeps after r/r0 have been scaled (ADI.cpp:55-264)
 int launch_sweeps(hamstr_ctx* x, double* linnorm) {
-  const int n4 = 4 * x->nCell;
   for (int s = 0; s < x->prm.msweep; s++) {
-    int rc = launch_adi_phases(x);
+    int rc = launch_sweep_phases(x, s, nullptr);
     if (rc) return rc;
-    if (s == 0)
-      hipLaunchKernelGGL((k_accum_dq<true>), dim3(nblk(n4)), dim3(kBlock), 0, x->stream, n4, x->r, x->dq);
-    else
-      hipLaunchKernelGGL((k_accum_dq<false>), dim3(nblk(n4)), dim3(kBlock), 0, x->stream, n4, x->r, x->dq);
-    HIPCHK(hipGetLastError());
     rc = launch_linear(x, s == x->prm.msweep - 1, linnorm);
     if (rc) return rc;
   }
@@ -1375,7 +1412,7 @@ void free_ctx(hamstr_ctx* x) {
   void* ptrs[] = {x->q, x->r, x->r0, x->dq, x->sigma, x->flux, x->sr, x->lflux, x->ff, x->pslot, x->src,
                   x->nvec, x->refmtx, x->pleft, x->pright, x->pface, x->cell_off, x->cell_ent, x->ch_pbase,
                   x->ch_nf, x->ch_ncell, x->ch_periodic, x->phase_chains, x->scratch, x->partial, x->norms,
-                  x->hist, x->nanflag};
+                  x->hist, x->nanflag, x->counter};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto e : x->ev) (void)hipEventDestroy(e);
@@ -1583,6 +1620,12 @@ int hamstr_create(const hamstr_mesh_desc* mesh, const hamstr_params* params, ham
   x->phase_off = phase_off;
   x->phase_maxrows = phase_maxrows;
   {
+    // cells within one phase are disjoint, so covering = counting
+    long cover = 0;
+    for (int k = phase_off[nPhase - 1]; k < phase_off[nPhase]; k++) cover += ch_ncell[phase_chains[k]];
+    x->last_phase_all_cells = (cover == nCell);
+  }
+  {
     const size_t tl = (P.lhs == HAMSTR_LHS_FP32) ? sizeof(float) : sizeof(double);
     hipDeviceProp_t prop;
     size_t lds_max = 160 * 1024;
@@ -1654,6 +1697,7 @@ int hamstr_create(const hamstr_mesh_desc* mesh, const hamstr_params* params, ham
   chk(hipMalloc(&x->partial, sizeof(double) * 2 * x->nPart), "alloc partial");
   chk(hipMalloc(&x->norms, sizeof(double) * 4), "alloc norms");
   chk(hipMalloc(&x->nanflag, sizeof(int)), "alloc nanflag");
+  chk(hipMalloc(&x->counter, sizeof(unsigned)), "alloc counter");
   if (rc) {
     free_ctx(x);
     delete x;
@@ -1671,6 +1715,7 @@ int hamstr_create(const hamstr_mesh_desc* mesh, const hamstr_params* params, ham
     return rc;
   }
   chk(hipMemset(x->nanflag, 0, sizeof(int)), "memset nanflag");
+  chk(hipMemset(x->counter, 0, sizeof(unsigned)), "memset counter");
   chk(hipMemset(x->norms, 0, sizeof(double) * 4), "memset norms");
   chk(hipMemset(x->dq, 0, c4), "memset dq");
   chk(hipMemset(x->r, 0, c4), "memset r");
@@ -1776,10 +1821,7 @@ int hamstr_compute_rhs(hamstr_ctx* x, int implicit, double* L2, double* Linf) {
   if (!x) return fail(HAMSTR_ERR_ARG, "null context");
   int rc = launch_rhs_faces(x, implicit != 0);
   if (rc) return rc;
-  hipLaunchKernelGGL((k_rhs_cells<false>), dim3(nblk(x->nCell)), dim3(kBlock), 0, x->stream, x->c, x->cell_off,
-                     x->cell_ent, x->flux, x->sr, x->r, x->r0, x->sigma, x->partial);
-  HIPCHK(hipGetLastError());
-  if ((rc = launch_norm(x, x->norms))) return rc;
+  if ((rc = launch_rhs_cells(x, false, x->norms))) return rc;
   rc = read_norms(x);
   if (L2) *L2 = x->L2rhs;
   if (Linf) *Linf = x->Linfrhs;
@@ -1847,23 +1889,11 @@ int hamstr_step(hamstr_ctx* x, int nsteps, double tol, int check_every, double* 
     if ((rc = mark("start", n))) return rc;
     if ((rc = launch_rhs_faces(x, true))) return rc;
     if ((rc = mark("rhs_faces", n))) return rc;
-    hipLaunchKernelGGL((k_rhs_cells<true>), dim3(nblk(x->nCell)), dim3(kBlock), 0, x->stream, x->c, x->cell_off,
-                       x->cell_ent, x->flux, x->sr, x->r, x->r0, x->sigma, x->partial);
-    HIPCHK(hipGetLastError());
-    if ((rc = launch_norm(x, hn))) return rc;
+    if ((rc = launch_rhs_cells(x, true, hn))) return rc;
     if ((rc = mark("rhs_cells", n))) return rc;
     for (int sw = 0; sw < x->prm.msweep; sw++) {
-      for (size_t ph = 0; ph + 1 < x->phase_off.size(); ph++) {
-        if ((rc = launch_adi_phase(x, (int)ph))) return rc;
-        if ((rc = mark(ph == 0 ? "adi_phase0" : (ph == 1 ? "adi_phase1" : "adi_phase"), n))) return rc;
-      }
-      const int n4 = 4 * x->nCell;
-      if (sw == 0)
-        hipLaunchKernelGGL((k_accum_dq<true>), dim3(nblk(n4)), dim3(kBlock), 0, x->stream, n4, x->r, x->dq);
-      else
-        hipLaunchKernelGGL((k_accum_dq<false>), dim3(nblk(n4)), dim3(kBlock), 0, x->stream, n4, x->r, x->dq);
-      HIPCHK(hipGetLastError());
-      if ((rc = mark("accum_dq", n))) return rc;
+      const int n_ = n;
+      if ((rc = launch_sweep_phases(x, sw, [&](const char* nm) { return mark(nm, n_); }))) return rc;
       if ((rc = launch_lin_faces(x))) return rc;
       if ((rc = mark("lin_faces", n))) return rc;
       if ((rc = launch_lin_cells(x, sw == x->prm.msweep - 1, hn + 2))) return rc;
