@@ -104,12 +104,13 @@ def main():
     ap.add_argument("--mach", type=float, default=0.5)
     ap.add_argument("--alpha", type=float, default=2.0)
     ap.add_argument("--cfl", type=float, default=10.0)
-    ap.add_argument("--adi", choices=["exact", "fast"], default="exact")
+    ap.add_argument("--adi", choices=["exact", "fast"], default="fast")
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--max-iters", type=int, default=20000)
     ap.add_argument("--no-converge", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-exact", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     args = ap.parse_args()
 
@@ -123,21 +124,24 @@ def main():
     phase_faces, n_proc = phase_face_counts(mesh)
     adi_mode = hip.ADI_FAST if args.adi == "fast" else hip.ADI_EXACT
 
-    def run(lhs):
-        s = hip.Solver(mesh, args.mach, args.alpha, args.order, args.cfl, lhs=lhs, adi_mode=adi_mode)
+    def run(lhs, mode=None, steps=None):
+        mode = adi_mode if mode is None else mode
+        steps = args.steps if steps is None else steps
+        s = hip.Solver(mesh, args.mach, args.alpha, args.order, args.cfl, lhs=lhs, adi_mode=mode)
         s.step(args.warmup, check_every=max(args.warmup, 1))
         s.set_timing(True)
         s.sync()
         t0 = time.perf_counter()
-        n, l2r, _ = s.step(args.steps, check_every=args.steps)
+        n, l2r, _ = s.step(steps, check_every=steps)
         s.sync()
         dt = time.perf_counter() - t0
         stages = s.last_timings()
         s.close()
         return n, dt, stages, float(l2r[-1])
 
-    def converge(lhs):
-        s = hip.Solver(mesh, args.mach, args.alpha, args.order, args.cfl, lhs=lhs, adi_mode=adi_mode)
+    def converge(lhs, mode=None):
+        mode = adi_mode if mode is None else mode
+        s = hip.Solver(mesh, args.mach, args.alpha, args.order, args.cfl, lhs=lhs, adi_mode=mode)
         s.sync()
         t0 = time.perf_counter()
         n, l2r, _ = s.step(args.max_iters, tol=args.tol, check_every=50)
@@ -174,7 +178,11 @@ def main():
         "data": "synthetic: generated O-grid, freestream initial state",
         "config": {"workload": f"NACA0012 O-grid {args.nc}x{args.nr} all-quad, order {args.order} (MUSCL), "
                                f"M={args.mach}, alpha={args.alpha}, CFL={args.cfl}, msweep=1",
-                   "cells": mesh.n_cell, "faces": n_proc, "adi": args.adi, "lhs": "fp64"},
+                   "cells": mesh.n_cell, "faces": n_proc, "lhs": "fp64",
+                   "adi": args.adi + (" (parallel cyclic reduction per chain: residual, norms and update "
+                                      "bit-identical to the reference; block solves equal up to rounding)"
+                                      if args.adi == "fast" else " (the reference's sequential block Thomas, "
+                                      "bit-identical)")},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "iteration_roofline": {"bytes_per_cell_iter": total_b,
@@ -186,6 +194,16 @@ def main():
     if not args.no_converge:
         c64, q64 = converge(hip.LHS_FP64)
         out["time_to_tol_fp64"] = c64
+    if args.adi == "fast" and not args.no_exact:
+        ne, dte, ste, _ = run(hip.LHS_FP64, hip.ADI_EXACT, steps=min(args.steps, 30))
+        out["exact_adi_fp64"] = {"value": mesh.n_cell * ne / dte, "ms_per_step": dte / ne * 1e3,
+                                 "stage_ms": {k: round(v, 5) for k, v in ste.items()}}
+        if not args.no_converge:
+            import numpy as np
+            ce, qe = converge(hip.LHS_FP64, hip.ADI_EXACT)
+            scale = np.abs(qe.reshape(-1, 4)).max(axis=0)
+            ce["max_field_diff_vs_fast_rel"] = float((np.abs(q64 - qe).reshape(-1, 4) / scale).max())
+            out["exact_adi_fp64"]["time_to_tol"] = ce
     if not args.no_fp32:
         n32, dt32, st32, _ = run(hip.LHS_FP32)
         out["fp32_lhs"] = {"value": mesh.n_cell * n32 / dt32, "ms_per_step": dt32 / n32 * 1e3,

@@ -193,14 +193,10 @@ __global__ __launch_bounds__(kBlock) void k_rhs_faces(FaceTab ft, Consts c, cons
 // --------------------------------------------------------------------------
 // block reductions for the density-residual norms
 // --------------------------------------------------------------------------
-__device__ __forceinline__ void block_norm(double sq, double mx, double* __restrict__ partial,
-                                           unsigned* __restrict__ counter, int nCell, double* __restrict__ out) {
-  // Per-block partial (sum of r_rho^2, max |r_rho|), then the last block to
-  // finish reduces the partials in block order, so the result does not depend
-  // on scheduling.  L2 = sqrt(sum)/nCell, Linf = max (computeRHS.cpp:415-425).
+__device__ __forceinline__ void block_norm_partial(double sq, double mx, double* __restrict__ partial) {
+  // per-block partial (sum of r_rho^2, max |r_rho|); k_norm_final reduces them
   __shared__ double ssq[kBlock];
   __shared__ double smx[kBlock];
-  __shared__ bool last;
   const int t = threadIdx.x;
   ssq[t] = sq;
   smx[t] = mx;
@@ -215,22 +211,25 @@ __device__ __forceinline__ void block_norm(double sq, double mx, double* __restr
   if (t == 0) {
     partial[2 * blockIdx.x] = ssq[0];
     partial[2 * blockIdx.x + 1] = smx[0];
-    __threadfence();
-    last = (atomicAdd(counter, 1u) == gridDim.x - 1);
   }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  const volatile double* vp = partial;
+}
+
+// L2 = sqrt(sum r_rho^2) / nCell, Linf = max |r_rho| (computeRHS.cpp:415-425),
+// partials reduced in block order so the result does not depend on scheduling
+__global__ __launch_bounds__(kRedBlock) void k_norm_final(const double* __restrict__ partial, int nPart,
+                                                          int nCell, double* __restrict__ out) {
+  __shared__ double ssq[kRedBlock];
+  __shared__ double smx[kRedBlock];
+  const int t = threadIdx.x;
   double a = 0.0, m = 0.0;
-  for (int i = t; i < (int)gridDim.x; i += kBlock) {
-    a += vp[2 * i];
-    m = fmax(m, vp[2 * i + 1]);
+  for (int i = t; i < nPart; i += kRedBlock) {
+    a += partial[2 * i];
+    m = fmax(m, partial[2 * i + 1]);
   }
   ssq[t] = a;
   smx[t] = m;
   __syncthreads();
-  for (int w = kBlock / 2; w > 0; w >>= 1) {
+  for (int w = kRedBlock / 2; w > 0; w >>= 1) {
     if (t < w) {
       ssq[t] += ssq[t + w];
       smx[t] = fmax(smx[t], smx[t + w]);
@@ -240,7 +239,6 @@ __device__ __forceinline__ void block_norm(double sq, double mx, double* __restr
   if (t == 0) {
     out[0] = sqrt(ssq[0]) / nCell;
     out[1] = smx[0];
-    *counter = 0u;
   }
 }
 
@@ -254,8 +252,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_cells(Consts c, const int* __res
                                                       const double* __restrict__ flux,
                                                       const double* __restrict__ sr, double* __restrict__ r,
                                                       double* __restrict__ r0, double* __restrict__ sigma,
-                                                      double* __restrict__ partial, unsigned* __restrict__ counter,
-                                                      double* __restrict__ normout) {
+                                                      double* __restrict__ partial) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double sq = 0.0, mx = 0.0;
   if (i < c.nCell) {
@@ -284,7 +281,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_cells(Consts c, const int* __res
       *reinterpret_cast<double4*>(r + 4 * (size_t)i) = make_double4(a0, a1, a2, a3);
     }
   }
-  block_norm(sq, mx, partial, counter, c.nCell, normout);
+  block_norm_partial(sq, mx, partial);
 }
 
 // ADI.cpp:40-49 on its own (API path)
@@ -717,8 +714,10 @@ __global__ __launch_bounds__(MAXT) void k_adi_fast(ChainTab ct, const int* __res
   const int i = threadIdx.x;
   const bool active = i < N;
 
+  // the thread's own normalised row stays in registers across the steps
+  T a[16], c[16], f[4];
   if (active) {
-    T a[16], B[16], c[16], f[4];
+    T B[16];
     int cell;
     assemble_row<T>(ct, ff, nProc, pb, nf, N, periodic, i, cfl, sigma, r, a, B, c, f, &cell);
     if (!periodic) {
@@ -740,38 +739,35 @@ __global__ __launch_bounds__(MAXT) void k_adi_fast(ChainTab ct, const int* __res
   int d = 1;
   int n = N;  // cyclic subsystem size
   while (periodic ? (n % 2 == 0 && n >= 4) : (d < N)) {
-    T M[16], na[16], nc[16], nfv[4];
     if (active) {
       const int iL = periodic ? ((i - d) % N + N) % N : i - d;
       const int iR = periodic ? (i + d) % N : i + d;
+      T M[16], na[16], nc[16];
 #pragma unroll
       for (int e = 0; e < 16; e++) {
         M[e] = (e % 5 == 0) ? T(1) : T(0);
         na[e] = T(0);
         nc[e] = T(0);
       }
-#pragma unroll
-      for (int e = 0; e < 4; e++) nfv[e] = lds[(32 + e) * N + i];
       if (periodic || iL >= 0) {
-        T own[16];
-#pragma unroll
-        for (int e = 0; e < 16; e++) own[e] = lds[e * N + i];
-        mm4_acc_lds(M, own, lds, N, 16, iL);   // M -= A^_i C^_L
-        mm4_acc_lds(na, own, lds, N, 0, iL);   // A^new = -A^_i A^_L
-        mv4_acc_lds(nfv, own, lds, N, iL);     // F -= A^_i F^_L
+        mm4_acc_lds(M, a, lds, N, 16, iL);  // M -= A^_i C^_L
+        mm4_acc_lds(na, a, lds, N, 0, iL);  // A^new = -A^_i A^_L
+        mv4_acc_lds(f, a, lds, N, iL);      // F -= A^_i F^_L
       }
       if (periodic || iR < N) {
-        T own[16];
-#pragma unroll
-        for (int e = 0; e < 16; e++) own[e] = lds[(16 + e) * N + i];
-        mm4_acc_lds(M, own, lds, N, 0, iR);    // M -= C^_i A^_R
-        mm4_acc_lds(nc, own, lds, N, 16, iR);  // C^new = -C^_i C^_R
-        mv4_acc_lds(nfv, own, lds, N, iR);     // F -= C^_i F^_R
+        mm4_acc_lds(M, c, lds, N, 0, iR);   // M -= C^_i A^_R
+        mm4_acc_lds(nc, c, lds, N, 16, iR); // C^new = -C^_i C^_R
+        mv4_acc_lds(f, c, lds, N, iR);      // F -= C^_i F^_R
       }
-      normalise_row(M, na, nc, nfv);
+      normalise_row(M, na, nc, f);
+#pragma unroll
+      for (int e = 0; e < 16; e++) {
+        a[e] = na[e];
+        c[e] = nc[e];
+      }
     }
     __syncthreads();
-    if (active) lds_store_row(lds, N, i, na, nc, nfv);
+    if (active) lds_store_row(lds, N, i, a, c, f);
     __syncthreads();
     d *= 2;
     n /= 2;
@@ -779,10 +775,8 @@ __global__ __launch_bounds__(MAXT) void k_adi_fast(ChainTab ct, const int* __res
 
   T x[4] = {T(0), T(0), T(0), T(0)};
   if (FINAL == kFinalOpen) {
-    if (active) {
 #pragma unroll
-      for (int e = 0; e < 4; e++) x[e] = lds[(32 + e) * N + i];
-    }
+    for (int e = 0; e < 4; e++) x[e] = f[e];
   } else if (FINAL == kFinalPair) {
     // rows i and j = i + N/2 couple only to each other:
     // x_i = (I - P Q)^-1 (f_i - P f_j), P = A^_i + C^_i, Q = A^_j + C^_j
@@ -791,13 +785,13 @@ __global__ __launch_bounds__(MAXT) void k_adi_fast(ChainTab ct, const int* __res
       T P[16], Q[16], M[16];
 #pragma unroll
       for (int e = 0; e < 16; e++) {
-        P[e] = lds[e * N + i] + lds[(16 + e) * N + i];
+        P[e] = a[e] + c[e];
         Q[e] = lds[e * N + j] + lds[(16 + e) * N + j];
         M[e] = (e % 5 == 0) ? T(1) : T(0);
       }
       mm4_acc(M, P, Q);
 #pragma unroll
-      for (int e = 0; e < 4; e++) x[e] = lds[(32 + e) * N + i];
+      for (int e = 0; e < 4; e++) x[e] = f[e];
       mv4_acc_lds(x, P, lds, N, j);
       const LU4<T> L = lu4_fast(M);
       lu4_solve_fast(L, x, 1);
@@ -960,8 +954,7 @@ __global__ __launch_bounds__(kBlock) void k_lin_cells(Consts c, const int* __res
                                                       const double* __restrict__ lflux,
                                                       const double* __restrict__ r0, const double* __restrict__ dq,
                                                       const double* __restrict__ sigma, double* __restrict__ r,
-                                                      double* __restrict__ q, double* __restrict__ partial,
-                                                      unsigned* __restrict__ counter, double* __restrict__ normout) {
+                                                      double* __restrict__ q, double* __restrict__ partial) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double sq = 0.0, mx = 0.0;
   if (i < c.nCell) {
@@ -989,7 +982,7 @@ __global__ __launch_bounds__(kBlock) void k_lin_cells(Consts c, const int* __res
       *reinterpret_cast<double4*>(q + 4 * (size_t)i) = qv;
     }
   }
-  block_norm(sq, mx, partial, counter, c.nCell, normout);
+  block_norm_partial(sq, mx, partial);
 }
 
 __global__ void k_update(int n4, double* __restrict__ q, const double* __restrict__ dq) {
@@ -1236,7 +1229,6 @@ struct hamstr_ctx {
   double* hist = nullptr;   // [2*histCap]
   int histCap = 0;
   int* nanflag = nullptr;
-  unsigned* counter = nullptr;  // last-block counter of the norm reductions
   std::vector<int> proc_face;  // host: face id per processed face
   // timing
   bool timing = false;
@@ -1293,16 +1285,21 @@ int launch_lin_faces(hamstr_ctx* x) {
   return HAMSTR_OK;
 }
 
+int launch_norm(hamstr_ctx* x, double* out) {
+  hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(kRedBlock), 0, x->stream, x->partial, x->nPart, x->nCell, out);
+  HIPCHK(hipGetLastError());
+  return HAMSTR_OK;
+}
+
 int launch_rhs_cells(hamstr_ctx* x, bool adi_init, double* normout) {
   if (adi_init)
     hipLaunchKernelGGL((k_rhs_cells<true>), dim3(nblk(x->nCell)), dim3(kBlock), 0, x->stream, x->c, x->cell_off,
-                       x->cell_ent, x->flux, x->sr, x->r, x->r0, x->sigma, x->partial, x->counter, normout);
+                       x->cell_ent, x->flux, x->sr, x->r, x->r0, x->sigma, x->partial);
   else
     hipLaunchKernelGGL((k_rhs_cells<false>), dim3(nblk(x->nCell)), dim3(kBlock), 0, x->stream, x->c,
-                       x->cell_off, x->cell_ent, x->flux, x->sr, x->r, x->r0, x->sigma, x->partial, x->counter,
-                       normout);
+                       x->cell_off, x->cell_ent, x->flux, x->sr, x->r, x->r0, x->sigma, x->partial);
   HIPCHK(hipGetLastError());
-  return HAMSTR_OK;
+  return launch_norm(x, normout);
 }
 
 int launch_adi_phase(hamstr_ctx* x, int ph, int dqmode) {
@@ -1368,12 +1365,12 @@ int launch_lin_cells(hamstr_ctx* x, bool update, double* normout) {
   dim3 g(nblk(x->nCell)), b(kBlock);
   if (update)
     hipLaunchKernelGGL((k_lin_cells<true>), g, b, 0, x->stream, x->c, x->cell_off, x->cell_ent, x->lflux, x->r0,
-                       x->dq, x->sigma, x->r, x->q, x->partial, x->counter, normout);
+                       x->dq, x->sigma, x->r, x->q, x->partial);
   else
     hipLaunchKernelGGL((k_lin_cells<false>), g, b, 0, x->stream, x->c, x->cell_off, x->cell_ent, x->lflux,
-                       x->r0, x->dq, x->sigma, x->r, x->q, x->partial, x->counter, normout);
+                       x->r0, x->dq, x->sigma, x->r, x->q, x->partial);
   HIPCHK(hipGetLastError());
-  return HAMSTR_OK;
+  return launch_norm(x, normout);
 }
 
 // computeLinearRHS; update: also q += dq
@@ -1412,7 +1409,7 @@ void free_ctx(hamstr_ctx* x) {
   void* ptrs[] = {x->q, x->r, x->r0, x->dq, x->sigma, x->flux, x->sr, x->lflux, x->ff, x->pslot, x->src,
                   x->nvec, x->refmtx, x->pleft, x->pright, x->pface, x->cell_off, x->cell_ent, x->ch_pbase,
                   x->ch_nf, x->ch_ncell, x->ch_periodic, x->phase_chains, x->scratch, x->partial, x->norms,
-                  x->hist, x->nanflag, x->counter};
+                  x->hist, x->nanflag};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto e : x->ev) (void)hipEventDestroy(e);
@@ -1697,7 +1694,6 @@ int hamstr_create(const hamstr_mesh_desc* mesh, const hamstr_params* params, ham
   chk(hipMalloc(&x->partial, sizeof(double) * 2 * x->nPart), "alloc partial");
   chk(hipMalloc(&x->norms, sizeof(double) * 4), "alloc norms");
   chk(hipMalloc(&x->nanflag, sizeof(int)), "alloc nanflag");
-  chk(hipMalloc(&x->counter, sizeof(unsigned)), "alloc counter");
   if (rc) {
     free_ctx(x);
     delete x;
@@ -1715,7 +1711,6 @@ int hamstr_create(const hamstr_mesh_desc* mesh, const hamstr_params* params, ham
     return rc;
   }
   chk(hipMemset(x->nanflag, 0, sizeof(int)), "memset nanflag");
-  chk(hipMemset(x->counter, 0, sizeof(unsigned)), "memset counter");
   chk(hipMemset(x->norms, 0, sizeof(double) * 4), "memset norms");
   chk(hipMemset(x->dq, 0, c4), "memset dq");
   chk(hipMemset(x->r, 0, c4), "memset r");
