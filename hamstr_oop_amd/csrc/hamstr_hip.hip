@@ -1944,6 +1944,15 @@ int hamstr_step(hamstr_ctx* x, int nsteps, double tol, int check_every, double* 
   return rc;
 }
 
+int hamstr_last_norms(const hamstr_ctx* x, double* out4) {
+  if (!x || !out4) return fail(HAMSTR_ERR_ARG, "null argument");
+  out4[0] = x->L2rhs;
+  out4[1] = x->Linfrhs;
+  out4[2] = x->L2lin;
+  out4[3] = x->Linflin;
+  return HAMSTR_OK;
+}
+
 int hamstr_sync(hamstr_ctx* x) {
   if (!x) return fail(HAMSTR_ERR_ARG, "null context");
   HIPCHK(hipStreamSynchronize(x->stream));

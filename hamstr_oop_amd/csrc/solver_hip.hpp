@@ -1,0 +1,287 @@
+// solver_hip.hpp -- C++ host surface of the MI355X solver, shaped like the
+// reference's MESHBLOCK / SOLNBLOCK / SOLVER classes (meshBlock.h:15-73,
+// solnBlock.h:13-91, solver.h:14-55).  Every SOLVER method forwards to the C
+// ABI of include/hamstr_hip.h; device memory lives in the hamstr_ctx.
+//
+// Error behaviour follows the reference: its methods are void and a fatal
+// condition prints a message and exits (interfaceFlux.cpp:117-124,
+// io.cpp:21,140,...).  Here a non-zero status from the library does the same
+// through hamstr::fatal(), which a caller may replace.
+#pragma once
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <time.h>
+#include <string>
+#include <vector>
+
+#include "../../include/hamstr_hip.h"
+
+namespace hamstr {
+
+inline void (*fatal)(const char* what, int code) = [](const char* what, int code) {
+  std::printf("\n#hamstr:\t%s failed (status %d): %s\n", what, code, hamstr_last_error());
+  std::exit(1);
+};
+
+inline void check(int rc, const char* what) {
+  if (rc != HAMSTR_OK) fatal(what, rc);
+}
+
+// MESHBLOCK after readMeshData + preprocess (2D)
+struct MeshBlock {
+  int nNode = 0, nCell = 0, nFace = 0, nColor = 0, nChain = 0, nChainFace = 0, nMaxChain = 0;
+  int sol_visc = 0;
+  std::vector<double> x;             // [2*nNode]
+  std::vector<int> conn;             // [4*nCell]
+  std::vector<int> chainsPerColor;   // [nColor]
+  std::vector<int> faceStartPerChain;  // [nChain+1]
+  std::vector<int> chainConn;        // [nChainFace]
+  std::vector<int> faces;            // [6*nFace] n1 n2 cL eL cR eR
+  std::vector<double> normVec;       // [2*nFace]
+  std::vector<double> refMtx;        // [4*nFace]
+
+  // MESHBLOCK::readMeshData (io.cpp:58-372, 2D): ./input/domainNNN/*.dat
+  // relative to the working directory.  Returns false (after printing) when a
+  // file is missing, as the reference stops there.
+  bool readMeshData(int myid, const std::string& root = ".") {
+    char dom[64];
+    std::snprintf(dom, sizeof(dom), "%s/input/domain%03d/", root.c_str(), myid);
+    const std::string d(dom);
+    auto open = [&](const char* name) -> FILE* {
+      FILE* fp = std::fopen((d + name).c_str(), "r");
+      if (!fp) std::printf("\n File missing - %s%s. Stopping\n", d.c_str(), name);
+      return fp;
+    };
+    FILE* fp;
+    if (!(fp = open("coord.dat"))) return false;
+    if (std::fscanf(fp, "%d", &nNode) != 1) return false;
+    x.assign(2 * (size_t)nNode, 0.0);
+    for (int i = 0; i < nNode; i++) {
+      double t;
+      if (std::fscanf(fp, "%lf %lf %lf\n", &x[2 * i], &x[2 * i + 1], &t) != 3) return false;
+    }
+    std::fclose(fp);
+    if (!(fp = open("conn.dat"))) return false;
+    if (std::fscanf(fp, "%d", &nCell) != 1) return false;
+    conn.assign(4 * (size_t)nCell, 0);
+    for (int i = 0; i < nCell; i++)
+      if (std::fscanf(fp, "%d %d %d %d\n", &conn[4 * i], &conn[4 * i + 1], &conn[4 * i + 2], &conn[4 * i + 3]) != 4)
+        return false;
+    std::fclose(fp);
+    if (!(fp = open("ncolors.dat"))) return false;
+    if (std::fscanf(fp, "%d", &nColor) != 1) return false;
+    chainsPerColor.assign(nColor, 0);
+    for (int i = 0; i < nColor; i++)
+      if (std::fscanf(fp, "%d\n", &chainsPerColor[i]) != 1) return false;
+    std::fclose(fp);
+    if (!(fp = open("iqloops.dat"))) return false;
+    int n = 0;
+    if (std::fscanf(fp, "%d\n", &n) != 1) return false;
+    // consecutive duplicate offsets are dropped (io.cpp:229-245)
+    faceStartPerChain.clear();
+    nMaxChain = 0;
+    for (int i = 0; i < n; i++) {
+      int v;
+      if (std::fscanf(fp, "%d\n", &v) != 1) return false;
+      if (!faceStartPerChain.empty() && faceStartPerChain.back() == v) continue;
+      if (!faceStartPerChain.empty() && v - faceStartPerChain.back() > nMaxChain)
+        nMaxChain = v - faceStartPerChain.back();
+      faceStartPerChain.push_back(v);
+    }
+    nChain = (int)faceStartPerChain.size() - 1;
+    std::fclose(fp);
+    if (!(fp = open("qloops.dat"))) return false;
+    if (std::fscanf(fp, "%d", &nChainFace) != 1) return false;
+    chainConn.assign(nChainFace, 0);
+    for (int i = 0; i < nChainFace; i++)
+      if (std::fscanf(fp, "%d", &chainConn[i]) != 1) return false;
+    std::fclose(fp);
+    if (!(fp = open("qedges.dat"))) return false;
+    if (std::fscanf(fp, "%d", &nFace) != 1) return false;
+    faces.assign(6 * (size_t)nFace, 0);
+    for (int i = 0; i < nFace; i++) {
+      int* f = &faces[6 * i];
+      // file order n1 n2 c1 c2 e1 e2 (io.cpp:349-355)
+      if (std::fscanf(fp, "%d %d %d %d %d %d\n", &f[0], &f[1], &f[2], &f[4], &f[3], &f[5]) != 6) return false;
+      if (f[2] == -1) {  // interior cell on the left (io.cpp:360-365)
+        std::swap(f[0], f[1]);
+        std::swap(f[2], f[4]);
+        std::swap(f[3], f[5]);
+      }
+    }
+    std::fclose(fp);
+    return true;
+  }
+
+  // MESHBLOCK::setViscosity + preprocess (preprocess.cpp:56-193)
+  void setViscosity(int visc) { sol_visc = visc; }
+  void preprocess() {
+    normVec.assign(2 * (size_t)nFace, 0.0);
+    refMtx.assign(4 * (size_t)nFace, 0.0);
+    check(hamstr_preprocess(nNode, x.data(), nFace, faces.data(), normVec.data(), refMtx.data(), sol_visc),
+          "MESHBLOCK::preprocess");
+  }
+};
+
+// SOLNBLOCK inputs (solnBlock.h:17-31) from input.hamstr (io.cpp:13-51)
+struct SolnBlock {
+  int order = 1, timeacc = 0, nsteps = 0, nwrite = 0, msweep = 1, visc = 0, test = 0, outform = 0;
+  double Mach = 0, alpha = 0, beta = 0, rey = 0, CFL = 1, dt = 0;
+  char timeInteg[20] = "", scheme[20] = "";
+  double L2Norm = 0, LInfNorm = 0;  // as printed: the linear norm after ADI
+
+  bool readSolutionInputs(const std::string& path = "input.hamstr") {
+    FILE* fp = std::fopen(path.c_str(), "r");
+    if (!fp) {
+      std::printf("Error: Input file 'input.hamstr' missing.\n");
+      return false;
+    }
+    int ok = 0;
+    ok += std::fscanf(fp, "Mach=%lf\n", &Mach);
+    ok += std::fscanf(fp, "alpha=%lf\n", &alpha);
+    ok += std::fscanf(fp, "beta=%lf\n", &beta);
+    ok += std::fscanf(fp, "rey=%lf\n", &rey);
+    ok += std::fscanf(fp, "scheme=%19s\n", scheme);
+    ok += std::fscanf(fp, "time integration=%19s\n", timeInteg);
+    ok += std::fscanf(fp, "order=%d\n", &order);
+    ok += std::fscanf(fp, "timeacc=%d\n", &timeacc);
+    ok += std::fscanf(fp, "nsteps=%d\n", &nsteps);
+    ok += std::fscanf(fp, "nwrite=%d\n", &nwrite);
+    ok += std::fscanf(fp, "dt=%lf\n", &dt);
+    ok += std::fscanf(fp, "CFL=%lf\n", &CFL);
+    ok += std::fscanf(fp, "msweep=%d\n", &msweep);
+    ok += std::fscanf(fp, "visc=%d\n", &visc);
+    ok += std::fscanf(fp, "testcase=%d\n", &test);
+    ok += std::fscanf(fp, "output=%d\n", &outform);
+    std::fclose(fp);
+    if (test != 0) {
+      std::printf("\nERROR: Code not implemented for all test cases\n");
+      return false;
+    }
+    return ok == 16;
+  }
+};
+
+// SOLVER (solver.h:14-55): the ADI path on the device.
+class SOLVER {
+ public:
+  SOLVER(MeshBlock* mb, SolnBlock* sb, int lhs = HAMSTR_LHS_FP64, int adi_mode = HAMSTR_ADI_FAST)
+      : mb_(mb), sb_(sb), lhs_(lhs), adi_(adi_mode) {}
+  ~SOLVER() {
+    if (ctx_) hamstr_destroy(ctx_);
+  }
+  SOLVER(const SOLVER&) = delete;
+  SOLVER& operator=(const SOLVER&) = delete;
+
+  // SOLVER::initFlow (initFlow.cpp:12-93); creates the device context on first use
+  void initFlow() {
+    if (!ctx_) {
+      if (std::strcmp(sb_->scheme, "ADI") != 0) {
+        std::printf(" HAMSTR: scheme '%s' is not on the device path (ADI only).\n", sb_->scheme);
+        std::exit(1);
+      }
+      hamstr_mesh_desc m{mb_->nCell, mb_->nFace, mb_->nChain, mb_->faces.data(), mb_->faceStartPerChain.data(),
+                         mb_->chainConn.data(), mb_->normVec.data(), mb_->refMtx.data()};
+      hamstr_params p{sb_->Mach, sb_->alpha, sb_->CFL, sb_->order, sb_->msweep, lhs_, adi_};
+      check(hamstr_create(&m, &p, &ctx_), "SOLVER::initFlow");
+    } else {
+      check(hamstr_init_flow(ctx_), "SOLVER::initFlow");
+    }
+  }
+
+  // SOLVER::computeRHS (computeRHS.cpp:17-427)
+  void computeRHS(const char* option) {
+    double l2 = 0, li = 0;
+    check(hamstr_compute_rhs(ctx_, std::strcmp(option, "implicit") == 0, &l2, &li), "SOLVER::computeRHS");
+    sb_->L2Norm = l2;
+    sb_->LInfNorm = li;
+  }
+
+  // SOLVER::ADI (ADI.cpp:15-266)
+  void ADI() {
+    double l2 = 0, li = 0;
+    check(hamstr_adi(ctx_, &l2, &li), "SOLVER::ADI");
+    sb_->L2Norm = l2;
+    sb_->LInfNorm = li;
+  }
+
+  // SOLVER::computeLinearRHS (computeLinearRHS.cpp:13-371)
+  void computeLinearRHS() {
+    double l2 = 0, li = 0;
+    check(hamstr_compute_linear_rhs(ctx_, &l2, &li), "SOLVER::computeLinearRHS");
+    sb_->L2Norm = l2;
+    sb_->LInfNorm = li;
+  }
+
+  // SOLVER::stepSolution, ADI branch (march.cpp:112-117), fused on the device
+  void stepSolution() {
+    double l2r = 0, l2l = 0;
+    int done = 0;
+    check(hamstr_step(ctx_, 1, 0.0, 1, &l2r, &l2l, &done), "SOLVER::stepSolution");
+    lastRHS_ = l2r;
+    sb_->L2Norm = l2l;
+  }
+
+  // SOLVER::marchInTime (march.cpp:15-59): same table; the L_2 column is the
+  // linear residual of the last sweep, as in the reference.  Wall-clock time
+  // of the device steps replaces clock()'s CPU time.
+  void marchInTime() {
+    double t = 0;
+    for (int n = 0; n < sb_->nsteps; n++) {
+      timespec a{}, b{};
+      clock_gettime(CLOCK_MONOTONIC, &a);
+      stepSolution();
+      clock_gettime(CLOCK_MONOTONIC, &b);
+      t += (b.tv_sec - a.tv_sec) + 1e-9 * (b.tv_nsec - a.tv_nsec);
+      double nrm[4];
+      check(hamstr_last_norms(ctx_, nrm), "SOLVER::marchInTime");
+      sb_->L2Norm = nrm[2];
+      sb_->LInfNorm = nrm[3];
+      if (n == 0) {
+        std::printf("\n --------------------------------------------------------------\n");
+        std::printf(" |   Iter  | L_2 (density) | L_inf (density) | CPU Time (sec) |\n");
+        std::printf(" --------------------------------------------------------------\n");
+      }
+      std::printf(" |  %*d  |  %e |   %e  |  %e  |\n", 5, n, sb_->L2Norm, sb_->LInfNorm, t);
+      if (n == sb_->nsteps - 1) {
+        std::printf(" --------------------------------------------------------------\n");
+        std::printf(" |   Iter  | L_2 (density) | L_inf (density) | CPU Time (sec) |\n");
+        std::printf(" --------------------------------------------------------------\n");
+      }
+    }
+  }
+
+  // SOLVER::fluxRoe2D (interfaceFlux.cpp:24-169), one face
+  void fluxRoe2D(double* n, double* ql, double* qr, double* flux, double* specRadius, double gamma) {
+    check(hamstr_flux_roe2d(1, n, ql, qr, gamma, flux, specRadius), "SOLVER::fluxRoe2D");
+  }
+
+  // SOLVER::jacobianRoe2D (jacobianRoe.cpp:9-70), one face
+  void jacobianRoe2D(double* n, double* ql, double* qr, double** lmat, double** rmat, double gamma) {
+    double l[16], r[16];
+    check(hamstr_jacobian_roe2d(1, n, ql, qr, gamma, lhs_, l, r), "SOLVER::jacobianRoe2D");
+    for (int i = 0; i < 4; i++)
+      for (int j = 0; j < 4; j++) {
+        lmat[i][j] = l[4 * i + j];
+        rmat[i][j] = r[4 * i + j];
+      }
+  }
+
+  // field access (the reference's sb->q, r, r0, dq, sigma, ff)
+  void get(int field, double* host) { check(hamstr_get_field(ctx_, field, host), "SOLVER::get"); }
+  void set(int field, const double* host) { check(hamstr_set_field(ctx_, field, host), "SOLVER::set"); }
+  double nonlinearL2() const { return lastRHS_; }
+  hamstr_ctx* context() { return ctx_; }
+
+ private:
+  MeshBlock* mb_;
+  SolnBlock* sb_;
+  int lhs_, adi_;
+  hamstr_ctx* ctx_ = nullptr;
+  double lastRHS_ = 0;
+};
+
+}  // namespace hamstr

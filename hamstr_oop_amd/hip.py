@@ -26,7 +26,7 @@ FIELDS = {"q": 0, "r": 1, "r0": 2, "dq": 3, "sigma": 4, "ff": 5}
 EXPORTS = [
     "hamstr_preprocess", "hamstr_create", "hamstr_destroy", "hamstr_init_flow", "hamstr_freestream",
     "hamstr_set_field", "hamstr_get_field", "hamstr_compute_rhs", "hamstr_adi",
-    "hamstr_compute_linear_rhs", "hamstr_step", "hamstr_sync", "hamstr_set_timing",
+    "hamstr_compute_linear_rhs", "hamstr_step", "hamstr_last_norms", "hamstr_sync", "hamstr_set_timing",
     "hamstr_last_timings", "hamstr_last_timing_names", "hamstr_flux_roe2d", "hamstr_jacobian_roe2d", "hamstr_muscl", "hamstr_weno",
     "hamstr_muscl_deriv", "hamstr_weno_deriv", "hamstr_block_tridag4", "hamstr_block_tridag_periodic4",
     "hamstr_last_error",
@@ -182,6 +182,11 @@ class Solver:
 
     def marchInTime(self, nsteps: int):
         return self.step(nsteps)
+
+    def last_norms(self) -> dict:
+        out = np.zeros(4)
+        _chk(lib().hamstr_last_norms(self.h, _d(out)))
+        return dict(zip(("l2_rhs", "linf_rhs", "l2_lin", "linf_lin"), out))
 
     def sync(self):
         _chk(lib().hamstr_sync(self.h))

@@ -123,6 +123,11 @@ int hamstr_compute_linear_rhs(hamstr_ctx *ctx, double *L2lin, double *Linflin);
 int hamstr_step(hamstr_ctx *ctx, int nsteps, double tol, int check_every,
                 double *l2rhs, double *l2lin, int *done);
 
+/* Norms of the last completed step or call: out4 = (L2, Linf) of the
+ * nonlinear residual (computeRHS) and (L2, Linf) of the linear residual
+ * (computeLinearRHS). */
+int hamstr_last_norms(const hamstr_ctx *ctx, double *out4);
+
 /* Blocks until the context's stream is idle. */
 int hamstr_sync(hamstr_ctx *ctx);
 

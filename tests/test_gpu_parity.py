@@ -6,6 +6,8 @@ residual norms, whose sums run in a different order on the GPU: those are
 compared to 1e-12 relative.  ``np.array_equal`` treats +0 and -0 as equal,
 the only difference the device's dropped exact-zero terms can make.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -244,3 +246,24 @@ def test_fast_adi_trajectory(hip):
     scale = np.abs(q0).max(axis=0)
     assert (np.abs(q1 - q0) / scale).max() < 1e-10
     np.testing.assert_allclose(res[hip.ADI_FAST][1], res[hip.ADI_EXACT][1], rtol=1e-8)
+
+
+def test_cpp_driver_matches_oracle(hip, oracle, tmp_path):
+    """The C++ drop-in driver (reference file formats, SOLVER surface) in the
+    EXACT mode: final state bit-identical to the oracle after 8 steps, and the
+    printed L_2 column equal to the reference's printed (linear) norm."""
+    import subprocess
+    from hamstr_oop_amd import meshgen
+    mesh = meshgen.naca_ogrid(32, 12)
+    meshgen.write_reference_case(mesh, str(tmp_path), 0.8, 1.25, 3, 8, 10.0)
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "hamstr_oop_amd", "hamstr_mi355x")
+    out = subprocess.run([exe, "--exact", "--dump", "q.bin"], cwd=str(tmp_path), capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    rows = [ln for ln in out.stdout.splitlines() if ln.strip().startswith("|") and "Iter" not in ln]
+    l2_printed = np.array([float(ln.split("|")[2]) for ln in rows])
+    q = np.fromfile(str(tmp_path / "q.bin"), dtype=np.float64)
+    o = oracle.case(mesh, 0.8, 1.25, 3, 10.0)
+    _, _, l2lin, _ = o.step(8)
+    _eq(q, o.get("q"), "driver q after 8 steps")
+    assert np.array_equal(l2_printed, np.array([float(f"{v:e}") for v in l2lin]))
