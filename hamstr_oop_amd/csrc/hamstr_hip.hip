@@ -59,15 +59,15 @@ inline int nblk(long n, int b = kBlock) { return (int)((n + b - 1) / b); }
 // --------------------------------------------------------------------------
 struct Consts {
   double cfl, gamma, qt, eps;
-  double qinf[4];
   int nCell, nProc, order;
 };
 
 struct FaceTab {
   const int* pslot;      // slot of row n (reference index), bit 30: last face of an open chain
-  const int2* src;       // per slot: (cell, wall face) ; cell < 0: freestream ; face < 0: copy
+  const int2* src;       // per slot: (cell, wall face); cell < 0: freestream of case .y; face < 0: copy
   const double* nvec;    // [nProc][2]
   const double* refmtx;  // [nFace][4]
+  const double4* qinf;   // [nCase] freestream state per flow case
 };
 
 struct ChainTab {
@@ -82,8 +82,9 @@ struct ChainTab {
 };
 
 __device__ __forceinline__ double4 load_state(const int2 e, const double* __restrict__ q,
-                                              const double* __restrict__ refmtx, const double4 qinf) {
-  if (e.x < 0) return qinf;
+                                              const double* __restrict__ refmtx,
+                                              const double4* __restrict__ qinf) {
+  if (e.x < 0) return qinf[e.y];
   double4 v = *reinterpret_cast<const double4*>(q + 4 * (size_t)e.x);
   if (e.y >= 0) {
     const double4 R = *reinterpret_cast<const double4*>(refmtx + 4 * (size_t)e.y);
@@ -97,7 +98,15 @@ __device__ __forceinline__ double4 load_state(const int2 e, const double* __rest
 // same rows for dq: the freestream ghost has dq = 0 (computeLinearRHS.cpp:262-270)
 __device__ __forceinline__ double4 load_dstate(const int2 e, const double* __restrict__ dq,
                                                const double* __restrict__ refmtx) {
-  return load_state(e, dq, refmtx, make_double4(0.0, 0.0, 0.0, 0.0));
+  if (e.x < 0) return make_double4(0.0, 0.0, 0.0, 0.0);
+  double4 v = *reinterpret_cast<const double4*>(dq + 4 * (size_t)e.x);
+  if (e.y >= 0) {
+    const double4 R = *reinterpret_cast<const double4*>(refmtx + 4 * (size_t)e.y);
+    const double a = v.y, b = v.z;
+    v.y = (a * R.x + b * R.y);
+    v.z = (a * R.z + b * R.w);
+  }
+  return v;
 }
 
 __device__ __forceinline__ void put4(double* d, const double4 v) {
@@ -161,10 +170,9 @@ __global__ __launch_bounds__(kBlock) void k_rhs_faces(FaceTab ft, Consts c, cons
   const bool flip = (code >> 30) & 1;
   constexpr int lo = (ORDER == 5) ? 0 : (ORDER == 3 ? 1 : 2);
   constexpr int hi = (ORDER == 5) ? 5 : (ORDER == 3 ? 4 : 3);
-  const double4 qinf = make_double4(c.qinf[0], c.qinf[1], c.qinf[2], c.qinf[3]);
   double f[6][4];
 #pragma unroll
-  for (int j = lo; j <= hi; j++) put4(f[j], load_state(ft.src[s - 2 + j], q, ft.refmtx, qinf));
+  for (int j = lo; j <= hi; j++) put4(f[j], load_state(ft.src[s - 2 + j], q, ft.refmtx, ft.qinf));
   double qln[4], qrn1[4];
   reconstruct_face<ORDER>(f, c.eps, c.qt, qln, qrn1);
   double left[4], right[4];
@@ -252,7 +260,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_cells(Consts c, const int* __res
                                                       const double* __restrict__ flux,
                                                       const double* __restrict__ sr, double* __restrict__ r,
                                                       double* __restrict__ r0, double* __restrict__ sigma,
-                                                      double* __restrict__ partial) {
+                                                      double* __restrict__ partial, double* __restrict__ rrho) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double sq = 0.0, mx = 0.0;
   if (i < c.nCell) {
@@ -271,6 +279,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_cells(Consts c, const int* __res
     }
     sq = a0 * a0;
     mx = fabs(a0);
+    if (rrho) rrho[i] = a0;
     sigma[i] = sg;
     if (ADI_INIT) {
       const double dtfac = c.cfl / sg;
@@ -700,15 +709,13 @@ __device__ __forceinline__ void mv4_acc_lds(T* x, const T* A, const T* lds, int 
 
 enum { kFinalOpen = 0, kFinalPair = 1, kFinalOdd = 2 };
 
-template <typename T, int FINAL, int MAXT>
-__global__ __launch_bounds__(MAXT) void k_adi_fast(ChainTab ct, const int* __restrict__ chains,
-                                                   const T* __restrict__ ff, int nProc, double cfl,
-                                                   const double* __restrict__ sigma, double* __restrict__ r,
-                                                   double* __restrict__ dq, int dqmode) {
+template <typename T, int FINAL>
+__device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const T* __restrict__ ff, int nProc,
+                                               double cfl, const double* __restrict__ sigma,
+                                               double* __restrict__ r, double* __restrict__ dq, int dqmode) {
 #pragma clang fp contract(fast)
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* lds = reinterpret_cast<T*>(smem_raw);
-  const int ch = chains[blockIdx.x];
   const int pb = ct.pbase[ch], nf = ct.nf[ch], N = ct.ncell[ch];
   const bool periodic = FINAL != kFinalOpen;
   const int i = threadIdx.x;
@@ -900,6 +907,20 @@ __global__ __launch_bounds__(MAXT) void k_adi_fast(ChainTab ct, const int* __res
   if (active) store_solution(r, dq, dqmode, ct.pleft[pb + i], x);
 }
 
+// one chain per workgroup pass; the workgroup walks the group's chains with a
+// stride of the grid size (a persistent launch: chains are sorted longest
+// first, so every pass of the grid does similar work)
+template <typename T, int FINAL, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_adi_fast(ChainTab ct, const int* __restrict__ chains, int nch,
+                                                   const T* __restrict__ ff, int nProc, double cfl,
+                                                   const double* __restrict__ sigma, double* __restrict__ r,
+                                                   double* __restrict__ dq, int dqmode) {
+  for (int w = blockIdx.x; w < nch; w += gridDim.x) {
+    adi_fast_chain<T, FINAL>(ct, chains[w], ff, nProc, cfl, sigma, r, dq, dqmode);
+    __syncthreads();
+  }
+}
+
 // --------------------------------------------------------------------------
 // lin_faces: reconstruction derivatives + lmat*dqL + rmat*dqR per face
 // (computeLinearRHS.cpp:52-325)
@@ -915,12 +936,11 @@ __global__ __launch_bounds__(kBlock) void k_lin_faces(FaceTab ft, Consts c, cons
   const bool flip = (code >> 30) & 1;
   constexpr int lo = (ORDER == 5) ? 0 : (ORDER == 3 ? 1 : 2);
   constexpr int hi = (ORDER == 5) ? 5 : (ORDER == 3 ? 4 : 3);
-  const double4 qinf = make_double4(c.qinf[0], c.qinf[1], c.qinf[2], c.qinf[3]);
   double f[6][4], d[6][4];
 #pragma unroll
   for (int j = lo; j <= hi; j++) {
     const int2 e = ft.src[s - 2 + j];
-    if (ORDER != 1) put4(f[j], load_state(e, q, ft.refmtx, qinf));
+    if (ORDER != 1) put4(f[j], load_state(e, q, ft.refmtx, ft.qinf));
     put4(d[j], load_dstate(e, dq, ft.refmtx));
   }
   double dln[4], drn1[4];
@@ -954,7 +974,8 @@ __global__ __launch_bounds__(kBlock) void k_lin_cells(Consts c, const int* __res
                                                       const double* __restrict__ lflux,
                                                       const double* __restrict__ r0, const double* __restrict__ dq,
                                                       const double* __restrict__ sigma, double* __restrict__ r,
-                                                      double* __restrict__ q, double* __restrict__ partial) {
+                                                      double* __restrict__ q, double* __restrict__ partial,
+                                                      double* __restrict__ rrho) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double sq = 0.0, mx = 0.0;
   if (i < c.nCell) {
@@ -976,6 +997,7 @@ __global__ __launch_bounds__(kBlock) void k_lin_cells(Consts c, const int* __res
     *reinterpret_cast<double4*>(r + 4 * (size_t)i) = make_double4(a0, a1, a2, a3);
     sq = a0 * a0;
     mx = fabs(a0);
+    if (rrho) rrho[i] = a0;
     if (UPDATE) {
       double4 qv = *reinterpret_cast<double4*>(q + 4 * (size_t)i);
       qv.x += b.x; qv.y += b.y; qv.z += b.z; qv.w += b.w;
@@ -990,9 +1012,42 @@ __global__ void k_update(int n4, double* __restrict__ q, const double* __restric
   if (i < n4) q[i] += dq[i];
 }
 
-__global__ void k_fill_state(int nCell, double4 v, double* __restrict__ q) {
+// initFlow (initFlow.cpp:152-176): q := freestream of the cell's case
+__global__ void k_init_flow(int nCell, const int* __restrict__ cell_case, const double4* __restrict__ qinf,
+                            double* __restrict__ q) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nCell) *reinterpret_cast<double4*>(q + 4 * (size_t)i) = v;
+  if (i < nCell) *reinterpret_cast<double4*>(q + 4 * (size_t)i) = qinf[cell_case[i]];
+}
+
+// per-case density-residual norms, one block per case, cells in order:
+// out[4*k + slot] = L2, out[4*k + slot + 1] = Linf
+__global__ __launch_bounds__(kRedBlock) void k_case_norms(const double* __restrict__ rrho,
+                                                          const int* __restrict__ case_off, int slot,
+                                                          double* __restrict__ out) {
+  __shared__ double ssq[kRedBlock];
+  __shared__ double smx[kRedBlock];
+  const int k = blockIdx.x, t = threadIdx.x;
+  const int c0 = case_off[k], c1 = case_off[k + 1];
+  double a = 0.0, m = 0.0;
+  for (int i = c0 + t; i < c1; i += kRedBlock) {
+    const double v = rrho[i];
+    a += v * v;
+    m = fmax(m, fabs(v));
+  }
+  ssq[t] = a;
+  smx[t] = m;
+  __syncthreads();
+  for (int w = kRedBlock / 2; w > 0; w >>= 1) {
+    if (t < w) {
+      ssq[t] += ssq[t + w];
+      smx[t] = fmax(smx[t], smx[t + w]);
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    out[4 * k + slot] = sqrt(ssq[0]) / (c1 - c0);
+    out[4 * k + slot + 1] = smx[0];
+  }
 }
 
 // --------------------------------------------------------------------------
@@ -1198,13 +1253,24 @@ struct DevBuf {
 // ==========================================================================
 // context
 // ==========================================================================
+struct Group {
+  int phase = 0;       // ADI phase
+  int fast = 0;        // LDS-resident FAST kernel (else the exact per-chain kernel)
+  int final = 0;       // FAST: kFinalOpen / kFinalPair / kFinalOdd
+  int off = 0, n = 0;  // slice of group_chains
+  int maxN = 0;        // rows of the longest chain
+  int grid = 0;        // workgroups of the launch (autotuned for FAST)
+  int block = 64;      // threads per workgroup
+};
+
 struct hamstr_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   hamstr_params prm{};
-  int nCell = 0, nFace = 0, nChain = 0, nProc = 0, nSlot = 0, nGhost = 2;
+  int nCell = 0, nFace = 0, nChain = 0, nProc = 0, nSlot = 0, nGhost = 2, nCase = 1;
   Consts c{};
   double L2rhs = 0, Linfrhs = 0, L2lin = 0, Linflin = 0;
+  std::vector<double> case_norm;  // [nCase][4] of the last read-back step
   // device arrays
   double *q = nullptr, *r = nullptr, *r0 = nullptr, *dq = nullptr, *sigma = nullptr;
   double *flux = nullptr, *sr = nullptr, *lflux = nullptr;
@@ -1212,21 +1278,25 @@ struct hamstr_ctx {
   int* pslot = nullptr;
   int2* src = nullptr;
   double *nvec = nullptr, *refmtx = nullptr;
+  double4* qinf = nullptr;
+  std::vector<double> qinf_h;  // [nCase][4]
+  int* cell_case = nullptr;
+  double* rrho = nullptr;  // raw density residual per cell (batches: per-case norms)
+  int* case_off = nullptr;
+  std::vector<int> case_off_h;
   int *pleft = nullptr, *pright = nullptr, *pface = nullptr;
   int *cell_off = nullptr, *cell_ent = nullptr;
   int *ch_pbase = nullptr, *ch_nf = nullptr, *ch_ncell = nullptr, *ch_periodic = nullptr;
-  int* phase_chains = nullptr;
-  std::vector<int> phase_off;       // host: offsets into phase_chains
-  std::vector<int> phase_maxrows;
-  std::vector<int> phase_fast;   // phase can use the LDS-resident FAST kernel
-  std::vector<int> phase_final;  // its final solve: open, two-row pairs, odd cyclic
+  int* group_chains = nullptr;
+  std::vector<Group> groups;   // in phase order
+  int nPhase = 0;
   bool last_phase_all_cells = false;
   void* scratch = nullptr;
   size_t scratch_stride = 0;
   double* partial = nullptr;
   int nPart = 0;
-  double* norms = nullptr;  // [4]: L2rhs, Linfrhs, L2lin, Linflin
-  double* hist = nullptr;   // [2*histCap]
+  double* norms = nullptr;  // [nCase][4]: L2rhs, Linfrhs, L2lin, Linflin
+  double* hist = nullptr;   // [histCap][nCase][4]
   int histCap = 0;
   int* nanflag = nullptr;
   std::vector<int> proc_face;  // host: face id per processed face
@@ -1236,7 +1306,7 @@ struct hamstr_ctx {
   std::vector<double> last_ms;
   std::vector<std::string> stage_names;
 
-  FaceTab ftab() const { return FaceTab{pslot, src, nvec, refmtx}; }
+  FaceTab ftab() const { return FaceTab{pslot, src, nvec, refmtx, qinf}; }
   ChainTab ctab() const {
     return ChainTab{ch_pbase, ch_nf, ch_ncell, ch_periodic, pleft, pright, pface, refmtx};
   }
@@ -1285,56 +1355,56 @@ int launch_lin_faces(hamstr_ctx* x) {
   return HAMSTR_OK;
 }
 
-int launch_norm(hamstr_ctx* x, double* out) {
-  hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(kRedBlock), 0, x->stream, x->partial, x->nPart, x->nCell, out);
+// norms of r into base[4*k + slot], base[4*k + slot + 1] for every case k
+int launch_norm(hamstr_ctx* x, double* base, int slot) {
+  if (x->nCase == 1)
+    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(kRedBlock), 0, x->stream, x->partial, x->nPart, x->nCell,
+                       base + slot);
+  else
+    hipLaunchKernelGGL(k_case_norms, dim3(x->nCase), dim3(kRedBlock), 0, x->stream, x->rrho, x->case_off, slot, base);
   HIPCHK(hipGetLastError());
   return HAMSTR_OK;
 }
 
-int launch_rhs_cells(hamstr_ctx* x, bool adi_init, double* normout) {
+int launch_rhs_cells(hamstr_ctx* x, bool adi_init, double* normbase) {
   if (adi_init)
     hipLaunchKernelGGL((k_rhs_cells<true>), dim3(nblk(x->nCell)), dim3(kBlock), 0, x->stream, x->c, x->cell_off,
-                       x->cell_ent, x->flux, x->sr, x->r, x->r0, x->sigma, x->partial);
+                       x->cell_ent, x->flux, x->sr, x->r, x->r0, x->sigma, x->partial, x->rrho);
   else
     hipLaunchKernelGGL((k_rhs_cells<false>), dim3(nblk(x->nCell)), dim3(kBlock), 0, x->stream, x->c,
-                       x->cell_off, x->cell_ent, x->flux, x->sr, x->r, x->r0, x->sigma, x->partial);
+                       x->cell_off, x->cell_ent, x->flux, x->sr, x->r, x->r0, x->sigma, x->partial, x->rrho);
   HIPCHK(hipGetLastError());
-  return launch_norm(x, normout);
+  return launch_norm(x, normbase, 0);
 }
 
-int launch_adi_phase(hamstr_ctx* x, int ph, int dqmode) {
+int launch_group(hamstr_ctx* x, const Group& g, int grid, int dqmode) {
   const ChainTab ct = x->ctab();
-  const int n = x->phase_off[ph + 1] - x->phase_off[ph];
-  const int* chains = x->phase_chains + x->phase_off[ph];
-  if (x->prm.adi_mode == HAMSTR_ADI_FAST && x->phase_fast[ph]) {
-    const int rows = x->phase_maxrows[ph];
-    const int threads = ((rows + 63) / 64) * 64;
-    const int fin = x->phase_final[ph];
-    const dim3 g(n), b(threads);
-#define FASTL(T, F, M)                                                                                    \
-  hipLaunchKernelGGL((k_adi_fast<T, F, M>), g, b, sizeof(T) * 36 * rows, x->stream, ct, chains, (const T*)x->ff, \
-                     x->nProc, x->c.cfl, x->sigma, x->r, x->dq, dqmode)
+  const int* chains = x->group_chains + g.off;
+  if (g.fast) {
+    const dim3 gr(grid), b(g.block);
+    const size_t lds = (x->lhs32() ? sizeof(float) : sizeof(double)) * 36 * (size_t)g.maxN;
+#define FASTL(T, F, M)                                                                                          \
+  hipLaunchKernelGGL((k_adi_fast<T, F, M>), gr, b, lds, x->stream, ct, chains, g.n, (const T*)x->ff, x->nProc, \
+                     x->c.cfl, x->sigma, x->r, x->dq, dqmode)
     if (x->lhs32()) {
-      if (fin == kFinalOpen) FASTL(float, kFinalOpen, 1024);
-      else if (fin == kFinalPair) FASTL(float, kFinalPair, 1024);
+      if (g.final == kFinalOpen) FASTL(float, kFinalOpen, 1024);
+      else if (g.final == kFinalPair) FASTL(float, kFinalPair, 1024);
       else FASTL(float, kFinalOdd, 1024);
     } else {
-      if (fin == kFinalOpen) FASTL(double, kFinalOpen, 512);
-      else if (fin == kFinalPair) FASTL(double, kFinalPair, 512);
+      if (g.final == kFinalOpen) FASTL(double, kFinalOpen, 512);
+      else if (g.final == kFinalPair) FASTL(double, kFinalPair, 512);
       else FASTL(double, kFinalOdd, 512);
     }
 #undef FASTL
-    HIPCHK(hipGetLastError());
-    return HAMSTR_OK;
-  }
-  if (x->lhs32())
-    hipLaunchKernelGGL((k_adi_exact<float>), dim3(nblk(n, 64)), dim3(64), 0, x->stream, ct, chains, n,
+  } else if (x->lhs32()) {
+    hipLaunchKernelGGL((k_adi_exact<float>), dim3(nblk(g.n, 64)), dim3(64), 0, x->stream, ct, chains, g.n,
                        (const float*)x->ff, x->nProc, x->c.cfl, x->sigma, x->r, (float*)x->scratch,
                        (int)x->scratch_stride, x->dq, dqmode);
-  else
-    hipLaunchKernelGGL((k_adi_exact<double>), dim3(nblk(n, 64)), dim3(64), 0, x->stream, ct, chains, n,
+  } else {
+    hipLaunchKernelGGL((k_adi_exact<double>), dim3(nblk(g.n, 64)), dim3(64), 0, x->stream, ct, chains, g.n,
                        (const double*)x->ff, x->nProc, x->c.cfl, x->sigma, x->r, (double*)x->scratch,
                        (int)x->scratch_stride, x->dq, dqmode);
+  }
   HIPCHK(hipGetLastError());
   return HAMSTR_OK;
 }
@@ -1342,12 +1412,15 @@ int launch_adi_phase(hamstr_ctx* x, int ph, int dqmode) {
 // all phases of one sweep; the last phase also accumulates dq when it covers
 // every cell, otherwise a separate kernel does (ADI.cpp:57-248)
 int launch_sweep_phases(hamstr_ctx* x, int sweep, const std::function<int(const char*)>& mark) {
-  const int nph = (int)x->phase_off.size() - 1;
-  for (int ph = 0; ph < nph; ph++) {
-    const bool fuse = (ph == nph - 1) && x->last_phase_all_cells;
-    const int rc = launch_adi_phase(x, ph, fuse ? (sweep == 0 ? 1 : 2) : 0);
+  for (size_t k = 0; k < x->groups.size(); k++) {
+    const Group& g = x->groups[k];
+    const bool fuse = (g.phase == x->nPhase - 1) && x->last_phase_all_cells;
+    const int rc = launch_group(x, g, g.grid, fuse ? (sweep == 0 ? 1 : 2) : 0);
     if (rc) return rc;
-    if (mark && (ph == 0 ? mark("adi_phase0") : (ph == 1 ? mark("adi_phase1") : mark("adi_phase")))) return HAMSTR_ERR_HIP;
+    const bool phase_end = (k + 1 == x->groups.size()) || x->groups[k + 1].phase != g.phase;
+    if (mark && phase_end &&
+        (g.phase == 0 ? mark("adi_phase0") : (g.phase == 1 ? mark("adi_phase1") : mark("adi_phase"))))
+      return HAMSTR_ERR_HIP;
   }
   if (!x->last_phase_all_cells) {
     const int n4 = 4 * x->nCell;
@@ -1361,42 +1434,42 @@ int launch_sweep_phases(hamstr_ctx* x, int sweep, const std::function<int(const 
   return HAMSTR_OK;
 }
 
-int launch_lin_cells(hamstr_ctx* x, bool update, double* normout) {
+int launch_lin_cells(hamstr_ctx* x, bool update, double* normbase) {
   dim3 g(nblk(x->nCell)), b(kBlock);
   if (update)
     hipLaunchKernelGGL((k_lin_cells<true>), g, b, 0, x->stream, x->c, x->cell_off, x->cell_ent, x->lflux, x->r0,
-                       x->dq, x->sigma, x->r, x->q, x->partial);
+                       x->dq, x->sigma, x->r, x->q, x->partial, x->rrho);
   else
     hipLaunchKernelGGL((k_lin_cells<false>), g, b, 0, x->stream, x->c, x->cell_off, x->cell_ent, x->lflux,
-                       x->r0, x->dq, x->sigma, x->r, x->q, x->partial);
+                       x->r0, x->dq, x->sigma, x->r, x->q, x->partial, x->rrho);
   HIPCHK(hipGetLastError());
-  return launch_norm(x, normout);
+  return launch_norm(x, normbase, 2);
 }
 
-// computeLinearRHS; update: also q += dq
-int launch_linear(hamstr_ctx* x, bool update, double* normout) {
+int launch_linear(hamstr_ctx* x, bool update, double* normbase) {
   int rc = launch_lin_faces(x);
   if (rc) return rc;
-  return launch_lin_cells(x, update, normout);
+  return launch_lin_cells(x, update, normbase);
 }
 
 // the ADI sweeps after r/r0 have been scaled (ADI.cpp:55-264)
-int launch_sweeps(hamstr_ctx* x, double* linnorm) {
+int launch_sweeps(hamstr_ctx* x, double* normbase) {
   for (int s = 0; s < x->prm.msweep; s++) {
     int rc = launch_sweep_phases(x, s, nullptr);
     if (rc) return rc;
-    rc = launch_linear(x, s == x->prm.msweep - 1, linnorm);
+    rc = launch_linear(x, s == x->prm.msweep - 1, normbase);
     if (rc) return rc;
   }
   return HAMSTR_OK;
 }
 
 int read_norms(hamstr_ctx* x) {
-  double h[4];
-  HIPCHK(hipMemcpyAsync(h, x->norms, sizeof(h), hipMemcpyDeviceToHost, x->stream));
+  std::vector<double> h(4 * (size_t)x->nCase);
+  HIPCHK(hipMemcpyAsync(h.data(), x->norms, sizeof(double) * h.size(), hipMemcpyDeviceToHost, x->stream));
   int flag = 0;
   HIPCHK(hipMemcpyAsync(&flag, x->nanflag, sizeof(int), hipMemcpyDeviceToHost, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
+  x->case_norm = h;
   x->L2rhs = h[0];
   x->Linfrhs = h[1];
   x->L2lin = h[2];
@@ -1407,54 +1480,38 @@ int read_norms(hamstr_ctx* x) {
 
 void free_ctx(hamstr_ctx* x) {
   void* ptrs[] = {x->q, x->r, x->r0, x->dq, x->sigma, x->flux, x->sr, x->lflux, x->ff, x->pslot, x->src,
-                  x->nvec, x->refmtx, x->pleft, x->pright, x->pface, x->cell_off, x->cell_ent, x->ch_pbase,
-                  x->ch_nf, x->ch_ncell, x->ch_periodic, x->phase_chains, x->scratch, x->partial, x->norms,
-                  x->hist, x->nanflag};
+                  x->nvec, x->refmtx, x->qinf, x->cell_case, x->case_off, x->pleft, x->pright, x->pface,
+                  x->cell_off, x->cell_ent, x->ch_pbase, x->ch_nf, x->ch_ncell, x->ch_periodic, x->group_chains,
+                  x->scratch, x->partial, x->norms, x->hist, x->nanflag, x->rrho};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto e : x->ev) (void)hipEventDestroy(e);
   if (x->stream) (void)hipStreamDestroy(x->stream);
 }
 
-}  // namespace
+struct CaseInfo {
+  int cellBase, cellCount;
+  int chainBase, chainCount;
+  double mach, alpha;
+};
 
-// ==========================================================================
-// C ABI
-// ==========================================================================
-extern "C" {
-
-const char* hamstr_last_error(void) { return g_err.c_str(); }
-
-int hamstr_preprocess(int nNode, const double* x, int nFace, int* faces, double* normVec, double* refMtx,
-                      int sol_visc) {
-  if (nNode <= 0 || nFace <= 0 || !x || !faces || !normVec || !refMtx)
-    return fail(HAMSTR_ERR_ARG, "hamstr_preprocess: bad arguments");
-  for (int i = 0; i < nFace; i++) {
-    const int i1 = faces[6 * i], i2 = faces[6 * i + 1];
-    if (i1 < 0 || i1 >= nNode || i2 < 0 || i2 >= nNode) return fail(HAMSTR_ERR_ARG, "face node out of range");
-    const double x1 = x[2 * i1], y1 = x[2 * i1 + 1], x2 = x[2 * i2], y2 = x[2 * i2 + 1];
-    const double n0 = (y2 - y1), n1 = -(x2 - x1);
-    normVec[2 * i] = n0;
-    normVec[2 * i + 1] = n1;
-    double dsnorm = n0 * n0 + n1 * n1;
-    dsnorm = 1. / dsnorm;
-    refMtx[4 * i + 0] = 1. - 2. * n0 * n0 * dsnorm;
-    refMtx[4 * i + 1] = -2. * n0 * n1 * dsnorm;
-    refMtx[4 * i + 2] = -2. * n1 * n0 * dsnorm;
-    refMtx[4 * i + 3] = 1. - 2. * n1 * n1 * dsnorm;
-    if (faces[6 * i + 4] == -1) {
-      const double r1 = std::sqrt(x[2 * i1] * x[2 * i1] + x[2 * i1 + 1] * x[2 * i1 + 1]);
-      const double r2 = std::sqrt(x[2 * i2] * x[2 * i2] + x[2 * i2 + 1] * x[2 * i2 + 1]);
-      if (0.5 * (r1 + r2) < 3.0) faces[6 * i + 4] = -(sol_visc + 2);
-    }
-  }
-  return HAMSTR_OK;
+// freestream (initFlow.cpp:107-142)
+void freestream_state(double mach, double alpha, double* q4) {
+  const double gamma = 1.4, gm1 = gamma - 1., rinf = 1.0, pinf = 1.0 / gamma;
+  const double PI = 3.1415926535897932e+00;
+  const double uinf = mach * std::cos(alpha * (PI / 180.0));
+  const double vinf = mach * std::sin(alpha * (PI / 180.0));
+  const double einf = pinf / (gm1) + 0.5 * rinf * (uinf * uinf + vinf * vinf);
+  q4[0] = rinf;
+  q4[1] = rinf * uinf;
+  q4[2] = rinf * vinf;
+  q4[3] = einf;
 }
 
-int hamstr_create(const hamstr_mesh_desc* mesh, const hamstr_params* params, hamstr_ctx** out) {
-  if (!mesh || !params || !out) return fail(HAMSTR_ERR_ARG, "hamstr_create: null argument");
+// Builds a context for the disjoint union of one or more cases' meshes.
+int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const std::vector<CaseInfo>& cases,
+                    hamstr_ctx** out) {
   *out = nullptr;
-  const hamstr_params P = *params;
   if (P.order != 1 && P.order != 3 && P.order != 5) return fail(HAMSTR_ERR_ARG, "order must be 1, 3 or 5");
   if (P.msweep < 1) return fail(HAMSTR_ERR_ARG, "msweep must be >= 1");
   if (P.lhs != HAMSTR_LHS_FP64 && P.lhs != HAMSTR_LHS_FP32) return fail(HAMSTR_ERR_ARG, "bad lhs precision");
@@ -1471,6 +1528,7 @@ int hamstr_create(const hamstr_mesh_desc* mesh, const hamstr_params* params, ham
   const int* cc = mesh->chainConn;
   const int nGhost = (P.order == 5) ? 3 : 2;
   const int nChainFace = fs[nChain];
+  const int nCase = (int)cases.size();
   if (fs[0] != 0 || nChainFace <= 0) return fail(HAMSTR_ERR_MESH, "chainStart must start at 0");
   for (int i = 0; i < nChain; i++)
     if (fs[i + 1] - fs[i] < 2) return fail(HAMSTR_ERR_MESH, "chain with fewer than 2 faces");
@@ -1480,6 +1538,13 @@ int hamstr_create(const hamstr_mesh_desc* mesh, const hamstr_params* params, ham
     const int L = faces[6 * i + 2], R = faces[6 * i + 4];
     if (L < 0 || L >= nCell || R >= nCell) return fail(HAMSTR_ERR_MESH, "face cell out of range");
     if (R == -5) return fail(HAMSTR_ERR_MESH, "inter-domain faces (-5) are not supported");
+  }
+  std::vector<int> chain_case(nChain, 0), cell_case(nCell, 0), case_off(nCase + 1, 0);
+  for (int k = 0; k < nCase; k++) {
+    for (int i = cases[k].chainBase; i < cases[k].chainBase + cases[k].chainCount; i++) chain_case[i] = k;
+    for (int c = cases[k].cellBase; c < cases[k].cellBase + cases[k].cellCount; c++) cell_case[c] = k;
+    case_off[k] = cases[k].cellBase;
+    case_off[k + 1] = cases[k].cellBase + cases[k].cellCount;
   }
 
   // ---- chains, slots and processed faces (computeRHS.cpp:53-298) ----
@@ -1526,6 +1591,8 @@ int hamstr_create(const hamstr_mesh_desc* mesh, const hamstr_params* params, ham
       cindx[0] = -cindx[3];
       cindx[1] = -cindx[2];
     }
+    // the reference's cell-0 tests apply to global cell 0 only; in a batch that
+    // is the first case's cell 0 (meshgen never puts cell 0 where they differ)
     const int slotBase = (int)src.size();
     for (int j = 0; j < chainSize; j++) {
       int icell = cindx[j];
@@ -1539,7 +1606,7 @@ int hamstr_create(const hamstr_mesh_desc* mesh, const hamstr_params* params, ham
         if (rightCell == -2)
           e = make_int2(-icell, iface);
         else
-          e = make_int2(-1, -1);  // freestream
+          e = make_int2(-1, chain_case[i]);  // freestream of this chain's case
       }
       src.push_back(e);
     }
@@ -1590,20 +1657,52 @@ int hamstr_create(const hamstr_mesh_desc* mesh, const hamstr_params* params, ham
     phase[i] = ph;
     nPhase = std::max(nPhase, ph + 1);
   }
-  std::vector<int> phase_chains;
-  std::vector<int> phase_off(1, 0), phase_maxrows;
-  size_t maxPhaseChains = 0, maxRows = 0;
+  // ---- launch groups: per phase; FAST groups by final-solve type, chains
+  // sorted longest first for the persistent launches ----
+  const size_t tl = (P.lhs == HAMSTR_LHS_FP32) ? sizeof(float) : sizeof(double);
+  size_t lds_max = 64 * 1024;
+  {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, 0) == hipSuccess && prop.sharedMemPerBlock > 0) lds_max = prop.sharedMemPerBlock;
+  }
+  const int maxt = (P.lhs == HAMSTR_LHS_FP32) ? 1024 : 512;
+  std::vector<Group> groups;
+  std::vector<int> group_chains;
+  size_t maxGroupChains = 0, maxRows = 0;
   for (int ph = 0; ph < nPhase; ph++) {
-    int mr = 0;
-    for (int i = 0; i < nChain; i++)
-      if (phase[i] == ph) {
-        phase_chains.push_back(i);
-        mr = std::max(mr, ch_ncell[i]);
+    // buckets: 0 exact, 1 fast open, 2 fast pair, 3 fast odd
+    std::vector<int> bucket[4];
+    for (int i = 0; i < nChain; i++) {
+      if (phase[i] != ph) continue;
+      const int N = ch_ncell[i];
+      const bool fast_ok = P.adi_mode == HAMSTR_ADI_FAST && N <= maxt && tl * 36 * (size_t)N <= lds_max;
+      if (!fast_ok) {
+        bucket[0].push_back(i);
+      } else if (!ch_periodic[i]) {
+        bucket[1].push_back(i);
+      } else {
+        int nn = N;
+        while (nn % 2 == 0 && nn >= 4) nn /= 2;
+        bucket[nn == 2 ? 2 : 3].push_back(i);
       }
-    phase_off.push_back((int)phase_chains.size());
-    phase_maxrows.push_back(mr);
-    maxPhaseChains = std::max(maxPhaseChains, (size_t)(phase_off[ph + 1] - phase_off[ph]));
-    maxRows = std::max(maxRows, (size_t)mr);
+    }
+    for (int b = 0; b < 4; b++) {
+      if (bucket[b].empty()) continue;
+      std::stable_sort(bucket[b].begin(), bucket[b].end(), [&](int a, int c) { return ch_ncell[a] > ch_ncell[c]; });
+      Group g;
+      g.phase = ph;
+      g.fast = b > 0;
+      g.final = b == 1 ? kFinalOpen : (b == 2 ? kFinalPair : kFinalOdd);
+      g.off = (int)group_chains.size();
+      g.n = (int)bucket[b].size();
+      g.maxN = ch_ncell[bucket[b][0]];
+      g.block = g.fast ? ((g.maxN + 63) / 64) * 64 : 64;
+      g.grid = g.fast ? g.n : nblk(g.n, 64);
+      for (int i : bucket[b]) group_chains.push_back(i);
+      groups.push_back(g);
+      maxGroupChains = std::max(maxGroupChains, (size_t)g.n);
+      if (!g.fast) maxRows = std::max(maxRows, (size_t)g.maxN);
+    }
   }
 
   hamstr_ctx* x = new hamstr_ctx();
@@ -1614,61 +1713,30 @@ int hamstr_create(const hamstr_mesh_desc* mesh, const hamstr_params* params, ham
   x->nProc = nProc;
   x->nSlot = nSlot;
   x->nGhost = nGhost;
-  x->phase_off = phase_off;
-  x->phase_maxrows = phase_maxrows;
+  x->nCase = nCase;
+  x->nPhase = nPhase;
+  x->groups = groups;
+  x->proc_face = pface;
+  x->case_off_h = case_off;
   {
-    // cells within one phase are disjoint, so covering = counting
     long cover = 0;
-    for (int k = phase_off[nPhase - 1]; k < phase_off[nPhase]; k++) cover += ch_ncell[phase_chains[k]];
+    for (int i = 0; i < nChain; i++)
+      if (phase[i] == nPhase - 1) cover += ch_ncell[i];
     x->last_phase_all_cells = (cover == nCell);
   }
-  {
-    const size_t tl = (P.lhs == HAMSTR_LHS_FP32) ? sizeof(float) : sizeof(double);
-    hipDeviceProp_t prop;
-    size_t lds_max = 160 * 1024;
-    if (hipGetDeviceProperties(&prop, 0) == hipSuccess && prop.sharedMemPerBlock > 0) lds_max = prop.sharedMemPerBlock;
-    for (size_t ph = 0; ph < phase_maxrows.size(); ph++) {
-      const size_t bytes = tl * 36 * (size_t)phase_maxrows[ph];
-      // every chain of the phase must use the same row count for one launch shape
-      bool uniform = true;
-      for (int k = phase_off[ph]; k < phase_off[ph + 1]; k++)
-        if (ch_ncell[phase_chains[k]] != phase_maxrows[ph]) uniform = false;
-      // every chain of the phase must share the periodicity too
-      int per = -1;
-      for (int k = phase_off[ph]; k < phase_off[ph + 1]; k++) {
-        const int pk = ch_periodic[phase_chains[k]];
-        if (per == -1) per = pk;
-        if (pk != per) uniform = false;
-      }
-      const int maxt = (P.lhs == HAMSTR_LHS_FP32) ? 1024 : 512;
-      x->phase_fast.push_back(uniform && phase_maxrows[ph] <= maxt && bytes <= lds_max ? 1 : 0);
-      int fin = kFinalOpen;
-      if (per == 1) {
-        int nn = phase_maxrows[ph];
-        while (nn % 2 == 0 && nn >= 4) nn /= 2;
-        fin = (nn == 2) ? kFinalPair : kFinalOdd;
-      }
-      x->phase_final.push_back(fin);
-    }
-  }
-  x->proc_face = pface;
-  // freestream (initFlow.cpp:107-142)
-  const double gamma = 1.4, gm1 = gamma - 1., rinf = 1.0, pinf = 1.0 / gamma;
-  const double PI = 3.1415926535897932e+00;
-  const double uinf = P.mach * std::cos(P.alpha * (PI / 180.0));
-  const double vinf = P.mach * std::sin(P.alpha * (PI / 180.0));
-  const double einf = pinf / (gm1) + 0.5 * rinf * (uinf * uinf + vinf * vinf);
+  x->qinf_h.assign(4 * (size_t)nCase, 0.0);
+  for (int k = 0; k < nCase; k++) freestream_state(cases[k].mach, cases[k].alpha, &x->qinf_h[4 * k]);
+  std::vector<double4> qinf4(nCase);
+  for (int k = 0; k < nCase; k++)
+    qinf4[k] = make_double4(x->qinf_h[4 * k], x->qinf_h[4 * k + 1], x->qinf_h[4 * k + 2], x->qinf_h[4 * k + 3]);
   x->c.cfl = P.cfl;
-  x->c.gamma = gamma;
+  x->c.gamma = 1.4;
   x->c.qt = (P.order == 1) ? 0.0 : 0.25;
   x->c.eps = 1e-10;
-  x->c.qinf[0] = rinf;
-  x->c.qinf[1] = rinf * uinf;
-  x->c.qinf[2] = rinf * vinf;
-  x->c.qinf[3] = einf;
   x->c.nCell = nCell;
   x->c.nProc = nProc;
   x->c.order = P.order;
+  x->case_norm.assign(4 * (size_t)nCase, 0.0);
 
   int rc = HAMSTR_OK;
   auto chk = [&](hipError_t e, const char* what) {
@@ -1685,14 +1753,13 @@ int hamstr_create(const hamstr_mesh_desc* mesh, const hamstr_params* params, ham
   chk(hipMalloc(&x->flux, sizeof(double) * 4 * nProc), "alloc flux");
   chk(hipMalloc(&x->sr, sizeof(double) * nProc), "alloc sr");
   chk(hipMalloc(&x->lflux, sizeof(double) * 4 * nProc), "alloc lflux");
-  const size_t tl = x->lhs32() ? sizeof(float) : sizeof(double);
   chk(hipMalloc(&x->ff, tl * 32 * nProc), "alloc ff");
-  x->scratch_stride = maxPhaseChains;
-  chk(hipMalloc(&x->scratch, tl * 36 * std::max<size_t>(maxRows, 1) * std::max<size_t>(maxPhaseChains, 1)),
-      "alloc scratch");
+  x->scratch_stride = std::max<size_t>(maxGroupChains, 1);
+  chk(hipMalloc(&x->scratch, tl * 36 * std::max<size_t>(maxRows, 1) * x->scratch_stride), "alloc scratch");
   x->nPart = nblk(nCell);
   chk(hipMalloc(&x->partial, sizeof(double) * 2 * x->nPart), "alloc partial");
-  chk(hipMalloc(&x->norms, sizeof(double) * 4), "alloc norms");
+  chk(hipMalloc(&x->norms, sizeof(double) * 4 * nCase), "alloc norms");
+  if (nCase > 1) chk(hipMalloc(&x->rrho, sizeof(double) * nCell), "alloc rrho");
   chk(hipMalloc(&x->nanflag, sizeof(int)), "alloc nanflag");
   if (rc) {
     free_ctx(x);
@@ -1701,17 +1768,18 @@ int hamstr_create(const hamstr_mesh_desc* mesh, const hamstr_params* params, ham
   }
   std::vector<double> rm(mesh->refMtx, mesh->refMtx + 4 * (size_t)nFace);
   if ((rc = upload(&x->pslot, pslot)) || (rc = upload(&x->src, src)) || (rc = upload(&x->nvec, nvec)) ||
-      (rc = upload(&x->refmtx, rm)) || (rc = upload(&x->pleft, pleft)) || (rc = upload(&x->pright, pright)) ||
-      (rc = upload(&x->pface, pface)) || (rc = upload(&x->cell_off, cnt)) ||
+      (rc = upload(&x->refmtx, rm)) || (rc = upload(&x->qinf, qinf4)) || (rc = upload(&x->cell_case, cell_case)) ||
+      (rc = upload(&x->case_off, case_off)) || (rc = upload(&x->pleft, pleft)) ||
+      (rc = upload(&x->pright, pright)) || (rc = upload(&x->pface, pface)) || (rc = upload(&x->cell_off, cnt)) ||
       (rc = upload(&x->cell_ent, cell_ent)) || (rc = upload(&x->ch_pbase, ch_pbase)) ||
       (rc = upload(&x->ch_nf, ch_nf)) || (rc = upload(&x->ch_ncell, ch_ncell)) ||
-      (rc = upload(&x->ch_periodic, ch_periodic)) || (rc = upload(&x->phase_chains, phase_chains))) {
+      (rc = upload(&x->ch_periodic, ch_periodic)) || (rc = upload(&x->group_chains, group_chains))) {
     free_ctx(x);
     delete x;
     return rc;
   }
   chk(hipMemset(x->nanflag, 0, sizeof(int)), "memset nanflag");
-  chk(hipMemset(x->norms, 0, sizeof(double) * 4), "memset norms");
+  chk(hipMemset(x->norms, 0, sizeof(double) * 4 * nCase), "memset norms");
   chk(hipMemset(x->dq, 0, c4), "memset dq");
   chk(hipMemset(x->r, 0, c4), "memset r");
   chk(hipMemset(x->r0, 0, c4), "memset r0");
@@ -1723,7 +1791,113 @@ int hamstr_create(const hamstr_mesh_desc* mesh, const hamstr_params* params, ham
     return rc;
   }
   *out = x;
-  return hamstr_init_flow(x);
+  return HAMSTR_OK;
+}
+
+}  // namespace
+
+// ==========================================================================
+// C ABI
+// ==========================================================================
+extern "C" {
+
+const char* hamstr_last_error(void) { return g_err.c_str(); }
+
+int hamstr_preprocess(int nNode, const double* x, int nFace, int* faces, double* normVec, double* refMtx,
+                      int sol_visc) {
+  if (nNode <= 0 || nFace <= 0 || !x || !faces || !normVec || !refMtx)
+    return fail(HAMSTR_ERR_ARG, "hamstr_preprocess: bad arguments");
+  for (int i = 0; i < nFace; i++) {
+    const int i1 = faces[6 * i], i2 = faces[6 * i + 1];
+    if (i1 < 0 || i1 >= nNode || i2 < 0 || i2 >= nNode) return fail(HAMSTR_ERR_ARG, "face node out of range");
+    const double x1 = x[2 * i1], y1 = x[2 * i1 + 1], x2 = x[2 * i2], y2 = x[2 * i2 + 1];
+    const double n0 = (y2 - y1), n1 = -(x2 - x1);
+    normVec[2 * i] = n0;
+    normVec[2 * i + 1] = n1;
+    double dsnorm = n0 * n0 + n1 * n1;
+    dsnorm = 1. / dsnorm;
+    refMtx[4 * i + 0] = 1. - 2. * n0 * n0 * dsnorm;
+    refMtx[4 * i + 1] = -2. * n0 * n1 * dsnorm;
+    refMtx[4 * i + 2] = -2. * n1 * n0 * dsnorm;
+    refMtx[4 * i + 3] = 1. - 2. * n1 * n1 * dsnorm;
+    if (faces[6 * i + 4] == -1) {
+      const double r1 = std::sqrt(x[2 * i1] * x[2 * i1] + x[2 * i1 + 1] * x[2 * i1 + 1]);
+      const double r2 = std::sqrt(x[2 * i2] * x[2 * i2] + x[2 * i2 + 1] * x[2 * i2 + 1]);
+      if (0.5 * (r1 + r2) < 3.0) faces[6 * i + 4] = -(sol_visc + 2);
+    }
+  }
+  return HAMSTR_OK;
+}
+
+int hamstr_create(const hamstr_mesh_desc* mesh, const hamstr_params* params, hamstr_ctx** out) {
+  if (!mesh || !params || !out) return fail(HAMSTR_ERR_ARG, "hamstr_create: null argument");
+  *out = nullptr;
+  std::vector<CaseInfo> cases(1, CaseInfo{0, mesh->nCell, 0, mesh->nChain, params->mach, params->alpha});
+  const int rc = create_internal(mesh, *params, cases, out);
+  if (rc) return rc;
+  return hamstr_init_flow(*out);
+}
+
+int hamstr_batch_create(int ncases, const hamstr_mesh_desc* meshes, const hamstr_params* params, hamstr_ctx** out) {
+  if (ncases <= 0 || !meshes || !params || !out) return fail(HAMSTR_ERR_ARG, "hamstr_batch_create: bad arguments");
+  *out = nullptr;
+  const hamstr_params& P0 = params[0];
+  for (int k = 1; k < ncases; k++)
+    if (params[k].cfl != P0.cfl || params[k].order != P0.order || params[k].msweep != P0.msweep ||
+        params[k].lhs != P0.lhs || params[k].adi_mode != P0.adi_mode)
+      return fail(HAMSTR_ERR_ARG, "batched cases must share cfl, order, msweep, lhs and adi_mode");
+  // disjoint union of the meshes
+  long nc = 0, nf = 0, nch = 0, ncf = 0;
+  for (int k = 0; k < ncases; k++) {
+    const hamstr_mesh_desc& m = meshes[k];
+    if (m.nCell <= 0 || m.nFace <= 0 || m.nChain <= 0 || !m.faces || !m.chainStart || !m.chainConn ||
+        !m.normVec || !m.refMtx)
+      return fail(HAMSTR_ERR_ARG, "hamstr_batch_create: bad mesh description");
+    nc += m.nCell;
+    nf += m.nFace;
+    nch += m.nChain;
+    ncf += m.chainStart[m.nChain];
+  }
+  if (nc >= (1L << 30) || nf >= (1L << 29)) return fail(HAMSTR_ERR_ARG, "batch too large");
+  std::vector<int> faces(6 * nf), fs(nch + 1), conn(ncf);
+  std::vector<double> nv(2 * nf), rm(4 * nf);
+  std::vector<CaseInfo> cases(ncases);
+  long c0 = 0, f0 = 0, h0 = 0, cf0 = 0;
+  fs[0] = 0;
+  for (int k = 0; k < ncases; k++) {
+    const hamstr_mesh_desc& m = meshes[k];
+    for (long i = 0; i < m.nFace; i++) {
+      const int* s = m.faces + 6 * i;
+      int* d = faces.data() + 6 * (f0 + i);
+      d[0] = s[0];
+      d[1] = s[1];
+      d[2] = s[2] + (int)c0;
+      d[3] = s[3];
+      d[4] = s[4] >= 0 ? s[4] + (int)c0 : s[4];
+      d[5] = s[5];
+    }
+    std::memcpy(nv.data() + 2 * f0, m.normVec, sizeof(double) * 2 * m.nFace);
+    std::memcpy(rm.data() + 4 * f0, m.refMtx, sizeof(double) * 4 * m.nFace);
+    for (int i = 0; i < m.nChain; i++) fs[h0 + i + 1] = (int)cf0 + m.chainStart[i + 1];
+    for (int f = 0; f < m.chainStart[m.nChain]; f++) conn[cf0 + f] = m.chainConn[f] + (int)f0;
+    cases[k] = CaseInfo{(int)c0, m.nCell, (int)h0, m.nChain, params[k].mach, params[k].alpha};
+    c0 += m.nCell;
+    f0 += m.nFace;
+    h0 += m.nChain;
+    cf0 += m.chainStart[m.nChain];
+  }
+  hamstr_mesh_desc u{(int)nc, (int)nf, (int)nch, faces.data(), fs.data(), conn.data(), nv.data(), rm.data()};
+  const int rc = create_internal(&u, P0, cases, out);
+  if (rc) return rc;
+  return hamstr_init_flow(*out);
+}
+
+int hamstr_batch_info(const hamstr_ctx* x, int* ncases, int* cell_off) {
+  if (!x || !ncases) return fail(HAMSTR_ERR_ARG, "null argument");
+  *ncases = x->nCase;
+  if (cell_off)
+    for (int k = 0; k <= x->nCase; k++) cell_off[k] = x->case_off_h[k];
+  return HAMSTR_OK;
 }
 
 int hamstr_destroy(hamstr_ctx* x) {
@@ -1736,14 +1910,14 @@ int hamstr_destroy(hamstr_ctx* x) {
 
 int hamstr_freestream(const hamstr_ctx* x, double* out4) {
   if (!x || !out4) return fail(HAMSTR_ERR_ARG, "null argument");
-  for (int k = 0; k < 4; k++) out4[k] = x->c.qinf[k];
+  for (int k = 0; k < 4; k++) out4[k] = x->qinf_h[k];
   return HAMSTR_OK;
 }
 
 int hamstr_init_flow(hamstr_ctx* x) {
   if (!x) return fail(HAMSTR_ERR_ARG, "null context");
-  const double4 v = make_double4(x->c.qinf[0], x->c.qinf[1], x->c.qinf[2], x->c.qinf[3]);
-  hipLaunchKernelGGL(k_fill_state, dim3(nblk(x->nCell)), dim3(kBlock), 0, x->stream, x->nCell, v, x->q);
+  hipLaunchKernelGGL(k_init_flow, dim3(nblk(x->nCell)), dim3(kBlock), 0, x->stream, x->nCell, x->cell_case, x->qinf,
+                     x->q);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(x->stream));
   return HAMSTR_OK;
@@ -1828,7 +2002,7 @@ int hamstr_adi(hamstr_ctx* x, double* L2lin, double* Linflin) {
   hipLaunchKernelGGL(k_adi_init, dim3(nblk(x->nCell)), dim3(kBlock), 0, x->stream, x->c, x->r, x->r0, x->dq,
                      x->sigma);
   HIPCHK(hipGetLastError());
-  int rc = launch_sweeps(x, x->norms + 2);
+  int rc = launch_sweeps(x, x->norms);
   if (rc) return rc;
   rc = read_norms(x);
   if (L2lin) *L2lin = x->L2lin;
@@ -1838,7 +2012,7 @@ int hamstr_adi(hamstr_ctx* x, double* L2lin, double* Linflin) {
 
 int hamstr_compute_linear_rhs(hamstr_ctx* x, double* L2lin, double* Linflin) {
   if (!x) return fail(HAMSTR_ERR_ARG, "null context");
-  int rc = launch_linear(x, false, x->norms + 2);
+  int rc = launch_linear(x, false, x->norms);
   if (rc) return rc;
   rc = read_norms(x);
   if (L2lin) *L2lin = x->L2lin;
@@ -1846,85 +2020,95 @@ int hamstr_compute_linear_rhs(hamstr_ctx* x, double* L2lin, double* Linflin) {
   return rc;
 }
 
-int hamstr_step(hamstr_ctx* x, int nsteps, double tol, int check_every, double* l2rhs, double* l2lin, int* done) {
-  if (!x || nsteps < 0) return fail(HAMSTR_ERR_ARG, "bad arguments");
+}  // extern "C"
+
+namespace {
+
+// one fused iteration (march.cpp:112-117); norms to nb[nCase][4]
+int enqueue_iteration(hamstr_ctx* x, double* nb, const std::function<int(const char*)>& mark) {
+  int rc;
+  if ((rc = launch_rhs_faces(x, true))) return rc;
+  if (mark && mark("rhs_faces")) return HAMSTR_ERR_HIP;
+  if ((rc = launch_rhs_cells(x, true, nb))) return rc;
+  if (mark && mark("rhs_cells")) return HAMSTR_ERR_HIP;
+  for (int sw = 0; sw < x->prm.msweep; sw++) {
+    if ((rc = launch_sweep_phases(x, sw, mark))) return rc;
+    if ((rc = launch_lin_faces(x))) return rc;
+    if (mark && mark("lin_faces")) return HAMSTR_ERR_HIP;
+    if ((rc = launch_lin_cells(x, sw == x->prm.msweep - 1, nb))) return rc;
+    if (mark && mark("lin_cells")) return HAMSTR_ERR_HIP;
+  }
+  return HAMSTR_OK;
+}
+
+// Runs up to nsteps iterations, reading the norm history back every
+// check_every steps.  iters[k] gets the first step (1-based) at which case k's
+// nonlinear L2 <= tol (-1 if never); the loop stops at the first read-back
+// after every case has converged (tol > 0).  *done = steps executed.
+int run_steps(hamstr_ctx* x, int nsteps, double tol, int check_every, std::vector<int>& iters,
+              std::vector<double>& series, int* done) {
+  if (nsteps < 0) return fail(HAMSTR_ERR_ARG, "bad arguments");
   if (check_every < 1) check_every = 1;
+  const int nc = x->nCase;
   if (x->histCap < nsteps) {
     if (x->hist) (void)hipFree(x->hist);
     x->hist = nullptr;
-    HIPCHK(hipMalloc(&x->hist, sizeof(double) * 4 * std::max(nsteps, 1)));
+    HIPCHK(hipMalloc(&x->hist, sizeof(double) * 4 * nc * std::max(nsteps, 1)));
     x->histCap = nsteps;
   }
+  iters.assign(nc, -1);
+  series.assign(4 * (size_t)nc * nsteps, 0.0);
   int rc = HAMSTR_OK;
-  // stage timing: events at every stage boundary of every step
   std::vector<hipEvent_t> evs;
   int nstage = 0;
-  if (x->timing) {
-    x->stage_names.clear();
-    const int nb = (int)x->ev.size();
-    (void)nb;
-  }
-  auto mark = [&](const char* name, int n) -> int {
-    if (!x->timing) return HAMSTR_OK;
-    if (n == 0) x->stage_names.push_back(name);
+  if (x->timing) x->stage_names.clear();
+  int cur = 0;
+  auto mark = [&](const char* name) -> int {
+    if (cur == 0) x->stage_names.push_back(name);
     const size_t need = evs.size() + 1;
     if (x->ev.size() < need) {
       hipEvent_t e;
-      HIPCHK(hipEventCreate(&e));
+      if (hipEventCreate(&e) != hipSuccess) return 1;
       x->ev.push_back(e);
     }
     hipEvent_t e = x->ev[need - 1];
     evs.push_back(e);
-    HIPCHK(hipEventRecord(e, x->stream));
-    return HAMSTR_OK;
+    return hipEventRecord(e, x->stream) != hipSuccess;
   };
+  std::function<int(const char*)> mk;
+  if (x->timing) mk = mark;
   int ran = 0, lastRead = 0;
   for (int n = 0; n < nsteps; n++) {
-    double* hn = x->hist + 4 * n;
-    if ((rc = mark("start", n))) return rc;
-    if ((rc = launch_rhs_faces(x, true))) return rc;
-    if ((rc = mark("rhs_faces", n))) return rc;
-    if ((rc = launch_rhs_cells(x, true, hn))) return rc;
-    if ((rc = mark("rhs_cells", n))) return rc;
-    for (int sw = 0; sw < x->prm.msweep; sw++) {
-      const int n_ = n;
-      if ((rc = launch_sweep_phases(x, sw, [&](const char* nm) { return mark(nm, n_); }))) return rc;
-      if ((rc = launch_lin_faces(x))) return rc;
-      if ((rc = mark("lin_faces", n))) return rc;
-      if ((rc = launch_lin_cells(x, sw == x->prm.msweep - 1, hn + 2))) return rc;
-      if ((rc = mark("lin_cells", n))) return rc;
-    }
+    cur = n;
+    if (x->timing && mark("start")) return fail(HAMSTR_ERR_HIP, "hipEventRecord");
+    if ((rc = enqueue_iteration(x, x->hist + 4 * (size_t)nc * n, mk))) return rc;
     if (n == 0) nstage = (int)evs.size() - 1;
     ran = n + 1;
     if (ran % check_every == 0 || ran == nsteps) {
-      std::vector<double> h(4 * (ran - lastRead));
-      HIPCHK(hipMemcpyAsync(h.data(), x->hist + 4 * lastRead, sizeof(double) * h.size(), hipMemcpyDeviceToHost,
-                            x->stream));
+      const size_t cnt = 4 * (size_t)nc * (ran - lastRead);
+      HIPCHK(hipMemcpyAsync(series.data() + 4 * (size_t)nc * lastRead, x->hist + 4 * (size_t)nc * lastRead,
+                            sizeof(double) * cnt, hipMemcpyDeviceToHost, x->stream));
       int flag = 0;
       HIPCHK(hipMemcpyAsync(&flag, x->nanflag, sizeof(int), hipMemcpyDeviceToHost, x->stream));
       HIPCHK(hipStreamSynchronize(x->stream));
-      for (int k = lastRead; k < ran; k++) {
-        if (l2rhs) l2rhs[k] = h[4 * (k - lastRead)];
-        if (l2lin) l2lin[k] = h[4 * (k - lastRead) + 2];
-      }
-      x->L2rhs = h[4 * (ran - 1 - lastRead)];
-      x->Linfrhs = h[4 * (ran - 1 - lastRead) + 1];
-      x->L2lin = h[4 * (ran - 1 - lastRead) + 2];
-      x->Linflin = h[4 * (ran - 1 - lastRead) + 3];
+      const double* last = series.data() + 4 * (size_t)nc * (ran - 1);
+      x->case_norm.assign(last, last + 4 * nc);
+      x->L2rhs = last[0];
+      x->Linfrhs = last[1];
+      x->L2lin = last[2];
+      x->Linflin = last[3];
       if (flag) {
         if (done) *done = ran;
         return fail(HAMSTR_ERR_NAN, "NaN spectral radius in the Roe flux");
       }
-      bool conv = false;
-      if (tol > 0)
-        for (int k = lastRead; k < ran; k++)
-          if (h[4 * (k - lastRead)] <= tol) {
-            conv = true;
-            ran = k + 1;
-            break;
-          }
-      lastRead = n + 1;
-      if (conv) break;
+      bool all = tol > 0;
+      for (int j = 0; j < nc; j++) {
+        for (int k = lastRead; k < ran && iters[j] < 0; k++)
+          if (series[4 * ((size_t)nc * k + j)] <= tol) iters[j] = k + 1;
+        if (iters[j] < 0) all = false;
+      }
+      lastRead = ran;
+      if (all) break;
     }
   }
   if (x->timing && nstage > 0) {
@@ -1944,12 +2128,114 @@ int hamstr_step(hamstr_ctx* x, int nsteps, double tol, int check_every, double* 
   return rc;
 }
 
+}  // namespace
+
+extern "C" {
+
+int hamstr_step(hamstr_ctx* x, int nsteps, double tol, int check_every, double* l2rhs, double* l2lin, int* done) {
+  if (!x) return fail(HAMSTR_ERR_ARG, "null context");
+  std::vector<int> iters;
+  std::vector<double> series;
+  int ran = 0;
+  const int rc = run_steps(x, nsteps, tol, check_every, iters, series, &ran);
+  for (int k = 0; k < ran; k++) {
+    if (l2rhs) l2rhs[k] = series[4 * (size_t)x->nCase * k];
+    if (l2lin) l2lin[k] = series[4 * (size_t)x->nCase * k + 2];
+  }
+  if (done) *done = ran;
+  return rc;
+}
+
+int hamstr_batch_step(hamstr_ctx* x, int nsteps, double tol, int check_every, int* iters, double* norms,
+                      int* done) {
+  if (!x) return fail(HAMSTR_ERR_ARG, "null context");
+  std::vector<int> it;
+  std::vector<double> series;
+  int ran = 0;
+  const int rc = run_steps(x, nsteps, tol, check_every, it, series, &ran);
+  if (iters)
+    for (int k = 0; k < x->nCase; k++) iters[k] = it[k];
+  if (norms && ran > 0) std::memcpy(norms, x->case_norm.data(), sizeof(double) * 4 * x->nCase);
+  if (done) *done = ran;
+  return rc;
+}
+
 int hamstr_last_norms(const hamstr_ctx* x, double* out4) {
   if (!x || !out4) return fail(HAMSTR_ERR_ARG, "null argument");
   out4[0] = x->L2rhs;
   out4[1] = x->Linfrhs;
   out4[2] = x->L2lin;
   out4[3] = x->Linflin;
+  return HAMSTR_OK;
+}
+
+// Times each FAST launch group at several persistent grid sizes (all chains,
+// 8, 4, 2, 1 workgroups per CU) and keeps the fastest.  r and dq are restored.
+int hamstr_autotune(hamstr_ctx* x, int reps, double* best_ms, int cap, int* ngroups) {
+  if (!x) return fail(HAMSTR_ERR_ARG, "null context");
+  if (reps < 1) reps = 3;
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, x->device));
+  const int cu = std::max(prop.multiProcessorCount, 1);
+  const size_t c4 = sizeof(double) * 4 * x->nCell;
+  double *r_save = nullptr, *dq_save = nullptr;
+  HIPCHK(hipMalloc(&r_save, c4));
+  HIPCHK(hipMalloc(&dq_save, c4));
+  HIPCHK(hipMemcpyAsync(r_save, x->r, c4, hipMemcpyDeviceToDevice, x->stream));
+  HIPCHK(hipMemcpyAsync(dq_save, x->dq, c4, hipMemcpyDeviceToDevice, x->stream));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  int k = 0;
+  for (Group& g : x->groups) {
+    double best = -1;
+    if (g.fast) {
+      std::vector<int> cand = {g.n};
+      for (int w : {8, 4, 2, 1})
+        if (w * cu < g.n) cand.push_back(w * cu);
+      for (int grid : cand) {
+        int rc = launch_group(x, g, grid, 0);  // warm-up
+        if (rc) return rc;
+        HIPCHK(hipEventRecord(e0, x->stream));
+        for (int i = 0; i < reps; i++)
+          if ((rc = launch_group(x, g, grid, 0))) return rc;
+        HIPCHK(hipEventRecord(e1, x->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        const double t = ms / reps;
+        if (best < 0 || t < best) {
+          best = t;
+          g.grid = grid;
+        }
+      }
+    }
+    if (best_ms && k < cap) best_ms[k] = best;
+    k++;
+  }
+  HIPCHK(hipMemcpyAsync(x->r, r_save, c4, hipMemcpyDeviceToDevice, x->stream));
+  HIPCHK(hipMemcpyAsync(x->dq, dq_save, c4, hipMemcpyDeviceToDevice, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  (void)hipFree(r_save);
+  (void)hipFree(dq_save);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (ngroups) *ngroups = (int)x->groups.size();
+  return HAMSTR_OK;
+}
+
+// launch groups: per group (phase, fast, final, chains, maxN, grid, block)
+int hamstr_launch_groups(const hamstr_ctx* x, int* out7, int cap, int* n) {
+  if (!x || !n) return fail(HAMSTR_ERR_ARG, "null argument");
+  int k = 0;
+  for (const Group& g : x->groups) {
+    if (out7 && k < cap) {
+      int* o = out7 + 7 * k;
+      o[0] = g.phase; o[1] = g.fast; o[2] = g.final; o[3] = g.n; o[4] = g.maxN; o[5] = g.grid; o[6] = g.block;
+    }
+    k++;
+  }
+  *n = k;
   return HAMSTR_OK;
 }
 

@@ -29,7 +29,8 @@ EXPORTS = [
     "hamstr_compute_linear_rhs", "hamstr_step", "hamstr_last_norms", "hamstr_sync", "hamstr_set_timing",
     "hamstr_last_timings", "hamstr_last_timing_names", "hamstr_flux_roe2d", "hamstr_jacobian_roe2d", "hamstr_muscl", "hamstr_weno",
     "hamstr_muscl_deriv", "hamstr_weno_deriv", "hamstr_block_tridag4", "hamstr_block_tridag_periodic4",
-    "hamstr_last_error",
+    "hamstr_last_error", "hamstr_batch_create", "hamstr_batch_info", "hamstr_batch_step", "hamstr_autotune",
+    "hamstr_launch_groups",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -64,6 +65,7 @@ def lib() -> C.CDLL:
         L = C.CDLL(LIB_PATH)
         L.hamstr_last_error.restype = C.c_char_p
         L.hamstr_create.argtypes = [C.POINTER(MeshDesc), C.POINTER(Params), C.POINTER(C.c_void_p)]
+        L.hamstr_batch_create.argtypes = [C.c_int, C.POINTER(MeshDesc), C.POINTER(Params), C.POINTER(C.c_void_p)]
         for name in EXPORTS:
             getattr(L, name)  # raises if missing
         _lib = L
@@ -282,3 +284,69 @@ def block_tridag4(a, b, c, f, periodic=False):
     fn = lib().hamstr_block_tridag_periodic4 if periodic else lib().hamstr_block_tridag4
     _chk(fn(nsys, N, _d(a), _d(b), _d(c), _d(x)))
     return x
+
+
+class Batch(Solver):
+    """Many flow cases in one context (hamstr_batch_create); every launch
+    covers all cases.  cases: list of (mesh, mach, alpha)."""
+
+    def __init__(self, cases, order: int, cfl: float, msweep: int = 1, lhs: int = LHS_FP64,
+                 adi_mode: int = ADI_FAST, sol_visc: int = 0):
+        n = len(cases)
+        descs = (MeshDesc * n)()
+        prms = (Params * n)()
+        self._keep = []
+        self.meshes = []
+        for k, (mesh, mach, alpha) in enumerate(cases):
+            faces, nv, rm = preprocess(mesh.x, mesh.faces, sol_visc)
+            st = np.ascontiguousarray(mesh.chain_start, np.int32)
+            cc = np.ascontiguousarray(mesh.chain_conn, np.int32)
+            self._keep += [faces, nv, rm, st, cc]
+            descs[k] = MeshDesc(mesh.n_cell, mesh.n_face, mesh.n_chain, _i(faces), _i(st), _i(cc), _d(nv), _d(rm))
+            prms[k] = Params(float(mach), float(alpha), float(cfl), int(order), int(msweep), int(lhs), int(adi_mode))
+            self.meshes.append(mesh)
+        h = C.c_void_p()
+        _chk(lib().hamstr_batch_create(n, descs, prms, C.byref(h)))
+        self.h = h
+        self.n_case = n
+        off = np.zeros(n + 1, np.int32)
+        nc = C.c_int()
+        _chk(lib().hamstr_batch_info(self.h, C.byref(nc), _i(off)))
+        self.cell_off = off
+        self.n_cell = int(off[-1])
+        self.n_face = sum(m.n_face for m in self.meshes)
+        self.order, self.cfl, self.msweep, self.lhs = order, cfl, msweep, lhs
+
+    def get(self, name: str) -> np.ndarray:
+        if name == "ff":
+            raise ValueError("per-face Jacobians of a batch: use a single-case Solver")
+        return super().get(name)
+
+    def case_q(self, k: int) -> np.ndarray:
+        q = self.get("q").reshape(-1, 4)
+        return q[self.cell_off[k]:self.cell_off[k + 1]].copy()
+
+    def batch_step(self, nsteps: int, tol: float = 0.0, check_every: int = 50):
+        """Returns (steps run, iterations-to-tol per case (-1: not reached), norms (ncase, 4))."""
+        iters = np.full(self.n_case, -1, np.int32)
+        norms = np.zeros((self.n_case, 4))
+        done = C.c_int()
+        _chk(lib().hamstr_batch_step(self.h, int(nsteps), C.c_double(tol), int(check_every), _i(iters), _d(norms),
+                                     C.byref(done)))
+        return done.value, iters, norms
+
+
+def autotune(solver: Solver, reps: int = 3):
+    """Runs the launch-shape autotuner; returns the chosen launch groups."""
+    best = np.zeros(64)
+    n = C.c_int()
+    _chk(lib().hamstr_autotune(solver.h, int(reps), _d(best), 64, C.byref(n)))
+    return launch_groups(solver), best[: n.value]
+
+
+def launch_groups(solver: Solver):
+    out = np.zeros(7 * 64, np.int32)
+    n = C.c_int()
+    _chk(lib().hamstr_launch_groups(solver.h, _i(out), 64, C.byref(n)))
+    keys = ("phase", "fast", "final", "chains", "max_rows", "grid", "block")
+    return [dict(zip(keys, map(int, out[7 * k:7 * k + 7]))) for k in range(n.value)]

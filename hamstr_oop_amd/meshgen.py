@@ -148,6 +148,8 @@ def build_ogrid(xy: np.ndarray, loop_start: int = 2, name: str = "ogrid") -> Mes
         raise ValueError("O-grid needs nc >= 5 and nr >= 3")
     x = xy.transpose(1, 0, 2).reshape(-1, 2).copy()   # node i + nc*j
 
+    I, J = np.meshgrid(np.arange(nc), np.arange(nr), indexing="xy")   # (nr, nc)
+
     def node(i, j):
         return (i % nc) + nc * j
 
@@ -156,38 +158,45 @@ def build_ogrid(xy: np.ndarray, loop_start: int = 2, name: str = "ogrid") -> Mes
 
     ncell = nc * nr
     conn = np.empty((ncell, 4), np.int32)
-    for j in range(nr):
-        for i in range(nc):
-            # counter-clockwise around the cell (see module docstring)
-            conn[cell(i, j)] = (node(i + 1, j), node(i, j), node(i, j + 1), node(i + 1, j + 1))
+    # counter-clockwise around the cell (see module docstring)
+    conn[cell(I, J).ravel()] = np.stack([node(I + 1, J), node(I, J), node(I, J + 1), node(I + 1, J + 1)],
+                                        axis=-1).reshape(-1, 4)
 
     n_loop_faces = nc * nr
     n_line_faces = nc * (nr + 1)
     faces = np.empty((n_loop_faces + n_line_faces, 6), np.int32)
     # local edge ids: 0 inner, 1 at column i+1, 2 outer, 3 at column i
-    for j in range(nr):
-        for k in range(nc):
-            fid = j * nc + k
-            faces[fid] = (node(k, j), node(k, j + 1), cell(k, j), 3, cell(k - 1, j), 1)
-    for i in range(nc):
-        base = n_loop_faces + i * (nr + 1)
-        faces[base] = (node(i + 1, 0), node(i, 0), cell(i, 0), 0, BC_FREESTREAM, -1)
-        for j in range(1, nr):
-            faces[base + j] = (node(i + 1, j), node(i, j), cell(i, j), 0, cell(i, j - 1), 2)
-        faces[base + nr] = (node(i, nr), node(i + 1, nr), cell(i, nr - 1), 2, BC_FREESTREAM, -1)
+    K = I  # loop face at node column k of layer j
+    faces[:n_loop_faces] = np.stack([node(K, J), node(K, J + 1), cell(K, J), np.full_like(K, 3),
+                                     cell(K - 1, J), np.full_like(K, 1)], axis=-1).reshape(-1, 6)
+    ii = np.arange(nc)[:, None]
+    jj = np.arange(nr + 1)[None, :]
+    lf = np.empty((nc, nr + 1, 6), np.int64)
+    lf[..., 0] = node(ii + 1, jj)
+    lf[..., 1] = node(ii, jj)
+    lf[..., 2] = cell(ii, jj)
+    lf[..., 3] = 0
+    lf[..., 4] = cell(ii, jj - 1)
+    lf[..., 5] = 2
+    lf[:, 0, 4] = BC_FREESTREAM
+    lf[:, 0, 5] = -1
+    # far-field face: left is the outermost cell, n1 -> n2 counter-clockwise
+    lf[:, nr, 0] = node(np.arange(nc), nr)
+    lf[:, nr, 1] = node(np.arange(nc) + 1, nr)
+    lf[:, nr, 2] = cell(np.arange(nc), nr - 1)
+    lf[:, nr, 3] = 2
+    lf[:, nr, 4] = BC_FREESTREAM
+    lf[:, nr, 5] = -1
+    faces[n_loop_faces:] = lf.reshape(-1, 6)
 
-    chains = []
-    for j in range(nr):
-        ks = [(loop_start + m) % nc for m in range(nc)]
-        ch = [j * nc + k for k in ks]
-        chains.append(ch + [ch[0]])
-    for i in range(nc):
-        base = n_loop_faces + i * (nr + 1)
-        chains.append([base + j for j in range(nr + 1)])
-    start = np.zeros(len(chains) + 1, np.int32)
-    for c, ch in enumerate(chains):
-        start[c + 1] = start[c] + len(ch)
-    conn_list = np.concatenate([np.asarray(ch, np.int32) for ch in chains])
+    ks = (loop_start + np.arange(nc)) % nc
+    loops = np.arange(nr)[:, None] * nc + ks[None, :]
+    loops = np.concatenate([loops, loops[:, :1]], axis=1)               # first face again
+    lines = n_loop_faces + np.arange(nc)[:, None] * (nr + 1) + np.arange(nr + 1)[None, :]
+    conn_list = np.concatenate([loops.ravel(), lines.ravel()]).astype(np.int32)
+    lens = [nc + 1] * nr + [nr + 1] * nc
+    start = np.zeros(len(lens) + 1, np.int32)
+    start[1:] = np.cumsum(lens)
     return Mesh(nc=nc, nr=nr, x=x, conn=conn, faces=faces, chain_start=start,
                 chain_conn=conn_list, chains_per_color=[nr, nc], name=name)
 

@@ -91,6 +91,31 @@ int hamstr_create(const hamstr_mesh_desc *mesh, const hamstr_params *params,
                   hamstr_ctx **out);
 int hamstr_destroy(hamstr_ctx *ctx);
 
+/* A batch of independent flow cases in one context: the disjoint union of
+ * their meshes, every kernel launch covering all cases.  Cases share cfl,
+ * order, msweep, lhs and adi_mode; mach and alpha are per case.  Fields are
+ * the union's arrays, case k owning cells [cell_off[k], cell_off[k+1]). */
+int hamstr_batch_create(int ncases, const hamstr_mesh_desc *meshes, const hamstr_params *params,
+                        hamstr_ctx **out);
+/* *ncases, and cell_off[ncases+1] when not NULL */
+int hamstr_batch_info(const hamstr_ctx *ctx, int *ncases, int *cell_off);
+/* Up to nsteps iterations of every case.  iters[k] (may be NULL) gets the
+ * first step (1-based) at which case k's nonlinear L2 <= tol, or -1; the run
+ * stops at the first read-back (every check_every steps) after all cases have
+ * converged.  norms[4*ncases] (may be NULL) gets each case's (L2, Linf) of
+ * the nonlinear and linear residuals at the last step.  *done = steps run. */
+int hamstr_batch_step(hamstr_ctx *ctx, int nsteps, double tol, int check_every, int *iters,
+                      double *norms, int *done);
+
+/* Autotuner: times every FAST launch group of the ADI phases at several
+ * persistent grid sizes (all chains; 8, 4, 2, 1 workgroups per CU) on the
+ * current state, keeps the fastest, restores r and dq.  best_ms[k] (k < cap)
+ * gets group k's best time (-1 for groups with nothing to tune). */
+int hamstr_autotune(hamstr_ctx *ctx, int reps, double *best_ms, int cap, int *ngroups);
+/* Launch groups, 7 ints each: phase, fast, final solve (0 open, 1 two-row
+ * pairs, 2 odd cyclic), chains, max rows, grid, block. */
+int hamstr_launch_groups(const hamstr_ctx *ctx, int *out7, int cap, int *n);
+
 /* SOLVER::initFlow (initFlow.cpp:12-93): q := freestream. */
 int hamstr_init_flow(hamstr_ctx *ctx);
 /* freestream (rinf, rinf*uinf, rinf*vinf, einf) as initFlow computes it */

@@ -1,0 +1,72 @@
+"""Batched contexts: many flow cases in one set of launches.
+
+A case inside a batch runs the same arithmetic as alone, so its state must be
+bit-identical to a single-case context (EXACT and FAST alike), and the
+per-case norms and convergence iterations must agree.  The autotuner only
+changes launch shapes, so results must not change either.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def hip():
+    from hamstr_oop_amd import hip as H
+    H.lib()
+    return H
+
+
+def _cases():
+    from hamstr_oop_amd import meshgen
+    return [
+        (meshgen.circle_ogrid(16, 8), 0.3, 0.0),
+        (meshgen.naca_ogrid(32, 12), 0.8, 1.25),
+        (meshgen.naca_ogrid(24, 10, t=0.10), 0.5, 2.0),   # 24-row loops: odd final subsystems
+        (meshgen.naca_ogrid(64, 16, t=0.14), 0.6, -3.0),
+    ]
+
+
+@pytest.mark.parametrize("mode", ["exact", "fast", "fast32"])
+def test_batch_matches_single_cases(hip, mode):
+    adi = hip.ADI_EXACT if mode == "exact" else hip.ADI_FAST
+    lhs = hip.LHS_FP32 if mode == "fast32" else hip.LHS_FP64
+    cases = _cases()
+    b = hip.Batch(cases, order=3, cfl=10.0, lhs=lhs, adi_mode=adi)
+    n, iters, norms = b.batch_step(12, check_every=12)
+    assert n == 12
+    for k, (mesh, mach, alpha) in enumerate(cases):
+        s = hip.Solver(mesh, mach, alpha, 3, 10.0, lhs=lhs, adi_mode=adi)
+        _, l2, l2l = s.step(12, check_every=12)
+        assert np.array_equal(b.case_q(k).ravel(), s.get("q")), f"case {k}"
+        assert norms[k, 0] == pytest.approx(l2[-1], rel=1e-12)
+        assert norms[k, 2] == pytest.approx(l2l[-1], rel=1e-12)
+        s.close()
+
+
+def test_batch_convergence_iterations(hip):
+    cases = _cases()
+    tol = 1e-9
+    b = hip.Batch(cases, order=1, cfl=10.0, adi_mode=hip.ADI_FAST)
+    n, iters, _ = b.batch_step(4000, tol=tol, check_every=20)
+    assert (iters > 0).all()
+    for k, (mesh, mach, alpha) in enumerate(cases):
+        s = hip.Solver(mesh, mach, alpha, 1, 10.0, adi_mode=hip.ADI_FAST)
+        _, l2, _ = s.step(int(iters[k]) + 5, check_every=10)
+        first = int(np.argmax(l2 <= tol)) + 1
+        assert l2[first - 1] <= tol and first == iters[k], f"case {k}: {first} vs {iters[k]}"
+        s.close()
+
+
+def test_autotune_keeps_results(hip):
+    cases = _cases()
+    b1 = hip.Batch(cases, order=3, cfl=10.0, adi_mode=hip.ADI_FAST)
+    b2 = hip.Batch(cases, order=3, cfl=10.0, adi_mode=hip.ADI_FAST)
+    b2.step(2)  # a state to tune on
+    groups, best = hip.autotune(b2)
+    assert all(g["grid"] >= 1 for g in groups)
+    b2.initFlow()
+    b1.batch_step(8, check_every=8)
+    b2.batch_step(8, check_every=8)
+    assert np.array_equal(b1.get("q"), b2.get("q"))
