@@ -7,7 +7,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SRC = os.path.join(HERE, "csrc", "hamstr_hip.hip")
+SRCS = [os.path.join(HERE, "csrc", "hamstr_hip.hip"), os.path.join(HERE, "csrc", "mlp_hip.hip")]
 LIB = os.path.join(HERE, "libhamstr_hip.so")
 DRIVER_SRC = os.path.join(HERE, "csrc", "hamstr_main.cpp")
 DRIVER = os.path.join(HERE, "hamstr_mi355x")
@@ -23,7 +23,7 @@ HIPCC_FLAGS = [
 
 def build(verbose: bool = False) -> str:
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    cmd = [hipcc, *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), SRC, "-o", LIB + ".tmp"]
+    cmd = [hipcc, *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), *SRCS, "-o", LIB + ".tmp"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)

@@ -1908,6 +1908,12 @@ int hamstr_destroy(hamstr_ctx* x) {
   return HAMSTR_OK;
 }
 
+int hamstr_case_freestream(const hamstr_ctx* x, int k, double* out4) {
+  if (!x || !out4 || k < 0 || k >= x->nCase) return fail(HAMSTR_ERR_ARG, "bad arguments");
+  for (int j = 0; j < 4; j++) out4[j] = x->qinf_h[4 * k + j];
+  return HAMSTR_OK;
+}
+
 int hamstr_freestream(const hamstr_ctx* x, double* out4) {
   if (!x || !out4) return fail(HAMSTR_ERR_ARG, "null argument");
   for (int k = 0; k < 4; k++) out4[k] = x->qinf_h[k];

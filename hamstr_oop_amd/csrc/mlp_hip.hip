@@ -1,0 +1,604 @@
+// mlp_hip.hip -- the learned warm start: an MLP (6 -> 256 x 4 SiLU -> 4) that
+// maps (Mach, alpha, thickness, cell centroid x, y, wall distance) to the
+// primitive state (rho, u, v, p).  The three 256x256 layers run on BF16
+// matrix cores (v_mfma_f32_32x32x8_bf16, FP32 accumulation) for the forward
+// pass and both backward products; the 6-wide input and 4-wide output layers
+// run on the vector ALUs in FP32.  Weights are FP32 masters with BF16 copies;
+// training is Adam on mean-squared error.  All reductions are split into fixed
+// partials and summed in a fixed order, so training is deterministic.
+//
+// The network only supplies the initial state of a solve (hamstr_warm_start);
+// the ADI iteration decides convergence, and cells whose prediction is not a
+// valid state fall back to the freestream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/hamstr_hip.h"
+
+namespace {
+
+thread_local std::string m_err;
+int mfail(int code, const std::string& msg) {
+  m_err = msg;
+  return code;
+}
+#define MCHK(x)                                                                                \
+  do {                                                                                         \
+    hipError_t e_ = (x);                                                                       \
+    if (e_ != hipSuccess) return mfail(HAMSTR_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr int kIn = 6, kH = 256, kOut = 4, kHidden = 4;
+// flat parameter layout (floats)
+constexpr int oW1 = 0, oB1 = oW1 + kIn * kH;
+constexpr int oW2 = oB1 + kH, oB2 = oW2 + kH * kH;
+constexpr int oW3 = oB2 + kH, oB3 = oW3 + kH * kH;
+constexpr int oW4 = oB3 + kH, oB4 = oW4 + kH * kH;
+constexpr int oW5 = oB4 + kH, oB5 = oW5 + kH * kOut;
+constexpr int kParams = oB5 + kOut;  // 200196
+constexpr int kSplit = 64;           // split-K partials for reductions over the batch
+
+typedef unsigned short bf16;
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v16f __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ bf16 f2bf(float f) {
+  unsigned u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+  return (bf16)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(bf16 h) { return __uint_as_float(((unsigned)h) << 16); }
+__device__ __forceinline__ float sigm(float z) { return 1.0f / (1.0f + __expf(-z)); }
+
+// --------------------------------------------------------------------------
+// C[M x 256] (+)= A[M x K] * B[K x 256] on matrix cores.  One wave per 32x32
+// tile, four waves per block covering 32 rows x 128 columns; K in steps of 8.
+// Element (m,k) of A is a[m*sam + k*sak], element (k,n) of B is b[k*sbk + n*sbn].
+// Lane l supplies A[m0 + l%32][k0 + 4*(l/32) + j] and B[k0 + 4*(l/32) + j][n0 + l%32],
+// j = 0..3, and holds D[m0 + 8*(v/4) + 4*(l/32) + v%4][n0 + l%32] in register v.
+// EPI 0: Z = acc + bias -> zout (f32), H = silu(Z) -> hout (bf16)
+// EPI 1: plain store of acc -> zout (f32)
+// EPI 2: split-K partial over blockIdx.z: rows of A in [z*kchunk, ...) -> zout + z*M*256
+// --------------------------------------------------------------------------
+template <int EPI>
+__global__ __launch_bounds__(256) void k_gemm_mfma(const bf16* __restrict__ a, long sam, long sak,
+                                                   const bf16* __restrict__ b, long sbk, long sbn, int M, int K,
+                                                   int kchunk, float* __restrict__ zout,
+                                                   const float* __restrict__ bias, bf16* __restrict__ hout) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int m0 = blockIdx.x * 32;
+  const int n0 = blockIdx.y * 128 + wave * 32;
+  int kb = 0, ke = K;
+  if (EPI == 2) {
+    kb = blockIdx.z * kchunk;
+    ke = min(K, kb + kchunk);
+  }
+  const int r = lane & 31, half = lane >> 5;
+  v16f acc = {};
+  for (int k0 = kb; k0 < ke; k0 += 8) {
+    v4s av, bv;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int k = k0 + 4 * half + j;
+      av[j] = (short)a[(long)(m0 + r) * sam + (long)k * sak];
+      bv[j] = (short)b[(long)k * sbk + (long)(n0 + r) * sbn];
+    }
+    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, acc, 0, 0, 0);
+  }
+  float* out = zout + (EPI == 2 ? (long)blockIdx.z * M * kH : 0);
+#pragma unroll
+  for (int v = 0; v < 16; v++) {
+    const int row = m0 + 8 * (v / 4) + 4 * half + (v % 4);
+    const int col = n0 + r;
+    const long o = (long)row * kH + col;
+    if (EPI == 0) {
+      const float z = acc[v] + bias[col];
+      out[o] = z;
+      hout[o] = f2bf(z * sigm(z));
+    } else {
+      out[o] = acc[v];
+    }
+  }
+}
+
+// layer 1 (K = 6) on the vector ALUs: Z1 = X W1 + b1, H1 = silu(Z1)
+__global__ void k_layer1(const float* __restrict__ x, const float* __restrict__ p, int B, float* __restrict__ z,
+                         bf16* __restrict__ h) {
+  const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (long)B * kH) return;
+  const int row = id / kH, col = id % kH;
+  float s = p[oB1 + col];
+#pragma unroll
+  for (int k = 0; k < kIn; k++) s += x[row * kIn + k] * p[oW1 + k * kH + col];
+  z[id] = s;
+  h[id] = f2bf(s * sigm(s));
+}
+
+// output layer (N = 4): Y = H4 W5 + b5
+__global__ void k_out(const bf16* __restrict__ h, const float* __restrict__ p, int B, float* __restrict__ y) {
+  const int row = blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= B) return;
+  float s[kOut];
+#pragma unroll
+  for (int j = 0; j < kOut; j++) s[j] = p[oB5 + j];
+  for (int k = 0; k < kH; k++) {
+    const float hv = bf2f(h[(long)row * kH + k]);
+#pragma unroll
+    for (int j = 0; j < kOut; j++) s[j] += hv * p[oW5 + k * kOut + j];
+  }
+#pragma unroll
+  for (int j = 0; j < kOut; j++) y[row * kOut + j] = s[j];
+}
+
+// dY = 2 (Y - T) / (B * 4); per-block loss partials
+__global__ void k_loss(const float* __restrict__ y, const float* __restrict__ t, int B, float* __restrict__ dy,
+                       float* __restrict__ lpart) {
+  __shared__ float sh[256];
+  const int id = blockIdx.x * blockDim.x + threadIdx.x;
+  float l = 0.f;
+  if (id < B * kOut) {
+    const float d = y[id] - t[id];
+    dy[id] = 2.0f * d / (float)(B * kOut);
+    l = d * d / (float)(B * kOut);
+  }
+  sh[threadIdx.x] = l;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) lpart[blockIdx.x] = sh[0];
+}
+
+// dH4 = dY W5^T (f32)
+__global__ void k_out_bwd_dh(const float* __restrict__ dy, const float* __restrict__ p, int B,
+                             float* __restrict__ dh) {
+  const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= (long)B * kH) return;
+  const int row = id / kH, col = id % kH;
+  float s = 0.f;
+#pragma unroll
+  for (int j = 0; j < kOut; j++) s += dy[row * kOut + j] * p[oW5 + col * kOut + j];
+  dh[id] = s;
+}
+
+// dZ = dH * silu'(Z), as f32 and bf16
+__global__ void k_silu_bwd(const float* __restrict__ dh, const float* __restrict__ z, long n,
+                           float* __restrict__ dz, bf16* __restrict__ dzb) {
+  const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (id >= n) return;
+  const float zz = z[id];
+  const float s = sigm(zz);
+  const float g = dh[id] * (s + zz * s * (1.0f - s));
+  dz[id] = g;
+  dzb[id] = f2bf(g);
+}
+
+// split-K partial column products over the batch for the VALU layers:
+// part[z][i][j] = sum_{rows in chunk z} L[row][i] * R[row][j]; L f32 or bf16
+template <typename TL>
+__global__ void k_outer_partial(const TL* __restrict__ L, int nl, const float* __restrict__ R, int nr, int B,
+                                int chunk, float* __restrict__ part) {
+  const int i = blockIdx.x, z = blockIdx.y;
+  const int r0 = z * chunk, r1 = min(B, r0 + chunk);
+  for (int j = threadIdx.x; j < nr; j += blockDim.x) {
+    float s = 0.f;
+    for (int row = r0; row < r1; row++) {
+      float lv;
+      if constexpr (sizeof(TL) == 2)
+        lv = bf2f(L[(long)row * nl + i]);
+      else
+        lv = L[(long)row * nl + i];
+      s += lv * R[(long)row * nr + j];
+    }
+    part[((long)z * nl + i) * nr + j] = s;
+  }
+}
+
+// column sums (bias gradients) split over the batch
+__global__ void k_colsum_partial(const float* __restrict__ d, int ncol, int B, int chunk,
+                                 float* __restrict__ part) {
+  const int z = blockIdx.y;
+  const int r0 = z * chunk, r1 = min(B, r0 + chunk);
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < ncol; j += gridDim.x * blockDim.x) {
+    float s = 0.f;
+    for (int row = r0; row < r1; row++) s += d[(long)row * ncol + j];
+    part[(long)z * ncol + j] = s;
+  }
+}
+
+// out[i] = sum_z part[z][i], fixed order
+__global__ void k_reduce(const float* __restrict__ part, int nz, long n, float* __restrict__ out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = 0.f;
+  for (int z = 0; z < nz; z++) s += part[(long)z * n + i];
+  out[i] = s;
+}
+
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, int n, float lr, float b1, float b2, float c1, float c2, float eps) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// BF16 copies of a 256x256 weight: w (row-major K x N) and wt (N x K)
+__global__ void k_w_bf16(const float* __restrict__ w, bf16* __restrict__ wb, bf16* __restrict__ wtb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= kH * kH) return;
+  const int k = i / kH, n = i % kH;
+  const bf16 h = f2bf(w[i]);
+  wb[i] = h;
+  wtb[n * kH + k] = h;
+}
+
+// gather a training batch by a per-step hash of (seed, step, i)
+__global__ void k_gather(const float* __restrict__ X, const float* __restrict__ T, long n, int B, unsigned seed,
+                         unsigned step, float* __restrict__ xb, float* __restrict__ tb) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= B) return;
+  unsigned long long h = ((unsigned long long)seed << 32) ^ ((unsigned long long)step * 0x9E3779B97F4A7C15ull) ^
+                         (unsigned long long)i * 0xBF58476D1CE4E5B9ull;
+  h ^= h >> 31;
+  h *= 0x94D049BB133111EBull;
+  h ^= h >> 29;
+  const long j = (long)(h % (unsigned long long)n);
+#pragma unroll
+  for (int k = 0; k < kIn; k++) xb[i * kIn + k] = X[j * kIn + k];
+#pragma unroll
+  for (int k = 0; k < kOut; k++) tb[i * kOut + k] = T[j * kOut + k];
+}
+
+// features -> normalised -> (forward done by the caller) ; y normalised -> q
+// primitive (rho, u, v, p) -> conserved with gamma = 1.4; invalid -> freestream
+__global__ void k_norm_features(const float* __restrict__ f, const float* __restrict__ nrm, int n,
+                                float* __restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n * kIn) return;
+  const int k = i % kIn;
+  x[i] = (f[i] - nrm[k]) / nrm[kIn + k];
+}
+
+__global__ void k_to_state(const float* __restrict__ y, const float* __restrict__ nrm, int n,
+                           const double4 qinf, double* __restrict__ q, int* __restrict__ nfallback) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double pr[kOut];
+#pragma unroll
+  for (int k = 0; k < kOut; k++) pr[k] = (double)y[i * kOut + k] * nrm[2 * kIn + kOut + k] + nrm[2 * kIn + k];
+  const double rho = pr[0], u = pr[1], v = pr[2], p = pr[3];
+  double4 s;
+  const bool ok = rho > 1e-3 && p > 1e-3 && rho == rho && u == u && v == v && p == p && fabs(u) < 10 && fabs(v) < 10;
+  if (ok) {
+    s = make_double4(rho, rho * u, rho * v, p / 0.4 + 0.5 * rho * (u * u + v * v));
+  } else {
+    s = qinf;
+    atomicAdd(nfallback, 1);
+  }
+  *reinterpret_cast<double4*>(q + 4 * (size_t)i) = s;
+}
+
+}  // namespace
+
+struct hamstr_mlp {
+  float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
+  bf16 *wb[3] = {nullptr, nullptr, nullptr}, *wtb[3] = {nullptr, nullptr, nullptr};
+  // activations for a batch of up to cap rows
+  int cap = 0;
+  float *x = nullptr, *t = nullptr, *y = nullptr, *dy = nullptr, *part = nullptr, *lpart = nullptr;
+  float *z[kHidden] = {nullptr, nullptr, nullptr, nullptr};
+  bf16* h[kHidden] = {nullptr, nullptr, nullptr, nullptr};
+  float *dh = nullptr, *dz = nullptr;
+  bf16* dzb = nullptr;
+  // training set
+  float *X = nullptr, *T = nullptr;
+  long n = 0;
+  long adam_t = 0;
+  hipStream_t stream = nullptr;
+};
+
+namespace {
+
+const int kHW[3] = {oW2, oW3, oW4};
+const int kHB[3] = {oB2, oB3, oB4};
+
+int refresh_bf16(hamstr_mlp* M) {
+  for (int l = 0; l < 3; l++)
+    hipLaunchKernelGGL(k_w_bf16, dim3((kH * kH + 255) / 256), dim3(256), 0, M->stream, M->p + kHW[l], M->wb[l],
+                       M->wtb[l]);
+  MCHK(hipGetLastError());
+  return HAMSTR_OK;
+}
+
+int ensure_cap(hamstr_mlp* M, int B) {
+  if (M->cap >= B) return HAMSTR_OK;
+  void* old[] = {M->x, M->t, M->y, M->dy, M->part, M->lpart, M->dh, M->dz, M->dzb,
+                 M->z[0], M->z[1], M->z[2], M->z[3], M->h[0], M->h[1], M->h[2], M->h[3]};
+  for (void* q : old)
+    if (q) (void)hipFree(q);
+  const long bh = (long)B * kH;
+  MCHK(hipMalloc(&M->x, sizeof(float) * B * kIn));
+  MCHK(hipMalloc(&M->t, sizeof(float) * B * kOut));
+  MCHK(hipMalloc(&M->y, sizeof(float) * B * kOut));
+  MCHK(hipMalloc(&M->dy, sizeof(float) * B * kOut));
+  MCHK(hipMalloc(&M->part, sizeof(float) * (long)kSplit * kH * kH));
+  MCHK(hipMalloc(&M->lpart, sizeof(float) * ((B * kOut + 255) / 256)));
+  MCHK(hipMalloc(&M->dh, sizeof(float) * bh));
+  MCHK(hipMalloc(&M->dz, sizeof(float) * bh));
+  MCHK(hipMalloc(&M->dzb, sizeof(bf16) * bh));
+  for (int l = 0; l < kHidden; l++) {
+    MCHK(hipMalloc(&M->z[l], sizeof(float) * bh));
+    MCHK(hipMalloc(&M->h[l], sizeof(bf16) * bh));
+  }
+  M->cap = B;
+  return HAMSTR_OK;
+}
+
+// forward on M->x (B rows, B a multiple of 32) -> M->y
+int forward(hamstr_mlp* M, int B) {
+  const long bh = (long)B * kH;
+  hipLaunchKernelGGL(k_layer1, dim3((bh + 255) / 256), dim3(256), 0, M->stream, M->x, M->p, B, M->z[0], M->h[0]);
+  for (int l = 0; l < 3; l++)
+    hipLaunchKernelGGL((k_gemm_mfma<0>), dim3(B / 32, kH / 128), dim3(256), 0, M->stream, M->h[l], (long)kH, 1L,
+                       M->wtb[l], 1L, (long)kH, B, kH, 0, M->z[l + 1], M->p + kHB[l], M->h[l + 1]);
+  hipLaunchKernelGGL(k_out, dim3((B + 255) / 256), dim3(256), 0, M->stream, M->h[3], M->p, B, M->y);
+  MCHK(hipGetLastError());
+  return HAMSTR_OK;
+}
+
+// loss and gradient (into M->g) for the batch in M->x, M->t; *loss on the host
+int backward(hamstr_mlp* M, int B, float* loss) {
+  const long bh = (long)B * kH;
+  const int nlb = (B * kOut + 255) / 256;
+  hipLaunchKernelGGL(k_loss, dim3(nlb), dim3(256), 0, M->stream, M->y, M->t, B, M->dy, M->lpart);
+  const int chunk = (B + kSplit - 1) / kSplit;
+  const int chunk8 = (chunk + 7) / 8 * 8;  // matrix-core K steps of 8 rows
+  // output layer: dW5 = H4^T dY, db5 = sum dY, dH4 = dY W5^T
+  hipLaunchKernelGGL((k_outer_partial<bf16>), dim3(kH, kSplit), dim3(64), 0, M->stream, M->h[3], kH, M->dy, kOut, B,
+                     chunk, M->part);
+  hipLaunchKernelGGL(k_reduce, dim3((kH * kOut + 255) / 256), dim3(256), 0, M->stream, M->part, kSplit,
+                     (long)kH * kOut, M->g + oW5);
+  hipLaunchKernelGGL(k_colsum_partial, dim3(1, kSplit), dim3(64), 0, M->stream, M->dy, kOut, B, chunk, M->part);
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, M->stream, M->part, kSplit, (long)kOut, M->g + oB5);
+  hipLaunchKernelGGL(k_out_bwd_dh, dim3((bh + 255) / 256), dim3(256), 0, M->stream, M->dy, M->p, B, M->dh);
+  // hidden layers 4, 3, 2 on matrix cores
+  for (int l = 2; l >= 0; l--) {
+    hipLaunchKernelGGL(k_silu_bwd, dim3((bh + 255) / 256), dim3(256), 0, M->stream, M->dh, M->z[l + 1], bh, M->dz,
+                       M->dzb);
+    // dW = H_l^T dZ: A = H_l^T (m = input unit, k = row), B = dZ (k = row, n = output unit)
+    hipLaunchKernelGGL((k_gemm_mfma<2>), dim3(kH / 32, kH / 128, kSplit), dim3(256), 0, M->stream, M->h[l], 1L,
+                       (long)kH, M->dzb, (long)kH, 1L, kH, B, chunk8, M->part, (const float*)nullptr, (bf16*)nullptr);
+    hipLaunchKernelGGL(k_reduce, dim3((kH * kH + 255) / 256), dim3(256), 0, M->stream, M->part, kSplit,
+                       (long)kH * kH, M->g + kHW[l]);
+    hipLaunchKernelGGL(k_colsum_partial, dim3(1, kSplit), dim3(256), 0, M->stream, M->dz, kH, B, chunk, M->part);
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, M->stream, M->part, kSplit, (long)kH, M->g + kHB[l]);
+    // dH_l = dZ W^T: A = dZ (m = row, k = output unit), B(k = output unit, n = input unit) = W[n][k]
+    hipLaunchKernelGGL((k_gemm_mfma<1>), dim3(B / 32, kH / 128), dim3(256), 0, M->stream, M->dzb, (long)kH, 1L,
+                       M->wb[l], 1L, (long)kH, B, kH, 0, M->dh, (const float*)nullptr, (bf16*)nullptr);
+  }
+  // layer 1: dZ1 = dH1 silu'(Z1); dW1 = X^T dZ1, db1 = sum dZ1
+  hipLaunchKernelGGL(k_silu_bwd, dim3((bh + 255) / 256), dim3(256), 0, M->stream, M->dh, M->z[0], bh, M->dz, M->dzb);
+  hipLaunchKernelGGL((k_outer_partial<float>), dim3(kIn, kSplit), dim3(256), 0, M->stream, M->x, kIn, M->dz, kH, B,
+                     chunk, M->part);
+  hipLaunchKernelGGL(k_reduce, dim3((kIn * kH + 255) / 256), dim3(256), 0, M->stream, M->part, kSplit,
+                     (long)kIn * kH, M->g + oW1);
+  hipLaunchKernelGGL(k_colsum_partial, dim3(1, kSplit), dim3(256), 0, M->stream, M->dz, kH, B, chunk, M->part);
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, M->stream, M->part, kSplit, (long)kH, M->g + oB1);
+  MCHK(hipGetLastError());
+  if (loss) {
+    std::vector<float> lp(nlb);
+    MCHK(hipMemcpyAsync(lp.data(), M->lpart, sizeof(float) * nlb, hipMemcpyDeviceToHost, M->stream));
+    MCHK(hipStreamSynchronize(M->stream));
+    double s = 0;
+    for (float v : lp) s += v;
+    *loss = (float)s;
+  }
+  return HAMSTR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* hamstr_mlp_last_error(void) { return m_err.c_str(); }
+
+int hamstr_mlp_create(unsigned seed, hamstr_mlp** out) {
+  if (!out) return mfail(HAMSTR_ERR_ARG, "null argument");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return mfail(HAMSTR_ERR_NODEV, "no HIP device");
+  hamstr_mlp* M = new hamstr_mlp();
+  // Kaiming-uniform-like initialisation from a host RNG, zero biases
+  std::mt19937 rng(seed);
+  std::vector<float> p(kParams, 0.0f);
+  auto fill = [&](int off, int fan_in, int n) {
+    std::uniform_real_distribution<float> d(-std::sqrt(6.0f / fan_in), std::sqrt(6.0f / fan_in));
+    for (int i = 0; i < n; i++) p[off + i] = d(rng);
+  };
+  fill(oW1, kIn, kIn * kH);
+  fill(oW2, kH, kH * kH);
+  fill(oW3, kH, kH * kH);
+  fill(oW4, kH, kH * kH);
+  fill(oW5, kH, kH * kOut);
+  MCHK(hipStreamCreateWithFlags(&M->stream, hipStreamNonBlocking));
+  MCHK(hipMalloc(&M->p, sizeof(float) * kParams));
+  MCHK(hipMalloc(&M->g, sizeof(float) * kParams));
+  MCHK(hipMalloc(&M->m, sizeof(float) * kParams));
+  MCHK(hipMalloc(&M->v, sizeof(float) * kParams));
+  for (int l = 0; l < 3; l++) {
+    MCHK(hipMalloc(&M->wb[l], sizeof(bf16) * kH * kH));
+    MCHK(hipMalloc(&M->wtb[l], sizeof(bf16) * kH * kH));
+  }
+  MCHK(hipMemcpy(M->p, p.data(), sizeof(float) * kParams, hipMemcpyHostToDevice));
+  MCHK(hipMemset(M->g, 0, sizeof(float) * kParams));
+  MCHK(hipMemset(M->m, 0, sizeof(float) * kParams));
+  MCHK(hipMemset(M->v, 0, sizeof(float) * kParams));
+  int rc = refresh_bf16(M);
+  if (rc) return rc;
+  MCHK(hipStreamSynchronize(M->stream));
+  *out = M;
+  return HAMSTR_OK;
+}
+
+int hamstr_mlp_destroy(hamstr_mlp* M) {
+  if (!M) return HAMSTR_OK;
+  (void)hipStreamSynchronize(M->stream);
+  void* ptrs[] = {M->p, M->g, M->m, M->v, M->wb[0], M->wb[1], M->wb[2], M->wtb[0], M->wtb[1], M->wtb[2],
+                  M->x, M->t, M->y, M->dy, M->part, M->lpart, M->dh, M->dz, M->dzb, M->z[0], M->z[1],
+                  M->z[2], M->z[3], M->h[0], M->h[1], M->h[2], M->h[3], M->X, M->T};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  if (M->stream) (void)hipStreamDestroy(M->stream);
+  delete M;
+  return HAMSTR_OK;
+}
+
+int hamstr_mlp_num_params(void) { return kParams; }
+
+int hamstr_mlp_get_params(hamstr_mlp* M, float* host) {
+  if (!M || !host) return mfail(HAMSTR_ERR_ARG, "null argument");
+  MCHK(hipMemcpy(host, M->p, sizeof(float) * kParams, hipMemcpyDeviceToHost));
+  return HAMSTR_OK;
+}
+
+int hamstr_mlp_set_params(hamstr_mlp* M, const float* host) {
+  if (!M || !host) return mfail(HAMSTR_ERR_ARG, "null argument");
+  MCHK(hipMemcpy(M->p, host, sizeof(float) * kParams, hipMemcpyHostToDevice));
+  int rc = refresh_bf16(M);
+  if (rc) return rc;
+  MCHK(hipStreamSynchronize(M->stream));
+  M->adam_t = 0;
+  MCHK(hipMemset(M->m, 0, sizeof(float) * kParams));
+  MCHK(hipMemset(M->v, 0, sizeof(float) * kParams));
+  return HAMSTR_OK;
+}
+
+// forward pass on n rows of (normalised) features; n padded internally to 32
+int hamstr_mlp_forward(hamstr_mlp* M, int n, const float* x, float* y) {
+  if (!M || n <= 0 || !x || !y) return mfail(HAMSTR_ERR_ARG, "bad arguments");
+  const int B = (n + 31) / 32 * 32;
+  int rc = ensure_cap(M, B);
+  if (rc) return rc;
+  MCHK(hipMemsetAsync(M->x, 0, sizeof(float) * B * kIn, M->stream));
+  MCHK(hipMemcpyAsync(M->x, x, sizeof(float) * n * kIn, hipMemcpyHostToDevice, M->stream));
+  if ((rc = forward(M, B))) return rc;
+  MCHK(hipMemcpyAsync(y, M->y, sizeof(float) * n * kOut, hipMemcpyDeviceToHost, M->stream));
+  MCHK(hipStreamSynchronize(M->stream));
+  return HAMSTR_OK;
+}
+
+// mean-squared-error loss and its gradient on n rows (n a multiple of 32)
+int hamstr_mlp_loss_grad(hamstr_mlp* M, int n, const float* x, const float* t, float* loss, float* grad) {
+  if (!M || n <= 0 || n % 32 || !x || !t || !loss || !grad) return mfail(HAMSTR_ERR_ARG, "bad arguments");
+  int rc = ensure_cap(M, n);
+  if (rc) return rc;
+  MCHK(hipMemcpyAsync(M->x, x, sizeof(float) * n * kIn, hipMemcpyHostToDevice, M->stream));
+  MCHK(hipMemcpyAsync(M->t, t, sizeof(float) * n * kOut, hipMemcpyHostToDevice, M->stream));
+  if ((rc = forward(M, n)) || (rc = backward(M, n, loss))) return rc;
+  MCHK(hipMemcpyAsync(grad, M->g, sizeof(float) * kParams, hipMemcpyDeviceToHost, M->stream));
+  MCHK(hipStreamSynchronize(M->stream));
+  return HAMSTR_OK;
+}
+
+// device-resident training set: n rows of normalised features and targets
+int hamstr_mlp_set_data(hamstr_mlp* M, long n, const float* x, const float* t) {
+  if (!M || n <= 0 || !x || !t) return mfail(HAMSTR_ERR_ARG, "bad arguments");
+  if (M->X) (void)hipFree(M->X);
+  if (M->T) (void)hipFree(M->T);
+  M->X = M->T = nullptr;
+  MCHK(hipMalloc(&M->X, sizeof(float) * n * kIn));
+  MCHK(hipMalloc(&M->T, sizeof(float) * n * kOut));
+  MCHK(hipMemcpy(M->X, x, sizeof(float) * n * kIn, hipMemcpyHostToDevice));
+  MCHK(hipMemcpy(M->T, t, sizeof(float) * n * kOut, hipMemcpyHostToDevice));
+  M->n = n;
+  return HAMSTR_OK;
+}
+
+// Adam on random batches of the training set; losses[steps] (may be NULL)
+int hamstr_mlp_train(hamstr_mlp* M, int steps, int batch, float lr, unsigned seed, float* losses) {
+  if (!M || steps < 0 || batch <= 0 || batch % 32 || M->n <= 0) return mfail(HAMSTR_ERR_ARG, "bad arguments");
+  int rc = ensure_cap(M, batch);
+  if (rc) return rc;
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  for (int s = 0; s < steps; s++) {
+    hipLaunchKernelGGL(k_gather, dim3((batch + 255) / 256), dim3(256), 0, M->stream, M->X, M->T, M->n, batch, seed,
+                       (unsigned)s, M->x, M->t);
+    if ((rc = forward(M, batch))) return rc;
+    float l = 0;
+    if ((rc = backward(M, batch, losses ? &l : nullptr))) return rc;
+    if (losses) losses[s] = l;
+    M->adam_t++;
+    const float c1 = 1.0f - std::pow(b1, (float)M->adam_t), c2 = 1.0f - std::pow(b2, (float)M->adam_t);
+    hipLaunchKernelGGL(k_adam, dim3((kParams + 255) / 256), dim3(256), 0, M->stream, M->p, M->g, M->m, M->v, kParams,
+                       lr, b1, b2, c1, c2, eps);
+    if ((rc = refresh_bf16(M))) return rc;
+  }
+  MCHK(hipStreamSynchronize(M->stream));
+  return HAMSTR_OK;
+}
+
+}  // extern "C"
+
+// ---- warm start of a solver context ------------------------------------------
+// hamstr_ctx is defined in hamstr_hip.hip; these helpers reach the state
+// through the C ABI only.
+extern "C" int hamstr_warm_start(hamstr_ctx* ctx, hamstr_mlp* M, const float* feat, const float* nrm,
+                                 int* fallback_cells) {
+  if (!ctx || !M || !feat || !nrm) return mfail(HAMSTR_ERR_ARG, "null argument");
+  int ncase = 0;
+  int rc = hamstr_batch_info(ctx, &ncase, nullptr);
+  if (rc) return rc;
+  std::vector<int> off(ncase + 1);
+  hamstr_batch_info(ctx, &ncase, off.data());
+  const int n = off[ncase];
+  const int B = (n + 31) / 32 * 32;
+  if ((rc = ensure_cap(M, B))) return rc;
+  float *f = nullptr, *nr = nullptr;
+  double* q = nullptr;
+  int* nfb = nullptr;
+  MCHK(hipMalloc(&f, sizeof(float) * (size_t)n * kIn));
+  MCHK(hipMalloc(&nr, sizeof(float) * (2 * kIn + 2 * kOut)));
+  MCHK(hipMalloc(&q, sizeof(double) * 4 * (size_t)n));
+  MCHK(hipMalloc(&nfb, sizeof(int)));
+  MCHK(hipMemcpyAsync(f, feat, sizeof(float) * (size_t)n * kIn, hipMemcpyHostToDevice, M->stream));
+  MCHK(hipMemcpyAsync(nr, nrm, sizeof(float) * (2 * kIn + 2 * kOut), hipMemcpyHostToDevice, M->stream));
+  MCHK(hipMemsetAsync(nfb, 0, sizeof(int), M->stream));
+  MCHK(hipMemsetAsync(M->x, 0, sizeof(float) * (size_t)B * kIn, M->stream));
+  hipLaunchKernelGGL(k_norm_features, dim3((n * kIn + 255) / 256), dim3(256), 0, M->stream, f, nr, n, M->x);
+  if ((rc = forward(M, B))) return rc;
+  // each case's freestream is its fallback
+  std::vector<double> qinf(4 * (size_t)ncase);
+  for (int k = 0; k < ncase; k++) {
+    double s[4];
+    rc = hamstr_case_freestream(ctx, k, s);
+    if (rc) return rc;
+    const double4 qi = make_double4(s[0], s[1], s[2], s[3]);
+    const int c0 = off[k], c1 = off[k + 1];
+    hipLaunchKernelGGL(k_to_state, dim3((c1 - c0 + 255) / 256), dim3(256), 0, M->stream, M->y + (size_t)c0 * kOut,
+                       nr, c1 - c0, qi, q + 4 * (size_t)c0, nfb);
+  }
+  MCHK(hipGetLastError());
+  std::vector<double> hq(4 * (size_t)n);
+  int hfb = 0;
+  MCHK(hipMemcpyAsync(hq.data(), q, sizeof(double) * hq.size(), hipMemcpyDeviceToHost, M->stream));
+  MCHK(hipMemcpyAsync(&hfb, nfb, sizeof(int), hipMemcpyDeviceToHost, M->stream));
+  MCHK(hipStreamSynchronize(M->stream));
+  (void)hipFree(f);
+  (void)hipFree(nr);
+  (void)hipFree(q);
+  (void)hipFree(nfb);
+  if (fallback_cells) *fallback_cells = hfb;
+  return hamstr_set_field(ctx, HAMSTR_FIELD_Q, hq.data());
+}

@@ -30,7 +30,10 @@ EXPORTS = [
     "hamstr_last_timings", "hamstr_last_timing_names", "hamstr_flux_roe2d", "hamstr_jacobian_roe2d", "hamstr_muscl", "hamstr_weno",
     "hamstr_muscl_deriv", "hamstr_weno_deriv", "hamstr_block_tridag4", "hamstr_block_tridag_periodic4",
     "hamstr_last_error", "hamstr_batch_create", "hamstr_batch_info", "hamstr_batch_step", "hamstr_autotune",
-    "hamstr_launch_groups",
+    "hamstr_launch_groups", "hamstr_case_freestream",
+    "hamstr_mlp_create", "hamstr_mlp_destroy", "hamstr_mlp_num_params", "hamstr_mlp_get_params",
+    "hamstr_mlp_set_params", "hamstr_mlp_forward", "hamstr_mlp_loss_grad", "hamstr_mlp_set_data",
+    "hamstr_mlp_train", "hamstr_warm_start", "hamstr_mlp_last_error",
 ]
 
 _dp = C.POINTER(C.c_double)

@@ -121,6 +121,9 @@ int hamstr_init_flow(hamstr_ctx *ctx);
 /* freestream (rinf, rinf*uinf, rinf*vinf, einf) as initFlow computes it */
 int hamstr_freestream(const hamstr_ctx *ctx, double *out4);
 
+/* freestream state of case k of a batch */
+int hamstr_case_freestream(const hamstr_ctx *ctx, int k, double *out4);
+
 int hamstr_set_field(hamstr_ctx *ctx, int field, const double *host);
 int hamstr_get_field(hamstr_ctx *ctx, int field, double *host);
 
@@ -209,6 +212,32 @@ int hamstr_block_tridag_periodic4(int nsys, int N, const double *a, const double
 
 /* Text of the last error on this thread. */
 const char *hamstr_last_error(void);
+
+/* ---- learned warm start ----------------------------------------------------
+ * An MLP 6 -> 256 x 4 (SiLU) -> 4 mapping normalised (Mach, alpha, thickness,
+ * centroid x, centroid y, wall distance) to normalised primitive
+ * (rho, u, v, p).  The 256x256 layers run on BF16 matrix cores with FP32
+ * accumulation (forward and backward); FP32 master weights, Adam. */
+typedef struct hamstr_mlp hamstr_mlp;
+int hamstr_mlp_create(unsigned seed, hamstr_mlp **out);
+int hamstr_mlp_destroy(hamstr_mlp *mlp);
+int hamstr_mlp_num_params(void);                       /* 200196 */
+int hamstr_mlp_get_params(hamstr_mlp *mlp, float *host);
+int hamstr_mlp_set_params(hamstr_mlp *mlp, const float *host);
+/* y[n*4] = MLP(x[n*6]) */
+int hamstr_mlp_forward(hamstr_mlp *mlp, int n, const float *x, float *y);
+/* mean-squared-error loss and gradient (n a multiple of 32) */
+int hamstr_mlp_loss_grad(hamstr_mlp *mlp, int n, const float *x, const float *t, float *loss, float *grad);
+/* device-resident training set, then steps of Adam on random batches */
+int hamstr_mlp_set_data(hamstr_mlp *mlp, long n, const float *x, const float *t);
+int hamstr_mlp_train(hamstr_mlp *mlp, int steps, int batch, float lr, unsigned seed, float *losses);
+/* q := conserved form of the MLP's prediction from raw per-cell features
+ * feat[nCell*6] and the normalisation nrm[20] (feature mean[6], std[6],
+ * target mean[4], std[4]); cells whose prediction is not a valid state take
+ * their case's freestream (*fallback_cells counts them). */
+int hamstr_warm_start(hamstr_ctx *ctx, hamstr_mlp *mlp, const float *feat, const float *nrm,
+                      int *fallback_cells);
+const char *hamstr_mlp_last_error(void);
 
 #ifdef __cplusplus
 }

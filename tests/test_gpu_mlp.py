@@ -1,0 +1,128 @@
+"""The warm-start MLP's matrix-core kernels against the FP64 CPU checker, its
+trainer, and what the trained network gains over the freestream cold start."""
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as mo
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ws():
+    from hamstr_oop_amd import warmstart
+    return warmstart
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def test_forward_matches_checker(ws):
+    mlp = ws.MLP(seed=7)
+    p = mlp.params()
+    rng = np.random.default_rng(0)
+    for n in (32, 100, 517):  # padded to 32 rows inside
+        x = rng.normal(size=(n, 6)).astype(np.float32)
+        y = mlp.forward(x)
+        ref = mo.forward(p, x, bf16_points=True)
+        err = np.abs(y - ref).max() / np.abs(ref).max()
+        print(f"forward n={n}: max err / max |y| = {err:.2e}", flush=True)
+        assert err < 2e-3
+        # without the BF16 rounding points the difference is the BF16 error itself
+        assert np.abs(y - mo.forward(p, x)).max() / np.abs(ref).max() < 5e-2
+    mlp.close()
+
+
+@pytest.mark.parametrize("n", [32, 256, 1024])
+def test_gradient_matches_checker(ws, n):
+    mlp = ws.MLP(seed=11)
+    rng = np.random.default_rng(n)
+    # a trained-looking scale: move the parameters off their initial values
+    p = mlp.params() * np.float32(1.3) + rng.normal(scale=0.01, size=mo.N_PARAMS).astype(np.float32)
+    mlp.set_params(p)
+    x = rng.normal(size=(n, 6)).astype(np.float32)
+    t = rng.normal(size=(n, 4)).astype(np.float32)
+    loss, g = mlp.loss_grad(x, t)
+    lref, gref = mo.loss_grad(p, x, t, bf16_points=True)
+    _, g64 = mo.loss_grad(p, x, t)
+    assert abs(loss - lref) <= 1e-3 * lref
+    for k, (o, m) in mo.OFF.items():
+        e = _rel(g[o:o + m], gref[o:o + m])
+        e64 = _rel(g[o:o + m], g64[o:o + m])
+        print(f"n={n} {k}: rel err vs BF16-point checker {e:.2e}, vs plain FP64 {e64:.2e}", flush=True)
+        assert e < 5e-3, k
+        assert e64 < 5e-2, k
+    mlp.close()
+
+
+def test_training_is_deterministic_and_reduces_loss(ws):
+    rng = np.random.default_rng(5)
+    x = rng.normal(size=(8192, 6)).astype(np.float32)
+    t = np.stack([np.sin(x[:, 0]) + 0.5 * x[:, 1], x[:, 2] * x[:, 3], np.tanh(x[:, 4] - x[:, 5]),
+                  0.3 * x[:, 0] ** 2 - 0.2], axis=1).astype(np.float32)
+    runs = []
+    for _ in range(2):
+        mlp = ws.MLP(seed=3)
+        mlp.set_data(x, t)
+        losses = mlp.train(400, batch=512, lr=1e-3, seed=9)
+        runs.append((losses, mlp.params()))
+        mlp.close()
+    losses = runs[0][0]
+    print("loss", losses[:5], losses[-5:], flush=True)
+    assert np.all(np.isfinite(losses))
+    assert losses[-20:].mean() < 0.1 * losses[:5].mean()
+    np.testing.assert_array_equal(runs[0][1], runs[1][1])
+
+
+def test_warm_start_fallback_is_freestream(ws):
+    from hamstr_oop_amd import hip, meshgen
+    cases = [ws.Case(meshgen.naca_ogrid(48, 24, t=0.1), 0.4, 1.0, 0.1),
+             ws.Case(meshgen.naca_ogrid(32, 16, t=0.14), 0.6, -2.0, 0.14)]
+    b = hip.Batch([(c.mesh, c.mach, c.alpha) for c in cases], order=3, cfl=10.0, lhs=hip.LHS_FP32)
+    feat = ws.batch_features(cases)
+    mlp = ws.MLP(seed=1)
+    # a normalisation that maps every prediction to negative pressure
+    nrm = ws.Normalizer(np.zeros(6, np.float32), np.ones(6, np.float32),
+                        np.array([1, 0, 0, -5], np.float32), np.array([1e-6, 1e-6, 1e-6, 1e-6], np.float32))
+    nfb = ws.warm_start(b, mlp, feat, nrm)
+    assert nfb == b.n_cell
+    for k, c in enumerate(cases):
+        qk = b.case_q(k)
+        r, u, v, e = meshgen.freestream(c.mach, c.alpha)
+        np.testing.assert_array_equal(qk, np.tile([r, r * u, r * v, e], (qk.shape[0], 1)))
+    mlp.close()
+
+
+@pytest.mark.slow
+def test_trained_warm_start_cuts_iterations(ws):
+    """Train on converged fields of 8 cases, warm-start 3 held-out cases: every
+    case still reaches the residual tolerance under the plain ADI iteration, at
+    the same fixed point, in fewer iterations than from the freestream."""
+    from hamstr_oop_amd import hip
+    rng = np.random.default_rng(20261203)
+    tol, order, cfl = 1e-10, 3, 10.0
+    train = ws.draw_cases(8, rng, sizes=((64, 32),))
+    held = ws.draw_cases(3, rng, sizes=((64, 32),))
+    b, iters, _, _ = ws.solve_batch(train, order, cfl, tol, 6000, lhs=hip.LHS_FP64)
+    assert np.all(iters > 0), iters
+    x, t = ws.dataset(train, b, 10 ** 6, rng)
+    norm = ws.Normalizer.fit(x, t)
+    mlp = ws.MLP(seed=2)
+    mlp.set_data(norm.nx(x), norm.nt(t))
+    losses = mlp.train(3000, batch=1024, lr=1e-3, seed=4)
+    print("training loss", losses[:3], losses[-3:], flush=True)
+    assert losses[-50:].mean() < 0.05 * losses[:5].mean()
+
+    cold, cit, _, _ = ws.solve_batch(held, order, cfl, tol, 6000, lhs=hip.LHS_FP64)
+    feat = ws.batch_features(held)
+    nfb = []
+    warm, wit, _, _ = ws.solve_batch(held, order, cfl, tol, 6000, lhs=hip.LHS_FP64,
+                                     init=lambda bb: nfb.append(ws.warm_start(bb, mlp, feat, norm)))
+    print("cold iterations", cit, "warm iterations", wit, "fallback cells", nfb, flush=True)
+    assert np.all(cit > 0) and np.all(wit > 0)
+    assert wit.sum() < cit.sum()
+    for k in range(len(held)):
+        d = np.abs(warm.case_q(k) - cold.case_q(k)).max()
+        assert d < 1e-5, (k, d)
+    mlp.close()
