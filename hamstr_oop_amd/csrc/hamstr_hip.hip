@@ -51,6 +51,7 @@ int fail(int code, const std::string& msg) {
 
 constexpr int kBlock = 256;
 constexpr int kRedBlock = 256;
+constexpr int kGraphMaxIters = 64;  // iterations per captured graph
 
 inline int nblk(long n, int b = kBlock) { return (int)((n + b - 1) / b); }
 
@@ -68,6 +69,14 @@ struct FaceTab {
   const double* nvec;    // [nProc][2]
   const double* refmtx;  // [nFace][4]
   const double4* qinf;   // [nCase] freestream state per flow case
+};
+
+// Retired cases of a batch: one byte per processed face, cell and chain, zero
+// where the case no longer iterates.  Null pointers: every case is live.
+struct Live {
+  const unsigned char* face;
+  const unsigned char* cell;
+  const unsigned char* chain;
 };
 
 struct ChainTab {
@@ -160,11 +169,11 @@ __device__ __forceinline__ void reconstruct_dface(const double (*f)[4], const do
 // computeRHS.cpp:53-408
 // --------------------------------------------------------------------------
 template <int ORDER, typename TL, bool IMPL>
-__global__ __launch_bounds__(kBlock) void k_rhs_faces(FaceTab ft, Consts c, const double* __restrict__ q,
+__global__ __launch_bounds__(kBlock) void k_rhs_faces(FaceTab ft, Consts c, Live lv, const double* __restrict__ q,
                                                       double* __restrict__ flux, double* __restrict__ sr,
                                                       TL* __restrict__ ff, int* __restrict__ nanflag) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= c.nProc) return;
+  if (p >= c.nProc || (lv.face && !lv.face[p])) return;
   const int code = ft.pslot[p];
   const int s = code & 0x3fffffff;
   const bool flip = (code >> 30) & 1;
@@ -255,7 +264,7 @@ __global__ __launch_bounds__(kRedBlock) void k_norm_final(const double* __restri
 // fused iteration) ADI's scaling r *= CFL/sigma, r0 = r (ADI.cpp:40-49)
 // --------------------------------------------------------------------------
 template <bool ADI_INIT>
-__global__ __launch_bounds__(kBlock) void k_rhs_cells(Consts c, const int* __restrict__ off,
+__global__ __launch_bounds__(kBlock) void k_rhs_cells(Consts c, Live lv, const int* __restrict__ off,
                                                       const int* __restrict__ ent,
                                                       const double* __restrict__ flux,
                                                       const double* __restrict__ sr, double* __restrict__ r,
@@ -263,7 +272,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_cells(Consts c, const int* __res
                                                       double* __restrict__ partial, double* __restrict__ rrho) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double sq = 0.0, mx = 0.0;
-  if (i < c.nCell) {
+  if (i < c.nCell && (!lv.cell || lv.cell[i])) {
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, sg = 0.0;
     const int e1 = off[i + 1];
     for (int e = off[i]; e < e1; e++) {
@@ -294,10 +303,10 @@ __global__ __launch_bounds__(kBlock) void k_rhs_cells(Consts c, const int* __res
 }
 
 // ADI.cpp:40-49 on its own (API path)
-__global__ void k_adi_init(Consts c, double* __restrict__ r, double* __restrict__ r0, double* __restrict__ dq,
-                           const double* __restrict__ sigma) {
+__global__ void k_adi_init(Consts c, Live lv, double* __restrict__ r, double* __restrict__ r0,
+                           double* __restrict__ dq, const double* __restrict__ sigma) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= c.nCell) return;
+  if (i >= c.nCell || (lv.cell && !lv.cell[i])) return;
   const double dtfac = c.cfl / sigma[i];
   double4 v = *reinterpret_cast<double4*>(r + 4 * (size_t)i);
   v.x *= dtfac; v.y *= dtfac; v.z *= dtfac; v.w *= dtfac;
@@ -308,9 +317,9 @@ __global__ void k_adi_init(Consts c, double* __restrict__ r, double* __restrict_
 
 // dq += r (ADI.cpp:247-248); with FIRST the old dq is zero
 template <bool FIRST>
-__global__ void k_accum_dq(int n4, const double* __restrict__ r, double* __restrict__ dq) {
+__global__ void k_accum_dq(int n4, Live lv, const double* __restrict__ r, double* __restrict__ dq) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n4) return;
+  if (i >= n4 || (lv.cell && !lv.cell[i >> 2])) return;
   if (FIRST)
     dq[i] = 0.0 + r[i];
   else
@@ -440,7 +449,7 @@ __device__ __forceinline__ void store_solution(double* __restrict__ r, double* _
 // scratch[(i*36+e)*stride + t].
 // --------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(64) void k_adi_exact(ChainTab ct, const int* __restrict__ chains, int nch,
+__global__ __launch_bounds__(64) void k_adi_exact(ChainTab ct, Live lv, const int* __restrict__ chains, int nch,
                                                   const T* __restrict__ ff, int nProc, double cfl,
                                                   const double* __restrict__ sigma, double* __restrict__ r,
                                                   T* __restrict__ scratch, int stride, double* __restrict__ dq,
@@ -448,6 +457,7 @@ __global__ __launch_bounds__(64) void k_adi_exact(ChainTab ct, const int* __rest
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nch) return;
   const int ch = chains[t];
+  if (lv.chain && !lv.chain[ch]) return;
   const int pb = ct.pbase[ch], nf = ct.nf[ch], N = ct.ncell[ch];
   const bool periodic = ct.periodic[ch] != 0;
   auto S = [&](int i, int e) -> T& { return scratch[((size_t)i * 36 + e) * stride + t]; };
@@ -911,12 +921,14 @@ __device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const 
 // stride of the grid size (a persistent launch: chains are sorted longest
 // first, so every pass of the grid does similar work)
 template <typename T, int FINAL, int MAXT>
-__global__ __launch_bounds__(MAXT) void k_adi_fast(ChainTab ct, const int* __restrict__ chains, int nch,
+__global__ __launch_bounds__(MAXT) void k_adi_fast(ChainTab ct, Live lv, const int* __restrict__ chains, int nch,
                                                    const T* __restrict__ ff, int nProc, double cfl,
                                                    const double* __restrict__ sigma, double* __restrict__ r,
                                                    double* __restrict__ dq, int dqmode) {
   for (int w = blockIdx.x; w < nch; w += gridDim.x) {
-    adi_fast_chain<T, FINAL>(ct, chains[w], ff, nProc, cfl, sigma, r, dq, dqmode);
+    const int ch = chains[w];
+    if (lv.chain && !lv.chain[ch]) continue;  // the same for the whole workgroup
+    adi_fast_chain<T, FINAL>(ct, ch, ff, nProc, cfl, sigma, r, dq, dqmode);
     __syncthreads();
   }
 }
@@ -926,11 +938,11 @@ __global__ __launch_bounds__(MAXT) void k_adi_fast(ChainTab ct, const int* __res
 // (computeLinearRHS.cpp:52-325)
 // --------------------------------------------------------------------------
 template <int ORDER, typename TL>
-__global__ __launch_bounds__(kBlock) void k_lin_faces(FaceTab ft, Consts c, const double* __restrict__ q,
+__global__ __launch_bounds__(kBlock) void k_lin_faces(FaceTab ft, Consts c, Live lv, const double* __restrict__ q,
                                                       const double* __restrict__ dq, const TL* __restrict__ ff,
                                                       double* __restrict__ lflux) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= c.nProc) return;
+  if (p >= c.nProc || (lv.face && !lv.face[p])) return;
   const int code = ft.pslot[p];
   const int s = code & 0x3fffffff;
   const bool flip = (code >> 30) & 1;
@@ -969,7 +981,7 @@ __global__ __launch_bounds__(kBlock) void k_lin_faces(FaceTab ft, Consts c, cons
 // lin_cells: r = r0 - dq - CFL/sigma * sum(+-lflux), norms, optional q += dq
 // (computeLinearRHS.cpp:45, 326-368; ADI.cpp:258-264)
 template <bool UPDATE>
-__global__ __launch_bounds__(kBlock) void k_lin_cells(Consts c, const int* __restrict__ off,
+__global__ __launch_bounds__(kBlock) void k_lin_cells(Consts c, Live lv, const int* __restrict__ off,
                                                       const int* __restrict__ ent,
                                                       const double* __restrict__ lflux,
                                                       const double* __restrict__ r0, const double* __restrict__ dq,
@@ -978,7 +990,7 @@ __global__ __launch_bounds__(kBlock) void k_lin_cells(Consts c, const int* __res
                                                       double* __restrict__ rrho) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double sq = 0.0, mx = 0.0;
-  if (i < c.nCell) {
+  if (i < c.nCell && (!lv.cell || lv.cell[i])) {
     const double4 a = *reinterpret_cast<const double4*>(r0 + 4 * (size_t)i);
     const double4 b = *reinterpret_cast<const double4*>(dq + 4 * (size_t)i);
     double a0 = (a.x - b.x), a1 = (a.y - b.y), a2 = (a.z - b.z), a3 = (a.w - b.w);
@@ -1007,9 +1019,14 @@ __global__ __launch_bounds__(kBlock) void k_lin_cells(Consts c, const int* __res
   block_norm_partial(sq, mx, partial);
 }
 
-__global__ void k_update(int n4, double* __restrict__ q, const double* __restrict__ dq) {
+__global__ void k_update(int n4, Live lv, double* __restrict__ q, const double* __restrict__ dq) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n4) q[i] += dq[i];
+  if (i < n4 && (!lv.cell || lv.cell[i >> 2])) q[i] += dq[i];
+}
+
+__global__ void k_fill_u8(unsigned char* __restrict__ p, long n, unsigned char v) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
 }
 
 // initFlow (initFlow.cpp:152-176): q := freestream of the cell's case
@@ -1305,7 +1322,27 @@ struct hamstr_ctx {
   std::vector<hipEvent_t> ev;
   std::vector<double> last_ms;
   std::vector<std::string> stage_names;
+  // retirement of converged or stalled cases (batches)
+  unsigned char *face_on = nullptr, *cell_on = nullptr, *chain_on = nullptr;
+  bool any_retired = false;
+  bool retire = false;
+  // graph of K iterations (launch shapes and live masks are baked in)
+  bool use_graph = true;
+  hipGraphExec_t gexec = nullptr;
+  int gK = 0;
+  double* ghist = nullptr;
+  int ghistCap = 0;
+  int stall_window = 0;
+  long it_total = 0;                   // iterations run since the last hamstr_set_retire
+  std::vector<int> case_status;        // 0 live, 1 converged, 2 stalled
+  std::vector<long> case_iters;        // iteration of convergence / stall detection
+  std::vector<double> stall_ref;       // best norm that started the current window
+  std::vector<long> stall_ref_it;
+  std::vector<int> case_proc_off, case_chain_off;  // [nCase+1] processed-face and chain ranges
 
+  Live live() const {
+    return (any_retired || retire) ? Live{face_on, cell_on, chain_on} : Live{nullptr, nullptr, nullptr};
+  }
   FaceTab ftab() const { return FaceTab{pslot, src, nvec, refmtx, qinf}; }
   ChainTab ctab() const {
     return ChainTab{ch_pbase, ch_nf, ch_ncell, ch_periodic, pleft, pright, pface, refmtx};
@@ -1319,7 +1356,7 @@ int launch_rhs_faces(hamstr_ctx* x, bool implicit) {
   const FaceTab ft = x->ftab();
   dim3 g(nblk(x->nProc)), b(kBlock);
 #define RHSF(O, T, I) \
-  hipLaunchKernelGGL((k_rhs_faces<O, T, I>), g, b, 0, x->stream, ft, x->c, x->q, x->flux, x->sr, (T*)x->ff, x->nanflag)
+  hipLaunchKernelGGL((k_rhs_faces<O, T, I>), g, b, 0, x->stream, ft, x->c, x->live(), x->q, x->flux, x->sr, (T*)x->ff, x->nanflag)
   const int o = x->prm.order;
   if (x->lhs32()) {
     if (implicit) {
@@ -1343,7 +1380,7 @@ int launch_lin_faces(hamstr_ctx* x) {
   const FaceTab ft = x->ftab();
   dim3 g(nblk(x->nProc)), b(kBlock);
 #define LINF(O, T) \
-  hipLaunchKernelGGL((k_lin_faces<O, T>), g, b, 0, x->stream, ft, x->c, x->q, x->dq, (const T*)x->ff, x->lflux)
+  hipLaunchKernelGGL((k_lin_faces<O, T>), g, b, 0, x->stream, ft, x->c, x->live(), x->q, x->dq, (const T*)x->ff, x->lflux)
   const int o = x->prm.order;
   if (x->lhs32()) {
     if (o == 1) LINF(1, float); else if (o == 3) LINF(3, float); else LINF(5, float);
@@ -1368,10 +1405,10 @@ int launch_norm(hamstr_ctx* x, double* base, int slot) {
 
 int launch_rhs_cells(hamstr_ctx* x, bool adi_init, double* normbase) {
   if (adi_init)
-    hipLaunchKernelGGL((k_rhs_cells<true>), dim3(nblk(x->nCell)), dim3(kBlock), 0, x->stream, x->c, x->cell_off,
+    hipLaunchKernelGGL((k_rhs_cells<true>), dim3(nblk(x->nCell)), dim3(kBlock), 0, x->stream, x->c, x->live(), x->cell_off,
                        x->cell_ent, x->flux, x->sr, x->r, x->r0, x->sigma, x->partial, x->rrho);
   else
-    hipLaunchKernelGGL((k_rhs_cells<false>), dim3(nblk(x->nCell)), dim3(kBlock), 0, x->stream, x->c,
+    hipLaunchKernelGGL((k_rhs_cells<false>), dim3(nblk(x->nCell)), dim3(kBlock), 0, x->stream, x->c, x->live(),
                        x->cell_off, x->cell_ent, x->flux, x->sr, x->r, x->r0, x->sigma, x->partial, x->rrho);
   HIPCHK(hipGetLastError());
   return launch_norm(x, normbase, 0);
@@ -1384,7 +1421,7 @@ int launch_group(hamstr_ctx* x, const Group& g, int grid, int dqmode) {
     const dim3 gr(grid), b(g.block);
     const size_t lds = (x->lhs32() ? sizeof(float) : sizeof(double)) * 36 * (size_t)g.maxN;
 #define FASTL(T, F, M)                                                                                          \
-  hipLaunchKernelGGL((k_adi_fast<T, F, M>), gr, b, lds, x->stream, ct, chains, g.n, (const T*)x->ff, x->nProc, \
+  hipLaunchKernelGGL((k_adi_fast<T, F, M>), gr, b, lds, x->stream, ct, x->live(), chains, g.n, (const T*)x->ff, x->nProc, \
                      x->c.cfl, x->sigma, x->r, x->dq, dqmode)
     if (x->lhs32()) {
       if (g.final == kFinalOpen) FASTL(float, kFinalOpen, 1024);
@@ -1397,11 +1434,11 @@ int launch_group(hamstr_ctx* x, const Group& g, int grid, int dqmode) {
     }
 #undef FASTL
   } else if (x->lhs32()) {
-    hipLaunchKernelGGL((k_adi_exact<float>), dim3(nblk(g.n, 64)), dim3(64), 0, x->stream, ct, chains, g.n,
+    hipLaunchKernelGGL((k_adi_exact<float>), dim3(nblk(g.n, 64)), dim3(64), 0, x->stream, ct, x->live(), chains, g.n,
                        (const float*)x->ff, x->nProc, x->c.cfl, x->sigma, x->r, (float*)x->scratch,
                        (int)x->scratch_stride, x->dq, dqmode);
   } else {
-    hipLaunchKernelGGL((k_adi_exact<double>), dim3(nblk(g.n, 64)), dim3(64), 0, x->stream, ct, chains, g.n,
+    hipLaunchKernelGGL((k_adi_exact<double>), dim3(nblk(g.n, 64)), dim3(64), 0, x->stream, ct, x->live(), chains, g.n,
                        (const double*)x->ff, x->nProc, x->c.cfl, x->sigma, x->r, (double*)x->scratch,
                        (int)x->scratch_stride, x->dq, dqmode);
   }
@@ -1425,9 +1462,9 @@ int launch_sweep_phases(hamstr_ctx* x, int sweep, const std::function<int(const 
   if (!x->last_phase_all_cells) {
     const int n4 = 4 * x->nCell;
     if (sweep == 0)
-      hipLaunchKernelGGL((k_accum_dq<true>), dim3(nblk(n4)), dim3(kBlock), 0, x->stream, n4, x->r, x->dq);
+      hipLaunchKernelGGL((k_accum_dq<true>), dim3(nblk(n4)), dim3(kBlock), 0, x->stream, n4, x->live(), x->r, x->dq);
     else
-      hipLaunchKernelGGL((k_accum_dq<false>), dim3(nblk(n4)), dim3(kBlock), 0, x->stream, n4, x->r, x->dq);
+      hipLaunchKernelGGL((k_accum_dq<false>), dim3(nblk(n4)), dim3(kBlock), 0, x->stream, n4, x->live(), x->r, x->dq);
     HIPCHK(hipGetLastError());
     if (mark && mark("accum_dq")) return HAMSTR_ERR_HIP;
   }
@@ -1437,10 +1474,10 @@ int launch_sweep_phases(hamstr_ctx* x, int sweep, const std::function<int(const 
 int launch_lin_cells(hamstr_ctx* x, bool update, double* normbase) {
   dim3 g(nblk(x->nCell)), b(kBlock);
   if (update)
-    hipLaunchKernelGGL((k_lin_cells<true>), g, b, 0, x->stream, x->c, x->cell_off, x->cell_ent, x->lflux, x->r0,
+    hipLaunchKernelGGL((k_lin_cells<true>), g, b, 0, x->stream, x->c, x->live(), x->cell_off, x->cell_ent, x->lflux, x->r0,
                        x->dq, x->sigma, x->r, x->q, x->partial, x->rrho);
   else
-    hipLaunchKernelGGL((k_lin_cells<false>), g, b, 0, x->stream, x->c, x->cell_off, x->cell_ent, x->lflux,
+    hipLaunchKernelGGL((k_lin_cells<false>), g, b, 0, x->stream, x->c, x->live(), x->cell_off, x->cell_ent, x->lflux,
                        x->r0, x->dq, x->sigma, x->r, x->q, x->partial, x->rrho);
   HIPCHK(hipGetLastError());
   return launch_norm(x, normbase, 2);
@@ -1482,7 +1519,9 @@ void free_ctx(hamstr_ctx* x) {
   void* ptrs[] = {x->q, x->r, x->r0, x->dq, x->sigma, x->flux, x->sr, x->lflux, x->ff, x->pslot, x->src,
                   x->nvec, x->refmtx, x->qinf, x->cell_case, x->case_off, x->pleft, x->pright, x->pface,
                   x->cell_off, x->cell_ent, x->ch_pbase, x->ch_nf, x->ch_ncell, x->ch_periodic, x->group_chains,
-                  x->scratch, x->partial, x->norms, x->hist, x->nanflag, x->rrho};
+                  x->scratch, x->partial, x->norms, x->hist, x->nanflag, x->rrho, x->face_on, x->cell_on,
+                  x->chain_on, x->ghist};
+  if (x->gexec) (void)hipGraphExecDestroy(x->gexec);
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (auto e : x->ev) (void)hipEventDestroy(e);
@@ -1726,6 +1765,19 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
   }
   x->qinf_h.assign(4 * (size_t)nCase, 0.0);
   for (int k = 0; k < nCase; k++) freestream_state(cases[k].mach, cases[k].alpha, &x->qinf_h[4 * k]);
+  // chains, and so processed faces, are stored case by case
+  x->case_chain_off.assign(nCase + 1, nChain);
+  x->case_proc_off.assign(nCase + 1, nProc);
+  for (int k = 0; k < nCase; k++) {
+    x->case_chain_off[k] = cases[k].chainBase;
+    x->case_proc_off[k] = cases[k].chainCount > 0 ? ch_pbase[cases[k].chainBase] : nProc;
+  }
+  for (int k = nCase - 1; k >= 0; k--)
+    if (cases[k].chainCount == 0) x->case_proc_off[k] = x->case_proc_off[k + 1];
+  x->case_status.assign(nCase, 0);
+  x->case_iters.assign(nCase, -1);
+  x->stall_ref.assign(nCase, HUGE_VAL);
+  x->stall_ref_it.assign(nCase, 0);
   std::vector<double4> qinf4(nCase);
   for (int k = 0; k < nCase; k++)
     qinf4[k] = make_double4(x->qinf_h[4 * k], x->qinf_h[4 * k + 1], x->qinf_h[4 * k + 2], x->qinf_h[4 * k + 3]);
@@ -1760,6 +1812,9 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
   chk(hipMalloc(&x->partial, sizeof(double) * 2 * x->nPart), "alloc partial");
   chk(hipMalloc(&x->norms, sizeof(double) * 4 * nCase), "alloc norms");
   if (nCase > 1) chk(hipMalloc(&x->rrho, sizeof(double) * nCell), "alloc rrho");
+  chk(hipMalloc(&x->face_on, nProc), "alloc face mask");
+  chk(hipMalloc(&x->cell_on, nCell), "alloc cell mask");
+  chk(hipMalloc(&x->chain_on, nChain), "alloc chain mask");
   chk(hipMalloc(&x->nanflag, sizeof(int)), "alloc nanflag");
   if (rc) {
     free_ctx(x);
@@ -1785,6 +1840,9 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
   chk(hipMemset(x->r0, 0, c4), "memset r0");
   chk(hipMemset(x->sigma, 0, sizeof(double) * nCell), "memset sigma");
   chk(hipMemset(x->ff, 0, tl * 32 * nProc), "memset ff");
+  chk(hipMemset(x->face_on, 1, nProc), "memset face mask");
+  chk(hipMemset(x->cell_on, 1, nCell), "memset cell mask");
+  chk(hipMemset(x->chain_on, 1, nChain), "memset chain mask");
   if (rc) {
     free_ctx(x);
     delete x;
@@ -2005,7 +2063,7 @@ int hamstr_compute_rhs(hamstr_ctx* x, int implicit, double* L2, double* Linf) {
 
 int hamstr_adi(hamstr_ctx* x, double* L2lin, double* Linflin) {
   if (!x) return fail(HAMSTR_ERR_ARG, "null context");
-  hipLaunchKernelGGL(k_adi_init, dim3(nblk(x->nCell)), dim3(kBlock), 0, x->stream, x->c, x->r, x->r0, x->dq,
+  hipLaunchKernelGGL(k_adi_init, dim3(nblk(x->nCell)), dim3(kBlock), 0, x->stream, x->c, x->live(), x->r, x->r0, x->dq,
                      x->sigma);
   HIPCHK(hipGetLastError());
   int rc = launch_sweeps(x, x->norms);
@@ -2031,6 +2089,30 @@ int hamstr_compute_linear_rhs(hamstr_ctx* x, double* L2lin, double* Linflin) {
 namespace {
 
 // one fused iteration (march.cpp:112-117); norms to nb[nCase][4]
+// Stops case k from iterating: its faces, cells and chains drop out of every
+// launch (a fill kernel on the context's stream, ordered with the iterations).
+int retire_case(hamstr_ctx* x, int k, int status, long it) {
+  x->case_status[k] = status;
+  x->case_iters[k] = it;
+  auto fill = [&](unsigned char* p, int a, int b) {
+    if (b > a)
+      hipLaunchKernelGGL(k_fill_u8, dim3(nblk(b - a)), dim3(kBlock), 0, x->stream, p + a, (long)(b - a),
+                         (unsigned char)0);
+  };
+  fill(x->face_on, x->case_proc_off[k], x->case_proc_off[k + 1]);
+  fill(x->cell_on, x->case_off_h[k], x->case_off_h[k + 1]);
+  fill(x->chain_on, x->case_chain_off[k], x->case_chain_off[k + 1]);
+  HIPCHK(hipGetLastError());
+  x->any_retired = true;
+  return HAMSTR_OK;
+}
+
+void drop_graph(hamstr_ctx* x) {
+  if (x->gexec) (void)hipGraphExecDestroy(x->gexec);
+  x->gexec = nullptr;
+  x->gK = 0;
+}
+
 int enqueue_iteration(hamstr_ctx* x, double* nb, const std::function<int(const char*)>& mark) {
   int rc;
   if ((rc = launch_rhs_faces(x, true))) return rc;
@@ -2044,6 +2126,37 @@ int enqueue_iteration(hamstr_ctx* x, double* nb, const std::function<int(const c
     if ((rc = launch_lin_cells(x, sw == x->prm.msweep - 1, nb))) return rc;
     if (mark && mark("lin_cells")) return HAMSTR_ERR_HIP;
   }
+  return HAMSTR_OK;
+}
+
+// Captures K iterations into a graph whose norms go to ghist[K][nCase][4].
+int build_graph(hamstr_ctx* x, int K) {
+  drop_graph(x);
+  const int nc = x->nCase;
+  if (x->ghistCap < K) {
+    if (x->ghist) (void)hipFree(x->ghist);
+    x->ghist = nullptr;
+    x->ghistCap = 0;
+    HIPCHK(hipMalloc(&x->ghist, sizeof(double) * 4 * nc * K));
+    x->ghistCap = K;
+  }
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal));
+  int rc = HAMSTR_OK;
+  for (int n = 0; n < K && rc == HAMSTR_OK; n++) rc = enqueue_iteration(x, x->ghist + 4 * (size_t)nc * n, nullptr);
+  const hipError_t e = hipStreamEndCapture(x->stream, &g);
+  if (rc) {
+    if (e == hipSuccess && g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  HIPCHK(e);
+  const hipError_t ei = hipGraphInstantiate(&x->gexec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess) {
+    x->gexec = nullptr;
+    return fail(HAMSTR_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+  }
+  x->gK = K;
   return HAMSTR_OK;
 }
 
@@ -2083,13 +2196,31 @@ int run_steps(hamstr_ctx* x, int nsteps, double tol, int check_every, std::vecto
   };
   std::function<int(const char*)> mk;
   if (x->timing) mk = mark;
+  // blocks of K iterations between read-backs replay a captured graph
+  int K = 0;
+  if (x->use_graph && !x->timing && nsteps >= 2) {
+    for (int d = std::min(check_every, kGraphMaxIters); d >= 1; d--)
+      if (check_every % d == 0) {
+        K = d;
+        break;
+      }
+    if (K < 2 || nsteps < K) K = 0;
+    if (K && (!x->gexec || x->gK != K) && (rc = build_graph(x, K))) return rc;
+  }
   int ran = 0, lastRead = 0;
-  for (int n = 0; n < nsteps; n++) {
+  for (int n = 0; n < nsteps; n = ran) {
     cur = n;
-    if (x->timing && mark("start")) return fail(HAMSTR_ERR_HIP, "hipEventRecord");
-    if ((rc = enqueue_iteration(x, x->hist + 4 * (size_t)nc * n, mk))) return rc;
-    if (n == 0) nstage = (int)evs.size() - 1;
-    ran = n + 1;
+    if (K && nsteps - n >= K && (n % check_every) + K <= check_every) {
+      HIPCHK(hipGraphLaunch(x->gexec, x->stream));
+      HIPCHK(hipMemcpyAsync(x->hist + 4 * (size_t)nc * n, x->ghist, sizeof(double) * 4 * nc * K,
+                            hipMemcpyDeviceToDevice, x->stream));
+      ran = n + K;
+    } else {
+      if (x->timing && mark("start")) return fail(HAMSTR_ERR_HIP, "hipEventRecord");
+      if ((rc = enqueue_iteration(x, x->hist + 4 * (size_t)nc * n, mk))) return rc;
+      if (n == 0) nstage = (int)evs.size() - 1;
+      ran = n + 1;
+    }
     if (ran % check_every == 0 || ran == nsteps) {
       const size_t cnt = 4 * (size_t)nc * (ran - lastRead);
       HIPCHK(hipMemcpyAsync(series.data() + 4 * (size_t)nc * lastRead, x->hist + 4 * (size_t)nc * lastRead,
@@ -2113,6 +2244,30 @@ int run_steps(hamstr_ctx* x, int nsteps, double tol, int check_every, std::vecto
           if (series[4 * ((size_t)nc * k + j)] <= tol) iters[j] = k + 1;
         if (iters[j] < 0) all = false;
       }
+      if (x->retire) {
+        // retire cases that converged; a case whose norm has not halved within
+        // stall_window iterations is stalled (a limit cycle) and retires too
+        all = true;
+        for (int j = 0; j < nc; j++) {
+          if (x->case_status[j] != 0) continue;
+          if (tol > 0 && iters[j] > 0) {
+            if ((rc = retire_case(x, j, 1, x->it_total + iters[j]))) return rc;
+            continue;
+          }
+          for (int k = lastRead; k < ran; k++) {
+            const double v = series[4 * ((size_t)nc * k + j)];
+            if (v < 0.5 * x->stall_ref[j]) {
+              x->stall_ref[j] = v;
+              x->stall_ref_it[j] = x->it_total + k + 1;
+            }
+          }
+          if (x->stall_window > 0 && x->it_total + ran - x->stall_ref_it[j] > x->stall_window) {
+            if ((rc = retire_case(x, j, 2, x->it_total + ran))) return rc;
+            continue;
+          }
+          all = false;
+        }
+      }
       lastRead = ran;
       if (all) break;
     }
@@ -2130,6 +2285,7 @@ int run_steps(hamstr_ctx* x, int nsteps, double tol, int check_every, std::vecto
       }
     x->stage_names.erase(x->stage_names.begin());  // "start"
   }
+  x->it_total += ran;
   if (done) *done = ran;
   return rc;
 }
@@ -2166,6 +2322,46 @@ int hamstr_batch_step(hamstr_ctx* x, int nsteps, double tol, int check_every, in
   return rc;
 }
 
+int hamstr_set_graph(hamstr_ctx* x, int on) {
+  if (!x) return fail(HAMSTR_ERR_ARG, "null context");
+  x->use_graph = on != 0;
+  drop_graph(x);
+  return HAMSTR_OK;
+}
+
+int hamstr_set_retire(hamstr_ctx* x, int on, int stall_window) {
+  if (!x || stall_window < 0) return fail(HAMSTR_ERR_ARG, "bad arguments");
+  drop_graph(x);  // the live masks enter the launches only while retirement is on
+  x->retire = on != 0;
+  x->stall_window = stall_window;
+  x->it_total = 0;
+  x->case_status.assign(x->nCase, 0);
+  x->case_iters.assign(x->nCase, -1);
+  x->stall_ref.assign(x->nCase, HUGE_VAL);
+  x->stall_ref_it.assign(x->nCase, 0);
+  if (x->any_retired) {
+    auto fill = [&](unsigned char* p, long n) {
+      hipLaunchKernelGGL(k_fill_u8, dim3(nblk(n)), dim3(kBlock), 0, x->stream, p, n, (unsigned char)1);
+    };
+    fill(x->face_on, x->nProc);
+    fill(x->cell_on, x->nCell);
+    fill(x->chain_on, x->nChain);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(x->stream));
+    x->any_retired = false;
+  }
+  return HAMSTR_OK;
+}
+
+int hamstr_case_status(const hamstr_ctx* x, int* status, long* iters) {
+  if (!x) return fail(HAMSTR_ERR_ARG, "null context");
+  for (int k = 0; k < x->nCase; k++) {
+    if (status) status[k] = x->case_status[k];
+    if (iters) iters[k] = x->case_iters[k];
+  }
+  return HAMSTR_OK;
+}
+
 int hamstr_last_norms(const hamstr_ctx* x, double* out4) {
   if (!x || !out4) return fail(HAMSTR_ERR_ARG, "null argument");
   out4[0] = x->L2rhs;
@@ -2180,6 +2376,7 @@ int hamstr_last_norms(const hamstr_ctx* x, double* out4) {
 int hamstr_autotune(hamstr_ctx* x, int reps, double* best_ms, int cap, int* ngroups) {
   if (!x) return fail(HAMSTR_ERR_ARG, "null context");
   if (reps < 1) reps = 3;
+  drop_graph(x);  // launch shapes change
   hipDeviceProp_t prop;
   HIPCHK(hipGetDeviceProperties(&prop, x->device));
   const int cu = std::max(prop.multiProcessorCount, 1);

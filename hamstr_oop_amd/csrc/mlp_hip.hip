@@ -276,9 +276,13 @@ __global__ void k_to_state(const float* __restrict__ y, const float* __restrict_
                            const double4 qinf, double* __restrict__ q, int* __restrict__ nfallback) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  // the network predicts the deviation from the case's freestream primitive state
+  const double pinf[kOut] = {qinf.x, qinf.y / qinf.x, qinf.z / qinf.x,
+                             0.4 * (qinf.w - 0.5 * (qinf.y * qinf.y + qinf.z * qinf.z) / qinf.x)};
   double pr[kOut];
 #pragma unroll
-  for (int k = 0; k < kOut; k++) pr[k] = (double)y[i * kOut + k] * nrm[2 * kIn + kOut + k] + nrm[2 * kIn + k];
+  for (int k = 0; k < kOut; k++)
+    pr[k] = pinf[k] + (double)y[i * kOut + k] * nrm[2 * kIn + kOut + k] + nrm[2 * kIn + k];
   const double rho = pr[0], u = pr[1], v = pr[2], p = pr[3];
   double4 s;
   const bool ok = rho > 1e-3 && p > 1e-3 && rho == rho && u == u && v == v && p == p && fabs(u) < 10 && fabs(v) < 10;
@@ -526,26 +530,36 @@ int hamstr_mlp_set_data(hamstr_mlp* M, long n, const float* x, const float* t) {
   return HAMSTR_OK;
 }
 
-// Adam on random batches of the training set; losses[steps] (may be NULL)
-int hamstr_mlp_train(hamstr_mlp* M, int steps, int batch, float lr, unsigned seed, float* losses) {
-  if (!M || steps < 0 || batch <= 0 || batch % 32 || M->n <= 0) return mfail(HAMSTR_ERR_ARG, "bad arguments");
+// Adam on random batches of the training set, learning rate decaying from lr
+// to lr_final on a cosine; the batch losses stay on the device until the end.
+int hamstr_mlp_train(hamstr_mlp* M, int steps, int batch, float lr, float lr_final, unsigned seed, float* losses) {
+  if (!M || steps < 0 || batch <= 0 || batch % 32 || M->n <= 0 || !(lr > 0) || !(lr_final >= 0))
+    return mfail(HAMSTR_ERR_ARG, "bad arguments");
   int rc = ensure_cap(M, batch);
   if (rc) return rc;
+  float* dloss = nullptr;
+  if (losses && steps > 0) MCHK(hipMalloc(&dloss, sizeof(float) * steps));
+  const int nlb = (batch * kOut + 255) / 256;
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  const double kPi = 3.14159265358979323846;
   for (int s = 0; s < steps; s++) {
     hipLaunchKernelGGL(k_gather, dim3((batch + 255) / 256), dim3(256), 0, M->stream, M->X, M->T, M->n, batch, seed,
                        (unsigned)s, M->x, M->t);
     if ((rc = forward(M, batch))) return rc;
-    float l = 0;
-    if ((rc = backward(M, batch, losses ? &l : nullptr))) return rc;
-    if (losses) losses[s] = l;
+    if ((rc = backward(M, batch, nullptr))) return rc;
+    if (dloss)
+      hipLaunchKernelGGL(k_reduce, dim3(1), dim3(64), 0, M->stream, M->lpart, nlb, 1L, dloss + s);
     M->adam_t++;
     const float c1 = 1.0f - std::pow(b1, (float)M->adam_t), c2 = 1.0f - std::pow(b2, (float)M->adam_t);
+    const float lrs =
+        (float)(lr_final + 0.5 * (lr - lr_final) * (1.0 + std::cos(kPi * s / std::max(1, steps - 1))));
     hipLaunchKernelGGL(k_adam, dim3((kParams + 255) / 256), dim3(256), 0, M->stream, M->p, M->g, M->m, M->v, kParams,
-                       lr, b1, b2, c1, c2, eps);
+                       lrs, b1, b2, c1, c2, eps);
     if ((rc = refresh_bf16(M))) return rc;
   }
+  if (dloss) MCHK(hipMemcpyAsync(losses, dloss, sizeof(float) * steps, hipMemcpyDeviceToHost, M->stream));
   MCHK(hipStreamSynchronize(M->stream));
+  if (dloss) (void)hipFree(dloss);
   return HAMSTR_OK;
 }
 

@@ -30,7 +30,8 @@ EXPORTS = [
     "hamstr_last_timings", "hamstr_last_timing_names", "hamstr_flux_roe2d", "hamstr_jacobian_roe2d", "hamstr_muscl", "hamstr_weno",
     "hamstr_muscl_deriv", "hamstr_weno_deriv", "hamstr_block_tridag4", "hamstr_block_tridag_periodic4",
     "hamstr_last_error", "hamstr_batch_create", "hamstr_batch_info", "hamstr_batch_step", "hamstr_autotune",
-    "hamstr_launch_groups", "hamstr_case_freestream",
+    "hamstr_launch_groups", "hamstr_case_freestream", "hamstr_set_retire", "hamstr_case_status",
+    "hamstr_set_graph",
     "hamstr_mlp_create", "hamstr_mlp_destroy", "hamstr_mlp_num_params", "hamstr_mlp_get_params",
     "hamstr_mlp_set_params", "hamstr_mlp_forward", "hamstr_mlp_loss_grad", "hamstr_mlp_set_data",
     "hamstr_mlp_train", "hamstr_warm_start", "hamstr_mlp_last_error",
@@ -195,6 +196,22 @@ class Solver:
 
     def sync(self):
         _chk(lib().hamstr_sync(self.h))
+
+    def set_graph(self, on: bool):
+        """Replay captured HIP graphs between norm read-backs (default on)."""
+        _chk(lib().hamstr_set_graph(self.h, int(on)))
+
+    def set_retire(self, on: bool = True, stall_window: int = 0):
+        """Retire converged (and, with stall_window > 0, stalled) cases."""
+        _chk(lib().hamstr_set_retire(self.h, int(on), int(stall_window)))
+
+    def case_status(self):
+        """(status[k]: 0 live, 1 converged, 2 stalled; iteration of that event or -1)."""
+        n = getattr(self, "n_case", 1)
+        st = np.zeros(n, np.int32)
+        it = np.zeros(n, np.int64)
+        _chk(lib().hamstr_case_status(self.h, _i(st), it.ctypes.data_as(C.POINTER(C.c_long))))
+        return st, it
 
     def set_timing(self, on: bool):
         _chk(lib().hamstr_set_timing(self.h, int(on)))

@@ -99,19 +99,21 @@ def circle_nodes(nc: int, nr: int, r_outer: float = 20.0, ratio: float = 1.08) -
     return xy
 
 
-def naca00xx_thickness(xc: np.ndarray, t: float) -> np.ndarray:
+def naca00xx_thickness(xc: np.ndarray, t: float, closed_te: bool = False) -> np.ndarray:
     """Half thickness of a NACA 4-digit symmetric section, chord 1.
 
-    Uses the closed-trailing-edge coefficient -0.1036 so the O-grid wraps a
-    sharp trailing edge (the open form, -0.1015, leaves a 0.25%-chord base).
+    The standard formula (x^4 coefficient -0.1015) leaves a base of 0.021 t
+    chord at the trailing edge; closed_te=True uses -0.1036, which closes it to
+    a sharp edge.
     """
     xc = np.clip(xc, 0.0, 1.0)
+    c4 = -0.1036 if closed_te else -0.1015
     return 5.0 * t * (0.2969 * np.sqrt(xc) - 0.1260 * xc - 0.3516 * xc ** 2
-                      + 0.2843 * xc ** 3 - 0.1036 * xc ** 4)
+                      + 0.2843 * xc ** 3 + c4 * xc ** 4)
 
 
 def naca_nodes(nc: int, nr: int, t: float = 0.12, r_outer: float = 20.0,
-               first: Optional[float] = None) -> np.ndarray:
+               first: Optional[float] = None, closed_te: bool = False) -> np.ndarray:
     """Node grid (nc, nr+1, 2): NACA-00xx surface to a circle of radius r_outer.
 
     Surface stations use cosine clustering at both edges; node i sits at
@@ -122,7 +124,7 @@ def naca_nodes(nc: int, nr: int, t: float = 0.12, r_outer: float = 20.0,
     """
     phi = 2.0 * math.pi * np.arange(nc) / nc
     xs = 0.5 + 0.5 * np.cos(phi)
-    ys = np.sign(np.sin(phi)) * naca00xx_thickness(xs, t)
+    ys = np.sign(np.sin(phi)) * naca00xx_thickness(xs, t, closed_te)
     body = np.stack([xs, ys], axis=1)
     outer = np.stack([0.5 + r_outer * np.cos(phi), r_outer * np.sin(phi)], axis=1)
     if first is None:
@@ -206,8 +208,9 @@ def circle_ogrid(nc: int = 128, nr: int = 64, r_outer: float = 20.0, ratio: floa
 
 
 def naca_ogrid(nc: int = 512, nr: int = 256, t: float = 0.12, r_outer: float = 20.0,
-               first: Optional[float] = None) -> Mesh:
-    return build_ogrid(naca_nodes(nc, nr, t, r_outer, first), name=f"naca{int(round(t*100)):02d}_{nc}x{nr}")
+               first: Optional[float] = None, closed_te: bool = False) -> Mesh:
+    return build_ogrid(naca_nodes(nc, nr, t, r_outer, first, closed_te),
+                       name=f"naca{int(round(t*100)):02d}_{nc}x{nr}")
 
 
 # ----------------------------------------------------------------------------

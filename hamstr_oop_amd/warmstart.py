@@ -3,11 +3,15 @@
 An MLP (6 -> 256 x 4 SiLU -> 4, BF16 matrix-core kernels with FP32
 accumulation, trained on the GPU by hamstr_mlp_train) maps per-cell features
 (Mach, alpha in degrees, thickness, centroid x, centroid y, wall distance) to
-the primitive state (rho, u, v, p) of the converged flow.  It is trained on
-converged fields the solver itself produces and only supplies the initial
-state of a new solve: the ADI iteration still has to bring the nonlinear
-residual to the tolerance, and cells whose prediction is not a valid state
-fall back to the freestream (hamstr_warm_start).
+the primitive state (rho, u, v, p) of the converged flow, as its deviation
+from the case's freestream.  The centroid enters radially compressed,
+(x, y) * asinh(r) / r, and the wall distance as its logarithm, so that the
+cells near the section, where the field varies fastest, are spread over the
+input range.  The network is trained on converged fields the solver itself
+produces and only supplies the initial state of a new solve: the ADI
+iteration still has to bring the nonlinear residual to the tolerance, and
+cells whose prediction is not a valid state fall back to the freestream
+(hamstr_warm_start).
 """
 from __future__ import annotations
 
@@ -47,8 +51,9 @@ def cell_features(mesh: meshgen.Mesh, mach: float, alpha: float, thickness: floa
     out[:, 0] = mach
     out[:, 1] = alpha
     out[:, 2] = thickness
-    out[:, 3] = cen[:, 0]
-    out[:, 4] = cen[:, 1]
+    r = np.maximum(np.hypot(cen[:, 0], cen[:, 1]), 1e-12)
+    out[:, 3] = cen[:, 0] * np.arcsinh(r) / r
+    out[:, 4] = cen[:, 1] * np.arcsinh(r) / r
     out[:, 5] = np.log(d + 1e-4)
     return out
 
@@ -60,6 +65,12 @@ def primitive(q: np.ndarray, gamma: float = 1.4) -> np.ndarray:
     v = q[:, 2] / rho
     p = (gamma - 1.0) * (q[:, 3] - 0.5 * rho * (u * u + v * v))
     return np.stack([rho, u, v, p], axis=1)
+
+
+def freestream_primitive(mach: float, alpha: float, gamma: float = 1.4) -> np.ndarray:
+    """(rho, u, v, p) of the freestream initFlow sets (initFlow.cpp:107-142)."""
+    r, u, v, _ = meshgen.freestream(mach, alpha, gamma)
+    return np.array([r, u, v, 1.0 / gamma])
 
 
 @dataclass
@@ -138,9 +149,11 @@ class MLP:
         t = np.ascontiguousarray(t, np.float32)
         _chk(hip.lib().hamstr_mlp_set_data(self.h, C.c_long(x.shape[0]), _f(x), _f(t)))
 
-    def train(self, steps: int, batch: int = 4096, lr: float = 1e-3, seed: int = 1) -> np.ndarray:
+    def train(self, steps: int, batch: int = 4096, lr: float = 1e-3, lr_final: float = 1e-5,
+              seed: int = 1) -> np.ndarray:
         losses = np.zeros(max(steps, 1), np.float32)
-        _chk(hip.lib().hamstr_mlp_train(self.h, int(steps), int(batch), C.c_float(lr), C.c_uint(seed), _f(losses)))
+        _chk(hip.lib().hamstr_mlp_train(self.h, int(steps), int(batch), C.c_float(lr), C.c_float(lr_final),
+                                        C.c_uint(seed), _f(losses)))
         return losses[:steps]
 
 
@@ -189,7 +202,8 @@ def solve_batch(cases: Sequence[Case], order: int, cfl: float, tol: float, max_i
 
 
 def dataset(cases: Sequence[Case], batch: hip.Batch, per_case: int, rng: np.random.Generator):
-    """Features and primitive targets of a random subset of each case's cells."""
+    """Features and targets (primitive state minus the case's freestream) of a
+    random subset of each case's cells."""
     q = batch.get("q").reshape(-1, 4)
     xs, ts = [], []
     for k, c in enumerate(cases):
@@ -197,7 +211,7 @@ def dataset(cases: Sequence[Case], batch: hip.Batch, per_case: int, rng: np.rand
         qk = q[batch.cell_off[k]:batch.cell_off[k + 1]]
         idx = rng.choice(c.mesh.n_cell, size=min(per_case, c.mesh.n_cell), replace=False)
         xs.append(f[idx])
-        ts.append(primitive(qk[idx]).astype(np.float32))
+        ts.append((primitive(qk[idx]) - freestream_primitive(c.mach, c.alpha)).astype(np.float32))
     return np.concatenate(xs), np.concatenate(ts)
 
 

@@ -106,6 +106,21 @@ int hamstr_batch_info(const hamstr_ctx *ctx, int *ncases, int *cell_off);
  * the nonlinear and linear residuals at the last step.  *done = steps run. */
 int hamstr_batch_step(hamstr_ctx *ctx, int nsteps, double tol, int check_every, int *iters,
                       double *norms, int *done);
+/* Retirement (no counterpart in the reference, which runs one case for a fixed
+ * nsteps, march.cpp:15-59).  With on != 0, hamstr_step and hamstr_batch_step
+ * stop iterating a case at the first read-back where its nonlinear L2 <= tol
+ * (tol > 0), or once its L2 has not halved within stall_window iterations (a
+ * limit cycle; 0 = never).  A retired case keeps its state and drops out of
+ * every launch; the calls return once every case has retired.  Calling this
+ * again makes every case live and restarts the iteration count. */
+int hamstr_set_retire(hamstr_ctx *ctx, int on, int stall_window);
+/* hamstr_step and hamstr_batch_step replay a captured HIP graph for each run
+ * of up to 64 iterations between norm read-backs (on by default; not while
+ * timing).  on = 0 launches every kernel from the host instead. */
+int hamstr_set_graph(hamstr_ctx *ctx, int on);
+/* status[k]: 0 live, 1 converged, 2 stalled; iters[k]: the iteration, counted
+ * from hamstr_set_retire, at which case k converged or was found stalled. */
+int hamstr_case_status(const hamstr_ctx *ctx, int *status, long *iters);
 
 /* Autotuner: times every FAST launch group of the ADI phases at several
  * persistent grid sizes (all chains; 8, 4, 2, 1 workgroups per CU) on the
@@ -214,10 +229,12 @@ int hamstr_block_tridag_periodic4(int nsys, int N, const double *a, const double
 const char *hamstr_last_error(void);
 
 /* ---- learned warm start ----------------------------------------------------
- * An MLP 6 -> 256 x 4 (SiLU) -> 4 mapping normalised (Mach, alpha, thickness,
- * centroid x, centroid y, wall distance) to normalised primitive
- * (rho, u, v, p).  The 256x256 layers run on BF16 matrix cores with FP32
- * accumulation (forward and backward); FP32 master weights, Adam. */
+ * No counterpart in the reference (which starts every run from the freestream,
+ * initFlow.cpp:107-142).  An MLP 6 -> 256 x 4 (SiLU) -> 4 maps normalised
+ * per-cell features (Mach, alpha, thickness, centroid x, centroid y, wall
+ * distance) to the normalised deviation of the primitive state (rho, u, v, p)
+ * from the case's freestream.  The 256x256 layers run on BF16 matrix cores with
+ * FP32 accumulation (forward and backward); FP32 master weights, Adam. */
 typedef struct hamstr_mlp hamstr_mlp;
 int hamstr_mlp_create(unsigned seed, hamstr_mlp **out);
 int hamstr_mlp_destroy(hamstr_mlp *mlp);
@@ -228,13 +245,15 @@ int hamstr_mlp_set_params(hamstr_mlp *mlp, const float *host);
 int hamstr_mlp_forward(hamstr_mlp *mlp, int n, const float *x, float *y);
 /* mean-squared-error loss and gradient (n a multiple of 32) */
 int hamstr_mlp_loss_grad(hamstr_mlp *mlp, int n, const float *x, const float *t, float *loss, float *grad);
-/* device-resident training set, then steps of Adam on random batches */
+/* device-resident training set, then steps of Adam on random batches with a
+ * cosine learning-rate decay from lr to lr_final; losses[steps] may be NULL */
 int hamstr_mlp_set_data(hamstr_mlp *mlp, long n, const float *x, const float *t);
-int hamstr_mlp_train(hamstr_mlp *mlp, int steps, int batch, float lr, unsigned seed, float *losses);
-/* q := conserved form of the MLP's prediction from raw per-cell features
- * feat[nCell*6] and the normalisation nrm[20] (feature mean[6], std[6],
- * target mean[4], std[4]); cells whose prediction is not a valid state take
- * their case's freestream (*fallback_cells counts them). */
+int hamstr_mlp_train(hamstr_mlp *mlp, int steps, int batch, float lr, float lr_final, unsigned seed,
+                     float *losses);
+/* q := conserved form of freestream + the MLP's predicted deviation, from raw
+ * per-cell features feat[nCell*6] and the normalisation nrm[20] (feature
+ * mean[6], std[6], target mean[4], std[4]); cells whose prediction is not a
+ * valid state take their case's freestream (*fallback_cells counts them). */
 int hamstr_warm_start(hamstr_ctx *ctx, hamstr_mlp *mlp, const float *feat, const float *nrm,
                       int *fallback_cells);
 const char *hamstr_mlp_last_error(void);

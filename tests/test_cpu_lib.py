@@ -55,6 +55,13 @@ def test_bad_arguments_are_rejected():
     with pytest.raises(hip.HamstrError) as e:
         hip.preprocess(np.zeros((0, 2)), np.zeros((0, 6), np.int32))
     assert e.value.code == hip.ERR_ARG
+    L = hip.lib()
+    # null contexts and handles are errors, not crashes (no device needed)
+    assert L.hamstr_set_retire(None, 1, 0) == hip.ERR_ARG
+    assert L.hamstr_set_graph(None, 1) == hip.ERR_ARG
+    assert L.hamstr_case_status(None, None, None) == hip.ERR_ARG
+    assert L.hamstr_mlp_forward(None, 1, None, None) == hip.ERR_ARG
+    assert L.hamstr_warm_start(None, None, None, None, None) == hip.ERR_ARG
 
 
 def test_driver_built():

@@ -70,3 +70,72 @@ def test_autotune_keeps_results(hip):
     b1.batch_step(8, check_every=8)
     b2.batch_step(8, check_every=8)
     assert np.array_equal(b1.get("q"), b2.get("q"))
+
+
+@pytest.mark.parametrize("mode", ["exact", "fast", "fast32"])
+def test_graph_replays_match_eager_launches(hip, mode):
+    """Captured graphs replay the same kernels in the same order: every replay,
+    with eager work and synchronisations between them, is bit-identical."""
+    adi = hip.ADI_EXACT if mode == "exact" else hip.ADI_FAST
+    lhs = hip.LHS_FP32 if mode == "fast32" else hip.LHS_FP64
+    cases = _cases()
+    eager = hip.Batch(cases, order=3, cfl=10.0, lhs=lhs, adi_mode=adi)
+    eager.set_graph(False)
+    graph = hip.Batch(cases, order=3, cfl=10.0, lhs=lhs, adi_mode=adi)
+    for rep in range(4):
+        ne, _, norms_e = eager.batch_step(30, check_every=10)
+        ng, _, norms_g = graph.batch_step(30, check_every=10)   # 3 replays of a 10-iteration graph
+        assert ne == ng == 30
+        assert np.array_equal(eager.get("q"), graph.get("q")), f"replay block {rep}"
+        assert np.array_equal(norms_e, norms_g)
+        graph.get("r")   # eager work on the stream between replays
+    # single case, graph of 50 iterations with an eager tail of 7
+    mesh, mach, alpha = cases[1]
+    a = hip.Solver(mesh, mach, alpha, 3, 10.0, lhs=lhs, adi_mode=adi)
+    a.set_graph(False)
+    b = hip.Solver(mesh, mach, alpha, 3, 10.0, lhs=lhs, adi_mode=adi)
+    _, l2a, la = a.step(107, check_every=50)
+    _, l2b, lb = b.step(107, check_every=50)
+    assert np.array_equal(l2a, l2b) and np.array_equal(la, lb)
+    assert np.array_equal(a.get("q"), b.get("q"))
+
+
+def test_retired_cases_freeze_at_their_read_back(hip):
+    """A converged case drops out of the launches at the read-back where it
+    converged: its state equals a single-case run that stops there."""
+    cases = _cases()
+    tol, every = 1e-8, 25
+    b = hip.Batch(cases, order=3, cfl=10.0, adi_mode=hip.ADI_FAST)
+    b.set_retire(True)
+    n, iters, _ = b.batch_step(6000, tol=tol, check_every=every)
+    st, it = b.case_status()
+    assert (st == 1).all(), st
+    assert (it == iters).all()
+    assert n == max(int(np.ceil(i / every)) * every for i in iters)
+    for k, (mesh, mach, alpha) in enumerate(cases):
+        s = hip.Solver(mesh, mach, alpha, 3, 10.0, adi_mode=hip.ADI_FAST)
+        stop = int(np.ceil(iters[k] / every)) * every
+        s.step(stop, check_every=every)
+        assert np.array_equal(b.case_q(k).ravel(), s.get("q")), f"case {k}"
+        s.close()
+    # the frozen state stays frozen, and set_retire makes every case live again
+    q0 = b.get("q")
+    n2, _, _ = b.batch_step(50, tol=tol, check_every=every)
+    assert n2 == every and np.array_equal(b.get("q"), q0)
+    b.set_retire(False)
+    b.batch_step(5, check_every=5)
+    assert not np.array_equal(b.get("q"), q0)
+
+
+def test_stalled_cases_retire(hip):
+    """A tolerance no case can reach: each case's norm stops halving at the
+    rounding floor, it is found stalled and the call returns early."""
+    from hamstr_oop_amd import meshgen
+    cases = [(meshgen.circle_ogrid(16, 8), 0.3, 0.0), (meshgen.naca_ogrid(24, 10, t=0.10), 0.5, 2.0)]
+    b = hip.Batch(cases, order=1, cfl=10.0, adi_mode=hip.ADI_FAST)
+    b.set_retire(True, stall_window=300)
+    n, iters, norms = b.batch_step(20000, tol=1e-40, check_every=50)
+    st, it = b.case_status()
+    assert (st == 2).all(), st
+    assert n < 20000 and (it <= n).all()
+    assert (norms[:, 0] < 1e-12).all()

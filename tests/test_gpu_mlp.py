@@ -65,7 +65,7 @@ def test_training_is_deterministic_and_reduces_loss(ws):
     for _ in range(2):
         mlp = ws.MLP(seed=3)
         mlp.set_data(x, t)
-        losses = mlp.train(400, batch=512, lr=1e-3, seed=9)
+        losses = mlp.train(400, batch=512, lr=1e-3, lr_final=1e-4, seed=9)
         runs.append((losses, mlp.params()))
         mlp.close()
     losses = runs[0][0]
@@ -110,7 +110,7 @@ def test_trained_warm_start_cuts_iterations(ws):
     norm = ws.Normalizer.fit(x, t)
     mlp = ws.MLP(seed=2)
     mlp.set_data(norm.nx(x), norm.nt(t))
-    losses = mlp.train(3000, batch=1024, lr=1e-3, seed=4)
+    losses = mlp.train(3000, batch=1024, lr=1e-3, lr_final=1e-5, seed=4)
     print("training loss", losses[:3], losses[-3:], flush=True)
     assert losses[-50:].mean() < 0.05 * losses[:5].mean()
 
