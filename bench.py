@@ -112,6 +112,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-exact", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--sweep-cases", type=int, default=256)
+    ap.add_argument("--train-cases", type=int, default=128)
+    ap.add_argument("--train-steps", type=int, default=20000)
+    ap.add_argument("--stall-window", type=int, default=2000)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -125,19 +130,28 @@ def main():
     adi_mode = hip.ADI_FAST if args.adi == "fast" else hip.ADI_EXACT
 
     def run(lhs, mode=None, steps=None):
+        """Timed region 1: `steps` iterations replaying captured graphs (the
+        product path), no events.  Timed region 2: the same number of
+        iterations launched eagerly with a HIP event between kernels on the
+        context's stream, for per-kernel times."""
         mode = adi_mode if mode is None else mode
         steps = args.steps if steps is None else steps
         s = hip.Solver(mesh, args.mach, args.alpha, args.order, args.cfl, lhs=lhs, adi_mode=mode)
         s.step(args.warmup, check_every=max(args.warmup, 1))
-        s.set_timing(True)
         s.sync()
         t0 = time.perf_counter()
         n, l2r, _ = s.step(steps, check_every=steps)
         s.sync()
         dt = time.perf_counter() - t0
+        s.set_timing(True)
+        s.sync()
+        t1 = time.perf_counter()
+        s.step(steps, check_every=steps)
+        s.sync()
+        dt_ev = time.perf_counter() - t1
         stages = s.last_timings()
         s.close()
-        return n, dt, stages, float(l2r[-1])
+        return n, dt, stages, float(l2r[-1]), dt_ev
 
     def converge(lhs, mode=None):
         mode = adi_mode if mode is None else mode
@@ -151,7 +165,7 @@ def main():
         s.close()
         return {"iterations": n, "seconds": dt, "final_l2": float(l2r[-1]), "converged": bool(l2r[-1] <= args.tol)}, q
 
-    n, dt, stages, l2 = run(hip.LHS_FP64)
+    n, dt, stages, l2, dt_ev = run(hip.LHS_FP64)
     value = mesh.n_cell * n / dt
     total_b, stage_b = algorithmic_bytes(mesh.n_cell, n_proc, phase_faces, 8)
     # dominant kernel by event time; its algorithmic bytes per launch (a phase's
@@ -189,13 +203,14 @@ def main():
                                "achieved_gbs": value * total_b / 1e9,
                                "frac": value * total_b / 1e9 / HBM_PEAK_GBS},
         "stage_ms": {k: round(v, 5) for k, v in stages.items()},
+        "ms_per_step_eager_with_events": dt_ev / n * 1e3,
         "l2_after_timed_steps": l2,
     }
     if not args.no_converge:
         c64, q64 = converge(hip.LHS_FP64)
         out["time_to_tol_fp64"] = c64
     if args.adi == "fast" and not args.no_exact:
-        ne, dte, ste, _ = run(hip.LHS_FP64, hip.ADI_EXACT, steps=min(args.steps, 30))
+        ne, dte, ste, _, _ = run(hip.LHS_FP64, hip.ADI_EXACT, steps=min(args.steps, 30))
         out["exact_adi_fp64"] = {"value": mesh.n_cell * ne / dte, "ms_per_step": dte / ne * 1e3,
                                  "stage_ms": {k: round(v, 5) for k, v in ste.items()}}
         if not args.no_converge:
@@ -205,7 +220,7 @@ def main():
             ce["max_field_diff_vs_fast_rel"] = float((np.abs(q64 - qe).reshape(-1, 4) / scale).max())
             out["exact_adi_fp64"]["time_to_tol"] = ce
     if not args.no_fp32:
-        n32, dt32, st32, _ = run(hip.LHS_FP32)
+        n32, dt32, st32, _, _ = run(hip.LHS_FP32)
         out["fp32_lhs"] = {"value": mesh.n_cell * n32 / dt32, "ms_per_step": dt32 / n32 * 1e3,
                            "stage_ms": {k: round(v, 5) for k, v in st32.items()}}
         if not args.no_converge:
@@ -216,7 +231,97 @@ def main():
             out["fp32_lhs"]["time_to_tol"] = c32
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mesh, args.mach, args.alpha, args.order, args.cfl, args.cpu_budget)
+    if not args.no_sweep and args.sweep_cases > 0:
+        out["sweep"] = sweep_section(args, with_cpu=not args.no_cpu)
     print(json.dumps(out), flush=True)
+
+
+def sweep_section(args, with_cpu=True):
+    """Mixed batch of flow cases to L2 <= tol: plain FP64 one case at a time
+    from the freestream, against one batched FP32-LHS context (autotuned,
+    converged cases retired) from the freestream and from the learned warm
+    start.  Cases that settle into a limit cycle count as stalled in every run."""
+    import numpy as np
+    from hamstr_oop_amd import hip, sweep
+    log = lambda m: print(f"[sweep] {m}", file=sys.stderr, flush=True)  # noqa: E731
+    specs = sweep.draw_table(args.sweep_cases)
+    meshes = [c.mesh() for c in specs]
+    ncell = np.array([m.n_cell for m in meshes])
+    tol, mx, sw = args.tol, args.max_iters, args.stall_window
+    res = {"cases": len(specs), "seed": sweep.SEED, "tol": tol, "stall_window": sw,
+           "cells": int(ncell.sum()),
+           "mesh_mix": {f"{a}x{b}": int(sum(1 for c in specs if (c.nc, c.nr) == (a, b))) for a, b in sweep.SIZES}}
+    log(f"{len(specs)} cases, {ncell.sum()} cells")
+    cold64 = sweep.run_sequential(specs, meshes, tol, mx, sw, lhs=hip.LHS_FP64, progress=log)
+    res["sequential_fp64_cold"] = cold64.summary()
+    log(f"sequential FP64 cold: {cold64.seconds:.2f} s")
+    b64, o64, _ = sweep.run_batched(specs, meshes, tol, mx, sw, lhs=hip.LHS_FP64)
+    o64.close()
+    res["batched_fp64_cold"] = b64.summary()
+    b32, o32, i32 = sweep.run_batched(specs, meshes, tol, mx, sw, lhs=hip.LHS_FP32)
+    res["batched_fp32_cold"] = {**b32.summary(), **i32}
+    log(f"batched cold: FP64 {b64.seconds:.2f} s, FP32-LHS {b32.seconds:.2f} s ({i32})")
+    if args.train_cases > 0:
+        from hamstr_oop_amd import warmstart as ws
+        mlp, norm, info = sweep.train_network(args.train_cases, steps=args.train_steps, progress=log)
+        res["warm_start_training"] = info
+        feats = [ws.cell_features(m, c.mach, c.alpha, c.thickness) for c, m in zip(specs, meshes)]
+        fb = []
+        bw, ow, iw = sweep.run_batched(specs, meshes, tol, mx, sw, lhs=hip.LHS_FP32,
+                                       init=sweep.warm_init(mlp, norm, feats, fb))
+        mlp.close()
+        s = {**bw.summary(), **iw}
+        s["fallback_cells_first_pass"] = int(fb[0]) if fb else None
+        both = (bw.status == 1) & (b32.status == 1)
+        s["iterations_vs_cold_fp32"] = float(bw.iters[both].sum() / max(b32.iters[both].sum(), 1))
+        # the fixed point does not depend on the start: converged fields of both runs
+        diffs = []
+        for k in np.nonzero(both)[0]:
+            qc, qw = o32.case_q(k), ow.case_q(k)
+            diffs.append(float((np.abs(qw - qc) / np.abs(qc).max(axis=0)).max()))
+        s["max_field_diff_vs_cold_rel"] = max(diffs) if diffs else None
+        ow.close()
+        res["batched_fp32_warm"] = s
+        res["speedup_batched_fp32_warm_vs_sequential_fp64_cold"] = cold64.seconds / bw.seconds
+        log(f"batched FP32-LHS warm: {bw.seconds:.2f} s ({iw})")
+    o32.close()
+    res["speedup_batched_fp32_cold_vs_sequential_fp64_cold"] = cold64.seconds / b32.seconds
+    if with_cpu:
+        res["cpu_baseline"] = sweep_cpu_baseline(specs, meshes, cold64)
+    return res
+
+
+def sweep_cpu_baseline(specs, meshes, cold64, iters=40):
+    """The oracle's C++ restatement on the host cores, one case per core, on a
+    bounded sample (the first `iters` iterations of 2 cases per mesh size);
+    time to tol extrapolated from its per-core rate and the GPU run's iteration
+    counts.  Not the reference binary: one process per core needs the
+    leak-free restatement."""
+    import concurrent.futures as cf
+    import numpy as np
+    from hamstr_oop_amd import sweep
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py
+    cores = min(os.cpu_count() or 1, 16)
+    picks = []
+    for size in sweep.SIZES:
+        picks += [k for k, c in enumerate(specs) if (c.nc, c.nr) == size][:2]
+    orc = oracle_py.Oracle()
+
+    def one(k):
+        c = specs[k]
+        oc = orc.case(meshes[k], c.mach, c.alpha, 3, 10.0)
+        _, _, _, sec = oc.step(iters)
+        return meshes[k].n_cell * iters / sec
+
+    with cf.ThreadPoolExecutor(max_workers=min(cores, len(picks))) as ex:
+        rates = list(ex.map(one, picks))
+    rate = float(np.mean(rates))   # cell-iters/s per core with min(cores, samples) running at once
+    need = float(sum(m.n_cell * (it if st > 0 else 0) for m, it, st in zip(meshes, cold64.iters, cold64.status)))
+    return {"value": rate * cores, "unit": "cell-iters/s", "cores": cores, "kind": "port",
+            "per_core": rate, "time_to_tol_extrapolated_s": need / (rate * cores),
+            "sample": f"{iters} iterations of {len(picks)} cases (2 per mesh size), one case per thread; "
+                      f"time to tol = sum of cells x FP64 GPU iterations / (per-core rate x {cores})"}
 
 
 if __name__ == "__main__":

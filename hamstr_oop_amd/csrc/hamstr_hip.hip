@@ -1298,7 +1298,9 @@ struct hamstr_ctx {
   double4* qinf = nullptr;
   std::vector<double> qinf_h;  // [nCase][4]
   int* cell_case = nullptr;
-  double* rrho = nullptr;  // raw density residual per cell (batches: per-case norms)
+  // raw density residual per cell for the per-case norms of batches:
+  // [0, nCell) nonlinear, [nCell, 2 nCell) linear (a retired case keeps both)
+  double* rrho = nullptr;
   int* case_off = nullptr;
   std::vector<int> case_off_h;
   int *pleft = nullptr, *pright = nullptr, *pface = nullptr;
@@ -1398,7 +1400,8 @@ int launch_norm(hamstr_ctx* x, double* base, int slot) {
     hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(kRedBlock), 0, x->stream, x->partial, x->nPart, x->nCell,
                        base + slot);
   else
-    hipLaunchKernelGGL(k_case_norms, dim3(x->nCase), dim3(kRedBlock), 0, x->stream, x->rrho, x->case_off, slot, base);
+    hipLaunchKernelGGL(k_case_norms, dim3(x->nCase), dim3(kRedBlock), 0, x->stream,
+                       x->rrho + (slot == 0 ? 0 : (size_t)x->nCell), x->case_off, slot, base);
   HIPCHK(hipGetLastError());
   return HAMSTR_OK;
 }
@@ -1475,10 +1478,10 @@ int launch_lin_cells(hamstr_ctx* x, bool update, double* normbase) {
   dim3 g(nblk(x->nCell)), b(kBlock);
   if (update)
     hipLaunchKernelGGL((k_lin_cells<true>), g, b, 0, x->stream, x->c, x->live(), x->cell_off, x->cell_ent, x->lflux, x->r0,
-                       x->dq, x->sigma, x->r, x->q, x->partial, x->rrho);
+                       x->dq, x->sigma, x->r, x->q, x->partial, x->rrho ? x->rrho + x->nCell : nullptr);
   else
     hipLaunchKernelGGL((k_lin_cells<false>), g, b, 0, x->stream, x->c, x->live(), x->cell_off, x->cell_ent, x->lflux,
-                       x->r0, x->dq, x->sigma, x->r, x->q, x->partial, x->rrho);
+                       x->r0, x->dq, x->sigma, x->r, x->q, x->partial, x->rrho ? x->rrho + x->nCell : nullptr);
   HIPCHK(hipGetLastError());
   return launch_norm(x, normbase, 2);
 }
@@ -1811,7 +1814,7 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
   x->nPart = nblk(nCell);
   chk(hipMalloc(&x->partial, sizeof(double) * 2 * x->nPart), "alloc partial");
   chk(hipMalloc(&x->norms, sizeof(double) * 4 * nCase), "alloc norms");
-  if (nCase > 1) chk(hipMalloc(&x->rrho, sizeof(double) * nCell), "alloc rrho");
+  if (nCase > 1) chk(hipMalloc(&x->rrho, sizeof(double) * 2 * (size_t)nCell), "alloc rrho");
   chk(hipMalloc(&x->face_on, nProc), "alloc face mask");
   chk(hipMalloc(&x->cell_on, nCell), "alloc cell mask");
   chk(hipMalloc(&x->chain_on, nChain), "alloc chain mask");
@@ -2050,6 +2053,45 @@ int hamstr_get_field(hamstr_ctx* x, int field, double* host) {
   return HAMSTR_OK;
 }
 
+// cells [first, first+count) of a per-cell field (q, r, r0, dq: 4 values per
+// cell; sigma: 1)
+static int cell_range(hamstr_ctx* x, int field, int first, int count, double** dev, size_t* per) {
+  if (!x || first < 0 || count < 0 || (long)first + count > x->nCell) return fail(HAMSTR_ERR_ARG, "bad cell range");
+  *per = 4;
+  switch (field) {
+    case HAMSTR_FIELD_Q: *dev = x->q; break;
+    case HAMSTR_FIELD_R: *dev = x->r; break;
+    case HAMSTR_FIELD_R0: *dev = x->r0; break;
+    case HAMSTR_FIELD_DQ: *dev = x->dq; break;
+    case HAMSTR_FIELD_SIGMA: *dev = x->sigma; *per = 1; break;
+    default: return fail(HAMSTR_ERR_ARG, "not a per-cell field");
+  }
+  *dev += *per * (size_t)first;
+  return HAMSTR_OK;
+}
+
+int hamstr_get_cells(hamstr_ctx* x, int field, int first, int count, double* host) {
+  double* d = nullptr;
+  size_t per = 0;
+  const int rc = cell_range(x, field, first, count, &d, &per);
+  if (rc) return rc;
+  if (!host && count) return fail(HAMSTR_ERR_ARG, "null argument");
+  HIPCHK(hipMemcpyAsync(host, d, sizeof(double) * per * count, hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return HAMSTR_OK;
+}
+
+int hamstr_set_cells(hamstr_ctx* x, int field, int first, int count, const double* host) {
+  double* d = nullptr;
+  size_t per = 0;
+  const int rc = cell_range(x, field, first, count, &d, &per);
+  if (rc) return rc;
+  if (!host && count) return fail(HAMSTR_ERR_ARG, "null argument");
+  HIPCHK(hipMemcpyAsync(d, host, sizeof(double) * per * count, hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return HAMSTR_OK;
+}
+
 int hamstr_compute_rhs(hamstr_ctx* x, int implicit, double* L2, double* Linf) {
   if (!x) return fail(HAMSTR_ERR_ARG, "null context");
   int rc = launch_rhs_faces(x, implicit != 0);
@@ -2234,6 +2276,25 @@ int run_steps(hamstr_ctx* x, int nsteps, double tol, int check_every, std::vecto
       x->Linfrhs = last[1];
       x->L2lin = last[2];
       x->Linflin = last[3];
+      if (flag && x->retire && nc > 1) {
+        // a batch keeps going: cases whose norms are no longer finite retire as
+        // diverged (their cells are disjoint from every other case's)
+        int found = 0;
+        for (int j = 0; j < nc; j++) {
+          if (x->case_status[j] != 0) continue;
+          bool bad = false;
+          for (int k = lastRead; k < ran && !bad; k++)
+            for (int e = 0; e < 4; e++) bad = bad || !std::isfinite(series[4 * ((size_t)nc * k + j) + e]);
+          if (bad) {
+            if ((rc = retire_case(x, j, 3, x->it_total + ran))) return rc;
+            found++;
+          }
+        }
+        if (found) {
+          HIPCHK(hipMemsetAsync(x->nanflag, 0, sizeof(int), x->stream));  // eager, between graph replays
+          flag = 0;
+        }
+      }
       if (flag) {
         if (done) *done = ran;
         return fail(HAMSTR_ERR_NAN, "NaN spectral radius in the Roe flux");

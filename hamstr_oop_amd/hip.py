@@ -31,7 +31,7 @@ EXPORTS = [
     "hamstr_muscl_deriv", "hamstr_weno_deriv", "hamstr_block_tridag4", "hamstr_block_tridag_periodic4",
     "hamstr_last_error", "hamstr_batch_create", "hamstr_batch_info", "hamstr_batch_step", "hamstr_autotune",
     "hamstr_launch_groups", "hamstr_case_freestream", "hamstr_set_retire", "hamstr_case_status",
-    "hamstr_set_graph",
+    "hamstr_set_graph", "hamstr_get_cells", "hamstr_set_cells",
     "hamstr_mlp_create", "hamstr_mlp_destroy", "hamstr_mlp_num_params", "hamstr_mlp_get_params",
     "hamstr_mlp_set_params", "hamstr_mlp_forward", "hamstr_mlp_loss_grad", "hamstr_mlp_set_data",
     "hamstr_mlp_train", "hamstr_warm_start", "hamstr_mlp_last_error",
@@ -343,8 +343,15 @@ class Batch(Solver):
         return super().get(name)
 
     def case_q(self, k: int) -> np.ndarray:
-        q = self.get("q").reshape(-1, 4)
-        return q[self.cell_off[k]:self.cell_off[k + 1]].copy()
+        c0, c1 = int(self.cell_off[k]), int(self.cell_off[k + 1])
+        out = np.zeros((c1 - c0, 4))
+        _chk(lib().hamstr_get_cells(self.h, FIELDS["q"], c0, c1 - c0, _d(out)))
+        return out
+
+    def set_case_q(self, k: int, q) -> None:
+        c0, c1 = int(self.cell_off[k]), int(self.cell_off[k + 1])
+        q = np.ascontiguousarray(q, np.float64).reshape(c1 - c0, 4)
+        _chk(lib().hamstr_set_cells(self.h, FIELDS["q"], c0, c1 - c0, _d(q)))
 
     def batch_step(self, nsteps: int, tol: float = 0.0, check_every: int = 50):
         """Returns (steps run, iterations-to-tol per case (-1: not reached), norms (ncase, 4))."""

@@ -1,0 +1,217 @@
+"""Mixed-batch sweep: many flow cases to a residual tolerance, run one case at
+a time in plain FP64 from the freestream, and as one batched context (FP32 LHS,
+autotuned persistent launches, retirement of converged cases, optionally a
+learned warm start).
+
+Cases are drawn from a seed: Mach ~ U[0.2, 0.75], alpha ~ U[-4, 6] deg,
+thickness ~ U[0.08, 0.16], mesh drawn from 128x64, 256x128 and 384x192
+NACA-00xx O-grids (radial lines of 64 to 192 cells).  MUSCL (order 3), CFL 10.
+
+A case converges at the first iteration whose nonlinear L2 = sqrt(sum r_rho^2)
+/ nCell is <= tol.  Some cases of this scheme settle into a limit cycle just
+above 1e-10 (the MUSCL limiter's eps = 1e-10 makes it non-smooth at extrema;
+computeRHS.cpp:268-271): a case whose norm has not halved within stall_window
+iterations is called stalled, in every run alike, and counted separately.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import hip, meshgen
+
+SIZES = ((128, 64), (256, 128), (384, 192))
+SEED = 20261203
+
+
+@dataclass
+class CaseSpec:
+    mach: float
+    alpha: float
+    thickness: float
+    nc: int
+    nr: int
+
+    def mesh(self) -> meshgen.Mesh:
+        return meshgen.naca_ogrid(self.nc, self.nr, t=self.thickness)
+
+
+def draw_table(n: int = 256, seed: int = SEED, sizes: Sequence = SIZES) -> List[CaseSpec]:
+    """The case table: per case Mach, alpha, thickness, then the mesh index."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        m = float(rng.uniform(0.2, 0.75))
+        a = float(rng.uniform(-4.0, 6.0))
+        t = float(rng.uniform(0.08, 0.16))
+        nc, nr = sizes[int(rng.integers(len(sizes)))]
+        out.append(CaseSpec(m, a, t, nc, nr))
+    return out
+
+
+@dataclass
+class RunResult:
+    seconds: float
+    iters: np.ndarray       # iteration of convergence (or stall detection) per case
+    status: np.ndarray      # 1 converged, 2 stalled, 0 neither within max_iters
+    cell_iters: float       # sum over cases of cells x iterations run
+
+    def summary(self) -> dict:
+        return {"seconds": round(self.seconds, 4), "converged": int((self.status == 1).sum()),
+                "stalled": int((self.status == 2).sum()), "unfinished": int((self.status == 0).sum()),
+                "iterations_sum": int(self.iters[self.status > 0].sum()), "cell_iters": float(self.cell_iters)}
+
+
+def run_sequential(specs: Sequence[CaseSpec], meshes, tol: float, max_iters: int, stall_window: int,
+                   lhs: int = hip.LHS_FP64, check_every: int = 50, order: int = 3, cfl: float = 10.0,
+                   progress=None) -> RunResult:
+    """One single-case context per case, freestream start; the clock covers the
+    iterations only (context set-up excluded)."""
+    iters = np.full(len(specs), -1, np.int64)
+    status = np.zeros(len(specs), np.int32)
+    total = 0.0
+    cell_iters = 0.0
+    for k, (c, mesh) in enumerate(zip(specs, meshes)):
+        s = hip.Solver(mesh, c.mach, c.alpha, order, cfl, lhs=lhs, adi_mode=hip.ADI_FAST)
+        s.set_retire(True, stall_window)
+        s.sync()
+        t0 = time.perf_counter()
+        n, _, _ = s.step(max_iters, tol=tol, check_every=check_every)
+        s.sync()
+        total += time.perf_counter() - t0
+        st, it = s.case_status()
+        status[k], iters[k] = st[0], it[0]
+        cell_iters += mesh.n_cell * n
+        s.close()
+        if progress and (k + 1) % 32 == 0:
+            progress(f"sequential: {k + 1}/{len(specs)} cases, {total:.1f} s")
+    return RunResult(total, iters, status, cell_iters)
+
+
+class BatchOutcome:
+    """Final states of a batched run, per case (from the FP64 fallback context
+    for the cases it re-ran)."""
+
+    def __init__(self, main, sub=None, redo=()):
+        self.main, self.sub = main, sub
+        self.where = {k: j for j, k in enumerate(redo)}
+
+    def case_q(self, k: int) -> np.ndarray:
+        if self.sub is not None and k in self.where:
+            return self.sub.case_q(self.where[k])
+        return self.main.case_q(k)
+
+    def close(self):
+        for b in (self.main, self.sub):
+            if b is not None:
+                b.close()
+
+
+def run_batched(specs: Sequence[CaseSpec], meshes, tol: float, max_iters: int, stall_window: int,
+                lhs: int = hip.LHS_FP32, check_every: int = 50, order: int = 3, cfl: float = 10.0,
+                init=None, autotune: bool = True, fallback: bool = True):
+    """All cases in one batched context.  init(batch, case_indices) sets the
+    initial state (a warm start); it and the autotuner are timed with the
+    iterations.  With an FP32 LHS, cases that stall or diverge are re-run with
+    the FP64 LHS (the plain ADI iteration is the fallback): a stalled case from
+    the state it reached, a diverged one from its initial state.
+    Returns (RunResult, BatchOutcome, info)."""
+    n = len(specs)
+    b = hip.Batch([(m, c.mach, c.alpha) for c, m in zip(specs, meshes)], order=order, cfl=cfl, lhs=lhs,
+                  adi_mode=hip.ADI_FAST)
+    b.sync()
+    t0 = time.perf_counter()
+    if autotune:
+        b.computeRHS("implicit")   # a valid state for the timing launches (r and dq are restored)
+        hip.autotune(b)
+    if init is not None:
+        init(b, list(range(n)))
+    b.set_retire(True, stall_window)
+    b.batch_step(max_iters, tol=tol, check_every=check_every)
+    st, it = b.case_status()
+    status, iters = st.copy(), it.astype(np.int64)
+    ncell = np.diff(b.cell_off).astype(np.float64)
+    cell_iters = float((ncell * np.where(st > 0, it, max_iters)).sum())
+    info = {"stalled_first_pass": int((st == 2).sum()), "diverged_first_pass": int((st == 3).sum()),
+            "fallback_cases": 0}
+    sub, redo = None, []
+    if fallback and lhs == hip.LHS_FP32:
+        redo = [k for k in range(n) if st[k] in (2, 3)]
+        if redo:
+            sub = hip.Batch([(meshes[k], specs[k].mach, specs[k].alpha) for k in redo], order=order, cfl=cfl,
+                            lhs=hip.LHS_FP64, adi_mode=hip.ADI_FAST)
+            if init is not None:
+                init(sub, redo)
+            for j, k in enumerate(redo):
+                if st[k] == 2:
+                    sub.set_case_q(j, b.case_q(k))
+            sub.set_retire(True, stall_window)
+            sub.batch_step(max_iters, tol=tol, check_every=check_every)
+            st2, it2 = sub.case_status()
+            for j, k in enumerate(redo):
+                run2 = it2[j] if st2[j] > 0 else max_iters
+                status[k] = st2[j]
+                iters[k] = it[k] + run2
+                cell_iters += ncell[k] * run2
+            info["fallback_cases"] = len(redo)
+            info["fallback_converged"] = int((st2 == 1).sum())
+    b.sync()
+    sec = time.perf_counter() - t0
+    return RunResult(sec, iters, status, cell_iters), BatchOutcome(b, sub, redo), info
+
+
+def train_network(n_train: int, seed: int = SEED + 1, size=(256, 128), tol: float = 1e-8, max_iters: int = 20000,
+                  stall_window: int = 2000, per_case: int = 8192, chunk: int = 250, steps: int = 20000,
+                  batch: int = 4096, lr: float = 2e-3, lr_final: float = 1e-5, mlp_seed: int = 2, progress=None):
+    """Solves n_train cases (drawn like the sweep, on one mesh size) with the
+    batched FP32-LHS path to tol, samples per_case cells of each converged field,
+    and trains the warm-start MLP on the GPU.  Returns (mlp, normaliser, info)."""
+    from . import warmstart as ws
+    specs = draw_table(n_train, seed, sizes=(size,))
+    rng = np.random.default_rng(seed)
+    xs, ts = [], []
+    t_solve = 0.0
+    n_used = 0
+    for c0 in range(0, n_train, chunk):
+        part = specs[c0:c0 + chunk]
+        meshes = [c.mesh() for c in part]
+        res, out, _ = run_batched(part, meshes, tol, max_iters, stall_window, lhs=hip.LHS_FP32, autotune=False)
+        t_solve += res.seconds
+        for k in range(len(part)):
+            if res.status[k] != 1:
+                continue
+            c = ws.Case(meshes[k], part[k].mach, part[k].alpha, part[k].thickness)
+            x, t = ws.case_samples(c, out.case_q(k), per_case, rng)
+            xs.append(x)
+            ts.append(t)
+            n_used += 1
+        out.close()
+        if progress:
+            progress(f"training set: {c0 + len(part)}/{n_train} cases solved ({t_solve:.1f} s)")
+    x = np.concatenate(xs)
+    t = np.concatenate(ts)
+    norm = ws.Normalizer.fit(x, t)
+    mlp = ws.MLP(seed=mlp_seed)
+    mlp.set_data(norm.nx(x), norm.nt(t))
+    t0 = time.perf_counter()
+    losses = mlp.train(steps, batch=batch, lr=lr, lr_final=lr_final, seed=seed)
+    t_train = time.perf_counter() - t0
+    info = {"cases": n_train, "cases_converged": n_used, "mesh": f"{size[0]}x{size[1]}", "tol": tol,
+            "rows": int(x.shape[0]), "solve_s": round(t_solve, 3), "train_s": round(t_train, 3), "steps": steps,
+            "batch": batch, "loss_first": float(losses[:10].mean()), "loss_last": float(losses[-100:].mean())}
+    return mlp, norm, info
+
+
+def warm_init(mlp, norm, feats: Sequence[np.ndarray], fallback: Optional[list] = None):
+    """init(batch, case_indices) for run_batched: the MLP's prediction from the
+    per-case feature arrays feats[k] as initial state."""
+    from . import warmstart as ws
+
+    def init(b, idx):
+        n = ws.warm_start(b, mlp, np.concatenate([feats[k] for k in idx]), norm)
+        if fallback is not None:
+            fallback.append(n)
+    return init

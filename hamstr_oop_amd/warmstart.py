@@ -201,18 +201,29 @@ def solve_batch(cases: Sequence[Case], order: int, cfl: float, tol: float, max_i
     return b, iters, n, time.perf_counter() - t0
 
 
-def dataset(cases: Sequence[Case], batch: hip.Batch, per_case: int, rng: np.random.Generator):
+def dataset(cases: Sequence[Case], batch: hip.Batch, per_case: int, rng: np.random.Generator, only=None):
     """Features and targets (primitive state minus the case's freestream) of a
-    random subset of each case's cells."""
+    random subset of each case's cells (of the cases listed in only, if given)."""
     q = batch.get("q").reshape(-1, 4)
     xs, ts = [], []
     for k, c in enumerate(cases):
+        if only is not None and k not in only:
+            continue
         f = cell_features(c.mesh, c.mach, c.alpha, c.thickness)
         qk = q[batch.cell_off[k]:batch.cell_off[k + 1]]
         idx = rng.choice(c.mesh.n_cell, size=min(per_case, c.mesh.n_cell), replace=False)
         xs.append(f[idx])
         ts.append((primitive(qk[idx]) - freestream_primitive(c.mach, c.alpha)).astype(np.float32))
     return np.concatenate(xs), np.concatenate(ts)
+
+
+def case_samples(case: Case, q: np.ndarray, per_case: int, rng: np.random.Generator):
+    """Features and targets of per_case random cells of one converged field q."""
+    f = cell_features(case.mesh, case.mach, case.alpha, case.thickness)
+    q = np.asarray(q).reshape(-1, 4)
+    idx = rng.choice(case.mesh.n_cell, size=min(per_case, case.mesh.n_cell), replace=False)
+    t = (primitive(q[idx]) - freestream_primitive(case.mach, case.alpha)).astype(np.float32)
+    return f[idx], t
 
 
 def batch_features(cases: Sequence[Case]) -> np.ndarray:

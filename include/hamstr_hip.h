@@ -118,8 +118,10 @@ int hamstr_set_retire(hamstr_ctx *ctx, int on, int stall_window);
  * of up to 64 iterations between norm read-backs (on by default; not while
  * timing).  on = 0 launches every kernel from the host instead. */
 int hamstr_set_graph(hamstr_ctx *ctx, int on);
-/* status[k]: 0 live, 1 converged, 2 stalled; iters[k]: the iteration, counted
- * from hamstr_set_retire, at which case k converged or was found stalled. */
+/* status[k]: 0 live, 1 converged, 2 stalled, 3 diverged (with retirement on, a
+ * batch retires a case whose norms stop being finite instead of failing with
+ * HAMSTR_ERR_NAN); iters[k]: the iteration, counted from hamstr_set_retire, at
+ * which case k converged or was found stalled or diverged. */
 int hamstr_case_status(const hamstr_ctx *ctx, int *status, long *iters);
 
 /* Autotuner: times every FAST launch group of the ADI phases at several
@@ -141,6 +143,10 @@ int hamstr_case_freestream(const hamstr_ctx *ctx, int k, double *out4);
 
 int hamstr_set_field(hamstr_ctx *ctx, int field, const double *host);
 int hamstr_get_field(hamstr_ctx *ctx, int field, double *host);
+/* cells [first, first+count) of a per-cell field (q, r, r0, dq, sigma), e.g.
+ * one case of a batch */
+int hamstr_get_cells(hamstr_ctx *ctx, int field, int first, int count, double *host);
+int hamstr_set_cells(hamstr_ctx *ctx, int field, int first, int count, const double *host);
 
 /* ---- the iteration (SOLVER methods) ------------------------------------- */
 

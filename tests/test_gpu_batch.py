@@ -139,3 +139,48 @@ def test_stalled_cases_retire(hip):
     assert (st == 2).all(), st
     assert n < 20000 and (it <= n).all()
     assert (norms[:, 0] < 1e-12).all()
+
+
+def test_diverged_case_retires_without_stopping_the_batch(hip):
+    """A case whose state has no real sound speed (negative pressure) makes a
+    NaN spectral radius: with retirement on, only that case retires (diverged);
+    the other case runs on exactly as alone.  Without retirement the call
+    fails with HAMSTR_ERR_NAN, as the reference exits."""
+    cases = _cases()[:2]
+    b = hip.Batch(cases, order=3, cfl=10.0, adi_mode=hip.ADI_FAST)
+    q1 = b.case_q(1)
+    q1[:, 3] = -1.0     # E < kinetic energy: p < 0
+    b.set_case_q(1, q1)
+    b.set_retire(True)
+    n, iters, _ = b.batch_step(40, tol=1e-30, check_every=10)
+    st, it = b.case_status()
+    assert st[1] == 3 and it[1] == 10
+    assert st[0] == 0 and n == 40
+    mesh, mach, alpha = cases[0]
+    s = hip.Solver(mesh, mach, alpha, 3, 10.0, adi_mode=hip.ADI_FAST)
+    s.step(40, check_every=10)
+    assert np.array_equal(b.case_q(0).ravel(), s.get("q"))
+    b2 = hip.Batch(cases, order=3, cfl=10.0, adi_mode=hip.ADI_FAST)
+    b2.set_case_q(1, q1)
+    with pytest.raises(hip.HamstrError) as e:
+        b2.batch_step(10, check_every=10)
+    assert e.value.code == hip.ERR_NAN
+
+
+def test_fp32_lhs_reaches_the_fp64_fixed_point(hip):
+    """The converged state depends only on the FP64 residual: driven to a deep
+    tolerance, FP32-LHS and FP64-LHS runs agree to far below the FP32 rounding
+    of the LHS itself."""
+    from hamstr_oop_amd import meshgen
+    mesh = meshgen.naca_ogrid(64, 32, t=0.12)
+    qs = {}
+    for lhs in (hip.LHS_FP64, hip.LHS_FP32):
+        s = hip.Solver(mesh, 0.5, 2.0, 3, 10.0, lhs=lhs, adi_mode=hip.ADI_FAST)
+        n, l2, _ = s.step(40000, tol=1e-13, check_every=100)
+        assert l2[-1] <= 1e-13 or (l2 <= 1e-13).any(), l2[-1]
+        qs[lhs] = s.get("q").reshape(-1, 4)
+        s.close()
+    scale = np.abs(qs[hip.LHS_FP64]).max(axis=0)
+    d = (np.abs(qs[hip.LHS_FP32] - qs[hip.LHS_FP64]) / scale).max()
+    print(f"FP32-LHS vs FP64-LHS at L2 <= 1e-13: max rel diff {d:.2e}", flush=True)
+    assert d < 1e-6
