@@ -171,7 +171,7 @@ def main():
     # dominant kernel by event time; its algorithmic bytes per launch (a phase's
     # bytes are split over the kernels of that phase in proportion to time)
     group = {"rhs_faces": "rhs", "rhs_cells": "rhs", "accum_dq": "linear", "lin_faces": "linear",
-             "lin_cells": "linear"}
+             "lin_cells": "linear", "norms": "linear"}
     dom = max(stages, key=stages.get)
     g = group.get(dom, dom)
     g_time = sum(v for k, v in stages.items() if group.get(k, k) == g)

@@ -456,4 +456,149 @@ __device__ __forceinline__ void lu4_solve_fast(const LU4<T>& L, T* v, int s) {
   v[0] = d1 - L.u12 * v[s] - L.u13 * v[2 * s] - L.u14 * d4;
 }
 
+// --------------------------------------------------------------------------
+// FAST-mode Jacobians: the same frozen-eigenvalue Roe Jacobians as
+// jacobian_roe2d, with every division by rho, area and cbar replaced by a
+// multiplication with one reciprocal each (hardware reciprocal refined by
+// Newton steps).  Equal to jacobian_roe2d up to rounding; only the ADI's
+// left-hand side uses them, never the residual.
+// --------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ SideLin<T> side_lin_fast(const T* q, T nx, T ny, T gm1) {
+#pragma clang fp contract(fast)
+  SideLin<T> s;
+  const T ir = fast_rcp(q[0]);
+  s.ro = q[0];
+  s.u = q[1] * ir;
+  s.v = q[2] * ir;
+  s.E = q[3];
+  s.p = gm1 * (q[3] - T(0.5) * s.ro * (s.u * s.u + s.v * s.v));
+  s.ucon = s.u * nx + s.v * ny;
+  s.du0 = -s.u * ir;
+  s.du1 = ir;
+  s.dv0 = -s.v * ir;
+  s.dv2 = ir;
+  s.dp[0] = T(0.5) * gm1 * (s.u * s.u + s.v * s.v);
+  s.dp[1] = -gm1 * s.u;
+  s.dp[2] = -gm1 * s.v;
+  s.dp[3] = gm1;
+  s.dc[0] = -s.ucon * ir;
+  s.dc[1] = nx * ir;
+  s.dc[2] = ny * ir;
+  return s;
+}
+
+template <typename T, bool RIGHT>
+__device__ __forceinline__ void jac_side_fast(const SideLin<T>& s, T nx, T ny, T gm1, T eig1, T term1, T term2,
+                                              T ic, T ic2, T robar, T ubar, T vbar, T hbar, T uconbar, T harea,
+                                              T* M) {
+#pragma clang fp contract(fast)
+  const T sg = RIGHT ? T(1) : T(-1);
+  const T t1c2 = sg * term1 * ic2, t2c = sg * term2 * ic, t2rc = sg * term2 * robar * ic, t1r = sg * term1 * robar;
+  T dd1[4], dd2[4];
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    dd1[j] = t1c2 * s.dp[j] + t2rc * s.dc[j];
+    dd2[j] = t1r * s.dc[j] + t2c * s.dp[j];
+  }
+  dd1[3] = t1c2 * s.dp[3];
+  dd2[3] = t2c * s.dp[3];
+  const T e = sg * eig1;
+  T D[16];
+  D[0] = e + dd1[0];
+  D[1] = dd1[1];
+  D[2] = dd1[2];
+  D[3] = dd1[3];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    D[4 + j] = dd1[j] * ubar + dd2[j] * nx + ((j == 1) ? e : T(0));
+    D[8 + j] = dd1[j] * vbar + dd2[j] * ny + ((j == 2) ? e : T(0));
+    D[12 + j] = dd1[j] * hbar + dd2[j] * uconbar + ((j == 3) ? e : T(0));
+  }
+  const T rodc0 = s.ro * s.dc[0], rodc1 = s.ro * s.dc[1], rodc2 = s.ro * s.dc[2];
+  const T rouc = s.ro * s.ucon;
+  const T Ep = s.E + s.p;
+  T N[16];
+  N[0] = s.ucon + rodc0;
+  N[1] = rodc1;
+  N[2] = rodc2;
+  N[3] = T(0.0);
+  N[4] = s.ucon * s.u + rodc0 * s.u + rouc * s.du0 + nx * s.dp[0];
+  N[5] = rodc1 * s.u + rouc * s.du1 + nx * s.dp[1];
+  N[6] = rodc2 * s.u + nx * s.dp[2];
+  N[7] = nx * s.dp[3];
+  N[8] = s.ucon * s.v + rodc0 * s.v + rouc * s.dv0 + ny * s.dp[0];
+  N[9] = rodc1 * s.v + ny * s.dp[1];
+  N[10] = rodc2 * s.v + rouc * s.dv2 + ny * s.dp[2];
+  N[11] = ny * s.dp[3];
+  N[12] = s.dp[0] * s.ucon + Ep * s.dc[0];
+  N[13] = s.dp[1] * s.ucon + Ep * s.dc[1];
+  N[14] = s.dp[2] * s.ucon + Ep * s.dc[2];
+  N[15] = (T(1.0) + gm1) * s.ucon;
+#pragma unroll
+  for (int k = 0; k < 16; k++) M[k] = (N[k] - D[k]) * harea;
+}
+
+template <typename T>
+__device__ __forceinline__ void jacobian_roe2d_fast(double nxd_, double nyd_, const double* ql_, const double* qr_,
+                                                    double gamma, T* lmat, T* rmat) {
+#pragma clang fp contract(fast)
+  const T gam = T(gamma);
+  const T gm1 = gam - T(1.0);
+  const T nxd = T(nxd_), nyd = T(nyd_);
+  const T area = sqrt(nxd * nxd + nyd * nyd);
+  const T ia = fast_rcp(area);
+  const T nx = nxd * ia, ny = nyd * ia;
+  T ql[4] = {T(ql_[0]), T(ql_[1]), T(ql_[2]), T(ql_[3])};
+  T qr[4] = {T(qr_[0]), T(qr_[1]), T(qr_[2]), T(qr_[3])};
+  const SideLin<T> L = side_lin_fast<T>(ql, nx, ny, gm1);
+  const SideLin<T> R = side_lin_fast<T>(qr, nx, ny, gm1);
+  const T hl = (L.E + L.p) * L.du1;  // du1 = 1/rho
+  const T hr = (R.E + R.p) * R.du1;
+  const T fact = sqrt(R.ro * L.du1);
+  const T A = fast_rcp(T(1.0) + fact);
+  const T B = fact * A;
+  const T robar = L.ro * fact;
+  const T ubar = L.u * A + R.u * B;
+  const T vbar = L.v * A + R.v * B;
+  const T hbar = hl * A + hr * B;
+  const T cbar = sqrt(gm1 * (hbar - T(0.5) * (ubar * ubar + vbar * vbar)));
+  const T ic = fast_rcp(cbar);
+  const T uconbar = ubar * nx + vbar * ny;
+  const T eig1 = fabs(uconbar);
+  const T eig2 = fabs(uconbar + cbar);
+  const T eig3 = fabs(uconbar - cbar);
+  const T term1 = -eig1 + T(0.5) * (eig2 + eig3);
+  const T term2 = T(0.5) * (eig2 - eig3);
+  const T harea = T(0.5) * area;
+  jac_side_fast<T, false>(L, nx, ny, gm1, eig1, term1, term2, ic, ic * ic, robar, ubar, vbar, hbar, uconbar, harea,
+                          lmat);
+  jac_side_fast<T, true>(R, nx, ny, gm1, eig1, term1, term2, ic, ic * ic, robar, ubar, vbar, hbar, uconbar, harea,
+                         rmat);
+}
+
+// FAST-mode MUSCL derivative (the linear residual inside the ADI): one
+// reciprocal of a2 instead of three divisions; equal to muscl_deriv_pt up to
+// rounding
+__device__ __forceinline__ void muscl_deriv_pt_fast(double fm, double f0, double fp, double dm, double d0,
+                                                    double dp, double qt, double eps, double* ql, double* qr) {
+#pragma clang fp contract(fast)
+  const double thm = 1.0 - kThird, thp = 1.0 + kThird;
+  const double f2i = fp - f0;
+  const double f2i1 = f0 - fm;
+  const double df2i = dp - d0;
+  const double df2i1 = d0 - dm;
+  const double a1 = 3.0 * (f2i * f2i1 + eps);
+  const double da1 = 3.0 * (df2i * f2i1 + f2i * df2i1);
+  const double a2 = 2.0 * (f2i - f2i1) * (f2i - f2i1) + a1;
+  const double da2 = 4.0 * (f2i - f2i1) * (df2i - df2i1) + da1;
+  const double ia2 = fast_rcp(a2);
+  const double f3i = a1 * ia2;
+  const double df3i = (da1 - f3i * da2) * ia2;
+  const double f3qt = qt * f3i;
+  const double df3qt = qt * df3i;
+  *ql = d0 + df3qt * (thm * f2i1 + thp * f2i) + f3qt * (thm * df2i1 + thp * df2i);
+  *qr = d0 - df3qt * (thp * f2i1 + thm * f2i) - f3qt * (thp * df2i1 + thm * df2i);
+}
+
 }  // namespace hamstr

@@ -143,7 +143,7 @@ __device__ __forceinline__ void reconstruct_face(const double (*f)[4], double ep
   }
 }
 
-template <int ORDER>
+template <int ORDER, bool FASTJ>
 __device__ __forceinline__ void reconstruct_dface(const double (*f)[4], const double (*d)[4], double eps,
                                                   double qt, double* dln, double* drn1) {
 #pragma unroll
@@ -151,6 +151,10 @@ __device__ __forceinline__ void reconstruct_dface(const double (*f)[4], const do
     if (ORDER == 1) {
       dln[k] = d[2][k];
       drn1[k] = d[3][k];
+    } else if (ORDER == 3 && FASTJ) {
+      double a, b;
+      muscl_deriv_pt_fast(f[1][k], f[2][k], f[3][k], d[1][k], d[2][k], d[3][k], qt, eps, &dln[k], &a);
+      muscl_deriv_pt_fast(f[2][k], f[3][k], f[4][k], d[2][k], d[3][k], d[4][k], qt, eps, &b, &drn1[k]);
     } else if (ORDER == 3) {
       double a, b;
       muscl_deriv_pt(f[1][k], f[2][k], f[3][k], d[1][k], d[2][k], d[3][k], qt, eps, &dln[k], &a);
@@ -168,7 +172,9 @@ __device__ __forceinline__ void reconstruct_dface(const double (*f)[4], const do
 // rhs_faces: chain gather + reconstruction + Roe flux (+ Jacobians)
 // computeRHS.cpp:53-408
 // --------------------------------------------------------------------------
-template <int ORDER, typename TL, bool IMPL>
+// FASTJ: the FAST-mode Jacobians (reciprocals instead of divisions); the flux
+// and spectral radius are the reference's arithmetic either way
+template <int ORDER, typename TL, bool IMPL, bool FASTJ>
 __global__ __launch_bounds__(kBlock) void k_rhs_faces(FaceTab ft, Consts c, Live lv, const double* __restrict__ q,
                                                       double* __restrict__ flux, double* __restrict__ sr,
                                                       TL* __restrict__ ff, int* __restrict__ nanflag) {
@@ -198,7 +204,10 @@ __global__ __launch_bounds__(kBlock) void k_rhs_faces(FaceTab ft, Consts c, Live
   if (s_r != s_r) atomicOr(nanflag, 1);
   if (IMPL) {
     TL lm[16], rm[16];
-    jacobian_roe2d<TL>(n.x, n.y, left, right, c.gamma, lm, rm);
+    if (FASTJ)
+      jacobian_roe2d_fast<TL>(n.x, n.y, left, right, c.gamma, lm, rm);
+    else
+      jacobian_roe2d<TL>(n.x, n.y, left, right, c.gamma, lm, rm);
 #pragma unroll
     for (int e = 0; e < 16; e++) {
       ff[(size_t)e * c.nProc + p] = lm[e];
@@ -256,6 +265,36 @@ __global__ __launch_bounds__(kRedBlock) void k_norm_final(const double* __restri
   if (t == 0) {
     out[0] = sqrt(ssq[0]) / nCell;
     out[1] = smx[0];
+  }
+}
+
+// the same reduction for a run of iterations at once: block b reduces the
+// partials of iteration b/2, stage b%2 (0 nonlinear, 1 linear) into
+// out[4*(b/2) + 2*(b%2)], out[... + 1]
+__global__ __launch_bounds__(kRedBlock) void k_norm_slots(const double* __restrict__ part, int nPart, int nCell,
+                                                          double* __restrict__ out) {
+  __shared__ double ssq[kRedBlock];
+  __shared__ double smx[kRedBlock];
+  const int t = threadIdx.x, b = blockIdx.x;
+  const double* partial = part + (size_t)b * nPart * 2;
+  double a = 0.0, m = 0.0;
+  for (int i = t; i < nPart; i += kRedBlock) {
+    a += partial[2 * i];
+    m = fmax(m, partial[2 * i + 1]);
+  }
+  ssq[t] = a;
+  smx[t] = m;
+  __syncthreads();
+  for (int w = kRedBlock / 2; w > 0; w >>= 1) {
+    if (t < w) {
+      ssq[t] += ssq[t + w];
+      smx[t] = fmax(smx[t], smx[t + w]);
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    out[2 * b] = sqrt(ssq[0]) / nCell;
+    out[2 * b + 1] = smx[0];
   }
 }
 
@@ -937,7 +976,7 @@ __global__ __launch_bounds__(MAXT) void k_adi_fast(ChainTab ct, Live lv, const i
 // lin_faces: reconstruction derivatives + lmat*dqL + rmat*dqR per face
 // (computeLinearRHS.cpp:52-325)
 // --------------------------------------------------------------------------
-template <int ORDER, typename TL>
+template <int ORDER, typename TL, bool FASTJ>
 __global__ __launch_bounds__(kBlock) void k_lin_faces(FaceTab ft, Consts c, Live lv, const double* __restrict__ q,
                                                       const double* __restrict__ dq, const TL* __restrict__ ff,
                                                       double* __restrict__ lflux) {
@@ -956,7 +995,7 @@ __global__ __launch_bounds__(kBlock) void k_lin_faces(FaceTab ft, Consts c, Live
     put4(d[j], load_dstate(e, dq, ft.refmtx));
   }
   double dln[4], drn1[4];
-  reconstruct_dface<ORDER>(f, d, c.eps, c.qt, dln, drn1);
+  reconstruct_dface<ORDER, FASTJ>(f, d, c.eps, c.qt, dln, drn1);
   double dl[4], dr[4];
 #pragma unroll
   for (int k = 0; k < 4; k++) {
@@ -1036,14 +1075,17 @@ __global__ void k_init_flow(int nCell, const int* __restrict__ cell_case, const 
   if (i < nCell) *reinterpret_cast<double4*>(q + 4 * (size_t)i) = qinf[cell_case[i]];
 }
 
-// per-case density-residual norms, one block per case, cells in order:
-// out[4*k + slot] = L2, out[4*k + slot + 1] = Linf
-__global__ __launch_bounds__(kRedBlock) void k_case_norms(const double* __restrict__ rrho,
+// per-case density-residual norms, one block per case and stage, cells in
+// order: stage s = slot/2 + blockIdx.y reads rrho + s*nCell and writes
+// out[4*k + 2*s] = L2, out[4*k + 2*s + 1] = Linf
+__global__ __launch_bounds__(kRedBlock) void k_case_norms(const double* __restrict__ rrho_all, int nCell,
                                                           const int* __restrict__ case_off, int slot,
                                                           double* __restrict__ out) {
   __shared__ double ssq[kRedBlock];
   __shared__ double smx[kRedBlock];
   const int k = blockIdx.x, t = threadIdx.x;
+  const int stage = slot / 2 + blockIdx.y;
+  const double* rrho = rrho_all + (size_t)stage * nCell;
   const int c0 = case_off[k], c1 = case_off[k + 1];
   double a = 0.0, m = 0.0;
   for (int i = c0 + t; i < c1; i += kRedBlock) {
@@ -1062,8 +1104,8 @@ __global__ __launch_bounds__(kRedBlock) void k_case_norms(const double* __restri
     __syncthreads();
   }
   if (t == 0) {
-    out[4 * k + slot] = sqrt(ssq[0]) / (c1 - c0);
-    out[4 * k + slot + 1] = smx[0];
+    out[4 * k + 2 * stage] = sqrt(ssq[0]) / (c1 - c0);
+    out[4 * k + 2 * stage + 1] = smx[0];
   }
 }
 
@@ -1314,6 +1356,7 @@ struct hamstr_ctx {
   size_t scratch_stride = 0;
   double* partial = nullptr;
   int nPart = 0;
+  double* npart = nullptr;  // single case: [kGraphMaxIters][2 stages][nPart][2] block partials
   double* norms = nullptr;  // [nCase][4]: L2rhs, Linfrhs, L2lin, Linflin
   double* hist = nullptr;   // [histCap][nCase][4]
   int histCap = 0;
@@ -1357,22 +1400,23 @@ namespace {
 int launch_rhs_faces(hamstr_ctx* x, bool implicit) {
   const FaceTab ft = x->ftab();
   dim3 g(nblk(x->nProc)), b(kBlock);
-#define RHSF(O, T, I) \
-  hipLaunchKernelGGL((k_rhs_faces<O, T, I>), g, b, 0, x->stream, ft, x->c, x->live(), x->q, x->flux, x->sr, (T*)x->ff, x->nanflag)
+#define RHSF(O, T, I, F)                                                                                         \
+  hipLaunchKernelGGL((k_rhs_faces<O, T, I, F>), g, b, 0, x->stream, ft, x->c, x->live(), x->q, x->flux, x->sr, (T*)x->ff, \
+                     x->nanflag)
+#define RHSO(T, I, F) \
+  if (o == 1) RHSF(1, T, I, F); else if (o == 3) RHSF(3, T, I, F); else RHSF(5, T, I, F)
   const int o = x->prm.order;
+  const bool fj = x->prm.adi_mode == HAMSTR_ADI_FAST;
   if (x->lhs32()) {
-    if (implicit) {
-      if (o == 1) RHSF(1, float, true); else if (o == 3) RHSF(3, float, true); else RHSF(5, float, true);
-    } else {
-      if (o == 1) RHSF(1, float, false); else if (o == 3) RHSF(3, float, false); else RHSF(5, float, false);
-    }
+    if (!implicit) RHSO(float, false, false);
+    else if (fj) RHSO(float, true, true);
+    else RHSO(float, true, false);
   } else {
-    if (implicit) {
-      if (o == 1) RHSF(1, double, true); else if (o == 3) RHSF(3, double, true); else RHSF(5, double, true);
-    } else {
-      if (o == 1) RHSF(1, double, false); else if (o == 3) RHSF(3, double, false); else RHSF(5, double, false);
-    }
+    if (!implicit) RHSO(double, false, false);
+    else if (fj) RHSO(double, true, true);
+    else RHSO(double, true, false);
   }
+#undef RHSO
 #undef RHSF
   HIPCHK(hipGetLastError());
   return HAMSTR_OK;
@@ -1381,14 +1425,19 @@ int launch_rhs_faces(hamstr_ctx* x, bool implicit) {
 int launch_lin_faces(hamstr_ctx* x) {
   const FaceTab ft = x->ftab();
   dim3 g(nblk(x->nProc)), b(kBlock);
-#define LINF(O, T) \
-  hipLaunchKernelGGL((k_lin_faces<O, T>), g, b, 0, x->stream, ft, x->c, x->live(), x->q, x->dq, (const T*)x->ff, x->lflux)
+#define LINF(O, T, F)                                                                                         \
+  hipLaunchKernelGGL((k_lin_faces<O, T, F>), g, b, 0, x->stream, ft, x->c, x->live(), x->q, x->dq, (const T*)x->ff, \
+                     x->lflux)
+#define LINO(T, F) \
+  if (o == 1) LINF(1, T, F); else if (o == 3) LINF(3, T, F); else LINF(5, T, F)
   const int o = x->prm.order;
+  const bool fj = x->prm.adi_mode == HAMSTR_ADI_FAST;
   if (x->lhs32()) {
-    if (o == 1) LINF(1, float); else if (o == 3) LINF(3, float); else LINF(5, float);
+    if (fj) LINO(float, true); else LINO(float, false);
   } else {
-    if (o == 1) LINF(1, double); else if (o == 3) LINF(3, double); else LINF(5, double);
+    if (fj) LINO(double, true); else LINO(double, false);
   }
+#undef LINO
 #undef LINF
   HIPCHK(hipGetLastError());
   return HAMSTR_OK;
@@ -1400,21 +1449,25 @@ int launch_norm(hamstr_ctx* x, double* base, int slot) {
     hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(kRedBlock), 0, x->stream, x->partial, x->nPart, x->nCell,
                        base + slot);
   else
-    hipLaunchKernelGGL(k_case_norms, dim3(x->nCase), dim3(kRedBlock), 0, x->stream,
-                       x->rrho + (slot == 0 ? 0 : (size_t)x->nCell), x->case_off, slot, base);
+    hipLaunchKernelGGL(k_case_norms, dim3(x->nCase, 1), dim3(kRedBlock), 0, x->stream, x->rrho, x->nCell, x->case_off,
+                       slot, base);
   HIPCHK(hipGetLastError());
   return HAMSTR_OK;
 }
 
-int launch_rhs_cells(hamstr_ctx* x, bool adi_init, double* normbase) {
+// part == nullptr: block partials to x->partial and the norms at once into
+// normbase; otherwise block partials to part, reduced later (k_norm_slots /
+// k_case_norms) by the caller
+int launch_rhs_cells(hamstr_ctx* x, bool adi_init, double* normbase, double* part = nullptr) {
+  double* pp = part ? part : x->partial;
   if (adi_init)
     hipLaunchKernelGGL((k_rhs_cells<true>), dim3(nblk(x->nCell)), dim3(kBlock), 0, x->stream, x->c, x->live(), x->cell_off,
-                       x->cell_ent, x->flux, x->sr, x->r, x->r0, x->sigma, x->partial, x->rrho);
+                       x->cell_ent, x->flux, x->sr, x->r, x->r0, x->sigma, pp, x->rrho);
   else
     hipLaunchKernelGGL((k_rhs_cells<false>), dim3(nblk(x->nCell)), dim3(kBlock), 0, x->stream, x->c, x->live(),
-                       x->cell_off, x->cell_ent, x->flux, x->sr, x->r, x->r0, x->sigma, x->partial, x->rrho);
+                       x->cell_off, x->cell_ent, x->flux, x->sr, x->r, x->r0, x->sigma, pp, x->rrho);
   HIPCHK(hipGetLastError());
-  return launch_norm(x, normbase, 0);
+  return part ? HAMSTR_OK : launch_norm(x, normbase, 0);
 }
 
 int launch_group(hamstr_ctx* x, const Group& g, int grid, int dqmode) {
@@ -1474,16 +1527,17 @@ int launch_sweep_phases(hamstr_ctx* x, int sweep, const std::function<int(const 
   return HAMSTR_OK;
 }
 
-int launch_lin_cells(hamstr_ctx* x, bool update, double* normbase) {
+int launch_lin_cells(hamstr_ctx* x, bool update, double* normbase, double* part = nullptr) {
   dim3 g(nblk(x->nCell)), b(kBlock);
+  double* pp = part ? part : x->partial;
   if (update)
     hipLaunchKernelGGL((k_lin_cells<true>), g, b, 0, x->stream, x->c, x->live(), x->cell_off, x->cell_ent, x->lflux, x->r0,
-                       x->dq, x->sigma, x->r, x->q, x->partial, x->rrho ? x->rrho + x->nCell : nullptr);
+                       x->dq, x->sigma, x->r, x->q, pp, x->rrho ? x->rrho + x->nCell : nullptr);
   else
     hipLaunchKernelGGL((k_lin_cells<false>), g, b, 0, x->stream, x->c, x->live(), x->cell_off, x->cell_ent, x->lflux,
-                       x->r0, x->dq, x->sigma, x->r, x->q, x->partial, x->rrho ? x->rrho + x->nCell : nullptr);
+                       x->r0, x->dq, x->sigma, x->r, x->q, pp, x->rrho ? x->rrho + x->nCell : nullptr);
   HIPCHK(hipGetLastError());
-  return launch_norm(x, normbase, 2);
+  return part ? HAMSTR_OK : launch_norm(x, normbase, 2);
 }
 
 int launch_linear(hamstr_ctx* x, bool update, double* normbase) {
@@ -1523,7 +1577,7 @@ void free_ctx(hamstr_ctx* x) {
                   x->nvec, x->refmtx, x->qinf, x->cell_case, x->case_off, x->pleft, x->pright, x->pface,
                   x->cell_off, x->cell_ent, x->ch_pbase, x->ch_nf, x->ch_ncell, x->ch_periodic, x->group_chains,
                   x->scratch, x->partial, x->norms, x->hist, x->nanflag, x->rrho, x->face_on, x->cell_on,
-                  x->chain_on, x->ghist};
+                  x->chain_on, x->ghist, x->npart};
   if (x->gexec) (void)hipGraphExecDestroy(x->gexec);
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -1813,6 +1867,8 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
   chk(hipMalloc(&x->scratch, tl * 36 * std::max<size_t>(maxRows, 1) * x->scratch_stride), "alloc scratch");
   x->nPart = nblk(nCell);
   chk(hipMalloc(&x->partial, sizeof(double) * 2 * x->nPart), "alloc partial");
+  if (nCase == 1)
+    chk(hipMalloc(&x->npart, sizeof(double) * 2 * 2 * (size_t)kGraphMaxIters * x->nPart), "alloc norm partials");
   chk(hipMalloc(&x->norms, sizeof(double) * 4 * nCase), "alloc norms");
   if (nCase > 1) chk(hipMalloc(&x->rrho, sizeof(double) * 2 * (size_t)nCell), "alloc rrho");
   chk(hipMalloc(&x->face_on, nProc), "alloc face mask");
@@ -2130,7 +2186,6 @@ int hamstr_compute_linear_rhs(hamstr_ctx* x, double* L2lin, double* Linflin) {
 
 namespace {
 
-// one fused iteration (march.cpp:112-117); norms to nb[nCase][4]
 // Stops case k from iterating: its faces, cells and chains drop out of every
 // launch (a fill kernel on the context's stream, ordered with the iterations).
 int retire_case(hamstr_ctx* x, int k, int status, long it) {
@@ -2155,18 +2210,47 @@ void drop_graph(hamstr_ctx* x) {
   x->gK = 0;
 }
 
-int enqueue_iteration(hamstr_ctx* x, double* nb, const std::function<int(const char*)>& mark) {
+// one fused iteration (march.cpp:112-117).  Single case: the norms' block
+// partials go to iteration slot `slot` of x->npart (reduced per run of
+// iterations by enqueue_block); batches: per-case norms of both stages to
+// nb[nCase][4] at the end of the iteration.
+int enqueue_iteration(hamstr_ctx* x, double* nb, const std::function<int(const char*)>& mark, int slot) {
   int rc;
+  const bool one = x->nCase == 1;
+  double* p_nl = one ? x->npart + (size_t)(2 * slot) * x->nPart * 2 : x->partial;
+  double* p_lin = one ? x->npart + (size_t)(2 * slot + 1) * x->nPart * 2 : x->partial;
   if ((rc = launch_rhs_faces(x, true))) return rc;
   if (mark && mark("rhs_faces")) return HAMSTR_ERR_HIP;
-  if ((rc = launch_rhs_cells(x, true, nb))) return rc;
+  if ((rc = launch_rhs_cells(x, true, nb, p_nl))) return rc;
   if (mark && mark("rhs_cells")) return HAMSTR_ERR_HIP;
   for (int sw = 0; sw < x->prm.msweep; sw++) {
     if ((rc = launch_sweep_phases(x, sw, mark))) return rc;
     if ((rc = launch_lin_faces(x))) return rc;
     if (mark && mark("lin_faces")) return HAMSTR_ERR_HIP;
-    if ((rc = launch_lin_cells(x, sw == x->prm.msweep - 1, nb))) return rc;
+    if ((rc = launch_lin_cells(x, sw == x->prm.msweep - 1, nb, p_lin))) return rc;
     if (mark && mark("lin_cells")) return HAMSTR_ERR_HIP;
+  }
+  if (!one) {
+    hipLaunchKernelGGL(k_case_norms, dim3(x->nCase, 2), dim3(kRedBlock), 0, x->stream, x->rrho, x->nCell,
+                       x->case_off, 0, nb);
+    HIPCHK(hipGetLastError());
+    if (mark && mark("norms")) return HAMSTR_ERR_HIP;
+  }
+  return HAMSTR_OK;
+}
+
+// `count` iterations whose norms go to base[count][nCase][4]
+int enqueue_block(hamstr_ctx* x, int count, double* base, const std::function<int(const char*)>& mark) {
+  const int nc = x->nCase;
+  for (int s = 0; s < count; s++) {
+    const int rc = enqueue_iteration(x, base + 4 * (size_t)nc * s, mark, s);
+    if (rc) return rc;
+  }
+  if (nc == 1) {
+    hipLaunchKernelGGL(k_norm_slots, dim3(2 * count), dim3(kRedBlock), 0, x->stream, x->npart, x->nPart, x->nCell,
+                       base);
+    HIPCHK(hipGetLastError());
+    if (mark && mark("norms")) return HAMSTR_ERR_HIP;
   }
   return HAMSTR_OK;
 }
@@ -2185,7 +2269,7 @@ int build_graph(hamstr_ctx* x, int K) {
   hipGraph_t g = nullptr;
   HIPCHK(hipStreamBeginCapture(x->stream, hipStreamCaptureModeThreadLocal));
   int rc = HAMSTR_OK;
-  for (int n = 0; n < K && rc == HAMSTR_OK; n++) rc = enqueue_iteration(x, x->ghist + 4 * (size_t)nc * n, nullptr);
+  rc = enqueue_block(x, K, x->ghist, nullptr);
   const hipError_t e = hipStreamEndCapture(x->stream, &g);
   if (rc) {
     if (e == hipSuccess && g) (void)hipGraphDestroy(g);
@@ -2259,7 +2343,7 @@ int run_steps(hamstr_ctx* x, int nsteps, double tol, int check_every, std::vecto
       ran = n + K;
     } else {
       if (x->timing && mark("start")) return fail(HAMSTR_ERR_HIP, "hipEventRecord");
-      if ((rc = enqueue_iteration(x, x->hist + 4 * (size_t)nc * n, mk))) return rc;
+      if ((rc = enqueue_block(x, 1, x->hist + 4 * (size_t)nc * n, mk))) return rc;
       if (n == 0) nstage = (int)evs.size() - 1;
       ran = n + 1;
     }
