@@ -117,6 +117,8 @@ def main():
     ap.add_argument("--train-cases", type=int, default=128)
     ap.add_argument("--train-steps", type=int, default=20000)
     ap.add_argument("--stall-window", type=int, default=2000)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run one plain 256 MiB device-to-device copy (counter calibration under rocprofv3)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -231,9 +233,29 @@ def main():
             out["fp32_lhs"]["time_to_tol"] = c32
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mesh, args.mach, args.alpha, args.order, args.cfl, args.cpu_budget)
+    if args.copy_probe:
+        out["copy_probe_bytes"] = copy_probe(256 << 20)
     if not args.no_sweep and args.sweep_cases > 0:
         out["sweep"] = sweep_section(args, with_cpu=not args.no_cpu)
     print(json.dumps(out), flush=True)
+
+
+def copy_probe(nbytes):
+    """One plain device-to-device copy of nbytes through the HIP runtime (its
+    blit kernel reads nbytes and writes nbytes): the reference point that
+    scales the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import ctypes as C
+    rt = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    ok = rt.hipMalloc(C.byref(src), C.c_size_t(nbytes)) == 0 and rt.hipMalloc(C.byref(dst), C.c_size_t(nbytes)) == 0
+    if ok:
+        ok = rt.hipMemset(src, 1, C.c_size_t(nbytes)) == 0
+        ok = ok and rt.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) == 0   # hipMemcpyDeviceToDevice
+        ok = ok and rt.hipDeviceSynchronize() == 0
+    for p in (src, dst):
+        if p.value:
+            rt.hipFree(p)
+    return {"read": nbytes, "written": nbytes, "ok": bool(ok)}
 
 
 def sweep_section(args, with_cpu=True):

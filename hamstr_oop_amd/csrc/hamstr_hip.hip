@@ -55,6 +55,18 @@ constexpr int kGraphMaxIters = 64;  // iterations per captured graph
 
 inline int nblk(long n, int b = kBlock) { return (int)((n + b - 1) / b); }
 
+// The hardware deals the workgroups of a launch to the 8 XCDs round-robin
+// (workgroup b to XCD b % 8), and each XCD has its own L2.  Kernels work on
+// virtual block v = (b % 8) * (nb / 8) + b / 8 instead, so every XCD covers a
+// contiguous range: neighbouring radial lines, whose cells share cache lines,
+// then meet in the same L2.  A bijection on [0, nb); the tail beyond the last
+// multiple of 8 keeps its order.
+constexpr int kXcd = 8;
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  const int per = nb / kXcd;
+  return b < per * kXcd ? (b % kXcd) * per + b / kXcd : b;
+}
+
 // --------------------------------------------------------------------------
 // Device tables
 // --------------------------------------------------------------------------
@@ -178,7 +190,7 @@ template <int ORDER, typename TL, bool IMPL, bool FASTJ>
 __global__ __launch_bounds__(kBlock) void k_rhs_faces(FaceTab ft, Consts c, Live lv, const double* __restrict__ q,
                                                       double* __restrict__ flux, double* __restrict__ sr,
                                                       TL* __restrict__ ff, int* __restrict__ nanflag) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (p >= c.nProc || (lv.face && !lv.face[p])) return;
   const int code = ft.pslot[p];
   const int s = code & 0x3fffffff;
@@ -219,7 +231,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_faces(FaceTab ft, Consts c, Live
 // --------------------------------------------------------------------------
 // block reductions for the density-residual norms
 // --------------------------------------------------------------------------
-__device__ __forceinline__ void block_norm_partial(double sq, double mx, double* __restrict__ partial) {
+__device__ __forceinline__ void block_norm_partial(double sq, double mx, double* __restrict__ partial, int blk) {
   // per-block partial (sum of r_rho^2, max |r_rho|); k_norm_final reduces them
   __shared__ double ssq[kBlock];
   __shared__ double smx[kBlock];
@@ -235,8 +247,8 @@ __device__ __forceinline__ void block_norm_partial(double sq, double mx, double*
     __syncthreads();
   }
   if (t == 0) {
-    partial[2 * blockIdx.x] = ssq[0];
-    partial[2 * blockIdx.x + 1] = smx[0];
+    partial[2 * blk] = ssq[0];
+    partial[2 * blk + 1] = smx[0];
   }
 }
 
@@ -309,7 +321,8 @@ __global__ __launch_bounds__(kBlock) void k_rhs_cells(Consts c, Live lv, const i
                                                       const double* __restrict__ sr, double* __restrict__ r,
                                                       double* __restrict__ r0, double* __restrict__ sigma,
                                                       double* __restrict__ partial, double* __restrict__ rrho) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int vb = xcd_block(blockIdx.x, gridDim.x);
+  const int i = vb * blockDim.x + threadIdx.x;
   double sq = 0.0, mx = 0.0;
   if (i < c.nCell && (!lv.cell || lv.cell[i])) {
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, sg = 0.0;
@@ -338,7 +351,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_cells(Consts c, Live lv, const i
       *reinterpret_cast<double4*>(r + 4 * (size_t)i) = make_double4(a0, a1, a2, a3);
     }
   }
-  block_norm_partial(sq, mx, partial);
+  block_norm_partial(sq, mx, partial, vb);
 }
 
 // ADI.cpp:40-49 on its own (API path)
@@ -493,7 +506,7 @@ __global__ __launch_bounds__(64) void k_adi_exact(ChainTab ct, Live lv, const in
                                                   const double* __restrict__ sigma, double* __restrict__ r,
                                                   T* __restrict__ scratch, int stride, double* __restrict__ dq,
                                                   int dqmode) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= nch) return;
   const int ch = chains[t];
   if (lv.chain && !lv.chain[ch]) return;
@@ -964,7 +977,8 @@ __global__ __launch_bounds__(MAXT) void k_adi_fast(ChainTab ct, Live lv, const i
                                                    const T* __restrict__ ff, int nProc, double cfl,
                                                    const double* __restrict__ sigma, double* __restrict__ r,
                                                    double* __restrict__ dq, int dqmode) {
-  for (int w = blockIdx.x; w < nch; w += gridDim.x) {
+  const int vb = xcd_block(blockIdx.x, gridDim.x);
+  for (int w = vb; w < nch; w += gridDim.x) {
     const int ch = chains[w];
     if (lv.chain && !lv.chain[ch]) continue;  // the same for the whole workgroup
     adi_fast_chain<T, FINAL>(ct, ch, ff, nProc, cfl, sigma, r, dq, dqmode);
@@ -980,7 +994,7 @@ template <int ORDER, typename TL, bool FASTJ>
 __global__ __launch_bounds__(kBlock) void k_lin_faces(FaceTab ft, Consts c, Live lv, const double* __restrict__ q,
                                                       const double* __restrict__ dq, const TL* __restrict__ ff,
                                                       double* __restrict__ lflux) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  const int p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (p >= c.nProc || (lv.face && !lv.face[p])) return;
   const int code = ft.pslot[p];
   const int s = code & 0x3fffffff;
@@ -1027,7 +1041,8 @@ __global__ __launch_bounds__(kBlock) void k_lin_cells(Consts c, Live lv, const i
                                                       const double* __restrict__ sigma, double* __restrict__ r,
                                                       double* __restrict__ q, double* __restrict__ partial,
                                                       double* __restrict__ rrho) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int vb = xcd_block(blockIdx.x, gridDim.x);
+  const int i = vb * blockDim.x + threadIdx.x;
   double sq = 0.0, mx = 0.0;
   if (i < c.nCell && (!lv.cell || lv.cell[i])) {
     const double4 a = *reinterpret_cast<const double4*>(r0 + 4 * (size_t)i);
@@ -1055,7 +1070,7 @@ __global__ __launch_bounds__(kBlock) void k_lin_cells(Consts c, Live lv, const i
       *reinterpret_cast<double4*>(q + 4 * (size_t)i) = qv;
     }
   }
-  block_norm_partial(sq, mx, partial);
+  block_norm_partial(sq, mx, partial, vb);
 }
 
 __global__ void k_update(int n4, Live lv, double* __restrict__ q, const double* __restrict__ dq) {
