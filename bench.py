@@ -60,6 +60,22 @@ def phase_face_counts(mesh):
     return [int(sum(per[:nl])), int(sum(per[nl:]))], int(sum(per))
 
 
+def measured_traffic(stage, nc, nr, order, lhs, adi):
+    """HBM bytes per launch of the stage's kernel from the committed rocprofv3
+    counter summary of this workload (FETCH_SIZE and WRITE_SIZE scaled by a
+    plain copy of known size), or None for another workload."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{lhs}_naca{nc}x{nr}_o{order}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        t = json.load(fh)
+    w = t.get("workload", {})
+    if (w.get("nc"), w.get("nr"), w.get("order"), w.get("lhs"), w.get("adi")) != (nc, nr, order, lhs, adi):
+        return None
+    st = t["stages"].get(stage)
+    return None if st is None else st["traffic_bytes"]
+
+
 def cpu_baseline(mesh, mach, alpha, order, cfl, budget_s=12.0, max_iters=30):
     """The reference's own code (oracle/_ref, compiled from its sources) on a
     bounded sample: `n` iterations of the same case from freestream, 1 core.
@@ -200,7 +216,9 @@ def main():
                                       if args.adi == "fast" else " (the reference's sequential block Thomas, "
                                       "bit-identical)")},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(dom, args.nc, args.nr, args.order, "fp64", args.adi),
+                     "algorithmic_bytes_per_launch": dom_bytes, "avg_launch_us": stages[dom] * 1e3},
         "iteration_roofline": {"bytes_per_cell_iter": total_b,
                                "achieved_gbs": value * total_b / 1e9,
                                "frac": value * total_b / 1e9 / HBM_PEAK_GBS},
