@@ -2375,9 +2375,10 @@ int run_steps(hamstr_ctx* x, int nsteps, double tol, int check_every, std::vecto
       x->Linfrhs = last[1];
       x->L2lin = last[2];
       x->Linflin = last[3];
-      if (flag && x->retire && nc > 1) {
-        // a batch keeps going: cases whose norms are no longer finite retire as
-        // diverged (their cells are disjoint from every other case's)
+      if (flag && x->retire) {
+        // with retirement on the run keeps going: cases whose norms are no
+        // longer finite retire as diverged (their cells are disjoint from every
+        // other case's)
         int found = 0;
         for (int j = 0; j < nc; j++) {
           if (x->case_status[j] != 0) continue;

@@ -223,10 +223,18 @@ __global__ void k_reduce(const float* __restrict__ part, int nz, long n, float* 
   out[i] = s;
 }
 
+// Adam with the step's learning rate (cosine from lr to lr_final over `steps`)
+// and bias corrections computed from the device step counter, so that a
+// captured training step can be replayed unchanged
 __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
-                       float* __restrict__ v, int n, float lr, float b1, float b2, float c1, float c2, float eps) {
+                       float* __restrict__ v, int n, const int* __restrict__ step, long t0, int steps, float lr0,
+                       float lr_final, float b1, float b2, float eps) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const int s = *step;
+  const float t = (float)(t0 + s + 1);
+  const float c1 = 1.0f - powf(b1, t), c2 = 1.0f - powf(b2, t);
+  const float lr = lr_final + 0.5f * (lr0 - lr_final) * (1.0f + cosf(3.14159265358979f * (float)s / (float)max(1, steps - 1)));
   const float gi = g[i];
   const float mi = b1 * m[i] + (1.0f - b1) * gi;
   const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
@@ -245,11 +253,13 @@ __global__ void k_w_bf16(const float* __restrict__ w, bf16* __restrict__ wb, bf1
   wtb[n * kH + k] = h;
 }
 
-// gather a training batch by a per-step hash of (seed, step, i)
+// gather a training batch by a per-step hash of (seed, step, i); the step
+// comes from the device counter
 __global__ void k_gather(const float* __restrict__ X, const float* __restrict__ T, long n, int B, unsigned seed,
-                         unsigned step, float* __restrict__ xb, float* __restrict__ tb) {
+                         const int* __restrict__ stepp, float* __restrict__ xb, float* __restrict__ tb) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= B) return;
+  const unsigned step = (unsigned)*stepp;
   unsigned long long h = ((unsigned long long)seed << 32) ^ ((unsigned long long)step * 0x9E3779B97F4A7C15ull) ^
                          (unsigned long long)i * 0xBF58476D1CE4E5B9ull;
   h ^= h >> 31;
@@ -260,6 +270,19 @@ __global__ void k_gather(const float* __restrict__ X, const float* __restrict__ 
   for (int k = 0; k < kIn; k++) xb[i * kIn + k] = X[j * kIn + k];
 #pragma unroll
   for (int k = 0; k < kOut; k++) tb[i * kOut + k] = T[j * kOut + k];
+}
+
+// end of a training step: the batch loss into losses[step] (losses may be
+// null), then step += 1
+__global__ void k_step_end(const float* __restrict__ lpart, int nlb, float* __restrict__ losses,
+                           int* __restrict__ step) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (losses) {
+    float s = 0.f;
+    for (int z = 0; z < nlb; z++) s += lpart[z];
+    losses[*step] = s;
+  }
+  *step += 1;
 }
 
 // features -> normalised -> (forward done by the caller) ; y normalised -> q
@@ -311,6 +334,8 @@ struct hamstr_mlp {
   float *X = nullptr, *T = nullptr;
   long n = 0;
   long adam_t = 0;
+  int* dstep = nullptr;  // device step counter of the running hamstr_mlp_train
+  bool use_graph = true;
   hipStream_t stream = nullptr;
 };
 
@@ -464,12 +489,19 @@ int hamstr_mlp_destroy(hamstr_mlp* M) {
                   M->z[2], M->z[3], M->h[0], M->h[1], M->h[2], M->h[3], M->X, M->T};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
+  if (M->dstep) (void)hipFree(M->dstep);
   if (M->stream) (void)hipStreamDestroy(M->stream);
   delete M;
   return HAMSTR_OK;
 }
 
 int hamstr_mlp_num_params(void) { return kParams; }
+
+int hamstr_mlp_set_graph(hamstr_mlp* M, int on) {
+  if (!M) return mfail(HAMSTR_ERR_ARG, "null argument");
+  M->use_graph = on != 0;
+  return HAMSTR_OK;
+}
 
 int hamstr_mlp_get_params(hamstr_mlp* M, float* host) {
   if (!M || !host) return mfail(HAMSTR_ERR_ARG, "null argument");
@@ -532,33 +564,67 @@ int hamstr_mlp_set_data(hamstr_mlp* M, long n, const float* x, const float* t) {
 
 // Adam on random batches of the training set, learning rate decaying from lr
 // to lr_final on a cosine; the batch losses stay on the device until the end.
+// One training step (gather, forward, backward, Adam, BF16 weight copies,
+// step counter) is captured as a HIP graph and replayed `steps` times.
 int hamstr_mlp_train(hamstr_mlp* M, int steps, int batch, float lr, float lr_final, unsigned seed, float* losses) {
   if (!M || steps < 0 || batch <= 0 || batch % 32 || M->n <= 0 || !(lr > 0) || !(lr_final >= 0))
     return mfail(HAMSTR_ERR_ARG, "bad arguments");
+  if (steps == 0) return HAMSTR_OK;
   int rc = ensure_cap(M, batch);
   if (rc) return rc;
+  if (!M->dstep) MCHK(hipMalloc(&M->dstep, sizeof(int)));
   float* dloss = nullptr;
-  if (losses && steps > 0) MCHK(hipMalloc(&dloss, sizeof(float) * steps));
+  if (losses) MCHK(hipMalloc(&dloss, sizeof(float) * steps));
+  MCHK(hipMemsetAsync(M->dstep, 0, sizeof(int), M->stream));
   const int nlb = (batch * kOut + 255) / 256;
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-  const double kPi = 3.14159265358979323846;
-  for (int s = 0; s < steps; s++) {
+  const long t0 = M->adam_t;
+  auto one_step = [&]() -> int {
     hipLaunchKernelGGL(k_gather, dim3((batch + 255) / 256), dim3(256), 0, M->stream, M->X, M->T, M->n, batch, seed,
-                       (unsigned)s, M->x, M->t);
-    if ((rc = forward(M, batch))) return rc;
-    if ((rc = backward(M, batch, nullptr))) return rc;
-    if (dloss)
-      hipLaunchKernelGGL(k_reduce, dim3(1), dim3(64), 0, M->stream, M->lpart, nlb, 1L, dloss + s);
-    M->adam_t++;
-    const float c1 = 1.0f - std::pow(b1, (float)M->adam_t), c2 = 1.0f - std::pow(b2, (float)M->adam_t);
-    const float lrs =
-        (float)(lr_final + 0.5 * (lr - lr_final) * (1.0 + std::cos(kPi * s / std::max(1, steps - 1))));
+                       M->dstep, M->x, M->t);
+    int r;
+    if ((r = forward(M, batch))) return r;
+    if ((r = backward(M, batch, nullptr))) return r;
     hipLaunchKernelGGL(k_adam, dim3((kParams + 255) / 256), dim3(256), 0, M->stream, M->p, M->g, M->m, M->v, kParams,
-                       lrs, b1, b2, c1, c2, eps);
-    if ((rc = refresh_bf16(M))) return rc;
+                       M->dstep, t0, steps, lr, lr_final, b1, b2, eps);
+    if ((r = refresh_bf16(M))) return r;
+    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, M->stream, M->lpart, nlb, dloss, M->dstep);
+    MCHK(hipGetLastError());
+    return HAMSTR_OK;
+  };
+  if (!M->use_graph) {
+    for (int st = 0; st < steps; st++)
+      if ((rc = one_step())) return rc;
+    M->adam_t += steps;
+    if (dloss) MCHK(hipMemcpyAsync(losses, dloss, sizeof(float) * steps, hipMemcpyDeviceToHost, M->stream));
+    MCHK(hipStreamSynchronize(M->stream));
+    if (dloss) (void)hipFree(dloss);
+    return HAMSTR_OK;
   }
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
+  MCHK(hipStreamBeginCapture(M->stream, hipStreamCaptureModeThreadLocal));
+  rc = one_step();
+  const hipError_t e = hipStreamEndCapture(M->stream, &g);
+  if (rc) {
+    if (e == hipSuccess && g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  MCHK(e);
+  const hipError_t ei = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  MCHK(ei);
+  for (int st = 0; st < steps; st++) {
+    const hipError_t el = hipGraphLaunch(ge, M->stream);
+    if (el != hipSuccess) {
+      (void)hipGraphExecDestroy(ge);
+      return mfail(HAMSTR_ERR_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(el));
+    }
+  }
+  M->adam_t += steps;
   if (dloss) MCHK(hipMemcpyAsync(losses, dloss, sizeof(float) * steps, hipMemcpyDeviceToHost, M->stream));
   MCHK(hipStreamSynchronize(M->stream));
+  (void)hipGraphExecDestroy(ge);
   if (dloss) (void)hipFree(dloss);
   return HAMSTR_OK;
 }

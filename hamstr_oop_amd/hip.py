@@ -34,7 +34,7 @@ EXPORTS = [
     "hamstr_set_graph", "hamstr_get_cells", "hamstr_set_cells",
     "hamstr_mlp_create", "hamstr_mlp_destroy", "hamstr_mlp_num_params", "hamstr_mlp_get_params",
     "hamstr_mlp_set_params", "hamstr_mlp_forward", "hamstr_mlp_loss_grad", "hamstr_mlp_set_data",
-    "hamstr_mlp_train", "hamstr_warm_start", "hamstr_mlp_last_error",
+    "hamstr_mlp_train", "hamstr_warm_start", "hamstr_mlp_last_error", "hamstr_mlp_set_graph",
 ]
 
 _dp = C.POINTER(C.c_double)

@@ -11,7 +11,9 @@ A case converges at the first iteration whose nonlinear L2 = sqrt(sum r_rho^2)
 / nCell is <= tol.  Some cases of this scheme settle into a limit cycle just
 above 1e-10 (the MUSCL limiter's eps = 1e-10 makes it non-smooth at extrema;
 computeRHS.cpp:268-271): a case whose norm has not halved within stall_window
-iterations is called stalled, in every run alike, and counted separately.
+iterations is called stalled, in every run alike, and counted separately.  A
+few cases blow up from the freestream at CFL 10 (a NaN sound speed, where the
+reference exits): they count as diverged.
 """
 from __future__ import annotations
 
@@ -56,12 +58,13 @@ def draw_table(n: int = 256, seed: int = SEED, sizes: Sequence = SIZES) -> List[
 class RunResult:
     seconds: float
     iters: np.ndarray       # iteration of convergence (or stall detection) per case
-    status: np.ndarray      # 1 converged, 2 stalled, 0 neither within max_iters
+    status: np.ndarray      # 1 converged, 2 stalled, 3 diverged, 0 none of these within max_iters
     cell_iters: float       # sum over cases of cells x iterations run
 
     def summary(self) -> dict:
         return {"seconds": round(self.seconds, 4), "converged": int((self.status == 1).sum()),
-                "stalled": int((self.status == 2).sum()), "unfinished": int((self.status == 0).sum()),
+                "stalled": int((self.status == 2).sum()), "diverged": int((self.status == 3).sum()),
+                "unfinished": int((self.status == 0).sum()),
                 "iterations_sum": int(self.iters[self.status > 0].sum()), "cell_iters": float(self.cell_iters)}
 
 
@@ -112,12 +115,15 @@ class BatchOutcome:
 
 def run_batched(specs: Sequence[CaseSpec], meshes, tol: float, max_iters: int, stall_window: int,
                 lhs: int = hip.LHS_FP32, check_every: int = 50, order: int = 3, cfl: float = 10.0,
-                init=None, autotune: bool = True, fallback: bool = True):
+                init=None, autotune: bool = True, fallback: bool = True, fp32_stall_window: int = 500):
     """All cases in one batched context.  init(batch, case_indices) sets the
     initial state (a warm start); it and the autotuner are timed with the
     iterations.  With an FP32 LHS, cases that stall or diverge are re-run with
-    the FP64 LHS (the plain ADI iteration is the fallback): a stalled case from
-    the state it reached, a diverged one from its initial state.
+    the FP64 LHS (the plain ADI iteration is the fallback and the judge): a
+    stalled case from the state it reached, a diverged one from its initial
+    state.  The FP32 pass calls a case stalled after fp32_stall_window
+    iterations without halving its norm; the FP64 pass applies stall_window,
+    the rule of the sequential FP64 runs.
     Returns (RunResult, BatchOutcome, info)."""
     n = len(specs)
     b = hip.Batch([(m, c.mach, c.alpha) for c, m in zip(specs, meshes)], order=order, cfl=cfl, lhs=lhs,
@@ -129,7 +135,8 @@ def run_batched(specs: Sequence[CaseSpec], meshes, tol: float, max_iters: int, s
         hip.autotune(b)
     if init is not None:
         init(b, list(range(n)))
-    b.set_retire(True, stall_window)
+    first_window = fp32_stall_window if (fallback and lhs == hip.LHS_FP32) else stall_window
+    b.set_retire(True, first_window)
     b.batch_step(max_iters, tol=tol, check_every=check_every)
     st, it = b.case_status()
     status, iters = st.copy(), it.astype(np.int64)

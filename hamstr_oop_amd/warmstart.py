@@ -121,6 +121,10 @@ class MLP:
         except Exception:
             pass
 
+    def set_graph(self, on: bool):
+        """Replay one captured training step per step (default) or launch eagerly."""
+        _chk(hip.lib().hamstr_mlp_set_graph(self.h, int(on)))
+
     def params(self) -> np.ndarray:
         p = np.zeros(self.N_PARAMS, np.float32)
         _chk(hip.lib().hamstr_mlp_get_params(self.h, _f(p)))

@@ -119,7 +119,7 @@ int hamstr_set_retire(hamstr_ctx *ctx, int on, int stall_window);
  * timing).  on = 0 launches every kernel from the host instead. */
 int hamstr_set_graph(hamstr_ctx *ctx, int on);
 /* status[k]: 0 live, 1 converged, 2 stalled, 3 diverged (with retirement on, a
- * batch retires a case whose norms stop being finite instead of failing with
+ * case whose norms stop being finite retires instead of the call failing with
  * HAMSTR_ERR_NAN); iters[k]: the iteration, counted from hamstr_set_retire, at
  * which case k converged or was found stalled or diverged. */
 int hamstr_case_status(const hamstr_ctx *ctx, int *status, long *iters);
@@ -256,6 +256,9 @@ int hamstr_mlp_loss_grad(hamstr_mlp *mlp, int n, const float *x, const float *t,
 int hamstr_mlp_set_data(hamstr_mlp *mlp, long n, const float *x, const float *t);
 int hamstr_mlp_train(hamstr_mlp *mlp, int steps, int batch, float lr, float lr_final, unsigned seed,
                      float *losses);
+/* hamstr_mlp_train replays one captured training step as a HIP graph (on by
+ * default); on = 0 launches its kernels from the host every step. */
+int hamstr_mlp_set_graph(hamstr_mlp *mlp, int on);
 /* q := conserved form of freestream + the MLP's predicted deviation, from raw
  * per-cell features feat[nCell*6] and the normalisation nrm[20] (feature
  * mean[6], std[6], target mean[4], std[4]); cells whose prediction is not a

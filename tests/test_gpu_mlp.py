@@ -73,6 +73,19 @@ def test_training_is_deterministic_and_reduces_loss(ws):
     assert np.all(np.isfinite(losses))
     assert losses[-20:].mean() < 0.1 * losses[:5].mean()
     np.testing.assert_array_equal(runs[0][1], runs[1][1])
+    # the captured training step replays exactly what eager launches do, over
+    # several calls (replays) with host work in between
+    mlps = [ws.MLP(seed=3), ws.MLP(seed=3)]
+    mlps[1].set_graph(False)
+    for m in mlps:
+        m.set_data(x, t)
+    for call in range(3):
+        la = mlps[0].train(50, batch=512, lr=1e-3, lr_final=1e-4, seed=9 + call)
+        lb = mlps[1].train(50, batch=512, lr=1e-3, lr_final=1e-4, seed=9 + call)
+        np.testing.assert_array_equal(la, lb)
+        np.testing.assert_array_equal(mlps[0].params(), mlps[1].params())
+    for m in mlps:
+        m.close()
 
 
 def test_warm_start_fallback_is_freestream(ws):
