@@ -131,7 +131,7 @@ def main():
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--sweep-cases", type=int, default=256)
     ap.add_argument("--train-cases", type=int, default=384)
-    ap.add_argument("--train-steps", type=int, default=10000)
+    ap.add_argument("--train-steps", type=int, default=20000)
     ap.add_argument("--stall-window", type=int, default=2000)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one plain 256 MiB device-to-device copy (counter calibration under rocprofv3)")

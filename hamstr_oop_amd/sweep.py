@@ -115,15 +115,15 @@ class BatchOutcome:
 
 def run_batched(specs: Sequence[CaseSpec], meshes, tol: float, max_iters: int, stall_window: int,
                 lhs: int = hip.LHS_FP32, check_every: int = 50, order: int = 3, cfl: float = 10.0,
-                init=None, autotune: bool = True, fallback: bool = True, fp32_stall_window: int = 500):
+                init=None, autotune: bool = True, fallback: bool = True):
     """All cases in one batched context.  init(batch, case_indices) sets the
     initial state (a warm start); it and the autotuner are timed with the
     iterations.  With an FP32 LHS, cases that stall or diverge are re-run with
     the FP64 LHS (the plain ADI iteration is the fallback and the judge): a
     stalled case from the state it reached, a diverged one from its initial
-    state.  The FP32 pass calls a case stalled after fp32_stall_window
-    iterations without halving its norm; the FP64 pass applies stall_window,
-    the rule of the sequential FP64 runs.
+    state.  Every pass applies the stall rule of the sequential runs (a shorter
+    window for the FP32 pass hands cases that are merely on a plateau to the
+    FP64 pass and costs more than it saves).
     Returns (RunResult, BatchOutcome, info)."""
     n = len(specs)
     b = hip.Batch([(m, c.mach, c.alpha) for c, m in zip(specs, meshes)], order=order, cfl=cfl, lhs=lhs,
@@ -135,8 +135,7 @@ def run_batched(specs: Sequence[CaseSpec], meshes, tol: float, max_iters: int, s
         hip.autotune(b)
     if init is not None:
         init(b, list(range(n)))
-    first_window = fp32_stall_window if (fallback and lhs == hip.LHS_FP32) else stall_window
-    b.set_retire(True, first_window)
+    b.set_retire(True, stall_window)
     b.batch_step(max_iters, tol=tol, check_every=check_every)
     st, it = b.case_status()
     status, iters = st.copy(), it.astype(np.int64)
