@@ -129,6 +129,7 @@ def main():
     ap.add_argument("--no-exact", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--weno", action="store_true", help="also run the 1024x384 WENO FP32-LHS error case")
     ap.add_argument("--sweep-cases", type=int, default=256)
     ap.add_argument("--train-cases", type=int, default=384)
     ap.add_argument("--train-steps", type=int, default=20000)
@@ -251,11 +252,42 @@ def main():
             out["fp32_lhs"]["time_to_tol"] = c32
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mesh, args.mach, args.alpha, args.order, args.cfl, args.cpu_budget)
+    if args.weno:
+        out["weno_transonic_fp32_lhs"] = weno_section(args)
     if args.copy_probe:
         out["copy_probe_bytes"] = copy_probe(256 << 20)
     if not args.no_sweep and args.sweep_cases > 0:
         out["sweep"] = sweep_section(args, with_cpu=not args.no_cpu)
     print(json.dumps(out), flush=True)
+
+
+def weno_section(args, nc=1024, nr=384, mach=0.8, alpha=1.25):
+    """WENO (order 5), transonic: NACA0012 1024x384 all-quad, M=0.8, alpha=1.25
+    deg.  FP64 and FP32-LHS runs from the freestream to L2 <= tol under the
+    sweep's stall rule; the FP32-LHS error is stated against the FP64 field."""
+    import numpy as np
+    from hamstr_oop_amd import hip, meshgen
+    mesh = meshgen.naca_ogrid(nc, nr, t=0.12)
+    res, qs = {}, {}
+    for name, lhs in (("fp64", hip.LHS_FP64), ("fp32_lhs", hip.LHS_FP32)):
+        s = hip.Solver(mesh, mach, alpha, 5, args.cfl, lhs=lhs, adi_mode=hip.ADI_FAST)
+        s.set_retire(True, args.stall_window)
+        s.sync()
+        t0 = time.perf_counter()
+        n, l2, _ = s.step(args.max_iters, tol=args.tol, check_every=50)
+        s.sync()
+        dt = time.perf_counter() - t0
+        st, it = s.case_status()
+        res[name] = {"status": ["running", "converged", "stalled", "diverged"][int(st[0])], "iterations": int(n),
+                     "seconds": dt, "cell_iters_per_s": mesh.n_cell * n / dt, "final_l2": float(l2[-1]),
+                     "min_l2": float(l2.min())}
+        qs[name] = s.get("q").reshape(-1, 4)
+        s.close()
+    scale = np.abs(qs["fp64"]).max(axis=0)
+    res["fp32_lhs"]["max_field_diff_vs_fp64_rel"] = float((np.abs(qs["fp32_lhs"] - qs["fp64"]) / scale).max())
+    res["workload"] = f"NACA0012 O-grid {nc}x{nr} all-quad ({mesh.n_cell} cells), order 5 (WENO), M={mach}, " \
+                      f"alpha={alpha}, CFL={args.cfl}"
+    return res
 
 
 def copy_probe(nbytes):
