@@ -157,6 +157,7 @@ def main():
         steps = args.steps if steps is None else steps
         s = hip.Solver(mesh, args.mach, args.alpha, args.order, args.cfl, lhs=lhs, adi_mode=mode)
         s.step(args.warmup, check_every=max(args.warmup, 1))
+        s.prepare_graph(steps)
         s.sync()
         t0 = time.perf_counter()
         n, l2r, _ = s.step(steps, check_every=steps)
@@ -175,6 +176,7 @@ def main():
     def converge(lhs, mode=None):
         mode = adi_mode if mode is None else mode
         s = hip.Solver(mesh, args.mach, args.alpha, args.order, args.cfl, lhs=lhs, adi_mode=mode)
+        s.prepare_graph(50)
         s.sync()
         t0 = time.perf_counter()
         n, l2r, _ = s.step(args.max_iters, tol=args.tol, check_every=50)

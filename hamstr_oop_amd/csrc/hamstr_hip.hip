@@ -52,6 +52,7 @@ int fail(int code, const std::string& msg) {
 constexpr int kBlock = 256;
 constexpr int kRedBlock = 256;
 constexpr int kGraphMaxIters = 64;  // iterations per captured graph
+constexpr int kMaxCR = 3;           // cyclic-reduction levels the autotuner tries before PCR
 
 inline int nblk(long n, int b = kBlock) { return (int)((n + b - 1) / b); }
 
@@ -774,7 +775,8 @@ enum { kFinalOpen = 0, kFinalPair = 1, kFinalOdd = 2 };
 template <typename T, int FINAL>
 __device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const T* __restrict__ ff, int nProc,
                                                double cfl, const double* __restrict__ sigma,
-                                               double* __restrict__ r, double* __restrict__ dq, int dqmode) {
+                                               double* __restrict__ r, double* __restrict__ dq, int dqmode,
+                                               int ncr) {
 #pragma clang fp contract(fast)
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* lds = reinterpret_cast<T*>(smem_raw);
@@ -783,10 +785,9 @@ __device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const 
   const int i = threadIdx.x;
   const bool active = i < N;
 
-  // the thread's own normalised row stays in registers across the steps
-  T a[16], c[16], f[4];
+  // every row, assembled and normalised, goes to LDS
   if (active) {
-    T B[16];
+    T a[16], c[16], f[4], B[16];
     int cell;
     assemble_row<T>(ct, ff, nProc, pb, nf, N, periodic, i, cfl, sigma, r, a, B, c, f, &cell);
     if (!periodic) {
@@ -804,13 +805,59 @@ __device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const 
   }
   __syncthreads();
 
-  // parallel cyclic reduction
-  int d = 1;
-  int n = N;  // cyclic subsystem size
-  while (periodic ? (n % 2 == 0 && n >= 4) : (d < N)) {
-    if (active) {
-      const int iL = periodic ? ((i - d) % N + N) % N : i - d;
-      const int iR = periodic ? (i + d) % N : i + d;
+  // (1) ncr levels of cyclic reduction: at stride d the survivors (rows at
+  // multiples of 2d) absorb rows i - d and i + d, which drop out.  The work is
+  // compacted: thread t updates survivor row t*2d, so only the first
+  // ceil(survivors/64) waves issue instructions.  A row is read only at the
+  // level that eliminates it, so one barrier per level suffices.
+  int d = 1;  // stride between the rows still in the system
+  int n = N;  // cycle length of the remaining system (closed chains)
+  int ncr_done = 0;
+  while (ncr_done < ncr && (periodic ? (n % 2 == 0 && n >= 4) : (d < N))) {
+    const int nsurv = (N + 2 * d - 1) / (2 * d);
+    if (i < nsurv) {
+      const int row = i * 2 * d;
+      const int iL = periodic ? (row - d + N) % N : row - d;
+      const int iR = periodic ? (row + d) % N : row + d;
+      T ra[16], rc[16], rf[4], M[16], na[16], nc[16];
+      lds_load_row(lds, N, row, ra, rc, rf);
+#pragma unroll
+      for (int e = 0; e < 16; e++) {
+        M[e] = (e % 5 == 0) ? T(1) : T(0);
+        na[e] = T(0);
+        nc[e] = T(0);
+      }
+      if (periodic || iL >= 0) {
+        mm4_acc_lds(M, ra, lds, N, 16, iL);  // M -= A^_i C^_L
+        mm4_acc_lds(na, ra, lds, N, 0, iL);  // A^new = -A^_i A^_L
+        mv4_acc_lds(rf, ra, lds, N, iL);     // F -= A^_i F^_L
+      }
+      if (periodic || iR < N) {
+        mm4_acc_lds(M, rc, lds, N, 0, iR);   // M -= C^_i A^_R
+        mm4_acc_lds(nc, rc, lds, N, 16, iR); // C^new = -C^_i C^_R
+        mv4_acc_lds(rf, rc, lds, N, iR);     // F -= C^_i F^_R
+      }
+      normalise_row(M, na, nc, rf);
+      lds_store_row(lds, N, row, na, nc, rf);
+    }
+    __syncthreads();
+    d *= 2;
+    n /= 2;
+    ncr_done++;
+  }
+
+  // (2) parallel cyclic reduction of the rows at multiples of d (members),
+  // thread t holding member row t*d in registers
+  const int nmem = (N + d - 1) / d;
+  const bool member = i < nmem;
+  const int mrow = i * d;
+  T a[16], c[16], f[4];
+  if (member) lds_load_row(lds, N, mrow, a, c, f);
+  int dd = d;
+  while (periodic ? (n % 2 == 0 && n >= 4) : (dd < N)) {
+    if (member) {
+      const int iL = periodic ? (mrow - dd + N) % N : mrow - dd;
+      const int iR = periodic ? (mrow + dd) % N : mrow + dd;
       T M[16], na[16], nc[16];
 #pragma unroll
       for (int e = 0; e < 16; e++) {
@@ -819,14 +866,14 @@ __device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const 
         nc[e] = T(0);
       }
       if (periodic || iL >= 0) {
-        mm4_acc_lds(M, a, lds, N, 16, iL);  // M -= A^_i C^_L
-        mm4_acc_lds(na, a, lds, N, 0, iL);  // A^new = -A^_i A^_L
-        mv4_acc_lds(f, a, lds, N, iL);      // F -= A^_i F^_L
+        mm4_acc_lds(M, a, lds, N, 16, iL);
+        mm4_acc_lds(na, a, lds, N, 0, iL);
+        mv4_acc_lds(f, a, lds, N, iL);
       }
       if (periodic || iR < N) {
-        mm4_acc_lds(M, c, lds, N, 0, iR);   // M -= C^_i A^_R
-        mm4_acc_lds(nc, c, lds, N, 16, iR); // C^new = -C^_i C^_R
-        mv4_acc_lds(f, c, lds, N, iR);      // F -= C^_i F^_R
+        mm4_acc_lds(M, c, lds, N, 0, iR);
+        mm4_acc_lds(nc, c, lds, N, 16, iR);
+        mv4_acc_lds(f, c, lds, N, iR);
       }
       normalise_row(M, na, nc, f);
 #pragma unroll
@@ -836,21 +883,21 @@ __device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const 
       }
     }
     __syncthreads();
-    if (active) lds_store_row(lds, N, i, a, c, f);
+    if (member) lds_store_row(lds, N, mrow, a, c, f);
     __syncthreads();
-    d *= 2;
+    dd *= 2;
     n /= 2;
   }
 
-  T x[4] = {T(0), T(0), T(0), T(0)};
+  // (3) the members' final systems; their solutions go to their F slots
   if (FINAL == kFinalOpen) {
-#pragma unroll
-    for (int e = 0; e < 4; e++) x[e] = f[e];
+    // decoupled: x = f, already in the F slot
   } else if (FINAL == kFinalPair) {
     // rows i and j = i + N/2 couple only to each other:
     // x_i = (I - P Q)^-1 (f_i - P f_j), P = A^_i + C^_i, Q = A^_j + C^_j
-    if (active) {
-      const int j = (i + N / 2) % N;
+    T x[4];
+    if (member) {
+      const int j = (mrow + N / 2) % N;
       T P[16], Q[16], M[16];
 #pragma unroll
       for (int e = 0; e < 16; e++) {
@@ -865,13 +912,19 @@ __device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const 
       const LU4<T> L = lu4_fast(M);
       lu4_solve_fast(L, x, 1);
     }
+    __syncthreads();
+    if (member) {
+#pragma unroll
+      for (int e = 0; e < 4; e++) lds[(32 + e) * N + mrow] = x[e];
+    }
+    __syncthreads();
   } else {
-    // odd cyclic subsystems {r, r+d, r+2d, ...} of n rows: the leader r < d
-    // runs the cyclic block Thomas on the normalised rows (B = I) in place
-    if (i < d) {
+    // odd cycles {r, r+dd, r+2dd, ...} of n rows: the leader r < dd runs the
+    // cyclic block Thomas on the normalised rows (B = I) in place
+    if (member && mrow < dd) {
       T Cp[16], Fp[4], ap[16];
       for (int k = 0; k < n; k++) {
-        const int rk = i + k * d;
+        const int rk = mrow + k * dd;
         T A[16], Bm[16], C[16], F[4], sp[16];
         lds_load_row(lds, N, rk, A, C, F);
 #pragma unroll
@@ -920,7 +973,7 @@ __device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const 
 #pragma unroll
       for (int e = 0; e < 4; e++) Fl[e] = Fp[e];
       for (int k = 0; k < n - 1; k++) {
-        const int rk = i + k * d;
+        const int rk = mrow + k * dd;
         T tt[16];
 #pragma unroll
         for (int e = 0; e < 16; e++) tt[e] = T(0);
@@ -935,7 +988,7 @@ __device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const 
       T xn[4] = {Fl[0], Fl[1], Fl[2], Fl[3]};
       // back sweep; the solution overwrites the F slots
       for (int k = n - 1; k >= 0; k--) {
-        const int rk = i + k * d;
+        const int rk = mrow + k * dd;
         T sf[4];
         if (k == n - 1) {
 #pragma unroll
@@ -961,12 +1014,33 @@ __device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const 
       }
     }
     __syncthreads();
-    if (active) {
-#pragma unroll
-      for (int e = 0; e < 4; e++) x[e] = lds[(32 + e) * N + i];
-    }
   }
-  if (active) store_solution(r, dq, dqmode, ct.pleft[pb + i], x);
+
+  // (4) back substitution through the cyclic-reduction levels, compacted:
+  // thread t takes row (2t+1)*dl, eliminated at stride dl, whose LDS row
+  // still holds its coefficients as eliminated: x = F^ - A^ x_(i-dl) - C^ x_(i+dl)
+  for (int l = ncr_done - 1; l >= 0; l--) {
+    const int dl = 1 << l;
+    const int nel = ((N + dl - 1) / dl) / 2;  // odd multiples of dl below N
+    if (i < nel) {
+      const int row = (2 * i + 1) * dl;
+      const int iL = periodic ? (row - dl + N) % N : row - dl;
+      const int iR = periodic ? (row + dl) % N : row + dl;
+      T ra[16], rc[16], rf[4];
+      lds_load_row(lds, N, row, ra, rc, rf);
+      if (periodic || iL >= 0) mv4_acc_lds(rf, ra, lds, N, iL);
+      if (periodic || iR < N) mv4_acc_lds(rf, rc, lds, N, iR);
+#pragma unroll
+      for (int e = 0; e < 4; e++) lds[(32 + e) * N + row] = rf[e];
+    }
+    __syncthreads();
+  }
+  if (active) {
+    T x[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) x[e] = lds[(32 + e) * N + i];
+    store_solution(r, dq, dqmode, ct.pleft[pb + i], x);
+  }
 }
 
 // one chain per workgroup pass; the workgroup walks the group's chains with a
@@ -976,12 +1050,12 @@ template <typename T, int FINAL, int MAXT>
 __global__ __launch_bounds__(MAXT) void k_adi_fast(ChainTab ct, Live lv, const int* __restrict__ chains, int nch,
                                                    const T* __restrict__ ff, int nProc, double cfl,
                                                    const double* __restrict__ sigma, double* __restrict__ r,
-                                                   double* __restrict__ dq, int dqmode) {
+                                                   double* __restrict__ dq, int dqmode, int ncr) {
   const int vb = xcd_block(blockIdx.x, gridDim.x);
   for (int w = vb; w < nch; w += gridDim.x) {
     const int ch = chains[w];
     if (lv.chain && !lv.chain[ch]) continue;  // the same for the whole workgroup
-    adi_fast_chain<T, FINAL>(ct, ch, ff, nProc, cfl, sigma, r, dq, dqmode);
+    adi_fast_chain<T, FINAL>(ct, ch, ff, nProc, cfl, sigma, r, dq, dqmode, ncr);
     __syncthreads();
   }
 }
@@ -1335,6 +1409,7 @@ struct Group {
   int maxN = 0;        // rows of the longest chain
   int grid = 0;        // workgroups of the launch (autotuned for FAST)
   int block = 64;      // threads per workgroup
+  int ncr = 0;         // FAST: cyclic-reduction levels before the parallel cyclic reduction (autotuned)
 };
 
 struct hamstr_ctx {
@@ -1485,7 +1560,7 @@ int launch_rhs_cells(hamstr_ctx* x, bool adi_init, double* normbase, double* par
   return part ? HAMSTR_OK : launch_norm(x, normbase, 0);
 }
 
-int launch_group(hamstr_ctx* x, const Group& g, int grid, int dqmode) {
+int launch_group(hamstr_ctx* x, const Group& g, int grid, int dqmode, int ncr) {
   const ChainTab ct = x->ctab();
   const int* chains = x->group_chains + g.off;
   if (g.fast) {
@@ -1493,7 +1568,7 @@ int launch_group(hamstr_ctx* x, const Group& g, int grid, int dqmode) {
     const size_t lds = (x->lhs32() ? sizeof(float) : sizeof(double)) * 36 * (size_t)g.maxN;
 #define FASTL(T, F, M)                                                                                          \
   hipLaunchKernelGGL((k_adi_fast<T, F, M>), gr, b, lds, x->stream, ct, x->live(), chains, g.n, (const T*)x->ff, x->nProc, \
-                     x->c.cfl, x->sigma, x->r, x->dq, dqmode)
+                     x->c.cfl, x->sigma, x->r, x->dq, dqmode, ncr)
     if (x->lhs32()) {
       if (g.final == kFinalOpen) FASTL(float, kFinalOpen, 1024);
       else if (g.final == kFinalPair) FASTL(float, kFinalPair, 1024);
@@ -1523,7 +1598,7 @@ int launch_sweep_phases(hamstr_ctx* x, int sweep, const std::function<int(const 
   for (size_t k = 0; k < x->groups.size(); k++) {
     const Group& g = x->groups[k];
     const bool fuse = (g.phase == x->nPhase - 1) && x->last_phase_all_cells;
-    const int rc = launch_group(x, g, g.grid, fuse ? (sweep == 0 ? 1 : 2) : 0);
+    const int rc = launch_group(x, g, g.grid, fuse ? (sweep == 0 ? 1 : 2) : 0, g.ncr);
     if (rc) return rc;
     const bool phase_end = (k + 1 == x->groups.size()) || x->groups[k + 1].phase != g.phase;
     if (mark && phase_end &&
@@ -1809,6 +1884,9 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
       g.maxN = ch_ncell[bucket[b][0]];
       g.block = g.fast ? ((g.maxN + 63) / 64) * 64 : 64;
       g.grid = g.fast ? g.n : nblk(g.n, 64);
+      // two levels of cyclic reduction before the PCR (one before an odd
+      // cycle); measured best or close to it on single meshes and batches
+      g.ncr = g.fast ? (b == 3 ? 1 : 2) : 0;
       for (int i : bucket[b]) group_chains.push_back(i);
       groups.push_back(g);
       maxGroupChains = std::max(maxGroupChains, (size_t)g.n);
@@ -2270,6 +2348,14 @@ int enqueue_block(hamstr_ctx* x, int count, double* base, const std::function<in
   return HAMSTR_OK;
 }
 
+// iterations per captured graph for a read-back interval: the largest divisor
+// of check_every up to kGraphMaxIters (0: no graph)
+int graph_block(int check_every) {
+  for (int d = std::min(check_every, kGraphMaxIters); d >= 2; d--)
+    if (check_every % d == 0) return d;
+  return 0;
+}
+
 // Captures K iterations into a graph whose norms go to ghist[K][nCase][4].
 int build_graph(hamstr_ctx* x, int K) {
   drop_graph(x);
@@ -2340,12 +2426,8 @@ int run_steps(hamstr_ctx* x, int nsteps, double tol, int check_every, std::vecto
   // blocks of K iterations between read-backs replay a captured graph
   int K = 0;
   if (x->use_graph && !x->timing && nsteps >= 2) {
-    for (int d = std::min(check_every, kGraphMaxIters); d >= 1; d--)
-      if (check_every % d == 0) {
-        K = d;
-        break;
-      }
-    if (K < 2 || nsteps < K) K = 0;
+    K = graph_block(check_every);
+    if (nsteps < K) K = 0;
     if (K && (!x->gexec || x->gK != K) && (rc = build_graph(x, K))) return rc;
   }
   int ran = 0, lastRead = 0;
@@ -2483,6 +2565,27 @@ int hamstr_batch_step(hamstr_ctx* x, int nsteps, double tol, int check_every, in
   return rc;
 }
 
+int hamstr_set_launch(hamstr_ctx* x, int group, int grid, int cr_levels) {
+  if (!x || group < 0 || group >= (int)x->groups.size()) return fail(HAMSTR_ERR_ARG, "bad launch group");
+  Group& g = x->groups[group];
+  if (!g.fast) return fail(HAMSTR_ERR_ARG, "launch shape of a non-FAST group");
+  if (grid < 1 || cr_levels < 0 || cr_levels > kMaxCR) return fail(HAMSTR_ERR_ARG, "bad grid or cyclic-reduction levels");
+  g.grid = grid;
+  g.ncr = cr_levels;
+  drop_graph(x);
+  return HAMSTR_OK;
+}
+
+int hamstr_prepare_graph(hamstr_ctx* x, int check_every) {
+  if (!x || check_every < 1) return fail(HAMSTR_ERR_ARG, "bad arguments");
+  const int K = graph_block(check_every);
+  if (!x->use_graph || K == 0 || (x->gexec && x->gK == K)) return HAMSTR_OK;
+  const int rc = build_graph(x, K);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return HAMSTR_OK;
+}
+
 int hamstr_set_graph(hamstr_ctx* x, int on) {
   if (!x) return fail(HAMSTR_ERR_ARG, "null context");
   x->use_graph = on != 0;
@@ -2557,22 +2660,24 @@ int hamstr_autotune(hamstr_ctx* x, int reps, double* best_ms, int cap, int* ngro
       std::vector<int> cand = {g.n};
       for (int w : {8, 4, 2, 1})
         if (w * cu < g.n) cand.push_back(w * cu);
-      for (int grid : cand) {
-        int rc = launch_group(x, g, grid, 0);  // warm-up
-        if (rc) return rc;
-        HIPCHK(hipEventRecord(e0, x->stream));
-        for (int i = 0; i < reps; i++)
-          if ((rc = launch_group(x, g, grid, 0))) return rc;
-        HIPCHK(hipEventRecord(e1, x->stream));
-        HIPCHK(hipEventSynchronize(e1));
-        float ms = 0;
-        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-        const double t = ms / reps;
-        if (best < 0 || t < best) {
-          best = t;
-          g.grid = grid;
+      for (int ncr = 0; ncr <= kMaxCR; ncr++)
+        for (int grid : cand) {
+          int rc = launch_group(x, g, grid, 0, ncr);  // warm-up
+          if (rc) return rc;
+          HIPCHK(hipEventRecord(e0, x->stream));
+          for (int i = 0; i < reps; i++)
+            if ((rc = launch_group(x, g, grid, 0, ncr))) return rc;
+          HIPCHK(hipEventRecord(e1, x->stream));
+          HIPCHK(hipEventSynchronize(e1));
+          float ms = 0;
+          HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+          const double t = ms / reps;
+          if (best < 0 || t < best) {
+            best = t;
+            g.grid = grid;
+            g.ncr = ncr;
+          }
         }
-      }
     }
     if (best_ms && k < cap) best_ms[k] = best;
     k++;
@@ -2589,13 +2694,14 @@ int hamstr_autotune(hamstr_ctx* x, int reps, double* best_ms, int cap, int* ngro
 }
 
 // launch groups: per group (phase, fast, final, chains, maxN, grid, block)
-int hamstr_launch_groups(const hamstr_ctx* x, int* out7, int cap, int* n) {
+int hamstr_launch_groups(const hamstr_ctx* x, int* out8, int cap, int* n) {
   if (!x || !n) return fail(HAMSTR_ERR_ARG, "null argument");
   int k = 0;
   for (const Group& g : x->groups) {
-    if (out7 && k < cap) {
-      int* o = out7 + 7 * k;
+    if (out8 && k < cap) {
+      int* o = out8 + 8 * k;
       o[0] = g.phase; o[1] = g.fast; o[2] = g.final; o[3] = g.n; o[4] = g.maxN; o[5] = g.grid; o[6] = g.block;
+      o[7] = g.ncr;
     }
     k++;
   }

@@ -31,7 +31,8 @@ EXPORTS = [
     "hamstr_muscl_deriv", "hamstr_weno_deriv", "hamstr_block_tridag4", "hamstr_block_tridag_periodic4",
     "hamstr_last_error", "hamstr_batch_create", "hamstr_batch_info", "hamstr_batch_step", "hamstr_autotune",
     "hamstr_launch_groups", "hamstr_case_freestream", "hamstr_set_retire", "hamstr_case_status",
-    "hamstr_set_graph", "hamstr_get_cells", "hamstr_set_cells",
+    "hamstr_set_graph", "hamstr_get_cells", "hamstr_set_cells", "hamstr_prepare_graph",
+    "hamstr_set_launch",
     "hamstr_mlp_create", "hamstr_mlp_destroy", "hamstr_mlp_num_params", "hamstr_mlp_get_params",
     "hamstr_mlp_set_params", "hamstr_mlp_forward", "hamstr_mlp_loss_grad", "hamstr_mlp_set_data",
     "hamstr_mlp_train", "hamstr_warm_start", "hamstr_mlp_last_error", "hamstr_mlp_set_graph",
@@ -200,6 +201,10 @@ class Solver:
     def set_graph(self, on: bool):
         """Replay captured HIP graphs between norm read-backs (default on)."""
         _chk(lib().hamstr_set_graph(self.h, int(on)))
+
+    def prepare_graph(self, check_every: int):
+        """Capture the graph step(…, check_every) replays, outside any timed region."""
+        _chk(lib().hamstr_prepare_graph(self.h, int(check_every)))
 
     def set_retire(self, on: bool = True, stall_window: int = 0):
         """Retire converged (and, with stall_window > 0, stalled) cases."""
@@ -371,9 +376,14 @@ def autotune(solver: Solver, reps: int = 3):
     return launch_groups(solver), best[: n.value]
 
 
+def set_launch(solver: Solver, group: int, grid: int, cr_levels: int):
+    """Persistent grid and cyclic-reduction levels of a FAST launch group."""
+    _chk(lib().hamstr_set_launch(solver.h, int(group), int(grid), int(cr_levels)))
+
+
 def launch_groups(solver: Solver):
-    out = np.zeros(7 * 64, np.int32)
+    out = np.zeros(8 * 64, np.int32)
     n = C.c_int()
     _chk(lib().hamstr_launch_groups(solver.h, _i(out), 64, C.byref(n)))
-    keys = ("phase", "fast", "final", "chains", "max_rows", "grid", "block")
-    return [dict(zip(keys, map(int, out[7 * k:7 * k + 7]))) for k in range(n.value)]
+    keys = ("phase", "fast", "final", "chains", "max_rows", "grid", "block", "cr_levels")
+    return [dict(zip(keys, map(int, out[8 * k:8 * k + 8]))) for k in range(n.value)]

@@ -118,6 +118,10 @@ int hamstr_set_retire(hamstr_ctx *ctx, int on, int stall_window);
  * of up to 64 iterations between norm read-backs (on by default; not while
  * timing).  on = 0 launches every kernel from the host instead. */
 int hamstr_set_graph(hamstr_ctx *ctx, int on);
+/* Captures now (instead of inside the first step call) the graph that
+ * hamstr_step / hamstr_batch_step use for read-backs every check_every steps;
+ * runs no iteration. */
+int hamstr_prepare_graph(hamstr_ctx *ctx, int check_every);
 /* status[k]: 0 live, 1 converged, 2 stalled, 3 diverged (with retirement on, a
  * case whose norms stop being finite retires instead of the call failing with
  * HAMSTR_ERR_NAN); iters[k]: the iteration, counted from hamstr_set_retire, at
@@ -125,13 +129,18 @@ int hamstr_set_graph(hamstr_ctx *ctx, int on);
 int hamstr_case_status(const hamstr_ctx *ctx, int *status, long *iters);
 
 /* Autotuner: times every FAST launch group of the ADI phases at several
- * persistent grid sizes (all chains; 8, 4, 2, 1 workgroups per CU) on the
- * current state, keeps the fastest, restores r and dq.  best_ms[k] (k < cap)
- * gets group k's best time (-1 for groups with nothing to tune). */
+ * persistent grid sizes (all chains; 8, 4, 2, 1 workgroups per CU) and 0 to 3
+ * levels of cyclic reduction before the parallel cyclic reduction, on the
+ * current state; keeps the fastest and restores r and dq.  best_ms[k]
+ * (k < cap) gets group k's best time (-1 for groups with nothing to tune). */
 int hamstr_autotune(hamstr_ctx *ctx, int reps, double *best_ms, int cap, int *ngroups);
-/* Launch groups, 7 ints each: phase, fast, final solve (0 open, 1 two-row
- * pairs, 2 odd cyclic), chains, max rows, grid, block. */
-int hamstr_launch_groups(const hamstr_ctx *ctx, int *out7, int cap, int *n);
+/* Launch groups, 8 ints each: phase, fast, final solve (0 open, 1 two-row
+ * pairs, 2 odd cyclic), chains, max rows, grid, block, cyclic-reduction
+ * levels. */
+int hamstr_launch_groups(const hamstr_ctx *ctx, int *out8, int cap, int *n);
+/* Sets FAST launch group `group`'s persistent grid and its cyclic-reduction
+ * levels (0..3) by hand, as the autotuner would. */
+int hamstr_set_launch(hamstr_ctx *ctx, int group, int grid, int cr_levels);
 
 /* SOLVER::initFlow (initFlow.cpp:12-93): q := freestream. */
 int hamstr_init_flow(hamstr_ctx *ctx);

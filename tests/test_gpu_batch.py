@@ -60,16 +60,46 @@ def test_batch_convergence_iterations(hip):
 
 
 def test_autotune_keeps_results(hip):
+    """The autotuner changes persistent grids (the same arithmetic) and the
+    cyclic-reduction levels of the FAST solve (the same systems, reduced in
+    another order): the states agree to rounding."""
     cases = _cases()
     b1 = hip.Batch(cases, order=3, cfl=10.0, adi_mode=hip.ADI_FAST)
     b2 = hip.Batch(cases, order=3, cfl=10.0, adi_mode=hip.ADI_FAST)
     b2.step(2)  # a state to tune on
     groups, best = hip.autotune(b2)
-    assert all(g["grid"] >= 1 for g in groups)
+    assert all(g["grid"] >= 1 and 0 <= g["cr_levels"] <= 3 for g in groups if g["fast"])
     b2.initFlow()
     b1.batch_step(8, check_every=8)
     b2.batch_step(8, check_every=8)
-    assert np.array_equal(b1.get("q"), b2.get("q"))
+    q1, q2 = b1.get("q").reshape(-1, 4), b2.get("q").reshape(-1, 4)
+    assert (np.abs(q1 - q2) / np.abs(q1).max(axis=0)).max() < 1e-11
+    # grids alone do not change the arithmetic
+    b3 = hip.Batch(cases, order=3, cfl=10.0, adi_mode=hip.ADI_FAST)
+    for k, g in enumerate(hip.launch_groups(b3)):
+        if g["fast"]:
+            hip.set_launch(b3, k, max(1, g["chains"] // 3), g["cr_levels"])
+    b3.batch_step(8, check_every=8)
+    assert np.array_equal(b3.get("q"), q1.ravel())
+
+
+@pytest.mark.parametrize("cr", [0, 1, 2, 3])
+def test_cyclic_reduction_levels_match_exact(hip, cr):
+    """Every number of cyclic-reduction levels before the PCR solves the same
+    block systems as the reference's sequential order, to rounding: open
+    chains, two-row cycles (16 and 64 rows) and odd cycles (24 rows)."""
+    cases = _cases()
+    fast = hip.Batch(cases, order=3, cfl=10.0, adi_mode=hip.ADI_FAST)
+    exact = hip.Batch(cases, order=3, cfl=10.0, adi_mode=hip.ADI_EXACT)
+    for k, g in enumerate(hip.launch_groups(fast)):
+        if g["fast"]:
+            hip.set_launch(fast, k, g["grid"], cr)
+    for b in (fast, exact):
+        b.computeRHS("implicit")
+        b.ADI()
+    xe, xf = exact.get("dq").reshape(-1, 4), fast.get("dq").reshape(-1, 4)
+    scale = np.abs(xe).max(axis=0)
+    assert (np.abs(xf - xe) / scale).max() < 1e-11
 
 
 @pytest.mark.parametrize("mode", ["exact", "fast", "fast32"])
