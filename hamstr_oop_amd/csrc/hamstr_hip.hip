@@ -1166,16 +1166,22 @@ __global__ void k_init_flow(int nCell, const int* __restrict__ cell_case, const 
 
 // per-case density-residual norms, one block per case and stage, cells in
 // order: stage s = slot/2 + blockIdx.y reads rrho + s*nCell and writes
-// out[4*k + 2*s] = L2, out[4*k + 2*s + 1] = Linf
+// out[4*k + 2*s] = L2, out[4*k + 2*s + 1] = Linf.  keep[4*k + ...] holds each
+// case's last norms; a retired case repeats them instead of reducing again.
 __global__ __launch_bounds__(kRedBlock) void k_case_norms(const double* __restrict__ rrho_all, int nCell,
                                                           const int* __restrict__ case_off, int slot,
-                                                          double* __restrict__ out) {
+                                                          double* __restrict__ out, Live lv,
+                                                          double* __restrict__ keep) {
   __shared__ double ssq[kRedBlock];
   __shared__ double smx[kRedBlock];
   const int k = blockIdx.x, t = threadIdx.x;
   const int stage = slot / 2 + blockIdx.y;
   const double* rrho = rrho_all + (size_t)stage * nCell;
   const int c0 = case_off[k], c1 = case_off[k + 1];
+  if (lv.cell && !lv.cell[c0]) {  // the same for the whole block
+    if (t < 2) out[4 * k + 2 * stage + t] = keep[4 * k + 2 * stage + t];
+    return;
+  }
   double a = 0.0, m = 0.0;
   for (int i = c0 + t; i < c1; i += kRedBlock) {
     const double v = rrho[i];
@@ -1193,8 +1199,11 @@ __global__ __launch_bounds__(kRedBlock) void k_case_norms(const double* __restri
     __syncthreads();
   }
   if (t == 0) {
-    out[4 * k + 2 * stage] = sqrt(ssq[0]) / (c1 - c0);
+    const double l2 = sqrt(ssq[0]) / (c1 - c0);
+    out[4 * k + 2 * stage] = l2;
     out[4 * k + 2 * stage + 1] = smx[0];
+    keep[4 * k + 2 * stage] = l2;
+    keep[4 * k + 2 * stage + 1] = smx[0];
   }
 }
 
@@ -1433,6 +1442,7 @@ struct hamstr_ctx {
   // raw density residual per cell for the per-case norms of batches:
   // [0, nCell) nonlinear, [nCell, 2 nCell) linear (a retired case keeps both)
   double* rrho = nullptr;
+  double* keep_norms = nullptr;  // batches: [nCase][4] last norms per case
   int* case_off = nullptr;
   std::vector<int> case_off_h;
   int *pleft = nullptr, *pright = nullptr, *pface = nullptr;
@@ -1540,7 +1550,7 @@ int launch_norm(hamstr_ctx* x, double* base, int slot) {
                        base + slot);
   else
     hipLaunchKernelGGL(k_case_norms, dim3(x->nCase, 1), dim3(kRedBlock), 0, x->stream, x->rrho, x->nCell, x->case_off,
-                       slot, base);
+                       slot, base, x->live(), x->keep_norms);
   HIPCHK(hipGetLastError());
   return HAMSTR_OK;
 }
@@ -1667,7 +1677,7 @@ void free_ctx(hamstr_ctx* x) {
                   x->nvec, x->refmtx, x->qinf, x->cell_case, x->case_off, x->pleft, x->pright, x->pface,
                   x->cell_off, x->cell_ent, x->ch_pbase, x->ch_nf, x->ch_ncell, x->ch_periodic, x->group_chains,
                   x->scratch, x->partial, x->norms, x->hist, x->nanflag, x->rrho, x->face_on, x->cell_on,
-                  x->chain_on, x->ghist, x->npart};
+                  x->chain_on, x->ghist, x->npart, x->keep_norms};
   if (x->gexec) (void)hipGraphExecDestroy(x->gexec);
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -1964,6 +1974,7 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
     chk(hipMalloc(&x->npart, sizeof(double) * 2 * 2 * (size_t)kGraphMaxIters * x->nPart), "alloc norm partials");
   chk(hipMalloc(&x->norms, sizeof(double) * 4 * nCase), "alloc norms");
   if (nCase > 1) chk(hipMalloc(&x->rrho, sizeof(double) * 2 * (size_t)nCell), "alloc rrho");
+  if (nCase > 1) chk(hipMalloc(&x->keep_norms, sizeof(double) * 4 * (size_t)nCase), "alloc case norms");
   chk(hipMalloc(&x->face_on, nProc), "alloc face mask");
   chk(hipMalloc(&x->cell_on, nCell), "alloc cell mask");
   chk(hipMalloc(&x->chain_on, nChain), "alloc chain mask");
@@ -1995,6 +2006,7 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
   chk(hipMemset(x->face_on, 1, nProc), "memset face mask");
   chk(hipMemset(x->cell_on, 1, nCell), "memset cell mask");
   chk(hipMemset(x->chain_on, 1, nChain), "memset chain mask");
+  if (x->keep_norms) chk(hipMemset(x->keep_norms, 0, sizeof(double) * 4 * (size_t)nCase), "memset case norms");
   if (rc) {
     free_ctx(x);
     delete x;
@@ -2325,7 +2337,7 @@ int enqueue_iteration(hamstr_ctx* x, double* nb, const std::function<int(const c
   }
   if (!one) {
     hipLaunchKernelGGL(k_case_norms, dim3(x->nCase, 2), dim3(kRedBlock), 0, x->stream, x->rrho, x->nCell,
-                       x->case_off, 0, nb);
+                       x->case_off, 0, nb, x->live(), x->keep_norms);
     HIPCHK(hipGetLastError());
     if (mark && mark("norms")) return HAMSTR_ERR_HIP;
   }
@@ -2616,6 +2628,31 @@ int hamstr_set_retire(hamstr_ctx* x, int on, int stall_window) {
   }
   return HAMSTR_OK;
 }
+
+int hamstr_set_case_live(hamstr_ctx* x, int k, int live) {
+  if (!x || k < 0 || k >= x->nCase) return fail(HAMSTR_ERR_ARG, "bad arguments");
+  if (!x->retire) return fail(HAMSTR_ERR_ARG, "hamstr_set_case_live needs retirement on (hamstr_set_retire)");
+  if (!live) {
+    if (x->case_status[k] == 0) return retire_case(x, k, 4, x->it_total);
+    return HAMSTR_OK;
+  }
+  auto fill = [&](unsigned char* p, int a, int b) {
+    if (b > a)
+      hipLaunchKernelGGL(k_fill_u8, dim3(nblk(b - a)), dim3(kBlock), 0, x->stream, p + a, (long)(b - a),
+                         (unsigned char)1);
+  };
+  fill(x->face_on, x->case_proc_off[k], x->case_proc_off[k + 1]);
+  fill(x->cell_on, x->case_off_h[k], x->case_off_h[k + 1]);
+  fill(x->chain_on, x->case_chain_off[k], x->case_chain_off[k + 1]);
+  HIPCHK(hipGetLastError());
+  x->case_status[k] = 0;
+  x->case_iters[k] = -1;
+  x->stall_ref[k] = HUGE_VAL;
+  x->stall_ref_it[k] = x->it_total;
+  return HAMSTR_OK;
+}
+
+long hamstr_iterations(const hamstr_ctx* x) { return x ? x->it_total : -1; }
 
 int hamstr_case_status(const hamstr_ctx* x, int* status, long* iters) {
   if (!x) return fail(HAMSTR_ERR_ARG, "null context");

@@ -32,7 +32,7 @@ EXPORTS = [
     "hamstr_last_error", "hamstr_batch_create", "hamstr_batch_info", "hamstr_batch_step", "hamstr_autotune",
     "hamstr_launch_groups", "hamstr_case_freestream", "hamstr_set_retire", "hamstr_case_status",
     "hamstr_set_graph", "hamstr_get_cells", "hamstr_set_cells", "hamstr_prepare_graph",
-    "hamstr_set_launch",
+    "hamstr_set_launch", "hamstr_set_case_live", "hamstr_iterations",
     "hamstr_mlp_create", "hamstr_mlp_destroy", "hamstr_mlp_num_params", "hamstr_mlp_get_params",
     "hamstr_mlp_set_params", "hamstr_mlp_forward", "hamstr_mlp_loss_grad", "hamstr_mlp_set_data",
     "hamstr_mlp_train", "hamstr_warm_start", "hamstr_mlp_last_error", "hamstr_mlp_set_graph",
@@ -69,6 +69,7 @@ def lib() -> C.CDLL:
             raise FileNotFoundError(f"{LIB_PATH} is not built: run python -m hamstr_oop_amd.build")
         L = C.CDLL(LIB_PATH)
         L.hamstr_last_error.restype = C.c_char_p
+        L.hamstr_iterations.restype = C.c_long
         L.hamstr_create.argtypes = [C.POINTER(MeshDesc), C.POINTER(Params), C.POINTER(C.c_void_p)]
         L.hamstr_batch_create.argtypes = [C.c_int, C.POINTER(MeshDesc), C.POINTER(Params), C.POINTER(C.c_void_p)]
         for name in EXPORTS:
@@ -209,6 +210,14 @@ class Solver:
     def set_retire(self, on: bool = True, stall_window: int = 0):
         """Retire converged (and, with stall_window > 0, stalled) cases."""
         _chk(lib().hamstr_set_retire(self.h, int(on), int(stall_window)))
+
+    def set_case_live(self, k: int, live: bool):
+        """Hold case k out of the launches, or make it live again (retirement on)."""
+        _chk(lib().hamstr_set_case_live(self.h, int(k), int(live)))
+
+    def iterations(self) -> int:
+        """Iterations run since set_retire."""
+        return int(lib().hamstr_iterations(self.h))
 
     def case_status(self):
         """(status[k]: 0 live, 1 converged, 2 stalled; iteration of that event or -1)."""

@@ -122,11 +122,18 @@ int hamstr_set_graph(hamstr_ctx *ctx, int on);
  * hamstr_step / hamstr_batch_step use for read-backs every check_every steps;
  * runs no iteration. */
 int hamstr_prepare_graph(hamstr_ctx *ctx, int check_every);
-/* status[k]: 0 live, 1 converged, 2 stalled, 3 diverged (with retirement on, a
+/* status[k]: 0 live, 1 converged, 2 stalled, 3 diverged, 4 held (with retirement on, a
  * case whose norms stop being finite retires instead of the call failing with
  * HAMSTR_ERR_NAN); iters[k]: the iteration, counted from hamstr_set_retire, at
  * which case k converged or was found stalled or diverged. */
 int hamstr_case_status(const hamstr_ctx *ctx, int *status, long *iters);
+/* With retirement on: live = 0 holds case k out of every launch (status 4),
+ * live = 1 makes it live again from its current state, with a fresh stall
+ * window (to continue a case another context gave up on, after
+ * hamstr_set_cells).  hamstr_iterations: iterations run since
+ * hamstr_set_retire. */
+int hamstr_set_case_live(hamstr_ctx *ctx, int k, int live);
+long hamstr_iterations(const hamstr_ctx *ctx);
 
 /* Autotuner: times every FAST launch group of the ADI phases at several
  * persistent grid sizes (all chains; 8, 4, 2, 1 workgroups per CU) and 0 to 3

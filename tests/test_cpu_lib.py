@@ -15,7 +15,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def _declared():
     txt = open(os.path.join(ROOT, "include", "hamstr_hip.h")).read()
-    return sorted(set(re.findall(r"^(?:int|const char \*)\s*(hamstr_\w+)\(", txt, re.M)))
+    return sorted(set(re.findall(r"^(?:int|long|const char \*)\s*(hamstr_\w+)\(", txt, re.M)))
 
 
 def test_library_exports_every_declared_symbol():

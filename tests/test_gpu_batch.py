@@ -214,3 +214,25 @@ def test_fp32_lhs_reaches_the_fp64_fixed_point(hip):
     d = (np.abs(qs[hip.LHS_FP32] - qs[hip.LHS_FP64]) / scale).max()
     print(f"FP32-LHS vs FP64-LHS at L2 <= 1e-13: max rel diff {d:.2e}", flush=True)
     assert d < 1e-6
+
+
+def test_held_case_resumes_exactly(hip):
+    """A case held out of the launches keeps its state; made live again it
+    continues exactly as if it had never been held, and the other cases run
+    on meanwhile as alone."""
+    cases = _cases()[:2]
+    b = hip.Batch(cases, order=3, cfl=10.0, adi_mode=hip.ADI_FAST)
+    b.set_retire(True)
+    b.set_case_live(1, False)
+    b.batch_step(20, tol=1e-30, check_every=10)
+    st, _ = b.case_status()
+    assert list(st) == [0, 4]
+    b.set_case_live(1, True)
+    b.batch_step(20, tol=1e-30, check_every=10)
+    assert b.iterations() == 40
+    for k, steps in ((0, 40), (1, 20)):
+        mesh, mach, alpha = cases[k]
+        s = hip.Solver(mesh, mach, alpha, 3, 10.0, adi_mode=hip.ADI_FAST)
+        s.step(steps, check_every=10)
+        assert np.array_equal(b.case_q(k).ravel(), s.get("q")), k
+        s.close()
