@@ -99,9 +99,9 @@ class BatchOutcome:
     """Final states of a batched run, per case (from the FP64 fallback context
     for the cases it re-ran)."""
 
-    def __init__(self, main, sub=None, redo=(), same_index=False):
+    def __init__(self, main, sub=None, redo=()):
         self.main, self.sub = main, sub
-        self.where = {k: (k if same_index else j) for j, k in enumerate(redo)}
+        self.where = {k: j for j, k in enumerate(redo)}
 
     def case_q(self, k: int) -> np.ndarray:
         if self.sub is not None and k in self.where:
@@ -119,30 +119,16 @@ def run_batched(specs: Sequence[CaseSpec], meshes, tol: float, max_iters: int, s
                 init=None, autotune: bool = True, fallback: bool = True):
     """All cases in one batched context.  init(batch, case_indices) sets the
     initial state (a warm start); it and the autotuner are timed with the
-    iterations.
-
-    With an FP32 LHS, the plain FP64 ADI iteration is the fallback and the
-    judge: a twin FP64 context holds every case out of its launches, and a
-    case the FP32 pass gives up on is made live there at once, on its own
-    stream, driven from a second host thread.  A stalled case continues from
-    the state it reached; a diverged one restarts from its initial state.
-    Every pass applies the stall rule of the sequential runs.
+    iterations.  With an FP32 LHS, cases that stall or diverge are re-run with
+    the FP64 LHS (the plain ADI iteration is the fallback and the judge): a
+    stalled case from the state it reached, a diverged one from its initial
+    state.  Every pass applies the stall rule of the sequential runs (a shorter
+    window for the FP32 pass hands cases that are merely on a plateau to the
+    FP64 pass and costs more than it saves).
     Returns (RunResult, BatchOutcome, info)."""
-    import queue as _queue
-    import threading
     n = len(specs)
     b = hip.Batch([(m, c.mach, c.alpha) for c, m in zip(specs, meshes)], order=order, cfl=cfl, lhs=lhs,
                   adi_mode=hip.ADI_FAST)
-    use_fb = fallback and lhs == hip.LHS_FP32
-    sub = None
-    if use_fb:
-        sub = hip.Batch([(m, c.mach, c.alpha) for c, m in zip(specs, meshes)], order=order, cfl=cfl,
-                        lhs=hip.LHS_FP64, adi_mode=hip.ADI_FAST)
-        sub.set_retire(True, stall_window)
-        for k in range(n):
-            sub.set_case_live(k, False)
-        sub.prepare_graph(check_every)
-        sub.sync()
     b.set_retire(True, stall_window)
     b.sync()
     t0 = time.perf_counter()
@@ -151,89 +137,40 @@ def run_batched(specs: Sequence[CaseSpec], meshes, tol: float, max_iters: int, s
         hip.autotune(b)
     if init is not None:
         init(b, list(range(n)))
-        if use_fb:
-            init(sub, list(range(n)))   # the restart state of cases that diverge
-    info = {"stalled_first_pass": 0, "diverged_first_pass": 0, "fallback_cases": 0}
-    handoff: "_queue.Queue" = _queue.Queue()
-    main_done = threading.Event()
-    started = {}
-    errors = []
-
-    def fallback_worker():
-        try:
-            live = set()
-            while True:
-                while True:
-                    try:
-                        k, q = handoff.get_nowait()
-                    except _queue.Empty:
-                        break
-                    if q is not None:
-                        sub.set_case_q(k, q)
-                    started[k] = sub.iterations()
-                    sub.set_case_live(k, True)
-                    live.add(k)
-                if live:
-                    sub.batch_step(check_every, tol=tol, check_every=check_every)
-                    st2, _ = sub.case_status()
-                    now = sub.iterations()
-                    for k in list(live):
-                        if st2[k] != 0:
-                            live.discard(k)
-                        elif now - started[k] >= max_iters:
-                            sub.set_case_live(k, False)   # unfinished within max_iters
-                            live.discard(k)
-                elif main_done.is_set() and handoff.empty():
-                    return
-                else:
-                    time.sleep(0.0005)
-        except Exception as e:  # reported by the caller
-            errors.append(e)
-
-    worker = threading.Thread(target=fallback_worker, daemon=True) if use_fb else None
-    if worker is not None:
-        worker.start()
-    handed = set()
-    while True:
-        b.batch_step(check_every, tol=tol, check_every=check_every)
-        st, it = b.case_status()
-        if use_fb:
-            for k in np.nonzero((st == 2) | (st == 3))[0]:
-                if k not in handed:
-                    handed.add(int(k))
-                    handoff.put((int(k), b.case_q(int(k)) if st[k] == 2 else None))
-        if (st != 0).all() or b.iterations() >= max_iters:
-            break
+    b.batch_step(max_iters, tol=tol, check_every=check_every)
     t_first = time.perf_counter() - t0
-    main_done.set()
-    if worker is not None:
-        worker.join()
-    if errors:
-        raise errors[0]
-    b.sync()
-    if sub is not None:
-        sub.sync()
-    sec = time.perf_counter() - t0
     st, it = b.case_status()
     status, iters = st.copy(), it.astype(np.int64)
     ncell = np.diff(b.cell_off).astype(np.float64)
-    done = np.where(st > 0, it, b.iterations())
-    cell_iters = float((ncell * done).sum())
-    info["stalled_first_pass"] = int((st == 2).sum())
-    info["diverged_first_pass"] = int((st == 3).sum())
-    redo = sorted(handed)
-    if redo:
-        st2, it2 = sub.case_status()
-        for k in redo:
-            run2 = (it2[k] if st2[k] > 0 and st2[k] != 4 else sub.iterations()) - started[k]
-            status[k] = st2[k] if st2[k] in (1, 2, 3) else 0
-            iters[k] = it[k] + run2
-            cell_iters += ncell[k] * run2
-        info["fallback_cases"] = len(redo)
-        info["fallback_converged"] = int(sum(1 for k in redo if st2[k] == 1))
+    cell_iters = float((ncell * np.where(st > 0, it, max_iters)).sum())
+    info = {"stalled_first_pass": int((st == 2).sum()), "diverged_first_pass": int((st == 3).sum()),
+            "fallback_cases": 0}
+    sub, redo = None, []
+    if fallback and lhs == hip.LHS_FP32:
+        redo = [k for k in range(n) if st[k] in (2, 3)]
+        if redo:
+            sub = hip.Batch([(meshes[k], specs[k].mach, specs[k].alpha) for k in redo], order=order, cfl=cfl,
+                            lhs=hip.LHS_FP64, adi_mode=hip.ADI_FAST)
+            if init is not None:
+                init(sub, redo)
+            for j, k in enumerate(redo):
+                if st[k] == 2:
+                    sub.set_case_q(j, b.case_q(k))
+            sub.set_retire(True, stall_window)
+            sub.batch_step(max_iters, tol=tol, check_every=check_every)
+            st2, it2 = sub.case_status()
+            for j, k in enumerate(redo):
+                run2 = it2[j] if st2[j] > 0 else max_iters
+                status[k] = st2[j]
+                iters[k] = it[k] + run2
+                cell_iters += ncell[k] * run2
+            info["fallback_cases"] = len(redo)
+            info["fallback_converged"] = int((st2 == 1).sum())
+    b.sync()
+    sec = time.perf_counter() - t0
     info["first_pass_s"] = round(t_first, 3)
-    info["fallback_tail_s"] = round(sec - t_first, 3)
-    return RunResult(sec, iters, status, cell_iters), BatchOutcome(b, sub, redo, same_index=True), info
+    info["fallback_s"] = round(sec - t_first, 3)
+    return RunResult(sec, iters, status, cell_iters), BatchOutcome(b, sub, redo), info
 
 
 def train_network(n_train: int, seed: int = SEED + 1, size=(256, 128), tol: float = 1e-8, max_iters: int = 20000,
