@@ -672,16 +672,18 @@ __global__ __launch_bounds__(64) void k_adi_exact(ChainTab ct, Live lv, const in
 }
 
 // --------------------------------------------------------------------------
-// ADI phase, FAST order: one workgroup per chain, one thread per block row.
-// The rows are assembled exactly as in the reference (assemble_row), each
-// row is normalised by its diagonal block (A^ = B^-1 A, C^ = B^-1 C,
-// F^ = B^-1 F), and parallel cyclic reduction in LDS eliminates the
-// neighbours at strides 1, 2, 4, ... until every row is decoupled (open
-// chains), the cyclic subsystems have two rows (solved directly) or an odd
-// number of rows (solved with the cyclic block Thomas of
-// triSolvers.cpp:850-1046 on the normalised rows).  The solution equals the
+// ADI phase, FAST order: one workgroup per chain.  The rows are assembled
+// exactly as in the reference (assemble_row), one thread per row, and
+// normalised by their diagonal block (A^ = B^-1 A, C^ = B^-1 C, F^ = B^-1 F).
+// Then `ncr` levels of block cyclic reduction (compacted: thread t updates
+// surviving row t*2d) halve the system, parallel cyclic reduction eliminates
+// the remaining rows' neighbours at strides d, 2d, 4d, ... until every row is
+// decoupled (open chains), the cyclic subsystems have two rows (solved
+// directly) or an odd number of rows (solved with the cyclic block Thomas of
+// triSolvers.cpp:850-1046 on the normalised rows), and back substitution
+// walks the cyclic-reduction levels in reverse.  The solution equals the
 // EXACT ordering up to rounding.  LDS holds the 36 normalised entries of every
-// row, element-major: lds[e*N + i].
+// row, element-major with a pad slot per 32 rows: lds[e*S + lpad(i)].
 // --------------------------------------------------------------------------
 template <typename T>
 __device__ __forceinline__ void mm4_acc(T* X, const T* A, const T* B) {
@@ -722,6 +724,12 @@ __device__ __forceinline__ void normalise_row(const T* B, T* a, T* c, T* f) {
   }
   lu4_solve_fast(L, f, 1);
 }
+
+// LDS row layout of the FAST solve: element e of row i at lds[e*S + lpad(i)],
+// S = lds_stride(N).  One pad slot per 32 rows makes the row strides of the
+// reduction levels (powers of two up to 32) free of bank conflicts.
+__host__ __device__ __forceinline__ int lpad(int i) { return i + (i >> 5); }
+__host__ __device__ __forceinline__ int lds_stride(int N) { return N + (N >> 5) + 1; }
 
 template <typename T>
 __device__ __forceinline__ void lds_load_row(const T* lds, int N, int i, T* a, T* c, T* f) {
@@ -784,6 +792,7 @@ __device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const 
   const bool periodic = FINAL != kFinalOpen;
   const int i = threadIdx.x;
   const bool active = i < N;
+  const int S = lds_stride(N);
 
   // every row, assembled and normalised, goes to LDS
   if (active) {
@@ -801,7 +810,7 @@ __device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const 
       }
     }
     normalise_row(B, a, c, f);
-    lds_store_row(lds, N, i, a, c, f);
+    lds_store_row(lds, S, lpad(i), a, c, f);
   }
   __syncthreads();
 
@@ -820,7 +829,7 @@ __device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const 
       const int iL = periodic ? (row - d + N) % N : row - d;
       const int iR = periodic ? (row + d) % N : row + d;
       T ra[16], rc[16], rf[4], M[16], na[16], nc[16];
-      lds_load_row(lds, N, row, ra, rc, rf);
+      lds_load_row(lds, S, lpad(row), ra, rc, rf);
 #pragma unroll
       for (int e = 0; e < 16; e++) {
         M[e] = (e % 5 == 0) ? T(1) : T(0);
@@ -828,17 +837,17 @@ __device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const 
         nc[e] = T(0);
       }
       if (periodic || iL >= 0) {
-        mm4_acc_lds(M, ra, lds, N, 16, iL);  // M -= A^_i C^_L
-        mm4_acc_lds(na, ra, lds, N, 0, iL);  // A^new = -A^_i A^_L
-        mv4_acc_lds(rf, ra, lds, N, iL);     // F -= A^_i F^_L
+        mm4_acc_lds(M, ra, lds, S, 16, lpad(iL));  // M -= A^_i C^_L
+        mm4_acc_lds(na, ra, lds, S, 0, lpad(iL));  // A^new = -A^_i A^_L
+        mv4_acc_lds(rf, ra, lds, S, lpad(iL));     // F -= A^_i F^_L
       }
       if (periodic || iR < N) {
-        mm4_acc_lds(M, rc, lds, N, 0, iR);   // M -= C^_i A^_R
-        mm4_acc_lds(nc, rc, lds, N, 16, iR); // C^new = -C^_i C^_R
-        mv4_acc_lds(rf, rc, lds, N, iR);     // F -= C^_i F^_R
+        mm4_acc_lds(M, rc, lds, S, 0, lpad(iR));   // M -= C^_i A^_R
+        mm4_acc_lds(nc, rc, lds, S, 16, lpad(iR)); // C^new = -C^_i C^_R
+        mv4_acc_lds(rf, rc, lds, S, lpad(iR));     // F -= C^_i F^_R
       }
       normalise_row(M, na, nc, rf);
-      lds_store_row(lds, N, row, na, nc, rf);
+      lds_store_row(lds, S, lpad(row), na, nc, rf);
     }
     __syncthreads();
     d *= 2;
@@ -852,7 +861,7 @@ __device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const 
   const bool member = i < nmem;
   const int mrow = i * d;
   T a[16], c[16], f[4];
-  if (member) lds_load_row(lds, N, mrow, a, c, f);
+  if (member) lds_load_row(lds, S, lpad(mrow), a, c, f);
   int dd = d;
   while (periodic ? (n % 2 == 0 && n >= 4) : (dd < N)) {
     if (member) {
@@ -866,14 +875,14 @@ __device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const 
         nc[e] = T(0);
       }
       if (periodic || iL >= 0) {
-        mm4_acc_lds(M, a, lds, N, 16, iL);
-        mm4_acc_lds(na, a, lds, N, 0, iL);
-        mv4_acc_lds(f, a, lds, N, iL);
+        mm4_acc_lds(M, a, lds, S, 16, lpad(iL));
+        mm4_acc_lds(na, a, lds, S, 0, lpad(iL));
+        mv4_acc_lds(f, a, lds, S, lpad(iL));
       }
       if (periodic || iR < N) {
-        mm4_acc_lds(M, c, lds, N, 0, iR);
-        mm4_acc_lds(nc, c, lds, N, 16, iR);
-        mv4_acc_lds(f, c, lds, N, iR);
+        mm4_acc_lds(M, c, lds, S, 0, lpad(iR));
+        mm4_acc_lds(nc, c, lds, S, 16, lpad(iR));
+        mv4_acc_lds(f, c, lds, S, lpad(iR));
       }
       normalise_row(M, na, nc, f);
 #pragma unroll
@@ -883,7 +892,7 @@ __device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const 
       }
     }
     __syncthreads();
-    if (member) lds_store_row(lds, N, mrow, a, c, f);
+    if (member) lds_store_row(lds, S, lpad(mrow), a, c, f);
     __syncthreads();
     dd *= 2;
     n /= 2;
@@ -902,20 +911,20 @@ __device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const 
 #pragma unroll
       for (int e = 0; e < 16; e++) {
         P[e] = a[e] + c[e];
-        Q[e] = lds[e * N + j] + lds[(16 + e) * N + j];
+        Q[e] = lds[e * S + lpad(j)] + lds[(16 + e) * S + lpad(j)];
         M[e] = (e % 5 == 0) ? T(1) : T(0);
       }
       mm4_acc(M, P, Q);
 #pragma unroll
       for (int e = 0; e < 4; e++) x[e] = f[e];
-      mv4_acc_lds(x, P, lds, N, j);
+      mv4_acc_lds(x, P, lds, S, lpad(j));
       const LU4<T> L = lu4_fast(M);
       lu4_solve_fast(L, x, 1);
     }
     __syncthreads();
     if (member) {
 #pragma unroll
-      for (int e = 0; e < 4; e++) lds[(32 + e) * N + mrow] = x[e];
+      for (int e = 0; e < 4; e++) lds[(32 + e) * S + lpad(mrow)] = x[e];
     }
     __syncthreads();
   } else {
@@ -926,7 +935,7 @@ __device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const 
       for (int k = 0; k < n; k++) {
         const int rk = mrow + k * dd;
         T A[16], Bm[16], C[16], F[4], sp[16];
-        lds_load_row(lds, N, rk, A, C, F);
+        lds_load_row(lds, S, lpad(rk), A, C, F);
 #pragma unroll
         for (int e = 0; e < 16; e++) Bm[e] = (e % 5 == 0) ? T(1) : T(0);
         if (k != 0) {
@@ -955,7 +964,7 @@ __device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const 
 #pragma unroll
           for (int e = 0; e < 16; e++) sp[e] = C[e];
         }
-        lds_store_row(lds, N, rk, sp, C, F);  // row k holds (spike a', C', F')
+        lds_store_row(lds, S, lpad(rk), sp, C, F);  // row k holds (spike a', C', F')
 #pragma unroll
         for (int e = 0; e < 16; e++) {
           Cp[e] = C[e];
@@ -977,9 +986,9 @@ __device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const 
         T tt[16];
 #pragma unroll
         for (int e = 0; e < 16; e++) tt[e] = T(0);
-        mm4_acc_lds(tt, Cl, lds, N, 16, rk);
-        mm4_acc_lds(Bl, Cl, lds, N, 0, rk);
-        mv4_acc_lds(Fl, Cl, lds, N, rk);
+        mm4_acc_lds(tt, Cl, lds, S, 16, lpad(rk));
+        mm4_acc_lds(Bl, Cl, lds, S, 0, lpad(rk));
+        mv4_acc_lds(Fl, Cl, lds, S, lpad(rk));
 #pragma unroll
         for (int e = 0; e < 16; e++) Cl[e] = tt[e];
       }
@@ -996,20 +1005,20 @@ __device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const 
         } else {
           T sc[16];
 #pragma unroll
-          for (int e = 0; e < 4; e++) sf[e] = lds[(32 + e) * N + rk];
+          for (int e = 0; e < 4; e++) sf[e] = lds[(32 + e) * S + lpad(rk)];
 #pragma unroll
-          for (int e = 0; e < 16; e++) sc[e] = lds[(16 + e) * N + rk];
+          for (int e = 0; e < 16; e++) sc[e] = lds[(16 + e) * S + lpad(rk)];
           mv4_acc(sf, sc, xn);
           if (k != n - 2) {
 #pragma unroll
-            for (int e = 0; e < 16; e++) sc[e] = lds[e * N + rk];
+            for (int e = 0; e < 16; e++) sc[e] = lds[e * S + lpad(rk)];
             mv4_acc(sf, sc, Fl);
           }
         }
 #pragma unroll
         for (int e = 0; e < 4; e++) {
           xn[e] = sf[e];
-          lds[(32 + e) * N + rk] = sf[e];
+          lds[(32 + e) * S + lpad(rk)] = sf[e];
         }
       }
     }
@@ -1027,18 +1036,18 @@ __device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const 
       const int iL = periodic ? (row - dl + N) % N : row - dl;
       const int iR = periodic ? (row + dl) % N : row + dl;
       T ra[16], rc[16], rf[4];
-      lds_load_row(lds, N, row, ra, rc, rf);
-      if (periodic || iL >= 0) mv4_acc_lds(rf, ra, lds, N, iL);
-      if (periodic || iR < N) mv4_acc_lds(rf, rc, lds, N, iR);
+      lds_load_row(lds, S, lpad(row), ra, rc, rf);
+      if (periodic || iL >= 0) mv4_acc_lds(rf, ra, lds, S, lpad(iL));
+      if (periodic || iR < N) mv4_acc_lds(rf, rc, lds, S, lpad(iR));
 #pragma unroll
-      for (int e = 0; e < 4; e++) lds[(32 + e) * N + row] = rf[e];
+      for (int e = 0; e < 4; e++) lds[(32 + e) * S + lpad(row)] = rf[e];
     }
     __syncthreads();
   }
   if (active) {
     T x[4];
 #pragma unroll
-    for (int e = 0; e < 4; e++) x[e] = lds[(32 + e) * N + i];
+    for (int e = 0; e < 4; e++) x[e] = lds[(32 + e) * S + lpad(i)];
     store_solution(r, dq, dqmode, ct.pleft[pb + i], x);
   }
 }
@@ -1575,7 +1584,7 @@ int launch_group(hamstr_ctx* x, const Group& g, int grid, int dqmode, int ncr) {
   const int* chains = x->group_chains + g.off;
   if (g.fast) {
     const dim3 gr(grid), b(g.block);
-    const size_t lds = (x->lhs32() ? sizeof(float) : sizeof(double)) * 36 * (size_t)g.maxN;
+    const size_t lds = (x->lhs32() ? sizeof(float) : sizeof(double)) * 36 * (size_t)lds_stride(g.maxN);
 #define FASTL(T, F, M)                                                                                          \
   hipLaunchKernelGGL((k_adi_fast<T, F, M>), gr, b, lds, x->stream, ct, x->live(), chains, g.n, (const T*)x->ff, x->nProc, \
                      x->c.cfl, x->sigma, x->r, x->dq, dqmode, ncr)
@@ -1871,7 +1880,7 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
     for (int i = 0; i < nChain; i++) {
       if (phase[i] != ph) continue;
       const int N = ch_ncell[i];
-      const bool fast_ok = P.adi_mode == HAMSTR_ADI_FAST && N <= maxt && tl * 36 * (size_t)N <= lds_max;
+      const bool fast_ok = P.adi_mode == HAMSTR_ADI_FAST && N <= maxt && tl * 36 * (size_t)lds_stride(N) <= lds_max;
       if (!fast_ok) {
         bucket[0].push_back(i);
       } else if (!ch_periodic[i]) {
