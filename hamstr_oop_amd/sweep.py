@@ -60,6 +60,19 @@ class RunResult:
     iters: np.ndarray       # iteration of convergence (or stall detection) per case
     status: np.ndarray      # 1 converged, 2 stalled, 3 diverged, 0 none of these within max_iters
     cell_iters: float       # sum over cases of cells x iterations run
+    case_seconds: Optional[np.ndarray] = None  # sequential: each case's solve time
+    conv_time: Optional[np.ndarray] = None     # batched: wall time from the start at which the case converged
+
+    def time_to_converge(self, cases) -> Optional[float]:
+        """Wall time to converge every case in `cases` (None if one of them did
+        not): the sum of their solve times for a sequential run, the time the
+        last of them converged for a batched one."""
+        cases = list(cases)
+        if not all(self.status[k] == 1 for k in cases):
+            return None
+        if self.case_seconds is not None:
+            return float(sum(self.case_seconds[k] for k in cases))
+        return float(max(self.conv_time[k] for k in cases)) if cases else 0.0
 
     def summary(self) -> dict:
         return {"seconds": round(self.seconds, 4), "converged": int((self.status == 1).sum()),
@@ -75,6 +88,7 @@ def run_sequential(specs: Sequence[CaseSpec], meshes, tol: float, max_iters: int
     iterations only (context set-up excluded)."""
     iters = np.full(len(specs), -1, np.int64)
     status = np.zeros(len(specs), np.int32)
+    secs = np.zeros(len(specs))
     total = 0.0
     cell_iters = 0.0
     for k, (c, mesh) in enumerate(zip(specs, meshes)):
@@ -85,14 +99,15 @@ def run_sequential(specs: Sequence[CaseSpec], meshes, tol: float, max_iters: int
         t0 = time.perf_counter()
         n, _, _ = s.step(max_iters, tol=tol, check_every=check_every)
         s.sync()
-        total += time.perf_counter() - t0
+        secs[k] = time.perf_counter() - t0
+        total += secs[k]
         st, it = s.case_status()
         status[k], iters[k] = st[0], it[0]
         cell_iters += mesh.n_cell * n
         s.close()
         if progress and (k + 1) % 32 == 0:
             progress(f"sequential: {k + 1}/{len(specs)} cases, {total:.1f} s")
-    return RunResult(total, iters, status, cell_iters)
+    return RunResult(total, iters, status, cell_iters, case_seconds=secs)
 
 
 class BatchOutcome:
@@ -137,7 +152,24 @@ def run_batched(specs: Sequence[CaseSpec], meshes, tol: float, max_iters: int, s
         hip.autotune(b)
     if init is not None:
         init(b, list(range(n)))
-    b.batch_step(max_iters, tol=tol, check_every=check_every)
+    conv_time = np.full(n, np.nan)
+
+    def stepped(ctx, idx, limit):
+        """Read-back-sized chunks until every case retired (or limit), noting
+        when each converged."""
+        ran = 0
+        while ran < limit:
+            done, _, _ = ctx.batch_step(min(check_every, limit - ran), tol=tol, check_every=check_every)
+            ran += done
+            stc, _ = ctx.case_status()
+            now = time.perf_counter() - t0
+            for j, k in enumerate(idx):
+                if stc[j] == 1 and np.isnan(conv_time[k]):
+                    conv_time[k] = now
+            if (stc != 0).all() or done == 0:
+                break
+
+    stepped(b, list(range(n)), max_iters)
     t_first = time.perf_counter() - t0
     st, it = b.case_status()
     status, iters = st.copy(), it.astype(np.int64)
@@ -157,7 +189,7 @@ def run_batched(specs: Sequence[CaseSpec], meshes, tol: float, max_iters: int, s
                 if st[k] == 2:
                     sub.set_case_q(j, b.case_q(k))
             sub.set_retire(True, stall_window)
-            sub.batch_step(max_iters, tol=tol, check_every=check_every)
+            stepped(sub, redo, max_iters)
             st2, it2 = sub.case_status()
             for j, k in enumerate(redo):
                 run2 = it2[j] if st2[j] > 0 else max_iters
@@ -170,7 +202,7 @@ def run_batched(specs: Sequence[CaseSpec], meshes, tol: float, max_iters: int, s
     sec = time.perf_counter() - t0
     info["first_pass_s"] = round(t_first, 3)
     info["fallback_s"] = round(sec - t_first, 3)
-    return RunResult(sec, iters, status, cell_iters), BatchOutcome(b, sub, redo), info
+    return RunResult(sec, iters, status, cell_iters, conv_time=conv_time), BatchOutcome(b, sub, redo), info
 
 
 def train_network(n_train: int, seed: int = SEED + 1, size=(256, 128), tol: float = 1e-8, max_iters: int = 20000,
