@@ -232,8 +232,10 @@ __global__ __launch_bounds__(kBlock) void k_rhs_faces(FaceTab ft, Consts c, Live
 // --------------------------------------------------------------------------
 // block reductions for the density-residual norms
 // --------------------------------------------------------------------------
-__device__ __forceinline__ void block_norm_partial(double sq, double mx, double* __restrict__ partial, int blk) {
-  // per-block partial (sum of r_rho^2, max |r_rho|); k_norm_final reduces them
+__device__ __forceinline__ void block_norm_partial(double sq, double mx, double* __restrict__ partial, int blk,
+                                                   double* __restrict__ partial2 = nullptr) {
+  // per-block partial (sum of r_rho^2, max |r_rho|); k_norm_final reduces them.
+  // Written to partial and, when given, to partial2 as well.
   __shared__ double ssq[kBlock];
   __shared__ double smx[kBlock];
   const int t = threadIdx.x;
@@ -250,6 +252,10 @@ __device__ __forceinline__ void block_norm_partial(double sq, double mx, double*
   if (t == 0) {
     partial[2 * blk] = ssq[0];
     partial[2 * blk + 1] = smx[0];
+    if (partial2) {
+      partial2[2 * blk] = ssq[0];
+      partial2[2 * blk + 1] = smx[0];
+    }
   }
 }
 
@@ -353,6 +359,71 @@ __global__ __launch_bounds__(kBlock) void k_rhs_cells(Consts c, Live lv, const i
     }
   }
   block_norm_partial(sq, mx, partial, vb);
+}
+
+// --------------------------------------------------------------------------
+// One explicit stage (march.cpp:73-108): computeRHS("explicit")'s cell sums,
+// summed as k_rhs_cells sums them, fused with up to two updateSolution calls
+// on this stage's r and sigma: dstA = srcA + (coefA*CFL/sigma) r, then, when
+// dstB is set, dstB = srcB + (coefB*CFL/sigma) r (rk3's first stage writes qt
+// and q from the same q).  r and sigma are stored as computeRHS leaves them.
+// Block norm partials go to partA and partB (either may be null; both null:
+// no norms), the raw density residual to rrhoA / rrhoB (batches).
+// --------------------------------------------------------------------------
+struct ExplicitStage {
+  const double* srcA;
+  double* dstA;
+  double coefA;
+  const double* srcB;
+  double* dstB;
+  double coefB;
+  double* partA;
+  double* partB;
+  double* rrhoA;
+  double* rrhoB;
+};
+
+__device__ __forceinline__ void explicit_update(const double* src, double* dst, double coef, double cfl, double sg,
+                                                double a0, double a1, double a2, double a3, int i) {
+  const double dtfac = coef * cfl / sg;
+  const double4 s = *reinterpret_cast<const double4*>(src + 4 * (size_t)i);
+  *reinterpret_cast<double4*>(dst + 4 * (size_t)i) =
+      make_double4(s.x + dtfac * a0, s.y + dtfac * a1, s.z + dtfac * a2, s.w + dtfac * a3);
+}
+
+__global__ __launch_bounds__(kBlock) void k_rhs_cells_explicit(Consts c, Live lv, const int* __restrict__ off,
+                                                               const int* __restrict__ ent,
+                                                               const double* __restrict__ flux,
+                                                               const double* __restrict__ sr, double* __restrict__ r,
+                                                               double* __restrict__ sigma, ExplicitStage st) {
+  const int vb = xcd_block(blockIdx.x, gridDim.x);
+  const int i = vb * blockDim.x + threadIdx.x;
+  double sq = 0.0, mx = 0.0;
+  if (i < c.nCell && (!lv.cell || lv.cell[i])) {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, sg = 0.0;
+    const int e1 = off[i + 1];
+    for (int e = off[i]; e < e1; e++) {
+      const int code = ent[e];
+      const int p = code >> 1;
+      const double4 fx = *reinterpret_cast<const double4*>(flux + 4 * (size_t)p);
+      if (code & 1) {
+        a0 += fx.x; a1 += fx.y; a2 += fx.z; a3 += fx.w;
+      } else {
+        a0 -= fx.x; a1 -= fx.y; a2 -= fx.z; a3 -= fx.w;
+      }
+      sg += sr[p];
+    }
+    sq = a0 * a0;
+    mx = fabs(a0);
+    if (st.rrhoA) st.rrhoA[i] = a0;
+    if (st.rrhoB) st.rrhoB[i] = a0;
+    sigma[i] = sg;
+    *reinterpret_cast<double4*>(r + 4 * (size_t)i) = make_double4(a0, a1, a2, a3);
+    explicit_update(st.srcA, st.dstA, st.coefA, c.cfl, sg, a0, a1, a2, a3, i);
+    if (st.dstB) explicit_update(st.srcB, st.dstB, st.coefB, c.cfl, sg, a0, a1, a2, a3, i);
+  }
+  // the condition is the same for every thread of the block
+  if (st.partA || st.partB) block_norm_partial(sq, mx, st.partA ? st.partA : st.partB, vb, st.partA ? st.partB : nullptr);
 }
 
 // ADI.cpp:40-49 on its own (API path)
@@ -1166,11 +1237,30 @@ __global__ void k_fill_u8(unsigned char* __restrict__ p, long n, unsigned char v
   if (i < n) p[i] = v;
 }
 
-// initFlow (initFlow.cpp:152-176): q := freestream of the cell's case
+// initFlow (initFlow.cpp:152-176): q := freestream of the cell's case, and
+// qt := q (initFlow.cpp:71-88)
 __global__ void k_init_flow(int nCell, const int* __restrict__ cell_case, const double4* __restrict__ qinf,
-                            double* __restrict__ q) {
+                            double* __restrict__ q, double* __restrict__ qt) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nCell) *reinterpret_cast<double4*>(q + 4 * (size_t)i) = qinf[cell_case[i]];
+  if (i < nCell) {
+    const double4 v = qinf[cell_case[i]];
+    *reinterpret_cast<double4*>(q + 4 * (size_t)i) = v;
+    *reinterpret_cast<double4*>(qt + 4 * (size_t)i) = v;
+  }
+}
+
+// SOLVER::updateSolution (march.cpp:142-164): dst = src + (coef*CFL/sigma) r,
+// per element in the reference's operation order.  src and dst may be the
+// same field (each element is read before it is written).
+__global__ void k_update_solution(int nCell, Live lv, const double* src, double* dst, const double* __restrict__ r,
+                                  const double* __restrict__ sigma, double coef, double cfl) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nCell || (lv.cell && !lv.cell[i])) return;
+  const double dtfac = coef * cfl / sigma[i];
+  const double4 a = *reinterpret_cast<const double4*>(src + 4 * (size_t)i);
+  const double4 rv = *reinterpret_cast<const double4*>(r + 4 * (size_t)i);
+  *reinterpret_cast<double4*>(dst + 4 * (size_t)i) =
+      make_double4(a.x + dtfac * rv.x, a.y + dtfac * rv.y, a.z + dtfac * rv.z, a.w + dtfac * rv.w);
 }
 
 // per-case density-residual norms, one block per case and stage, cells in
@@ -1440,6 +1530,8 @@ struct hamstr_ctx {
   std::vector<double> case_norm;  // [nCase][4] of the last read-back step
   // device arrays
   double *q = nullptr, *r = nullptr, *r0 = nullptr, *dq = nullptr, *sigma = nullptr;
+  double* qt = nullptr;  // rk3 stage-start state (SOLNBLOCK::qt)
+  int scheme = HAMSTR_SCHEME_ADI;
   double *flux = nullptr, *sr = nullptr, *lflux = nullptr;
   void* ff = nullptr;
   int* pslot = nullptr;
@@ -1682,7 +1774,7 @@ int read_norms(hamstr_ctx* x) {
 }
 
 void free_ctx(hamstr_ctx* x) {
-  void* ptrs[] = {x->q, x->r, x->r0, x->dq, x->sigma, x->flux, x->sr, x->lflux, x->ff, x->pslot, x->src,
+  void* ptrs[] = {x->q, x->qt, x->r, x->r0, x->dq, x->sigma, x->flux, x->sr, x->lflux, x->ff, x->pslot, x->src,
                   x->nvec, x->refmtx, x->qinf, x->cell_case, x->case_off, x->pleft, x->pright, x->pface,
                   x->cell_off, x->cell_ent, x->ch_pbase, x->ch_nf, x->ch_ncell, x->ch_periodic, x->group_chains,
                   x->scratch, x->partial, x->norms, x->hist, x->nanflag, x->rrho, x->face_on, x->cell_on,
@@ -1967,6 +2059,7 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
   chk(hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking), "hipStreamCreate");
   const size_t c4 = sizeof(double) * 4 * nCell;
   chk(hipMalloc(&x->q, c4), "alloc q");
+  chk(hipMalloc(&x->qt, c4), "alloc qt");
   chk(hipMalloc(&x->r, c4), "alloc r");
   chk(hipMalloc(&x->r0, c4), "alloc r0");
   chk(hipMalloc(&x->dq, c4), "alloc dq");
@@ -2154,7 +2247,7 @@ int hamstr_freestream(const hamstr_ctx* x, double* out4) {
 int hamstr_init_flow(hamstr_ctx* x) {
   if (!x) return fail(HAMSTR_ERR_ARG, "null context");
   hipLaunchKernelGGL(k_init_flow, dim3(nblk(x->nCell)), dim3(kBlock), 0, x->stream, x->nCell, x->cell_case, x->qinf,
-                     x->q);
+                     x->q, x->qt);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(x->stream));
   return HAMSTR_OK;
@@ -2166,6 +2259,7 @@ int hamstr_set_field(hamstr_ctx* x, int field, const double* host) {
   size_t n = 4 * (size_t)x->nCell;
   switch (field) {
     case HAMSTR_FIELD_Q: dst = x->q; break;
+    case HAMSTR_FIELD_QT: dst = x->qt; break;
     case HAMSTR_FIELD_R: dst = x->r; break;
     case HAMSTR_FIELD_R0: dst = x->r0; break;
     case HAMSTR_FIELD_DQ: dst = x->dq; break;
@@ -2198,6 +2292,7 @@ int hamstr_get_field(hamstr_ctx* x, int field, double* host) {
   size_t n = 4 * (size_t)x->nCell;
   switch (field) {
     case HAMSTR_FIELD_Q: srcp = x->q; break;
+    case HAMSTR_FIELD_QT: srcp = x->qt; break;
     case HAMSTR_FIELD_R: srcp = x->r; break;
     case HAMSTR_FIELD_R0: srcp = x->r0; break;
     case HAMSTR_FIELD_DQ: srcp = x->dq; break;
@@ -2223,13 +2318,14 @@ int hamstr_get_field(hamstr_ctx* x, int field, double* host) {
   return HAMSTR_OK;
 }
 
-// cells [first, first+count) of a per-cell field (q, r, r0, dq: 4 values per
-// cell; sigma: 1)
+// cells [first, first+count) of a per-cell field (q, qt, r, r0, dq: 4 values
+// per cell; sigma: 1)
 static int cell_range(hamstr_ctx* x, int field, int first, int count, double** dev, size_t* per) {
   if (!x || first < 0 || count < 0 || (long)first + count > x->nCell) return fail(HAMSTR_ERR_ARG, "bad cell range");
   *per = 4;
   switch (field) {
     case HAMSTR_FIELD_Q: *dev = x->q; break;
+    case HAMSTR_FIELD_QT: *dev = x->qt; break;
     case HAMSTR_FIELD_R: *dev = x->r; break;
     case HAMSTR_FIELD_R0: *dev = x->r0; break;
     case HAMSTR_FIELD_DQ: *dev = x->dq; break;
@@ -2286,6 +2382,21 @@ int hamstr_adi(hamstr_ctx* x, double* L2lin, double* Linflin) {
   return rc;
 }
 
+int hamstr_update_solution(hamstr_ctx* x, int src_field, int dst_field, double coef) {
+  if (!x) return fail(HAMSTR_ERR_ARG, "null context");
+  auto pick = [&](int f) -> double* {
+    return f == HAMSTR_FIELD_Q ? x->q : (f == HAMSTR_FIELD_QT ? x->qt : nullptr);
+  };
+  const double* src = pick(src_field);
+  double* dst = pick(dst_field);
+  if (!src || !dst) return fail(HAMSTR_ERR_ARG, "updateSolution works on q and qt");
+  hipLaunchKernelGGL(k_update_solution, dim3(nblk(x->nCell)), dim3(kBlock), 0, x->stream, x->nCell, x->live(), src,
+                     dst, x->r, x->sigma, coef, x->c.cfl);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(x->stream));
+  return HAMSTR_OK;
+}
+
 int hamstr_compute_linear_rhs(hamstr_ctx* x, double* L2lin, double* Linflin) {
   if (!x) return fail(HAMSTR_ERR_ARG, "null context");
   int rc = launch_linear(x, false, x->norms);
@@ -2324,15 +2435,69 @@ void drop_graph(hamstr_ctx* x) {
   x->gK = 0;
 }
 
-// one fused iteration (march.cpp:112-117).  Single case: the norms' block
-// partials go to iteration slot `slot` of x->npart (reduced per run of
-// iterations by enqueue_block); batches: per-case norms of both stages to
-// nb[nCase][4] at the end of the iteration.
+int enqueue_adi_iteration(hamstr_ctx* x, double* nb, const std::function<int(const char*)>& mark, double* p_nl,
+                          double* p_lin);
+
+// stepSolution's euler and rk3 branches (march.cpp:73-108): per stage one
+// face launch and one cell launch that also applies updateSolution.  The
+// first stage's density-residual norms (of the state the step starts from)
+// take the nonlinear slot, the last stage's (the norm marchInTime prints) the
+// second slot: single case as block partials p_first / p_last, batches as the
+// raw residual in rrho.
+int enqueue_explicit(hamstr_ctx* x, const std::function<int(const char*)>& mark, double* p_first, double* p_last) {
+  const bool one = x->nCase == 1;
+  double* rr_first = one ? nullptr : x->rrho;
+  double* rr_last = one ? nullptr : x->rrho + x->nCell;
+  const int nstage = x->scheme == HAMSTR_SCHEME_RK3 ? 3 : 1;
+  for (int s = 0; s < nstage; s++) {
+    ExplicitStage st{};
+    if (x->scheme == HAMSTR_SCHEME_EULER) {
+      st = ExplicitStage{x->q, x->q, 1., nullptr, nullptr, 0., p_first, p_last, rr_first, rr_last};
+    } else if (s == 0) {  // qt = q + 1/4 dt r, q = q + 8/15 dt r
+      st = ExplicitStage{x->q, x->qt, 0.25, x->q, x->q, 8.0 / 15, p_first, nullptr, rr_first, nullptr};
+    } else {  // q = qt + 5/12 dt r, then q = qt + 3/4 dt r
+      const bool last = s == 2;
+      st = ExplicitStage{x->qt, x->q, last ? 3.0 / 4.0 : 5.0 / 12, nullptr, nullptr, 0., last ? p_last : nullptr,
+                         nullptr, last ? rr_last : nullptr, nullptr};
+    }
+    int rc;
+    if ((rc = launch_rhs_faces(x, false))) return rc;
+    if (mark && mark("rhs_faces")) return HAMSTR_ERR_HIP;
+    hipLaunchKernelGGL(k_rhs_cells_explicit, dim3(nblk(x->nCell)), dim3(kBlock), 0, x->stream, x->c, x->live(),
+                       x->cell_off, x->cell_ent, x->flux, x->sr, x->r, x->sigma, st);
+    HIPCHK(hipGetLastError());
+    if (mark && mark("rhs_cells_update")) return HAMSTR_ERR_HIP;
+  }
+  return HAMSTR_OK;
+}
+
+// one fused iteration (march.cpp:64-136, the scheme's branch).  Single case:
+// the norms' block partials go to iteration slot `slot` of x->npart (reduced
+// per run of iterations by enqueue_block); batches: per-case norms of both
+// stages to nb[nCase][4] at the end of the iteration.
 int enqueue_iteration(hamstr_ctx* x, double* nb, const std::function<int(const char*)>& mark, int slot) {
   int rc;
   const bool one = x->nCase == 1;
   double* p_nl = one ? x->npart + (size_t)(2 * slot) * x->nPart * 2 : x->partial;
   double* p_lin = one ? x->npart + (size_t)(2 * slot + 1) * x->nPart * 2 : x->partial;
+  if (x->scheme != HAMSTR_SCHEME_ADI) {
+    if ((rc = enqueue_explicit(x, mark, one ? p_nl : nullptr, one ? p_lin : nullptr))) return rc;
+  } else if ((rc = enqueue_adi_iteration(x, nb, mark, p_nl, p_lin))) {
+    return rc;
+  }
+  if (!one) {
+    hipLaunchKernelGGL(k_case_norms, dim3(x->nCase, 2), dim3(kRedBlock), 0, x->stream, x->rrho, x->nCell,
+                       x->case_off, 0, nb, x->live(), x->keep_norms);
+    HIPCHK(hipGetLastError());
+    if (mark && mark("norms")) return HAMSTR_ERR_HIP;
+  }
+  return HAMSTR_OK;
+}
+
+// stepSolution's ADI branch (march.cpp:112-117) without the batch norms
+int enqueue_adi_iteration(hamstr_ctx* x, double* nb, const std::function<int(const char*)>& mark, double* p_nl,
+                          double* p_lin) {
+  int rc;
   if ((rc = launch_rhs_faces(x, true))) return rc;
   if (mark && mark("rhs_faces")) return HAMSTR_ERR_HIP;
   if ((rc = launch_rhs_cells(x, true, nb, p_nl))) return rc;
@@ -2343,12 +2508,6 @@ int enqueue_iteration(hamstr_ctx* x, double* nb, const std::function<int(const c
     if (mark && mark("lin_faces")) return HAMSTR_ERR_HIP;
     if ((rc = launch_lin_cells(x, sw == x->prm.msweep - 1, nb, p_lin))) return rc;
     if (mark && mark("lin_cells")) return HAMSTR_ERR_HIP;
-  }
-  if (!one) {
-    hipLaunchKernelGGL(k_case_norms, dim3(x->nCase, 2), dim3(kRedBlock), 0, x->stream, x->rrho, x->nCell,
-                       x->case_off, 0, nb, x->live(), x->keep_norms);
-    HIPCHK(hipGetLastError());
-    if (mark && mark("norms")) return HAMSTR_ERR_HIP;
   }
   return HAMSTR_OK;
 }
@@ -2606,6 +2765,17 @@ int hamstr_prepare_graph(hamstr_ctx* x, int check_every) {
   HIPCHK(hipStreamSynchronize(x->stream));
   return HAMSTR_OK;
 }
+
+int hamstr_set_scheme(hamstr_ctx* x, int scheme) {
+  if (!x) return fail(HAMSTR_ERR_ARG, "null context");
+  if (scheme != HAMSTR_SCHEME_ADI && scheme != HAMSTR_SCHEME_EULER && scheme != HAMSTR_SCHEME_RK3)
+    return fail(HAMSTR_ERR_ARG, "unknown scheme");
+  if (scheme != x->scheme) drop_graph(x);  // the captured iteration is the scheme's
+  x->scheme = scheme;
+  return HAMSTR_OK;
+}
+
+int hamstr_get_scheme(const hamstr_ctx* x) { return x ? x->scheme : -1; }
 
 int hamstr_set_graph(hamstr_ctx* x, int on) {
   if (!x) return fail(HAMSTR_ERR_ARG, "null context");

@@ -179,14 +179,20 @@ class SOLVER {
   // SOLVER::initFlow (initFlow.cpp:12-93); creates the device context on first use
   void initFlow() {
     if (!ctx_) {
-      if (std::strcmp(sb_->scheme, "ADI") != 0) {
-        std::printf(" HAMSTR: scheme '%s' is not on the device path (ADI only).\n", sb_->scheme);
+      // stepType = sb->scheme (solver.cpp:22); a name stepSolution has no
+      // branch for would leave the reference iterating nothing, here it is an error
+      if (!std::strcmp(sb_->scheme, "ADI")) scheme_ = HAMSTR_SCHEME_ADI;
+      else if (!std::strcmp(sb_->scheme, "euler")) scheme_ = HAMSTR_SCHEME_EULER;
+      else if (!std::strcmp(sb_->scheme, "rk3")) scheme_ = HAMSTR_SCHEME_RK3;
+      else {
+        std::printf(" HAMSTR: unknown scheme '%s' (ADI, euler or rk3).\n", sb_->scheme);
         std::exit(1);
       }
       hamstr_mesh_desc m{mb_->nCell, mb_->nFace, mb_->nChain, mb_->faces.data(), mb_->faceStartPerChain.data(),
                          mb_->chainConn.data(), mb_->normVec.data(), mb_->refMtx.data()};
       hamstr_params p{sb_->Mach, sb_->alpha, sb_->CFL, sb_->order, sb_->msweep, lhs_, adi_};
       check(hamstr_create(&m, &p, &ctx_), "SOLVER::initFlow");
+      check(hamstr_set_scheme(ctx_, scheme_), "SOLVER::initFlow");
     } else {
       check(hamstr_init_flow(ctx_), "SOLVER::initFlow");
     }
@@ -216,8 +222,18 @@ class SOLVER {
     sb_->LInfNorm = li;
   }
 
-  // SOLVER::stepSolution, ADI branch (march.cpp:112-117), fused on the device
+  // SOLVER::updateSolution (march.cpp:142-164) between HAMSTR_FIELD_Q and
+  // HAMSTR_FIELD_QT (the reference's sb->q and sb->qt)
+  void updateSolution(int qold, int qnew, double coef) {
+    check(hamstr_update_solution(ctx_, qold, qnew, coef), "SOLVER::updateSolution");
+  }
+
+  // SOLVER::stepSolution (march.cpp:64-136), the stepType branch fused on the device
   void stepSolution() {
+    if (scheme_ == HAMSTR_SCHEME_EULER && sb_->visc) {  // march.cpp:78-83
+      std::printf("Not yet implemented. exit.\n");
+      std::exit(1);
+    }
     double l2r = 0, l2l = 0;
     int done = 0;
     check(hamstr_step(ctx_, 1, 0.0, 1, &l2r, &l2l, &done), "SOLVER::stepSolution");
@@ -225,8 +241,9 @@ class SOLVER {
     sb_->L2Norm = l2l;
   }
 
-  // SOLVER::marchInTime (march.cpp:15-59): same table; the L_2 column is the
-  // linear residual of the last sweep, as in the reference.  Wall-clock time
+  // SOLVER::marchInTime (march.cpp:15-59): same table; the L_2 column is
+  // what the reference prints: the linear residual of the last sweep for ADI,
+  // the last computeRHS's residual for euler and rk3.  Wall-clock time
   // of the device steps replaces clock()'s CPU time.
   void marchInTime() {
     double t = 0;
@@ -280,6 +297,7 @@ class SOLVER {
   MeshBlock* mb_;
   SolnBlock* sb_;
   int lhs_, adi_;
+  int scheme_ = HAMSTR_SCHEME_ADI;
   hamstr_ctx* ctx_ = nullptr;
   double lastRHS_ = 0;
 };

@@ -20,7 +20,9 @@ LIB_PATH = os.path.join(HERE, "libhamstr_hip.so")
 OK, ERR_ARG, ERR_HIP, ERR_NAN, ERR_MESH, ERR_NODEV = range(6)
 LHS_FP64, LHS_FP32 = 0, 1
 ADI_EXACT, ADI_FAST = 0, 1
-FIELDS = {"q": 0, "r": 1, "r0": 2, "dq": 3, "sigma": 4, "ff": 5}
+FIELDS = {"q": 0, "r": 1, "r0": 2, "dq": 3, "sigma": 4, "ff": 5, "qt": 6}
+# SOLVER::stepType values (the input deck's scheme= line)
+SCHEMES = {"ADI": 0, "euler": 1, "rk3": 2}
 
 # every entry point declared in include/hamstr_hip.h
 EXPORTS = [
@@ -36,6 +38,7 @@ EXPORTS = [
     "hamstr_mlp_create", "hamstr_mlp_destroy", "hamstr_mlp_num_params", "hamstr_mlp_get_params",
     "hamstr_mlp_set_params", "hamstr_mlp_forward", "hamstr_mlp_loss_grad", "hamstr_mlp_set_data",
     "hamstr_mlp_train", "hamstr_warm_start", "hamstr_mlp_last_error", "hamstr_mlp_set_graph",
+    "hamstr_update_solution", "hamstr_set_scheme", "hamstr_get_scheme",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -174,8 +177,25 @@ class Solver:
         _chk(lib().hamstr_compute_linear_rhs(self.h, C.byref(l2), C.byref(li)))
         return l2.value, li.value
 
+    def updateSolution(self, qold: str, qnew: str, coef: float):
+        """SOLVER::updateSolution: qnew = qold + (coef*CFL/sigma) r, with qold
+        and qnew each "q" or "qt"."""
+        _chk(lib().hamstr_update_solution(self.h, FIELDS[qold], FIELDS[qnew], C.c_double(coef)))
+
+    def set_scheme(self, scheme: str):
+        """stepType: "ADI" (default), "euler" or "rk3"."""
+        if scheme not in SCHEMES:
+            raise ValueError(f"scheme must be one of {sorted(SCHEMES)}")
+        _chk(lib().hamstr_set_scheme(self.h, SCHEMES[scheme]))
+
+    @property
+    def scheme(self) -> str:
+        v = lib().hamstr_get_scheme(self.h)
+        return {i: k for k, i in SCHEMES.items()}[v]
+
     def step(self, nsteps: int, tol: float = 0.0, check_every: int = 1):
-        """nsteps x stepSolution (ADI); returns (steps run, l2rhs[], l2lin[])."""
+        """nsteps x stepSolution with the current scheme; returns (steps run,
+        l2rhs[] of each step's first computeRHS, the printed l2[])."""
         l2r = np.zeros(max(nsteps, 1))
         l2l = np.zeros(max(nsteps, 1))
         done = C.c_int(0)

@@ -231,7 +231,7 @@ def write_input_deck(path: str, mach: float, alpha: float, order: int, nsteps: i
 
 
 def write_reference_case(mesh: Mesh, root: str, mach: float, alpha: float, order: int,
-                         nsteps: int, cfl: float, msweep: int = 1) -> str:
+                         nsteps: int, cfl: float, msweep: int = 1, scheme: str = "ADI") -> str:
     """Write ``root/input.hamstr`` and ``root/input/domain000/*.dat``."""
     d = os.path.join(root, "input", "domain000")
     os.makedirs(d, exist_ok=True)
@@ -259,7 +259,7 @@ def write_reference_case(mesh: Mesh, root: str, mach: float, alpha: float, order
         for n1, n2, c1, e1, c2, e2 in mesh.faces:
             # file order is n1 n2 c1 c2 e1 e2 (io.cpp:349-355)
             fh.write(f"{n1} {n2} {c1} {c2} {e1} {e2}\n")
-    write_input_deck(os.path.join(root, "input.hamstr"), mach, alpha, order, nsteps, cfl, msweep)
+    write_input_deck(os.path.join(root, "input.hamstr"), mach, alpha, order, nsteps, cfl, msweep, scheme)
     return root
 
 

@@ -16,7 +16,7 @@
  *   chainConn  int    [nChainFace] face ids along each chain
  *   normVec    double [2*nFace]   (y2-y1, -(x2-x1))
  *   refMtx     double [4*nFace]   I - 2 n n^T / |n|^2, row-major
- *   q, r, r0, dq     double [4*nCell]  AoS (rho, rho u, rho v, E) per cell
+ *   q, qt, r, r0, dq double [4*nCell]  AoS (rho, rho u, rho v, E) per cell
  *   sigma            double [nCell]
  *   ff               double [32*nFace] lmat[4][4] then rmat[4][4] per face
  */
@@ -76,6 +76,14 @@ typedef struct hamstr_params {
 #define HAMSTR_FIELD_DQ 3
 #define HAMSTR_FIELD_SIGMA 4
 #define HAMSTR_FIELD_FF 5   /* converted to double when the LHS is FP32 */
+#define HAMSTR_FIELD_QT 6   /* SOLNBLOCK::qt, rk3's stage-start state (per cell, 4 values) */
+
+/* SOLVER::stepType (solver.cpp:22), which the reference reads from the input
+ * deck's scheme= line (io.cpp:30): the branch of stepSolution that
+ * hamstr_step runs (march.cpp:64-136). */
+#define HAMSTR_SCHEME_ADI 0    /* computeRHS("implicit") + ADI(): the default */
+#define HAMSTR_SCHEME_EULER 1  /* computeRHS("explicit") + updateSolution(q, q, 1) */
+#define HAMSTR_SCHEME_RK3 2    /* the reference's three-stage Runge-Kutta */
 
 /* ---- set-up ------------------------------------------------------------- */
 
@@ -103,7 +111,8 @@ int hamstr_batch_info(const hamstr_ctx *ctx, int *ncases, int *cell_off);
  * first step (1-based) at which case k's nonlinear L2 <= tol, or -1; the run
  * stops at the first read-back (every check_every steps) after all cases have
  * converged.  norms[4*ncases] (may be NULL) gets each case's (L2, Linf) of
- * the nonlinear and linear residuals at the last step.  *done = steps run. */
+ * the nonlinear and of the printed residual at the last step (hamstr_step's
+ * l2rhs and l2lin).  *done = steps run. */
 int hamstr_batch_step(hamstr_ctx *ctx, int nsteps, double tol, int check_every, int *iters,
                       double *norms, int *done);
 /* Retirement (no counterpart in the reference, which runs one case for a fixed
@@ -159,7 +168,7 @@ int hamstr_case_freestream(const hamstr_ctx *ctx, int k, double *out4);
 
 int hamstr_set_field(hamstr_ctx *ctx, int field, const double *host);
 int hamstr_get_field(hamstr_ctx *ctx, int field, double *host);
-/* cells [first, first+count) of a per-cell field (q, r, r0, dq, sigma), e.g.
+/* cells [first, first+count) of a per-cell field (q, qt, r, r0, dq, sigma), e.g.
  * one case of a batch */
 int hamstr_get_cells(hamstr_ctx *ctx, int field, int first, int count, double *host);
 int hamstr_set_cells(hamstr_ctx *ctx, int field, int first, int count, const double *host);
@@ -179,12 +188,27 @@ int hamstr_adi(hamstr_ctx *ctx, double *L2lin, double *Linflin);
  * q, dq, r0, sigma, ff. */
 int hamstr_compute_linear_rhs(hamstr_ctx *ctx, double *L2lin, double *Linflin);
 
-/* nsteps x SOLVER::stepSolution's ADI branch (march.cpp:112-117).  Per-step
- * nonlinear (computeRHS) and linear (last computeLinearRHS) L2 norms are
- * written to l2rhs[n] / l2lin[n] when the arrays are not NULL.  When tol > 0
- * the loop stops after the first step whose nonlinear L2 <= tol; *done gets
- * the number of steps run.  The norms are read back every check_every steps
- * (>= 1); the steps in between run without host synchronisation. */
+/* SOLVER::updateSolution (march.cpp:142-164) with the current r and sigma:
+ * dst = src + (coef*CFL/sigma) r per cell, src and dst each HAMSTR_FIELD_Q or
+ * HAMSTR_FIELD_QT (may be the same). */
+int hamstr_update_solution(hamstr_ctx *ctx, int src_field, int dst_field, double coef);
+
+/* Selects the stepSolution branch hamstr_step / hamstr_batch_step run
+ * (HAMSTR_SCHEME_*; SOLVER::stepType, solver.cpp:22).  The explicit schemes
+ * compute no Jacobians, so the LHS precision and ADI mode do not affect them.
+ * hamstr_get_scheme returns the current one (-1 for a NULL context). */
+int hamstr_set_scheme(hamstr_ctx *ctx, int scheme);
+int hamstr_get_scheme(const hamstr_ctx *ctx);
+
+/* nsteps x SOLVER::stepSolution (march.cpp:64-136) with the selected scheme,
+ * by default its ADI branch (march.cpp:112-117).  Per-step norms are written
+ * to l2rhs[n] / l2lin[n] when the arrays are not NULL: l2rhs the nonlinear L2
+ * of the step's first computeRHS (the residual of the state the step starts
+ * from), l2lin the norm the reference prints after the step (ADI: the last
+ * computeLinearRHS; euler, rk3: the last computeRHS).  When tol > 0 the loop
+ * stops after the first step whose l2rhs <= tol; *done gets the number of
+ * steps run.  The norms are read back every check_every steps (>= 1); the
+ * steps in between run without host synchronisation. */
 int hamstr_step(hamstr_ctx *ctx, int nsteps, double tol, int check_every,
                 double *l2rhs, double *l2lin, int *done);
 

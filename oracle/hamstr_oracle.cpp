@@ -599,6 +599,7 @@ struct Oracle {
   double CFL = 1.0, gamma = 1.4, gm1 = 0.4;
   double rinf = 1, uinf = 0, vinf = 0, einf = 0, pinf = 0;
   std::vector<double> q, r, r0, dq, sigma, ff;  // ff: 32 per face (lmat then rmat)
+  std::vector<double> qt;                       // rk3 stage-start state (SOLNBLOCK::qt)
   double L2 = 0, Linf = 0;       // printed norm (overwritten by computeLinearRHS)
   double L2rhs = 0, Linfrhs = 0; // nonlinear norm of the last computeRHS
   double L2lin = 0, Linflin = 0; // linear norm of the last computeLinearRHS
@@ -935,6 +936,48 @@ void init_flow(Oracle& o, double mach, double alpha) {
     o.q[NQ * i + 2] = o.rinf * o.vinf;
     o.q[NQ * i + 3] = o.einf;
   }
+  o.qt = o.q;  // initFlow.cpp:71-88 sets qt alongside q
+}
+
+// SOLVER::updateSolution (march.cpp:142-164): qnew = qold + (coef*CFL/sigma) r.
+// qold and qnew may be the same array (each element is read before it is written).
+void update_solution(Oracle& o, const double* qold, double* qnew, double coef) {
+  int m = 0;
+  for (int i = 0; i < o.nCell; i++) {
+    for (int j = 0; j < NQ; j++) {
+      double dtfac = coef * o.CFL / o.sigma[i];
+      qnew[m] = qold[m] + dtfac * o.r[m];
+      m++;
+    }
+  }
+}
+
+// SOLVER::stepSolution (march.cpp:64-136) for stepType "ADI" (0), "euler" (1)
+// and "rk3" (2).  *first gets the nonlinear norm of the step's first
+// computeRHS (the residual of the state the step starts from).  The rk3 branch
+// keeps the reference's stage coefficients and its qt handling: qt is written
+// in stage 1 only, stages 2 and 3 both restart from it.
+void step_solution(Oracle& o, int scheme, double* first, double* first_inf) {
+  if (scheme == 0) {
+    compute_rhs(o, true);
+    *first = o.L2rhs;
+    *first_inf = o.Linfrhs;
+    adi(o);
+    return;
+  }
+  compute_rhs(o, false);
+  *first = o.L2rhs;
+  *first_inf = o.Linfrhs;
+  if (scheme == 1) {
+    update_solution(o, o.q.data(), o.q.data(), 1.);
+    return;
+  }
+  update_solution(o, o.q.data(), o.qt.data(), 0.25);
+  update_solution(o, o.q.data(), o.q.data(), 8.0 / 15);
+  compute_rhs(o, false);
+  update_solution(o, o.qt.data(), o.q.data(), 5.0 / 12);
+  compute_rhs(o, false);
+  update_solution(o, o.qt.data(), o.q.data(), 3.0 / 4.0);
 }
 
 }  // namespace
@@ -1026,12 +1069,13 @@ void* orc_create(int nCell, int nFace, int nChain, const int* faces, const int* 
   o->dq.assign(4 * nCell, 0.0);
   o->sigma.assign(nCell, 0.0);
   o->ff.assign(32 * (size_t)nFace, 0.0);
+  o->qt.assign(4 * nCell, 0.0);
   init_flow(*o, mach, alpha);
   return o;
 }
 void orc_destroy(void* h) { delete static_cast<Oracle*>(h); }
 
-// field ids: 0 q, 1 r, 2 r0, 3 dq, 4 sigma, 5 ff
+// field ids: 0 q, 1 r, 2 r0, 3 dq, 4 sigma, 5 ff, 6 qt
 static std::vector<double>* field(Oracle* o, int id) {
   switch (id) {
     case 0: return &o->q;
@@ -1040,6 +1084,7 @@ static std::vector<double>* field(Oracle* o, int id) {
     case 3: return &o->dq;
     case 4: return &o->sigma;
     case 5: return &o->ff;
+    case 6: return &o->qt;
     default: return nullptr;
   }
 }
@@ -1091,6 +1136,34 @@ int orc_step(void* h, int nsteps, double* l2rhs, double* l2lin, double* seconds)
     adi(*o);
     if (l2rhs) l2rhs[n] = o->L2rhs;
     if (l2lin) l2lin[n] = o->L2lin;
+  }
+  auto t1 = std::chrono::steady_clock::now();
+  if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+  return o->nanFlag;
+}
+
+// SOLVER::updateSolution between two per-cell fields (0 q, 6 qt) with the
+// current r and sigma
+int orc_update_solution(void* h, int src, int dst, double coef) {
+  Oracle* o = static_cast<Oracle*>(h);
+  if ((src != 0 && src != 6) || (dst != 0 && dst != 6)) return -1;
+  update_solution(*o, field(o, src)->data(), field(o, dst)->data(), coef);
+  return 0;
+}
+
+// n x SOLVER::stepSolution with stepType scheme (0 ADI, 1 euler, 2 rk3).
+// first[n]: nonlinear L2 of the step's first computeRHS; printed[n]: the norm
+// marchInTime prints after the step (sb->L2Norm: the last computeLinearRHS for
+// ADI, the last computeRHS for euler and rk3).
+int orc_step_scheme(void* h, int scheme, int nsteps, double* first, double* printed, double* seconds) {
+  Oracle* o = static_cast<Oracle*>(h);
+  if (scheme < 0 || scheme > 2) return -1;
+  auto t0 = std::chrono::steady_clock::now();
+  for (int n = 0; n < nsteps; n++) {
+    double f = 0, fi = 0;
+    step_solution(*o, scheme, &f, &fi);
+    if (first) first[n] = f;
+    if (printed) printed[n] = (scheme == 0) ? o->L2lin : o->L2rhs;
   }
   auto t1 = std::chrono::steady_clock::now();
   if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();

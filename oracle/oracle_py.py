@@ -17,7 +17,8 @@ REF_DIR = os.path.join(HERE, "_ref")
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 
-FIELD_IDS = {"q": 0, "r": 1, "r0": 2, "dq": 3, "sigma": 4, "ff": 5}
+FIELD_IDS = {"q": 0, "r": 1, "r0": 2, "dq": 3, "sigma": 4, "ff": 5, "qt": 6}
+SCHEMES = {"ADI": 0, "euler": 1, "rk3": 2}
 
 
 def _d(a: np.ndarray):
@@ -200,6 +201,19 @@ class OracleCase:
         rc = self.lib.orc_step(self.h, n, _d(l2r), _d(l2l), C.byref(sec))
         return rc, l2r[:n], l2l[:n], sec.value
 
+    def step_scheme(self, scheme, n):
+        """n x stepSolution with stepType scheme ("ADI", "euler", "rk3"):
+        (rc, first-computeRHS L2 per step, printed L2 per step, seconds)."""
+        first = np.zeros(max(n, 1))
+        printed = np.zeros(max(n, 1))
+        sec = C.c_double()
+        rc = self.lib.orc_step_scheme(self.h, SCHEMES[scheme], n, _d(first), _d(printed), C.byref(sec))
+        return rc, first[:n], printed[:n], sec.value
+
+    def update_solution(self, src, dst, coef):
+        rc = self.lib.orc_update_solution(self.h, FIELD_IDS[src], FIELD_IDS[dst], C.c_double(coef))
+        assert rc == 0
+
 
 class Reference(_Unit):
     """The reference's own code, compiled from its sources by oracle/Makefile."""
@@ -239,7 +253,7 @@ class ReferenceCase:
         return {4: self.n_cell, 5: 32 * self.n_face, 6: 2 * self.n_face, 7: 4 * self.n_face}.get(fid, 4 * self.n_cell)
 
     def get(self, name):
-        fid = {"normvec": 6, "refmtx": 7, **FIELD_IDS}[name]
+        fid = {**FIELD_IDS, "normvec": 6, "refmtx": 7, "qt": 8}[name]
         out = np.zeros(self._size(fid))
         self.lib.ref_get(self.h, fid, _d(out))
         return out
@@ -277,3 +291,19 @@ class ReferenceCase:
         sec = C.c_double()
         self.lib.ref_step(self.h, n, _d(l2r), _d(l2l), C.byref(sec))
         return l2r[:n], l2l[:n], sec.value
+
+    def set_scheme(self, name):
+        assert self.lib.ref_set_scheme(self.h, name.encode()) == 0
+
+    def step_solution(self, n):
+        """n x SOLVER::stepSolution with the current scheme: (printed L2,
+        printed Linf, seconds) per step."""
+        l2 = np.zeros(max(n, 1))
+        li = np.zeros(max(n, 1))
+        sec = C.c_double()
+        self.lib.ref_step_solution(self.h, n, _d(l2), _d(li), C.byref(sec))
+        return l2[:n], li[:n], sec.value
+
+    def update_solution(self, src, dst, coef):
+        ids = {"q": 0, "qt": 8}
+        assert self.lib.ref_update_solution(self.h, ids[src], ids[dst], C.c_double(coef)) == 0

@@ -10,7 +10,7 @@
 //
 // Call surface used: MESHBLOCK::readMeshData / setViscosity / preprocess,
 // SOLNBLOCK::readSolutionInputs, SOLVER::{SOLVER, initFlow, computeRHS, ADI,
-// computeLinearRHS, fluxRoe2D, jacobianRoe2D}, RECONSTRUCTION::{MUSCL, WENO,
+// computeLinearRHS, stepSolution, updateSolution, fluxRoe2D, jacobianRoe2D}, RECONSTRUCTION::{MUSCL, WENO,
 // MUSCL_DERIV, WENO_DERIV}, TRISOLVER::{blockTridag4, blockTridagPeriodic4}.
 // ============================================================================
 #include <unistd.h>
@@ -93,7 +93,7 @@ void ref_params(void* h, double* out) {
   out[4] = s->rinf; out[5] = s->uinf; out[6] = s->vinf; out[7] = s->einf;
 }
 
-// 0 q, 1 r, 2 r0, 3 dq, 4 sigma, 5 ff (32/face), 6 normVec (2/face), 7 refMtx (4/face)
+// 0 q, 1 r, 2 r0, 3 dq, 4 sigma, 5 ff (32/face), 6 normVec (2/face), 7 refMtx (4/face), 8 qt
 int ref_get(void* h, int id, double* out) {
   RefCase* c = static_cast<RefCase*>(h);
   MESHBLOCK* mb = c->mb;
@@ -125,6 +125,7 @@ int ref_get(void* h, int id, double* out) {
         for (int j = 0; j < 2; j++)
           for (int k = 0; k < 2; k++) out[4 * f + 2 * j + k] = mb->refMtx[f][j][k];
       return 0;
+    case 8: std::memcpy(out, sb->qt, sizeof(double) * 4 * n); return 0;
   }
   return -1;
 }
@@ -183,6 +184,40 @@ void ref_step(void* h, int nsteps, double* l2rhs, double* l2lin, double* seconds
   }
   auto t1 = std::chrono::steady_clock::now();
   if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+}
+
+// SOLVER::stepType points at SOLNBLOCK::scheme (solver.cpp:22): "ADI",
+// "euler" or "rk3" as the input deck's scheme= line would set it
+int ref_set_scheme(void* h, const char* name) {
+  RefCase* c = static_cast<RefCase*>(h);
+  if (std::strlen(name) >= sizeof(c->sb->scheme)) return -1;
+  std::strcpy(c->sb->scheme, name);
+  return 0;
+}
+
+// n x SOLVER::stepSolution (march.cpp:64-136) with the current stepType;
+// printed[n] / printed_inf[n]: sb->L2Norm / LInfNorm after the step, the
+// values marchInTime prints (march.cpp:45-46)
+void ref_step_solution(void* h, int nsteps, double* printed, double* printed_inf, double* seconds) {
+  RefCase* c = static_cast<RefCase*>(h);
+  auto t0 = std::chrono::steady_clock::now();
+  for (int n = 0; n < nsteps; n++) {
+    c->solv->stepSolution();
+    if (printed) printed[n] = c->sb->L2Norm;
+    if (printed_inf) printed_inf[n] = c->sb->LInfNorm;
+  }
+  auto t1 = std::chrono::steady_clock::now();
+  if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+}
+
+// SOLVER::updateSolution(qold, qnew, coef) between q (0) and qt (8)
+int ref_update_solution(void* h, int src, int dst, double coef) {
+  RefCase* c = static_cast<RefCase*>(h);
+  auto pick = [&](int id) -> double* { return id == 0 ? c->sb->q : (id == 8 ? c->sb->qt : nullptr); };
+  double *a = pick(src), *b = pick(dst);
+  if (!a || !b) return -1;
+  c->solv->updateSolution(a, b, coef);
+  return 0;
 }
 
 // ---- unit functions ---------------------------------------------------------

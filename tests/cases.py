@@ -45,3 +45,17 @@ def perturbed_state(solver_like, rng, amp=0.02):
     q[:, 2] += amp * rng.uniform(-1, 1, n)
     q[:, 3] *= 1.0 + amp * rng.uniform(-1, 1, n)
     return q.reshape(-1)
+
+
+class GoldenMesh:
+    """The mesh stored in a tests/golden/case_*.npz fixture, shaped like
+    meshgen.Mesh for the solver constructors."""
+
+    def __init__(self, d):
+        self.x = d["x"]
+        self.faces = d["faces_in"]
+        self.chain_start = d["chain_start"]
+        self.chain_conn = d["chain_conn"]
+        self.n_cell = int(d["q0"].size // 4)
+        self.n_face = int(self.faces.shape[0])
+        self.n_chain = int(self.chain_start.size - 1)

@@ -12,7 +12,9 @@ Fixtures (numpy .npz, inputs and outputs only):
                  MUSCL_DERIV, WENO_DERIV, blockTridag4, blockTridagPeriodic4
   case_<name>.npz  a mesh (coordinates, faces, chains), its preprocess
                  output, a perturbed initial state, and computeRHS / ADI /
-                 5-step results of the reference SOLVER
+                 5-step results of the reference SOLVER; 5 steps of the
+                 explicit schemes (stepType "euler" and "rk3", CFL 0.8) from
+                 the same state
 """
 from __future__ import annotations
 
@@ -34,6 +36,7 @@ import oracle_py  # noqa: E402
 from cases import random_block_system, random_normals, random_states  # noqa: E402
 
 THIRD = 0.3333333333333333
+EXPLICIT_CFL = 0.8
 
 CASES = [
     # name, mesh factory, Mach, alpha, order
@@ -125,6 +128,16 @@ def run_case(ref: oracle_py.Reference, mesh, mach, alpha, order, workdir) -> dic
     l2r, l2l, _ = rc.step(5)
     out.update(step5_q=rc.get("q"), step5_l2rhs=l2r, step5_l2lin=l2l)
     shutil.rmtree(d, ignore_errors=True)
+    for scheme in ("euler", "rk3"):
+        d = tempfile.mkdtemp(dir=workdir)
+        meshgen.write_reference_case(mesh, d, mach, alpha, order, 5, EXPLICIT_CFL, scheme=scheme)
+        rc = ref.load(d)
+        rc.set_q(q)
+        l2, li, _ = rc.step_solution(5)
+        out.update({f"{scheme}5_q": rc.get("q"), f"{scheme}5_qt": rc.get("qt"), f"{scheme}5_r": rc.get("r"),
+                    f"{scheme}5_sigma": rc.get("sigma"), f"{scheme}5_l2": l2, f"{scheme}5_linf": li})
+        shutil.rmtree(d, ignore_errors=True)
+    out["explicit_cfl"] = EXPLICIT_CFL
     return out
 
 
