@@ -9,7 +9,8 @@ computeRHS("implicit") + ADI (two chain phases) + computeLinearRHS + update.
 
 Prints one JSON line: cell-iterations/s (value), the roofline of the dominant
 kernel from HIP events over the timed region, the time to nonlinear
-L2 <= 1e-10 from freestream, the FP32-LHS variant, and the reference's own CPU
+L2 <= 1e-10 from freestream, the FP32-LHS variant, the explicit euler / rk3
+steps of the same mesh, the mixed-batch sweep, and the reference's own CPU
 code timed on this host on a bounded sample of the same workload.
 """
 from __future__ import annotations
@@ -129,6 +130,7 @@ def main():
     ap.add_argument("--no-exact", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-explicit", action="store_true", help="skip the euler / rk3 stepSolution leg")
     ap.add_argument("--weno", action="store_true", help="also run the 1024x384 WENO FP32-LHS error case")
     ap.add_argument("--sweep-cases", type=int, default=256)
     ap.add_argument("--train-cases", type=int, default=384)
@@ -254,6 +256,8 @@ def main():
             out["fp32_lhs"]["time_to_tol"] = c32
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mesh, args.mach, args.alpha, args.order, args.cfl, args.cpu_budget)
+    if not args.no_explicit:
+        out["explicit_schemes"] = explicit_section(args, mesh, n_proc)
     if args.weno:
         out["weno_transonic_fp32_lhs"] = weno_section(args)
     if args.copy_probe:
@@ -261,6 +265,80 @@ def main():
     if not args.no_sweep and args.sweep_cases > 0:
         out["sweep"] = sweep_section(args, with_cpu=not args.no_cpu)
     print(json.dumps(out), flush=True)
+
+
+def explicit_bytes(n_cell, n_proc, scheme):
+    """Bytes per cell-iteration of an explicit step's dataflow when every
+    materialised array is read or written once per stage (face fluxes kept on
+    chip): per stage q gathered (32), r and sigma written (40), the update's
+    source read unless it is the gathered q (32) and its destination written
+    (32), face normals read (16 per face).  euler: 104 + 16 phi.  rk3: three
+    stages of 136 + 16 phi (stage 1 writes qt and q from the gathered q,
+    stages 2 and 3 read qt and write q)."""
+    phi = n_proc / n_cell
+    if scheme == "euler":
+        return 104 + 16 * phi
+    return 3 * (136 + 16 * phi)
+
+
+def explicit_section(args, mesh, n_proc, cfl=0.8, steps=200):
+    """stepSolution's euler and rk3 branches on the bench mesh (CFL 0.8, the
+    explicit schemes' stable range): graph-replayed steps timed like the ADI
+    value, per-kernel event times, and the oracle (or the reference build when
+    present) on a bounded CPU sample."""
+    from hamstr_oop_amd import hip
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    out = {}
+    for scheme in ("euler", "rk3"):
+        s = hip.Solver(mesh, args.mach, args.alpha, args.order, cfl, adi_mode=hip.ADI_FAST)
+        s.set_scheme(scheme)
+        s.step(args.warmup, check_every=max(args.warmup, 1))
+        s.prepare_graph(steps)
+        s.sync()
+        t0 = time.perf_counter()
+        n, _, printed = s.step(steps, check_every=steps)
+        s.sync()
+        dt = time.perf_counter() - t0
+        s.set_timing(True)
+        s.step(20, check_every=20)
+        stages = s.last_timings()
+        s.close()
+        value = mesh.n_cell * n / dt
+        b = explicit_bytes(mesh.n_cell, n_proc, scheme)
+        rec = {"value": value, "unit": "cell-iters/s", "ms_per_step": dt / n * 1e3, "cfl": cfl,
+               "stage_ms": {k: round(v, 5) for k, v in stages.items()},
+               "iteration_roofline": {"bytes_per_cell_iter": b, "achieved_gbs": value * b / 1e9,
+                                      "frac": value * b / 1e9 / HBM_PEAK_GBS},
+               "printed_l2_last": float(printed[-1])}
+        if not args.no_cpu:
+            rec["cpu_baseline"] = explicit_cpu_baseline(mesh, args.mach, args.alpha, args.order, cfl, scheme)
+        out[scheme] = rec
+    return out
+
+
+def explicit_cpu_baseline(mesh, mach, alpha, order, cfl, scheme, budget_s=4.0, max_iters=30):
+    """One core, a bounded number of explicit steps of the same case from
+    freestream: the reference's stepSolution when its build is present, else
+    the oracle's restatement."""
+    import oracle_py
+    from hamstr_oop_amd import meshgen
+    if oracle_py.Reference.available("O2"):
+        ref = oracle_py.Reference("O2")
+        with tempfile.TemporaryDirectory() as d:
+            meshgen.write_reference_case(mesh, d, mach, alpha, order, 1, cfl, scheme=scheme)
+            rc = ref.load(d)
+            _, _, t1 = rc.step_solution(1)
+            n = int(max(1, min(max_iters, budget_s / max(t1, 1e-6))))
+            _, _, sec = rc.step_solution(n)
+        kind, what = "reference", "reference SOLVER::stepSolution compiled -O2 from its sources"
+    else:
+        oc = oracle_py.Oracle().case(mesh, mach, alpha, order, cfl)
+        _, _, _, t1 = oc.step_scheme(scheme, 1)
+        n = int(max(1, min(max_iters, budget_s / max(t1, 1e-6))))
+        _, _, _, sec = oc.step_scheme(scheme, n)
+        kind, what = "port", "oracle C++ restatement, -O2"
+    return {"value": mesh.n_cell * n / sec, "unit": "cell-iters/s", "cores": 1, "kind": kind,
+            "sample": f"{n} {scheme} steps (after 1 warm-up) from freestream; {what}; {sec:.2f} s wall"}
 
 
 def weno_section(args, nc=1024, nr=384, mach=0.8, alpha=1.25):

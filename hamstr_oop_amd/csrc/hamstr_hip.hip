@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <functional>
@@ -1539,6 +1540,8 @@ struct hamstr_ctx {
   double *nvec = nullptr, *refmtx = nullptr;
   double4* qinf = nullptr;
   std::vector<double> qinf_h;  // [nCase][4]
+  std::vector<double> case_mach, case_alpha;
+  uint64_t mesh_hash = 0;      // faces, chainStart and chainConn of the (union) mesh
   int* cell_case = nullptr;
   // raw density residual per cell for the per-case norms of batches:
   // [0, nCell) nonlinear, [nCell, 2 nCell) linear (a retired case keeps both)
@@ -1786,6 +1789,17 @@ void free_ctx(hamstr_ctx* x) {
   if (x->stream) (void)hipStreamDestroy(x->stream);
 }
 
+// FNV-1a, 64 bit, over a byte range
+constexpr uint64_t kFnvBasis = 1469598103934665603ULL;
+uint64_t fnv1a(uint64_t h, const void* p, size_t n) {
+  const unsigned char* b = static_cast<const unsigned char*>(p);
+  for (size_t i = 0; i < n; i++) {
+    h ^= b[i];
+    h *= 1099511628211ULL;
+  }
+  return h;
+}
+
 struct CaseInfo {
   int cellBase, cellCount;
   int chainBase, chainCount;
@@ -2026,6 +2040,17 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
   }
   x->qinf_h.assign(4 * (size_t)nCase, 0.0);
   for (int k = 0; k < nCase; k++) freestream_state(cases[k].mach, cases[k].alpha, &x->qinf_h[4 * k]);
+  for (int k = 0; k < nCase; k++) {
+    x->case_mach.push_back(cases[k].mach);
+    x->case_alpha.push_back(cases[k].alpha);
+  }
+  {
+    // fingerprint of the mesh the state belongs to (solution files carry it)
+    uint64_t h = fnv1a(kFnvBasis, &nCell, sizeof(int));
+    h = fnv1a(h, faces, sizeof(int) * 6 * (size_t)nFace);
+    h = fnv1a(h, fs, sizeof(int) * (size_t)(nChain + 1));
+    x->mesh_hash = fnv1a(h, cc, sizeof(int) * (size_t)nChainFace);
+  }
   // chains, and so processed faces, are stored case by case
   x->case_chain_off.assign(nCase + 1, nChain);
   x->case_proc_off.assign(nCase + 1, nProc);
@@ -2355,6 +2380,86 @@ int hamstr_set_cells(hamstr_ctx* x, int field, int first, int count, const doubl
   if (!host && count) return fail(HAMSTR_ERR_ARG, "null argument");
   HIPCHK(hipMemcpyAsync(d, host, sizeof(double) * per * count, hipMemcpyHostToDevice, x->stream));
   HIPCHK(hipStreamSynchronize(x->stream));
+  return HAMSTR_OK;
+}
+
+// ---- solution files ---------------------------------------------------------
+// Little-endian binary: a 64-byte header, one 32-byte record per case, then
+// q[4*nCell] as doubles.
+//   0 char[8] "HMSTRSOL"   8 int32 version (1)   12 int32 nCase   16 int64 nCell
+//  24 int64 step          32 uint64 mesh hash   40 int32 order   44 int32 scheme
+//  48 double CFL          56 8 bytes zero
+//  per case: double Mach, double alpha, int64 first cell, int64 cell count
+namespace {
+constexpr char kSolMagic[8] = {'H', 'M', 'S', 'T', 'R', 'S', 'O', 'L'};
+struct SolHeader {
+  char magic[8];
+  int32_t version, ncase;
+  int64_t ncell, step;
+  uint64_t mesh_hash;
+  int32_t order, scheme;
+  double cfl;
+  int64_t reserved;
+};
+struct SolCase {
+  double mach, alpha;
+  int64_t first, count;
+};
+static_assert(sizeof(SolHeader) == 64 && sizeof(SolCase) == 32, "solution file layout");
+}  // namespace
+
+int hamstr_write_solution(hamstr_ctx* x, const char* path, long step) {
+  if (!x || !path) return fail(HAMSTR_ERR_ARG, "null argument");
+  std::vector<double> q(4 * (size_t)x->nCell);
+  HIPCHK(hipMemcpyAsync(q.data(), x->q, sizeof(double) * q.size(), hipMemcpyDeviceToHost, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  SolHeader h{};
+  std::memcpy(h.magic, kSolMagic, 8);
+  h.version = 1;
+  h.ncase = x->nCase;
+  h.ncell = x->nCell;
+  h.step = step;
+  h.mesh_hash = x->mesh_hash;
+  h.order = x->prm.order;
+  h.scheme = x->scheme;
+  h.cfl = x->prm.cfl;
+  std::vector<SolCase> cs(x->nCase);
+  for (int k = 0; k < x->nCase; k++)
+    cs[k] = SolCase{x->case_mach[k], x->case_alpha[k], x->case_off_h[k], x->case_off_h[k + 1] - x->case_off_h[k]};
+  // written next to the target and renamed over it, so a reader never sees a partial file
+  const std::string tmp = std::string(path) + ".tmp";
+  FILE* fp = std::fopen(tmp.c_str(), "wb");
+  if (!fp) return fail(HAMSTR_ERR_ARG, "cannot open " + tmp + " for writing");
+  bool ok = std::fwrite(&h, sizeof(h), 1, fp) == 1 &&
+            std::fwrite(cs.data(), sizeof(SolCase), cs.size(), fp) == cs.size() &&
+            std::fwrite(q.data(), sizeof(double), q.size(), fp) == q.size();
+  ok = (std::fclose(fp) == 0) && ok;
+  if (!ok || std::rename(tmp.c_str(), path) != 0) {
+    std::remove(tmp.c_str());
+    return fail(HAMSTR_ERR_ARG, std::string("writing ") + path + " failed");
+  }
+  return HAMSTR_OK;
+}
+
+int hamstr_read_solution(hamstr_ctx* x, const char* path, long* step) {
+  if (!x || !path) return fail(HAMSTR_ERR_ARG, "null argument");
+  FILE* fp = std::fopen(path, "rb");
+  if (!fp) return fail(HAMSTR_ERR_ARG, std::string("cannot open ") + path);
+  SolHeader h{};
+  std::string err;
+  std::vector<double> q(4 * (size_t)x->nCell);
+  if (std::fread(&h, sizeof(h), 1, fp) != 1 || std::memcmp(h.magic, kSolMagic, 8) != 0 || h.version != 1)
+    err = "not a solution file (version 1)";
+  else if (h.ncase != x->nCase || h.ncell != x->nCell || h.mesh_hash != x->mesh_hash)
+    err = "solution file is for another mesh or batch";
+  else if (std::fseek(fp, (long)(sizeof(SolHeader) + sizeof(SolCase) * (size_t)h.ncase), SEEK_SET) != 0 ||
+           std::fread(q.data(), sizeof(double), q.size(), fp) != q.size())
+    err = "solution file is truncated";
+  std::fclose(fp);
+  if (!err.empty()) return fail(HAMSTR_ERR_ARG, std::string(path) + ": " + err);
+  HIPCHK(hipMemcpyAsync(x->q, q.data(), sizeof(double) * q.size(), hipMemcpyHostToDevice, x->stream));
+  HIPCHK(hipStreamSynchronize(x->stream));
+  if (step) *step = (long)h.step;
   return HAMSTR_OK;
 }
 

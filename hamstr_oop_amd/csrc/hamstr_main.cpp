@@ -3,7 +3,10 @@
 // input.hamstr and ./input/domain000/*.dat from the working directory, then
 // readInputs -> preprocessor -> marchInTime with the reference's iteration
 // table.  Options: --lhs32 (FP32 LHS), --exact (the reference's sequential
-// block-tridiagonal order), --dump <file> (final q, binary doubles).
+// block-tridiagonal order), --dump <file> (final q, binary doubles),
+// --restart <file> (start from a solution file instead of freestream),
+// --solution <file> (where nwrite > 0 in input.hamstr writes the solution
+// file every nwrite steps; default solution.bin).
 #include <cstdio>
 #include <cstring>
 #include <vector>
@@ -12,11 +15,13 @@
 
 int main(int argc, char** argv) {
   int lhs = HAMSTR_LHS_FP64, adi = HAMSTR_ADI_FAST;
-  const char* dump = nullptr;
+  const char *dump = nullptr, *restart = nullptr, *solution = nullptr;
   for (int i = 1; i < argc; i++) {
     if (!std::strcmp(argv[i], "--lhs32")) lhs = HAMSTR_LHS_FP32;
     else if (!std::strcmp(argv[i], "--exact")) adi = HAMSTR_ADI_EXACT;
     else if (!std::strcmp(argv[i], "--dump") && i + 1 < argc) dump = argv[++i];
+    else if (!std::strcmp(argv[i], "--restart") && i + 1 < argc) restart = argv[++i];
+    else if (!std::strcmp(argv[i], "--solution") && i + 1 < argc) solution = argv[++i];
   }
   std::printf("\n HAMSTR (MI355X): Read input files ... ");
   hamstr::MeshBlock mb;
@@ -28,6 +33,8 @@ int main(int argc, char** argv) {
   mb.preprocess();
   std::printf(" [done]\n HAMSTR (MI355X): Initialize flow  ... ");
   solv.initFlow();
+  if (restart) solv.restart(restart);
+  if (solution) solv.setSolutionFile(solution);
   std::printf(" [done]\n cells %d faces %d chains %d max chain %d order %d CFL %g Mach %g alpha %g\n", mb.nCell,
               mb.nFace, mb.nChain, mb.nMaxChain, sb.order, sb.CFL, sb.Mach, sb.alpha);
   std::printf(" HAMSTR (MI355X): Marching in time ... \n");

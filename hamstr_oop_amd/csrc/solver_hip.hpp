@@ -241,8 +241,8 @@ class SOLVER {
     sb_->L2Norm = l2l;
   }
 
-  // SOLVER::marchInTime (march.cpp:15-59): same table; the L_2 column is
-  // what the reference prints: the linear residual of the last sweep for ADI,
+  // SOLVER::marchInTime (march.cpp:15-59): same table (iterations counted on
+  // from a restart file's step); the L_2 column is what the reference prints: the linear residual of the last sweep for ADI,
   // the last computeRHS's residual for euler and rk3.  Wall-clock time
   // of the device steps replaces clock()'s CPU time.
   void marchInTime() {
@@ -262,7 +262,10 @@ class SOLVER {
         std::printf(" |   Iter  | L_2 (density) | L_inf (density) | CPU Time (sec) |\n");
         std::printf(" --------------------------------------------------------------\n");
       }
-      std::printf(" |  %*d  |  %e |   %e  |  %e  |\n", 5, n, sb_->L2Norm, sb_->LInfNorm, t);
+      std::printf(" |  %*ld  |  %e |   %e  |  %e  |\n", 5, step0_ + n, sb_->L2Norm, sb_->LInfNorm, t);
+      // every nwrite steps and at the end (nwrite > 0): the solution file
+      if (sb_->nwrite > 0 && ((n + 1) % sb_->nwrite == 0 || n == sb_->nsteps - 1))
+        writeSolution(solution_path_.c_str(), step0_ + n + 1);
       if (n == sb_->nsteps - 1) {
         std::printf(" --------------------------------------------------------------\n");
         std::printf(" |   Iter  | L_2 (density) | L_inf (density) | CPU Time (sec) |\n");
@@ -287,6 +290,21 @@ class SOLVER {
       }
   }
 
+  // where marchInTime writes the solution every nwrite steps
+  void setSolutionFile(const std::string& path) { solution_path_ = path; }
+  // continue from a solution file: q and the step count
+  void restart(const char* path) { step0_ = readSolution(path); }
+
+  // solution files (the nwrite / outform intent, solnBlock.h:23,26)
+  void writeSolution(const char* path, long step) {
+    check(hamstr_write_solution(ctx_, path, step), "SOLVER::writeSolution");
+  }
+  long readSolution(const char* path) {
+    long step = 0;
+    check(hamstr_read_solution(ctx_, path, &step), "SOLVER::readSolution");
+    return step;
+  }
+
   // field access (the reference's sb->q, r, r0, dq, sigma, ff)
   void get(int field, double* host) { check(hamstr_get_field(ctx_, field, host), "SOLVER::get"); }
   void set(int field, const double* host) { check(hamstr_set_field(ctx_, field, host), "SOLVER::set"); }
@@ -298,6 +316,8 @@ class SOLVER {
   SolnBlock* sb_;
   int lhs_, adi_;
   int scheme_ = HAMSTR_SCHEME_ADI;
+  long step0_ = 0;
+  std::string solution_path_ = "solution.bin";
   hamstr_ctx* ctx_ = nullptr;
   double lastRHS_ = 0;
 };

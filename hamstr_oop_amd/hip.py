@@ -39,6 +39,7 @@ EXPORTS = [
     "hamstr_mlp_set_params", "hamstr_mlp_forward", "hamstr_mlp_loss_grad", "hamstr_mlp_set_data",
     "hamstr_mlp_train", "hamstr_warm_start", "hamstr_mlp_last_error", "hamstr_mlp_set_graph",
     "hamstr_update_solution", "hamstr_set_scheme", "hamstr_get_scheme",
+    "hamstr_write_solution", "hamstr_read_solution",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -155,6 +156,16 @@ class Solver:
         out = np.zeros(4)
         _chk(lib().hamstr_freestream(self.h, _d(out)))
         return out
+
+    def write_solution(self, path: str, step: int = 0) -> None:
+        """Solution file of q (every case), the step count and a mesh fingerprint."""
+        _chk(lib().hamstr_write_solution(self.h, os.fsencode(path), C.c_long(step)))
+
+    def read_solution(self, path: str) -> int:
+        """Loads q from a solution file of this mesh (or batch); returns its step."""
+        step = C.c_long()
+        _chk(lib().hamstr_read_solution(self.h, os.fsencode(path), C.byref(step)))
+        return step.value
 
     # ---- SOLVER surface -----------------------------------------------------
     def initFlow(self):

@@ -218,12 +218,12 @@ def naca_ogrid(nc: int = 512, nr: int = 256, t: float = 0.12, r_outer: float = 2
 # ----------------------------------------------------------------------------
 
 def write_input_deck(path: str, mach: float, alpha: float, order: int, nsteps: int,
-                     cfl: float, msweep: int = 1, scheme: str = "ADI") -> None:
+                     cfl: float, msweep: int = 1, scheme: str = "ADI", nwrite: int = 0) -> None:
     """``input.hamstr``: the 16 positional key=value lines of io.cpp:26-41."""
     lines = [
         f"Mach={float(mach)!r}", f"alpha={float(alpha)!r}", "beta=0.0", "rey=1000000.0",
         f"scheme={scheme}", "time integration=steady", f"order={order}",
-        "timeacc=0", f"nsteps={nsteps}", "nwrite=0", "dt=0.0", f"CFL={float(cfl)!r}",
+        "timeacc=0", f"nsteps={nsteps}", f"nwrite={nwrite}", "dt=0.0", f"CFL={float(cfl)!r}",
         f"msweep={msweep}", "visc=0", "testcase=0", "output=0",
     ]
     with open(path, "w") as fh:
@@ -231,7 +231,7 @@ def write_input_deck(path: str, mach: float, alpha: float, order: int, nsteps: i
 
 
 def write_reference_case(mesh: Mesh, root: str, mach: float, alpha: float, order: int,
-                         nsteps: int, cfl: float, msweep: int = 1, scheme: str = "ADI") -> str:
+                         nsteps: int, cfl: float, msweep: int = 1, scheme: str = "ADI", nwrite: int = 0) -> str:
     """Write ``root/input.hamstr`` and ``root/input/domain000/*.dat``."""
     d = os.path.join(root, "input", "domain000")
     os.makedirs(d, exist_ok=True)
@@ -259,7 +259,7 @@ def write_reference_case(mesh: Mesh, root: str, mach: float, alpha: float, order
         for n1, n2, c1, e1, c2, e2 in mesh.faces:
             # file order is n1 n2 c1 c2 e1 e2 (io.cpp:349-355)
             fh.write(f"{n1} {n2} {c1} {c2} {e1} {e2}\n")
-    write_input_deck(os.path.join(root, "input.hamstr"), mach, alpha, order, nsteps, cfl, msweep, scheme)
+    write_input_deck(os.path.join(root, "input.hamstr"), mach, alpha, order, nsteps, cfl, msweep, scheme, nwrite)
     return root
 
 

@@ -173,6 +173,18 @@ int hamstr_get_field(hamstr_ctx *ctx, int field, double *host);
 int hamstr_get_cells(hamstr_ctx *ctx, int field, int first, int count, double *host);
 int hamstr_set_cells(hamstr_ctx *ctx, int field, int first, int count, const double *host);
 
+/* Solution files: the intent of SOLNBLOCK::nwrite / outform (solnBlock.h:23,26,
+ * read at io.cpp:35,41 and never used by the reference).  A file holds the
+ * state q of every case, the iteration count `step` given by the caller, the
+ * per-case Mach and alpha, and a fingerprint of the mesh (faces, chainStart,
+ * chainConn); little-endian, 64-byte header, 32 bytes per case, then
+ * q[4*nCell] doubles.  Writing goes through `path`.tmp and a rename.  Every
+ * scheme's step depends on q alone (rk3 rewrites qt before reading it), so a
+ * run restarted from a file continues bit for bit.  Reading fails with
+ * HAMSTR_ERR_ARG for a file of another mesh or batch. */
+int hamstr_write_solution(hamstr_ctx *ctx, const char *path, long step);
+int hamstr_read_solution(hamstr_ctx *ctx, const char *path, long *step);
+
 /* ---- the iteration (SOLVER methods) ------------------------------------- */
 
 /* SOLVER::computeRHS(char*) (computeRHS.cpp:17-427); implicit != 0 is

@@ -2,6 +2,7 @@
 entry point include/hamstr_hip.h declares, runs its host-only set-up
 (MESHBLOCK::preprocess) bit-identically to the oracle, and refuses device
 work with HAMSTR_ERR_NODEV instead of falling back to the CPU."""
+import ctypes
 import os
 import re
 
@@ -62,6 +63,11 @@ def test_bad_arguments_are_rejected():
     assert L.hamstr_case_status(None, None, None) == hip.ERR_ARG
     assert L.hamstr_mlp_forward(None, 1, None, None) == hip.ERR_ARG
     assert L.hamstr_warm_start(None, None, None, None, None) == hip.ERR_ARG
+    assert L.hamstr_write_solution(None, b"x", 0) == hip.ERR_ARG
+    assert L.hamstr_read_solution(None, b"x", None) == hip.ERR_ARG
+    assert L.hamstr_set_scheme(None, 1) == hip.ERR_ARG
+    assert L.hamstr_get_scheme(None) == -1
+    assert L.hamstr_update_solution(None, 0, 6, ctypes.c_double(1.0)) == hip.ERR_ARG
 
 
 def test_driver_built():
