@@ -75,6 +75,7 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
 struct Consts {
   double cfl, gamma, qt, eps;
   int nCell, nProc, order;
+  int nChainProc;  // processed faces [0, nChainProc) lie on chains, [nChainProc, nProc) are orphan faces
 };
 
 struct FaceTab {
@@ -102,6 +103,12 @@ struct ChainTab {
   const int* pright;  // per processed face: right cell (or boundary id)
   const int* pface;   // per processed face: face id
   const double* refmtx;
+  // orphan faces (null when there are none): od_chain[cell] is the pbase of
+  // the chain whose rows take the cell's orphan diagonal blocks; od_off /
+  // od_ent list the cell's orphan processed faces, code = p << 1 | (cell is left)
+  const int* od_chain;
+  const int* od_off;
+  const int* od_ent;
 };
 
 __device__ __forceinline__ double4 load_state(const int2 e, const double* __restrict__ q,
@@ -193,7 +200,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_faces(FaceTab ft, Consts c, Live
                                                       double* __restrict__ flux, double* __restrict__ sr,
                                                       TL* __restrict__ ff, int* __restrict__ nanflag) {
   const int p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (p >= c.nProc || (lv.face && !lv.face[p])) return;
+  if (p >= c.nChainProc || (lv.face && !lv.face[p])) return;
   const int code = ft.pslot[p];
   const int s = code & 0x3fffffff;
   const bool flip = (code >> 30) & 1;
@@ -531,6 +538,21 @@ __device__ __forceinline__ void assemble_row(const ChainTab& ct, const T* __rest
   }
   const int cell = ct.pleft[pm];
   *cell_out = cell;
+  if (ct.od_chain && ct.od_chain[cell] == pb) {
+    // the cell's orphan faces: + lmat where it is the left cell, - rmat where right
+    const int e1 = ct.od_off[cell + 1];
+    for (int e = ct.od_off[cell]; e < e1; e++) {
+      const int code = ct.od_ent[e];
+      const int po = code >> 1;
+      if (code & 1) {
+#pragma unroll
+        for (int k = 0; k < 16; k++) B[k] += ff[(size_t)k * nProc + po];
+      } else {
+#pragma unroll
+        for (int k = 0; k < 16; k++) B[k] -= ff[(size_t)(16 + k) * nProc + po];
+      }
+    }
+  }
   const T dtfac = T(cfl / sigma[cell]);
 #pragma unroll
   for (int e = 0; e < 16; e++) {
@@ -1150,7 +1172,7 @@ __global__ __launch_bounds__(kBlock) void k_lin_faces(FaceTab ft, Consts c, Live
                                                       const double* __restrict__ dq, const TL* __restrict__ ff,
                                                       double* __restrict__ lflux) {
   const int p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (p >= c.nProc || (lv.face && !lv.face[p])) return;
+  if (p >= c.nChainProc || (lv.face && !lv.face[p])) return;
   const int code = ft.pslot[p];
   const int s = code & 0x3fffffff;
   const bool flip = (code >> 30) & 1;
@@ -1171,6 +1193,71 @@ __global__ __launch_bounds__(kBlock) void k_lin_faces(FaceTab ft, Consts c, Live
     dl[k] = flip ? dln[k] : drn1[k];
     dr[k] = flip ? drn1[k] : dln[k];
   }
+  double out[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    double acc = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const double lm = double(ff[(size_t)(4 * j + k) * c.nProc + p]);
+      const double rm = double(ff[(size_t)(16 + 4 * j + k) * c.nProc + p]);
+      acc += ((lm * dl[k]) + (rm * dr[k]));
+    }
+    out[j] = acc;
+  }
+  *reinterpret_cast<double4*>(lflux + 4 * (size_t)p) = make_double4(out[0], out[1], out[2], out[3]);
+}
+
+// --------------------------------------------------------------------------
+// Orphan faces (interior faces in no chain; meshes with triangles).  An
+// extension with no counterpart in the reference: the first-order Roe flux of
+// the two cell states (and its frozen-Roe Jacobians), and in the linear
+// residual the full product lmat dq_L + rmat dq_R.  Their entries come after
+// the chain entries in each cell's list, as the oracle sums them.
+// --------------------------------------------------------------------------
+template <typename TL, bool IMPL, bool FASTJ>
+__global__ __launch_bounds__(kBlock) void k_orphan_faces(Consts c, Live lv, const int* __restrict__ pleft,
+                                                         const int* __restrict__ pright,
+                                                         const double* __restrict__ nvec,
+                                                         const double* __restrict__ q, double* __restrict__ flux,
+                                                         double* __restrict__ sr, TL* __restrict__ ff,
+                                                         int* __restrict__ nanflag) {
+  const int p = c.nChainProc + blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= c.nProc || (lv.face && !lv.face[p])) return;
+  const double4 a = *reinterpret_cast<const double4*>(q + 4 * (size_t)pleft[p]);
+  const double4 b = *reinterpret_cast<const double4*>(q + 4 * (size_t)pright[p]);
+  double left[4], right[4];
+  put4(left, a);
+  put4(right, b);
+  const double2 n = *reinterpret_cast<const double2*>(nvec + 2 * (size_t)p);
+  double fl[4], s_r;
+  flux_roe2d(n.x, n.y, left, right, c.gamma, fl, &s_r);
+  *reinterpret_cast<double4*>(flux + 4 * (size_t)p) = make_double4(fl[0], fl[1], fl[2], fl[3]);
+  sr[p] = s_r;
+  if (s_r != s_r) atomicOr(nanflag, 1);
+  if (IMPL) {
+    TL lm[16], rm[16];
+    if (FASTJ)
+      jacobian_roe2d_fast<TL>(n.x, n.y, left, right, c.gamma, lm, rm);
+    else
+      jacobian_roe2d<TL>(n.x, n.y, left, right, c.gamma, lm, rm);
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+      ff[(size_t)e * c.nProc + p] = lm[e];
+      ff[(size_t)(16 + e) * c.nProc + p] = rm[e];
+    }
+  }
+}
+
+template <typename TL>
+__global__ __launch_bounds__(kBlock) void k_orphan_lin(Consts c, Live lv, const int* __restrict__ pleft,
+                                                       const int* __restrict__ pright, const double* __restrict__ dq,
+                                                       const TL* __restrict__ ff, double* __restrict__ lflux) {
+  const int p = c.nChainProc + blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= c.nProc || (lv.face && !lv.face[p])) return;
+  double dl[4], dr[4];
+  put4(dl, *reinterpret_cast<const double4*>(dq + 4 * (size_t)pleft[p]));
+  put4(dr, *reinterpret_cast<const double4*>(dq + 4 * (size_t)pright[p]));
   double out[4];
 #pragma unroll
   for (int j = 0; j < 4; j++) {
@@ -1550,6 +1637,9 @@ struct hamstr_ctx {
   int* case_off = nullptr;
   std::vector<int> case_off_h;
   int *pleft = nullptr, *pright = nullptr, *pface = nullptr;
+  int nChainProc = 0;                                       // processed faces on chains; orphans follow
+  int *od_chain = nullptr, *od_off = nullptr, *od_ent = nullptr;  // orphan diagonal tables (or null)
+  std::vector<int> case_orph_off;                           // [nCase+1] orphan processed-face ranges
   int *cell_off = nullptr, *cell_ent = nullptr;
   int *ch_pbase = nullptr, *ch_nf = nullptr, *ch_ncell = nullptr, *ch_periodic = nullptr;
   int* group_chains = nullptr;
@@ -1594,7 +1684,7 @@ struct hamstr_ctx {
   }
   FaceTab ftab() const { return FaceTab{pslot, src, nvec, refmtx, qinf}; }
   ChainTab ctab() const {
-    return ChainTab{ch_pbase, ch_nf, ch_ncell, ch_periodic, pleft, pright, pface, refmtx};
+    return ChainTab{ch_pbase, ch_nf, ch_ncell, ch_periodic, pleft, pright, pface, refmtx, od_chain, od_off, od_ent};
   }
   bool lhs32() const { return prm.lhs == HAMSTR_LHS_FP32; }
 };
@@ -1603,7 +1693,7 @@ namespace {
 
 int launch_rhs_faces(hamstr_ctx* x, bool implicit) {
   const FaceTab ft = x->ftab();
-  dim3 g(nblk(x->nProc)), b(kBlock);
+  dim3 g(nblk(x->nChainProc)), b(kBlock);
 #define RHSF(O, T, I, F)                                                                                         \
   hipLaunchKernelGGL((k_rhs_faces<O, T, I, F>), g, b, 0, x->stream, ft, x->c, x->live(), x->q, x->flux, x->sr, (T*)x->ff, \
                      x->nanflag)
@@ -1623,12 +1713,29 @@ int launch_rhs_faces(hamstr_ctx* x, bool implicit) {
 #undef RHSO
 #undef RHSF
   HIPCHK(hipGetLastError());
+  const int nOrph = x->nProc - x->nChainProc;
+  if (nOrph > 0) {
+#define ORPH(T, I, F)                                                                                            \
+  hipLaunchKernelGGL((k_orphan_faces<T, I, F>), dim3(nblk(nOrph)), b, 0, x->stream, x->c, x->live(), x->pleft,     \
+                     x->pright, x->nvec, x->q, x->flux, x->sr, (T*)x->ff, x->nanflag)
+    if (x->lhs32()) {
+      if (!implicit) ORPH(float, false, false);
+      else if (fj) ORPH(float, true, true);
+      else ORPH(float, true, false);
+    } else {
+      if (!implicit) ORPH(double, false, false);
+      else if (fj) ORPH(double, true, true);
+      else ORPH(double, true, false);
+    }
+#undef ORPH
+    HIPCHK(hipGetLastError());
+  }
   return HAMSTR_OK;
 }
 
 int launch_lin_faces(hamstr_ctx* x) {
   const FaceTab ft = x->ftab();
-  dim3 g(nblk(x->nProc)), b(kBlock);
+  dim3 g(nblk(x->nChainProc)), b(kBlock);
 #define LINF(O, T, F)                                                                                         \
   hipLaunchKernelGGL((k_lin_faces<O, T, F>), g, b, 0, x->stream, ft, x->c, x->live(), x->q, x->dq, (const T*)x->ff, \
                      x->lflux)
@@ -1644,6 +1751,16 @@ int launch_lin_faces(hamstr_ctx* x) {
 #undef LINO
 #undef LINF
   HIPCHK(hipGetLastError());
+  const int nOrph = x->nProc - x->nChainProc;
+  if (nOrph > 0) {
+    if (x->lhs32())
+      hipLaunchKernelGGL((k_orphan_lin<float>), dim3(nblk(nOrph)), b, 0, x->stream, x->c, x->live(), x->pleft,
+                         x->pright, x->dq, (const float*)x->ff, x->lflux);
+    else
+      hipLaunchKernelGGL((k_orphan_lin<double>), dim3(nblk(nOrph)), b, 0, x->stream, x->c, x->live(), x->pleft,
+                         x->pright, x->dq, (const double*)x->ff, x->lflux);
+    HIPCHK(hipGetLastError());
+  }
   return HAMSTR_OK;
 }
 
@@ -1781,7 +1898,7 @@ void free_ctx(hamstr_ctx* x) {
                   x->nvec, x->refmtx, x->qinf, x->cell_case, x->case_off, x->pleft, x->pright, x->pface,
                   x->cell_off, x->cell_ent, x->ch_pbase, x->ch_nf, x->ch_ncell, x->ch_periodic, x->group_chains,
                   x->scratch, x->partial, x->norms, x->hist, x->nanflag, x->rrho, x->face_on, x->cell_on,
-                  x->chain_on, x->ghist, x->npart, x->keep_norms};
+                  x->chain_on, x->ghist, x->npart, x->keep_norms, x->od_chain, x->od_off, x->od_ent};
   if (x->gexec) (void)hipGraphExecDestroy(x->gexec);
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -1940,6 +2057,23 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
       if (++proc_count[iface] > 1) return fail(HAMSTR_ERR_MESH, "face processed by more than one chain");
     }
   }
+  const int nChainProc = (int)pslot.size();
+  // ---- orphan faces (extension for meshes with triangles): interior faces in
+  // no chain, processed after every chain face in list order ----
+  if (mesh->nOrphan < 0 || (mesh->nOrphan > 0 && !mesh->orphans))
+    return fail(HAMSTR_ERR_ARG, "bad orphan face list");
+  for (int o = 0; o < mesh->nOrphan; o++) {
+    const int iface = mesh->orphans[o];
+    if (iface < 0 || iface >= nFace) return fail(HAMSTR_ERR_MESH, "orphan face out of range");
+    if (faces[6 * iface + 4] < 0) return fail(HAMSTR_ERR_MESH, "orphan faces must be interior faces");
+    if (++proc_count[iface] > 1) return fail(HAMSTR_ERR_MESH, "orphan face also on a chain or listed twice");
+    pslot.push_back(0);  // no chain slot
+    pleft.push_back(faces[6 * iface + 2]);
+    pright.push_back(faces[6 * iface + 4]);
+    pface.push_back(iface);
+    nvec.push_back(mesh->normVec[2 * iface]);
+    nvec.push_back(mesh->normVec[2 * iface + 1]);
+  }
   const int nProc = (int)pslot.size();
   const int nSlot = (int)src.size();
   if (nSlot >= (1 << 30)) return fail(HAMSTR_ERR_MESH, "too many chain slots");
@@ -1956,6 +2090,25 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
   for (int p = 0; p < nProc; p++) {
     cell_ent[fill[pleft[p]]++] = (p << 1) | 0;
     if (pright[p] > -1) cell_ent[fill[pright[p]]++] = (p << 1) | 1;
+  }
+  // orphan diagonal blocks go to the rows of the last chain listing the cell
+  std::vector<int> od_chain, od_off, od_ent;
+  if (nProc > nChainProc) {
+    od_chain.assign(nCell, -1);
+    for (int i = 0; i < nChain; i++)
+      for (int k = 0; k < ch_ncell[i]; k++) od_chain[pleft[ch_pbase[i] + k]] = ch_pbase[i];
+    od_off.assign(nCell + 1, 0);
+    for (int p = nChainProc; p < nProc; p++) {
+      od_off[pleft[p] + 1]++;
+      od_off[pright[p] + 1]++;
+    }
+    for (int i = 0; i < nCell; i++) od_off[i + 1] += od_off[i];
+    od_ent.assign(od_off[nCell], 0);
+    std::vector<int> at(od_off.begin(), od_off.end() - 1);
+    for (int p = nChainProc; p < nProc; p++) {
+      od_ent[at[pleft[p]]++] = (p << 1) | 1;
+      od_ent[at[pright[p]]++] = (p << 1);
+    }
   }
 
   // ---- ADI phases: chain i runs after every earlier chain sharing a cell ----
@@ -2025,6 +2178,7 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
   x->nFace = nFace;
   x->nChain = nChain;
   x->nProc = nProc;
+  x->nChainProc = nChainProc;
   x->nSlot = nSlot;
   x->nGhost = nGhost;
   x->nCase = nCase;
@@ -2053,13 +2207,25 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
   }
   // chains, and so processed faces, are stored case by case
   x->case_chain_off.assign(nCase + 1, nChain);
-  x->case_proc_off.assign(nCase + 1, nProc);
+  x->case_proc_off.assign(nCase + 1, nChainProc);
   for (int k = 0; k < nCase; k++) {
     x->case_chain_off[k] = cases[k].chainBase;
-    x->case_proc_off[k] = cases[k].chainCount > 0 ? ch_pbase[cases[k].chainBase] : nProc;
+    x->case_proc_off[k] = cases[k].chainCount > 0 ? ch_pbase[cases[k].chainBase] : nChainProc;
   }
   for (int k = nCase - 1; k >= 0; k--)
     if (cases[k].chainCount == 0) x->case_proc_off[k] = x->case_proc_off[k + 1];
+  // orphan faces, listed case by case: [case_orph_off[k], case_orph_off[k+1])
+  x->case_orph_off.assign(nCase + 1, nProc);
+  for (int p = nProc - 1; p >= nChainProc; p--) x->case_orph_off[cell_case[pleft[p]]] = p;
+  for (int k = nCase - 1; k >= 0; k--) x->case_orph_off[k] = std::min(x->case_orph_off[k], x->case_orph_off[k + 1]);
+  for (int p = nChainProc; p < nProc; p++) {
+    const int k = cell_case[pleft[p]];
+    if (p < x->case_orph_off[k] || p >= x->case_orph_off[k + 1] || cell_case[pright[p]] != k) {
+      free_ctx(x);
+      delete x;
+      return fail(HAMSTR_ERR_MESH, "orphan faces must be listed case by case");
+    }
+  }
   x->case_status.assign(nCase, 0);
   x->case_iters.assign(nCase, -1);
   x->stall_ref.assign(nCase, HUGE_VAL);
@@ -2073,6 +2239,7 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
   x->c.eps = 1e-10;
   x->c.nCell = nCell;
   x->c.nProc = nProc;
+  x->c.nChainProc = nChainProc;
   x->c.order = P.order;
   x->case_norm.assign(4 * (size_t)nCase, 0.0);
 
@@ -2118,7 +2285,9 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
       (rc = upload(&x->pright, pright)) || (rc = upload(&x->pface, pface)) || (rc = upload(&x->cell_off, cnt)) ||
       (rc = upload(&x->cell_ent, cell_ent)) || (rc = upload(&x->ch_pbase, ch_pbase)) ||
       (rc = upload(&x->ch_nf, ch_nf)) || (rc = upload(&x->ch_ncell, ch_ncell)) ||
-      (rc = upload(&x->ch_periodic, ch_periodic)) || (rc = upload(&x->group_chains, group_chains))) {
+      (rc = upload(&x->ch_periodic, ch_periodic)) || (rc = upload(&x->group_chains, group_chains)) ||
+      (!od_chain.empty() && ((rc = upload(&x->od_chain, od_chain)) || (rc = upload(&x->od_off, od_off)) ||
+                             (rc = upload(&x->od_ent, od_ent))))) {
     free_ctx(x);
     delete x;
     return rc;
@@ -2196,19 +2365,21 @@ int hamstr_batch_create(int ncases, const hamstr_mesh_desc* meshes, const hamstr
         params[k].lhs != P0.lhs || params[k].adi_mode != P0.adi_mode)
       return fail(HAMSTR_ERR_ARG, "batched cases must share cfl, order, msweep, lhs and adi_mode");
   // disjoint union of the meshes
-  long nc = 0, nf = 0, nch = 0, ncf = 0;
+  long nc = 0, nf = 0, nch = 0, ncf = 0, nor = 0;
   for (int k = 0; k < ncases; k++) {
     const hamstr_mesh_desc& m = meshes[k];
     if (m.nCell <= 0 || m.nFace <= 0 || m.nChain <= 0 || !m.faces || !m.chainStart || !m.chainConn ||
-        !m.normVec || !m.refMtx)
+        !m.normVec || !m.refMtx || m.nOrphan < 0 || (m.nOrphan > 0 && !m.orphans))
       return fail(HAMSTR_ERR_ARG, "hamstr_batch_create: bad mesh description");
     nc += m.nCell;
     nf += m.nFace;
     nch += m.nChain;
     ncf += m.chainStart[m.nChain];
+    nor += m.nOrphan;
   }
   if (nc >= (1L << 30) || nf >= (1L << 29)) return fail(HAMSTR_ERR_ARG, "batch too large");
-  std::vector<int> faces(6 * nf), fs(nch + 1), conn(ncf);
+  std::vector<int> faces(6 * nf), fs(nch + 1), conn(ncf), orph;
+  orph.reserve(nor);
   std::vector<double> nv(2 * nf), rm(4 * nf);
   std::vector<CaseInfo> cases(ncases);
   long c0 = 0, f0 = 0, h0 = 0, cf0 = 0;
@@ -2229,13 +2400,15 @@ int hamstr_batch_create(int ncases, const hamstr_mesh_desc* meshes, const hamstr
     std::memcpy(rm.data() + 4 * f0, m.refMtx, sizeof(double) * 4 * m.nFace);
     for (int i = 0; i < m.nChain; i++) fs[h0 + i + 1] = (int)cf0 + m.chainStart[i + 1];
     for (int f = 0; f < m.chainStart[m.nChain]; f++) conn[cf0 + f] = m.chainConn[f] + (int)f0;
+    for (int o = 0; o < m.nOrphan; o++) orph.push_back(m.orphans[o] + (int)f0);
     cases[k] = CaseInfo{(int)c0, m.nCell, (int)h0, m.nChain, params[k].mach, params[k].alpha};
     c0 += m.nCell;
     f0 += m.nFace;
     h0 += m.nChain;
     cf0 += m.chainStart[m.nChain];
   }
-  hamstr_mesh_desc u{(int)nc, (int)nf, (int)nch, faces.data(), fs.data(), conn.data(), nv.data(), rm.data()};
+  hamstr_mesh_desc u{(int)nc,     (int)nf,   (int)nch,  faces.data(),        fs.data(), conn.data(),
+                     nv.data(),    rm.data(), (int)nor,  orph.empty() ? nullptr : orph.data()};
   const int rc = create_internal(&u, P0, cases, out);
   if (rc) return rc;
   return hamstr_init_flow(*out);
@@ -2527,6 +2700,7 @@ int retire_case(hamstr_ctx* x, int k, int status, long it) {
                          (unsigned char)0);
   };
   fill(x->face_on, x->case_proc_off[k], x->case_proc_off[k + 1]);
+  fill(x->face_on, x->case_orph_off[k], x->case_orph_off[k + 1]);
   fill(x->cell_on, x->case_off_h[k], x->case_off_h[k + 1]);
   fill(x->chain_on, x->case_chain_off[k], x->case_chain_off[k + 1]);
   HIPCHK(hipGetLastError());
@@ -2926,6 +3100,7 @@ int hamstr_set_case_live(hamstr_ctx* x, int k, int live) {
                          (unsigned char)1);
   };
   fill(x->face_on, x->case_proc_off[k], x->case_proc_off[k + 1]);
+  fill(x->face_on, x->case_orph_off[k], x->case_orph_off[k + 1]);
   fill(x->cell_on, x->case_off_h[k], x->case_off_h[k + 1]);
   fill(x->chain_on, x->case_chain_off[k], x->case_chain_off[k + 1]);
   HIPCHK(hipGetLastError());

@@ -42,6 +42,7 @@ struct MeshBlock {
   std::vector<int> faces;            // [6*nFace] n1 n2 cL eL cR eR
   std::vector<double> normVec;       // [2*nFace]
   std::vector<double> refMtx;        // [4*nFace]
+  std::vector<int> orphans;          // interior faces on no chain (orphans.dat; meshes with triangles)
 
   // MESHBLOCK::readMeshData (io.cpp:58-372, 2D): ./input/domainNNN/*.dat
   // relative to the working directory.  Returns false (after printing) when a
@@ -113,6 +114,17 @@ struct MeshBlock {
       }
     }
     std::fclose(fp);
+    // orphans.dat (count, then face ids): optional, no counterpart in the
+    // reference, which has no way to represent faces outside every chain
+    orphans.clear();
+    if ((fp = std::fopen((d + "orphans.dat").c_str(), "r"))) {
+      int no = 0;
+      if (std::fscanf(fp, "%d", &no) != 1 || no < 0) return false;
+      orphans.assign(no, 0);
+      for (int i = 0; i < no; i++)
+        if (std::fscanf(fp, "%d", &orphans[i]) != 1) return false;
+      std::fclose(fp);
+    }
     return true;
   }
 
@@ -189,7 +201,8 @@ class SOLVER {
         std::exit(1);
       }
       hamstr_mesh_desc m{mb_->nCell, mb_->nFace, mb_->nChain, mb_->faces.data(), mb_->faceStartPerChain.data(),
-                         mb_->chainConn.data(), mb_->normVec.data(), mb_->refMtx.data()};
+                         mb_->chainConn.data(), mb_->normVec.data(), mb_->refMtx.data(),
+                         (int)mb_->orphans.size(), mb_->orphans.empty() ? nullptr : mb_->orphans.data()};
       hamstr_params p{sb_->Mach, sb_->alpha, sb_->CFL, sb_->order, sb_->msweep, lhs_, adi_};
       check(hamstr_create(&m, &p, &ctx_), "SOLVER::initFlow");
       check(hamstr_set_scheme(ctx_, scheme_), "SOLVER::initFlow");

@@ -55,7 +55,7 @@ class HamstrError(RuntimeError):
 class MeshDesc(C.Structure):
     _fields_ = [("nCell", C.c_int), ("nFace", C.c_int), ("nChain", C.c_int),
                 ("faces", _ip), ("chainStart", _ip), ("chainConn", _ip),
-                ("normVec", _dp), ("refMtx", _dp)]
+                ("normVec", _dp), ("refMtx", _dp), ("nOrphan", C.c_int), ("orphans", _ip)]
 
 
 class Params(C.Structure):
@@ -95,6 +95,11 @@ def _i(a):
     return a.ctypes.data_as(_ip)
 
 
+def _orphans(mesh) -> np.ndarray:
+    """The mesh's orphan faces (interior faces on no chain), or an empty array."""
+    return np.ascontiguousarray(getattr(mesh, "orphans", np.zeros(0, np.int32)), np.int32)
+
+
 def preprocess(x: np.ndarray, faces: np.ndarray, sol_visc: int = 0):
     """MESHBLOCK::preprocess: returns (faces with wall tags, normVec, refMtx)."""
     x = np.ascontiguousarray(x, np.float64)
@@ -116,10 +121,11 @@ class Solver:
         self.faces, self.normvec, self.refmtx = faces, nv, rm
         self.n_cell = mesh.n_cell
         self.n_face = mesh.n_face
+        orph = _orphans(mesh)
         self._keep = [faces, nv, rm, np.ascontiguousarray(mesh.chain_start, np.int32),
-                      np.ascontiguousarray(mesh.chain_conn, np.int32)]
+                      np.ascontiguousarray(mesh.chain_conn, np.int32), orph]
         md = MeshDesc(mesh.n_cell, mesh.n_face, mesh.n_chain, _i(faces), _i(self._keep[3]),
-                      _i(self._keep[4]), _d(nv), _d(rm))
+                      _i(self._keep[4]), _d(nv), _d(rm), int(orph.size), _i(orph) if orph.size else None)
         prm = Params(float(mach), float(alpha), float(cfl), int(order), int(msweep), int(lhs), int(adi_mode))
         h = C.c_void_p()
         _chk(lib().hamstr_create(C.byref(md), C.byref(prm), C.byref(h)))
@@ -366,8 +372,10 @@ class Batch(Solver):
             faces, nv, rm = preprocess(mesh.x, mesh.faces, sol_visc)
             st = np.ascontiguousarray(mesh.chain_start, np.int32)
             cc = np.ascontiguousarray(mesh.chain_conn, np.int32)
-            self._keep += [faces, nv, rm, st, cc]
-            descs[k] = MeshDesc(mesh.n_cell, mesh.n_face, mesh.n_chain, _i(faces), _i(st), _i(cc), _d(nv), _d(rm))
+            orph = _orphans(mesh)
+            self._keep += [faces, nv, rm, st, cc, orph]
+            descs[k] = MeshDesc(mesh.n_cell, mesh.n_face, mesh.n_chain, _i(faces), _i(st), _i(cc), _d(nv), _d(rm),
+                                int(orph.size), _i(orph) if orph.size else None)
             prms[k] = Params(float(mach), float(alpha), float(cfl), int(order), int(msweep), int(lhs), int(adi_mode))
             self.meshes.append(mesh)
         h = C.c_void_p()

@@ -43,6 +43,8 @@ class Mesh:
     chain_conn: np.ndarray   # (nChainFace,) int32 face ids
     chains_per_color: List[int]
     name: str = "ogrid"
+    # interior faces in no chain (meshes with triangles; empty for all-quad meshes)
+    orphans: np.ndarray = dataclasses.field(default_factory=lambda: np.zeros(0, np.int32))
 
     @property
     def n_cell(self) -> int:
@@ -203,14 +205,116 @@ def build_ogrid(xy: np.ndarray, loop_start: int = 2, name: str = "ogrid") -> Mes
                 chain_conn=conn_list, chains_per_color=[nr, nc], name=name)
 
 
-def circle_ogrid(nc: int = 128, nr: int = 64, r_outer: float = 20.0, ratio: float = 1.08) -> Mesh:
-    return build_ogrid(circle_nodes(nc, nr, r_outer, ratio), name=f"circle{nc}x{nr}")
+def build_ogrid_tri(xy: np.ndarray, tri_layers: int, loop_start: int = 2, name: str = "ogrid_tri") -> Mesh:
+    """An O-grid whose outer ``tri_layers`` layers are split into triangles.
+
+    Quad (i, j) of an outer layer is cut along the diagonal from node (i, j)
+    to node (i+1, j+1) into an inner triangle (on ring face (i, j)) and an
+    outer one (on ring face (i, j+1)).  A chain enters a cell through one face
+    and leaves through another, so a triangle leaves one face uncovered:
+    radial lines run ring face -> inner triangle -> diagonal -> outer triangle
+    -> ring face, closed loops exist for the quad layers only, and the radial
+    faces of the triangle layers (nc per layer, each between the outer
+    triangle of column k and the inner triangle of column k-1) are orphan
+    faces, which the solver treats face by face.
+
+    Cells: quads c = i + nc*j for j < nq = nr - tri_layers, then per triangle
+    layer and column the inner and the outer triangle.  Faces: loop faces of
+    the quad layers, then per line its ring faces and diagonals in chain
+    order, then the orphans.  conn lists a triangle's three corners
+    counter-clockwise with the last one repeated.
+    """
+    nc, nr1, _ = xy.shape
+    nr = nr1 - 1
+    nq = nr - tri_layers
+    if tri_layers <= 0:
+        return build_ogrid(xy, loop_start, name)
+    if nc < 5 or nq < 1 or nr < 3:
+        raise ValueError("needs nc >= 5, nr >= 3 and at least one quad layer")
+    x = xy.transpose(1, 0, 2).reshape(-1, 2).copy()
+
+    def node(i, j):
+        return (np.asarray(i) % nc) + nc * np.asarray(j)
+
+    def quad(i, j):
+        return (np.asarray(i) % nc) + nc * np.asarray(j)
+
+    def tri_in(i, j):
+        return nc * nq + 2 * ((np.asarray(i) % nc) + nc * (np.asarray(j) - nq))
+
+    def tri_out(i, j):
+        return tri_in(i, j) + 1
+
+    def above(i, j):   # cell on the outer side of ring face (i, j)
+        return quad(i, j) if j < nq else tri_in(i, j)
+
+    def below(i, j):   # cell on the inner side of ring face (i, j), j >= 1
+        return quad(i, j - 1) if j - 1 < nq else tri_out(i, j - 1)
+
+    ncell = nc * nq + 2 * nc * tri_layers
+    conn = np.empty((ncell, 4), np.int32)
+    I, J = np.meshgrid(np.arange(nc), np.arange(nq), indexing="xy")
+    conn[quad(I, J).ravel()] = np.stack([node(I + 1, J), node(I, J), node(I, J + 1), node(I + 1, J + 1)],
+                                        axis=-1).reshape(-1, 4)
+    It, Jt = np.meshgrid(np.arange(nc), np.arange(nq, nr), indexing="xy")
+    a, b, c, d = node(It, Jt), node(It + 1, Jt), node(It + 1, Jt + 1), node(It, Jt + 1)
+    conn[tri_in(It, Jt).ravel()] = np.stack([a, c, b, b], axis=-1).reshape(-1, 4)
+    conn[tri_out(It, Jt).ravel()] = np.stack([a, d, c, c], axis=-1).reshape(-1, 4)
+
+    faces = []
+    # closed loops of the quad layers: radial face at node column k of layer j
+    K = I
+    loop_faces = np.stack([node(K, J), node(K, J + 1), quad(K, J), np.full_like(K, 3), quad(K - 1, J),
+                           np.full_like(K, 1)], axis=-1).reshape(-1, 6)
+    faces.append(loop_faces)
+    n_loop = loop_faces.shape[0]
+    # lines: ring faces and diagonals in chain order
+    per_line = nr + 1 + tri_layers
+    lines = np.empty((nc, per_line, 6), np.int64)
+    for i in range(nc):
+        rows = [[node(i + 1, 0), node(i, 0), above(i, 0), 0, BC_FREESTREAM, -1]]
+        for j in range(1, nr + 1):
+            if j - 1 >= nq:   # diagonal of layer j-1: outer triangle left, inner right
+                rows.append([node(i + 1, j), node(i, j - 1), tri_out(i, j - 1), 0, tri_in(i, j - 1), 0])
+            if j < nr:
+                rows.append([node(i + 1, j), node(i, j), above(i, j), 0, below(i, j), 2])
+            else:
+                rows.append([node(i, nr), node(i + 1, nr), below(i, nr), 2, BC_FREESTREAM, -1])
+        lines[i] = np.array(rows, np.int64)
+    faces.append(lines.reshape(-1, 6))
+    # orphans: radial faces of the triangle layers
+    Ko, Jo = np.meshgrid(np.arange(nc), np.arange(nq, nr), indexing="xy")
+    orph = np.stack([node(Ko, Jo), node(Ko, Jo + 1), tri_out(Ko, Jo), np.full_like(Ko, 2), tri_in(Ko - 1, Jo),
+                     np.full_like(Ko, 1)], axis=-1).reshape(-1, 6)
+    faces.append(orph)
+    faces = np.concatenate(faces).astype(np.int32)
+    n_line = nc * per_line
+
+    ks = (loop_start + np.arange(nc)) % nc
+    loops = np.arange(nq)[:, None] * nc + ks[None, :]
+    loops = np.concatenate([loops, loops[:, :1]], axis=1)
+    line_ids = n_loop + np.arange(n_line).reshape(nc, per_line)
+    conn_list = np.concatenate([loops.ravel(), line_ids.ravel()]).astype(np.int32)
+    lens = [nc + 1] * nq + [per_line] * nc
+    start = np.zeros(len(lens) + 1, np.int32)
+    start[1:] = np.cumsum(lens)
+    orphans = (n_loop + n_line + np.arange(orph.shape[0])).astype(np.int32)
+    return Mesh(nc=nc, nr=nr, x=x, conn=conn, faces=faces, chain_start=start, chain_conn=conn_list,
+                chains_per_color=[nq, nc], name=name, orphans=orphans)
+
+
+def circle_ogrid(nc: int = 128, nr: int = 64, r_outer: float = 20.0, ratio: float = 1.08,
+                 tri_layers: int = 0) -> Mesh:
+    name = f"circle{nc}x{nr}" + (f"_tri{tri_layers}" if tri_layers else "")
+    return build_ogrid_tri(circle_nodes(nc, nr, r_outer, ratio), tri_layers, name=name)
 
 
 def naca_ogrid(nc: int = 512, nr: int = 256, t: float = 0.12, r_outer: float = 20.0,
-               first: Optional[float] = None, closed_te: bool = False) -> Mesh:
-    return build_ogrid(naca_nodes(nc, nr, t, r_outer, first, closed_te),
-                       name=f"naca{int(round(t*100)):02d}_{nc}x{nr}")
+               first: Optional[float] = None, closed_te: bool = False, tri_layers: int = 0) -> Mesh:
+    """NACA-00xx O-grid; tri_layers > 0 splits that many outer layers into
+    triangles (build_ogrid_tri)."""
+    name = f"naca{int(round(t*100)):02d}_{nc}x{nr}" + (f"_tri{tri_layers}" if tri_layers else "")
+    return build_ogrid_tri(naca_nodes(nc, nr, t, r_outer, first, closed_te), tri_layers, name=name)
 
 
 # ----------------------------------------------------------------------------
@@ -259,6 +363,11 @@ def write_reference_case(mesh: Mesh, root: str, mach: float, alpha: float, order
         for n1, n2, c1, e1, c2, e2 in mesh.faces:
             # file order is n1 n2 c1 c2 e1 e2 (io.cpp:349-355)
             fh.write(f"{n1} {n2} {c1} {c2} {e1} {e2}\n")
+    if mesh.orphans.size:
+        # no counterpart in the reference: faces outside every chain
+        with open(os.path.join(d, "orphans.dat"), "w") as fh:
+            fh.write(f"{mesh.orphans.size}\n")
+            fh.write("\n".join(str(int(v)) for v in mesh.orphans) + "\n")
     write_input_deck(os.path.join(root, "input.hamstr"), mach, alpha, order, nsteps, cfl, msweep, scheme, nwrite)
     return root
 
@@ -300,9 +409,13 @@ def read_reference_case(root: str) -> Mesh:
     faces[:, 3], faces[:, 5] = raw[:, 4], raw[:, 5]
     sw = faces[:, 2] == BC_FREESTREAM
     faces[sw] = faces[sw][:, [1, 0, 4, 5, 2, 3]]
+    orph = np.zeros(0, np.int32)
+    if os.path.exists(os.path.join(d, "orphans.dat")):
+        ov = ints("orphans.dat")
+        orph = np.array(ov[1:1 + ov[0]], np.int32)
     return Mesh(nc=0, nr=0, x=x, conn=conn, faces=faces,
                 chain_start=np.array(offs, np.int32), chain_conn=chain_conn,
-                chains_per_color=col[1:1 + col[0]], name=os.path.basename(root.rstrip("/")))
+                chains_per_color=col[1:1 + col[0]], name=os.path.basename(root.rstrip("/")), orphans=orph)
 
 
 def freestream(mach: float, alpha_deg: float, gamma: float = 1.4):

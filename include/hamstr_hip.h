@@ -57,6 +57,16 @@ typedef struct hamstr_mesh_desc {
   const int *chainConn;
   const double *normVec;
   const double *refMtx;
+  /* Orphan faces: interior faces that lie on no chain (a mesh with
+   * triangles, whose cells a chain cannot cover face by face).  An extension
+   * with no counterpart in the reference; nOrphan = 0 / orphans = NULL is the
+   * reference's all-chain mesh.  Each orphan face gets the first-order Roe
+   * flux of its two cell states; its Jacobian blocks join the diagonal of the
+   * ADI rows of the last chain listing each of its cells, and its full
+   * linearised flux joins computeLinearRHS.  In a batch, list them case by
+   * case. */
+  int nOrphan;
+  const int *orphans;
 } hamstr_mesh_desc;
 
 typedef struct hamstr_params {

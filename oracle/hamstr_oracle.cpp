@@ -600,6 +600,12 @@ struct Oracle {
   double rinf = 1, uinf = 0, vinf = 0, einf = 0, pinf = 0;
   std::vector<double> q, r, r0, dq, sigma, ff;  // ff: 32 per face (lmat then rmat)
   std::vector<double> qt;                       // rk3 stage-start state (SOLNBLOCK::qt)
+  // Orphan faces (an extension for meshes with triangles, which chains cannot
+  // cover face by face): interior faces in no chain, in list order.  od_chain
+  // [cell]: the chain whose ADI rows take the cell's orphan diagonal blocks
+  // (the last chain listing the cell as a row; -1 none); od_off/od_ent: per
+  // cell, its orphan faces in list order, code = face << 1 | (cell is left).
+  std::vector<int> orphans, od_chain, od_off, od_ent;
   double L2 = 0, Linf = 0;       // printed norm (overwritten by computeLinearRHS)
   double L2rhs = 0, Linfrhs = 0; // nonlinear norm of the last computeRHS
   double L2lin = 0, Linflin = 0; // linear norm of the last computeLinearRHS
@@ -760,6 +766,21 @@ void compute_rhs(Oracle& o, bool implicit) {
       n++;
     }
   }
+  // orphan faces: first-order Roe flux of the two cell states, summed after
+  // every chain's contributions (extension; no orphans = the reference)
+  for (int iface : o.orphans) {
+    int leftCell = FACE(o, iface, 2), rightCell = FACE(o, iface, 4);
+    double flux[NQ], sr;
+    double ds[2] = {o.normVec[2 * iface], o.normVec[2 * iface + 1]};
+    o.nanFlag |= flux_roe2d(ds, &o.q[NQ * leftCell], &o.q[NQ * rightCell], flux, &sr, o.gamma);
+    if (implicit)
+      jacobian_roe2d(ds, &o.q[NQ * leftCell], &o.q[NQ * rightCell], &o.ff[32 * iface], &o.ff[32 * iface + 16],
+                     o.gamma);
+    for (int j = 0; j < NQ; j++) o.r[NQ * leftCell + j] -= flux[j];
+    o.sigma[leftCell] += sr;
+    for (int j = 0; j < NQ; j++) o.r[NQ * rightCell + j] += flux[j];
+    o.sigma[rightCell] += sr;
+  }
   norms(o, &o.L2, &o.Linf);
   o.L2rhs = o.L2;
   o.Linfrhs = o.Linf;
@@ -810,6 +831,23 @@ void compute_linear_rhs(Oracle& o) {
       }
       n++;
     }
+  }
+  // orphan faces: the full linearised flux lmat dq_L + rmat dq_R (extension)
+  for (int iface : o.orphans) {
+    int leftCell = FACE(o, iface, 2), rightCell = FACE(o, iface, 4);
+    const double* L = &o.ff[32 * iface];
+    const double* R = &o.ff[32 * iface + 16];
+    const double* dl = &o.dq[NQ * leftCell];
+    const double* dr = &o.dq[NQ * rightCell];
+    double flux[NQ];
+    for (int j = 0; j < NQ; j++) {
+      flux[j] = 0;
+      for (int k = 0; k < NQ; k++) flux[j] += ((L[4 * j + k] * dl[k]) + (R[4 * j + k] * dr[k]));
+    }
+    double dtfac = o.CFL / o.sigma[leftCell];
+    for (int j = 0; j < NQ; j++) o.r[NQ * leftCell + j] -= (flux[j] * dtfac);
+    dtfac = o.CFL / o.sigma[rightCell];
+    for (int j = 0; j < NQ; j++) o.r[NQ * rightCell + j] += (flux[j] * dtfac);
   }
   norms(o, &o.L2, &o.Linf);
   o.L2lin = o.L2;
@@ -891,6 +929,27 @@ void adi(Oracle& o) {
       }
       m = 0;
       chainSize = chainSize - (iPeriodic == 0);
+      // orphan diagonal blocks of the rows whose cell designates this chain
+      // (extension): + lmat where the cell is the face's left, - rmat where right
+      if (!o.orphans.empty()) {
+        for (int f = f1; f < f2 - 1; f++, m++) {
+          int cell = FACE(o, o.chainConn[f], 2);
+          if (o.od_chain[cell] != i) continue;
+          for (int e = o.od_off[cell]; e < o.od_off[cell + 1]; e++) {
+            int iface = o.od_ent[e] >> 1;
+            bool left = o.od_ent[e] & 1;
+            const double* M = &o.ff[32 * iface + (left ? 0 : 16)];
+            for (int j = 0; j < 4; j++)
+              for (int k = 0; k < 4; k++) {
+                if (left)
+                  M3(B, m, j, k) += M[4 * j + k];
+                else
+                  M3(B, m, j, k) -= M[4 * j + k];
+              }
+          }
+        }
+        m = 0;
+      }
       for (int f = f1; f < f2 - 1; f++) {
         int iface = o.chainConn[f];
         int leftCell = FACE(o, iface, 2);
@@ -1041,7 +1100,7 @@ void orc_block_tridag_periodic4(double* a, double* b, double* c, double* f, int 
 // ---- solver -----------------------------------------------------------------
 void* orc_create(int nCell, int nFace, int nChain, const int* faces, const int* fstart, const int* chainConn,
                  const double* normVec, const double* refMtx, double mach, double alpha, int order, double cfl,
-                 int msweep) {
+                 int msweep, int nOrphan, const int* orphans) {
   Oracle* o = new Oracle();
   o->nCell = nCell;
   o->nFace = nFace;
@@ -1070,6 +1129,24 @@ void* orc_create(int nCell, int nFace, int nChain, const int* faces, const int* 
   o->sigma.assign(nCell, 0.0);
   o->ff.assign(32 * (size_t)nFace, 0.0);
   o->qt.assign(4 * nCell, 0.0);
+  if (nOrphan > 0) {
+    o->orphans.assign(orphans, orphans + nOrphan);
+    o->od_chain.assign(nCell, -1);
+    for (int i = 0; i < nChain; i++)
+      for (int f = fstart[i]; f < fstart[i + 1] - 1; f++) o->od_chain[faces[6 * chainConn[f] + 2]] = i;
+    std::vector<int> cnt(nCell + 1, 0);
+    for (int f : o->orphans) {
+      cnt[faces[6 * f + 2] + 1]++;
+      cnt[faces[6 * f + 4] + 1]++;
+    }
+    for (int c = 0; c < nCell; c++) cnt[c + 1] += cnt[c];
+    o->od_off = cnt;
+    o->od_ent.assign(cnt[nCell], 0);
+    for (int f : o->orphans) {
+      o->od_ent[cnt[faces[6 * f + 2]]++] = (f << 1) | 1;
+      o->od_ent[cnt[faces[6 * f + 4]]++] = (f << 1);
+    }
+  }
   init_flow(*o, mach, alpha);
   return o;
 }

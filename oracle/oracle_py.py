@@ -152,9 +152,14 @@ class OracleCase:
         self.n_face = mesh.n_face
         start = np.ascontiguousarray(mesh.chain_start, np.int32)
         cc = np.ascontiguousarray(mesh.chain_conn, np.int32)
+        orph = np.ascontiguousarray(getattr(mesh, "orphans", np.zeros(0, np.int32)), np.int32)
+        if orph.size and ((faces[orph, 2] < 0).any() or (faces[orph, 4] < 0).any()):
+            raise ValueError("orphan faces must be interior faces")
+        self._keep = (start, cc, orph)
         self.h = C.c_void_p(self.lib.orc_create(
             mesh.n_cell, mesh.n_face, mesh.n_chain, _i(faces), _i(start), _i(cc), _d(nv), _d(rm),
-            C.c_double(mach), C.c_double(alpha), order, C.c_double(cfl), msweep))
+            C.c_double(mach), C.c_double(alpha), order, C.c_double(cfl), msweep, int(orph.size),
+            _i(orph) if orph.size else None))
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -1,0 +1,177 @@
+"""GPU parity on meshes with triangles (the orphan-face extension): O-grids
+whose outer layers are split into triangles, against the oracle running the
+same extension.  EXACT mode is bit-identical (norms to 1e-12 relative); FAST
+mode equals EXACT to rounding; batches, the explicit schemes and the C++
+driver (reading orphans.dat) carry the orphans too."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from cases import perturbed_state
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def hip():
+    from hamstr_oop_amd import hip as H
+    H.lib()
+    return H
+
+
+def _eq(a, b, what):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    if not np.array_equal(a, b):
+        d = np.abs(a - b)
+        i = int(np.argmax(d))
+        raise AssertionError(f"{what}: {np.count_nonzero(a != b)} of {a.size} differ; max |diff| {d.flat[i]:.3e}")
+
+
+MESHES = {
+    "circle24x10_tri4": lambda m: m.circle_ogrid(24, 10, tri_layers=4),
+    "naca40x16_tri6": lambda m: m.naca_ogrid(40, 16, tri_layers=6),
+}
+
+
+@pytest.mark.parametrize("order", [1, 3, 5])
+@pytest.mark.parametrize("name", list(MESHES))
+def test_orphan_api_iterations_bitwise(hip, oracle, name, order):
+    """computeRHS + ADI through the API, 4 iterations from a perturbed state, EXACT."""
+    from hamstr_oop_amd import meshgen
+    mesh = MESHES[name](meshgen)
+    print(f"orphans api {name} order={order}", flush=True)
+    g = hip.Solver(mesh, 0.5, 2.0, order, 10.0, adi_mode=hip.ADI_EXACT)
+    o = oracle.case(mesh, 0.5, 2.0, order, 10.0)
+    q = perturbed_state(o, np.random.default_rng(21))
+    g.set("q", q)
+    o.set("q", q)
+    for it in range(4):
+        l2g, _ = g.computeRHS("implicit")
+        _, l2o, _ = o.compute_rhs(True)
+        assert l2g == pytest.approx(l2o, rel=1e-12)
+        if it == 0:
+            for k in ("r", "sigma", "ff"):
+                _eq(g.get(k), o.get(k), k)
+        lg, _ = g.ADI()
+        lo, _ = o.adi()
+        assert lg == pytest.approx(lo, rel=1e-12)
+        for k in ("q", "dq", "r0", "r"):
+            _eq(g.get(k), o.get(k), f"{k} after iteration {it}")
+    g.close()
+
+
+@pytest.mark.parametrize("order", [1, 3])
+def test_orphan_step_bitwise(hip, oracle, order):
+    """The fused, graph-replayed iteration from freestream, 30 steps, EXACT."""
+    from hamstr_oop_amd import meshgen
+    mesh = meshgen.naca_ogrid(64, 24, tri_layers=8)
+    g = hip.Solver(mesh, 0.6, 1.5, order, 10.0, adi_mode=hip.ADI_EXACT)
+    o = oracle.case(mesh, 0.6, 1.5, order, 10.0)
+    n, l2g, llg = g.step(30, check_every=10)
+    assert n == 30
+    _, l2o, llo, _ = o.step(30)
+    np.testing.assert_allclose(l2g, l2o, rtol=1e-12)
+    np.testing.assert_allclose(llg, llo, rtol=1e-12)
+    _eq(g.get("q"), o.get("q"), "q after 30 steps")
+    g.close()
+
+
+def test_orphan_fast_matches_exact_and_converges(hip):
+    """FAST (parallel cyclic reduction) against EXACT on a triangulated
+    O-grid: the first 20 steps agree to rounding, and the FAST run converges
+    to 1e-10 at the EXACT run's fixed point."""
+    from hamstr_oop_amd import meshgen
+    mesh = meshgen.naca_ogrid(64, 24, tri_layers=8)
+    runs = {}
+    for mode in (hip.ADI_EXACT, hip.ADI_FAST):
+        s = hip.Solver(mesh, 0.5, 2.0, 3, 10.0, adi_mode=mode)
+        _, l2a, _ = s.step(20, check_every=10)
+        q20 = s.get("q")
+        n, l2, _ = s.step(4000, tol=1e-10, check_every=50)
+        runs[mode] = (q20, l2a, s.get("q"), l2[-1], n)
+        s.close()
+    qe, le, qe_end, ende, ne = runs[hip.ADI_EXACT]
+    qf, lf, qf_end, endf, nf = runs[hip.ADI_FAST]
+    np.testing.assert_allclose(lf, le, rtol=1e-9)
+    scale = np.abs(qe.reshape(-1, 4)).max(axis=0)
+    assert (np.abs(qf - qe).reshape(-1, 4) / scale).max() < 1e-10
+    assert ende <= 1e-10 and endf <= 1e-10, (ende, endf, ne, nf)
+    assert (np.abs(qf_end - qe_end).reshape(-1, 4) / scale).max() < 1e-8
+
+
+@pytest.mark.parametrize("scheme", ["euler", "rk3"])
+def test_orphan_explicit_bitwise(hip, oracle, scheme):
+    from hamstr_oop_amd import meshgen
+    mesh = meshgen.circle_ogrid(24, 10, tri_layers=4)
+    g = hip.Solver(mesh, 0.4, 1.0, 3, 0.8, adi_mode=hip.ADI_FAST)
+    g.set_scheme(scheme)
+    o = oracle.case(mesh, 0.4, 1.0, 3, 0.8)
+    n, first, printed = g.step(20, check_every=10)
+    _, f0, p0, _ = o.step_scheme(scheme, 20)
+    np.testing.assert_allclose(printed, p0, rtol=1e-12)
+    for k in ("q", "qt", "r", "sigma"):
+        _eq(g.get(k), o.get(k), k)
+    g.close()
+
+
+def test_orphan_batch_bitwise_and_retire(hip, oracle):
+    """A batch of two triangulated meshes and one all-quad mesh (EXACT): each
+    case bit-identical to its own oracle run after 12 steps; with retirement
+    every case then converges, and a retired case's state stays put."""
+    from hamstr_oop_amd import meshgen
+    meshes = [meshgen.circle_ogrid(24, 10, tri_layers=4), meshgen.naca_ogrid(32, 12),
+              meshgen.naca_ogrid(40, 16, tri_layers=6)]
+    flows = [(0.3, 0.0), (0.6, 1.0), (0.5, 2.0)]
+    b = hip.Batch([(m, ma, al) for m, (ma, al) in zip(meshes, flows)], order=3, cfl=10.0,
+                  adi_mode=hip.ADI_EXACT)
+    done, _, _ = b.batch_step(12, check_every=4)
+    assert done == 12
+    for k, (m, (ma, al)) in enumerate(zip(meshes, flows)):
+        o = oracle.case(m, ma, al, 3, 10.0)
+        o.step(12)
+        _eq(b.case_q(k).reshape(-1), o.get("q"), f"case {k}")
+    b.set_retire(True, 0)
+    b.batch_step(3000, tol=1e-10, check_every=50)
+    st, it = b.case_status()
+    assert (st == 1).all(), (st, it)
+    k0 = int(np.argmin(it))
+    q0 = b.case_q(k0)
+    b.set_retire(True, 0)
+    b.set_case_live(k0, False)
+    b.batch_step(50, check_every=50)
+    assert np.array_equal(b.case_q(k0), q0)
+    b.close()
+
+
+def test_orphan_cpp_driver(hip, oracle, tmp_path):
+    """The drop-in driver reads orphans.dat next to the reference's mesh files."""
+    from hamstr_oop_amd import meshgen
+    mesh = meshgen.naca_ogrid(40, 16, tri_layers=6)
+    meshgen.write_reference_case(mesh, str(tmp_path), 0.5, 2.0, 3, 8, 10.0)
+    back = meshgen.read_reference_case(str(tmp_path))
+    assert np.array_equal(back.orphans, mesh.orphans)
+    exe = os.path.join(ROOT, "hamstr_oop_amd", "hamstr_mi355x")
+    out = subprocess.run([exe, "--exact", "--dump", "q.bin"], cwd=str(tmp_path), capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stdout + out.stderr
+    o = oracle.case(mesh, 0.5, 2.0, 3, 10.0)
+    o.step(8)
+    _eq(np.fromfile(str(tmp_path / "q.bin"), dtype=np.float64), o.get("q"), "driver q")
+
+
+def test_bad_orphan_lists_are_rejected(hip):
+    from hamstr_oop_amd import meshgen
+    mesh = meshgen.circle_ogrid(24, 10, tri_layers=4)
+    chain_face = int(mesh.chain_conn[0])
+    boundary = int(np.nonzero(mesh.faces[:, 4] < 0)[0][0])
+    for bad in ([chain_face], [boundary], [int(mesh.orphans[0])] * 2, [mesh.n_face]):
+        m2 = meshgen.circle_ogrid(24, 10, tri_layers=4)
+        m2.orphans = np.concatenate([mesh.orphans, np.array(bad, np.int32)])
+        with pytest.raises(hip.HamstrError) as e:
+            hip.Solver(m2, 0.5, 1.0, 3, 10.0)
+        assert e.value.code == hip.ERR_MESH
