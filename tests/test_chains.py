@@ -1,0 +1,129 @@
+"""The chain builder (hamstr_oop_amd.chains) on meshes given as nodes and
+cells only: it recovers the generator's chain layout on O-grids, handles
+quadrilateral meshes with randomly split cells, writes and reads back the
+reference's file set, and its checker catches each broken rule."""
+import tempfile
+
+import numpy as np
+import pytest
+
+from hamstr_oop_amd import chains, meshgen
+
+
+def split_cells(mesh, frac, seed, first_layer=0):
+    """conn of an O-grid with a random share of its quadrilaterals (from
+    layer first_layer outwards) cut along a random diagonal"""
+    rng = np.random.default_rng(seed)
+    out = []
+    for c, row in enumerate(mesh.conn.tolist()):
+        if c // mesh.nc >= first_layer and rng.random() < frac:
+            a, b, cc, d = row
+            out += ([[a, b, cc, cc], [a, cc, d, d]] if rng.random() < 0.5 else [[a, b, d, d], [b, cc, d, d]])
+        else:
+            out.append(row)
+    return np.array(out, np.int32)
+
+
+@pytest.mark.parametrize("mk", [lambda: meshgen.circle_ogrid(24, 10), lambda: meshgen.naca_ogrid(40, 16, tri_layers=6),
+                                lambda: meshgen.naca_ogrid(128, 64, tri_layers=32)])
+def test_builder_recovers_ogrid_layout(mk):
+    m = mk()
+    chains.validate(m, 5)
+    b = chains.build_chains(m.x, m.conn)
+    chains.validate(b, 5)
+    assert b.chains_per_color == m.chains_per_color
+    assert b.n_chain == m.n_chain and len(b.orphans) == len(m.orphans)
+    assert sorted(np.diff(b.chain_start).tolist()) == sorted(np.diff(m.chain_start).tolist())
+
+
+@pytest.mark.parametrize("frac,seed", [(0.15, 1), (0.5, 2), (1.0, 3)])
+def test_split_meshes_build_and_converge(oracle, frac, seed):
+    """Randomly split cells: valid chains, and the oracle (running the
+    orphan-face extension) converges; at order 1 the face-by-face residual of
+    the converged state vanishes."""
+    base = meshgen.naca_ogrid(32, 12)
+    m = chains.build_chains(base.x, split_cells(base, frac, seed, first_layer=3))
+    chains.validate(m, 5)
+    assert len(m.orphans) > 0
+    for order in (1, 3):
+        o = oracle.case(m, 0.5, 2.0, order, 10.0)
+        rc, l2, _, _ = o.step(1500)
+        assert rc == 0 and l2.min() <= 1e-10, (order, l2.min())
+    r = np.zeros((m.n_cell, 4))
+    o1 = oracle.case(m, 0.5, 2.0, 1, 10.0)
+    o1.step(1500)
+    q = o1.get("q").reshape(-1, 4)
+    qinf = o1.freestream()
+    for f, (n1, n2, cl, el, cr, er) in enumerate(o1.faces):
+        if cr >= 0:
+            qr = q[cr]
+        elif cr == -1:
+            qr = qinf
+        else:
+            R = o1.refmtx[f]
+            qr = q[cl].copy()
+            qr[1], qr[2] = R[0] * q[cl][1] + R[1] * q[cl][2], R[2] * q[cl][1] + R[3] * q[cl][2]
+        fl, _ = oracle.flux_roe2d(o1.normvec[f], q[cl], qr)
+        r[cl] -= fl
+        if cr >= 0:
+            r[cr] += fl
+    assert np.sqrt((r[:, 0] ** 2).sum()) / m.n_cell < 1e-9
+
+
+def test_file_round_trip():
+    base = meshgen.naca_ogrid(32, 12)
+    m = chains.build_chains(base.x, split_cells(base, 0.4, 5, first_layer=4))
+    with tempfile.TemporaryDirectory() as d:
+        meshgen.write_reference_case(m, d, 0.5, 1.0, 3, 10, 10.0)
+        back = meshgen.read_reference_case(d)
+    for k in ("x", "conn", "faces", "chain_start", "chain_conn", "orphans"):
+        assert np.array_equal(getattr(back, k), getattr(m, k)), k
+    assert list(back.chains_per_color) == list(m.chains_per_color)
+    chains.validate(back, 5)
+
+
+def test_checker_catches_broken_rules():
+    def fresh():
+        return meshgen.naca_ogrid(24, 10, tri_layers=3)
+    m = fresh()
+    chains.validate(m, 5)
+    bad = fresh()                               # a face turned round
+    f = int(bad.chain_conn[5])
+    bad.faces[f] = bad.faces[f][[1, 0, 4, 5, 2, 3]]
+    with pytest.raises(chains.ChainError):
+        chains.validate(bad, 5)
+    bad = fresh()                               # an orphan that is also on a chain
+    bad.orphans = np.append(bad.orphans, bad.chain_conn[3]).astype(np.int32)
+    with pytest.raises(chains.ChainError, match="exactly one"):
+        chains.validate(bad, 5)
+    bad = fresh()                               # a face on no chain and not an orphan
+    bad.orphans = bad.orphans[1:]
+    with pytest.raises(chains.ChainError, match="exactly one"):
+        chains.validate(bad, 5)
+    bad = fresh()                               # loops and lines in one colour
+    bad.chains_per_color = [bad.n_chain]
+    with pytest.raises(chains.ChainError, match="share a cell"):
+        chains.validate(bad, 5)
+    bad = fresh()                               # cell 0 second on its line
+    line0 = bad.chain_conn[bad.chain_start[bad.chains_per_color[0]]:bad.chain_start[bad.chains_per_color[0] + 1]]
+    c1 = int(bad.faces[line0[1], 2])
+    swap = {0: c1, c1: 0}
+    for col in (2, 4):
+        bad.faces[:, col] = [swap.get(int(v), int(v)) for v in bad.faces[:, col]]
+    bad.conn[[0, c1]] = bad.conn[[c1, 0]]
+    with pytest.raises(chains.ChainError, match="cell 0"):
+        chains.validate(bad, 5)
+
+
+def test_odd_triangle_count_is_refused():
+    base = meshgen.naca_ogrid(24, 8)
+    conn = base.conn.tolist()
+    a, b, c, d = conn[100]
+    conn[100] = [a, b, c, c]     # cell 100 cut into two triangles: an even count builds
+    conn.append([a, c, d, d])
+    conn = np.array(conn)
+    m = chains.build_chains(base.x, conn)
+    chains.validate(m, 5)
+    odd = np.array(conn[:-1].tolist())        # one of them removed: one triangle left over
+    with pytest.raises(chains.ChainError):
+        chains.build_chains(base.x, odd)

@@ -175,3 +175,27 @@ def test_bad_orphan_lists_are_rejected(hip):
         with pytest.raises(hip.HamstrError) as e:
             hip.Solver(m2, 0.5, 1.0, 3, 10.0)
         assert e.value.code == hip.ERR_MESH
+
+
+def test_built_chains_bitwise_and_phases(hip, oracle):
+    """A mesh given only as nodes and cells (an O-grid with 40% of its outer
+    cells split at random), chained by hamstr_oop_amd.chains: EXACT steps are
+    bit-identical to the oracle, and the ADI phases the library derives from
+    the chain order are the builder's colours."""
+    from hamstr_oop_amd import chains, meshgen
+    from test_chains import split_cells
+    base = meshgen.naca_ogrid(48, 16)
+    mesh = chains.build_chains(base.x, split_cells(base, 0.4, 7, first_layer=4))
+    chains.validate(mesh, 5)
+    g = hip.Solver(mesh, 0.5, 2.0, 3, 10.0, adi_mode=hip.ADI_EXACT)
+    assert max(gr["phase"] for gr in hip.launch_groups(g)) + 1 == len(mesh.chains_per_color)
+    o = oracle.case(mesh, 0.5, 2.0, 3, 10.0)
+    n, l2g, _ = g.step(20, check_every=10)
+    _, l2o, _, _ = o.step(20)
+    np.testing.assert_allclose(l2g, l2o, rtol=1e-12)
+    _eq(g.get("q"), o.get("q"), "q after 20 steps")
+    g.close()
+    f = hip.Solver(mesh, 0.5, 2.0, 3, 10.0, adi_mode=hip.ADI_FAST)
+    n, l2, _ = f.step(3000, tol=1e-10, check_every=50)
+    assert l2[-1] <= 1e-10
+    f.close()
