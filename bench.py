@@ -2,10 +2,13 @@
 """Benchmark of the line-implicit steady Euler iteration on one MI355X.
 
 Workload (default): O-grid around a NACA 0012 thickness-formula section,
-512 x 256 quads (131,072 cells, all-quad variant), MUSCL (order 3) with the
-reference's smooth limiter, M = 0.5, alpha = 2 deg, CFL = 10, msweep = 1,
-synthetic freestream start.  A "step" is one stepSolution ADI iteration:
-computeRHS("implicit") + ADI (two chain phases) + computeLinearRHS + update.
+512 x 256 cells whose outer 128 layers are split into triangles (65,536 quads
++ 131,072 triangles = 196,608 cells; radial lines run through both triangles
+of each split cell, the 65,536 faces left over are orphan faces), MUSCL
+(order 3) with the reference's smooth limiter, M = 0.5, alpha = 2 deg, CFL =
+10, msweep = 1, synthetic freestream start.  --tri-layers 0 gives the all-quad
+variant.  A "step" is one stepSolution ADI iteration: computeRHS("implicit")
++ ADI (two chain phases) + computeLinearRHS + update.
 
 Prints one JSON line: cell-iterations/s (value), the roofline of the dominant
 kernel from HIP events over the timed region, the time to nonlinear
@@ -49,7 +52,9 @@ def algorithmic_bytes(n_cell, n_proc, phase_faces, b, msweep=1):
 
 
 def phase_face_counts(mesh):
-    """Processed faces per ADI phase for an O-grid (loops, then lines)."""
+    """Processed faces per ADI phase for an O-grid (loops, then lines), and in
+    all; orphan faces count with the last phase, whose rows take their
+    diagonal blocks."""
     import numpy as np
     start = np.asarray(mesh.chain_start)
     conn = np.asarray(mesh.chain_conn)
@@ -58,20 +63,23 @@ def phase_face_counts(mesh):
         f1, f2 = start[c], start[c + 1]
         per.append((f2 - f1) - (1 if conn[f1] == conn[f2 - 1] else 0))
     nl = mesh.chains_per_color[0]
-    return [int(sum(per[:nl])), int(sum(per[nl:]))], int(sum(per))
+    no = int(len(mesh.orphans))
+    return [int(sum(per[:nl])), int(sum(per[nl:])) + no], int(sum(per)) + no
 
 
-def measured_traffic(stage, nc, nr, order, lhs, adi):
+def measured_traffic(stage, nc, nr, order, lhs, adi, tri=0):
     """HBM bytes per launch of the stage's kernel from the committed rocprofv3
     counter summary of this workload (FETCH_SIZE and WRITE_SIZE scaled by a
     plain copy of known size), or None for another workload."""
-    path = os.path.join(ROOT, "profiles", f"traffic_{lhs}_naca{nc}x{nr}_o{order}.json")
+    tag = f"_tri{tri}" if tri else ""
+    path = os.path.join(ROOT, "profiles", f"traffic_{lhs}_naca{nc}x{nr}{tag}_o{order}.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
         t = json.load(fh)
     w = t.get("workload", {})
-    if (w.get("nc"), w.get("nr"), w.get("order"), w.get("lhs"), w.get("adi")) != (nc, nr, order, lhs, adi):
+    if (w.get("nc"), w.get("nr"), w.get("order"), w.get("lhs"), w.get("adi"), w.get("tri_layers", 0)) != \
+            (nc, nr, order, lhs, adi, tri):
         return None
     st = t["stages"].get(stage)
     return None if st is None else st["traffic_bytes"]
@@ -80,12 +88,13 @@ def measured_traffic(stage, nc, nr, order, lhs, adi):
 def cpu_baseline(mesh, mach, alpha, order, cfl, budget_s=12.0, max_iters=30):
     """The reference's own code (oracle/_ref, compiled from its sources) on a
     bounded sample: `n` iterations of the same case from freestream, 1 core.
-    Falls back to the oracle (a leak-free C++ restatement) if the reference
-    build is absent."""
+    The oracle (a leak-free C++ restatement) instead when the reference build
+    is absent, or when the mesh has orphan faces: the reference computes chain
+    faces only, so a mesh with triangles is outside what it can run."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py
     from hamstr_oop_amd import meshgen
-    if oracle_py.Reference.available("O2"):
+    if oracle_py.Reference.available("O2") and not len(mesh.orphans):
         ref = oracle_py.Reference("O2")
         with tempfile.TemporaryDirectory() as d:
             meshgen.write_reference_case(mesh, d, mach, alpha, order, 1, cfl)
@@ -117,6 +126,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--nc", type=int, default=512)
     ap.add_argument("--nr", type=int, default=256)
+    ap.add_argument("--tri-layers", type=int, default=128, help="outer layers split into triangles (0: all quads)")
     ap.add_argument("--order", type=int, default=3)
     ap.add_argument("--mach", type=float, default=0.5)
     ap.add_argument("--alpha", type=float, default=2.0)
@@ -146,7 +156,7 @@ def main():
         sys.exit(2)
 
     from hamstr_oop_amd import hip, meshgen
-    mesh = meshgen.naca_ogrid(args.nc, args.nr, t=0.12)
+    mesh = meshgen.naca_ogrid(args.nc, args.nr, t=0.12, tri_layers=args.tri_layers)
     phase_faces, n_proc = phase_face_counts(mesh)
     adi_mode = hip.ADI_FAST if args.adi == "fast" else hip.ADI_EXACT
 
@@ -213,16 +223,20 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic: generated O-grid, freestream initial state",
-        "config": {"workload": f"NACA0012 O-grid {args.nc}x{args.nr} all-quad, order {args.order} (MUSCL), "
-                               f"M={args.mach}, alpha={args.alpha}, CFL={args.cfl}, msweep=1",
-                   "cells": mesh.n_cell, "faces": n_proc, "lhs": "fp64",
+        "config": {"workload": f"NACA0012 O-grid {args.nc}x{args.nr}, " +
+                               (f"outer {args.tri_layers} layers split into triangles" if args.tri_layers
+                                else "all-quad") +
+                               f", order {args.order} (MUSCL), M={args.mach}, alpha={args.alpha}, CFL={args.cfl}, "
+                               f"msweep=1",
+                   "cells": mesh.n_cell, "faces": n_proc, "orphan_faces": int(len(mesh.orphans)), "lhs": "fp64",
                    "adi": args.adi + (" (parallel cyclic reduction per chain: residual, norms and update "
                                       "bit-identical to the reference; block solves equal up to rounding)"
                                       if args.adi == "fast" else " (the reference's sequential block Thomas, "
                                       "bit-identical)")},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(dom, args.nc, args.nr, args.order, "fp64", args.adi),
+                     "traffic": measured_traffic(dom, args.nc, args.nr, args.order, "fp64", args.adi,
+                                                 args.tri_layers),
                      "algorithmic_bytes_per_launch": dom_bytes, "avg_launch_us": stages[dom] * 1e3},
         "iteration_roofline": {"bytes_per_cell_iter": total_b,
                                "achieved_gbs": value * total_b / 1e9,
@@ -322,7 +336,7 @@ def explicit_cpu_baseline(mesh, mach, alpha, order, cfl, scheme, budget_s=4.0, m
     the oracle's restatement."""
     import oracle_py
     from hamstr_oop_amd import meshgen
-    if oracle_py.Reference.available("O2"):
+    if oracle_py.Reference.available("O2") and not len(mesh.orphans):
         ref = oracle_py.Reference("O2")
         with tempfile.TemporaryDirectory() as d:
             meshgen.write_reference_case(mesh, d, mach, alpha, order, 1, cfl, scheme=scheme)
@@ -341,13 +355,15 @@ def explicit_cpu_baseline(mesh, mach, alpha, order, cfl, scheme, budget_s=4.0, m
             "sample": f"{n} {scheme} steps (after 1 warm-up) from freestream; {what}; {sec:.2f} s wall"}
 
 
-def weno_section(args, nc=1024, nr=384, mach=0.8, alpha=1.25):
-    """WENO (order 5), transonic: NACA0012 1024x384 all-quad, M=0.8, alpha=1.25
-    deg.  FP64 and FP32-LHS runs from the freestream to L2 <= tol under the
-    sweep's stall rule; the FP32-LHS error is stated against the FP64 field."""
+def weno_section(args, nc=1024, nr=412, tri=28, mach=0.8, alpha=1.25):
+    """WENO (order 5), transonic: NACA0012 O-grid of 1024 x 384 quads plus a
+    triangulated far-field ring of 28 layers (1024 x 2 x 28 triangles; 450,560
+    cells), M=0.8, alpha=1.25 deg.  FP64 and FP32-LHS runs from the freestream
+    to L2 <= tol under the sweep's stall rule; the FP32-LHS error is stated
+    against the FP64 field."""
     import numpy as np
     from hamstr_oop_amd import hip, meshgen
-    mesh = meshgen.naca_ogrid(nc, nr, t=0.12)
+    mesh = meshgen.naca_ogrid(nc, nr, t=0.12, tri_layers=tri)
     res, qs = {}, {}
     for name, lhs in (("fp64", hip.LHS_FP64), ("fp32_lhs", hip.LHS_FP32)):
         s = hip.Solver(mesh, mach, alpha, 5, args.cfl, lhs=lhs, adi_mode=hip.ADI_FAST)
@@ -365,7 +381,8 @@ def weno_section(args, nc=1024, nr=384, mach=0.8, alpha=1.25):
         s.close()
     scale = np.abs(qs["fp64"]).max(axis=0)
     res["fp32_lhs"]["max_field_diff_vs_fp64_rel"] = float((np.abs(qs["fp32_lhs"] - qs["fp64"]) / scale).max())
-    res["workload"] = f"NACA0012 O-grid {nc}x{nr} all-quad ({mesh.n_cell} cells), order 5 (WENO), M={mach}, " \
+    res["workload"] = f"NACA0012 O-grid {nc}x{nr - tri} quads + {tri} triangulated far-field layers " \
+                      f"({mesh.n_cell} cells, {len(mesh.orphans)} orphan faces), order 5 (WENO), M={mach}, " \
                       f"alpha={alpha}, CFL={args.cfl}"
     return res
 
