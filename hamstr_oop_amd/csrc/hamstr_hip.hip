@@ -109,6 +109,7 @@ struct ChainTab {
   const int* od_chain;
   const int* od_off;
   const int* od_ent;
+  const int* hbase;   // FAST chains longer than a workgroup: first scratch row of their odd rows
 };
 
 __device__ __forceinline__ double4 load_state(const int2 e, const double* __restrict__ q,
@@ -874,39 +875,63 @@ __device__ __forceinline__ void mv4_acc_lds(T* x, const T* A, const T* lds, int 
 
 enum { kFinalOpen = 0, kFinalPair = 1, kFinalOdd = 2 };
 
-template <typename T, int FINAL>
-__device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const T* __restrict__ ff, int nProc,
-                                               double cfl, const double* __restrict__ sigma,
-                                               double* __restrict__ r, double* __restrict__ dq, int dqmode,
-                                               int ncr) {
-#pragma clang fp contract(fast)
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* lds = reinterpret_cast<T*>(smem_raw);
-  const int pb = ct.pbase[ch], nf = ct.nf[ch], N = ct.ncell[ch];
-  const bool periodic = FINAL != kFinalOpen;
-  const int i = threadIdx.x;
-  const bool active = i < N;
-  const int S = lds_stride(N);
-
-  // every row, assembled and normalised, goes to LDS
-  if (active) {
-    T a[16], c[16], f[4], B[16];
-    int cell;
-    assemble_row<T>(ct, ff, nProc, pb, nf, N, periodic, i, cfl, sigma, r, a, B, c, f, &cell);
-    if (!periodic) {
-      if (i == 0) {
+// row i of chain ch assembled (assemble_row) and normalised by its diagonal
+// block; an open chain's end rows lose the coupling past the chain's ends
+template <typename T>
+__device__ __forceinline__ void fast_row(const ChainTab& ct, const T* __restrict__ ff, int nProc, int pb, int nf,
+                                         int N, bool periodic, int i, double cfl, const double* __restrict__ sigma,
+                                         const double* __restrict__ r, T* a, T* c, T* f) {
+  T B[16];
+  int cell;
+  assemble_row<T>(ct, ff, nProc, pb, nf, N, periodic, i, cfl, sigma, r, a, B, c, f, &cell);
+  if (!periodic) {
+    if (i == 0) {
 #pragma unroll
-        for (int e = 0; e < 16; e++) a[e] = T(0);
-      }
-      if (i == N - 1) {
-#pragma unroll
-        for (int e = 0; e < 16; e++) c[e] = T(0);
-      }
+      for (int e = 0; e < 16; e++) a[e] = T(0);
     }
-    normalise_row(B, a, c, f);
-    lds_store_row(lds, S, lpad(i), a, c, f);
+    if (i == N - 1) {
+#pragma unroll
+      for (int e = 0; e < 16; e++) c[e] = T(0);
+    }
   }
-  __syncthreads();
+  normalise_row(B, a, c, f);
+}
+
+// One cyclic-reduction elimination: the surviving row (ra, rc, rf) absorbs
+// its neighbours, read from `src` (element-major with stride SS, at slots sL
+// and sR); the new row is normalised into (na, nc, rf).
+template <typename T>
+__device__ __forceinline__ void cr_eliminate(const T* ra, const T* rc, T* rf, const T* src, int SS, bool hasL,
+                                             int sL, bool hasR, int sR, T* na, T* nc) {
+#pragma clang fp contract(fast)
+  T M[16];
+#pragma unroll
+  for (int e = 0; e < 16; e++) {
+    M[e] = (e % 5 == 0) ? T(1) : T(0);
+    na[e] = T(0);
+    nc[e] = T(0);
+  }
+  if (hasL) {
+    mm4_acc_lds(M, ra, src, SS, 16, sL);   // M -= A^_i C^_L
+    mm4_acc_lds(na, ra, src, SS, 0, sL);   // A^new = -A^_i A^_L
+    mv4_acc_lds(rf, ra, src, SS, sL);      // F -= A^_i F^_L
+  }
+  if (hasR) {
+    mm4_acc_lds(M, rc, src, SS, 0, sR);    // M -= C^_i A^_R
+    mm4_acc_lds(nc, rc, src, SS, 16, sR);  // C^new = -C^_i C^_R
+    mv4_acc_lds(rf, rc, src, SS, sR);      // F -= C^_i F^_R
+  }
+  normalise_row(M, na, nc, rf);
+}
+
+// Solves the N-row normalised system held in LDS (row i at lds[e*S + lpad(i)])
+// in place; the solution ends in the F slots.  Every thread of the workgroup
+// calls it (it holds barriers); thread i works on the rows each step assigns.
+template <typename T, int FINAL>
+__device__ __forceinline__ void fast_solve_lds(T* lds, const int N, const int ncr, const int i) {
+#pragma clang fp contract(fast)
+  const bool periodic = FINAL != kFinalOpen;
+  const int S = lds_stride(N);
 
   // (1) ncr levels of cyclic reduction: at stride d the survivors (rows at
   // multiples of 2d) absorb rows i - d and i + d, which drop out.  The work is
@@ -922,25 +947,9 @@ __device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const 
       const int row = i * 2 * d;
       const int iL = periodic ? (row - d + N) % N : row - d;
       const int iR = periodic ? (row + d) % N : row + d;
-      T ra[16], rc[16], rf[4], M[16], na[16], nc[16];
+      T ra[16], rc[16], rf[4], na[16], nc[16];
       lds_load_row(lds, S, lpad(row), ra, rc, rf);
-#pragma unroll
-      for (int e = 0; e < 16; e++) {
-        M[e] = (e % 5 == 0) ? T(1) : T(0);
-        na[e] = T(0);
-        nc[e] = T(0);
-      }
-      if (periodic || iL >= 0) {
-        mm4_acc_lds(M, ra, lds, S, 16, lpad(iL));  // M -= A^_i C^_L
-        mm4_acc_lds(na, ra, lds, S, 0, lpad(iL));  // A^new = -A^_i A^_L
-        mv4_acc_lds(rf, ra, lds, S, lpad(iL));     // F -= A^_i F^_L
-      }
-      if (periodic || iR < N) {
-        mm4_acc_lds(M, rc, lds, S, 0, lpad(iR));   // M -= C^_i A^_R
-        mm4_acc_lds(nc, rc, lds, S, 16, lpad(iR)); // C^new = -C^_i C^_R
-        mv4_acc_lds(rf, rc, lds, S, lpad(iR));     // F -= C^_i F^_R
-      }
-      normalise_row(M, na, nc, rf);
+      cr_eliminate(ra, rc, rf, lds, S, periodic || iL >= 0, lpad(iL), periodic || iR < N, lpad(iR), na, nc);
       lds_store_row(lds, S, lpad(row), na, nc, rf);
     }
     __syncthreads();
@@ -1138,7 +1147,28 @@ __device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const 
     }
     __syncthreads();
   }
-  if (active) {
+}
+
+// ADI phase, FAST order, one thread and one LDS row per block row
+template <typename T, int FINAL>
+__device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const T* __restrict__ ff, int nProc,
+                                               double cfl, const double* __restrict__ sigma,
+                                               double* __restrict__ r, double* __restrict__ dq, int dqmode,
+                                               int ncr) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* lds = reinterpret_cast<T*>(smem_raw);
+  const int pb = ct.pbase[ch], nf = ct.nf[ch], N = ct.ncell[ch];
+  const bool periodic = FINAL != kFinalOpen;
+  const int i = threadIdx.x;
+  const int S = lds_stride(N);
+  if (i < N) {
+    T a[16], c[16], f[4];
+    fast_row<T>(ct, ff, nProc, pb, nf, N, periodic, i, cfl, sigma, r, a, c, f);
+    lds_store_row(lds, S, lpad(i), a, c, f);
+  }
+  __syncthreads();
+  fast_solve_lds<T, FINAL>(lds, N, ncr, i);
+  if (i < N) {
     T x[4];
 #pragma unroll
     for (int e = 0; e < 4; e++) x[e] = lds[(32 + e) * S + lpad(i)];
@@ -1146,19 +1176,79 @@ __device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const 
   }
 }
 
+// ADI phase, FAST order, for chains longer than a workgroup: thread t takes
+// rows 2t and 2t+1.  The first cyclic-reduction level runs before anything
+// enters LDS: the odd rows go to global scratch `hs` (element-major, stride
+// hstride, the chain's rows from ct.hbase[ch]), each even row absorbs its two
+// odd neighbours in registers, and only the reduced even rows enter LDS, as a
+// system of ceil(N/2) rows that fast_solve_lds finishes with ncr-1 more
+// levels.  The odd rows then back-substitute from scratch.  The arithmetic is
+// that of adi_fast_chain with ncr levels.  Needs open chains of N >= 2 rows or
+// closed chains of an even N >= 4, and ncr >= 1.
+template <typename T, int FINAL>
+__device__ __forceinline__ void adi_fast_chain_long(ChainTab ct, const int ch, const T* __restrict__ ff, int nProc,
+                                                    double cfl, const double* __restrict__ sigma,
+                                                    double* __restrict__ r, double* __restrict__ dq, int dqmode,
+                                                    int ncr, T* __restrict__ hs, int hstride) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* lds = reinterpret_cast<T*>(smem_raw);
+  const int pb = ct.pbase[ch], nf = ct.nf[ch], N = ct.ncell[ch];
+  const bool periodic = FINAL != kFinalOpen;
+  const int t = threadIdx.x;
+  const int M = (N + 1) / 2;
+  const int SM = lds_stride(M);
+  T* hrow = hs + ct.hbase[ch];
+  if (2 * t + 1 < N) {
+    T a[16], c[16], f[4];
+    fast_row<T>(ct, ff, nProc, pb, nf, N, periodic, 2 * t + 1, cfl, sigma, r, a, c, f);
+    lds_store_row(hrow, hstride, t, a, c, f);
+  }
+  __syncthreads();
+  if (t < M) {
+    const int i = 2 * t;
+    const int iL = periodic ? (i - 1 + N) % N : i - 1;
+    const int iR = periodic ? (i + 1) % N : i + 1;
+    T a[16], c[16], f[4], na[16], nc[16];
+    fast_row<T>(ct, ff, nProc, pb, nf, N, periodic, i, cfl, sigma, r, a, c, f);
+    cr_eliminate(a, c, f, hrow, hstride, periodic || iL >= 0, iL >> 1, periodic || iR < N, iR >> 1, na, nc);
+    lds_store_row(lds, SM, lpad(t), na, nc, f);
+  }
+  __syncthreads();
+  fast_solve_lds<T, FINAL>(lds, M, ncr - 1, t);
+  if (2 * t + 1 < N) {
+    // x = F^ - A^ x_2t - C^ x_2t+2, with the even rows' x in the F slots
+    const int row = 2 * t + 1;
+    T ra[16], rc[16], x[4];
+    lds_load_row(hrow, hstride, t, ra, rc, x);
+    mv4_acc_lds(x, ra, lds, SM, lpad(t));
+    if (periodic || row + 1 < N) mv4_acc_lds(x, rc, lds, SM, lpad(((row + 1) % N) >> 1));
+    store_solution(r, dq, dqmode, ct.pleft[pb + row], x);
+  }
+  if (t < M) {
+    T x[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) x[e] = lds[(32 + e) * SM + lpad(t)];
+    store_solution(r, dq, dqmode, ct.pleft[pb + 2 * t], x);
+  }
+}
+
 // one chain per workgroup pass; the workgroup walks the group's chains with a
 // stride of the grid size (a persistent launch: chains are sorted longest
 // first, so every pass of the grid does similar work)
-template <typename T, int FINAL, int MAXT>
+template <typename T, int FINAL, int MAXT, bool LONG>
 __global__ __launch_bounds__(MAXT) void k_adi_fast(ChainTab ct, Live lv, const int* __restrict__ chains, int nch,
                                                    const T* __restrict__ ff, int nProc, double cfl,
                                                    const double* __restrict__ sigma, double* __restrict__ r,
-                                                   double* __restrict__ dq, int dqmode, int ncr) {
+                                                   double* __restrict__ dq, int dqmode, int ncr,
+                                                   T* __restrict__ hs, int hstride) {
   const int vb = xcd_block(blockIdx.x, gridDim.x);
   for (int w = vb; w < nch; w += gridDim.x) {
     const int ch = chains[w];
     if (lv.chain && !lv.chain[ch]) continue;  // the same for the whole workgroup
-    adi_fast_chain<T, FINAL>(ct, ch, ff, nProc, cfl, sigma, r, dq, dqmode, ncr);
+    if (LONG)
+      adi_fast_chain_long<T, FINAL>(ct, ch, ff, nProc, cfl, sigma, r, dq, dqmode, ncr, hs, hstride);
+    else
+      adi_fast_chain<T, FINAL>(ct, ch, ff, nProc, cfl, sigma, r, dq, dqmode, ncr);
     __syncthreads();
   }
 }
@@ -1606,6 +1696,7 @@ struct Group {
   int grid = 0;        // workgroups of the launch (autotuned for FAST)
   int block = 64;      // threads per workgroup
   int ncr = 0;         // FAST: cyclic-reduction levels before the parallel cyclic reduction (autotuned)
+  int two = 0;         // FAST: chains longer than a workgroup, two rows per thread (adi_fast_chain_long)
 };
 
 struct hamstr_ctx {
@@ -1642,6 +1733,9 @@ struct hamstr_ctx {
   std::vector<int> case_orph_off;                           // [nCase+1] orphan processed-face ranges
   int *cell_off = nullptr, *cell_ent = nullptr;
   int *ch_pbase = nullptr, *ch_nf = nullptr, *ch_ncell = nullptr, *ch_periodic = nullptr;
+  int* ch_hbase = nullptr;      // per chain: first row of its odd rows in long_scratch (long chains)
+  void* long_scratch = nullptr; // FAST long chains: [36][nLong] odd rows eliminated by the first level
+  int nLong = 0;
   int* group_chains = nullptr;
   std::vector<Group> groups;   // in phase order
   int nPhase = 0;
@@ -1684,7 +1778,8 @@ struct hamstr_ctx {
   }
   FaceTab ftab() const { return FaceTab{pslot, src, nvec, refmtx, qinf}; }
   ChainTab ctab() const {
-    return ChainTab{ch_pbase, ch_nf, ch_ncell, ch_periodic, pleft, pright, pface, refmtx, od_chain, od_off, od_ent};
+    return ChainTab{ch_pbase, ch_nf, ch_ncell, ch_periodic, pleft, pright, pface, refmtx,
+                    od_chain, od_off, od_ent, ch_hbase};
   }
   bool lhs32() const { return prm.lhs == HAMSTR_LHS_FP32; }
 };
@@ -1796,19 +1891,25 @@ int launch_group(hamstr_ctx* x, const Group& g, int grid, int dqmode, int ncr) {
   const int* chains = x->group_chains + g.off;
   if (g.fast) {
     const dim3 gr(grid), b(g.block);
-    const size_t lds = (x->lhs32() ? sizeof(float) : sizeof(double)) * 36 * (size_t)lds_stride(g.maxN);
-#define FASTL(T, F, M)                                                                                          \
-  hipLaunchKernelGGL((k_adi_fast<T, F, M>), gr, b, lds, x->stream, ct, x->live(), chains, g.n, (const T*)x->ff, x->nProc, \
-                     x->c.cfl, x->sigma, x->r, x->dq, dqmode, ncr)
+    // long chains always run the first reduction level (it is what halves them)
+    const int nc = g.two ? std::max(ncr, 1) : ncr;
+    const size_t lds = (x->lhs32() ? sizeof(float) : sizeof(double)) * 36 *
+                       (size_t)lds_stride(g.two ? (g.maxN + 1) / 2 : g.maxN);
+#define FASTL(T, F, M, L)                                                                                        \
+  hipLaunchKernelGGL((k_adi_fast<T, F, M, L>), gr, b, lds, x->stream, ct, x->live(), chains, g.n, (const T*)x->ff,  \
+                     x->nProc, x->c.cfl, x->sigma, x->r, x->dq, dqmode, nc, (T*)x->long_scratch, x->nLong)
+#define FASTT(T, F, M) \
+  if (g.two) FASTL(T, F, M, true); else FASTL(T, F, M, false)
     if (x->lhs32()) {
-      if (g.final == kFinalOpen) FASTL(float, kFinalOpen, 1024);
-      else if (g.final == kFinalPair) FASTL(float, kFinalPair, 1024);
-      else FASTL(float, kFinalOdd, 1024);
+      if (g.final == kFinalOpen) FASTT(float, kFinalOpen, 1024);
+      else if (g.final == kFinalPair) FASTT(float, kFinalPair, 1024);
+      else FASTT(float, kFinalOdd, 1024);
     } else {
-      if (g.final == kFinalOpen) FASTL(double, kFinalOpen, 512);
-      else if (g.final == kFinalPair) FASTL(double, kFinalPair, 512);
-      else FASTL(double, kFinalOdd, 512);
+      if (g.final == kFinalOpen) FASTT(double, kFinalOpen, 512);
+      else if (g.final == kFinalPair) FASTT(double, kFinalPair, 512);
+      else FASTT(double, kFinalOdd, 512);
     }
+#undef FASTT
 #undef FASTL
   } else if (x->lhs32()) {
     hipLaunchKernelGGL((k_adi_exact<float>), dim3(nblk(g.n, 64)), dim3(64), 0, x->stream, ct, x->live(), chains, g.n,
@@ -1898,7 +1999,8 @@ void free_ctx(hamstr_ctx* x) {
                   x->nvec, x->refmtx, x->qinf, x->cell_case, x->case_off, x->pleft, x->pright, x->pface,
                   x->cell_off, x->cell_ent, x->ch_pbase, x->ch_nf, x->ch_ncell, x->ch_periodic, x->group_chains,
                   x->scratch, x->partial, x->norms, x->hist, x->nanflag, x->rrho, x->face_on, x->cell_on,
-                  x->chain_on, x->ghist, x->npart, x->keep_norms, x->od_chain, x->od_off, x->od_ent};
+                  x->chain_on, x->ghist, x->npart, x->keep_norms, x->od_chain, x->od_off, x->od_ent,
+                  x->ch_hbase, x->long_scratch};
   if (x->gexec) (void)hipGraphExecDestroy(x->gexec);
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -2133,38 +2235,56 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
   std::vector<Group> groups;
   std::vector<int> group_chains;
   size_t maxGroupChains = 0, maxRows = 0;
+  std::vector<int> ch_hbase(nChain, 0);
+  long nLong = 0;  // odd rows of the long chains, the rows their scratch holds
   for (int ph = 0; ph < nPhase; ph++) {
-    // buckets: 0 exact, 1 fast open, 2 fast pair, 3 fast odd
-    std::vector<int> bucket[4];
+    // buckets: 0 exact, 1 fast open, 2 fast pair, 3 fast odd; 4-6 the same
+    // three for chains longer than a workgroup (two rows per thread, half in LDS)
+    std::vector<int> bucket[7];
     for (int i = 0; i < nChain; i++) {
       if (phase[i] != ph) continue;
       const int N = ch_ncell[i];
-      const bool fast_ok = P.adi_mode == HAMSTR_ADI_FAST && N <= maxt && tl * 36 * (size_t)lds_stride(N) <= lds_max;
-      if (!fast_ok) {
-        bucket[0].push_back(i);
+      const bool fits = tl * 36 * (size_t)lds_stride(N) <= lds_max;
+      const bool fast_ok = P.adi_mode == HAMSTR_ADI_FAST && N <= maxt && fits;
+      const bool long_ok = P.adi_mode == HAMSTR_ADI_FAST && !fast_ok && N <= 2 * maxt &&
+                           tl * 36 * (size_t)lds_stride((N + 1) / 2) <= lds_max &&
+                           (ch_periodic[i] ? (N % 2 == 0 && N >= 4) : N >= 2);
+      int kind;
+      if (!fast_ok && !long_ok) {
+        kind = 0;
       } else if (!ch_periodic[i]) {
-        bucket[1].push_back(i);
+        kind = 1;
       } else {
         int nn = N;
         while (nn % 2 == 0 && nn >= 4) nn /= 2;
-        bucket[nn == 2 ? 2 : 3].push_back(i);
+        kind = nn == 2 ? 2 : 3;
       }
+      if (long_ok) {
+        kind += 3;
+        ch_hbase[i] = (int)nLong;
+        nLong += N / 2;
+      }
+      bucket[kind].push_back(i);
     }
-    for (int b = 0; b < 4; b++) {
+    if (nLong >= (1L << 31) / 36) return fail(HAMSTR_ERR_MESH, "too many rows in long chains");
+    for (int b = 0; b < 7; b++) {
       if (bucket[b].empty()) continue;
       std::stable_sort(bucket[b].begin(), bucket[b].end(), [&](int a, int c) { return ch_ncell[a] > ch_ncell[c]; });
       Group g;
       g.phase = ph;
       g.fast = b > 0;
-      g.final = b == 1 ? kFinalOpen : (b == 2 ? kFinalPair : kFinalOdd);
+      g.two = b > 3;
+      const int kind = g.two ? b - 3 : b;
+      g.final = kind == 1 ? kFinalOpen : (kind == 2 ? kFinalPair : kFinalOdd);
       g.off = (int)group_chains.size();
       g.n = (int)bucket[b].size();
       g.maxN = ch_ncell[bucket[b][0]];
-      g.block = g.fast ? ((g.maxN + 63) / 64) * 64 : 64;
+      const int threads = g.two ? (g.maxN + 1) / 2 : g.maxN;
+      g.block = g.fast ? ((threads + 63) / 64) * 64 : 64;
       g.grid = g.fast ? g.n : nblk(g.n, 64);
       // two levels of cyclic reduction before the PCR (one before an odd
       // cycle); measured best or close to it on single meshes and batches
-      g.ncr = g.fast ? (b == 3 ? 1 : 2) : 0;
+      g.ncr = g.fast ? (kind == 3 ? 1 : 2) : 0;
       for (int i : bucket[b]) group_chains.push_back(i);
       groups.push_back(g);
       maxGroupChains = std::max(maxGroupChains, (size_t)g.n);
@@ -2179,6 +2299,7 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
   x->nChain = nChain;
   x->nProc = nProc;
   x->nChainProc = nChainProc;
+  x->nLong = (int)std::max<long>(nLong, 1);
   x->nSlot = nSlot;
   x->nGhost = nGhost;
   x->nCase = nCase;
@@ -2262,6 +2383,7 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
   chk(hipMalloc(&x->ff, tl * 32 * nProc), "alloc ff");
   x->scratch_stride = std::max<size_t>(maxGroupChains, 1);
   chk(hipMalloc(&x->scratch, tl * 36 * std::max<size_t>(maxRows, 1) * x->scratch_stride), "alloc scratch");
+  chk(hipMalloc(&x->long_scratch, tl * 36 * (size_t)x->nLong), "alloc long-chain scratch");
   x->nPart = nblk(nCell);
   chk(hipMalloc(&x->partial, sizeof(double) * 2 * x->nPart), "alloc partial");
   if (nCase == 1)
@@ -2286,6 +2408,7 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
       (rc = upload(&x->cell_ent, cell_ent)) || (rc = upload(&x->ch_pbase, ch_pbase)) ||
       (rc = upload(&x->ch_nf, ch_nf)) || (rc = upload(&x->ch_ncell, ch_ncell)) ||
       (rc = upload(&x->ch_periodic, ch_periodic)) || (rc = upload(&x->group_chains, group_chains)) ||
+      (rc = upload(&x->ch_hbase, ch_hbase)) ||
       (!od_chain.empty() && ((rc = upload(&x->od_chain, od_chain)) || (rc = upload(&x->od_off, od_off)) ||
                              (rc = upload(&x->od_ent, od_ent))))) {
     free_ctx(x);

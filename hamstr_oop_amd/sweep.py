@@ -205,12 +205,20 @@ def run_batched(specs: Sequence[CaseSpec], meshes, tol: float, max_iters: int, s
     return RunResult(sec, iters, status, cell_iters, conv_time=conv_time), BatchOutcome(b, sub, redo), info
 
 
-def train_network(n_train: int, seed: int = SEED + 1, size=(256, 128), tol: float = 1e-8, max_iters: int = 20000,
+def train_network(n_train: int, seed: int = SEED + 1, size=(256, 128), tol: float = 1e-11, max_iters: int = 20000,
                   stall_window: int = 2000, per_case: int = 8192, chunk: int = 250, steps: int = 20000,
                   batch: int = 4096, lr: float = 2e-3, lr_final: float = 1e-5, mlp_seed: int = 2, progress=None):
     """Solves n_train cases (drawn like the sweep, on one mesh size) with the
     batched FP32-LHS path to tol, samples per_case cells of each converged field,
-    and trains the warm-start MLP on the GPU.  Returns (mlp, normaliser, info)."""
+    and trains the warm-start MLP on the GPU.
+
+    The depth of the training fields matters more than their number: fields at
+    L2 <= 1e-8 still carry a circulation several per cent off, the slowest
+    mode of the iteration, and a network trained on them leaves it to the
+    solver; trained on fields at 1e-11 it removes about half of the remaining
+    iterations (warm/cold iteration ratio 0.63 -> 0.28 on 64 held-out cases).
+    Cases that stall in a limit cycle above tol are left out of the set.
+    Returns (mlp, normaliser, info)."""
     from . import warmstart as ws
     specs = draw_table(n_train, seed, sizes=(size,))
     rng = np.random.default_rng(seed)
