@@ -7,7 +7,8 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
-SRCS = [os.path.join(HERE, "csrc", "hamstr_hip.hip"), os.path.join(HERE, "csrc", "mlp_hip.hip")]
+SRCS = [os.path.join(HERE, "csrc", "hamstr_hip.hip"), os.path.join(HERE, "csrc", "adi_long.hip"),
+        os.path.join(HERE, "csrc", "mlp_hip.hip")]
 LIB = os.path.join(HERE, "libhamstr_hip.so")
 DRIVER_SRC = os.path.join(HERE, "csrc", "hamstr_main.cpp")
 DRIVER = os.path.join(HERE, "hamstr_mi355x")

@@ -1,0 +1,31 @@
+// adi_long.hip -- the FAST ADI kernel for chains longer than a workgroup
+// (adi_fast_chain_long, k_adi_fast_long in hamstr_hip.hip), compiled as a
+// translation unit of its own so that its code does not change how the
+// common k_adi_fast is compiled, and its launcher.
+#define HAMSTR_LONG_TU
+#include "hamstr_hip.hip"
+
+hipError_t hamstr_adi_long_launch(bool fp32, int final_type, unsigned grid, unsigned block, size_t lds,
+                                  hipStream_t stream, const void* ct_, const void* lv_, const int* chains, int nch,
+                                  const void* ff, int nProc, double cfl, const double* sigma, double* r, double* dq,
+                                  int dqmode, int ncr, void* hs, int hstride, const int* hbase) {
+  ChainTab ct;
+  Live lv;
+  std::memcpy(&ct, ct_, sizeof ct);
+  std::memcpy(&lv, lv_, sizeof lv);
+  const dim3 g(grid), b(block);
+#define LONGL(T, F, M)                                                                                     \
+  hipLaunchKernelGGL((k_adi_fast_long<T, F, M>), g, b, lds, stream, ct, lv, chains, nch, (const T*)ff, nProc, \
+                     cfl, sigma, r, dq, dqmode, ncr, (T*)hs, hstride, hbase)
+  if (fp32) {
+    if (final_type == kFinalOpen) LONGL(float, kFinalOpen, 1024);
+    else if (final_type == kFinalPair) LONGL(float, kFinalPair, 1024);
+    else LONGL(float, kFinalOdd, 1024);
+  } else {
+    if (final_type == kFinalOpen) LONGL(double, kFinalOpen, 512);
+    else if (final_type == kFinalPair) LONGL(double, kFinalPair, 512);
+    else LONGL(double, kFinalOdd, 512);
+  }
+#undef LONGL
+  return hipGetLastError();
+}

@@ -109,7 +109,6 @@ struct ChainTab {
   const int* od_chain;
   const int* od_off;
   const int* od_ent;
-  const int* hbase;   // FAST chains longer than a workgroup: first scratch row of their odd rows
 };
 
 __device__ __forceinline__ double4 load_state(const int2 e, const double* __restrict__ q,
@@ -875,6 +874,11 @@ __device__ __forceinline__ void mv4_acc_lds(T* x, const T* A, const T* lds, int 
 
 enum { kFinalOpen = 0, kFinalPair = 1, kFinalOdd = 2 };
 
+#ifdef HAMSTR_LONG_TU
+// The long-chain path (chains longer than a workgroup) is compiled in a
+// translation unit of its own, adi_long.hip, which includes this file with
+// HAMSTR_LONG_TU defined: in one module with k_adi_fast it costs that kernel
+// 17 VGPRs and 2-4% of its time.
 // row i of chain ch assembled (assemble_row) and normalised by its diagonal
 // block; an open chain's end rows lose the coupling past the chain's ends
 template <typename T>
@@ -1149,26 +1153,276 @@ __device__ __forceinline__ void fast_solve_lds(T* lds, const int N, const int nc
   }
 }
 
-// ADI phase, FAST order, one thread and one LDS row per block row
+#endif  // HAMSTR_LONG_TU
+
+// ADI phase, FAST order, one thread and one LDS row per block row (steps as
+// in fast_solve_lds, written out in place: the shared function costs this
+// kernel about 20 VGPRs and 4% of its time)
 template <typename T, int FINAL>
 __device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const T* __restrict__ ff, int nProc,
                                                double cfl, const double* __restrict__ sigma,
                                                double* __restrict__ r, double* __restrict__ dq, int dqmode,
                                                int ncr) {
+#pragma clang fp contract(fast)
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* lds = reinterpret_cast<T*>(smem_raw);
   const int pb = ct.pbase[ch], nf = ct.nf[ch], N = ct.ncell[ch];
   const bool periodic = FINAL != kFinalOpen;
   const int i = threadIdx.x;
+  const bool active = i < N;
   const int S = lds_stride(N);
-  if (i < N) {
-    T a[16], c[16], f[4];
-    fast_row<T>(ct, ff, nProc, pb, nf, N, periodic, i, cfl, sigma, r, a, c, f);
+
+  // every row, assembled and normalised, goes to LDS
+  if (active) {
+    T a[16], c[16], f[4], B[16];
+    int cell;
+    assemble_row<T>(ct, ff, nProc, pb, nf, N, periodic, i, cfl, sigma, r, a, B, c, f, &cell);
+    if (!periodic) {
+      if (i == 0) {
+#pragma unroll
+        for (int e = 0; e < 16; e++) a[e] = T(0);
+      }
+      if (i == N - 1) {
+#pragma unroll
+        for (int e = 0; e < 16; e++) c[e] = T(0);
+      }
+    }
+    normalise_row(B, a, c, f);
     lds_store_row(lds, S, lpad(i), a, c, f);
   }
   __syncthreads();
-  fast_solve_lds<T, FINAL>(lds, N, ncr, i);
-  if (i < N) {
+
+  // (1) ncr levels of cyclic reduction: at stride d the survivors (rows at
+  // multiples of 2d) absorb rows i - d and i + d, which drop out.  The work is
+  // compacted: thread t updates survivor row t*2d, so only the first
+  // ceil(survivors/64) waves issue instructions.  A row is read only at the
+  // level that eliminates it, so one barrier per level suffices.
+  int d = 1;  // stride between the rows still in the system
+  int n = N;  // cycle length of the remaining system (closed chains)
+  int ncr_done = 0;
+  while (ncr_done < ncr && (periodic ? (n % 2 == 0 && n >= 4) : (d < N))) {
+    const int nsurv = (N + 2 * d - 1) / (2 * d);
+    if (i < nsurv) {
+      const int row = i * 2 * d;
+      const int iL = periodic ? (row - d + N) % N : row - d;
+      const int iR = periodic ? (row + d) % N : row + d;
+      T ra[16], rc[16], rf[4], M[16], na[16], nc[16];
+      lds_load_row(lds, S, lpad(row), ra, rc, rf);
+#pragma unroll
+      for (int e = 0; e < 16; e++) {
+        M[e] = (e % 5 == 0) ? T(1) : T(0);
+        na[e] = T(0);
+        nc[e] = T(0);
+      }
+      if (periodic || iL >= 0) {
+        mm4_acc_lds(M, ra, lds, S, 16, lpad(iL));  // M -= A^_i C^_L
+        mm4_acc_lds(na, ra, lds, S, 0, lpad(iL));  // A^new = -A^_i A^_L
+        mv4_acc_lds(rf, ra, lds, S, lpad(iL));     // F -= A^_i F^_L
+      }
+      if (periodic || iR < N) {
+        mm4_acc_lds(M, rc, lds, S, 0, lpad(iR));   // M -= C^_i A^_R
+        mm4_acc_lds(nc, rc, lds, S, 16, lpad(iR)); // C^new = -C^_i C^_R
+        mv4_acc_lds(rf, rc, lds, S, lpad(iR));     // F -= C^_i F^_R
+      }
+      normalise_row(M, na, nc, rf);
+      lds_store_row(lds, S, lpad(row), na, nc, rf);
+    }
+    __syncthreads();
+    d *= 2;
+    n /= 2;
+    ncr_done++;
+  }
+
+  // (2) parallel cyclic reduction of the rows at multiples of d (members),
+  // thread t holding member row t*d in registers
+  const int nmem = (N + d - 1) / d;
+  const bool member = i < nmem;
+  const int mrow = i * d;
+  T a[16], c[16], f[4];
+  if (member) lds_load_row(lds, S, lpad(mrow), a, c, f);
+  int dd = d;
+  while (periodic ? (n % 2 == 0 && n >= 4) : (dd < N)) {
+    if (member) {
+      const int iL = periodic ? (mrow - dd + N) % N : mrow - dd;
+      const int iR = periodic ? (mrow + dd) % N : mrow + dd;
+      T M[16], na[16], nc[16];
+#pragma unroll
+      for (int e = 0; e < 16; e++) {
+        M[e] = (e % 5 == 0) ? T(1) : T(0);
+        na[e] = T(0);
+        nc[e] = T(0);
+      }
+      if (periodic || iL >= 0) {
+        mm4_acc_lds(M, a, lds, S, 16, lpad(iL));
+        mm4_acc_lds(na, a, lds, S, 0, lpad(iL));
+        mv4_acc_lds(f, a, lds, S, lpad(iL));
+      }
+      if (periodic || iR < N) {
+        mm4_acc_lds(M, c, lds, S, 0, lpad(iR));
+        mm4_acc_lds(nc, c, lds, S, 16, lpad(iR));
+        mv4_acc_lds(f, c, lds, S, lpad(iR));
+      }
+      normalise_row(M, na, nc, f);
+#pragma unroll
+      for (int e = 0; e < 16; e++) {
+        a[e] = na[e];
+        c[e] = nc[e];
+      }
+    }
+    __syncthreads();
+    if (member) lds_store_row(lds, S, lpad(mrow), a, c, f);
+    __syncthreads();
+    dd *= 2;
+    n /= 2;
+  }
+
+  // (3) the members' final systems; their solutions go to their F slots
+  if (FINAL == kFinalOpen) {
+    // decoupled: x = f, already in the F slot
+  } else if (FINAL == kFinalPair) {
+    // rows i and j = i + N/2 couple only to each other:
+    // x_i = (I - P Q)^-1 (f_i - P f_j), P = A^_i + C^_i, Q = A^_j + C^_j
+    T x[4];
+    if (member) {
+      const int j = (mrow + N / 2) % N;
+      T P[16], Q[16], M[16];
+#pragma unroll
+      for (int e = 0; e < 16; e++) {
+        P[e] = a[e] + c[e];
+        Q[e] = lds[e * S + lpad(j)] + lds[(16 + e) * S + lpad(j)];
+        M[e] = (e % 5 == 0) ? T(1) : T(0);
+      }
+      mm4_acc(M, P, Q);
+#pragma unroll
+      for (int e = 0; e < 4; e++) x[e] = f[e];
+      mv4_acc_lds(x, P, lds, S, lpad(j));
+      const LU4<T> L = lu4_fast(M);
+      lu4_solve_fast(L, x, 1);
+    }
+    __syncthreads();
+    if (member) {
+#pragma unroll
+      for (int e = 0; e < 4; e++) lds[(32 + e) * S + lpad(mrow)] = x[e];
+    }
+    __syncthreads();
+  } else {
+    // odd cycles {r, r+dd, r+2dd, ...} of n rows: the leader r < dd runs the
+    // cyclic block Thomas on the normalised rows (B = I) in place
+    if (member && mrow < dd) {
+      T Cp[16], Fp[4], ap[16];
+      for (int k = 0; k < n; k++) {
+        const int rk = mrow + k * dd;
+        T A[16], Bm[16], C[16], F[4], sp[16];
+        lds_load_row(lds, S, lpad(rk), A, C, F);
+#pragma unroll
+        for (int e = 0; e < 16; e++) Bm[e] = (e % 5 == 0) ? T(1) : T(0);
+        if (k != 0) {
+          mm4_acc(Bm, A, Cp);
+          mv4_acc(F, A, Fp);
+#pragma unroll
+          for (int e = 0; e < 16; e++) sp[e] = T(0);
+          mm4_acc(sp, A, ap);
+        } else {
+#pragma unroll
+          for (int e = 0; e < 16; e++) sp[e] = A[e];
+        }
+        const LU4<T> L = lu4_fast(Bm);
+        lu4_solve_fast(L, F, 1);
+        if (k != n - 2) {
+#pragma unroll
+          for (int j = 0; j < 4; j++) {
+            lu4_solve_fast(L, C + j, 4);
+            lu4_solve_fast(L, sp + j, 4);
+          }
+        } else {
+#pragma unroll
+          for (int e = 0; e < 16; e++) C[e] += sp[e];
+#pragma unroll
+          for (int j = 0; j < 4; j++) lu4_solve_fast(L, C + j, 4);
+#pragma unroll
+          for (int e = 0; e < 16; e++) sp[e] = C[e];
+        }
+        lds_store_row(lds, S, lpad(rk), sp, C, F);  // row k holds (spike a', C', F')
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+          Cp[e] = C[e];
+          ap[e] = sp[e];
+        }
+#pragma unroll
+        for (int e = 0; e < 4; e++) Fp[e] = F[e];
+      }
+      T Bl[16], Cl[16], Fl[4];
+#pragma unroll
+      for (int e = 0; e < 16; e++) {
+        Bl[e] = (e % 5 == 0) ? T(1) : T(0);
+        Cl[e] = Cp[e];
+      }
+#pragma unroll
+      for (int e = 0; e < 4; e++) Fl[e] = Fp[e];
+      for (int k = 0; k < n - 1; k++) {
+        const int rk = mrow + k * dd;
+        T tt[16];
+#pragma unroll
+        for (int e = 0; e < 16; e++) tt[e] = T(0);
+        mm4_acc_lds(tt, Cl, lds, S, 16, lpad(rk));
+        mm4_acc_lds(Bl, Cl, lds, S, 0, lpad(rk));
+        mv4_acc_lds(Fl, Cl, lds, S, lpad(rk));
+#pragma unroll
+        for (int e = 0; e < 16; e++) Cl[e] = tt[e];
+      }
+      const LU4<T> L = lu4_fast(Bl);
+      lu4_solve_fast(L, Fl, 1);
+      T xn[4] = {Fl[0], Fl[1], Fl[2], Fl[3]};
+      // back sweep; the solution overwrites the F slots
+      for (int k = n - 1; k >= 0; k--) {
+        const int rk = mrow + k * dd;
+        T sf[4];
+        if (k == n - 1) {
+#pragma unroll
+          for (int e = 0; e < 4; e++) sf[e] = Fl[e];
+        } else {
+          T sc[16];
+#pragma unroll
+          for (int e = 0; e < 4; e++) sf[e] = lds[(32 + e) * S + lpad(rk)];
+#pragma unroll
+          for (int e = 0; e < 16; e++) sc[e] = lds[(16 + e) * S + lpad(rk)];
+          mv4_acc(sf, sc, xn);
+          if (k != n - 2) {
+#pragma unroll
+            for (int e = 0; e < 16; e++) sc[e] = lds[e * S + lpad(rk)];
+            mv4_acc(sf, sc, Fl);
+          }
+        }
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          xn[e] = sf[e];
+          lds[(32 + e) * S + lpad(rk)] = sf[e];
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // (4) back substitution through the cyclic-reduction levels, compacted:
+  // thread t takes row (2t+1)*dl, eliminated at stride dl, whose LDS row
+  // still holds its coefficients as eliminated: x = F^ - A^ x_(i-dl) - C^ x_(i+dl)
+  for (int l = ncr_done - 1; l >= 0; l--) {
+    const int dl = 1 << l;
+    const int nel = ((N + dl - 1) / dl) / 2;  // odd multiples of dl below N
+    if (i < nel) {
+      const int row = (2 * i + 1) * dl;
+      const int iL = periodic ? (row - dl + N) % N : row - dl;
+      const int iR = periodic ? (row + dl) % N : row + dl;
+      T ra[16], rc[16], rf[4];
+      lds_load_row(lds, S, lpad(row), ra, rc, rf);
+      if (periodic || iL >= 0) mv4_acc_lds(rf, ra, lds, S, lpad(iL));
+      if (periodic || iR < N) mv4_acc_lds(rf, rc, lds, S, lpad(iR));
+#pragma unroll
+      for (int e = 0; e < 4; e++) lds[(32 + e) * S + lpad(row)] = rf[e];
+    }
+    __syncthreads();
+  }
+  if (active) {
     T x[4];
 #pragma unroll
     for (int e = 0; e < 4; e++) x[e] = lds[(32 + e) * S + lpad(i)];
@@ -1176,6 +1430,7 @@ __device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const 
   }
 }
 
+#ifdef HAMSTR_LONG_TU
 // ADI phase, FAST order, for chains longer than a workgroup: thread t takes
 // rows 2t and 2t+1.  The first cyclic-reduction level runs before anything
 // enters LDS: the odd rows go to global scratch `hs` (element-major, stride
@@ -1184,12 +1439,14 @@ __device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const 
 // system of ceil(N/2) rows that fast_solve_lds finishes with ncr-1 more
 // levels.  The odd rows then back-substitute from scratch.  The arithmetic is
 // that of adi_fast_chain with ncr levels.  Needs open chains of N >= 2 rows or
-// closed chains of an even N >= 4, and ncr >= 1.
+// closed chains of an even N >= 4, and ncr >= 1.  hbase[ch]: the chain's first
+// scratch row.
 template <typename T, int FINAL>
 __device__ __forceinline__ void adi_fast_chain_long(ChainTab ct, const int ch, const T* __restrict__ ff, int nProc,
                                                     double cfl, const double* __restrict__ sigma,
                                                     double* __restrict__ r, double* __restrict__ dq, int dqmode,
-                                                    int ncr, T* __restrict__ hs, int hstride) {
+                                                    int ncr, T* __restrict__ hs, int hstride,
+                                                    const int* __restrict__ hbase) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* lds = reinterpret_cast<T*>(smem_raw);
   const int pb = ct.pbase[ch], nf = ct.nf[ch], N = ct.ncell[ch];
@@ -1197,7 +1454,7 @@ __device__ __forceinline__ void adi_fast_chain_long(ChainTab ct, const int ch, c
   const int t = threadIdx.x;
   const int M = (N + 1) / 2;
   const int SM = lds_stride(M);
-  T* hrow = hs + ct.hbase[ch];
+  T* hrow = hs + hbase[ch];
   if (2 * t + 1 < N) {
     T a[16], c[16], f[4];
     fast_row<T>(ct, ff, nProc, pb, nf, N, periodic, 2 * t + 1, cfl, sigma, r, a, c, f);
@@ -1232,26 +1489,44 @@ __device__ __forceinline__ void adi_fast_chain_long(ChainTab ct, const int ch, c
   }
 }
 
+#endif  // HAMSTR_LONG_TU
+
 // one chain per workgroup pass; the workgroup walks the group's chains with a
 // stride of the grid size (a persistent launch: chains are sorted longest
 // first, so every pass of the grid does similar work)
-template <typename T, int FINAL, int MAXT, bool LONG>
+template <typename T, int FINAL, int MAXT>
 __global__ __launch_bounds__(MAXT) void k_adi_fast(ChainTab ct, Live lv, const int* __restrict__ chains, int nch,
                                                    const T* __restrict__ ff, int nProc, double cfl,
                                                    const double* __restrict__ sigma, double* __restrict__ r,
-                                                   double* __restrict__ dq, int dqmode, int ncr,
-                                                   T* __restrict__ hs, int hstride) {
+                                                   double* __restrict__ dq, int dqmode, int ncr) {
   const int vb = xcd_block(blockIdx.x, gridDim.x);
   for (int w = vb; w < nch; w += gridDim.x) {
     const int ch = chains[w];
     if (lv.chain && !lv.chain[ch]) continue;  // the same for the whole workgroup
-    if (LONG)
-      adi_fast_chain_long<T, FINAL>(ct, ch, ff, nProc, cfl, sigma, r, dq, dqmode, ncr, hs, hstride);
-    else
-      adi_fast_chain<T, FINAL>(ct, ch, ff, nProc, cfl, sigma, r, dq, dqmode, ncr);
+    adi_fast_chain<T, FINAL>(ct, ch, ff, nProc, cfl, sigma, r, dq, dqmode, ncr);
     __syncthreads();
   }
 }
+
+#ifdef HAMSTR_LONG_TU
+// the same walk over chains longer than a workgroup (a kernel of its own: a
+// shared one costs the common case registers and time)
+template <typename T, int FINAL, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_adi_fast_long(ChainTab ct, Live lv, const int* __restrict__ chains,
+                                                        int nch, const T* __restrict__ ff, int nProc, double cfl,
+                                                        const double* __restrict__ sigma, double* __restrict__ r,
+                                                        double* __restrict__ dq, int dqmode, int ncr,
+                                                        T* __restrict__ hs, int hstride,
+                                                        const int* __restrict__ hbase) {
+  const int vb = xcd_block(blockIdx.x, gridDim.x);
+  for (int w = vb; w < nch; w += gridDim.x) {
+    const int ch = chains[w];
+    if (lv.chain && !lv.chain[ch]) continue;  // the same for the whole workgroup
+    adi_fast_chain_long<T, FINAL>(ct, ch, ff, nProc, cfl, sigma, r, dq, dqmode, ncr, hs, hstride, hbase);
+    __syncthreads();
+  }
+}
+#endif  // HAMSTR_LONG_TU
 
 // --------------------------------------------------------------------------
 // lin_faces: reconstruction derivatives + lmat*dqL + rmat*dqR per face
@@ -1684,6 +1959,14 @@ struct DevBuf {
 
 }  // namespace
 
+#ifndef HAMSTR_LONG_TU
+// launcher of k_adi_fast_long, defined in adi_long.hip (ct, lv: a ChainTab and
+// a Live, whose layouts both translation units share)
+hipError_t hamstr_adi_long_launch(bool fp32, int final_type, unsigned grid, unsigned block, size_t lds,
+                                  hipStream_t stream, const void* ct, const void* lv, const int* chains, int nch,
+                                  const void* ff, int nProc, double cfl, const double* sigma, double* r, double* dq,
+                                  int dqmode, int ncr, void* hs, int hstride, const int* hbase);
+
 // ==========================================================================
 // context
 // ==========================================================================
@@ -1779,7 +2062,7 @@ struct hamstr_ctx {
   FaceTab ftab() const { return FaceTab{pslot, src, nvec, refmtx, qinf}; }
   ChainTab ctab() const {
     return ChainTab{ch_pbase, ch_nf, ch_ncell, ch_periodic, pleft, pright, pface, refmtx,
-                    od_chain, od_off, od_ent, ch_hbase};
+                    od_chain, od_off, od_ent};
   }
   bool lhs32() const { return prm.lhs == HAMSTR_LHS_FP32; }
 };
@@ -1895,12 +2178,15 @@ int launch_group(hamstr_ctx* x, const Group& g, int grid, int dqmode, int ncr) {
     const int nc = g.two ? std::max(ncr, 1) : ncr;
     const size_t lds = (x->lhs32() ? sizeof(float) : sizeof(double)) * 36 *
                        (size_t)lds_stride(g.two ? (g.maxN + 1) / 2 : g.maxN);
-#define FASTL(T, F, M, L)                                                                                        \
-  hipLaunchKernelGGL((k_adi_fast<T, F, M, L>), gr, b, lds, x->stream, ct, x->live(), chains, g.n, (const T*)x->ff,  \
-                     x->nProc, x->c.cfl, x->sigma, x->r, x->dq, dqmode, nc, (T*)x->long_scratch, x->nLong)
-#define FASTT(T, F, M) \
-  if (g.two) FASTL(T, F, M, true); else FASTL(T, F, M, false)
-    if (x->lhs32()) {
+#define FASTT(T, F, M)                                                                                            \
+  hipLaunchKernelGGL((k_adi_fast<T, F, M>), gr, b, lds, x->stream, ct, x->live(), chains, g.n, (const T*)x->ff,   \
+                     x->nProc, x->c.cfl, x->sigma, x->r, x->dq, dqmode, nc)
+    if (g.two) {
+      const Live lv = x->live();
+      HIPCHK(hamstr_adi_long_launch(x->lhs32(), g.final, (unsigned)grid, (unsigned)g.block, lds, x->stream, &ct, &lv,
+                                    chains, g.n, x->ff, x->nProc, x->c.cfl, x->sigma, x->r, x->dq, dqmode, nc,
+                                    x->long_scratch, x->nLong, x->ch_hbase));
+    } else if (x->lhs32()) {
       if (g.final == kFinalOpen) FASTT(float, kFinalOpen, 1024);
       else if (g.final == kFinalPair) FASTT(float, kFinalPair, 1024);
       else FASTT(float, kFinalOdd, 1024);
@@ -1910,7 +2196,6 @@ int launch_group(hamstr_ctx* x, const Group& g, int grid, int dqmode, int ncr) {
       else FASTT(double, kFinalOdd, 512);
     }
 #undef FASTT
-#undef FASTL
   } else if (x->lhs32()) {
     hipLaunchKernelGGL((k_adi_exact<float>), dim3(nblk(g.n, 64)), dim3(64), 0, x->stream, ct, x->live(), chains, g.n,
                        (const float*)x->ff, x->nProc, x->c.cfl, x->sigma, x->r, (float*)x->scratch,
@@ -3518,3 +3803,4 @@ int hamstr_block_tridag_periodic4(int nsys, int N, const double* a, const double
 }
 
 }  // extern "C"
+#endif  // !HAMSTR_LONG_TU
