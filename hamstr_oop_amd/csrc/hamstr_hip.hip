@@ -78,8 +78,13 @@ struct Consts {
   int nChainProc;  // processed faces [0, nChainProc) lie on chains, [nChainProc, nProc) are orphan faces
 };
 
+// pslot codes: the slot index in bits 0-28; bit 30: last face of an open
+// chain; bit 29: an orphan face, whose two cells sit in slots s and s+1
+constexpr int kSlotMask = (1 << 29) - 1;
+constexpr int kOrphanSlot = 1 << 29;
+
 struct FaceTab {
-  const int* pslot;      // slot of row n (reference index), bit 30: last face of an open chain
+  const int* pslot;      // slot code of the face (see kSlotMask)
   const int2* src;       // per slot: (cell, wall face); cell < 0: freestream of case .y; face < 0: copy
   const double* nvec;    // [nProc][2]
   const double* refmtx;  // [nFace][4]
@@ -200,22 +205,28 @@ __global__ __launch_bounds__(kBlock) void k_rhs_faces(FaceTab ft, Consts c, Live
                                                       double* __restrict__ flux, double* __restrict__ sr,
                                                       TL* __restrict__ ff, int* __restrict__ nanflag) {
   const int p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (p >= c.nChainProc || (lv.face && !lv.face[p])) return;
+  if (p >= c.nProc || (lv.face && !lv.face[p])) return;
   const int code = ft.pslot[p];
-  const int s = code & 0x3fffffff;
-  const bool flip = (code >> 30) & 1;
-  constexpr int lo = (ORDER == 5) ? 0 : (ORDER == 3 ? 1 : 2);
-  constexpr int hi = (ORDER == 5) ? 5 : (ORDER == 3 ? 4 : 3);
-  double f[6][4];
-#pragma unroll
-  for (int j = lo; j <= hi; j++) put4(f[j], load_state(ft.src[s - 2 + j], q, ft.refmtx, ft.qinf));
-  double qln[4], qrn1[4];
-  reconstruct_face<ORDER>(f, c.eps, c.qt, qln, qrn1);
+  const int s = code & kSlotMask;
   double left[4], right[4];
+  if (code & kOrphanSlot) {
+    // orphan face: first order, the two cell states
+    put4(left, load_state(ft.src[s], q, ft.refmtx, ft.qinf));
+    put4(right, load_state(ft.src[s + 1], q, ft.refmtx, ft.qinf));
+  } else {
+    const bool flip = (code >> 30) & 1;
+    constexpr int lo = (ORDER == 5) ? 0 : (ORDER == 3 ? 1 : 2);
+    constexpr int hi = (ORDER == 5) ? 5 : (ORDER == 3 ? 4 : 3);
+    double f[6][4];
 #pragma unroll
-  for (int k = 0; k < 4; k++) {
-    left[k] = flip ? qln[k] : qrn1[k];
-    right[k] = flip ? qrn1[k] : qln[k];
+    for (int j = lo; j <= hi; j++) put4(f[j], load_state(ft.src[s - 2 + j], q, ft.refmtx, ft.qinf));
+    double qln[4], qrn1[4];
+    reconstruct_face<ORDER>(f, c.eps, c.qt, qln, qrn1);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      left[k] = flip ? qln[k] : qrn1[k];
+      right[k] = flip ? qrn1[k] : qln[k];
+    }
   }
   const double2 n = *reinterpret_cast<const double2*>(ft.nvec + 2 * (size_t)p);
   double fl[4], s_r;
@@ -1537,26 +1548,32 @@ __global__ __launch_bounds__(kBlock) void k_lin_faces(FaceTab ft, Consts c, Live
                                                       const double* __restrict__ dq, const TL* __restrict__ ff,
                                                       double* __restrict__ lflux) {
   const int p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (p >= c.nChainProc || (lv.face && !lv.face[p])) return;
+  if (p >= c.nProc || (lv.face && !lv.face[p])) return;
   const int code = ft.pslot[p];
-  const int s = code & 0x3fffffff;
-  const bool flip = (code >> 30) & 1;
-  constexpr int lo = (ORDER == 5) ? 0 : (ORDER == 3 ? 1 : 2);
-  constexpr int hi = (ORDER == 5) ? 5 : (ORDER == 3 ? 4 : 3);
-  double f[6][4], d[6][4];
-#pragma unroll
-  for (int j = lo; j <= hi; j++) {
-    const int2 e = ft.src[s - 2 + j];
-    if (ORDER != 1) put4(f[j], load_state(e, q, ft.refmtx, ft.qinf));
-    put4(d[j], load_dstate(e, dq, ft.refmtx));
-  }
-  double dln[4], drn1[4];
-  reconstruct_dface<ORDER, FASTJ>(f, d, c.eps, c.qt, dln, drn1);
+  const int s = code & kSlotMask;
   double dl[4], dr[4];
+  if (code & kOrphanSlot) {
+    // orphan face: the full product of the two cells' dq
+    put4(dl, load_dstate(ft.src[s], dq, ft.refmtx));
+    put4(dr, load_dstate(ft.src[s + 1], dq, ft.refmtx));
+  } else {
+    const bool flip = (code >> 30) & 1;
+    constexpr int lo = (ORDER == 5) ? 0 : (ORDER == 3 ? 1 : 2);
+    constexpr int hi = (ORDER == 5) ? 5 : (ORDER == 3 ? 4 : 3);
+    double f[6][4], d[6][4];
 #pragma unroll
-  for (int k = 0; k < 4; k++) {
-    dl[k] = flip ? dln[k] : drn1[k];
-    dr[k] = flip ? drn1[k] : dln[k];
+    for (int j = lo; j <= hi; j++) {
+      const int2 e = ft.src[s - 2 + j];
+      if (ORDER != 1) put4(f[j], load_state(e, q, ft.refmtx, ft.qinf));
+      put4(d[j], load_dstate(e, dq, ft.refmtx));
+    }
+    double dln[4], drn1[4];
+    reconstruct_dface<ORDER, FASTJ>(f, d, c.eps, c.qt, dln, drn1);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      dl[k] = flip ? dln[k] : drn1[k];
+      dr[k] = flip ? drn1[k] : dln[k];
+    }
   }
   double out[4];
 #pragma unroll
@@ -1573,70 +1590,12 @@ __global__ __launch_bounds__(kBlock) void k_lin_faces(FaceTab ft, Consts c, Live
   *reinterpret_cast<double4*>(lflux + 4 * (size_t)p) = make_double4(out[0], out[1], out[2], out[3]);
 }
 
-// --------------------------------------------------------------------------
-// Orphan faces (interior faces in no chain; meshes with triangles).  An
-// extension with no counterpart in the reference: the first-order Roe flux of
-// the two cell states (and its frozen-Roe Jacobians), and in the linear
-// residual the full product lmat dq_L + rmat dq_R.  Their entries come after
-// the chain entries in each cell's list, as the oracle sums them.
-// --------------------------------------------------------------------------
-template <typename TL, bool IMPL, bool FASTJ>
-__global__ __launch_bounds__(kBlock) void k_orphan_faces(Consts c, Live lv, const int* __restrict__ pleft,
-                                                         const int* __restrict__ pright,
-                                                         const double* __restrict__ nvec,
-                                                         const double* __restrict__ q, double* __restrict__ flux,
-                                                         double* __restrict__ sr, TL* __restrict__ ff,
-                                                         int* __restrict__ nanflag) {
-  const int p = c.nChainProc + blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= c.nProc || (lv.face && !lv.face[p])) return;
-  const double4 a = *reinterpret_cast<const double4*>(q + 4 * (size_t)pleft[p]);
-  const double4 b = *reinterpret_cast<const double4*>(q + 4 * (size_t)pright[p]);
-  double left[4], right[4];
-  put4(left, a);
-  put4(right, b);
-  const double2 n = *reinterpret_cast<const double2*>(nvec + 2 * (size_t)p);
-  double fl[4], s_r;
-  flux_roe2d(n.x, n.y, left, right, c.gamma, fl, &s_r);
-  *reinterpret_cast<double4*>(flux + 4 * (size_t)p) = make_double4(fl[0], fl[1], fl[2], fl[3]);
-  sr[p] = s_r;
-  if (s_r != s_r) atomicOr(nanflag, 1);
-  if (IMPL) {
-    TL lm[16], rm[16];
-    if (FASTJ)
-      jacobian_roe2d_fast<TL>(n.x, n.y, left, right, c.gamma, lm, rm);
-    else
-      jacobian_roe2d<TL>(n.x, n.y, left, right, c.gamma, lm, rm);
-#pragma unroll
-    for (int e = 0; e < 16; e++) {
-      ff[(size_t)e * c.nProc + p] = lm[e];
-      ff[(size_t)(16 + e) * c.nProc + p] = rm[e];
-    }
-  }
-}
-
-template <typename TL>
-__global__ __launch_bounds__(kBlock) void k_orphan_lin(Consts c, Live lv, const int* __restrict__ pleft,
-                                                       const int* __restrict__ pright, const double* __restrict__ dq,
-                                                       const TL* __restrict__ ff, double* __restrict__ lflux) {
-  const int p = c.nChainProc + blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= c.nProc || (lv.face && !lv.face[p])) return;
-  double dl[4], dr[4];
-  put4(dl, *reinterpret_cast<const double4*>(dq + 4 * (size_t)pleft[p]));
-  put4(dr, *reinterpret_cast<const double4*>(dq + 4 * (size_t)pright[p]));
-  double out[4];
-#pragma unroll
-  for (int j = 0; j < 4; j++) {
-    double acc = 0;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const double lm = double(ff[(size_t)(4 * j + k) * c.nProc + p]);
-      const double rm = double(ff[(size_t)(16 + 4 * j + k) * c.nProc + p]);
-      acc += ((lm * dl[k]) + (rm * dr[k]));
-    }
-    out[j] = acc;
-  }
-  *reinterpret_cast<double4*>(lflux + 4 * (size_t)p) = make_double4(out[0], out[1], out[2], out[3]);
-}
+// Orphan faces (interior faces in no chain; meshes with triangles) run in
+// k_rhs_faces and k_lin_faces with the chain faces: an extension with no
+// counterpart in the reference, the first-order Roe flux of the two cell
+// states (and its frozen-Roe Jacobians), and in the linear residual the full
+// product lmat dq_L + rmat dq_R.  Their entries come after the chain entries
+// in each cell's list, as the oracle sums them.
 
 // lin_cells: r = r0 - dq - CFL/sigma * sum(+-lflux), norms, optional q += dq
 // (computeLinearRHS.cpp:45, 326-368; ADI.cpp:258-264)
@@ -2071,7 +2030,7 @@ namespace {
 
 int launch_rhs_faces(hamstr_ctx* x, bool implicit) {
   const FaceTab ft = x->ftab();
-  dim3 g(nblk(x->nChainProc)), b(kBlock);
+  dim3 g(nblk(x->nProc)), b(kBlock);
 #define RHSF(O, T, I, F)                                                                                         \
   hipLaunchKernelGGL((k_rhs_faces<O, T, I, F>), g, b, 0, x->stream, ft, x->c, x->live(), x->q, x->flux, x->sr, (T*)x->ff, \
                      x->nanflag)
@@ -2091,29 +2050,12 @@ int launch_rhs_faces(hamstr_ctx* x, bool implicit) {
 #undef RHSO
 #undef RHSF
   HIPCHK(hipGetLastError());
-  const int nOrph = x->nProc - x->nChainProc;
-  if (nOrph > 0) {
-#define ORPH(T, I, F)                                                                                            \
-  hipLaunchKernelGGL((k_orphan_faces<T, I, F>), dim3(nblk(nOrph)), b, 0, x->stream, x->c, x->live(), x->pleft,     \
-                     x->pright, x->nvec, x->q, x->flux, x->sr, (T*)x->ff, x->nanflag)
-    if (x->lhs32()) {
-      if (!implicit) ORPH(float, false, false);
-      else if (fj) ORPH(float, true, true);
-      else ORPH(float, true, false);
-    } else {
-      if (!implicit) ORPH(double, false, false);
-      else if (fj) ORPH(double, true, true);
-      else ORPH(double, true, false);
-    }
-#undef ORPH
-    HIPCHK(hipGetLastError());
-  }
   return HAMSTR_OK;
 }
 
 int launch_lin_faces(hamstr_ctx* x) {
   const FaceTab ft = x->ftab();
-  dim3 g(nblk(x->nChainProc)), b(kBlock);
+  dim3 g(nblk(x->nProc)), b(kBlock);
 #define LINF(O, T, F)                                                                                         \
   hipLaunchKernelGGL((k_lin_faces<O, T, F>), g, b, 0, x->stream, ft, x->c, x->live(), x->q, x->dq, (const T*)x->ff, \
                      x->lflux)
@@ -2129,16 +2071,6 @@ int launch_lin_faces(hamstr_ctx* x) {
 #undef LINO
 #undef LINF
   HIPCHK(hipGetLastError());
-  const int nOrph = x->nProc - x->nChainProc;
-  if (nOrph > 0) {
-    if (x->lhs32())
-      hipLaunchKernelGGL((k_orphan_lin<float>), dim3(nblk(nOrph)), b, 0, x->stream, x->c, x->live(), x->pleft,
-                         x->pright, x->dq, (const float*)x->ff, x->lflux);
-    else
-      hipLaunchKernelGGL((k_orphan_lin<double>), dim3(nblk(nOrph)), b, 0, x->stream, x->c, x->live(), x->pleft,
-                         x->pright, x->dq, (const double*)x->ff, x->lflux);
-    HIPCHK(hipGetLastError());
-  }
   return HAMSTR_OK;
 }
 
@@ -2454,7 +2386,9 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
     if (iface < 0 || iface >= nFace) return fail(HAMSTR_ERR_MESH, "orphan face out of range");
     if (faces[6 * iface + 4] < 0) return fail(HAMSTR_ERR_MESH, "orphan faces must be interior faces");
     if (++proc_count[iface] > 1) return fail(HAMSTR_ERR_MESH, "orphan face also on a chain or listed twice");
-    pslot.push_back(0);  // no chain slot
+    pslot.push_back((int)src.size() | kOrphanSlot);  // the two cells, as plain copies
+    src.push_back(make_int2(faces[6 * iface + 2], -1));
+    src.push_back(make_int2(faces[6 * iface + 4], -1));
     pleft.push_back(faces[6 * iface + 2]);
     pright.push_back(faces[6 * iface + 4]);
     pface.push_back(iface);
@@ -2463,7 +2397,7 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
   }
   const int nProc = (int)pslot.size();
   const int nSlot = (int)src.size();
-  if (nSlot >= (1 << 30)) return fail(HAMSTR_ERR_MESH, "too many chain slots");
+  if (nSlot > kSlotMask) return fail(HAMSTR_ERR_MESH, "too many chain slots");
 
   // ---- per-cell accumulation lists in processing order ----
   std::vector<int> cnt(nCell + 1, 0);
