@@ -455,13 +455,15 @@ def sweep_section(args, with_cpu=True):
         log(f"batched FP32-LHS warm: {bw.seconds:.2f} s ({iw})")
     o32.close()
     res["speedup_batched_fp32_cold_vs_sequential_fp64_cold"] = cold64.seconds / b32.seconds
-    # the same comparison over the cases the sequential FP64 runs converge,
-    # without the stall-detection time of cases that never converge
-    conv = [k for k in range(len(specs)) if cold64.status[k] == 1]
+    # the same comparison over the cases every run converges, without the
+    # stall-detection time of cases that never converge (a batched run's time
+    # is when the last of them converged)
+    runs = (("batched_fp64_cold", b64), ("batched_fp32_cold", b32)) + \
+        ((("batched_fp32_warm", bw),) if args.train_cases > 0 else ())
+    conv = [k for k in range(len(specs)) if cold64.status[k] == 1 and all(r.status[k] == 1 for _, r in runs)]
     t_seq = cold64.time_to_converge(conv)
     cmp = {"cases": len(conv), "sequential_fp64_cold_s": t_seq}
-    for name, r in (("batched_fp64_cold", b64), ("batched_fp32_cold", b32)) + \
-            ((("batched_fp32_warm", bw),) if args.train_cases > 0 else ()):
+    for name, r in runs:
         t = r.time_to_converge(conv)
         cmp[name + "_s"] = t
         cmp["speedup_" + name] = (t_seq / t) if t else None
