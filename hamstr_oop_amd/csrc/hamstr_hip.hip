@@ -2118,6 +2118,12 @@ int launch_group(hamstr_ctx* x, const Group& g, int grid, int dqmode, int ncr) {
       HIPCHK(hamstr_adi_long_launch(x->lhs32(), g.final, (unsigned)grid, (unsigned)g.block, lds, x->stream, &ct, &lv,
                                     chains, g.n, x->ff, x->nProc, x->c.cfl, x->sigma, x->r, x->dq, dqmode, nc,
                                     x->long_scratch, x->nLong, x->ch_hbase));
+    } else if (x->lhs32() && g.block <= 512) {
+      // a 512-thread bound gives the FP32 kernels 256 VGPRs instead of 128
+      // (no spills): 10% faster on chains that fit
+      if (g.final == kFinalOpen) FASTT(float, kFinalOpen, 512);
+      else if (g.final == kFinalPair) FASTT(float, kFinalPair, 512);
+      else FASTT(float, kFinalOdd, 512);
     } else if (x->lhs32()) {
       if (g.final == kFinalOpen) FASTT(float, kFinalOpen, 1024);
       else if (g.final == kFinalPair) FASTT(float, kFinalPair, 1024);
