@@ -337,6 +337,86 @@ __global__ __launch_bounds__(kRedBlock) void k_norm_slots(const double* __restri
 }
 
 // --------------------------------------------------------------------------
+// per-cell sums over the cell's face list in list order (the reference's
+// order).  The first kGather entries and their face values are all loaded
+// before any is added, so their loads are in flight together; cells of
+// triangles and quadrilaterals have at most 4 entries.
+// --------------------------------------------------------------------------
+constexpr int kGather = 4;
+
+__device__ __forceinline__ void cell_flux_sum(const int* __restrict__ ent, int e0, int e1,
+                                              const double* __restrict__ flux, const double* __restrict__ sr,
+                                              double& a0, double& a1, double& a2, double& a3, double& sg) {
+  const int n = min(e1 - e0, kGather);
+  int code[kGather];
+  double4 fx[kGather];
+  double s4[kGather];
+#pragma unroll
+  for (int k = 0; k < kGather; k++) code[k] = (k < n) ? ent[e0 + k] : 0;
+#pragma unroll
+  for (int k = 0; k < kGather; k++) {
+    if (k < n) {
+      fx[k] = *reinterpret_cast<const double4*>(flux + 4 * (size_t)(code[k] >> 1));
+      s4[k] = sr[code[k] >> 1];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kGather; k++) {
+    if (k < n) {
+      if (code[k] & 1) {
+        a0 += fx[k].x; a1 += fx[k].y; a2 += fx[k].z; a3 += fx[k].w;
+      } else {
+        a0 -= fx[k].x; a1 -= fx[k].y; a2 -= fx[k].z; a3 -= fx[k].w;
+      }
+      sg += s4[k];
+    }
+  }
+  for (int e = e0 + kGather; e < e1; e++) {
+    const int cd = ent[e];
+    const int p = cd >> 1;
+    const double4 f = *reinterpret_cast<const double4*>(flux + 4 * (size_t)p);
+    if (cd & 1) {
+      a0 += f.x; a1 += f.y; a2 += f.z; a3 += f.w;
+    } else {
+      a0 -= f.x; a1 -= f.y; a2 -= f.z; a3 -= f.w;
+    }
+    sg += sr[p];
+  }
+}
+
+__device__ __forceinline__ void cell_lin_sum(const int* __restrict__ ent, int e0, int e1,
+                                             const double* __restrict__ lflux, double dtfac, double& a0, double& a1,
+                                             double& a2, double& a3) {
+  const int n = min(e1 - e0, kGather);
+  int code[kGather];
+  double4 fx[kGather];
+#pragma unroll
+  for (int k = 0; k < kGather; k++) code[k] = (k < n) ? ent[e0 + k] : 0;
+#pragma unroll
+  for (int k = 0; k < kGather; k++)
+    if (k < n) fx[k] = *reinterpret_cast<const double4*>(lflux + 4 * (size_t)(code[k] >> 1));
+#pragma unroll
+  for (int k = 0; k < kGather; k++) {
+    if (k < n) {
+      if (code[k] & 1) {
+        a0 += (fx[k].x * dtfac); a1 += (fx[k].y * dtfac); a2 += (fx[k].z * dtfac); a3 += (fx[k].w * dtfac);
+      } else {
+        a0 -= (fx[k].x * dtfac); a1 -= (fx[k].y * dtfac); a2 -= (fx[k].z * dtfac); a3 -= (fx[k].w * dtfac);
+      }
+    }
+  }
+  for (int e = e0 + kGather; e < e1; e++) {
+    const int cd = ent[e];
+    const double4 f = *reinterpret_cast<const double4*>(lflux + 4 * (size_t)(cd >> 1));
+    if (cd & 1) {
+      a0 += (f.x * dtfac); a1 += (f.y * dtfac); a2 += (f.z * dtfac); a3 += (f.w * dtfac);
+    } else {
+      a0 -= (f.x * dtfac); a1 -= (f.y * dtfac); a2 -= (f.z * dtfac); a3 -= (f.w * dtfac);
+    }
+  }
+}
+
+// --------------------------------------------------------------------------
 // rhs_cells: r and sigma summed in the reference's order, norms, and (in the
 // fused iteration) ADI's scaling r *= CFL/sigma, r0 = r (ADI.cpp:40-49)
 // --------------------------------------------------------------------------
@@ -352,18 +432,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_cells(Consts c, Live lv, const i
   double sq = 0.0, mx = 0.0;
   if (i < c.nCell && (!lv.cell || lv.cell[i])) {
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, sg = 0.0;
-    const int e1 = off[i + 1];
-    for (int e = off[i]; e < e1; e++) {
-      const int code = ent[e];
-      const int p = code >> 1;
-      const double4 fx = *reinterpret_cast<const double4*>(flux + 4 * (size_t)p);
-      if (code & 1) {
-        a0 += fx.x; a1 += fx.y; a2 += fx.z; a3 += fx.w;
-      } else {
-        a0 -= fx.x; a1 -= fx.y; a2 -= fx.z; a3 -= fx.w;
-      }
-      sg += sr[p];
-    }
+    cell_flux_sum(ent, off[i], off[i + 1], flux, sr, a0, a1, a2, a3, sg);
     sq = a0 * a0;
     mx = fabs(a0);
     if (rrho) rrho[i] = a0;
@@ -420,18 +489,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_cells_explicit(Consts c, Live lv
   double sq = 0.0, mx = 0.0;
   if (i < c.nCell && (!lv.cell || lv.cell[i])) {
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, sg = 0.0;
-    const int e1 = off[i + 1];
-    for (int e = off[i]; e < e1; e++) {
-      const int code = ent[e];
-      const int p = code >> 1;
-      const double4 fx = *reinterpret_cast<const double4*>(flux + 4 * (size_t)p);
-      if (code & 1) {
-        a0 += fx.x; a1 += fx.y; a2 += fx.z; a3 += fx.w;
-      } else {
-        a0 -= fx.x; a1 -= fx.y; a2 -= fx.z; a3 -= fx.w;
-      }
-      sg += sr[p];
-    }
+    cell_flux_sum(ent, off[i], off[i + 1], flux, sr, a0, a1, a2, a3, sg);
     sq = a0 * a0;
     mx = fabs(a0);
     if (st.rrhoA) st.rrhoA[i] = a0;
@@ -1615,17 +1673,7 @@ __global__ __launch_bounds__(kBlock) void k_lin_cells(Consts c, Live lv, const i
     const double4 b = *reinterpret_cast<const double4*>(dq + 4 * (size_t)i);
     double a0 = (a.x - b.x), a1 = (a.y - b.y), a2 = (a.z - b.z), a3 = (a.w - b.w);
     const double dtfac = c.cfl / sigma[i];
-    const int e1 = off[i + 1];
-    for (int e = off[i]; e < e1; e++) {
-      const int code = ent[e];
-      const int p = code >> 1;
-      const double4 fx = *reinterpret_cast<const double4*>(lflux + 4 * (size_t)p);
-      if (code & 1) {
-        a0 += (fx.x * dtfac); a1 += (fx.y * dtfac); a2 += (fx.z * dtfac); a3 += (fx.w * dtfac);
-      } else {
-        a0 -= (fx.x * dtfac); a1 -= (fx.y * dtfac); a2 -= (fx.z * dtfac); a3 -= (fx.w * dtfac);
-      }
-    }
+    cell_lin_sum(ent, off[i], off[i + 1], lflux, dtfac, a0, a1, a2, a3);
     *reinterpret_cast<double4*>(r + 4 * (size_t)i) = make_double4(a0, a1, a2, a3);
     sq = a0 * a0;
     mx = fabs(a0);
