@@ -146,6 +146,8 @@ def main():
     ap.add_argument("--train-cases", type=int, default=384)
     ap.add_argument("--train-steps", type=int, default=20000)
     ap.add_argument("--stall-window", type=int, default=2000)
+    ap.add_argument("--heldout", type=int, default=0,
+                    help="also evaluate the warm start on this many held-out 256x128 cases (e.g. 256)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one plain 256 MiB device-to-device copy (counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -438,6 +440,8 @@ def sweep_section(args, with_cpu=True):
         fb = []
         bw, ow, iw = sweep.run_batched(specs, meshes, tol, mx, sw, lhs=hip.LHS_FP32,
                                        init=sweep.warm_init(mlp, norm, feats, fb))
+        if args.heldout > 0:
+            res["heldout_256x128"] = heldout_section(args, mlp, norm, log)
         mlp.close()
         s = {**bw.summary(), **iw}
         s["fallback_cells_first_pass"] = int(fb[0]) if fb else None
@@ -471,6 +475,35 @@ def sweep_section(args, with_cpu=True):
     if with_cpu:
         res["cpu_baseline"] = sweep_cpu_baseline(specs, meshes, cold64)
     return res
+
+
+def heldout_section(args, mlp, norm, log):
+    """The warm start's own evaluation: held-out cases on 256x128 meshes (a
+    seed of their own, the sweep's parameter ranges), batched FP32-LHS with the
+    FP64 fallback from the freestream and from the network; iterations and
+    wall-clock to L2 <= tol."""
+    from hamstr_oop_amd import hip, sweep
+    from hamstr_oop_amd import warmstart as ws
+    specs = sweep.draw_table(args.heldout, seed=sweep.SEED + 2, sizes=((256, 128),))
+    meshes = [c.mesh() for c in specs]
+    tol, mx, sw = args.tol, args.max_iters, args.stall_window
+    cold, oc, ic = sweep.run_batched(specs, meshes, tol, mx, sw, lhs=hip.LHS_FP32)
+    oc.close()
+    feats = [ws.cell_features(m, c.mach, c.alpha, c.thickness) for c, m in zip(specs, meshes)]
+    warm, ow, iw = sweep.run_batched(specs, meshes, tol, mx, sw, lhs=hip.LHS_FP32,
+                                     init=sweep.warm_init(mlp, norm, feats))
+    ow.close()
+    both = (cold.status == 1) & (warm.status == 1)
+    out = {"cases": len(specs), "seed": sweep.SEED + 2, "mesh": "256x128",
+           "cold": {**cold.summary(), **ic}, "warm": {**warm.summary(), **iw},
+           "converged_both": int(both.sum()),
+           "iterations_warm_over_cold": float(warm.iters[both].sum() / max(cold.iters[both].sum(), 1)),
+           "wallclock_warm_over_cold": warm.seconds / cold.seconds,
+           "time_to_converge_both_s": {"cold": cold.time_to_converge(list(both.nonzero()[0])),
+                                       "warm": warm.time_to_converge(list(both.nonzero()[0]))}}
+    log(f"held-out 256x128: cold {cold.seconds:.2f} s, warm {warm.seconds:.2f} s, "
+        f"iterations x{out['iterations_warm_over_cold']:.3f}")
+    return out
 
 
 def sweep_cpu_baseline(specs, meshes, cold64, iters=40):
