@@ -29,3 +29,32 @@ hipError_t hamstr_adi_long_launch(bool fp32, int final_type, unsigned grid, unsi
 #undef LONGL
   return hipGetLastError();
 }
+
+// launcher of k_adi_fast_half (FP64 left-hand sides, chains of at most 384
+// rows: one row per thread, at most 168 VGPRs); pairs != 0: k_adi_fast_half2,
+// two open chains per 768-thread workgroup
+hipError_t hamstr_adi_half_launch(int final_type, int pairs, unsigned grid, unsigned block, size_t lds,
+                                  hipStream_t stream, const void* ct_, const void* lv_, const int* chains, int nch,
+                                  const void* ff, int nProc, double cfl, const double* sigma, double* r, double* dq,
+                                  int dqmode, int ncr, void* hs, int hstride, const int* hbase) {
+  ChainTab ct;
+  Live lv;
+  std::memcpy(&ct, ct_, sizeof ct);
+  std::memcpy(&lv, lv_, sizeof lv);
+  const dim3 g(grid), b(block);
+  if (pairs) {
+    if (final_type != kFinalOpen || block != 2 * 384) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_adi_fast_half2<double, 384>), g, b, lds, stream, ct, lv, chains, nch, (const double*)ff,
+                       nProc, cfl, sigma, r, dq, dqmode, ncr, (double*)hs, hstride, hbase);
+    return hipGetLastError();
+  }
+  if (block > 384) return hipErrorInvalidValue;
+#define HALFL(F)                                                                                                \
+  hipLaunchKernelGGL((k_adi_fast_half<double, F, 384, 3>), g, b, lds, stream, ct, lv, chains, nch,              \
+                     (const double*)ff, nProc, cfl, sigma, r, dq, dqmode, ncr, (double*)hs, hstride, hbase)
+  if (final_type == kFinalOpen) HALFL(kFinalOpen);
+  else if (final_type == kFinalPair) HALFL(kFinalPair);
+  else HALFL(kFinalOdd);
+#undef HALFL
+  return hipGetLastError();
+}

@@ -20,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <limits>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -891,8 +892,8 @@ __device__ __forceinline__ void normalise_row(const T* B, T* a, T* c, T* f) {
 // LDS row layout of the FAST solve: element e of row i at lds[e*S + lpad(i)],
 // S = lds_stride(N).  One pad slot per 32 rows makes the row strides of the
 // reduction levels (powers of two up to 32) free of bank conflicts.
-__host__ __device__ __forceinline__ int lpad(int i) { return i + (i >> 5); }
-__host__ __device__ __forceinline__ int lds_stride(int N) { return N + (N >> 5) + 1; }
+constexpr __host__ __device__ __forceinline__ int lpad(int i) { return i + (i >> 5); }
+constexpr __host__ __device__ __forceinline__ int lds_stride(int N) { return N + (N >> 5) + 1; }
 
 template <typename T>
 __device__ __forceinline__ void lds_load_row(const T* lds, int N, int i, T* a, T* c, T* f) {
@@ -997,14 +998,111 @@ __device__ __forceinline__ void cr_eliminate(const T* ra, const T* rc, T* rf, co
   normalise_row(M, na, nc, rf);
 }
 
+// The same elimination with the row updated in place (A, C, F in registers):
+// M = I - A C_L - C A_R, then A <- -M^-1 A A_L, C <- -M^-1 C C_R and
+// F <- M^-1 (F - A F_L - C F_R), every element computed by the operations
+// cr_eliminate uses.  The order keeps M's factors, F, one new block and at
+// most both old blocks live (68 values), and the scheduling barriers keep the
+// neighbours' loads of one stage from going in flight during another, so the
+// kernels that use it fit in 168 VGPRs.
+// X[:, j] -= A * (column j of the LDS matrix at element offset e0, row s);
+// with has == false the column reads as zeros (row s must still be in range)
+template <typename T>
+__device__ __forceinline__ void mcol_acc_lds(T* X, const T* A, const T* src, int SS, int e0, int s, int j, bool has) {
+#pragma clang fp contract(fast)
+#pragma unroll
+  for (int l = 0; l < 4; l++) {
+    const T v = src[(e0 + 4 * l + j) * SS + s];
+    const T b = has ? v : T(0);
+#pragma unroll
+    for (int k = 0; k < 4; k++) X[4 * k + j] -= A[4 * k + l] * b;
+  }
+}
+
+// x -= A * (the F slots of LDS row s), zeros with has == false
+template <typename T>
+__device__ __forceinline__ void mv4_acc_lds_m(T* x, const T* A, const T* src, int SS, int s, bool has) {
+#pragma clang fp contract(fast)
+#pragma unroll
+  for (int l = 0; l < 4; l++) {
+    const T v = src[(32 + l) * SS + s];
+    const T b = has ? v : T(0);
+#pragma unroll
+    for (int k = 0; k < 4; k++) x[k] -= A[4 * k + l] * b;
+  }
+}
+
+// A missing neighbour (hasL / hasR false) reads row 0 of src and counts as a
+// zero block: the products then add exact zeros, as skipping them would (up
+// to the sign of a zero).  With no branch around the products the compiler
+// keeps no second copy of the row: in a branch the same code needs 30 more
+// VGPRs.
+template <typename T>
+__device__ __forceinline__ void row_reduce_lds(T* A, T* C, T* F, const T* src, int SS, bool hasL, int sL, bool hasR,
+                                               int sR) {
+#pragma clang fp contract(fast)
+  sL = hasL ? sL : 0;
+  sR = hasR ? sR : 0;
+  T M[16];
+#pragma unroll
+  for (int e = 0; e < 16; e++) M[e] = (e % 5 == 0) ? T(1) : T(0);
+#pragma unroll
+  for (int j = 0; j < 4; j += 2) {
+    mcol_acc_lds(M, A, src, SS, 16, sL, j, hasL);  // M -= A C_L
+    mcol_acc_lds(M, A, src, SS, 16, sL, j + 1, hasL);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  mv4_acc_lds_m(F, A, src, SS, sL, hasL);  // F -= A F_L
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int j = 0; j < 4; j += 2) {
+    mcol_acc_lds(M, C, src, SS, 0, sR, j, hasR);  // M -= C A_R
+    mcol_acc_lds(M, C, src, SS, 0, sR, j + 1, hasR);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  mv4_acc_lds_m(F, C, src, SS, sR, hasR);  // F -= C F_R
+  __builtin_amdgcn_sched_barrier(0);
+  const LU4<T> L = lu4_fast(M);
+  lu4_solve_fast(L, F, 1);
+  T X[16];
+#pragma unroll
+  for (int e = 0; e < 16; e++) X[e] = T(0);
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    __builtin_amdgcn_sched_barrier(0);
+    mcol_acc_lds(X, A, src, SS, 0, sL, j, hasL);  // -A A_L
+    lu4_solve_fast(L, X + j, 4);
+  }
+#pragma unroll
+  for (int e = 0; e < 16; e++) {
+    A[e] = X[e];
+    X[e] = T(0);
+  }
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    __builtin_amdgcn_sched_barrier(0);
+    mcol_acc_lds(X, C, src, SS, 16, sR, j, hasR);  // -C C_R
+    lu4_solve_fast(L, X + j, 4);
+  }
+#pragma unroll
+  for (int e = 0; e < 16; e++) C[e] = X[e];
+}
+
 // Solves the N-row normalised system held in LDS (row i at lds[e*S + lpad(i)])
 // in place; the solution ends in the F slots.  Every thread of the workgroup
 // calls it (it holds barriers); thread i works on the rows each step assigns.
-template <typename T, int FINAL>
-__device__ __forceinline__ void fast_solve_lds(T* lds, const int N, const int ncr, const int i) {
+// S = SFIX when SFIX > 0 (a stride fixed at compile time, so that every
+// element of a row is one immediate offset from one address), else
+// lds_stride(N).  The reduction levels run as for an NL-row system (NL >= N;
+// NL == N for closed chains), so that groups of threads solving open chains
+// of different lengths side by side meet the same barriers: the levels an
+// N-row system does not need leave its rows without neighbours, which only
+// renormalises them by the identity.
+template <typename T, int FINAL, int SFIX = 0>
+__device__ __forceinline__ void fast_solve_lds(T* lds, const int N, const int NL, const int ncr, const int i) {
 #pragma clang fp contract(fast)
   const bool periodic = FINAL != kFinalOpen;
-  const int S = lds_stride(N);
+  const int S = SFIX > 0 ? SFIX : lds_stride(N);
 
   // (1) ncr levels of cyclic reduction: at stride d the survivors (rows at
   // multiples of 2d) absorb rows i - d and i + d, which drop out.  The work is
@@ -1014,16 +1112,16 @@ __device__ __forceinline__ void fast_solve_lds(T* lds, const int N, const int nc
   int d = 1;  // stride between the rows still in the system
   int n = N;  // cycle length of the remaining system (closed chains)
   int ncr_done = 0;
-  while (ncr_done < ncr && (periodic ? (n % 2 == 0 && n >= 4) : (d < N))) {
+  while (ncr_done < ncr && (periodic ? (n % 2 == 0 && n >= 4) : (d < NL))) {
     const int nsurv = (N + 2 * d - 1) / (2 * d);
     if (i < nsurv) {
       const int row = i * 2 * d;
       const int iL = periodic ? (row - d + N) % N : row - d;
       const int iR = periodic ? (row + d) % N : row + d;
-      T ra[16], rc[16], rf[4], na[16], nc[16];
+      T ra[16], rc[16], rf[4];
       lds_load_row(lds, S, lpad(row), ra, rc, rf);
-      cr_eliminate(ra, rc, rf, lds, S, periodic || iL >= 0, lpad(iL), periodic || iR < N, lpad(iR), na, nc);
-      lds_store_row(lds, S, lpad(row), na, nc, rf);
+      row_reduce_lds(ra, rc, rf, lds, S, periodic || iL >= 0, lpad(iL), periodic || iR < N, lpad(iR));
+      lds_store_row(lds, S, lpad(row), ra, rc, rf);
     }
     __syncthreads();
     d *= 2;
@@ -1039,33 +1137,11 @@ __device__ __forceinline__ void fast_solve_lds(T* lds, const int N, const int nc
   T a[16], c[16], f[4];
   if (member) lds_load_row(lds, S, lpad(mrow), a, c, f);
   int dd = d;
-  while (periodic ? (n % 2 == 0 && n >= 4) : (dd < N)) {
+  while (periodic ? (n % 2 == 0 && n >= 4) : (dd < NL)) {
     if (member) {
       const int iL = periodic ? (mrow - dd + N) % N : mrow - dd;
       const int iR = periodic ? (mrow + dd) % N : mrow + dd;
-      T M[16], na[16], nc[16];
-#pragma unroll
-      for (int e = 0; e < 16; e++) {
-        M[e] = (e % 5 == 0) ? T(1) : T(0);
-        na[e] = T(0);
-        nc[e] = T(0);
-      }
-      if (periodic || iL >= 0) {
-        mm4_acc_lds(M, a, lds, S, 16, lpad(iL));
-        mm4_acc_lds(na, a, lds, S, 0, lpad(iL));
-        mv4_acc_lds(f, a, lds, S, lpad(iL));
-      }
-      if (periodic || iR < N) {
-        mm4_acc_lds(M, c, lds, S, 0, lpad(iR));
-        mm4_acc_lds(nc, c, lds, S, 16, lpad(iR));
-        mv4_acc_lds(f, c, lds, S, lpad(iR));
-      }
-      normalise_row(M, na, nc, f);
-#pragma unroll
-      for (int e = 0; e < 16; e++) {
-        a[e] = na[e];
-        c[e] = nc[e];
-      }
+      row_reduce_lds(a, c, f, lds, S, periodic || iL >= 0, lpad(iL), periodic || iR < N, lpad(iR));
     }
     __syncthreads();
     if (member) lds_store_row(lds, S, lpad(mrow), a, c, f);
@@ -1534,13 +1610,13 @@ __device__ __forceinline__ void adi_fast_chain_long(ChainTab ct, const int ch, c
     const int i = 2 * t;
     const int iL = periodic ? (i - 1 + N) % N : i - 1;
     const int iR = periodic ? (i + 1) % N : i + 1;
-    T a[16], c[16], f[4], na[16], nc[16];
+    T a[16], c[16], f[4];
     fast_row<T>(ct, ff, nProc, pb, nf, N, periodic, i, cfl, sigma, r, a, c, f);
-    cr_eliminate(a, c, f, hrow, hstride, periodic || iL >= 0, iL >> 1, periodic || iR < N, iR >> 1, na, nc);
-    lds_store_row(lds, SM, lpad(t), na, nc, f);
+    row_reduce_lds(a, c, f, hrow, hstride, periodic || iL >= 0, iL >> 1, periodic || iR < N, iR >> 1);
+    lds_store_row(lds, SM, lpad(t), a, c, f);
   }
   __syncthreads();
-  fast_solve_lds<T, FINAL>(lds, M, ncr - 1, t);
+  fast_solve_lds<T, FINAL>(lds, M, M, ncr - 1, t);
   if (2 * t + 1 < N) {
     // x = F^ - A^ x_2t - C^ x_2t+2, with the even rows' x in the F slots
     const int row = 2 * t + 1;
@@ -1555,6 +1631,173 @@ __device__ __forceinline__ void adi_fast_chain_long(ChainTab ct, const int ch, c
 #pragma unroll
     for (int e = 0; e < 4; e++) x[e] = lds[(32 + e) * SM + lpad(t)];
     store_solution(r, dq, dqmode, ct.pleft[pb + 2 * t], x);
+  }
+}
+
+// fast_row's result (row m assembled as assemble_row does and normalised by
+// its diagonal block, an open chain's end rows without the coupling past the
+// chain's ends), computed in an order that needs fewer registers: the
+// diagonal block is summed and factored first, then the couplings A = R_m
+// and C = 0 - L_(m+1) are scaled and solved a column at a time, selected
+// rather than branched on (a branch around a block costs a second copy of
+// it).  Every entry takes the operations assemble_row and normalise_row give
+// it.
+template <typename T>
+__device__ __forceinline__ void fast_row_lean(const ChainTab& ct, const T* __restrict__ ff, int nProc, int pb,
+                                              int nf, int N, bool periodic, int m, double cfl,
+                                              const double* __restrict__ sigma, const double* __restrict__ r, T* A,
+                                              T* C, T* F) {
+  // no contraction here (as in assemble_row): B's diagonal is rounded after
+  // the scaling and again after the + 1
+  const int pm = pb + m;
+  const bool has_next = periodic || m + 1 < nf;
+  const int pn = periodic ? pb + ((m + 1 == N) ? 0 : m + 1) : (has_next ? pm + 1 : pm);
+  const int rc = periodic ? 0 : ct.pright[pm];
+  const int rn = (periodic || !has_next) ? 0 : ct.pright[pn];
+  const bool end = !periodic && has_next && m + 1 == nf - 1 && rn < 0;
+  const bool hasA = periodic || (m > 0 && rc > -1);
+  const bool hasC = periodic || (has_next && !end && rn > -1 && m != N - 1);
+  T B[16];
+  {
+    T L[16], R[16];
+    load_fmat(ff, nProc, pm, L, R);
+#pragma unroll
+    for (int e = 0; e < 16; e++) B[e] = L[e];
+    if (!periodic && rc == -2) {
+      reflect_cols(R, ct.refmtx + 4 * (size_t)ct.pface[pm]);
+#pragma unroll
+      for (int e = 0; e < 16; e++) B[e] += R[e];
+    }
+  }
+  if (has_next) {
+    T L[16], R[16];
+    load_fmat(ff, nProc, pn, L, R);
+    if (periodic) {
+#pragma unroll
+      for (int e = 0; e < 16; e++) B[e] -= R[e];
+    } else if (end) {
+      if (rn == -2) reflect_cols(R, ct.refmtx + 4 * (size_t)ct.pface[pn]);
+#pragma unroll
+      for (int e = 0; e < 16; e++) B[e] += L[e];
+      if (rn == -2) {
+#pragma unroll
+        for (int e = 0; e < 16; e++) B[e] += R[e];
+      }
+    } else if (rn > -1) {
+#pragma unroll
+      for (int e = 0; e < 16; e++) B[e] -= R[e];
+    }
+  }
+  const int cell = ct.pleft[pm];
+  if (ct.od_chain && ct.od_chain[cell] == pb) {
+    const int e1 = ct.od_off[cell + 1];
+    for (int e = ct.od_off[cell]; e < e1; e++) {
+      const int code = ct.od_ent[e];
+      const int po = code >> 1;
+      if (code & 1) {
+#pragma unroll
+        for (int k = 0; k < 16; k++) B[k] += ff[(size_t)k * nProc + po];
+      } else {
+#pragma unroll
+        for (int k = 0; k < 16; k++) B[k] -= ff[(size_t)(16 + k) * nProc + po];
+      }
+    }
+  }
+  const T dtfac = T(cfl / sigma[cell]);
+#pragma unroll
+  for (int e = 0; e < 16; e++) B[e] *= dtfac;
+  B[0] += T(1.0);
+  B[5] += T(1.0);
+  B[10] += T(1.0);
+  B[15] += T(1.0);
+  const LU4<T> LU = lu4_fast(B);
+  const double4 rv = *reinterpret_cast<const double4*>(r + 4 * (size_t)cell);
+  F[0] = T(rv.x);
+  F[1] = T(rv.y);
+  F[2] = T(rv.z);
+  F[3] = T(rv.w);
+  lu4_solve_fast(LU, F, 1);
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const T v = ff[(size_t)(16 + 4 * k + j) * nProc + pm];  // R_m
+      A[4 * k + j] = (hasA ? v : T(0)) * dtfac;
+    }
+    lu4_solve_fast(LU, A + j, 4);
+  }
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const T v = ff[(size_t)(4 * k + j) * nProc + pn];  // L_(m+1)
+      C[4 * k + j] = (hasC ? T(0) - v : T(0)) * dtfac;
+    }
+    lu4_solve_fast(LU, C + j, 4);
+  }
+}
+
+// ADI phase, FAST order, one row per thread with half the rows in LDS: thread
+// t < M = ceil(N/2) holds even row 2t, thread t >= M odd row 2(t-M)+1.  The
+// odd rows, which the first cyclic-reduction level eliminates, go to LDS and
+// to global scratch `hs` (element-major, stride hstride, the chain's rows from
+// hbase[ch]); each even row absorbs its two odd neighbours in registers and
+// replaces them in LDS, as a system of M rows that fast_solve_lds finishes
+// with ncr-1 more levels; the odd rows then back-substitute from scratch.
+// Half the LDS of adi_fast_chain, in 164-166 VGPRs.  The arithmetic is that
+// of adi_fast_chain with ncr levels.
+// Arguments: t, the thread's index in the group of threads solving chain ch
+// (ch < 0: none; that group only meets the barriers), its LDS slice `lds`,
+// and NL >= N, the row count the reduction levels run for (NL == N for
+// closed chains).  Needs N <= the group's threads, open chains or closed
+// chains of an even N >= 4, ncr >= 1, and ceil(NL/2) rows within the fixed
+// LDS stride SM (SM >= lds_stride(ceil(NL/2))).
+template <typename T, int FINAL, int SM>
+__device__ __forceinline__ void adi_fast_chain_half(ChainTab ct, const int ch, const int NL, T* lds, const int t,
+                                                    const T* __restrict__ ff, int nProc, double cfl,
+                                                    const double* __restrict__ sigma, double* __restrict__ r,
+                                                    double* __restrict__ dq, int dqmode, int ncr,
+                                                    T* __restrict__ hs, int hstride, const int* __restrict__ hbase) {
+  const int pb = ch >= 0 ? ct.pbase[ch] : 0, nf = ch >= 0 ? ct.nf[ch] : 0, N = ch >= 0 ? ct.ncell[ch] : 0;
+  const bool periodic = FINAL != kFinalOpen;
+  const int M = (N + 1) / 2;
+  T* hrow = hs + (ch >= 0 ? hbase[ch] : 0);
+  const bool even = t < M;
+  const int row = even ? 2 * t : 2 * (t - M) + 1;
+  const bool active = row < N;
+  {
+    T a[16], c[16], f[4];
+    if (active) fast_row_lean<T>(ct, ff, nProc, pb, nf, N, periodic, row, cfl, sigma, r, a, c, f);
+    if (active && !even) {
+      lds_store_row(lds, SM, lpad(t - M), a, c, f);
+      lds_store_row(hrow, hstride, t - M, a, c, f);
+    }
+    __syncthreads();
+    if (active && even) {
+      // odd row 2k+1 sits in slot k
+      const int iL = periodic ? (row - 1 + N) % N : row - 1;
+      const int iR = periodic ? (row + 1) % N : row + 1;
+      row_reduce_lds(a, c, f, lds, SM, periodic || iL >= 0, lpad(iL >> 1), periodic || iR < N, lpad(iR >> 1));
+    }
+    __syncthreads();
+    if (active && even) lds_store_row(lds, SM, lpad(t), a, c, f);
+  }
+  __syncthreads();
+  fast_solve_lds<T, FINAL, SM>(lds, M, (NL + 1) / 2, ncr - 1, t);
+  if (active) {
+    T x[4];
+    if (even) {
+#pragma unroll
+      for (int e = 0; e < 4; e++) x[e] = lds[(32 + e) * SM + lpad(t)];
+    } else {
+      // x = F^ - A^ x_(row-1) - C^ x_(row+1), the even rows' x in the F slots
+      const int k = t - M;
+      T ra[16], rc[16];
+      lds_load_row(hrow, hstride, k, ra, rc, x);
+      mv4_acc_lds(x, ra, lds, SM, lpad(k));
+      if (periodic || row + 1 < N) mv4_acc_lds(x, rc, lds, SM, lpad(((row + 1) % N) >> 1));
+    }
+    store_solution(r, dq, dqmode, ct.pleft[pb + row], x);
   }
 }
 
@@ -1592,6 +1835,59 @@ __global__ __launch_bounds__(MAXT) void k_adi_fast_long(ChainTab ct, Live lv, co
     const int ch = chains[w];
     if (lv.chain && !lv.chain[ch]) continue;  // the same for the whole workgroup
     adi_fast_chain_long<T, FINAL>(ct, ch, ff, nProc, cfl, sigma, r, dq, dqmode, ncr, hs, hstride, hbase);
+    __syncthreads();
+  }
+}
+
+// the same walk with adi_fast_chain_half, one chain per workgroup; WPE:
+// minimum waves per SIMD the register allocation must leave room for (3: at
+// most 168 VGPRs).  The LDS stride is fixed at lds_stride(ceil(MAXT/2)); the
+// launch provides 36 such element rows.
+template <typename T, int FINAL, int MAXT, int WPE>
+__global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) void k_adi_fast_half(
+    ChainTab ct, Live lv, const int* __restrict__ chains, int nch, const T* __restrict__ ff, int nProc, double cfl,
+    const double* __restrict__ sigma, double* __restrict__ r, double* __restrict__ dq, int dqmode, int ncr,
+    T* __restrict__ hs, int hstride, const int* __restrict__ hbase) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T* lds = reinterpret_cast<T*>(smem_raw);
+  const int vb = xcd_block(blockIdx.x, gridDim.x);
+  for (int w = vb; w < nch; w += gridDim.x) {
+    const int ch = chains[w];
+    if (lv.chain && !lv.chain[ch]) continue;  // the same for the whole workgroup
+    adi_fast_chain_half<T, FINAL, lds_stride((MAXT + 1) / 2)>(ct, ch, ct.ncell[ch], lds, threadIdx.x, ff, nProc, cfl,
+                                                              sigma, r, dq, dqmode, ncr, hs, hstride, hbase);
+    __syncthreads();
+  }
+}
+
+// Two open chains per workgroup, each on its own MAXH threads and its own
+// LDS slice (2 x 36 element rows of stride lds_stride(ceil(MAXH/2))): chains
+// 2w and 2w+1 of the group.  With MAXH = 384 a workgroup is 12 waves, 3 per
+// SIMD, which 168 VGPRs allow; two workgroups of 6 waves each do not share a
+// CU (their waves do not pack evenly onto the 4 SIMDs), so this is how two
+// 384-row chains run on one CU.  Both halves run the reduction levels of the
+// longer chain.
+template <typename T, int MAXH>
+__global__ __launch_bounds__(2 * MAXH) __attribute__((amdgpu_waves_per_eu(3))) void k_adi_fast_half2(
+    ChainTab ct, Live lv, const int* __restrict__ chains, int nch, const T* __restrict__ ff, int nProc, double cfl,
+    const double* __restrict__ sigma, double* __restrict__ r, double* __restrict__ dq, int dqmode, int ncr,
+    T* __restrict__ hs, int hstride, const int* __restrict__ hbase) {
+  constexpr int SM = lds_stride((MAXH + 1) / 2);
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int sub = threadIdx.x >= MAXH ? 1 : 0;
+  T* lds = reinterpret_cast<T*>(smem_raw) + sub * 36 * SM;
+  const int t = threadIdx.x - sub * MAXH;
+  const int npair = (nch + 1) / 2;
+  const int vb = xcd_block(blockIdx.x, gridDim.x);
+  for (int w = vb; w < npair; w += gridDim.x) {
+    int c0 = chains[2 * w];
+    int c1 = 2 * w + 1 < nch ? chains[2 * w + 1] : -1;
+    if (lv.chain && !lv.chain[c0]) c0 = -1;
+    if (c1 >= 0 && lv.chain && !lv.chain[c1]) c1 = -1;
+    const int NL = max(c0 >= 0 ? ct.ncell[c0] : 0, c1 >= 0 ? ct.ncell[c1] : 0);
+    if (NL == 0) continue;  // the same for the whole workgroup
+    adi_fast_chain_half<T, kFinalOpen, SM>(ct, sub ? c1 : c0, NL, lds, t, ff, nProc, cfl, sigma, r, dq, dqmode, ncr,
+                                           hs, hstride, hbase);
     __syncthreads();
   }
 }
@@ -1973,6 +2269,15 @@ hipError_t hamstr_adi_long_launch(bool fp32, int final_type, unsigned grid, unsi
                                   hipStream_t stream, const void* ct, const void* lv, const int* chains, int nch,
                                   const void* ff, int nProc, double cfl, const double* sigma, double* r, double* dq,
                                   int dqmode, int ncr, void* hs, int hstride, const int* hbase);
+// launcher of k_adi_fast_half / k_adi_fast_half2 (FP64, chains of at most
+// kHalfMaxRows rows), in adi_long.hip
+hipError_t hamstr_adi_half_launch(int final_type, int pairs, unsigned grid, unsigned block, size_t lds,
+                                  hipStream_t stream, const void* ct, const void* lv, const int* chains, int nch,
+                                  const void* ff, int nProc, double cfl, const double* sigma, double* r, double* dq,
+                                  int dqmode, int ncr, void* hs, int hstride, const int* hbase);
+constexpr int kHalfMaxRows = 384;  // the workgroup bound of k_adi_fast_half, per chain of k_adi_fast_half2
+// ADI kernels of a launch group (hamstr_launch_groups' second field)
+enum { kKernExact = 0, kKernRow = 1, kKernHalf = 2, kKernTwo = 3, kKernHalf2 = 4 };
 
 // ==========================================================================
 // context
@@ -1987,6 +2292,8 @@ struct Group {
   int block = 64;      // threads per workgroup
   int ncr = 0;         // FAST: cyclic-reduction levels before the parallel cyclic reduction (autotuned)
   int two = 0;         // FAST: chains longer than a workgroup, two rows per thread (adi_fast_chain_long)
+  int half_ok = 0;     // FAST, FP64: every chain fits adi_fast_chain_half (scratch rows reserved)
+  int kern = 0;        // FAST one-row kernel: kKernRow, kKernHalf or (open chains) kKernHalf2
 };
 
 struct hamstr_ctx {
@@ -2154,14 +2461,24 @@ int launch_group(hamstr_ctx* x, const Group& g, int grid, int dqmode, int ncr) {
   const int* chains = x->group_chains + g.off;
   if (g.fast) {
     const dim3 gr(grid), b(g.block);
-    // long chains always run the first reduction level (it is what halves them)
-    const int nc = g.two ? std::max(ncr, 1) : ncr;
+    // long chains always run the first reduction level (it is what halves
+    // them), and so does the half-LDS kernel
+    const bool half = g.kern == kKernHalf || g.kern == kKernHalf2;
+    const int nc = (g.two || half) ? std::max(ncr, 1) : ncr;
     const size_t lds = (x->lhs32() ? sizeof(float) : sizeof(double)) * 36 *
-                       (size_t)lds_stride(g.two ? (g.maxN + 1) / 2 : g.maxN);
+                       (half ? (g.kern == kKernHalf2 ? 2 : 1) * (size_t)lds_stride((kHalfMaxRows + 1) / 2)
+                             : (size_t)lds_stride(g.two ? (g.maxN + 1) / 2 : g.maxN));
 #define FASTT(T, F, M)                                                                                            \
   hipLaunchKernelGGL((k_adi_fast<T, F, M>), gr, b, lds, x->stream, ct, x->live(), chains, g.n, (const T*)x->ff,   \
                      x->nProc, x->c.cfl, x->sigma, x->r, x->dq, dqmode, nc)
-    if (g.two) {
+    if (half) {
+      const Live lv = x->live();
+      const bool pairs = g.kern == kKernHalf2;
+      const int gw = pairs ? std::min(grid, (g.n + 1) / 2) : grid;  // workgroups of two chains
+      HIPCHK(hamstr_adi_half_launch(g.final, pairs, (unsigned)gw, pairs ? 2u * kHalfMaxRows : (unsigned)g.block, lds,
+                                    x->stream, &ct, &lv, chains, g.n, x->ff, x->nProc, x->c.cfl, x->sigma, x->r,
+                                    x->dq, dqmode, nc, x->long_scratch, x->nLong, x->ch_hbase));
+    } else if (g.two) {
       const Live lv = x->live();
       HIPCHK(hamstr_adi_long_launch(x->lhs32(), g.final, (unsigned)grid, (unsigned)g.block, lds, x->stream, &ct, &lv,
                                     chains, g.n, x->ff, x->nProc, x->c.cfl, x->sigma, x->r, x->dq, dqmode, nc,
@@ -2432,14 +2749,36 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
   }
   const int nChainProc = (int)pslot.size();
   // ---- orphan faces (extension for meshes with triangles): interior faces in
-  // no chain, processed after every chain face in list order ----
+  // no chain, processed after every chain face.  Their processed-face slots
+  // follow the ADI rows that take their diagonal blocks (case, then chain,
+  // then row), so that a chain's rows read them from consecutive addresses;
+  // every per-cell sum still takes them in list order (cell_ent, od_ent). ----
   if (mesh->nOrphan < 0 || (mesh->nOrphan > 0 && !mesh->orphans))
     return fail(HAMSTR_ERR_ARG, "bad orphan face list");
-  for (int o = 0; o < mesh->nOrphan; o++) {
+  std::vector<int> orph_order(mesh->nOrphan), orph_p(mesh->nOrphan);
+  {
+    std::vector<long> cell_rank(nCell, std::numeric_limits<long>::max());
+    long rank = 0;
+    for (int i = 0; i < nChain; i++)  // the last chain listing a cell takes its blocks
+      for (int k = 0; k < ch_ncell[i]; k++) cell_rank[pleft[ch_pbase[i] + k]] = rank++;
+    std::vector<long> key(mesh->nOrphan);
+    for (int o = 0; o < mesh->nOrphan; o++) {
+      const int iface = mesh->orphans[o];
+      if (iface < 0 || iface >= nFace) return fail(HAMSTR_ERR_MESH, "orphan face out of range");
+      if (faces[6 * iface + 4] < 0) return fail(HAMSTR_ERR_MESH, "orphan faces must be interior faces");
+      key[o] = std::min(cell_rank[faces[6 * iface + 2]], cell_rank[faces[6 * iface + 4]]);
+      orph_order[o] = o;
+    }
+    std::stable_sort(orph_order.begin(), orph_order.end(), [&](int a, int b) {
+      const int ka = cell_case[faces[6 * mesh->orphans[a] + 2]], kb = cell_case[faces[6 * mesh->orphans[b] + 2]];
+      return ka != kb ? ka < kb : key[a] < key[b];
+    });
+  }
+  for (int n = 0; n < mesh->nOrphan; n++) {
+    const int o = orph_order[n];
     const int iface = mesh->orphans[o];
-    if (iface < 0 || iface >= nFace) return fail(HAMSTR_ERR_MESH, "orphan face out of range");
-    if (faces[6 * iface + 4] < 0) return fail(HAMSTR_ERR_MESH, "orphan faces must be interior faces");
     if (++proc_count[iface] > 1) return fail(HAMSTR_ERR_MESH, "orphan face also on a chain or listed twice");
+    orph_p[o] = (int)pslot.size();
     pslot.push_back((int)src.size() | kOrphanSlot);  // the two cells, as plain copies
     src.push_back(make_int2(faces[6 * iface + 2], -1));
     src.push_back(make_int2(faces[6 * iface + 4], -1));
@@ -2462,7 +2801,8 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
   for (int i = 0; i < nCell; i++) cnt[i + 1] += cnt[i];
   std::vector<int> cell_ent(cnt[nCell]);
   std::vector<int> fill(cnt.begin(), cnt.end() - 1);
-  for (int p = 0; p < nProc; p++) {
+  for (int n = 0; n < nProc; n++) {  // chain faces, then the orphans in list order
+    const int p = n < nChainProc ? n : orph_p[n - nChainProc];
     cell_ent[fill[pleft[p]]++] = (p << 1) | 0;
     if (pright[p] > -1) cell_ent[fill[pright[p]]++] = (p << 1) | 1;
   }
@@ -2480,7 +2820,8 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
     for (int i = 0; i < nCell; i++) od_off[i + 1] += od_off[i];
     od_ent.assign(od_off[nCell], 0);
     std::vector<int> at(od_off.begin(), od_off.end() - 1);
-    for (int p = nChainProc; p < nProc; p++) {
+    for (int o = 0; o < nProc - nChainProc; o++) {  // list order
+      const int p = orph_p[o];
       od_ent[at[pleft[p]]++] = (p << 1) | 1;
       od_ent[at[pright[p]]++] = (p << 1);
     }
@@ -2539,7 +2880,6 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
       }
       bucket[kind].push_back(i);
     }
-    if (nLong >= (1L << 31) / 36) return fail(HAMSTR_ERR_MESH, "too many rows in long chains");
     for (int b = 0; b < 7; b++) {
       if (bucket[b].empty()) continue;
       std::stable_sort(bucket[b].begin(), bucket[b].end(), [&](int a, int c) { return ch_ncell[a] > ch_ncell[c]; });
@@ -2558,6 +2898,23 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
       // two levels of cyclic reduction before the PCR (one before an odd
       // cycle); measured best or close to it on single meshes and batches
       g.ncr = g.fast ? (kind == 3 ? 1 : 2) : 0;
+      if (g.fast && !g.two && tl == sizeof(double) && g.maxN <= kHalfMaxRows && kind != 3) {
+        // the half-LDS kernel: closed chains need an even length of at least
+        // 4; odd cycles stay with the one-row kernel (their final cyclic
+        // Thomas solve would spill at 168 VGPRs)
+        bool ok = true;
+        for (int i : bucket[b]) ok = ok && (!ch_periodic[i] || (ch_ncell[i] % 2 == 0 && ch_ncell[i] >= 4));
+        if (ok) {
+          g.half_ok = 1;
+          // two open chains per workgroup where one chain's waves would not
+          // pack onto the SIMDs twice (over 256 rows), one chain elsewhere
+          g.kern = (kind == 1 && g.maxN > 256) ? kKernHalf2 : kKernHalf;
+          for (int i : bucket[b]) {
+            ch_hbase[i] = (int)nLong;
+            nLong += ch_ncell[i] / 2;
+          }
+        }
+      }
       for (int i : bucket[b]) group_chains.push_back(i);
       groups.push_back(g);
       maxGroupChains = std::max(maxGroupChains, (size_t)g.n);
@@ -2565,6 +2922,8 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
     }
   }
 
+  // scratch rows (long and half-LDS chains) are indexed with int, 36 per row
+  if (nLong >= (1L << 31) / 36) return fail(HAMSTR_ERR_MESH, "too many rows in long chains");
   hamstr_ctx* x = new hamstr_ctx();
   x->prm = P;
   x->nCell = nCell;
@@ -3431,6 +3790,19 @@ int hamstr_set_launch(hamstr_ctx* x, int group, int grid, int cr_levels) {
   return HAMSTR_OK;
 }
 
+int hamstr_set_adi_kernel(hamstr_ctx* x, int group, int kernel) {
+  if (!x || group < 0 || group >= (int)x->groups.size()) return fail(HAMSTR_ERR_ARG, "bad launch group");
+  Group& g = x->groups[group];
+  if (!g.fast || g.two) return fail(HAMSTR_ERR_ARG, "the group has one ADI kernel only");
+  if (kernel != kKernRow && kernel != kKernHalf && kernel != kKernHalf2) return fail(HAMSTR_ERR_ARG, "unknown ADI kernel");
+  if (kernel != kKernRow && !g.half_ok) return fail(HAMSTR_ERR_ARG, "the group's chains do not fit the half-LDS kernels");
+  if (kernel == kKernHalf2 && g.final != kFinalOpen)
+    return fail(HAMSTR_ERR_ARG, "two chains per workgroup need open chains");
+  g.kern = kernel;
+  drop_graph(x);
+  return HAMSTR_OK;
+}
+
 int hamstr_prepare_graph(hamstr_ctx* x, int check_every) {
   if (!x || check_every < 1) return fail(HAMSTR_ERR_ARG, "bad arguments");
   const int K = graph_block(check_every);
@@ -3552,24 +3924,34 @@ int hamstr_autotune(hamstr_ctx* x, int reps, double* best_ms, int cap, int* ngro
       std::vector<int> cand = {g.n};
       for (int w : {8, 4, 2, 1})
         if (w * cu < g.n) cand.push_back(w * cu);
-      for (int ncr = 0; ncr <= kMaxCR; ncr++)
-        for (int grid : cand) {
-          int rc = launch_group(x, g, grid, 0, ncr);  // warm-up
-          if (rc) return rc;
-          HIPCHK(hipEventRecord(e0, x->stream));
-          for (int i = 0; i < reps; i++)
-            if ((rc = launch_group(x, g, grid, 0, ncr))) return rc;
-          HIPCHK(hipEventRecord(e1, x->stream));
-          HIPCHK(hipEventSynchronize(e1));
-          float ms = 0;
-          HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-          const double t = ms / reps;
-          if (best < 0 || t < best) {
-            best = t;
-            g.grid = grid;
-            g.ncr = ncr;
+      std::vector<int> kerns = {kKernRow};
+      if (g.half_ok) kerns.push_back(kKernHalf);
+      if (g.half_ok && g.final == kFinalOpen) kerns.push_back(kKernHalf2);
+      int best_kern = g.kern;
+      for (int kern : kerns) {
+        g.kern = kern;
+        // the half-LDS kernels always run the first reduction level
+        for (int ncr = kern == kKernRow ? 0 : 1; ncr <= kMaxCR; ncr++)
+          for (int grid : cand) {
+            int rc = launch_group(x, g, grid, 0, ncr);  // warm-up
+            if (rc) return rc;
+            HIPCHK(hipEventRecord(e0, x->stream));
+            for (int i = 0; i < reps; i++)
+              if ((rc = launch_group(x, g, grid, 0, ncr))) return rc;
+            HIPCHK(hipEventRecord(e1, x->stream));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            const double t = ms / reps;
+            if (best < 0 || t < best) {
+              best = t;
+              g.grid = grid;
+              g.ncr = ncr;
+              best_kern = kern;
+            }
           }
-        }
+      }
+      g.kern = best_kern;
     }
     if (best_ms && k < cap) best_ms[k] = best;
     k++;
@@ -3592,7 +3974,9 @@ int hamstr_launch_groups(const hamstr_ctx* x, int* out8, int cap, int* n) {
   for (const Group& g : x->groups) {
     if (out8 && k < cap) {
       int* o = out8 + 8 * k;
-      o[0] = g.phase; o[1] = g.fast; o[2] = g.final; o[3] = g.n; o[4] = g.maxN; o[5] = g.grid; o[6] = g.block;
+      o[0] = g.phase;
+      o[1] = !g.fast ? kKernExact : (g.two ? kKernTwo : (g.kern ? g.kern : kKernRow));
+      o[2] = g.final; o[3] = g.n; o[4] = g.maxN; o[5] = g.grid; o[6] = g.block;
       o[7] = g.ncr;
     }
     k++;

@@ -20,6 +20,8 @@ LIB_PATH = os.path.join(HERE, "libhamstr_hip.so")
 OK, ERR_ARG, ERR_HIP, ERR_NAN, ERR_MESH, ERR_NODEV = range(6)
 LHS_FP64, LHS_FP32 = 0, 1
 ADI_EXACT, ADI_FAST = 0, 1
+# ADI kernels of a launch group (HAMSTR_KERNEL_*)
+KERNEL_EXACT, KERNEL_ROW, KERNEL_HALF, KERNEL_TWO, KERNEL_HALF2 = range(5)
 FIELDS = {"q": 0, "r": 1, "r0": 2, "dq": 3, "sigma": 4, "ff": 5, "qt": 6}
 # SOLVER::stepType values (the input deck's scheme= line)
 SCHEMES = {"ADI": 0, "euler": 1, "rk3": 2}
@@ -34,7 +36,7 @@ EXPORTS = [
     "hamstr_last_error", "hamstr_batch_create", "hamstr_batch_info", "hamstr_batch_step", "hamstr_autotune",
     "hamstr_launch_groups", "hamstr_case_freestream", "hamstr_set_retire", "hamstr_case_status",
     "hamstr_set_graph", "hamstr_get_cells", "hamstr_set_cells", "hamstr_prepare_graph",
-    "hamstr_set_launch", "hamstr_set_case_live", "hamstr_iterations",
+    "hamstr_set_launch", "hamstr_set_adi_kernel", "hamstr_set_case_live", "hamstr_iterations",
     "hamstr_mlp_create", "hamstr_mlp_destroy", "hamstr_mlp_num_params", "hamstr_mlp_get_params",
     "hamstr_mlp_set_params", "hamstr_mlp_forward", "hamstr_mlp_loss_grad", "hamstr_mlp_set_data",
     "hamstr_mlp_train", "hamstr_warm_start", "hamstr_mlp_last_error", "hamstr_mlp_set_graph",
@@ -429,9 +431,20 @@ def set_launch(solver: Solver, group: int, grid: int, cr_levels: int):
     _chk(lib().hamstr_set_launch(solver.h, int(group), int(grid), int(cr_levels)))
 
 
+def set_adi_kernel(solver: Solver, group: int, kernel: int):
+    """ADI kernel of a FAST launch group: KERNEL_ROW, or KERNEL_HALF /
+    KERNEL_HALF2 (open chains) where the group's chains fit them."""
+    _chk(lib().hamstr_set_adi_kernel(solver.h, int(group), int(kernel)))
+
+
 def launch_groups(solver: Solver):
+    """Launch groups as dicts: phase, kernel (KERNEL_*; "fast" is true for the
+    FAST kernels), final solve, chains, max_rows, grid, block, cr_levels."""
     out = np.zeros(8 * 64, np.int32)
     n = C.c_int()
     _chk(lib().hamstr_launch_groups(solver.h, _i(out), 64, C.byref(n)))
-    keys = ("phase", "fast", "final", "chains", "max_rows", "grid", "block", "cr_levels")
-    return [dict(zip(keys, map(int, out[8 * k:8 * k + 8]))) for k in range(n.value)]
+    keys = ("phase", "kernel", "final", "chains", "max_rows", "grid", "block", "cr_levels")
+    groups = [dict(zip(keys, map(int, out[8 * k:8 * k + 8]))) for k in range(n.value)]
+    for g in groups:
+        g["fast"] = int(g["kernel"] != KERNEL_EXACT)
+    return groups

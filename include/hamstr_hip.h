@@ -155,18 +155,38 @@ int hamstr_set_case_live(hamstr_ctx *ctx, int k, int live);
 long hamstr_iterations(const hamstr_ctx *ctx);
 
 /* Autotuner: times every FAST launch group of the ADI phases at several
- * persistent grid sizes (all chains; 8, 4, 2, 1 workgroups per CU) and 0 to 3
- * levels of cyclic reduction before the parallel cyclic reduction, on the
- * current state; keeps the fastest and restores r and dq.  best_ms[k]
- * (k < cap) gets group k's best time (-1 for groups with nothing to tune). */
+ * persistent grid sizes (all chains; 8, 4, 2, 1 workgroups per CU), 0 to 3
+ * levels of cyclic reduction before the parallel cyclic reduction and, where
+ * the group's chains fit it, both one-row-per-thread kernels (all rows in
+ * LDS, or half of them: HAMSTR_KERNEL_*), on the current state; keeps the
+ * fastest and restores r and dq.  best_ms[k] (k < cap) gets group k's best
+ * time (-1 for groups with nothing to tune). */
 int hamstr_autotune(hamstr_ctx *ctx, int reps, double *best_ms, int cap, int *ngroups);
-/* Launch groups, 8 ints each: phase, fast, final solve (0 open, 1 two-row
- * pairs, 2 odd cyclic), chains, max rows, grid, block, cyclic-reduction
- * levels. */
+/* ADI kernels of a launch group */
+#define HAMSTR_KERNEL_EXACT 0 /* one thread per chain, the reference's order */
+#define HAMSTR_KERNEL_ROW 1   /* FAST: one thread per row, every row in LDS */
+#define HAMSTR_KERNEL_HALF 2  /* FAST, FP64: one thread per row, half the rows in LDS
+                                 (the rest in global scratch), at most 168 VGPRs:
+                                 two 384-row chains per CU */
+#define HAMSTR_KERNEL_TWO 3   /* FAST: chains longer than a workgroup, two rows per thread */
+#define HAMSTR_KERNEL_HALF2 4 /* FAST, FP64, open chains: HAMSTR_KERNEL_HALF for two chains
+                                 per 768-thread workgroup (12 waves, 3 per SIMD) */
+/* Launch groups, 8 ints each: phase, ADI kernel (HAMSTR_KERNEL_*), final
+ * solve (0 open, 1 two-row pairs, 2 odd cyclic), chains, max rows, grid,
+ * block, cyclic-reduction levels. */
 int hamstr_launch_groups(const hamstr_ctx *ctx, int *out8, int cap, int *n);
 /* Sets FAST launch group `group`'s persistent grid and its cyclic-reduction
  * levels (0..3) by hand, as the autotuner would. */
 int hamstr_set_launch(hamstr_ctx *ctx, int group, int grid, int cr_levels);
+/* Chooses a FAST launch group's kernel by hand: HAMSTR_KERNEL_ROW, or
+ * HAMSTR_KERNEL_HALF where the group's chains fit it (FP64 left-hand side, at
+ * most 384 rows, open chains or closed chains whose final solve is two-row
+ * cycles; it always runs at least one cyclic-reduction level), or
+ * HAMSTR_KERNEL_HALF2 for such open chains.  All give the same solution up to
+ * the sign of zero entries.  FP64 contexts start with HAMSTR_KERNEL_HALF2 for
+ * open chains of more than 256 rows and HAMSTR_KERNEL_HALF for the others
+ * that fit. */
+int hamstr_set_adi_kernel(hamstr_ctx *ctx, int group, int kernel);
 
 /* SOLVER::initFlow (initFlow.cpp:12-93): q := freestream. */
 int hamstr_init_flow(hamstr_ctx *ctx);
