@@ -944,6 +944,116 @@ __device__ __forceinline__ void mv4_acc_lds(T* x, const T* A, const T* lds, int 
 
 enum { kFinalOpen = 0, kFinalPair = 1, kFinalOdd = 2 };
 
+// lane J's value of the calling lane's quad (lanes 4g..4g+3), by DPP
+// quad_perm; the whole quad must be active
+template <int J>
+__device__ __forceinline__ float quad_get(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), J * 0x55, 0xF, 0xF, false));
+}
+template <int J>
+__device__ __forceinline__ double quad_get(double v) {
+  const long long u = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(u & 0xffffffffLL), J * 0x55, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(u >> 32), J * 0x55, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// One parallel-cyclic-reduction step of the normalised row in LDS slot sO,
+// spread over a quad of lanes: lane q computes column q of the new blocks.
+// M = I - A C_L - C A_R (column q here, the full M gathered from the quad),
+// F' = M^-1 (F - A F_L - C F_R) (every lane), A'[:, q] = -M^-1 A A_L[:, q],
+// C'[:, q] = -M^-1 C C_R[:, q]: every entry takes the operations
+// row_reduce_lds gives it.  A missing neighbour reads slot 0 as zeros.  The
+// caller writes (Ac, Cc, F[q]) back after a barrier.
+template <typename T>
+__device__ __forceinline__ void pcr_quad_row(const T* lds, int S, int sO, bool hasL, int sL, bool hasR, int sR,
+                                             int q, T* Ac, T* Cc, T* F) {
+#pragma clang fp contract(fast)
+  sL = hasL ? sL : 0;
+  sR = hasR ? sR : 0;
+  T A[16], C[16];
+  lds_load_row(lds, S, sO, A, C, F);
+  T m[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) m[k] = (k == q) ? T(1) : T(0);
+#pragma unroll
+  for (int l = 0; l < 4; l++) {
+    const T v = lds[(16 + 4 * l + q) * S + sL];  // C_L[l][q]
+    const T b = hasL ? v : T(0);
+#pragma unroll
+    for (int k = 0; k < 4; k++) m[k] -= A[4 * k + l] * b;
+  }
+#pragma unroll
+  for (int l = 0; l < 4; l++) {
+    const T v = lds[(32 + l) * S + sL];
+    const T b = hasL ? v : T(0);
+#pragma unroll
+    for (int k = 0; k < 4; k++) F[k] -= A[4 * k + l] * b;
+  }
+#pragma unroll
+  for (int l = 0; l < 4; l++) {
+    const T v = lds[(4 * l + q) * S + sR];  // A_R[l][q]
+    const T b = hasR ? v : T(0);
+#pragma unroll
+    for (int k = 0; k < 4; k++) m[k] -= C[4 * k + l] * b;
+  }
+#pragma unroll
+  for (int l = 0; l < 4; l++) {
+    const T v = lds[(32 + l) * S + sR];
+    const T b = hasR ? v : T(0);
+#pragma unroll
+    for (int k = 0; k < 4; k++) F[k] -= C[4 * k + l] * b;
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  T M[16];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    M[4 * k + 0] = quad_get<0>(m[k]);
+    M[4 * k + 1] = quad_get<1>(m[k]);
+    M[4 * k + 2] = quad_get<2>(m[k]);
+    M[4 * k + 3] = quad_get<3>(m[k]);
+  }
+  const LU4<T> L = lu4_fast(M);
+  lu4_solve_fast(L, F, 1);
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int k = 0; k < 4; k++) Ac[k] = T(0);
+#pragma unroll
+  for (int l = 0; l < 4; l++) {
+    const T v = lds[(4 * l + q) * S + sL];  // A_L[l][q]
+    const T b = hasL ? v : T(0);
+#pragma unroll
+    for (int k = 0; k < 4; k++) Ac[k] -= A[4 * k + l] * b;
+  }
+  lu4_solve_fast(L, Ac, 1);
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int k = 0; k < 4; k++) Cc[k] = T(0);
+#pragma unroll
+  for (int l = 0; l < 4; l++) {
+    const T v = lds[(16 + 4 * l + q) * S + sR];  // C_R[l][q]
+    const T b = hasR ? v : T(0);
+#pragma unroll
+    for (int k = 0; k < 4; k++) Cc[k] -= C[4 * k + l] * b;
+  }
+  lu4_solve_fast(L, Cc, 1);
+}
+
+// the quad's results into row slot sO: column q of A' and C', and F'[q]
+template <typename T>
+__device__ __forceinline__ void pcr_quad_store(T* lds, int S, int sO, int q, const T* Ac, const T* Cc, const T* F) {
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    lds[(4 * k + q) * S + sO] = Ac[k];
+    lds[(16 + 4 * k + q) * S + sO] = Cc[k];
+  }
+  // one predicated store per entry (a select chain on q turns into a
+  // dynamically indexed private array)
+#pragma unroll
+  for (int k = 0; k < 4; k++)
+    if (q == k) lds[(32 + k) * S + sO] = F[k];
+}
+
 #ifdef HAMSTR_LONG_TU
 // The long-chain path (chains longer than a workgroup) is compiled in a
 // translation unit of its own, adi_long.hip, which includes this file with
@@ -1097,9 +1207,13 @@ __device__ __forceinline__ void row_reduce_lds(T* A, T* C, T* F, const T* src, i
 // NL == N for closed chains), so that groups of threads solving open chains
 // of different lengths side by side meet the same barriers: the levels an
 // N-row system does not need leave its rows without neighbours, which only
-// renormalises them by the identity.
-template <typename T, int FINAL, int SFIX = 0>
-__device__ __forceinline__ void fast_solve_lds(T* lds, const int N, const int NL, const int ncr, const int i) {
+// renormalises them by the identity.  QUAD: run the parallel cyclic
+// reduction by quads of lanes where the threads allow it (it pays where the
+// SIMDs would otherwise idle; the two-chain and two-row kernels keep them
+// busy, and it would cost them registers).
+template <typename T, int FINAL, int SFIX = 0, bool QUAD = true>
+__device__ __forceinline__ void fast_solve_lds(T* lds, const int N, const int NL, const int ncr, const int i,
+                                               const int NT) {
 #pragma clang fp contract(fast)
   const bool periodic = FINAL != kFinalOpen;
   const int S = SFIX > 0 ? SFIX : lds_stride(N);
@@ -1129,14 +1243,32 @@ __device__ __forceinline__ void fast_solve_lds(T* lds, const int N, const int NL
     ncr_done++;
   }
 
-  // (2) parallel cyclic reduction of the rows at multiples of d (members),
-  // thread t holding member row t*d in registers
+  // (2) parallel cyclic reduction of the rows at multiples of d (members):
+  // by quads of lanes (pcr_quad_row) when QUAD and the NT threads give four
+  // per member (a uniform choice: nmem follows from NL), else thread t
+  // holding member row t*d in registers
   const int nmem = (N + d - 1) / d;
   const bool member = i < nmem;
   const int mrow = i * d;
   T a[16], c[16], f[4];
-  if (member) lds_load_row(lds, S, lpad(mrow), a, c, f);
   int dd = d;
+  if (QUAD && 4 * ((NL + d - 1) / d) <= NT) {
+    const int g = i >> 2, q = i & 3, grow = g * d;
+    while (periodic ? (n % 2 == 0 && n >= 4) : (dd < NL)) {
+      T Ac[4], Cc[4], F[4];
+      if (g < nmem) {
+        const int iL = periodic ? (grow - dd + N) % N : grow - dd;
+        const int iR = periodic ? (grow + dd) % N : grow + dd;
+        pcr_quad_row(lds, S, lpad(grow), periodic || iL >= 0, lpad(iL), periodic || iR < N, lpad(iR), q, Ac, Cc, F);
+      }
+      __syncthreads();
+      if (g < nmem) pcr_quad_store(lds, S, lpad(grow), q, Ac, Cc, F);
+      __syncthreads();
+      dd *= 2;
+      n /= 2;
+    }
+  }
+  if (member) lds_load_row(lds, S, lpad(mrow), a, c, f);
   while (periodic ? (n % 2 == 0 && n >= 4) : (dd < NL)) {
     if (member) {
       const int iL = periodic ? (mrow - dd + N) % N : mrow - dd;
@@ -1378,14 +1510,32 @@ __device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const 
     ncr_done++;
   }
 
-  // (2) parallel cyclic reduction of the rows at multiples of d (members),
-  // thread t holding member row t*d in registers
+  // (2) parallel cyclic reduction of the rows at multiples of d (members):
+  // by quads of lanes (pcr_quad_row) when the workgroup has four threads per
+  // member, else thread t holding member row t*d in registers.  Not for odd
+  // cycles, whose final solve leaves no registers for it.
   const int nmem = (N + d - 1) / d;
   const bool member = i < nmem;
   const int mrow = i * d;
   T a[16], c[16], f[4];
-  if (member) lds_load_row(lds, S, lpad(mrow), a, c, f);
   int dd = d;
+  if (FINAL != kFinalOdd && 4 * nmem <= (int)blockDim.x) {
+    const int g = i >> 2, q = i & 3, grow = g * d;
+    while (periodic ? (n % 2 == 0 && n >= 4) : (dd < N)) {
+      T Ac[4], Cc[4], F[4];
+      if (g < nmem) {
+        const int iL = periodic ? (grow - dd + N) % N : grow - dd;
+        const int iR = periodic ? (grow + dd) % N : grow + dd;
+        pcr_quad_row(lds, S, lpad(grow), periodic || iL >= 0, lpad(iL), periodic || iR < N, lpad(iR), q, Ac, Cc, F);
+      }
+      __syncthreads();
+      if (g < nmem) pcr_quad_store(lds, S, lpad(grow), q, Ac, Cc, F);
+      __syncthreads();
+      dd *= 2;
+      n /= 2;
+    }
+  }
+  if (member) lds_load_row(lds, S, lpad(mrow), a, c, f);
   while (periodic ? (n % 2 == 0 && n >= 4) : (dd < N)) {
     if (member) {
       const int iL = periodic ? (mrow - dd + N) % N : mrow - dd;
@@ -1616,7 +1766,7 @@ __device__ __forceinline__ void adi_fast_chain_long(ChainTab ct, const int ch, c
     lds_store_row(lds, SM, lpad(t), a, c, f);
   }
   __syncthreads();
-  fast_solve_lds<T, FINAL>(lds, M, M, ncr - 1, t);
+  fast_solve_lds<T, FINAL, 0, false>(lds, M, M, ncr - 1, t, (int)blockDim.x);
   if (2 * t + 1 < N) {
     // x = F^ - A^ x_2t - C^ x_2t+2, with the even rows' x in the F slots
     const int row = 2 * t + 1;
@@ -1746,14 +1896,15 @@ __device__ __forceinline__ void fast_row_lean(const ChainTab& ct, const T* __res
 // with ncr-1 more levels; the odd rows then back-substitute from scratch.
 // Half the LDS of adi_fast_chain, in 164-166 VGPRs.  The arithmetic is that
 // of adi_fast_chain with ncr levels.
-// Arguments: t, the thread's index in the group of threads solving chain ch
-// (ch < 0: none; that group only meets the barriers), its LDS slice `lds`,
+// Arguments: t, the thread's index in the group of NT threads solving chain
+// ch (ch < 0: none; that group only meets the barriers), its LDS slice `lds`,
 // and NL >= N, the row count the reduction levels run for (NL == N for
 // closed chains).  Needs N <= the group's threads, open chains or closed
 // chains of an even N >= 4, ncr >= 1, and ceil(NL/2) rows within the fixed
 // LDS stride SM (SM >= lds_stride(ceil(NL/2))).
 template <typename T, int FINAL, int SM>
 __device__ __forceinline__ void adi_fast_chain_half(ChainTab ct, const int ch, const int NL, T* lds, const int t,
+                                                    const int NT,
                                                     const T* __restrict__ ff, int nProc, double cfl,
                                                     const double* __restrict__ sigma, double* __restrict__ r,
                                                     double* __restrict__ dq, int dqmode, int ncr,
@@ -1783,7 +1934,7 @@ __device__ __forceinline__ void adi_fast_chain_half(ChainTab ct, const int ch, c
     if (active && even) lds_store_row(lds, SM, lpad(t), a, c, f);
   }
   __syncthreads();
-  fast_solve_lds<T, FINAL, SM>(lds, M, (NL + 1) / 2, ncr - 1, t);
+  fast_solve_lds<T, FINAL, SM, false>(lds, M, (NL + 1) / 2, ncr - 1, t, NT);
   if (active) {
     T x[4];
     if (even) {
@@ -1807,7 +1958,9 @@ __device__ __forceinline__ void adi_fast_chain_half(ChainTab ct, const int ch, c
 // stride of the grid size (a persistent launch: chains are sorted longest
 // first, so every pass of the grid does similar work)
 template <typename T, int FINAL, int MAXT>
-__global__ __launch_bounds__(MAXT) void k_adi_fast(ChainTab ct, Live lv, const int* __restrict__ chains, int nch,
+__global__ __launch_bounds__(MAXT)
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && MAXT == 512 && FINAL != kFinalOdd ? 4 : 1)))
+void k_adi_fast(ChainTab ct, Live lv, const int* __restrict__ chains, int nch,
                                                    const T* __restrict__ ff, int nProc, double cfl,
                                                    const double* __restrict__ sigma, double* __restrict__ r,
                                                    double* __restrict__ dq, int dqmode, int ncr) {
@@ -1854,8 +2007,9 @@ __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) voi
   for (int w = vb; w < nch; w += gridDim.x) {
     const int ch = chains[w];
     if (lv.chain && !lv.chain[ch]) continue;  // the same for the whole workgroup
-    adi_fast_chain_half<T, FINAL, lds_stride((MAXT + 1) / 2)>(ct, ch, ct.ncell[ch], lds, threadIdx.x, ff, nProc, cfl,
-                                                              sigma, r, dq, dqmode, ncr, hs, hstride, hbase);
+    adi_fast_chain_half<T, FINAL, lds_stride((MAXT + 1) / 2)>(ct, ch, ct.ncell[ch], lds, threadIdx.x, (int)blockDim.x,
+                                                              ff, nProc, cfl, sigma, r, dq, dqmode, ncr, hs, hstride,
+                                                              hbase);
     __syncthreads();
   }
 }
@@ -1886,8 +2040,8 @@ __global__ __launch_bounds__(2 * MAXH) __attribute__((amdgpu_waves_per_eu(3))) v
     if (c1 >= 0 && lv.chain && !lv.chain[c1]) c1 = -1;
     const int NL = max(c0 >= 0 ? ct.ncell[c0] : 0, c1 >= 0 ? ct.ncell[c1] : 0);
     if (NL == 0) continue;  // the same for the whole workgroup
-    adi_fast_chain_half<T, kFinalOpen, SM>(ct, sub ? c1 : c0, NL, lds, t, ff, nProc, cfl, sigma, r, dq, dqmode, ncr,
-                                           hs, hstride, hbase);
+    adi_fast_chain_half<T, kFinalOpen, SM>(ct, sub ? c1 : c0, NL, lds, t, MAXH, ff, nProc, cfl, sigma, r, dq, dqmode,
+                                           ncr, hs, hstride, hbase);
     __syncthreads();
   }
 }
@@ -2907,8 +3061,9 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
         if (ok) {
           g.half_ok = 1;
           // two open chains per workgroup where one chain's waves would not
-          // pack onto the SIMDs twice (over 256 rows), one chain elsewhere
-          g.kern = (kind == 1 && g.maxN > 256) ? kKernHalf2 : kKernHalf;
+          // pack onto the SIMDs twice (over 256 rows); elsewhere the one-row
+          // kernel with its quad-lane PCR is faster
+          g.kern = (kind == 1 && g.maxN > 256) ? kKernHalf2 : kKernRow;
           for (int i : bucket[b]) {
             ch_hbase[i] = (int)nLong;
             nLong += ch_ncell[i] / 2;

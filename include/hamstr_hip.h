@@ -184,8 +184,7 @@ int hamstr_set_launch(hamstr_ctx *ctx, int group, int grid, int cr_levels);
  * cycles; it always runs at least one cyclic-reduction level), or
  * HAMSTR_KERNEL_HALF2 for such open chains.  All give the same solution up to
  * the sign of zero entries.  FP64 contexts start with HAMSTR_KERNEL_HALF2 for
- * open chains of more than 256 rows and HAMSTR_KERNEL_HALF for the others
- * that fit. */
+ * open chains of more than 256 rows and HAMSTR_KERNEL_ROW elsewhere. */
 int hamstr_set_adi_kernel(hamstr_ctx *ctx, int group, int kernel);
 
 /* SOLVER::initFlow (initFlow.cpp:12-93): q := freestream. */

@@ -1,11 +1,12 @@
 """The half-LDS FAST ADI kernels (HAMSTR_KERNEL_HALF: one row per thread, the
 rows the first cyclic-reduction level eliminates in global scratch;
 HAMSTR_KERNEL_HALF2: the same for two open chains per workgroup) against the
-one-row kernel with every row in LDS (HAMSTR_KERNEL_ROW): the same operations
-in the same order, so the states after a run of iterations are equal (up to
-the sign of zero entries, which numpy's comparison ignores), for open lines,
-two-row-cycle loops, odd chain lengths, an odd number of chains, batches and
-every cyclic-reduction depth."""
+one-row kernel with every row in LDS (HAMSTR_KERNEL_ROW, whose parallel cyclic
+reduction runs by quads of lanes where the workgroup allows): the same
+operations in the same order, so the states after a run of iterations are
+equal (up to the sign of zero entries, which numpy's comparison ignores), for
+open lines, two-row-cycle loops, odd chain lengths, an odd number of chains,
+batches and every cyclic-reduction depth."""
 import numpy as np
 import pytest
 
@@ -24,12 +25,15 @@ def _kernels(hip):
 
 
 def _set(hip, s, k, g, kernel):
-    """`kernel` on a group that starts with a half-LDS kernel (two chains per
-    workgroup on open groups only)"""
-    if g["kernel"] in (hip.KERNEL_HALF, hip.KERNEL_HALF2):
+    """`kernel` on a one-row group whose chains fit the half-LDS kernels (two
+    chains per workgroup on open groups only)"""
+    if g["kernel"] in (hip.KERNEL_ROW, hip.KERNEL_HALF, hip.KERNEL_HALF2):
         if kernel == hip.KERNEL_HALF2 and g["final"] != 0:
             kernel = hip.KERNEL_HALF
-        hip.set_adi_kernel(s, k, kernel)
+        try:
+            hip.set_adi_kernel(s, k, kernel)
+        except hip.HamstrError:
+            assert kernel != hip.KERNEL_ROW  # the group's chains do not fit the half-LDS kernels
 
 
 def _run(hip, s, kernel, ncr, steps):
@@ -50,14 +54,17 @@ def _eq(a, b, what):
 
 
 def test_default_groups(hip):
-    """FP64 contexts pick the half-LDS kernels for open lines and two-row-cycle
-    loops of up to 384 rows (two lines per workgroup above 256 rows); FP32
-    contexts and longer chains keep theirs."""
+    """FP64 contexts pick two lines per workgroup (half-LDS) for open lines of
+    257 to 384 rows, the one-row kernel elsewhere; the half-LDS kernels stay
+    available where the chains fit them (two-row-cycle loops and open lines of
+    up to 384 rows); FP32 contexts and longer chains keep theirs."""
     from hamstr_oop_amd import meshgen
     mesh = meshgen.naca_ogrid(256, 128)  # loops of 256 rows, lines of 128
     s = hip.Solver(mesh, 0.5, 2.0, 3, 10.0, adi_mode=hip.ADI_FAST)
     ks = {g["max_rows"]: g["kernel"] for g in hip.launch_groups(s)}
-    assert ks == {256: hip.KERNEL_HALF, 128: hip.KERNEL_HALF}, ks
+    assert ks == {256: hip.KERNEL_ROW, 128: hip.KERNEL_ROW}, ks
+    for k in range(2):
+        hip.set_adi_kernel(s, k, hip.KERNEL_HALF)
     loops = [g["max_rows"] for g in hip.launch_groups(s)].index(256)
     with pytest.raises(hip.HamstrError):
         hip.set_adi_kernel(s, loops, hip.KERNEL_HALF2)
