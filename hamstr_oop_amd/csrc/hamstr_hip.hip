@@ -149,6 +149,28 @@ __device__ __forceinline__ void put4(double* d, const double4 v) {
   d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
 }
 
+// The face Jacobians from (n, left, right), behind an empty asm barrier on
+// the inputs: the compiler can share no subexpression with the code around
+// the call (k_rhs_faces' flux), so its multiply-add fusion under FASTJ is the
+// same wherever it is called, and k_lin_faces can compute again, bit for bit,
+// the Jacobians k_rhs_faces stored.
+template <typename TL, bool FASTJ>
+__device__ __forceinline__ void face_jacobians(double nx, double ny, const double* left, const double* right,
+                                               double gamma, TL* lm, TL* rm) {
+  double l[4], r[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    l[k] = left[k];
+    r[k] = right[k];
+    asm volatile("" : "+v"(l[k]), "+v"(r[k]));
+  }
+  asm volatile("" : "+v"(nx), "+v"(ny));
+  if (FASTJ)
+    jacobian_roe2d_fast<TL>(nx, ny, l, r, gamma, lm, rm);
+  else
+    jacobian_roe2d<TL>(nx, ny, l, r, gamma, lm, rm);
+}
+
 // ql[n] and qr[n+1] along the chain for one face (reconstruction.cpp)
 template <int ORDER>
 __device__ __forceinline__ void reconstruct_face(const double (*f)[4], double eps, double qt, double* qln,
@@ -237,10 +259,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_faces(FaceTab ft, Consts c, Live
   if (s_r != s_r) atomicOr(nanflag, 1);
   if (IMPL) {
     TL lm[16], rm[16];
-    if (FASTJ)
-      jacobian_roe2d_fast<TL>(n.x, n.y, left, right, c.gamma, lm, rm);
-    else
-      jacobian_roe2d<TL>(n.x, n.y, left, right, c.gamma, lm, rm);
+    face_jacobians<TL, FASTJ>(n.x, n.y, left, right, c.gamma, lm, rm);
 #pragma unroll
     for (int e = 0; e < 16; e++) {
       ff[(size_t)e * c.nProc + p] = lm[e];
@@ -2051,7 +2070,11 @@ __global__ __launch_bounds__(2 * MAXH) __attribute__((amdgpu_waves_per_eu(3))) v
 // lin_faces: reconstruction derivatives + lmat*dqL + rmat*dqR per face
 // (computeLinearRHS.cpp:52-325)
 // --------------------------------------------------------------------------
-template <int ORDER, typename TL, bool FASTJ>
+// RECOMP: the face's Jacobians are computed again here, by the code and from
+// the states k_rhs_faces used (the same q), instead of read back from ff: 1.1
+// kflop per face in place of 32 loads.  Only for the fused iteration, where ff
+// is k_rhs_faces' output for the current q.
+template <int ORDER, typename TL, bool FASTJ, bool RECOMP>
 __global__ __launch_bounds__(kBlock) void k_lin_faces(FaceTab ft, Consts c, Live lv, const double* __restrict__ q,
                                                       const double* __restrict__ dq, const TL* __restrict__ ff,
                                                       double* __restrict__ lflux) {
@@ -2060,10 +2083,15 @@ __global__ __launch_bounds__(kBlock) void k_lin_faces(FaceTab ft, Consts c, Live
   const int code = ft.pslot[p];
   const int s = code & kSlotMask;
   double dl[4], dr[4];
+  double left[4], right[4];  // the face states (RECOMP)
   if (code & kOrphanSlot) {
     // orphan face: the full product of the two cells' dq
     put4(dl, load_dstate(ft.src[s], dq, ft.refmtx));
     put4(dr, load_dstate(ft.src[s + 1], dq, ft.refmtx));
+    if (RECOMP) {
+      put4(left, load_state(ft.src[s], q, ft.refmtx, ft.qinf));
+      put4(right, load_state(ft.src[s + 1], q, ft.refmtx, ft.qinf));
+    }
   } else {
     const bool flip = (code >> 30) & 1;
     constexpr int lo = (ORDER == 5) ? 0 : (ORDER == 3 ? 1 : 2);
@@ -2072,7 +2100,7 @@ __global__ __launch_bounds__(kBlock) void k_lin_faces(FaceTab ft, Consts c, Live
 #pragma unroll
     for (int j = lo; j <= hi; j++) {
       const int2 e = ft.src[s - 2 + j];
-      if (ORDER != 1) put4(f[j], load_state(e, q, ft.refmtx, ft.qinf));
+      if (ORDER != 1 || RECOMP) put4(f[j], load_state(e, q, ft.refmtx, ft.qinf));
       put4(d[j], load_dstate(e, dq, ft.refmtx));
     }
     double dln[4], drn1[4];
@@ -2082,6 +2110,36 @@ __global__ __launch_bounds__(kBlock) void k_lin_faces(FaceTab ft, Consts c, Live
       dl[k] = flip ? dln[k] : drn1[k];
       dr[k] = flip ? drn1[k] : dln[k];
     }
+    if (RECOMP) {
+      // the states from opaque copies of the stencil: shared with the
+      // derivative above, common subexpressions would change how the
+      // derivative's multiply-adds fuse (FASTJ) from k_lin_faces without RECOMP
+      double fb[6][4], qln[4], qrn1[4];
+#pragma unroll
+      for (int j = lo; j <= hi; j++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          fb[j][k] = f[j][k];
+          asm volatile("" : "+v"(fb[j][k]));
+        }
+      reconstruct_face<ORDER>(fb, c.eps, c.qt, qln, qrn1);
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        left[k] = flip ? qln[k] : qrn1[k];
+        right[k] = flip ? qrn1[k] : qln[k];
+      }
+    }
+  }
+  TL lmat[16], rmat[16];
+  if (RECOMP) {
+    const double2 n = *reinterpret_cast<const double2*>(ft.nvec + 2 * (size_t)p);
+    face_jacobians<TL, FASTJ>(n.x, n.y, left, right, c.gamma, lmat, rmat);
+  } else {
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+      lmat[e] = ff[(size_t)e * c.nProc + p];
+      rmat[e] = ff[(size_t)(16 + e) * c.nProc + p];
+    }
   }
   double out[4];
 #pragma unroll
@@ -2089,8 +2147,8 @@ __global__ __launch_bounds__(kBlock) void k_lin_faces(FaceTab ft, Consts c, Live
     double acc = 0;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      const double lm = double(ff[(size_t)(4 * j + k) * c.nProc + p]);
-      const double rm = double(ff[(size_t)(16 + 4 * j + k) * c.nProc + p]);
+      const double lm = double(lmat[4 * j + k]);
+      const double rm = double(rmat[4 * j + k]);
       acc += ((lm * dl[k]) + (rm * dr[k]));
     }
     out[j] = acc;
@@ -2562,20 +2620,26 @@ int launch_rhs_faces(hamstr_ctx* x, bool implicit) {
   return HAMSTR_OK;
 }
 
-int launch_lin_faces(hamstr_ctx* x) {
+// recompute: the Jacobians from q (the fused iteration, where ff is
+// k_rhs_faces' output for the current q), else read from ff
+int launch_lin_faces(hamstr_ctx* x, bool recompute = false) {
   const FaceTab ft = x->ftab();
   dim3 g(nblk(x->nProc)), b(kBlock);
-#define LINF(O, T, F)                                                                                         \
-  hipLaunchKernelGGL((k_lin_faces<O, T, F>), g, b, 0, x->stream, ft, x->c, x->live(), x->q, x->dq, (const T*)x->ff, \
-                     x->lflux)
+#define LINF(O, T, F)                                                                                            \
+  if (recompute)                                                                                                 \
+    hipLaunchKernelGGL((k_lin_faces<O, T, F, true>), g, b, 0, x->stream, ft, x->c, x->live(), x->q, x->dq,       \
+                       (const T*)x->ff, x->lflux);                                                               \
+  else                                                                                                           \
+    hipLaunchKernelGGL((k_lin_faces<O, T, F, false>), g, b, 0, x->stream, ft, x->c, x->live(), x->q, x->dq,      \
+                       (const T*)x->ff, x->lflux)
 #define LINO(T, F) \
-  if (o == 1) LINF(1, T, F); else if (o == 3) LINF(3, T, F); else LINF(5, T, F)
+  if (o == 1) { LINF(1, T, F); } else if (o == 3) { LINF(3, T, F); } else { LINF(5, T, F); }
   const int o = x->prm.order;
   const bool fj = x->prm.adi_mode == HAMSTR_ADI_FAST;
   if (x->lhs32()) {
-    if (fj) LINO(float, true); else LINO(float, false);
+    if (fj) { LINO(float, true); } else { LINO(float, false); }
   } else {
-    if (fj) LINO(double, true); else LINO(double, false);
+    if (fj) { LINO(double, true); } else { LINO(double, false); }
   }
 #undef LINO
 #undef LINF
@@ -3693,7 +3757,9 @@ int enqueue_adi_iteration(hamstr_ctx* x, double* nb, const std::function<int(con
   if (mark && mark("rhs_cells")) return HAMSTR_ERR_HIP;
   for (int sw = 0; sw < x->prm.msweep; sw++) {
     if ((rc = launch_sweep_phases(x, sw, mark))) return rc;
-    if ((rc = launch_lin_faces(x))) return rc;
+    // FP64: the Jacobians computed again beat reading 256 B per face back;
+    // FP32 ones (128 B) are as fast to read
+    if ((rc = launch_lin_faces(x, !x->lhs32()))) return rc;
     if (mark && mark("lin_faces")) return HAMSTR_ERR_HIP;
     if ((rc = launch_lin_cells(x, sw == x->prm.msweep - 1, nb, p_lin))) return rc;
     if (mark && mark("lin_cells")) return HAMSTR_ERR_HIP;

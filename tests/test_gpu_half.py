@@ -146,3 +146,25 @@ def test_half2_mixed_pairs(hip, retire):
         b.close()
     for k in range(2):
         _eq(out[hip.KERNEL_HALF2][k], out[hip.KERNEL_ROW][k], f"case {k}")
+
+
+@pytest.mark.parametrize("lhs", ["fp64", "fp32"])
+@pytest.mark.parametrize("order", [1, 3, 5])
+def test_fused_step_equals_api_calls(hip, order, lhs):
+    """The fused iteration's linear residual computes each face's Jacobians
+    again from q instead of reading the stored ones: FAST iterations through
+    hamstr_step equal computeRHS("implicit") + ADI() through the API (which
+    reads them), bit for bit, on a mesh with orphan faces."""
+    from hamstr_oop_amd import meshgen
+    mesh = meshgen.naca_ogrid(64, 24, tri_layers=8)
+    lm = hip.LHS_FP64 if lhs == "fp64" else hip.LHS_FP32
+    a = hip.Solver(mesh, 0.5, 2.0, order, 10.0, lhs=lm, adi_mode=hip.ADI_FAST)
+    b = hip.Solver(mesh, 0.5, 2.0, order, 10.0, lhs=lm, adi_mode=hip.ADI_FAST)
+    a.step(5, check_every=5)
+    for _ in range(5):
+        b.computeRHS("implicit")
+        b.ADI()
+    _eq(a.get("q"), b.get("q"), "q after 5 iterations")
+    _eq(a.get("r"), b.get("r"), "linear residual")
+    a.close()
+    b.close()
