@@ -1468,7 +1468,9 @@ __device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const 
   const bool active = i < N;
   const int S = lds_stride(N);
 
-  // every row, assembled and normalised, goes to LDS
+  // every row, assembled and normalised, goes to LDS.  (Passing the faces
+  // through LDS, as the half-LDS kernels do, made this kernel slower: here
+  // the second read of a face hits in cache.)
   if (active) {
     T a[16], c[16], f[4], B[16];
     int cell;
@@ -1906,6 +1908,130 @@ __device__ __forceinline__ void fast_row_lean(const ChainTab& ct, const T* __res
   }
 }
 
+// Row m assembled as assemble_row does and normalised by its diagonal block
+// (an open chain's end rows without the coupling past the chain's ends), for
+// a chain whose rows are all assembled at once by one group of threads, each row loading only its own face m (L_m,
+// R_m): face m+1, which row m also needs, comes through LDS (`xl`, element e
+// of face j at xl[e * XS + j], 16 entries, XS > N; L first, then R), so that
+// no face is read from memory twice.  Row m holds A (= R_m scaled) and C (= 0 -
+// L_(m+1) scaled) before the diagonal block is factored.  Every thread of the
+// group calls it (it holds barriers); inactive threads pass active = false.
+// An open chain's last face (no row owns it) is read by the last row.  The
+// entries take the operations assemble_row and normalise_row give them.
+constexpr int kXStride = 385;  // the half-LDS kernels' XS (chains of at most 384 rows)
+template <typename T>
+__device__ __forceinline__ void fast_row_xchg(const ChainTab& ct, const T* __restrict__ ff, int nProc, int pb,
+                                              int nf, int N, bool periodic, int m, bool active, double cfl,
+                                              const double* __restrict__ sigma, const double* __restrict__ r,
+                                              T* xl, const int XS, T* A, T* C, T* F) {
+  const int pm = pb + m;
+  const bool has_next = active && (periodic || m + 1 < nf);
+  const int mn = periodic ? ((m + 1 == N) ? 0 : m + 1) : m + 1;  // the next face, as a row index
+  const bool next_owned = periodic || m + 1 < N;                 // else the chain's last face
+  const int pn = pb + mn;
+  const int rc = (periodic || !active) ? 0 : ct.pright[pm];
+  const int rn = (periodic || !has_next) ? 0 : ct.pright[pn];
+  const bool end = !periodic && has_next && m + 1 == nf - 1 && rn < 0;
+  const bool hasA = periodic || (m > 0 && rc > -1);
+  const bool hasC = periodic || (has_next && !end && rn > -1 && m != N - 1);
+  T B[16];
+  // L: own face into LDS, the next face's back
+  if (active) {
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+      B[e] = ff[(size_t)e * nProc + pm];
+      xl[e * XS + m] = B[e];
+    }
+  }
+  __syncthreads();
+  if (has_next) {
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+      const T v = next_owned ? xl[e * XS + mn] : ff[(size_t)e * nProc + pn];
+      C[e] = v;  // L_(m+1)
+    }
+  }
+  __syncthreads();
+  // R: own face into LDS (A, and the wall case of B), the next face's back
+  if (active) {
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+      A[e] = ff[(size_t)(16 + e) * nProc + pm];
+      xl[e * XS + m] = A[e];
+    }
+    if (!periodic && rc == -2) {
+      T R[16];
+#pragma unroll
+      for (int e = 0; e < 16; e++) R[e] = A[e];
+      reflect_cols(R, ct.refmtx + 4 * (size_t)ct.pface[pm]);
+#pragma unroll
+      for (int e = 0; e < 16; e++) B[e] += R[e];
+    }
+  }
+  __syncthreads();
+  if (has_next) {
+    T R[16];
+#pragma unroll
+    for (int e = 0; e < 16; e++) R[e] = next_owned ? xl[e * XS + mn] : ff[(size_t)(16 + e) * nProc + pn];
+    if (periodic) {
+#pragma unroll
+      for (int e = 0; e < 16; e++) B[e] -= R[e];
+    } else if (end) {
+      if (rn == -2) reflect_cols(R, ct.refmtx + 4 * (size_t)ct.pface[pn]);
+#pragma unroll
+      for (int e = 0; e < 16; e++) B[e] += C[e];  // L_(m+1)
+      if (rn == -2) {
+#pragma unroll
+        for (int e = 0; e < 16; e++) B[e] += R[e];
+      }
+    } else if (rn > -1) {
+#pragma unroll
+      for (int e = 0; e < 16; e++) B[e] -= R[e];
+    }
+  }
+  __syncthreads();  // the LDS slice is free again
+  if (!active) return;
+  const int cell = ct.pleft[pm];
+  if (ct.od_chain && ct.od_chain[cell] == pb) {
+    const int e1 = ct.od_off[cell + 1];
+    for (int e = ct.od_off[cell]; e < e1; e++) {
+      const int code = ct.od_ent[e];
+      const int po = code >> 1;
+      if (code & 1) {
+#pragma unroll
+        for (int k = 0; k < 16; k++) B[k] += ff[(size_t)k * nProc + po];
+      } else {
+#pragma unroll
+        for (int k = 0; k < 16; k++) B[k] -= ff[(size_t)(16 + k) * nProc + po];
+      }
+    }
+  }
+  const T dtfac = T(cfl / sigma[cell]);
+#pragma unroll
+  for (int e = 0; e < 16; e++) B[e] *= dtfac;
+  B[0] += T(1.0);
+  B[5] += T(1.0);
+  B[10] += T(1.0);
+  B[15] += T(1.0);
+  const LU4<T> LU = lu4_fast(B);
+  const double4 rv = *reinterpret_cast<const double4*>(r + 4 * (size_t)cell);
+  F[0] = T(rv.x);
+  F[1] = T(rv.y);
+  F[2] = T(rv.z);
+  F[3] = T(rv.w);
+  lu4_solve_fast(LU, F, 1);
+#pragma unroll
+  for (int e = 0; e < 16; e++) {
+    A[e] = (hasA ? A[e] : T(0)) * dtfac;
+    C[e] = (hasC ? T(0) - C[e] : T(0)) * dtfac;
+  }
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    lu4_solve_fast(LU, A + j, 4);
+    lu4_solve_fast(LU, C + j, 4);
+  }
+}
+
 // ADI phase, FAST order, one row per thread with half the rows in LDS: thread
 // t < M = ceil(N/2) holds even row 2t, thread t >= M odd row 2(t-M)+1.  The
 // odd rows, which the first cyclic-reduction level eliminates, go to LDS and
@@ -1937,7 +2063,7 @@ __device__ __forceinline__ void adi_fast_chain_half(ChainTab ct, const int ch, c
   const bool active = row < N;
   {
     T a[16], c[16], f[4];
-    if (active) fast_row_lean<T>(ct, ff, nProc, pb, nf, N, periodic, row, cfl, sigma, r, a, c, f);
+    fast_row_xchg<T>(ct, ff, nProc, pb, nf, N, periodic, row, active, cfl, sigma, r, lds, kXStride, a, c, f);
     if (active && !even) {
       lds_store_row(lds, SM, lpad(t - M), a, c, f);
       lds_store_row(hrow, hstride, t - M, a, c, f);
@@ -2020,6 +2146,7 @@ __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) voi
     ChainTab ct, Live lv, const int* __restrict__ chains, int nch, const T* __restrict__ ff, int nProc, double cfl,
     const double* __restrict__ sigma, double* __restrict__ r, double* __restrict__ dq, int dqmode, int ncr,
     T* __restrict__ hs, int hstride, const int* __restrict__ hbase) {
+  static_assert(MAXT < kXStride && 16 * kXStride <= 36 * lds_stride((MAXT + 1) / 2), "face exchange fits the slice");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* lds = reinterpret_cast<T*>(smem_raw);
   const int vb = xcd_block(blockIdx.x, gridDim.x);
@@ -2046,6 +2173,7 @@ __global__ __launch_bounds__(2 * MAXH) __attribute__((amdgpu_waves_per_eu(3))) v
     const double* __restrict__ sigma, double* __restrict__ r, double* __restrict__ dq, int dqmode, int ncr,
     T* __restrict__ hs, int hstride, const int* __restrict__ hbase) {
   constexpr int SM = lds_stride((MAXH + 1) / 2);
+  static_assert(MAXH < kXStride && 16 * kXStride <= 36 * SM, "face exchange fits the slice");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int sub = threadIdx.x >= MAXH ? 1 : 0;
   T* lds = reinterpret_cast<T*>(smem_raw) + sub * 36 * SM;
