@@ -30,10 +30,10 @@ hipError_t hamstr_adi_long_launch(bool fp32, int final_type, unsigned grid, unsi
   return hipGetLastError();
 }
 
-// launcher of k_adi_fast_half (FP64 left-hand sides, chains of at most 384
-// rows: one row per thread, at most 168 VGPRs); pairs != 0: k_adi_fast_half2,
-// two open chains per 768-thread workgroup
-hipError_t hamstr_adi_half_launch(int final_type, int pairs, unsigned grid, unsigned block, size_t lds,
+// launcher of k_adi_fast_half (FP64) / k_adi_fast_half2 (FP64 or FP32, pairs
+// != 0: two open chains per 768-thread workgroup); chains of at most 384 rows,
+// one row per thread, at most 168 VGPRs
+hipError_t hamstr_adi_half_launch(bool fp32, int final_type, int pairs, unsigned grid, unsigned block, size_t lds,
                                   hipStream_t stream, const void* ct_, const void* lv_, const int* chains, int nch,
                                   const void* ff, int nProc, double cfl, const double* sigma, double* r, double* dq,
                                   int dqmode, int ncr, void* hs, int hstride, const int* hbase) {
@@ -44,11 +44,15 @@ hipError_t hamstr_adi_half_launch(int final_type, int pairs, unsigned grid, unsi
   const dim3 g(grid), b(block);
   if (pairs) {
     if (final_type != kFinalOpen || block != 2 * 384) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_adi_fast_half2<double, 384>), g, b, lds, stream, ct, lv, chains, nch, (const double*)ff,
-                       nProc, cfl, sigma, r, dq, dqmode, ncr, (double*)hs, hstride, hbase);
+    if (fp32)
+      hipLaunchKernelGGL((k_adi_fast_half2<float, 384>), g, b, lds, stream, ct, lv, chains, nch, (const float*)ff,
+                         nProc, cfl, sigma, r, dq, dqmode, ncr, (float*)hs, hstride, hbase);
+    else
+      hipLaunchKernelGGL((k_adi_fast_half2<double, 384>), g, b, lds, stream, ct, lv, chains, nch, (const double*)ff,
+                         nProc, cfl, sigma, r, dq, dqmode, ncr, (double*)hs, hstride, hbase);
     return hipGetLastError();
   }
-  if (block > 384) return hipErrorInvalidValue;
+  if (fp32 || block > 384) return hipErrorInvalidValue;
 #define HALFL(F)                                                                                                \
   hipLaunchKernelGGL((k_adi_fast_half<double, F, 384, 3>), g, b, lds, stream, ct, lv, chains, nch,              \
                      (const double*)ff, nProc, cfl, sigma, r, dq, dqmode, ncr, (double*)hs, hstride, hbase)

@@ -2611,7 +2611,7 @@ hipError_t hamstr_adi_long_launch(bool fp32, int final_type, unsigned grid, unsi
                                   int dqmode, int ncr, void* hs, int hstride, const int* hbase);
 // launcher of k_adi_fast_half / k_adi_fast_half2 (FP64, chains of at most
 // kHalfMaxRows rows), in adi_long.hip
-hipError_t hamstr_adi_half_launch(int final_type, int pairs, unsigned grid, unsigned block, size_t lds,
+hipError_t hamstr_adi_half_launch(bool fp32, int final_type, int pairs, unsigned grid, unsigned block, size_t lds,
                                   hipStream_t stream, const void* ct, const void* lv, const int* chains, int nch,
                                   const void* ff, int nProc, double cfl, const double* sigma, double* r, double* dq,
                                   int dqmode, int ncr, void* hs, int hstride, const int* hbase);
@@ -2633,6 +2633,7 @@ struct Group {
   int ncr = 0;         // FAST: cyclic-reduction levels before the parallel cyclic reduction (autotuned)
   int two = 0;         // FAST: chains longer than a workgroup, two rows per thread (adi_fast_chain_long)
   int half_ok = 0;     // FAST, FP64: every chain fits adi_fast_chain_half (scratch rows reserved)
+  int half2_ok = 0;    // FAST, open chains that fit k_adi_fast_half2 (FP64 or FP32; scratch rows reserved)
   int kern = 0;        // FAST one-row kernel: kKernRow, kKernHalf or (open chains) kKernHalf2
 };
 
@@ -2821,7 +2822,7 @@ int launch_group(hamstr_ctx* x, const Group& g, int grid, int dqmode, int ncr) {
       const Live lv = x->live();
       const bool pairs = g.kern == kKernHalf2;
       const int gw = pairs ? std::min(grid, (g.n + 1) / 2) : grid;  // workgroups of two chains
-      HIPCHK(hamstr_adi_half_launch(g.final, pairs, (unsigned)gw, pairs ? 2u * kHalfMaxRows : (unsigned)g.block, lds,
+      HIPCHK(hamstr_adi_half_launch(x->lhs32(), g.final, pairs, (unsigned)gw, pairs ? 2u * kHalfMaxRows : (unsigned)g.block, lds,
                                     x->stream, &ct, &lv, chains, g.n, x->ff, x->nProc, x->c.cfl, x->sigma, x->r,
                                     x->dq, dqmode, nc, x->long_scratch, x->nLong, x->ch_hbase));
     } else if (g.two) {
@@ -3244,18 +3245,21 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
       // two levels of cyclic reduction before the PCR (one before an odd
       // cycle); measured best or close to it on single meshes and batches
       g.ncr = g.fast ? (kind == 3 ? 1 : 2) : 0;
-      if (g.fast && !g.two && tl == sizeof(double) && g.maxN <= kHalfMaxRows && kind != 3) {
-        // the half-LDS kernel: closed chains need an even length of at least
-        // 4; odd cycles stay with the one-row kernel (their final cyclic
-        // Thomas solve would spill at 168 VGPRs)
+      if (g.fast && !g.two && g.maxN <= kHalfMaxRows && kind != 3) {
+        // the half-LDS kernels: FP64 for one chain per workgroup (closed
+        // chains of an even length of at least 4; odd cycles stay with the
+        // one-row kernel, whose final cyclic Thomas solve would spill at 168
+        // VGPRs), FP64 or FP32 open chains for two per workgroup
         bool ok = true;
         for (int i : bucket[b]) ok = ok && (!ch_periodic[i] || (ch_ncell[i] % 2 == 0 && ch_ncell[i] >= 4));
-        if (ok) {
-          g.half_ok = 1;
-          // two open chains per workgroup where one chain's waves would not
-          // pack onto the SIMDs twice (over 256 rows); elsewhere the one-row
-          // kernel with its quad-lane PCR is faster
-          g.kern = (kind == 1 && g.maxN > 256) ? kKernHalf2 : kKernRow;
+        g.half_ok = ok && tl == sizeof(double);
+        g.half2_ok = kind == 1;
+        if (g.half_ok || g.half2_ok) {
+          // FP64: two open chains per workgroup where one chain's waves
+          // would not pack onto the SIMDs twice (over 256 rows); elsewhere,
+          // and for FP32 (whose one-row workgroups already pair up on a CU),
+          // the one-row kernel with its quad-lane PCR is faster
+          g.kern = (g.half2_ok && g.maxN > 256 && tl == sizeof(double)) ? kKernHalf2 : kKernRow;
           for (int i : bucket[b]) {
             ch_hbase[i] = (int)nLong;
             nLong += ch_ncell[i] / 2;
@@ -4144,9 +4148,9 @@ int hamstr_set_adi_kernel(hamstr_ctx* x, int group, int kernel) {
   Group& g = x->groups[group];
   if (!g.fast || g.two) return fail(HAMSTR_ERR_ARG, "the group has one ADI kernel only");
   if (kernel != kKernRow && kernel != kKernHalf && kernel != kKernHalf2) return fail(HAMSTR_ERR_ARG, "unknown ADI kernel");
-  if (kernel != kKernRow && !g.half_ok) return fail(HAMSTR_ERR_ARG, "the group's chains do not fit the half-LDS kernels");
-  if (kernel == kKernHalf2 && g.final != kFinalOpen)
-    return fail(HAMSTR_ERR_ARG, "two chains per workgroup need open chains");
+  if ((kernel == kKernHalf && !g.half_ok) || (kernel == kKernHalf2 && !g.half2_ok))
+    return fail(HAMSTR_ERR_ARG, "the group's chains do not fit that half-LDS kernel (FP64 only for one chain per "
+                                "workgroup, open chains only for two)");
   g.kern = kernel;
   drop_graph(x);
   return HAMSTR_OK;
@@ -4275,7 +4279,7 @@ int hamstr_autotune(hamstr_ctx* x, int reps, double* best_ms, int cap, int* ngro
         if (w * cu < g.n) cand.push_back(w * cu);
       std::vector<int> kerns = {kKernRow};
       if (g.half_ok) kerns.push_back(kKernHalf);
-      if (g.half_ok && g.final == kFinalOpen) kerns.push_back(kKernHalf2);
+      if (g.half2_ok) kerns.push_back(kKernHalf2);
       int best_kern = g.kern;
       for (int kern : kerns) {
         g.kern = kern;

@@ -169,8 +169,8 @@ int hamstr_autotune(hamstr_ctx *ctx, int reps, double *best_ms, int cap, int *ng
                                  (the rest in global scratch), at most 168 VGPRs:
                                  two 384-row chains per CU */
 #define HAMSTR_KERNEL_TWO 3   /* FAST: chains longer than a workgroup, two rows per thread */
-#define HAMSTR_KERNEL_HALF2 4 /* FAST, FP64, open chains: HAMSTR_KERNEL_HALF for two chains
-                                 per 768-thread workgroup (12 waves, 3 per SIMD) */
+#define HAMSTR_KERNEL_HALF2 4 /* FAST, open chains (FP64 or FP32): half the rows in LDS, two
+                                 chains per 768-thread workgroup (12 waves, 3 per SIMD) */
 /* Launch groups, 8 ints each: phase, ADI kernel (HAMSTR_KERNEL_*), final
  * solve (0 open, 1 two-row pairs, 2 odd cyclic), chains, max rows, grid,
  * block, cyclic-reduction levels. */
@@ -182,9 +182,10 @@ int hamstr_set_launch(hamstr_ctx *ctx, int group, int grid, int cr_levels);
  * HAMSTR_KERNEL_HALF where the group's chains fit it (FP64 left-hand side, at
  * most 384 rows, open chains or closed chains whose final solve is two-row
  * cycles; it always runs at least one cyclic-reduction level), or
- * HAMSTR_KERNEL_HALF2 for such open chains.  All give the same solution up to
- * the sign of zero entries.  FP64 contexts start with HAMSTR_KERNEL_HALF2 for
- * open chains of more than 256 rows and HAMSTR_KERNEL_ROW elsewhere. */
+ * HAMSTR_KERNEL_HALF2 for open chains of at most 384 rows (FP64 or FP32).
+ * All give the same solution up to the sign of zero entries.  FP64 contexts
+ * start with HAMSTR_KERNEL_HALF2 for open chains of more than 256 rows;
+ * everything else starts with HAMSTR_KERNEL_ROW. */
 int hamstr_set_adi_kernel(hamstr_ctx *ctx, int group, int kernel);
 
 /* SOLVER::initFlow (initFlow.cpp:12-93): q := freestream. */

@@ -168,3 +168,23 @@ def test_fused_step_equals_api_calls(hip, order, lhs):
     _eq(a.get("r"), b.get("r"), "linear residual")
     a.close()
     b.close()
+
+
+def test_half2_fp32_equals_row_kernel(hip):
+    """FP32 left-hand side: two lines per workgroup (an autotuner option there)
+    against the one-row kernel, 12 steps on the mesh with 384-row lines."""
+    from hamstr_oop_amd import meshgen
+    mesh = meshgen.naca_ogrid(512, 256, tri_layers=128)
+    out = {}
+    for kernel in (hip.KERNEL_ROW, hip.KERNEL_HALF2):
+        s = hip.Solver(mesh, 0.5, 2.0, 3, 10.0, lhs=hip.LHS_FP32, adi_mode=hip.ADI_FAST)
+        for k, g in enumerate(hip.launch_groups(s)):
+            if g["final"] == 0 and g["max_rows"] == 384:
+                hip.set_adi_kernel(s, k, kernel)
+        with pytest.raises(hip.HamstrError):
+            hip.set_adi_kernel(s, [g["final"] for g in hip.launch_groups(s)].index(0), hip.KERNEL_HALF)
+        s.step(12, check_every=12)
+        out[kernel] = s.get("q")
+        s.close()
+    _eq(out[hip.KERNEL_HALF2], out[hip.KERNEL_ROW], "FP32 q after 12 steps")
+
