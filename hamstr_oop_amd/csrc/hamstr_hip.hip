@@ -3252,8 +3252,9 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
         // VGPRs), FP64 or FP32 open chains for two per workgroup
         bool ok = true;
         for (int i : bucket[b]) ok = ok && (!ch_periodic[i] || (ch_ncell[i] % 2 == 0 && ch_ncell[i] >= 4));
-        g.half_ok = ok && tl == sizeof(double);
-        g.half2_ok = kind == 1;
+        const size_t slice = 36 * tl * (size_t)lds_stride((kHalfMaxRows + 1) / 2);  // one chain's LDS
+        g.half_ok = ok && tl == sizeof(double) && slice <= lds_max;
+        g.half2_ok = kind == 1 && 2 * slice <= lds_max;
         if (g.half_ok || g.half2_ok) {
           // FP64: two open chains per workgroup where one chain's waves
           // would not pack onto the SIMDs twice (over 256 rows); elsewhere,
