@@ -3254,7 +3254,10 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
         for (int i : bucket[b]) ok = ok && (!ch_periodic[i] || (ch_ncell[i] % 2 == 0 && ch_ncell[i] >= 4));
         const size_t slice = 36 * tl * (size_t)lds_stride((kHalfMaxRows + 1) / 2);  // one chain's LDS
         g.half_ok = ok && tl == sizeof(double) && slice <= lds_max;
-        g.half2_ok = kind == 1 && 2 * slice <= lds_max;
+        // (two per workgroup only pays for lines of more than 256 rows; it was
+        // slower on every shorter length measured, and the autotuner's time
+        // counts in batched runs)
+        g.half2_ok = kind == 1 && g.maxN > 256 && 2 * slice <= lds_max;
         if (g.half_ok || g.half2_ok) {
           // FP64: two open chains per workgroup where one chain's waves
           // would not pack onto the SIMDs twice (over 256 rows); elsewhere,

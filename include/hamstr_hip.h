@@ -182,7 +182,7 @@ int hamstr_set_launch(hamstr_ctx *ctx, int group, int grid, int cr_levels);
  * HAMSTR_KERNEL_HALF where the group's chains fit it (FP64 left-hand side, at
  * most 384 rows, open chains or closed chains whose final solve is two-row
  * cycles; it always runs at least one cyclic-reduction level), or
- * HAMSTR_KERNEL_HALF2 for open chains of at most 384 rows (FP64 or FP32).
+ * HAMSTR_KERNEL_HALF2 for open chains of 257 to 384 rows (FP64 or FP32).
  * All give the same solution up to the sign of zero entries.  FP64 contexts
  * start with HAMSTR_KERNEL_HALF2 for open chains of more than 256 rows;
  * everything else starts with HAMSTR_KERNEL_ROW. */

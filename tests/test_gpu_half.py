@@ -65,6 +65,9 @@ def test_default_groups(hip):
     assert ks == {256: hip.KERNEL_ROW, 128: hip.KERNEL_ROW}, ks
     for k in range(2):
         hip.set_adi_kernel(s, k, hip.KERNEL_HALF)
+    lines = [g["max_rows"] for g in hip.launch_groups(s)].index(128)
+    with pytest.raises(hip.HamstrError):   # two per workgroup: lines of more than 256 rows only
+        hip.set_adi_kernel(s, lines, hip.KERNEL_HALF2)
     loops = [g["max_rows"] for g in hip.launch_groups(s)].index(256)
     with pytest.raises(hip.HamstrError):
         hip.set_adi_kernel(s, loops, hip.KERNEL_HALF2)
@@ -86,6 +89,7 @@ def test_default_groups(hip):
     ("naca512x256_tri128", lambda m: m.naca_ogrid(512, 256, tri_layers=128)),
     ("naca256x128", lambda m: m.naca_ogrid(256, 128)),
     ("circle96x45", lambda m: m.circle_ogrid(96, 45)),       # odd line length
+    ("circle48x301", lambda m: m.circle_ogrid(48, 301)),     # odd, long enough for two per workgroup
 ])
 def test_half_equals_row_kernel(hip, name, mk, ncr):
     from hamstr_oop_amd import meshgen
@@ -103,9 +107,10 @@ def test_half_equals_row_kernel(hip, name, mk, ncr):
 
 def test_half_in_batch_and_small_grid(hip):
     """A batch of meshes of different sizes (persistent launches walking
-    several chains per workgroup), half against row kernel."""
+    several chains per workgroup), half against row kernel; lines of 300 and
+    270 rows share the two-per-workgroup group."""
     from hamstr_oop_amd import meshgen
-    meshes = [meshgen.naca_ogrid(128, 64), meshgen.naca_ogrid(256, 128), meshgen.circle_ogrid(96, 45)]
+    meshes = [meshgen.naca_ogrid(128, 64), meshgen.circle_ogrid(48, 300), meshgen.circle_ogrid(40, 270)]
     flows = [(0.3, 0.0), (0.6, 1.0), (0.5, 2.0)]
     out = {}
     for kernel in _kernels(hip):
@@ -125,11 +130,11 @@ def test_half_in_batch_and_small_grid(hip):
 @pytest.mark.parametrize("retire", [False, True])
 def test_half2_mixed_pairs(hip, retire):
     """Two chains per workgroup where a pair holds lines of different cases and
-    lengths (63 lines of 45 rows, then 64 of 41): both live, or the first case
-    retired, whose lines then drop out of their pairs; the results equal the
-    one-row kernel's."""
+    lengths (63 lines of 301 rows, then 64 of 291): both live, or the first
+    case retired, whose lines then drop out of their pairs; the results equal
+    the one-row kernel's."""
     from hamstr_oop_amd import meshgen
-    meshes = [meshgen.circle_ogrid(63, 45), meshgen.circle_ogrid(64, 41)]
+    meshes = [meshgen.circle_ogrid(63, 301), meshgen.circle_ogrid(64, 291)]
     out = {}
     for kernel in (hip.KERNEL_ROW, hip.KERNEL_HALF2):
         b = hip.Batch([(m, 0.4, 1.0) for m in meshes], order=3, cfl=10.0, adi_mode=hip.ADI_FAST)
