@@ -274,26 +274,35 @@ __global__ __launch_bounds__(kBlock) void k_rhs_faces(FaceTab ft, Consts c, Live
 __device__ __forceinline__ void block_norm_partial(double sq, double mx, double* __restrict__ partial, int blk,
                                                    double* __restrict__ partial2 = nullptr) {
   // per-block partial (sum of r_rho^2, max |r_rho|); k_norm_final reduces them.
-  // Written to partial and, when given, to partial2 as well.
-  __shared__ double ssq[kBlock];
-  __shared__ double smx[kBlock];
-  const int t = threadIdx.x;
-  ssq[t] = sq;
-  smx[t] = mx;
-  __syncthreads();
-  for (int w = kBlock / 2; w > 0; w >>= 1) {
-    if (t < w) {
-      ssq[t] += ssq[t + w];
-      smx[t] = fmax(smx[t], smx[t + w]);
-    }
-    __syncthreads();
+  // Written to partial and, when given, to partial2 as well.  A butterfly
+  // over each wave's 64 lanes (lane 0's order is fixed), then the waves'
+  // results in wave order: deterministic, with one barrier.
+  static_assert(kBlock % 64 == 0, "whole waves");
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    sq += __shfl_xor(sq, o);
+    mx = fmax(mx, __shfl_xor(mx, o));
   }
+  __shared__ double wsq[kBlock / 64];
+  __shared__ double wmx[kBlock / 64];
+  const int t = threadIdx.x;
+  if ((t & 63) == 0) {
+    wsq[t >> 6] = sq;
+    wmx[t >> 6] = mx;
+  }
+  __syncthreads();
   if (t == 0) {
-    partial[2 * blk] = ssq[0];
-    partial[2 * blk + 1] = smx[0];
+    double a = wsq[0], b = wmx[0];
+#pragma unroll
+    for (int w = 1; w < kBlock / 64; w++) {
+      a += wsq[w];
+      b = fmax(b, wmx[w]);
+    }
+    partial[2 * blk] = a;
+    partial[2 * blk + 1] = b;
     if (partial2) {
-      partial2[2 * blk] = ssq[0];
-      partial2[2 * blk + 1] = smx[0];
+      partial2[2 * blk] = a;
+      partial2[2 * blk + 1] = b;
     }
   }
 }
