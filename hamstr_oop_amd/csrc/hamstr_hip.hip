@@ -693,12 +693,56 @@ __device__ __forceinline__ void store_solution(double* __restrict__ r, double* _
 // the spike a' for the back sweep: element e of row i of chain t lives at
 // scratch[(i*36+e)*stride + t].
 // --------------------------------------------------------------------------
+// EXACT mode, first pass of an ADI launch group: every row of the group's
+// chains assembled at once (assemble_row, one thread per row, a workgroup
+// per chain) into rows[e * nRows + pb + i] (e < 52: A, B, C, F), so that
+// k_adi_exact's sequential walk reads each row with one round of loads it can
+// issue a row ahead, instead of the dependent loads of the assembly.
+template <typename T>
+__global__ __launch_bounds__(256) void k_exact_assemble(ChainTab ct, Live lv, const int* __restrict__ chains, int nch,
+                                                        const T* __restrict__ ff, int nProc, double cfl,
+                                                        const double* __restrict__ sigma,
+                                                        const double* __restrict__ r, T* __restrict__ rows,
+                                                        int nRows) {
+  const int w = blockIdx.x;
+  if (w >= nch) return;
+  const int ch = chains[w];
+  if (lv.chain && !lv.chain[ch]) return;  // the same for the whole workgroup
+  const int pb = ct.pbase[ch], nf = ct.nf[ch], N = ct.ncell[ch];
+  const bool periodic = ct.periodic[ch] != 0;
+  for (int i = threadIdx.x; i < N; i += blockDim.x) {
+    T A[16], B[16], C[16], F[4];
+    int cell;
+    assemble_row<T>(ct, ff, nProc, pb, nf, N, periodic, i, cfl, sigma, r, A, B, C, F, &cell);
+    T* o = rows + pb + i;
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+      o[(size_t)e * nRows] = A[e];
+      o[(size_t)(16 + e) * nRows] = B[e];
+      o[(size_t)(32 + e) * nRows] = C[e];
+    }
+#pragma unroll
+    for (int e = 0; e < 4; e++) o[(size_t)(48 + e) * nRows] = F[e];
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void load_exact_row(const T* __restrict__ rows, int nRows, int p, T* A, T* B, T* C, T* F) {
+#pragma unroll
+  for (int e = 0; e < 16; e++) {
+    A[e] = rows[(size_t)e * nRows + p];
+    B[e] = rows[(size_t)(16 + e) * nRows + p];
+    C[e] = rows[(size_t)(32 + e) * nRows + p];
+  }
+#pragma unroll
+  for (int e = 0; e < 4; e++) F[e] = rows[(size_t)(48 + e) * nRows + p];
+}
+
 template <typename T>
 __global__ __launch_bounds__(64) void k_adi_exact(ChainTab ct, Live lv, const int* __restrict__ chains, int nch,
-                                                  const T* __restrict__ ff, int nProc, double cfl,
-                                                  const double* __restrict__ sigma, double* __restrict__ r,
-                                                  T* __restrict__ scratch, int stride, double* __restrict__ dq,
-                                                  int dqmode) {
+                                                  const T* __restrict__ rows, int nRows,
+                                                  double* __restrict__ r, T* __restrict__ scratch, int stride,
+                                                  double* __restrict__ dq, int dqmode) {
   const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= nch) return;
   const int ch = chains[t];
@@ -708,10 +752,20 @@ __global__ __launch_bounds__(64) void k_adi_exact(ChainTab ct, Live lv, const in
   auto S = [&](int i, int e) -> T& { return scratch[((size_t)i * 36 + e) * stride + t]; };
 
   T Cp[16], Fp[4], ap[16];
+  // row i+1's entries are loaded while row i is eliminated
+  T nA[16], nB[16], nC[16], nF[4];
+  load_exact_row(rows, nRows, pb, nA, nB, nC, nF);
   for (int i = 0; i < N; i++) {
     T A[16], B[16], C[16], F[4], a[16];
-    int cell;
-    assemble_row<T>(ct, ff, nProc, pb, nf, N, periodic, i, cfl, sigma, r, A, B, C, F, &cell);
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+      A[e] = nA[e];
+      B[e] = nB[e];
+      C[e] = nC[e];
+    }
+#pragma unroll
+    for (int e = 0; e < 4; e++) F[e] = nF[e];
+    if (i + 1 < N) load_exact_row(rows, nRows, pb + i + 1, nA, nB, nC, nF);
     if (i != 0) {
       mat_sub_mul(B, A, Cp);
       vec_sub_mul(F, A, Fp);
@@ -2689,6 +2743,7 @@ struct hamstr_ctx {
   bool last_phase_all_cells = false;
   void* scratch = nullptr;
   size_t scratch_stride = 0;
+  void* exact_rows = nullptr;  // EXACT groups: T[52][nChainProc] assembled rows (k_exact_assemble)
   double* partial = nullptr;
   int nPart = 0;
   double* npart = nullptr;  // single case: [kGraphMaxIters][2 stages][nPart][2] block partials
@@ -2856,12 +2911,20 @@ int launch_group(hamstr_ctx* x, const Group& g, int grid, int dqmode, int ncr) {
     }
 #undef FASTT
   } else if (x->lhs32()) {
+    hipLaunchKernelGGL((k_exact_assemble<float>), dim3(g.n), dim3(256), 0, x->stream, ct, x->live(), chains, g.n,
+                       (const float*)x->ff, x->nProc, x->c.cfl, x->sigma, x->r, (float*)x->exact_rows,
+                       x->nChainProc);
+    HIPCHK(hipGetLastError());
     hipLaunchKernelGGL((k_adi_exact<float>), dim3(nblk(g.n, 64)), dim3(64), 0, x->stream, ct, x->live(), chains, g.n,
-                       (const float*)x->ff, x->nProc, x->c.cfl, x->sigma, x->r, (float*)x->scratch,
-                       (int)x->scratch_stride, x->dq, dqmode);
+                       (const float*)x->exact_rows, x->nChainProc, x->r, (float*)x->scratch, (int)x->scratch_stride,
+                       x->dq, dqmode);
   } else {
+    hipLaunchKernelGGL((k_exact_assemble<double>), dim3(g.n), dim3(256), 0, x->stream, ct, x->live(), chains, g.n,
+                       (const double*)x->ff, x->nProc, x->c.cfl, x->sigma, x->r, (double*)x->exact_rows,
+                       x->nChainProc);
+    HIPCHK(hipGetLastError());
     hipLaunchKernelGGL((k_adi_exact<double>), dim3(nblk(g.n, 64)), dim3(64), 0, x->stream, ct, x->live(), chains, g.n,
-                       (const double*)x->ff, x->nProc, x->c.cfl, x->sigma, x->r, (double*)x->scratch,
+                       (const double*)x->exact_rows, x->nChainProc, x->r, (double*)x->scratch,
                        (int)x->scratch_stride, x->dq, dqmode);
   }
   HIPCHK(hipGetLastError());
@@ -2944,7 +3007,7 @@ void free_ctx(hamstr_ctx* x) {
                   x->cell_off, x->cell_ent, x->ch_pbase, x->ch_nf, x->ch_ncell, x->ch_periodic, x->group_chains,
                   x->scratch, x->partial, x->norms, x->hist, x->nanflag, x->rrho, x->face_on, x->cell_on,
                   x->chain_on, x->ghist, x->npart, x->keep_norms, x->od_chain, x->od_off, x->od_ent,
-                  x->ch_hbase, x->long_scratch};
+                  x->ch_hbase, x->long_scratch, x->exact_rows};
   if (x->gexec) (void)hipGraphExecDestroy(x->gexec);
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -3379,6 +3442,7 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
   chk(hipMalloc(&x->ff, tl * 32 * nProc), "alloc ff");
   x->scratch_stride = std::max<size_t>(maxGroupChains, 1);
   chk(hipMalloc(&x->scratch, tl * 36 * std::max<size_t>(maxRows, 1) * x->scratch_stride), "alloc scratch");
+  if (maxRows > 0) chk(hipMalloc(&x->exact_rows, tl * 52 * (size_t)std::max(nChainProc, 1)), "alloc exact rows");
   chk(hipMalloc(&x->long_scratch, tl * 36 * (size_t)x->nLong), "alloc long-chain scratch");
   x->nPart = nblk(nCell);
   chk(hipMalloc(&x->partial, sizeof(double) * 2 * x->nPart), "alloc partial");
