@@ -2164,10 +2164,14 @@ __device__ __forceinline__ void adi_fast_chain_half(ChainTab ct, const int ch, c
 
 // one chain per workgroup pass; the workgroup walks the group's chains with a
 // stride of the grid size (a persistent launch: chains are sorted longest
-// first, so every pass of the grid does similar work)
+// first, so every pass of the grid does similar work).  FP32 workgroups of up
+// to 512 threads are held to 128 VGPRs (4 waves per SIMD), so that two
+// 6-wave workgroups (384-row chains) pack onto a CU: for odd cycles that costs
+// 22 spilled VGPRs and still took a 256-case batch's 384-row loops from 1.34
+// to 0.95 ms.
 template <typename T, int FINAL, int MAXT>
 __global__ __launch_bounds__(MAXT)
-__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && MAXT == 512 && FINAL != kFinalOdd ? 4 : 1)))
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && MAXT == 512 ? 4 : 1)))
 void k_adi_fast(ChainTab ct, Live lv, const int* __restrict__ chains, int nch,
                                                    const T* __restrict__ ff, int nProc, double cfl,
                                                    const double* __restrict__ sigma, double* __restrict__ r,
@@ -2702,6 +2706,7 @@ struct Group {
 
 struct hamstr_ctx {
   int device = 0;
+  int ncu = 1;  // compute units of the device
   hipStream_t stream = nullptr;
   hamstr_params prm{};
   int nCell = 0, nFace = 0, nChain = 0, nProc = 0, nSlot = 0, nGhost = 2, nCase = 1;
@@ -2894,9 +2899,13 @@ int launch_group(hamstr_ctx* x, const Group& g, int grid, int dqmode, int ncr) {
       HIPCHK(hamstr_adi_long_launch(x->lhs32(), g.final, (unsigned)grid, (unsigned)g.block, lds, x->stream, &ct, &lv,
                                     chains, g.n, x->ff, x->nProc, x->c.cfl, x->sigma, x->r, x->dq, dqmode, nc,
                                     x->long_scratch, x->nLong, x->ch_hbase));
+    } else if (x->lhs32() && g.final == kFinalOdd && g.block <= 384 && grid <= x->ncu) {
+      // odd cycles in one round of workgroups (one per CU): without the
+      // 128-VGPR bound, which would only add spills here
+      FASTT(float, kFinalOdd, 384);
     } else if (x->lhs32() && g.block <= 512) {
-      // a 512-thread bound gives the FP32 kernels 256 VGPRs instead of 128
-      // (no spills): 10% faster on chains that fit
+      // up to 512 threads: held to 128 VGPRs, so that two 384-row
+      // workgroups share a CU
       if (g.final == kFinalOpen) FASTT(float, kFinalOpen, 512);
       else if (g.final == kFinalPair) FASTT(float, kFinalPair, 512);
       else FASTT(float, kFinalOdd, 512);
@@ -3428,6 +3437,11 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
     if (e != hipSuccess && rc == HAMSTR_OK) rc = fail(HAMSTR_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
   };
   chk(hipGetDevice(&x->device), "hipGetDevice");
+  {
+    int cu = 0;
+    if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, x->device) == hipSuccess && cu > 0)
+      x->ncu = cu;
+  }
   chk(hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking), "hipStreamCreate");
   const size_t c4 = sizeof(double) * 4 * nCell;
   chk(hipMalloc(&x->q, c4), "alloc q");
