@@ -1596,14 +1596,15 @@ __device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const 
 
   // (2) parallel cyclic reduction of the rows at multiples of d (members):
   // by quads of lanes (pcr_quad_row) when the workgroup has four threads per
-  // member, else thread t holding member row t*d in registers.  Not for odd
-  // cycles, whose final solve leaves no registers for it.
+  // member, else thread t holding member row t*d in registers.  Not for FP64
+  // odd cycles, whose final solve leaves no registers for it (in FP32 it took
+  // a 256-case batch's 384-row loops from 0.95 to 0.87 ms).
   const int nmem = (N + d - 1) / d;
   const bool member = i < nmem;
   const int mrow = i * d;
   T a[16], c[16], f[4];
   int dd = d;
-  if (FINAL != kFinalOdd && 4 * nmem <= (int)blockDim.x) {
+  if ((FINAL != kFinalOdd || sizeof(T) == sizeof(float)) && 4 * nmem <= (int)blockDim.x) {
     const int g = i >> 2, q = i & 3, grow = g * d;
     while (periodic ? (n % 2 == 0 && n >= 4) : (dd < N)) {
       T Ac[4], Cc[4], F[4];
