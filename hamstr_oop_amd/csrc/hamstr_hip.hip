@@ -1163,37 +1163,11 @@ __device__ __forceinline__ void fast_row(const ChainTab& ct, const T* __restrict
   normalise_row(B, a, c, f);
 }
 
-// One cyclic-reduction elimination: the surviving row (ra, rc, rf) absorbs
-// its neighbours, read from `src` (element-major with stride SS, at slots sL
-// and sR); the new row is normalised into (na, nc, rf).
-template <typename T>
-__device__ __forceinline__ void cr_eliminate(const T* ra, const T* rc, T* rf, const T* src, int SS, bool hasL,
-                                             int sL, bool hasR, int sR, T* na, T* nc) {
-#pragma clang fp contract(fast)
-  T M[16];
-#pragma unroll
-  for (int e = 0; e < 16; e++) {
-    M[e] = (e % 5 == 0) ? T(1) : T(0);
-    na[e] = T(0);
-    nc[e] = T(0);
-  }
-  if (hasL) {
-    mm4_acc_lds(M, ra, src, SS, 16, sL);   // M -= A^_i C^_L
-    mm4_acc_lds(na, ra, src, SS, 0, sL);   // A^new = -A^_i A^_L
-    mv4_acc_lds(rf, ra, src, SS, sL);      // F -= A^_i F^_L
-  }
-  if (hasR) {
-    mm4_acc_lds(M, rc, src, SS, 0, sR);    // M -= C^_i A^_R
-    mm4_acc_lds(nc, rc, src, SS, 16, sR);  // C^new = -C^_i C^_R
-    mv4_acc_lds(rf, rc, src, SS, sR);      // F -= C^_i F^_R
-  }
-  normalise_row(M, na, nc, rf);
-}
-
-// The same elimination with the row updated in place (A, C, F in registers):
-// M = I - A C_L - C A_R, then A <- -M^-1 A A_L, C <- -M^-1 C C_R and
-// F <- M^-1 (F - A F_L - C F_R), every element computed by the operations
-// cr_eliminate uses.  The order keeps M's factors, F, one new block and at
+// One cyclic-reduction elimination, the row updated in place (A, C, F in
+// registers): it absorbs the rows at slots sL and sR of `src` (element-major,
+// stride SS): M = I - A C_L - C A_R, then A <- -M^-1 A A_L, C <- -M^-1 C C_R
+// and F <- M^-1 (F - A F_L - C F_R), every element computed by the
+// operations (and in the order) the one-row kernel's reduction levels use.  The order keeps M's factors, F, one new block and at
 // most both old blocks live (68 values), and the scheduling barriers keep the
 // neighbours' loads of one stage from going in flight during another, so the
 // kernels that use it fit in 168 VGPRs.
@@ -1866,109 +1840,6 @@ __device__ __forceinline__ void adi_fast_chain_long(ChainTab ct, const int ch, c
 #pragma unroll
     for (int e = 0; e < 4; e++) x[e] = lds[(32 + e) * SM + lpad(t)];
     store_solution(r, dq, dqmode, ct.pleft[pb + 2 * t], x);
-  }
-}
-
-// fast_row's result (row m assembled as assemble_row does and normalised by
-// its diagonal block, an open chain's end rows without the coupling past the
-// chain's ends), computed in an order that needs fewer registers: the
-// diagonal block is summed and factored first, then the couplings A = R_m
-// and C = 0 - L_(m+1) are scaled and solved a column at a time, selected
-// rather than branched on (a branch around a block costs a second copy of
-// it).  Every entry takes the operations assemble_row and normalise_row give
-// it.
-template <typename T>
-__device__ __forceinline__ void fast_row_lean(const ChainTab& ct, const T* __restrict__ ff, int nProc, int pb,
-                                              int nf, int N, bool periodic, int m, double cfl,
-                                              const double* __restrict__ sigma, const double* __restrict__ r, T* A,
-                                              T* C, T* F) {
-  // no contraction here (as in assemble_row): B's diagonal is rounded after
-  // the scaling and again after the + 1
-  const int pm = pb + m;
-  const bool has_next = periodic || m + 1 < nf;
-  const int pn = periodic ? pb + ((m + 1 == N) ? 0 : m + 1) : (has_next ? pm + 1 : pm);
-  const int rc = periodic ? 0 : ct.pright[pm];
-  const int rn = (periodic || !has_next) ? 0 : ct.pright[pn];
-  const bool end = !periodic && has_next && m + 1 == nf - 1 && rn < 0;
-  const bool hasA = periodic || (m > 0 && rc > -1);
-  const bool hasC = periodic || (has_next && !end && rn > -1 && m != N - 1);
-  T B[16];
-  {
-    T L[16], R[16];
-    load_fmat(ff, nProc, pm, L, R);
-#pragma unroll
-    for (int e = 0; e < 16; e++) B[e] = L[e];
-    if (!periodic && rc == -2) {
-      reflect_cols(R, ct.refmtx + 4 * (size_t)ct.pface[pm]);
-#pragma unroll
-      for (int e = 0; e < 16; e++) B[e] += R[e];
-    }
-  }
-  if (has_next) {
-    T L[16], R[16];
-    load_fmat(ff, nProc, pn, L, R);
-    if (periodic) {
-#pragma unroll
-      for (int e = 0; e < 16; e++) B[e] -= R[e];
-    } else if (end) {
-      if (rn == -2) reflect_cols(R, ct.refmtx + 4 * (size_t)ct.pface[pn]);
-#pragma unroll
-      for (int e = 0; e < 16; e++) B[e] += L[e];
-      if (rn == -2) {
-#pragma unroll
-        for (int e = 0; e < 16; e++) B[e] += R[e];
-      }
-    } else if (rn > -1) {
-#pragma unroll
-      for (int e = 0; e < 16; e++) B[e] -= R[e];
-    }
-  }
-  const int cell = ct.pleft[pm];
-  if (ct.od_chain && ct.od_chain[cell] == pb) {
-    const int e1 = ct.od_off[cell + 1];
-    for (int e = ct.od_off[cell]; e < e1; e++) {
-      const int code = ct.od_ent[e];
-      const int po = code >> 1;
-      if (code & 1) {
-#pragma unroll
-        for (int k = 0; k < 16; k++) B[k] += ff[(size_t)k * nProc + po];
-      } else {
-#pragma unroll
-        for (int k = 0; k < 16; k++) B[k] -= ff[(size_t)(16 + k) * nProc + po];
-      }
-    }
-  }
-  const T dtfac = T(cfl / sigma[cell]);
-#pragma unroll
-  for (int e = 0; e < 16; e++) B[e] *= dtfac;
-  B[0] += T(1.0);
-  B[5] += T(1.0);
-  B[10] += T(1.0);
-  B[15] += T(1.0);
-  const LU4<T> LU = lu4_fast(B);
-  const double4 rv = *reinterpret_cast<const double4*>(r + 4 * (size_t)cell);
-  F[0] = T(rv.x);
-  F[1] = T(rv.y);
-  F[2] = T(rv.z);
-  F[3] = T(rv.w);
-  lu4_solve_fast(LU, F, 1);
-#pragma unroll
-  for (int j = 0; j < 4; j++) {
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const T v = ff[(size_t)(16 + 4 * k + j) * nProc + pm];  // R_m
-      A[4 * k + j] = (hasA ? v : T(0)) * dtfac;
-    }
-    lu4_solve_fast(LU, A + j, 4);
-  }
-#pragma unroll
-  for (int j = 0; j < 4; j++) {
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-      const T v = ff[(size_t)(4 * k + j) * nProc + pn];  // L_(m+1)
-      C[4 * k + j] = (hasC ? T(0) - v : T(0)) * dtfac;
-    }
-    lu4_solve_fast(LU, C + j, 4);
   }
 }
 
