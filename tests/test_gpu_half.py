@@ -193,3 +193,25 @@ def test_half2_fp32_equals_row_kernel(hip):
         s.close()
     _eq(out[hip.KERNEL_HALF2], out[hip.KERNEL_ROW], "FP32 q after 12 steps")
 
+
+
+def test_fp32_odd_cycle_kernels_agree(hip):
+    """FP32 odd cycles run the 128-VGPR kernel when a launch has more
+    workgroups than CUs and the unbounded one when it fits one round: the
+    same arithmetic, so a batch stepped with either grid ends bit-identical."""
+    from hamstr_oop_amd import meshgen
+    meshes = [meshgen.circle_ogrid(24, 10) for _ in range(40)]  # 400 loops of 24 rows: odd 3-cycles
+    out = {}
+    for grid in (64, 4096):  # one round on any MI355X (256 CUs), and many
+        b = hip.Batch([(m, 0.3 + 0.005 * k, 1.0) for k, m in enumerate(meshes)], order=3, cfl=10.0,
+                      lhs=hip.LHS_FP32, adi_mode=hip.ADI_FAST)
+        for k, g in enumerate(hip.launch_groups(b)):
+            if g["final"] == 2:
+                assert g["chains"] == 400
+                hip.set_launch(b, k, grid, 1)
+        b.batch_step(8, check_every=4)
+        out[grid] = [b.case_q(k).copy() for k in range(len(meshes))]
+        b.close()
+    a, c = out.values()
+    for k in range(len(meshes)):
+        _eq(a[k], c[k], f"case {k}")
