@@ -53,12 +53,19 @@ hipError_t hamstr_adi_half_launch(bool fp32, int final_type, int pairs, unsigned
     return hipGetLastError();
   }
   if (fp32 || block > 384) return hipErrorInvalidValue;
-#define HALFL(F)                                                                                                \
-  hipLaunchKernelGGL((k_adi_fast_half<double, F, 384, 3>), g, b, lds, stream, ct, lv, chains, nch,              \
+#define HALFL(F, M)                                                                                             \
+  hipLaunchKernelGGL((k_adi_fast_half<double, F, M, 3>), g, b, lds, stream, ct, lv, chains, nch,                \
                      (const double*)ff, nProc, cfl, sigma, r, dq, dqmode, ncr, (double*)hs, hstride, hbase)
-  if (final_type == kFinalOpen) HALFL(kFinalOpen);
-  else if (final_type == kFinalPair) HALFL(kFinalPair);
-  else HALFL(kFinalOdd);
+  // 192-thread instance (a 28.5 KB LDS slice) for chains of up to 192 rows
+  if (block <= 192) {
+    if (final_type == kFinalOpen) HALFL(kFinalOpen, 192);
+    else if (final_type == kFinalPair) HALFL(kFinalPair, 192);
+    else HALFL(kFinalOdd, 192);
+  } else {
+    if (final_type == kFinalOpen) HALFL(kFinalOpen, 384);
+    else if (final_type == kFinalPair) HALFL(kFinalPair, 384);
+    else HALFL(kFinalOdd, 384);
+  }
 #undef HALFL
   return hipGetLastError();
 }

@@ -1853,7 +1853,7 @@ __device__ __forceinline__ void adi_fast_chain_long(ChainTab ct, const int ch, c
 // group calls it (it holds barriers); inactive threads pass active = false.
 // An open chain's last face (no row owns it) is read by the last row.  The
 // entries take the operations assemble_row and normalise_row give them.
-constexpr int kXStride = 385;  // the half-LDS kernels' XS (chains of at most 384 rows)
+constexpr int kXStride = 385;  // the 384-thread half-LDS kernels' XS (MAXT + 1)
 template <typename T>
 __device__ __forceinline__ void fast_row_xchg(const ChainTab& ct, const T* __restrict__ ff, int nProc, int pb,
                                               int nf, int N, bool periodic, int m, bool active, double cfl,
@@ -1982,7 +1982,7 @@ __device__ __forceinline__ void fast_row_xchg(const ChainTab& ct, const T* __res
 // closed chains).  Needs N <= the group's threads, open chains or closed
 // chains of an even N >= 4, ncr >= 1, and ceil(NL/2) rows within the fixed
 // LDS stride SM (SM >= lds_stride(ceil(NL/2))).
-template <typename T, int FINAL, int SM>
+template <typename T, int FINAL, int SM, int XS>
 __device__ __forceinline__ void adi_fast_chain_half(ChainTab ct, const int ch, const int NL, T* lds, const int t,
                                                     const int NT,
                                                     const T* __restrict__ ff, int nProc, double cfl,
@@ -1998,7 +1998,7 @@ __device__ __forceinline__ void adi_fast_chain_half(ChainTab ct, const int ch, c
   const bool active = row < N;
   {
     T a[16], c[16], f[4];
-    fast_row_xchg<T>(ct, ff, nProc, pb, nf, N, periodic, row, active, cfl, sigma, r, lds, kXStride, a, c, f);
+    fast_row_xchg<T>(ct, ff, nProc, pb, nf, N, periodic, row, active, cfl, sigma, r, lds, XS, a, c, f);
     if (active && !even) {
       lds_store_row(lds, SM, lpad(t - M), a, c, f);
       lds_store_row(hrow, hstride, t - M, a, c, f);
@@ -2085,14 +2085,15 @@ __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) voi
     ChainTab ct, Live lv, const int* __restrict__ chains, int nch, const T* __restrict__ ff, int nProc, double cfl,
     const double* __restrict__ sigma, double* __restrict__ r, double* __restrict__ dq, int dqmode, int ncr,
     T* __restrict__ hs, int hstride, const int* __restrict__ hbase) {
-  static_assert(MAXT < kXStride && 16 * kXStride <= 36 * lds_stride((MAXT + 1) / 2), "face exchange fits the slice");
+  static_assert(16 * (MAXT + 1) <= 36 * lds_stride((MAXT + 1) / 2), "face exchange fits the slice");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* lds = reinterpret_cast<T*>(smem_raw);
   const int vb = xcd_block(blockIdx.x, gridDim.x);
   for (int w = vb; w < nch; w += gridDim.x) {
     const int ch = chains[w];
     if (lv.chain && !lv.chain[ch]) continue;  // the same for the whole workgroup
-    adi_fast_chain_half<T, FINAL, lds_stride((MAXT + 1) / 2)>(ct, ch, ct.ncell[ch], lds, threadIdx.x, (int)blockDim.x,
+    adi_fast_chain_half<T, FINAL, lds_stride((MAXT + 1) / 2), MAXT + 1>(ct, ch, ct.ncell[ch], lds, threadIdx.x,
+                                                                        (int)blockDim.x,
                                                               ff, nProc, cfl, sigma, r, dq, dqmode, ncr, hs, hstride,
                                                               hbase);
     __syncthreads();
@@ -2112,7 +2113,7 @@ __global__ __launch_bounds__(2 * MAXH) __attribute__((amdgpu_waves_per_eu(3))) v
     const double* __restrict__ sigma, double* __restrict__ r, double* __restrict__ dq, int dqmode, int ncr,
     T* __restrict__ hs, int hstride, const int* __restrict__ hbase) {
   constexpr int SM = lds_stride((MAXH + 1) / 2);
-  static_assert(MAXH < kXStride && 16 * kXStride <= 36 * SM, "face exchange fits the slice");
+  static_assert(16 * (MAXH + 1) <= 36 * SM, "face exchange fits the slice");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int sub = threadIdx.x >= MAXH ? 1 : 0;
   T* lds = reinterpret_cast<T*>(smem_raw) + sub * 36 * SM;
@@ -2126,7 +2127,8 @@ __global__ __launch_bounds__(2 * MAXH) __attribute__((amdgpu_waves_per_eu(3))) v
     if (c1 >= 0 && lv.chain && !lv.chain[c1]) c1 = -1;
     const int NL = max(c0 >= 0 ? ct.ncell[c0] : 0, c1 >= 0 ? ct.ncell[c1] : 0);
     if (NL == 0) continue;  // the same for the whole workgroup
-    adi_fast_chain_half<T, kFinalOpen, SM>(ct, sub ? c1 : c0, NL, lds, t, MAXH, ff, nProc, cfl, sigma, r, dq, dqmode,
+    adi_fast_chain_half<T, kFinalOpen, SM, MAXH + 1>(ct, sub ? c1 : c0, NL, lds, t, MAXH, ff, nProc, cfl, sigma, r, dq,
+                                                     dqmode,
                                            ncr, hs, hstride, hbase);
     __syncthreads();
   }
@@ -2555,6 +2557,9 @@ hipError_t hamstr_adi_half_launch(bool fp32, int final_type, int pairs, unsigned
                                   const void* ff, int nProc, double cfl, const double* sigma, double* r, double* dq,
                                   int dqmode, int ncr, void* hs, int hstride, const int* hbase);
 constexpr int kHalfMaxRows = 384;  // the workgroup bound of k_adi_fast_half, per chain of k_adi_fast_half2
+// the k_adi_fast_half instance for a workgroup of `block` threads: 192 (a
+// 28.5 KB LDS slice, four workgroups per CU) or 384 (57 KB)
+inline int half_maxt(int block) { return block <= 192 ? 192 : kHalfMaxRows; }
 // ADI kernels of a launch group (hamstr_launch_groups' second field)
 enum { kKernExact = 0, kKernRow = 1, kKernHalf = 2, kKernTwo = 3, kKernHalf2 = 4 };
 
@@ -2754,7 +2759,8 @@ int launch_group(hamstr_ctx* x, const Group& g, int grid, int dqmode, int ncr) {
     const bool half = g.kern == kKernHalf || g.kern == kKernHalf2;
     const int nc = (g.two || half) ? std::max(ncr, 1) : ncr;
     const size_t lds = (x->lhs32() ? sizeof(float) : sizeof(double)) * 36 *
-                       (half ? (g.kern == kKernHalf2 ? 2 : 1) * (size_t)lds_stride((kHalfMaxRows + 1) / 2)
+                       (half ? (g.kern == kKernHalf2 ? 2 * (size_t)lds_stride((kHalfMaxRows + 1) / 2)
+                                                     : (size_t)lds_stride((half_maxt(g.block) + 1) / 2))
                              : (size_t)lds_stride(g.two ? (g.maxN + 1) / 2 : g.maxN));
 #define FASTT(T, F, M)                                                                                            \
   hipLaunchKernelGGL((k_adi_fast<T, F, M>), gr, b, lds, x->stream, ct, x->live(), chains, g.n, (const T*)x->ff,   \
