@@ -167,7 +167,8 @@ int hamstr_autotune(hamstr_ctx *ctx, int reps, double *best_ms, int cap, int *ng
 #define HAMSTR_KERNEL_ROW 1   /* FAST: one thread per row, every row in LDS */
 #define HAMSTR_KERNEL_HALF 2  /* FAST, FP64: one thread per row, half the rows in LDS
                                  (the rest in global scratch), at most 168 VGPRs:
-                                 two 384-row chains per CU */
+                                 two 384-row chains per CU, or four of at most
+                                 192 rows (a 192-thread build, 28.5 KB of LDS) */
 #define HAMSTR_KERNEL_TWO 3   /* FAST: chains longer than a workgroup, two rows per thread */
 #define HAMSTR_KERNEL_HALF2 4 /* FAST, open chains (FP64 or FP32): half the rows in LDS, two
                                  chains per 768-thread workgroup (12 waves, 3 per SIMD) */
