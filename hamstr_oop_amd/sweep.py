@@ -266,3 +266,49 @@ def warm_init(mlp, norm, feats: Sequence[np.ndarray], fallback: Optional[list] =
         if fallback is not None:
             fallback.append(n)
     return init
+
+
+def time_batch_iteration(n_cases: int = 256, lhs: int = hip.LHS_FP32, warmup: int = 20, iters: int = 50,
+                         autotune: bool = True, seed: int = SEED) -> dict:
+    """ms per batched iteration of the drawn case table, every case stepping
+    (no tolerance, no retirement): first `iters` iterations replaying captured
+    graphs, then `iters` launched eagerly with events between kernels for the
+    per-stage times.  The launch groups are autotuned first unless
+    autotune=False."""
+    specs = draw_table(n_cases, seed)
+    b = hip.Batch([(c.mesh(), c.mach, c.alpha) for c in specs], order=3, cfl=10.0, lhs=lhs,
+                  adi_mode=hip.ADI_FAST)
+    try:
+        b.computeRHS("implicit")
+        groups = hip.autotune(b)[0] if autotune else hip.launch_groups(b)
+        b.batch_step(warmup, check_every=max(warmup, 1))
+        b.sync()
+        t0 = time.perf_counter()
+        b.batch_step(iters, check_every=iters)
+        b.sync()
+        ms = (time.perf_counter() - t0) / iters * 1e3
+        b.set_timing(True)
+        b.batch_step(iters, check_every=iters)
+        stages = b.last_timings()
+        b.set_timing(False)
+        return {"cases": n_cases, "cells": int(b.cell_off[-1]), "lhs": "fp32" if lhs == hip.LHS_FP32 else "fp64",
+                "ms_per_iter": round(ms, 4), "stage_ms": {k: round(v, 4) for k, v in stages.items()},
+                "groups": [{k: g[k] for k in ("phase", "kernel", "final", "chains", "max_rows", "grid", "cr_levels")}
+                           for g in groups]}
+    finally:
+        b.close()
+
+
+if __name__ == "__main__":
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description="Time the batched iteration of the seeded case table (one GPU).")
+    ap.add_argument("--cases", type=int, default=256)
+    ap.add_argument("--lhs", choices=["fp32", "fp64"], default="fp32")
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--iters", type=int, default=50)
+    ap.add_argument("--no-autotune", action="store_true")
+    a = ap.parse_args()
+    print(json.dumps(time_batch_iteration(a.cases, hip.LHS_FP32 if a.lhs == "fp32" else hip.LHS_FP64, a.warmup,
+                                          a.iters, not a.no_autotune)), flush=True)

@@ -236,3 +236,14 @@ def test_held_case_resumes_exactly(hip):
         s.step(steps, check_every=10)
         assert np.array_equal(b.case_q(k).ravel(), s.get("q")), k
         s.close()
+
+
+@pytest.mark.parametrize("lhs", ["fp32", "fp64"])
+def test_time_batch_iteration_runs(hip, lhs):
+    """The batch timing entry (python -m hamstr_oop_amd.sweep) on a few drawn
+    cases: every stage timed, one launch group per ADI phase at least."""
+    from hamstr_oop_amd import sweep
+    out = sweep.time_batch_iteration(6, hip.LHS_FP32 if lhs == "fp32" else hip.LHS_FP64, warmup=2, iters=3)
+    assert out["cases"] == 6 and out["lhs"] == lhs and out["ms_per_iter"] > 0
+    assert {"rhs_faces", "adi_phase0", "adi_phase1", "lin_faces"} <= set(out["stage_ms"])
+    assert {g["phase"] for g in out["groups"]} == {0, 1}
