@@ -153,7 +153,7 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU runs are not part of it "
+        print(json.dumps({"error": "this benchmark runs on one GPU only; multi-GPU runs are not part of it "
                                    "(cases would shard whole meshes per GPU)", "n_gpus": args.gpus}))
         sys.exit(2)
 

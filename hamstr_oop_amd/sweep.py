@@ -135,7 +135,7 @@ def run_batched(specs: Sequence[CaseSpec], meshes, tol: float, max_iters: int, s
     """All cases in one batched context.  init(batch, case_indices) sets the
     initial state (a warm start); it and the autotuner are timed with the
     iterations.  With an FP32 LHS, cases that stall or diverge are re-run with
-    the FP64 LHS (the plain ADI iteration is the fallback and the judge): a
+    the FP64 LHS (the plain ADI iteration is the fallback, and decides convergence): a
     stalled case from the state it reached, a diverged one from its initial
     state.  Every pass applies the stall rule of the sequential runs (a shorter
     window for the FP32 pass hands cases that are merely on a plateau to the

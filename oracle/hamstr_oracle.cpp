@@ -2,7 +2,7 @@
 // hamstr_oracle.cpp -- CPU restatement of HAMSTR_OOP's line-implicit steady
 // Euler iteration.  TEST INFRASTRUCTURE ONLY: the parity tests, the smoke
 // check and the CPU-baseline leg of the benchmark load this library as the
-// checker.  The GPU product never links or calls it.
+// oracle.  The GPU product never links or calls it.
 //
 // Every function follows the reference file:line named in its comment, with
 // the same floating-point operation order (build with -ffp-contract=off), so

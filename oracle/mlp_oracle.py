@@ -1,4 +1,4 @@
-"""CPU checker for the warm-start MLP (test infrastructure only).
+"""CPU oracle for the warm-start MLP (test infrastructure only).
 
 The MLP is not part of the reference code: it is the network of the learned
 warm start, 6 -> 256 x 4 (SiLU) -> 4, with a flat FP32 parameter vector laid
@@ -10,7 +10,7 @@ the same points as the device kernels do: the hidden activations H1..H4, the
 weights W2..W4 and the hidden-layer output gradients dZ2..dZ4 that feed the
 matrix-core products; everything else stays unrounded.  The plain FP64 path is
 pinned by central finite differences (tests/test_mlp_oracle.py).  This module
-is imported only by tests and checkers, never by the library.
+is imported only by tests, never by the library.
 """
 from __future__ import annotations
 

@@ -1,7 +1,7 @@
 """ctypes bindings for the CPU oracle (liboracle.so) and the reference harness.
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by
-bench.py's cpu_baseline leg, as the checker.  The GPU product never imports it.
+bench.py's cpu_baseline leg, as the oracle.  The GPU product never imports it.
 """
 from __future__ import annotations
 

@@ -1,6 +1,6 @@
 """Writes the golden fixtures in this directory from the reference's own code.
 
-Runs in the build container only (the reference tree is not on the GPU box):
+Runs only where the reference tree is present (it is absent where the GPU tests run):
 the reference harness in oracle/_ref/ is compiled from the reference's
 sources by ``make -C oracle ref``.  Two builds exist, -O2 and -O0; order-5
 (WENO) cases and the WENO unit vectors use -O0 because the reference's -O2 build crashes there (WENO

@@ -1,7 +1,7 @@
 """The chain builder (hamstr_oop_amd.chains) on meshes given as nodes and
 cells only: it recovers the generator's chain layout on O-grids, handles
 quadrilateral meshes with randomly split cells, writes and reads back the
-reference's file set, and its checker catches each broken rule."""
+reference's file set, and its validator (chains.validate) catches each broken rule."""
 import tempfile
 
 import numpy as np
@@ -82,7 +82,7 @@ def test_file_round_trip():
     chains.validate(back, 5)
 
 
-def test_checker_catches_broken_rules():
+def test_validator_catches_broken_rules():
     def fresh():
         return meshgen.naca_ogrid(24, 10, tri_layers=3)
     m = fresh()

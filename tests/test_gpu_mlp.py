@@ -1,4 +1,4 @@
-"""The warm-start MLP's matrix-core kernels against the FP64 CPU checker, its
+"""The warm-start MLP's matrix-core kernels against the FP64 CPU oracle, its
 trainer, and what the trained network gains over the freestream cold start."""
 import numpy as np
 import pytest
@@ -18,7 +18,7 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
 
 
-def test_forward_matches_checker(ws):
+def test_forward_matches_oracle(ws):
     mlp = ws.MLP(seed=7)
     p = mlp.params()
     rng = np.random.default_rng(0)
@@ -35,7 +35,7 @@ def test_forward_matches_checker(ws):
 
 
 @pytest.mark.parametrize("n", [32, 256, 1024])
-def test_gradient_matches_checker(ws, n):
+def test_gradient_matches_oracle(ws, n):
     mlp = ws.MLP(seed=11)
     rng = np.random.default_rng(n)
     # a trained-looking scale: move the parameters off their initial values
@@ -50,7 +50,7 @@ def test_gradient_matches_checker(ws, n):
     for k, (o, m) in mo.OFF.items():
         e = _rel(g[o:o + m], gref[o:o + m])
         e64 = _rel(g[o:o + m], g64[o:o + m])
-        print(f"n={n} {k}: rel err vs BF16-point checker {e:.2e}, vs plain FP64 {e64:.2e}", flush=True)
+        print(f"n={n} {k}: rel err vs BF16-point oracle {e:.2e}, vs plain FP64 {e64:.2e}", flush=True)
         assert e < 5e-3, k
         assert e64 < 5e-2, k
     mlp.close()

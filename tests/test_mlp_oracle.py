@@ -1,4 +1,4 @@
-"""The FP64 MLP checker against mathematics: finite differences and closed forms."""
+"""The FP64 MLP oracle against mathematics: finite differences and closed forms."""
 import numpy as np
 
 from oracle import mlp_oracle as mo
