@@ -4222,8 +4222,11 @@ int hamstr_last_norms(const hamstr_ctx* x, double* out4) {
   return HAMSTR_OK;
 }
 
-// Times each FAST launch group at several persistent grid sizes (all chains,
-// 8, 4, 2, 1 workgroups per CU) and keeps the fastest.  r and dq are restored.
+// Times each FAST launch group's kernels at several persistent grid sizes (all
+// chains, 8, 4, 2, 1 workgroups per CU) and cyclic-reduction depths, and keeps
+// the fastest.  Per kernel the search is by coordinates: every grid at the
+// group's current depth, then every other depth at the best grid (8 shapes
+// instead of 20).  r and dq are restored.
 int hamstr_autotune(hamstr_ctx* x, int reps, double* best_ms, int cap, int* ngroups) {
   if (!x) return fail(HAMSTR_ERR_ARG, "null context");
   if (reps < 1) reps = 3;
@@ -4233,29 +4236,35 @@ int hamstr_autotune(hamstr_ctx* x, int reps, double* best_ms, int cap, int* ngro
   const int cu = std::max(prop.multiProcessorCount, 1);
   const size_t c4 = sizeof(double) * 4 * x->nCell;
   double *r_save = nullptr, *dq_save = nullptr;
-  HIPCHK(hipMalloc(&r_save, c4));
-  HIPCHK(hipMalloc(&dq_save, c4));
-  HIPCHK(hipMemcpyAsync(r_save, x->r, c4, hipMemcpyDeviceToDevice, x->stream));
-  HIPCHK(hipMemcpyAsync(dq_save, x->dq, c4, hipMemcpyDeviceToDevice, x->stream));
-  hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
-  int k = 0;
-  for (Group& g : x->groups) {
-    double best = -1;
-    if (g.fast) {
-      std::vector<int> cand = {g.n};
-      for (int w : {8, 4, 2, 1})
-        if (w * cu < g.n) cand.push_back(w * cu);
-      std::vector<int> kerns = {kKernRow};
-      if (g.half_ok) kerns.push_back(kKernHalf);
-      if (g.half2_ok) kerns.push_back(kKernHalf2);
-      int best_kern = g.kern;
-      for (int kern : kerns) {
-        g.kern = kern;
-        // the half-LDS kernels always run the first reduction level
-        for (int ncr = kern == kKernRow ? 0 : 1; ncr <= kMaxCR; ncr++)
-          for (int grid : cand) {
+  bool saved = false;  // r and dq copied out: restore them
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  // the trials; a failure stops them, and r, dq and the buffers are still
+  // restored and released below
+  auto search = [&]() -> int {
+    HIPCHK(hipMalloc(&r_save, c4));
+    HIPCHK(hipMalloc(&dq_save, c4));
+    HIPCHK(hipMemcpyAsync(r_save, x->r, c4, hipMemcpyDeviceToDevice, x->stream));
+    HIPCHK(hipMemcpyAsync(dq_save, x->dq, c4, hipMemcpyDeviceToDevice, x->stream));
+    saved = true;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    int k = 0;
+    for (Group& g : x->groups) {
+      double best = -1;
+      if (g.fast) {
+        std::vector<int> cand = {g.n};
+        for (int w : {8, 4, 2, 1})
+          if (w * cu < g.n) cand.push_back(w * cu);
+        std::vector<int> kerns = {kKernRow};
+        if (g.half_ok) kerns.push_back(kKernHalf);
+        if (g.half2_ok) kerns.push_back(kKernHalf2);
+        int best_kern = g.kern, best_grid = g.grid, best_ncr = g.ncr;
+        const int ncr0 = std::min(std::max(g.ncr, 1), kMaxCR);
+        for (int kern : kerns) {
+          g.kern = kern;
+          double kbest = -1;
+          int kgrid = cand[0];
+          auto trial = [&](int grid, int ncr) -> int {
             int rc = launch_group(x, g, grid, 0, ncr);  // warm-up
             if (rc) return rc;
             HIPCHK(hipEventRecord(e0, x->stream));
@@ -4266,26 +4275,47 @@ int hamstr_autotune(hamstr_ctx* x, int reps, double* best_ms, int cap, int* ngro
             float ms = 0;
             HIPCHK(hipEventElapsedTime(&ms, e0, e1));
             const double t = ms / reps;
+            if (kbest < 0 || t < kbest) {
+              kbest = t;
+              kgrid = grid;
+            }
             if (best < 0 || t < best) {
               best = t;
-              g.grid = grid;
-              g.ncr = ncr;
+              best_grid = grid;
+              best_ncr = ncr;
               best_kern = kern;
             }
-          }
+            return HAMSTR_OK;
+          };
+          int rc;
+          for (int grid : cand)
+            if ((rc = trial(grid, ncr0))) return rc;
+          // the half-LDS kernels always run the first reduction level
+          const int grid = kgrid;
+          for (int ncr = kern == kKernRow ? 0 : 1; ncr <= kMaxCR; ncr++)
+            if (ncr != ncr0 && (rc = trial(grid, ncr))) return rc;
+        }
+        g.kern = best_kern;
+        g.grid = best_grid;
+        g.ncr = best_ncr;
       }
-      g.kern = best_kern;
+      if (best_ms && k < cap) best_ms[k] = best;
+      k++;
     }
-    if (best_ms && k < cap) best_ms[k] = best;
-    k++;
+    return HAMSTR_OK;
+  };
+  const int rc = search();
+  if (saved) {
+    (void)hipMemcpyAsync(x->r, r_save, c4, hipMemcpyDeviceToDevice, x->stream);
+    (void)hipMemcpyAsync(x->dq, dq_save, c4, hipMemcpyDeviceToDevice, x->stream);
   }
-  HIPCHK(hipMemcpyAsync(x->r, r_save, c4, hipMemcpyDeviceToDevice, x->stream));
-  HIPCHK(hipMemcpyAsync(x->dq, dq_save, c4, hipMemcpyDeviceToDevice, x->stream));
-  HIPCHK(hipStreamSynchronize(x->stream));
-  (void)hipFree(r_save);
-  (void)hipFree(dq_save);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  const hipError_t se = hipStreamSynchronize(x->stream);
+  if (r_save) (void)hipFree(r_save);
+  if (dq_save) (void)hipFree(dq_save);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  HIPCHK(se);
   if (ngroups) *ngroups = (int)x->groups.size();
   return HAMSTR_OK;
 }

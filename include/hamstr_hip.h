@@ -159,8 +159,10 @@ long hamstr_iterations(const hamstr_ctx *ctx);
  * levels of cyclic reduction before the parallel cyclic reduction and, where
  * the group's chains fit it, both one-row-per-thread kernels (all rows in
  * LDS, or half of them: HAMSTR_KERNEL_*), on the current state; keeps the
- * fastest and restores r and dq.  best_ms[k] (k < cap) gets group k's best
- * time (-1 for groups with nothing to tune). */
+ * fastest and restores r and dq (on an error as well).  Per kernel it times
+ * every grid at the group's current depth, then every other depth at the
+ * best grid.  best_ms[k] (k < cap) gets group k's best time (-1 for groups
+ * with nothing to tune). */
 int hamstr_autotune(hamstr_ctx *ctx, int reps, double *best_ms, int cap, int *ngroups);
 /* ADI kernels of a launch group */
 #define HAMSTR_KERNEL_EXACT 0 /* one thread per chain, the reference's order */
