@@ -247,3 +247,25 @@ def test_time_batch_iteration_runs(hip, lhs):
     assert out["cases"] == 6 and out["lhs"] == lhs and out["ms_per_iter"] > 0
     assert {"rhs_faces", "adi_phase0", "adi_phase1", "lin_faces"} <= set(out["stage_ms"])
     assert {g["phase"] for g in out["groups"]} == {0, 1}
+
+
+@pytest.mark.parametrize("lhs", ["fp64", "fp32"])
+def test_large_batch_matches_single_cases(hip, lhs):
+    """72 cases of a 256x128 mesh (2.4M cells, 4.7M faces, 18,432 radial
+    lines): cases stay bit-identical to their single-case contexts at a size
+    where every launch of the iteration spans the whole chip many times."""
+    from hamstr_oop_amd import meshgen
+    lh = hip.LHS_FP32 if lhs == "fp32" else hip.LHS_FP64
+    mesh = meshgen.naca_ogrid(256, 128, t=0.12)
+    machs = np.linspace(0.3, 0.6, 72)
+    b = hip.Batch([(mesh, float(m), 1.0) for m in machs], order=3, cfl=10.0, lhs=lh, adi_mode=hip.ADI_FAST)
+    assert int(b.cell_off[-1]) > 8192 * 256
+    n, _, norms = b.batch_step(6, check_every=6)
+    assert n == 6
+    for k in (0, 37, 71):
+        s = hip.Solver(mesh, float(machs[k]), 1.0, 3, 10.0, lhs=lh, adi_mode=hip.ADI_FAST)
+        _, l2, _ = s.step(6, check_every=6)
+        assert np.array_equal(b.case_q(k).ravel(), s.get("q")), k
+        assert norms[k, 0] == pytest.approx(l2[-1], rel=1e-12), k  # per-case and block-partial sums
+        s.close()
+    b.close()
