@@ -3930,17 +3930,20 @@ int build_graph(hamstr_ctx* x, int K) {
 // after every case has converged (tol > 0).  *done = steps executed.
 int run_steps(hamstr_ctx* x, int nsteps, double tol, int check_every, std::vector<int>& iters,
               std::vector<double>& series, int* done) {
-  if (nsteps < 0) return fail(HAMSTR_ERR_ARG, "bad arguments");
-  if (check_every < 1) check_every = 1;
   const int nc = x->nCase;
+  // the outputs are sized before any early return, so callers may index them
+  iters.assign(nc, -1);
+  if (done) *done = 0;
+  if (nsteps < 0) return fail(HAMSTR_ERR_ARG, "bad arguments");
+  series.assign(4 * (size_t)nc * nsteps, 0.0);
+  if (check_every < 1) check_every = 1;
   if (x->histCap < nsteps) {
     if (x->hist) (void)hipFree(x->hist);
     x->hist = nullptr;
+    x->histCap = 0;  // a failed allocation leaves no capacity behind
     HIPCHK(hipMalloc(&x->hist, sizeof(double) * 4 * nc * std::max(nsteps, 1)));
     x->histCap = nsteps;
   }
-  iters.assign(nc, -1);
-  series.assign(4 * (size_t)nc * nsteps, 0.0);
   int rc = HAMSTR_OK;
   std::vector<hipEvent_t> evs;
   int nstage = 0;
@@ -4095,7 +4098,7 @@ int hamstr_batch_step(hamstr_ctx* x, int nsteps, double tol, int check_every, in
   std::vector<double> series;
   int ran = 0;
   const int rc = run_steps(x, nsteps, tol, check_every, it, series, &ran);
-  if (iters)
+  if (iters && (int)it.size() == x->nCase)
     for (int k = 0; k < x->nCase; k++) iters[k] = it[k];
   if (norms && ran > 0) std::memcpy(norms, x->case_norm.data(), sizeof(double) * 4 * x->nCase);
   if (done) *done = ran;

@@ -1,4 +1,6 @@
 """Small meshes and random inputs shared by the tests."""
+import hashlib
+
 import numpy as np
 
 from hamstr_oop_amd import meshgen
@@ -8,6 +10,16 @@ SMALL_CASES = {
     "circle16x8": (lambda: meshgen.circle_ogrid(16, 8), 0.3, 0.0),
     "naca32x12": (lambda: meshgen.naca_ogrid(32, 12), 0.8, 1.25),
     "naca24x10_a2": (lambda: meshgen.naca_ogrid(24, 10, t=0.10), 0.5, 2.0),
+}
+
+
+# fixtures of the sweep loop (msweep > 1, ADI.cpp:54-252): name -> (mesh
+# factory, Mach, alpha, order); tests/golden/msweep.npz holds the reference's
+# results for msweep 2 and 3
+MSWEEP_CASES = {
+    "circle12x6_o3": (lambda: meshgen.circle_ogrid(12, 6), 0.3, 0.0, 3),
+    "naca16x8_o3": (lambda: meshgen.naca_ogrid(16, 8), 0.8, 1.25, 3),
+    "naca16x8_o5": (lambda: meshgen.naca_ogrid(16, 8), 0.5, 2.0, 5),
 }
 
 
@@ -59,3 +71,13 @@ class GoldenMesh:
         self.n_cell = int(d["q0"].size // 4)
         self.n_face = int(self.faces.shape[0])
         self.n_chain = int(self.chain_start.size - 1)
+
+
+def mesh_digest(mesh) -> str:
+    """SHA-256 of a mesh's arrays, so that a test can tell a changed mesh
+    generator apart from a changed solver."""
+    h = hashlib.sha256()
+    for a in (np.ascontiguousarray(mesh.x, np.float64), np.ascontiguousarray(mesh.faces, np.int32),
+              np.ascontiguousarray(mesh.chain_start, np.int32), np.ascontiguousarray(mesh.chain_conn, np.int32)):
+        h.update(a.tobytes())
+    return h.hexdigest()

@@ -15,11 +15,21 @@ Fixtures (numpy .npz, inputs and outputs only):
                  5-step results of the reference SOLVER; 5 steps of the
                  explicit schemes (stepType "euler" and "rk3", CFL 0.8) from
                  the same state
+  msweep.npz     msweep = 2 and 3 (the sweep loop, ADI.cpp:54-252): one ADI
+                 call and 5 steps from a perturbed state, circle12x6 and
+                 naca16x8 at order 3 (and order 5 on naca16x8, -O0)
+  c1.npz         the 128x64 circle O-grid (first order, M=0.3, alpha=0,
+                 CFL 10): 500 steps of marchInTime from the freestream --
+                 per-step nonlinear and printed (linear) L2 / Linf, q after
+                 100 and 500 steps, and the reference's own seconds
+
+Usage: make_golden.py [unit] [cases] [msweep] [c1]   (no argument: all)
 """
 from __future__ import annotations
 
 import os
 import shutil
+import subprocess
 import sys
 import tempfile
 
@@ -33,7 +43,7 @@ sys.path.insert(0, os.path.dirname(HERE))
 
 from hamstr_oop_amd import meshgen  # noqa: E402
 import oracle_py  # noqa: E402
-from cases import random_block_system, random_normals, random_states  # noqa: E402
+from cases import MSWEEP_CASES, mesh_digest, random_block_system, random_normals, random_states  # noqa: E402
 
 THIRD = 0.3333333333333333
 EXPLICIT_CFL = 0.8
@@ -141,30 +151,122 @@ def run_case(ref: oracle_py.Reference, mesh, mach, alpha, order, workdir) -> dic
     return out
 
 
-def main():
+def msweep_case(ref: oracle_py.Reference, mesh, mach, alpha, order, msweep, workdir) -> dict:
+    """One ADI call and 5 steps at msweep sweeps from a perturbed state."""
+    d = tempfile.mkdtemp(dir=workdir)
+    meshgen.write_reference_case(mesh, d, mach, alpha, order, 5, 10.0, msweep=msweep)
+    rc = ref.load(d)
+    assert rc.msweep == msweep
+    rng = np.random.default_rng(1000 * msweep + order * 101 + mesh.n_cell)
+    q = rc.get("q").reshape(-1, 4).copy()
+    n = q.shape[0]
+    q[:, 0] *= 1 + 0.02 * rng.uniform(-1, 1, n)
+    q[:, 1] += 0.02 * rng.uniform(-1, 1, n)
+    q[:, 2] += 0.02 * rng.uniform(-1, 1, n)
+    q[:, 3] *= 1 + 0.02 * rng.uniform(-1, 1, n)
+    q = q.reshape(-1)
+    rc.set_q(q)
+    out = {"q0": q}
+    l2, li = rc.compute_rhs(True)
+    out.update(rhs_l2=l2, rhs_linf=li)
+    l2, li = rc.adi()
+    out.update(adi_q=rc.get("q"), adi_dq=rc.get("dq"), adi_r=rc.get("r"), adi_r0=rc.get("r0"),
+               adi_l2=l2, adi_linf=li)
+    l2r, l2l, _ = rc.step(5)
+    out.update(step5_q=rc.get("q"), step5_dq=rc.get("dq"), step5_l2rhs=l2r, step5_l2lin=l2l)
+    shutil.rmtree(d, ignore_errors=True)
+    return out
+
+
+def msweep_fixture(ref2, ref0, work) -> dict:
+    out = {}
+    for name, (make, mach, alpha, order) in MSWEEP_CASES.items():
+        mesh = make()
+        for msweep in (2, 3):
+            data0 = msweep_case(ref0, mesh, mach, alpha, order, msweep, work)
+            if order != 5:
+                data2 = msweep_case(ref2, mesh, mach, alpha, order, msweep, work)
+                for k in data0:
+                    if not np.array_equal(data0[k], data2[k]):
+                        raise SystemExit(f"msweep {name} {msweep}: reference -O2 and -O0 disagree on {k}")
+            for k, v in data0.items():
+                out[f"{name}_s{msweep}_{k}"] = v
+            print("msweep", name, msweep, flush=True)
+    return out
+
+
+C1 = dict(nc=128, nr=64, mach=0.3, alpha=0.0, order=1, cfl=10.0, steps=500)
+
+
+def c1_run(opt: str, out_path: str) -> None:
+    """Config 1 through the reference build `opt`, in a process of its own: the
+    reference's Jacobians leak about 12 GB over 500 steps of this mesh."""
+    ref = oracle_py.Reference(opt)
+    mesh = meshgen.circle_ogrid(C1["nc"], C1["nr"])
+    work = tempfile.mkdtemp()
+    meshgen.write_reference_case(mesh, work, C1["mach"], C1["alpha"], C1["order"], C1["steps"], C1["cfl"])
+    rc = ref.load(work)
+    l2r1, l2l1, s1 = rc.step(100)
+    q100 = rc.get("q")
+    l2r2, l2l2, s2 = rc.step(C1["steps"] - 100)
+    np.savez_compressed(out_path, q100=q100, q500=rc.get("q"), l2rhs=np.concatenate([l2r1, l2r2]),
+                        l2lin=np.concatenate([l2l1, l2l2]), seconds=s1 + s2)
+    shutil.rmtree(work, ignore_errors=True)
+
+
+def c1_fixture() -> dict:
+    work = tempfile.mkdtemp()
+    res = {}
+    for opt in ("O2", "O0"):
+        path = os.path.join(work, f"c1_{opt}.npz")
+        subprocess.run([sys.executable, os.path.abspath(__file__), "_c1run", opt, path], check=True)
+        res[opt] = dict(np.load(path))
+    for k in ("q100", "q500", "l2rhs", "l2lin"):
+        if not np.array_equal(res["O2"][k], res["O0"][k]):
+            raise SystemExit(f"c1: reference -O2 and -O0 disagree on {k}")
+    mesh = meshgen.circle_ogrid(C1["nc"], C1["nr"])
+    out = {k: res["O2"][k] for k in ("q100", "q500", "l2rhs", "l2lin")}
+    out.update({k: np.array(v) for k, v in C1.items()})
+    out["mesh_sha256"] = np.array(mesh_digest(mesh))
+    out["ref_seconds_O2"] = res["O2"]["seconds"]
+    shutil.rmtree(work, ignore_errors=True)
+    print(f"c1: reference -O2 took {float(res['O2']['seconds']):.2f} s for {C1['steps']} steps", flush=True)
+    return out
+
+
+def main(what):
     if not (oracle_py.Reference.available("O2") and oracle_py.Reference.available("O0")):
         oracle_py.build(reference=True)
     ref2 = oracle_py.Reference("O2")
     ref0 = oracle_py.Reference("O0")
     work = tempfile.mkdtemp()
-    u2 = unit_vectors(ref2, ref0)
-    u0 = unit_vectors(ref0, ref0)
-    for k in u2:
-        if not np.array_equal(u2[k], u0[k]):
-            raise SystemExit(f"reference -O2 and -O0 disagree on {k}")
-    np.savez_compressed(os.path.join(HERE, "unit.npz"), **u2)
-    for name, make, mach, alpha, order in CASES:
-        mesh = make()
-        data0 = run_case(ref0, mesh, mach, alpha, order, work)
-        if order != 5:
-            data2 = run_case(ref2, mesh, mach, alpha, order, work)
-            for k in data0:
-                if not np.array_equal(data0[k], data2[k]):
-                    raise SystemExit(f"{name}: reference -O2 and -O0 disagree on {k}")
-        np.savez_compressed(os.path.join(HERE, f"case_{name}.npz"), **data0)
-        print("wrote", name, flush=True)
+    if "unit" in what:
+        u2 = unit_vectors(ref2, ref0)
+        u0 = unit_vectors(ref0, ref0)
+        for k in u2:
+            if not np.array_equal(u2[k], u0[k]):
+                raise SystemExit(f"reference -O2 and -O0 disagree on {k}")
+        np.savez_compressed(os.path.join(HERE, "unit.npz"), **u2)
+    if "cases" in what:
+        for name, make, mach, alpha, order in CASES:
+            mesh = make()
+            data0 = run_case(ref0, mesh, mach, alpha, order, work)
+            if order != 5:
+                data2 = run_case(ref2, mesh, mach, alpha, order, work)
+                for k in data0:
+                    if not np.array_equal(data0[k], data2[k]):
+                        raise SystemExit(f"{name}: reference -O2 and -O0 disagree on {k}")
+            np.savez_compressed(os.path.join(HERE, f"case_{name}.npz"), **data0)
+            print("wrote", name, flush=True)
+    if "msweep" in what:
+        np.savez_compressed(os.path.join(HERE, "msweep.npz"), **msweep_fixture(ref2, ref0, work))
+    if "c1" in what:
+        np.savez_compressed(os.path.join(HERE, "c1.npz"), **c1_fixture())
     shutil.rmtree(work, ignore_errors=True)
 
 
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "_c1run":
+        c1_run(sys.argv[2], sys.argv[3])
+    else:
+        main(set(sys.argv[1:]) or {"unit", "cases", "msweep", "c1"})

@@ -269,3 +269,20 @@ def test_large_batch_matches_single_cases(hip, lhs):
         assert norms[k, 0] == pytest.approx(l2[-1], rel=1e-12), k  # per-case and block-partial sums
         s.close()
     b.close()
+
+
+def test_batch_step_rejects_negative_steps(hip):
+    """A negative step count returns HAMSTR_ERR_ARG before any launch, with the
+    per-case outputs still written (no case converged, nothing run), and the
+    context keeps working afterwards."""
+    import ctypes as C
+    b = hip.Batch(_cases()[:2], order=3, cfl=10.0, adi_mode=hip.ADI_FAST)
+    iters = np.full(b.n_case, 7, np.int32)
+    norms = np.zeros((b.n_case, 4))
+    done = C.c_int(5)
+    rc = hip.lib().hamstr_batch_step(b.h, -1, C.c_double(0.0), 1, hip._i(iters), hip._d(norms), C.byref(done))
+    assert rc == hip.ERR_ARG
+    assert done.value == 0 and np.array_equal(iters, np.full(b.n_case, -1))
+    n, it, _ = b.batch_step(3, check_every=3)
+    assert n == 3 and np.array_equal(it, np.full(b.n_case, -1))
+    b.close()

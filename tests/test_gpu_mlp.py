@@ -141,17 +141,19 @@ def test_trained_warm_start_cuts_iterations(ws):
     conv = (cit > 0) & (wit > 0)
     assert conv.sum() >= 3 and np.array_equal(cit > 0, wit > 0), (cit, wit)
     assert wit[conv].sum() < cit[conv].sum()
-    # deep convergence: the same fixed point from either start (a case that
-    # settles into a limit cycle above 1e-13 has no single fixed point to compare)
+    # deep convergence: the same fixed point from either start.  Each decade
+    # of the norm moves these fields about tenfold (1e-13 leaves ~1e-7), so the
+    # runs go on to 1e-17 before the fields are compared at 1e-10.  A case that
+    # settles into a limit cycle above that has no single fixed point to compare.
     deep = []
     for bb in (cold, warm):
-        _, it_deep, _ = bb.batch_step(30000, tol=1e-13, check_every=100)
+        _, it_deep, _ = bb.batch_step(40000, tol=1e-17, check_every=100)
         deep.append(it_deep > 0)
     both = [k for k in range(len(held)) if deep[0][k] and deep[1][k]]
     assert len(both) >= 3, deep
     for k in both:
         qc = cold.case_q(k)
         d = (np.abs(warm.case_q(k) - qc) / np.abs(qc).max(axis=0)).max()
-        print(f"case {k}: deep-converged warm vs cold max rel diff {d:.2e}", flush=True)
-        assert d < 1e-4, (k, d)
+        print(f"case {k}: deep-converged (1e-17) warm vs cold max scaled diff {d:.2e}", flush=True)
+        assert d <= 1e-10, (k, d)
     mlp.close()

@@ -83,25 +83,35 @@ def test_orphan_step_bitwise(hip, oracle, order):
 
 def test_orphan_fast_matches_exact_and_converges(hip):
     """FAST (parallel cyclic reduction) against EXACT on a triangulated
-    O-grid: the first 20 steps agree to rounding, and the FAST run converges
-    to 1e-10 at the EXACT run's fixed point."""
+    O-grid: the first 20 steps agree to rounding; both converge to 1e-10 in
+    the same number of steps with fields within 1e-10 (scaled per component),
+    and driven on to 1e-13 they still agree to 1e-10 -- the fixed point is
+    the residual's, which both modes compute bit for bit (ADI.cpp:258-264
+    over computeRHS.cpp:415-425).  Measured: 4e-15 at every depth."""
     from hamstr_oop_amd import meshgen
     mesh = meshgen.naca_ogrid(64, 24, tri_layers=8)
     runs = {}
     for mode in (hip.ADI_EXACT, hip.ADI_FAST):
+        print(f"orphans FAST vs EXACT: mode {mode}", flush=True)
         s = hip.Solver(mesh, 0.5, 2.0, 3, 10.0, adi_mode=mode)
         _, l2a, _ = s.step(20, check_every=10)
         q20 = s.get("q")
         n, l2, _ = s.step(4000, tol=1e-10, check_every=50)
-        runs[mode] = (q20, l2a, s.get("q"), l2[-1], n)
+        q10 = s.get("q")
+        n13, l213, _ = s.step(8000, tol=1e-13, check_every=50)
+        runs[mode] = (q20, l2a, q10, l2[-1], n, s.get("q"), l213[-1], n13)
         s.close()
-    qe, le, qe_end, ende, ne = runs[hip.ADI_EXACT]
-    qf, lf, qf_end, endf, nf = runs[hip.ADI_FAST]
+    qe, le, qe10, ende, ne, qe13, ende13, ne13 = runs[hip.ADI_EXACT]
+    qf, lf, qf10, endf, nf, qf13, endf13, nf13 = runs[hip.ADI_FAST]
     np.testing.assert_allclose(lf, le, rtol=1e-9)
     scale = np.abs(qe.reshape(-1, 4)).max(axis=0)
     assert (np.abs(qf - qe).reshape(-1, 4) / scale).max() < 1e-10
-    assert ende <= 1e-10 and endf <= 1e-10, (ende, endf, ne, nf)
-    assert (np.abs(qf_end - qe_end).reshape(-1, 4) / scale).max() < 1e-8
+    assert ende <= 1e-10 and endf <= 1e-10 and ne == nf, (ende, endf, ne, nf)
+    d10 = (np.abs(qf10 - qe10).reshape(-1, 4) / scale).max()
+    assert ende13 <= 1e-13 and endf13 <= 1e-13 and ne13 == nf13, (ende13, endf13, ne13, nf13)
+    d13 = (np.abs(qf13 - qe13).reshape(-1, 4) / scale).max()
+    print(f"FAST vs EXACT: {d10:.2e} at 1e-10, {d13:.2e} at 1e-13", flush=True)
+    assert d10 <= 1e-10 and d13 <= 1e-10
 
 
 @pytest.mark.parametrize("scheme", ["euler", "rk3"])
