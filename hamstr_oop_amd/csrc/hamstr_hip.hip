@@ -27,6 +27,7 @@
 #include <cstdio>
 #include <cstring>
 #include <functional>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -4521,9 +4522,56 @@ static int unit_tridag(int periodic, int nsys, int N, const double* a, const dou
   return HAMSTR_OK;
 }
 
+// ---- block solves that allocate nothing -------------------------------------
+// The scratch rows of k_unit_tridag (C', F' and the spike column, 36 doubles
+// per block row) live in this library-owned array instead of a hipMalloc per
+// call.  Each GPU holds its own copy for the life of the process; every call
+// on one GPU shares it, so calls that overlap on one GPU race.  The kernel
+// writes every scratch row before it reads it, so nothing needs clearing.
+static __device__ __align__(256) double scratch[36 * (size_t)HAMSTR_NOALLOC_MAX_ROWS];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<double*> g_scratch_dev;  // device address of `scratch`, per device
+
+// The device address of `scratch` on the current device, looked up once per
+// device.  Host code uses only this address: the array's name is a host-side
+// stand-in.
+int scratch_address(double** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((int)g_scratch_dev.size() <= dev) g_scratch_dev.resize(dev + 1, nullptr);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(HAMSTR_ERR_HIP, "hipGetSymbolAddress(scratch) gave a null address");
+    g_scratch_dev[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return HAMSTR_OK;
+}
+}  // namespace
+
 extern "C" {
 int hamstr_block_tridag4(int nsys, int N, const double* a, const double* b, const double* c, double* f) {
   return unit_tridag(0, nsys, N, a, b, c, f);
+}
+
+int hamstr_block_tridag4_noalloc(int periodic, int nsys, int N, const double* a, const double* b, const double* c,
+                                 double* f, void* stream) {
+  if ((periodic != 0 && periodic != 1) || nsys <= 0 || N < (periodic ? 3 : 1) || !a || !b || !c || !f)
+    return fail(HAMSTR_ERR_ARG, "bad arguments");
+  if ((long long)nsys * N > HAMSTR_NOALLOC_MAX_ROWS)
+    return fail(HAMSTR_ERR_ARG, "nsys*N = " + std::to_string((long long)nsys * N) + " block rows exceed the " +
+                                    std::to_string(HAMSTR_NOALLOC_MAX_ROWS) + " the allocation-free solve holds");
+  double* S = nullptr;
+  const int rc = scratch_address(&S);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_unit_tridag, dim3(nblk(nsys, 64)), dim3(64), 0, static_cast<hipStream_t>(stream), nsys, N,
+                     periodic, a, b, c, f, S);
+  HIPCHK(hipGetLastError());
+  return HAMSTR_OK;
 }
 
 int hamstr_block_tridag_periodic4(int nsys, int N, const double* a, const double* b, const double* c, double* f) {

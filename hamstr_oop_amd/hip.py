@@ -41,7 +41,7 @@ EXPORTS = [
     "hamstr_mlp_set_params", "hamstr_mlp_forward", "hamstr_mlp_loss_grad", "hamstr_mlp_set_data",
     "hamstr_mlp_train", "hamstr_warm_start", "hamstr_mlp_last_error", "hamstr_mlp_set_graph",
     "hamstr_update_solution", "hamstr_set_scheme", "hamstr_get_scheme",
-    "hamstr_write_solution", "hamstr_read_solution",
+    "hamstr_write_solution", "hamstr_read_solution", "hamstr_block_tridag4_noalloc",
 ]
 
 _dp = C.POINTER(C.c_double)

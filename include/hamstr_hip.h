@@ -316,6 +316,20 @@ int hamstr_block_tridag4(int nsys, int N, const double *a, const double *b, cons
 int hamstr_block_tridag_periodic4(int nsys, int N, const double *a, const double *b,
                                   const double *c, double *f);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): the solve of
+ * hamstr_block_tridag4 (periodic = 0) or hamstr_block_tridag_periodic4
+ * (periodic = 1) on DEVICE pointers a, b, c [nsys*N*16], f [nsys*N*4] (f
+ * overwritten), enqueued on `stream` (a hipStream_t; NULL: the null stream)
+ * without synchronising and without any allocation.  Its scratch rows live in
+ * a library-owned device array of HAMSTR_NOALLOC_MAX_ROWS * 36 doubles
+ * (4.5 MiB) on each GPU, held until the process ends and shared by every call
+ * on that GPU: calls that overlap on one GPU race, so issue them on one
+ * stream.  nsys*N above HAMSTR_NOALLOC_MAX_ROWS is HAMSTR_ERR_ARG. */
+#define HAMSTR_NOALLOC_MAX_ROWS 16384
+int hamstr_block_tridag4_noalloc(int periodic, int nsys, int N, const double *a, const double *b,
+                                 const double *c, double *f, void *stream);
+
 /* Text of the last error on this thread. */
 const char *hamstr_last_error(void);
 
