@@ -33,6 +33,7 @@
 
 #include "../../include/hamstr_hip.h"
 #include "device_math.hpp"
+#include "strand_math.hpp"
 
 using namespace hamstr;
 
@@ -2412,6 +2413,37 @@ __global__ void k_unit_recon(int kind, int nchain, int imax, int is, int iend, d
   qr[((size_t)ch * imax + i) * 4 + k] = b;
 }
 
+// the 3D / strand building blocks, thread per face or per system
+__global__ void k_unit_flux3d(int n, const double* nv, const double* ql, const double* qr, double gamma,
+                              double* flux, double* sr) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double f[5], s;
+  hamstr::flux_roe3d(nv + 3 * (size_t)i, ql + 5 * (size_t)i, qr + 5 * (size_t)i, gamma, f, &s);
+  for (int k = 0; k < 5; k++) flux[5 * (size_t)i + k] = f[k];
+  sr[i] = s;
+}
+
+__global__ void k_unit_jac3d(int n, const double* nv, const double* ql, const double* qr, double gamma, double* lm,
+                             double* rm) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double L[25], R[25];
+  hamstr::jacobian_roe3d(nv + 3 * (size_t)i, ql + 5 * (size_t)i, qr + 5 * (size_t)i, gamma, L, R);
+  for (int k = 0; k < 25; k++) {
+    lm[25 * (size_t)i + k] = L[k];
+    rm[25 * (size_t)i + k] = R[k];
+  }
+}
+
+// in place on the device copies of a, b, c, f
+__global__ void k_unit_tridag5(int nsys, int N, int periodic, double* a, double* b, double* c, double* f) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nsys) return;
+  const size_t m = (size_t)t * N;
+  hamstr::block_tridag5(a + 25 * m, b + 25 * m, c + 25 * m, f + 5 * m, N, periodic != 0);
+}
+
 // blockTridag4 / blockTridagPeriodic4 on stored systems, thread per system
 __global__ void k_unit_tridag(int nsys, int N, int periodic, const double* a_, const double* b_,
                               const double* c_, double* f_, double* scratch) {
@@ -4521,6 +4553,69 @@ static int unit_tridag(int periodic, int nsys, int N, const double* a, const dou
   HIPCHK(hipMemcpy(f, df, sizeof(double) * 4 * m, hipMemcpyDeviceToHost));
   return HAMSTR_OK;
 }
+
+// ---- the 3D / strand building blocks ------------------------------------------
+extern "C" {
+int hamstr_flux_roe3d(int n, const double* nxyz, const double* ql, const double* qr, double gamma, double* flux,
+                      double* sr) {
+  if (n <= 0 || !nxyz || !ql || !qr || !flux || !sr) return fail(HAMSTR_ERR_ARG, "bad arguments");
+  Scoped s;
+  double *dn, *dl, *dr, *df, *ds;
+  int rc;
+  if ((rc = s.up(&dn, nxyz, 3 * (size_t)n)) || (rc = s.up(&dl, ql, 5 * (size_t)n)) ||
+      (rc = s.up(&dr, qr, 5 * (size_t)n)) || (rc = s.up(&df, (const double*)nullptr, 5 * (size_t)n)) ||
+      (rc = s.up(&ds, (const double*)nullptr, (size_t)n)))
+    return rc;
+  hipLaunchKernelGGL(k_unit_flux3d, dim3(nblk(n)), dim3(kBlock), 0, 0, n, dn, dl, dr, gamma, df, ds);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(flux, df, sizeof(double) * 5 * n, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(sr, ds, sizeof(double) * n, hipMemcpyDeviceToHost));
+  return HAMSTR_OK;
+}
+
+int hamstr_jacobian_roe3d(int n, const double* nxyz, const double* ql, const double* qr, double gamma,
+                          double* lmat, double* rmat) {
+  if (n <= 0 || !nxyz || !ql || !qr || !lmat || !rmat) return fail(HAMSTR_ERR_ARG, "bad arguments");
+  Scoped s;
+  double *dn, *dl, *dr, *dlm, *drm;
+  int rc;
+  if ((rc = s.up(&dn, nxyz, 3 * (size_t)n)) || (rc = s.up(&dl, ql, 5 * (size_t)n)) ||
+      (rc = s.up(&dr, qr, 5 * (size_t)n)) || (rc = s.up(&dlm, (const double*)nullptr, 25 * (size_t)n)) ||
+      (rc = s.up(&drm, (const double*)nullptr, 25 * (size_t)n)))
+    return rc;
+  hipLaunchKernelGGL(k_unit_jac3d, dim3(nblk(n)), dim3(kBlock), 0, 0, n, dn, dl, dr, gamma, dlm, drm);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(lmat, dlm, sizeof(double) * 25 * n, hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(rmat, drm, sizeof(double) * 25 * n, hipMemcpyDeviceToHost));
+  return HAMSTR_OK;
+}
+}  // extern "C"
+
+static int unit_tridag5(int periodic, int nsys, int N, const double* a, const double* b, const double* c,
+                        double* f) {
+  if (nsys <= 0 || N < (periodic ? 3 : 1) || !a || !b || !c || !f) return fail(HAMSTR_ERR_ARG, "bad arguments");
+  Scoped s;
+  double *da, *db, *dc, *df;
+  int rc;
+  const size_t m = (size_t)nsys * N;
+  if ((rc = s.up(&da, a, 25 * m)) || (rc = s.up(&db, b, 25 * m)) || (rc = s.up(&dc, c, 25 * m)) ||
+      (rc = s.up(&df, f, 5 * m)))
+    return rc;
+  hipLaunchKernelGGL(k_unit_tridag5, dim3(nblk(nsys, 64)), dim3(64), 0, 0, nsys, N, periodic, da, db, dc, df);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpy(f, df, sizeof(double) * 5 * m, hipMemcpyDeviceToHost));
+  return HAMSTR_OK;
+}
+
+extern "C" {
+int hamstr_block_tridag5(int nsys, int N, const double* a, const double* b, const double* c, double* f) {
+  return unit_tridag5(0, nsys, N, a, b, c, f);
+}
+
+int hamstr_block_tridag_periodic5(int nsys, int N, const double* a, const double* b, const double* c, double* f) {
+  return unit_tridag5(1, nsys, N, a, b, c, f);
+}
+}  // extern "C"
 
 // ---- block solves that allocate nothing -------------------------------------
 // The scratch rows of k_unit_tridag (C', F' and the spike column, 36 doubles

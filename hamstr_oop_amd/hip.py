@@ -42,6 +42,7 @@ EXPORTS = [
     "hamstr_mlp_train", "hamstr_warm_start", "hamstr_mlp_last_error", "hamstr_mlp_set_graph",
     "hamstr_update_solution", "hamstr_set_scheme", "hamstr_get_scheme",
     "hamstr_write_solution", "hamstr_read_solution", "hamstr_block_tridag4_noalloc",
+    "hamstr_flux_roe3d", "hamstr_jacobian_roe3d", "hamstr_block_tridag5", "hamstr_block_tridag_periodic5",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -355,6 +356,46 @@ def block_tridag4(a, b, c, f, periodic=False):
     x = np.array(f, np.float64, copy=True, order="C")
     nsys, N = a.shape[0], a.shape[1]
     fn = lib().hamstr_block_tridag_periodic4 if periodic else lib().hamstr_block_tridag4
+    _chk(fn(nsys, N, _d(a), _d(b), _d(c), _d(x)))
+    return x
+
+
+# ---- the 3D / strand building blocks (5 conserved variables) ----------------
+
+def flux_roe3d(nxyz, ql, qr, gamma=1.4):
+    """fluxRoe3D on n faces: (flux (n, 5), spectral radius (n,)), unit normal."""
+    nxyz = np.ascontiguousarray(nxyz, np.float64).reshape(-1, 3)
+    ql = np.ascontiguousarray(ql, np.float64).reshape(-1, 5)
+    qr = np.ascontiguousarray(qr, np.float64).reshape(-1, 5)
+    n = nxyz.shape[0]
+    flux = np.zeros((n, 5))
+    sr = np.zeros(n)
+    _chk(lib().hamstr_flux_roe3d(n, _d(nxyz), _d(ql), _d(qr), C.c_double(gamma), _d(flux), _d(sr)))
+    return flux, sr
+
+
+def jacobian_roe3d(nxyz, ql, qr, gamma=1.4):
+    """jacobianRoe3D on n faces: (lmat (n, 5, 5), rmat (n, 5, 5))."""
+    nxyz = np.ascontiguousarray(nxyz, np.float64).reshape(-1, 3)
+    ql = np.ascontiguousarray(ql, np.float64).reshape(-1, 5)
+    qr = np.ascontiguousarray(qr, np.float64).reshape(-1, 5)
+    n = nxyz.shape[0]
+    lm = np.zeros((n, 5, 5))
+    rm = np.zeros((n, 5, 5))
+    _chk(lib().hamstr_jacobian_roe3d(n, _d(nxyz), _d(ql), _d(qr), C.c_double(gamma), _d(lm), _d(rm)))
+    return lm, rm
+
+
+def block_tridag5(a, b, c, f, periodic=False):
+    """a, b, c: (nsys, N, 5, 5); f: (nsys, N, 5).  Returns the solution."""
+    a = np.ascontiguousarray(a, np.float64)
+    if a.ndim == 3:
+        a, b, c, f = a[None], np.asarray(b)[None], np.asarray(c)[None], np.asarray(f)[None]
+    b = np.ascontiguousarray(b, np.float64)
+    c = np.ascontiguousarray(c, np.float64)
+    x = np.array(f, np.float64, copy=True, order="C")
+    nsys, N = a.shape[0], a.shape[1]
+    fn = lib().hamstr_block_tridag_periodic5 if periodic else lib().hamstr_block_tridag5
     _chk(fn(nsys, N, _d(a), _d(b), _d(c), _d(x)))
     return x
 

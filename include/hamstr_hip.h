@@ -330,6 +330,27 @@ int hamstr_block_tridag_periodic4(int nsys, int N, const double *a, const double
 int hamstr_block_tridag4_noalloc(int periodic, int nsys, int N, const double *a, const double *b,
                                  const double *c, double *f, void *stream);
 
+/* ---- the 3D / strand building blocks -----------------------------------------
+ * Five conserved variables (rho, rho u, rho v, rho w, E).  The reference's 3D
+ * build does not compile (io.cpp:306), but these functions are compiled in its
+ * 2D build; each is bit-identical to it.
+ * SOLVER::fluxRoe3D (interfaceFlux.cpp:176-319): nxyz[n*3], ql, qr [n*5] ->
+ * flux [n*5], spectral radius sr[n]; the unit-normal flux, NOT scaled by the
+ * face area (unlike fluxRoe2D). */
+int hamstr_flux_roe3d(int n, const double *nxyz, const double *ql, const double *qr, double gamma,
+                      double *flux, double *sr);
+/* SOLVER::jacobianRoe3D, IMODE 1 (jacobianRoe.cpp:92-620): lmat, rmat
+ * [n*25], row-major 5x5, not scaled by the face area. */
+int hamstr_jacobian_roe3d(int n, const double *nxyz, const double *ql, const double *qr, double gamma,
+                          double *lmat, double *rmat);
+/* TRISOLVER::blockTridag / blockTridagPeriodic with nq = 5
+ * (triSolvers.cpp:443-536, 540-766): a, b, c [nsys*N*25], f [nsys*N*5];
+ * f is overwritten with the solution, a, b, c are not modified. */
+int hamstr_block_tridag5(int nsys, int N, const double *a, const double *b, const double *c,
+                         double *f);
+int hamstr_block_tridag_periodic5(int nsys, int N, const double *a, const double *b,
+                                  const double *c, double *f);
+
 /* Text of the last error on this thread. */
 const char *hamstr_last_error(void);
 

@@ -586,6 +586,215 @@ void block_tridag_periodic4(double* a, double* b, double* c, double* f, int N) {
 }
 
 // --------------------------------------------------------------------------
+// The 3D / strand building blocks (5 conserved variables).  The reference's
+// 3D build does not compile (io.cpp:306), but these functions are compiled in
+// its 2D build and are pinned here against it.
+// --------------------------------------------------------------------------
+
+// fluxRoe3D (interfaceFlux.cpp:176-319): Roe flux with the unit normal; unlike
+// fluxRoe2D neither the flux nor the spectral radius is scaled by the area.
+void flux_roe3d(const double* nxyz, const double* ql, const double* qr, double* flux, double* specRadius,
+                double gamma) {
+  double gm1 = gamma - 1.0;
+  double nxd = nxyz[0], nyd = nxyz[1], nzd = nxyz[2];
+  double area = std::sqrt(nxd * nxd + nyd * nyd + nzd * nzd);
+  double invArea = 1. / area;
+  double nx = nxd * invArea, ny = nyd * invArea, nz = nzd * invArea;
+
+  double rol = ql[0];
+  double ul = ql[1] / ql[0];
+  double vl = ql[2] / ql[0];
+  double wl = ql[3] / ql[0];
+  double pl = gm1 * (ql[4] - 0.5 * rol * (ul * ul + vl * vl + wl * wl));
+  double hl = (ql[4] + pl) / rol;
+
+  double ror = qr[0];
+  double ur = qr[1] / qr[0];
+  double vr = qr[2] / qr[0];
+  double wr = qr[3] / qr[0];
+  double pr = gm1 * (qr[4] - 0.5 * ror * (ur * ur + vr * vr + wr * wr));
+  double hr = (qr[4] + pr) / ror;
+
+  double dp = pr - pl;
+  double uconr = ur * nx + vr * ny + wr * nz;
+  double uconl = ul * nx + vl * ny + wl * nz;
+
+  double fact = std::sqrt(ror / rol);
+  double A = 1.0 / (1.0 + fact);
+  double B = fact / (1.0 + fact);
+  double robar = rol * fact;
+  double ubar = ul * A + ur * B;
+  double vbar = vl * A + vr * B;
+  double wbar = wl * A + wr * B;
+  double hbar = hl * A + hr * B;
+  double cbar = gm1 * (hbar - 0.5 * (ubar * ubar + vbar * vbar + wbar * wbar));
+  cbar = std::sqrt(cbar);
+  double uconbar = ubar * nx + vbar * ny + wbar * nz;
+
+  double eig1 = std::fabs(uconbar);
+  double eig2 = std::fabs(uconbar + cbar);
+  double eig3 = std::fabs(uconbar - cbar);
+  *specRadius = eig1 + cbar;
+
+  double term1 = -eig1 + 0.5 * (eig2 + eig3);
+  double term2 = 0.5 * (eig2 - eig3);
+  double del1 = term1 * dp / cbar / cbar + term2 * robar * (uconr - uconl) / cbar;
+  double del2 = term1 * (uconr - uconl) * robar + term2 * dp / cbar;
+
+  double fm[5], fl[5], fr[5];
+  for (int i = 0; i < 5; i++) fm[i] = eig1 * (qr[i] - ql[i]);
+  fm[0] = fm[0] + del1;
+  fm[1] = fm[1] + del1 * ubar + del2 * nx;
+  fm[2] = fm[2] + del1 * vbar + del2 * ny;
+  fm[3] = fm[3] + del1 * wbar + del2 * nz;
+  fm[4] = fm[4] + del1 * hbar + del2 * uconbar;
+
+  fl[0] = rol * uconl;
+  fr[0] = ror * uconr;
+  fl[1] = rol * uconl * ul + nx * pl;
+  fr[1] = ror * uconr * ur + nx * pr;
+  fl[2] = rol * uconl * vl + ny * pl;
+  fr[2] = ror * uconr * vr + ny * pr;
+  fl[3] = rol * uconl * wl + nz * pl;
+  fr[3] = ror * uconr * wr + nz * pr;
+  fl[4] = (ql[4] + pl) * uconl;
+  fr[4] = (qr[4] + pr) * uconr;
+  for (int i = 0; i < 5; i++) flux[i] = 0.5 * (fl[i] + fr[i] - fm[i]);
+}
+
+// 5x5 block tridiagonal solvers (triSolvers.cpp:443-536 blockTridag,
+// :540-766 blockTridagPeriodic).  a, b, c: [N][5][5] row-major, f: [N][5], all
+// overwritten in place as the reference does.  The unpivoted LU keeps the
+// reference's unrolled formulas (:480-509): L[r][k] = b[r][k] - sum_{m<k}
+// L[r][m] U[m][k] and U[r][k] = (b[r][k] - sum_{m<r} L[r][m] U[m][k]) / pivot,
+// with the pivot's reciprocal, every sum taken left to right.
+#define M5(p, i, k, j) p[((i) * 5 + (k)) * 5 + (j)]
+#define V5(p, i, k) p[(i) * 5 + (k)]
+
+struct LU5 {
+  double L[5][5];   // strictly lower part, the original column-0 entries on k = 0
+  double inv[5];    // reciprocal pivots
+  double U[5][5];   // strictly upper part
+};
+
+inline LU5 lu5(const double* b) {
+  LU5 F;
+  for (int r = 0; r < 5; r++) {
+    for (int k = 0; k < r; k++) {
+      double v = b[5 * r + k];
+      for (int m = 0; m < k; m++) v = v - F.L[r][m] * F.U[m][k];
+      F.L[r][k] = v;
+    }
+    double piv = b[5 * r + r];
+    for (int m = 0; m < r; m++) piv = piv - F.L[r][m] * F.U[m][r];
+    F.inv[r] = 1. / piv;
+    for (int k = r + 1; k < 5; k++) {
+      double v = b[5 * r + k];
+      for (int m = 0; m < r; m++) v = v - F.L[r][m] * F.U[m][k];
+      F.U[r][k] = v * F.inv[r];
+    }
+  }
+  return F;
+}
+
+// solve in place for a vector stored with stride s (:511-521)
+inline void lu5_solve(const LU5& F, double* v, int s) {
+  double d[5];
+  for (int r = 0; r < 5; r++) {
+    double x = v[r * s];
+    for (int m = 0; m < r; m++) x = x - F.L[r][m] * d[m];
+    d[r] = x * F.inv[r];
+  }
+  v[4 * s] = d[4];
+  for (int r = 3; r >= 0; r--) {
+    double x = d[r];
+    for (int k = r + 1; k < 5; k++) x = x - F.U[r][k] * v[k * s];
+    v[r * s] = x;
+  }
+}
+
+void block_tridag5(double* a, double* b, double* c, double* f, int N) {
+  for (int i = 0; i < N; i++) {
+    if (i != 0) {
+      for (int k = 0; k < 5; k++)
+        for (int j = 0; j < 5; j++)
+          for (int l = 0; l < 5; l++) M5(b, i, k, j) -= M5(a, i, k, l) * M5(c, i - 1, l, j);
+      for (int k = 0; k < 5; k++)
+        for (int j = 0; j < 5; j++) V5(f, i, k) -= M5(a, i, k, j) * V5(f, i - 1, j);
+    }
+    LU5 F = lu5(&M5(b, i, 0, 0));
+    lu5_solve(F, &V5(f, i, 0), 1);
+    if (i != N - 1)
+      for (int j = 0; j < 5; j++) lu5_solve(F, &M5(c, i, 0, j), 5);
+  }
+  for (int i = N - 2; i >= 0; i--)
+    for (int k = 0; k < 5; k++)
+      for (int j = 0; j < 5; j++) V5(f, i, k) -= M5(c, i, k, j) * V5(f, i + 1, j);
+}
+
+void block_tridag_periodic5(double* a, double* b, double* c, double* f, int N) {
+  double t[5][5];
+  for (int i = 0; i < N; i++) {
+    if (i != 0) {
+      for (int k = 0; k < 5; k++)
+        for (int j = 0; j < 5; j++)
+          for (int l = 0; l < 5; l++) M5(b, i, k, j) -= M5(a, i, k, l) * M5(c, i - 1, l, j);
+      for (int k = 0; k < 5; k++)
+        for (int j = 0; j < 5; j++) V5(f, i, k) -= M5(a, i, k, j) * V5(f, i - 1, j);
+      for (int k = 0; k < 5; k++)
+        for (int j = 0; j < 5; j++) t[k][j] = 0;
+      for (int k = 0; k < 5; k++)
+        for (int j = 0; j < 5; j++)
+          for (int l = 0; l < 5; l++) t[k][j] -= M5(a, i, k, l) * M5(a, i - 1, l, j);
+      for (int k = 0; k < 5; k++)
+        for (int j = 0; j < 5; j++) M5(a, i, k, j) = t[k][j];
+    }
+    LU5 F = lu5(&M5(b, i, 0, 0));
+    lu5_solve(F, &V5(f, i, 0), 1);
+    if (i != N - 2) {
+      for (int j = 0; j < 5; j++) {
+        lu5_solve(F, &M5(c, i, 0, j), 5);
+        lu5_solve(F, &M5(a, i, 0, j), 5);
+      }
+    }
+    if (i == N - 2) {
+      for (int k = 0; k < 5; k++)
+        for (int j = 0; j < 5; j++) M5(c, i, j, k) += M5(a, i, j, k);
+      for (int j = 0; j < 5; j++) lu5_solve(F, &M5(c, i, 0, j), 5);
+      for (int k = 0; k < 5; k++)
+        for (int j = 0; j < 5; j++) M5(a, i, k, j) = M5(c, i, k, j);
+    }
+  }
+  int i = N - 1;
+  for (int k = 0; k < 5; k++)
+    for (int j = 0; j < 5; j++) M5(b, i, j, k) = (j == k) ? 1.0 : 0.0;
+  for (i = 0; i < N - 1; i++) {
+    for (int k = 0; k < 5; k++)
+      for (int j = 0; j < 5; j++) t[k][j] = 0;
+    for (int k = 0; k < 5; k++)
+      for (int j = 0; j < 5; j++)
+        for (int l = 0; l < 5; l++) {
+          t[k][j] -= M5(c, N - 1, k, l) * M5(c, i, l, j);
+          M5(b, N - 1, k, j) -= M5(c, N - 1, k, l) * M5(a, i, l, j);
+        }
+    for (int k = 0; k < 5; k++)
+      for (int j = 0; j < 5; j++) V5(f, N - 1, k) -= M5(c, N - 1, k, j) * V5(f, i, j);
+    for (int k = 0; k < 5; k++)
+      for (int j = 0; j < 5; j++) M5(c, N - 1, k, j) = t[k][j];
+  }
+  i = N - 1;
+  LU5 F = lu5(&M5(b, i, 0, 0));
+  lu5_solve(F, &V5(f, i, 0), 1);
+  i = N - 2;
+  for (int k = 0; k < 5; k++)
+    for (int j = 0; j < 5; j++) V5(f, i, k) -= (M5(c, i, k, j) * V5(f, i + 1, j));
+  for (i = N - 3; i >= 0; i--)
+    for (int k = 0; k < 5; k++)
+      for (int j = 0; j < 5; j++)
+        V5(f, i, k) -= (M5(c, i, k, j) * V5(f, i + 1, j) + M5(a, i, k, j) * V5(f, N - 1, j));
+}
+
+// --------------------------------------------------------------------------
 // Solver state: SOLVER + MESHBLOCK + SOLNBLOCK fields on the hot path.
 // --------------------------------------------------------------------------
 struct Oracle {
@@ -1093,6 +1302,20 @@ void orc_weno_deriv(const double* f, const double* dq, double* ql, double* qr, i
   weno_deriv(f, ql, qr, dq, is, ie, eps, imax, NQ);
 }
 void orc_block_tridag4(double* a, double* b, double* c, double* f, int N) { block_tridag4(a, b, c, f, N); }
+void orc_flux_roe3d(const double* n, const double* ql, const double* qr, double* flux, double* sr, double gamma) {
+  flux_roe3d(n, ql, qr, flux, sr, gamma);
+}
+void orc_jacobian_roe3d(const double* n, const double* ql, const double* qr, double* lmat, double* rmat,
+                        double gamma) {
+  double L[5][5], R[5][5];
+  jacobian_roe3d(n, ql, qr, L, R, gamma);
+  std::memcpy(lmat, L, sizeof L);
+  std::memcpy(rmat, R, sizeof R);
+}
+void orc_block_tridag5(double* a, double* b, double* c, double* f, int N) { block_tridag5(a, b, c, f, N); }
+void orc_block_tridag_periodic5(double* a, double* b, double* c, double* f, int N) {
+  block_tridag_periodic5(a, b, c, f, N);
+}
 void orc_block_tridag_periodic4(double* a, double* b, double* c, double* f, int N) {
   block_tridag_periodic4(a, b, c, f, N);
 }

@@ -118,6 +118,35 @@ class _Unit:
         return f
 
 
+    # ---- the 3D / strand building blocks (5 conserved variables) ----
+    def flux_roe3d(self, n, ql, qr, gamma=1.4):
+        n = np.ascontiguousarray(n, np.float64)
+        ql = np.ascontiguousarray(ql, np.float64)
+        qr = np.ascontiguousarray(qr, np.float64)
+        flux = np.zeros(5)
+        sr = C.c_double(0.0)
+        self.fn("flux_roe3d")(_d(n), _d(ql), _d(qr), _d(flux), C.byref(sr), C.c_double(gamma))
+        return flux, sr.value
+
+    def jacobian_roe3d(self, n, ql, qr, gamma=1.4):
+        n = np.ascontiguousarray(n, np.float64)
+        ql = np.ascontiguousarray(ql, np.float64)
+        qr = np.ascontiguousarray(qr, np.float64)
+        lm = np.zeros(25)
+        rm = np.zeros(25)
+        self.fn("jacobian_roe3d")(_d(n), _d(ql), _d(qr), _d(lm), _d(rm), C.c_double(gamma))
+        return lm.reshape(5, 5), rm.reshape(5, 5)
+
+    def block_tridag5(self, a, b, c, f, periodic=False):
+        a = np.array(a, np.float64, copy=True, order="C")
+        b = np.array(b, np.float64, copy=True, order="C")
+        c = np.array(c, np.float64, copy=True, order="C")
+        f = np.array(f, np.float64, copy=True, order="C")
+        name = "block_tridag_periodic5" if periodic else "block_tridag5"
+        self.fn(name)(_d(a), _d(b), _d(c), _d(f), f.shape[0])
+        return f
+
+
 class Oracle(_Unit):
     """The CPU restatement (liboracle.so)."""
 

@@ -10,8 +10,9 @@
 //
 // Call surface used: MESHBLOCK::readMeshData / setViscosity / preprocess,
 // SOLNBLOCK::readSolutionInputs, SOLVER::{SOLVER, initFlow, computeRHS, ADI,
-// computeLinearRHS, stepSolution, updateSolution, fluxRoe2D, jacobianRoe2D}, RECONSTRUCTION::{MUSCL, WENO,
-// MUSCL_DERIV, WENO_DERIV}, TRISOLVER::{blockTridag4, blockTridagPeriodic4}.
+// computeLinearRHS, stepSolution, updateSolution, fluxRoe2D, jacobianRoe2D, fluxRoe3D, jacobianRoe3D},
+// RECONSTRUCTION::{MUSCL, WENO, MUSCL_DERIV, WENO_DERIV}, TRISOLVER::{blockTridag4, blockTridagPeriodic4,
+// blockTridag, blockTridagPeriodic}.
 // ============================================================================
 #include <unistd.h>
 #include <chrono>
@@ -293,6 +294,60 @@ void ref_block_tridag_periodic4(double* a, double* b, double* c, double* f, int 
   blocks(c, N, c2, c3);
   auto F = rows(f, N);
   TRISOLVER::blockTridagPeriodic4(a3.data(), b3.data(), c3.data(), F.data(), N, 4);
+}
+
+// ---- the 3D / strand building blocks (compiled in the reference's 2D build) ----
+void ref_flux_roe3d(const double* n, const double* ql, const double* qr, double* flux, double* sr,
+                    double gamma) {
+  double nn[3] = {n[0], n[1], n[2]};
+  double l[5] = {ql[0], ql[1], ql[2], ql[3], ql[4]};
+  double r[5] = {qr[0], qr[1], qr[2], qr[3], qr[4]};
+  unit_solver()->fluxRoe3D(nn, l, r, flux, sr, gamma);
+}
+
+void ref_jacobian_roe3d(const double* n, const double* ql, const double* qr, double* lmat, double* rmat,
+                        double gamma) {
+  double nn[3] = {n[0], n[1], n[2]};
+  double l[5] = {ql[0], ql[1], ql[2], ql[3], ql[4]};
+  double r[5] = {qr[0], qr[1], qr[2], qr[3], qr[4]};
+  double* lp[5] = {lmat, lmat + 5, lmat + 10, lmat + 15, lmat + 20};
+  double* rp[5] = {rmat, rmat + 5, rmat + 10, rmat + 15, rmat + 20};
+  unit_solver()->jacobianRoe3D(nn, l, r, lp, rp, gamma);
+}
+
+static void blocks5(double* base, int N, std::vector<double*>& r2, std::vector<double**>& r3) {
+  r2.resize(5 * N);
+  r3.resize(N);
+  for (int i = 0; i < N; i++) {
+    for (int k = 0; k < 5; k++) r2[5 * i + k] = base + 25 * i + 5 * k;
+    r3[i] = &r2[5 * i];
+  }
+}
+
+static std::vector<double*> rows5(double* base, int n) {
+  std::vector<double*> p(n);
+  for (int i = 0; i < n; i++) p[i] = base + 5 * i;
+  return p;
+}
+
+void ref_block_tridag5(double* a, double* b, double* c, double* f, int N) {
+  std::vector<double*> a2, b2, c2;
+  std::vector<double**> a3, b3, c3;
+  blocks5(a, N, a2, a3);
+  blocks5(b, N, b2, b3);
+  blocks5(c, N, c2, c3);
+  auto F = rows5(f, N);
+  TRISOLVER::blockTridag(a3.data(), b3.data(), c3.data(), F.data(), N, 5);
+}
+
+void ref_block_tridag_periodic5(double* a, double* b, double* c, double* f, int N) {
+  std::vector<double*> a2, b2, c2;
+  std::vector<double**> a3, b3, c3;
+  blocks5(a, N, a2, a3);
+  blocks5(b, N, b2, b3);
+  blocks5(c, N, c2, c3);
+  auto F = rows5(f, N);
+  TRISOLVER::blockTridagPeriodic(a3.data(), b3.data(), c3.data(), F.data(), N, 5);
 }
 
 }  // extern "C"

@@ -33,6 +33,30 @@ def random_states(rng, n, rho=(0.5, 1.5), vel=0.8, p=(0.4, 1.6), gamma=1.4):
     return np.stack([r, r * u, r * v, E], axis=1)
 
 
+def random_states5(rng, n, rho=(0.5, 1.5), vel=0.8, p=(0.4, 1.6), gamma=1.4):
+    """n 3D conserved states (rho, rho u, rho v, rho w, E) with positive pressure."""
+    r = rng.uniform(*rho, n)
+    u, v, w = (rng.uniform(-vel, vel, n) for _ in range(3))
+    pr = rng.uniform(*p, n)
+    E = pr / (gamma - 1.0) + 0.5 * r * (u * u + v * v + w * w)
+    return np.stack([r, r * u, r * v, r * w, E], axis=1)
+
+
+def random_normals3(rng, n):
+    """n face normals in 3D with magnitudes in [0.01, 2]."""
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return d * rng.uniform(0.01, 2.0, (n, 1))
+
+
+def random_block_system5(rng, nsys, N, dominance=7.0):
+    a = rng.uniform(-1, 1, (nsys, N, 5, 5))
+    b = rng.uniform(-1, 1, (nsys, N, 5, 5)) + dominance * np.eye(5)
+    c = rng.uniform(-1, 1, (nsys, N, 5, 5))
+    f = rng.uniform(-1, 1, (nsys, N, 5))
+    return a, b, c, f
+
+
 def random_normals(rng, n):
     ang = rng.uniform(0, 2 * np.pi, n)
     mag = rng.uniform(0.01, 2.0, n)

@@ -15,6 +15,9 @@ Fixtures (numpy .npz, inputs and outputs only):
                  5-step results of the reference SOLVER; 5 steps of the
                  explicit schemes (stepType "euler" and "rk3", CFL 0.8) from
                  the same state
+  unit3d.npz     the 3D / strand building blocks (compiled in the reference's
+                 2D build): fluxRoe3D, jacobianRoe3D (5x5), blockTridag and
+                 blockTridagPeriodic (5x5 blocks)
   msweep.npz     msweep = 2 and 3 (the sweep loop, ADI.cpp:54-252): one ADI
                  call and 5 steps from a perturbed state, circle12x6 and
                  naca16x8 at order 3 (and order 5 on naca16x8, -O0)
@@ -23,7 +26,7 @@ Fixtures (numpy .npz, inputs and outputs only):
                  per-step nonlinear and printed (linear) L2 / Linf, q after
                  100 and 500 steps, and the reference's own seconds
 
-Usage: make_golden.py [unit] [cases] [msweep] [c1]   (no argument: all)
+Usage: make_golden.py [unit] [unit3d] [cases] [msweep] [c1]   (no argument: all)
 """
 from __future__ import annotations
 
@@ -43,7 +46,8 @@ sys.path.insert(0, os.path.dirname(HERE))
 
 from hamstr_oop_amd import meshgen  # noqa: E402
 import oracle_py  # noqa: E402
-from cases import MSWEEP_CASES, mesh_digest, random_block_system, random_normals, random_states  # noqa: E402
+from cases import (MSWEEP_CASES, mesh_digest, random_block_system, random_block_system5,  # noqa: E402
+                   random_normals, random_normals3, random_states, random_states5)
 
 THIRD = 0.3333333333333333
 EXPLICIT_CFL = 0.8
@@ -105,6 +109,32 @@ def unit_vectors(ref: oracle_py.Reference, ref_weno: oracle_py.Reference) -> dic
         for N in (3, 6, 11):
             a, b, c, rhs = random_block_system(rng, 2, N)
             x = np.stack([ref.block_tridag4(a[s], b[s], c[s], rhs[s], periodic=periodic) for s in range(2)])
+            out[f"{tag}{N}_a"], out[f"{tag}{N}_b"], out[f"{tag}{N}_c"] = a, b, c
+            out[f"{tag}{N}_f"], out[f"{tag}{N}_x"] = rhs, x
+    return out
+
+
+def unit3d_vectors(ref: oracle_py.Reference) -> dict:
+    rng = np.random.default_rng(20261204)
+    out = {}
+    n = 64
+    nv = random_normals3(rng, n)
+    ql = random_states5(rng, n)
+    qr = random_states5(rng, n)
+    ql[:8] = qr[:8]  # equal states: the consistency case
+    flux = np.zeros((n, 5))
+    sr = np.zeros(n)
+    lm = np.zeros((n, 5, 5))
+    rm = np.zeros((n, 5, 5))
+    for i in range(n):
+        flux[i], sr[i] = ref.flux_roe3d(nv[i], ql[i], qr[i])
+        lm[i], rm[i] = ref.jacobian_roe3d(nv[i], ql[i], qr[i])
+    out.update(flux_n=nv, flux_ql=ql, flux_qr=qr, flux=flux, flux_sr=sr, jac_l=lm, jac_r=rm)
+    for periodic in (False, True):
+        tag = "per" if periodic else "tri"
+        for N in (3, 6, 11):
+            a, b, c, rhs = random_block_system5(rng, 2, N)
+            x = np.stack([ref.block_tridag5(a[s], b[s], c[s], rhs[s], periodic=periodic) for s in range(2)])
             out[f"{tag}{N}_a"], out[f"{tag}{N}_b"], out[f"{tag}{N}_c"] = a, b, c
             out[f"{tag}{N}_f"], out[f"{tag}{N}_x"] = rhs, x
     return out
@@ -247,6 +277,13 @@ def main(what):
             if not np.array_equal(u2[k], u0[k]):
                 raise SystemExit(f"reference -O2 and -O0 disagree on {k}")
         np.savez_compressed(os.path.join(HERE, "unit.npz"), **u2)
+    if "unit3d" in what:
+        u2 = unit3d_vectors(ref2)
+        u0 = unit3d_vectors(ref0)
+        for k in u2:
+            if not np.array_equal(u2[k], u0[k]):
+                raise SystemExit(f"reference -O2 and -O0 disagree on {k} (3D)")
+        np.savez_compressed(os.path.join(HERE, "unit3d.npz"), **u2)
     if "cases" in what:
         for name, make, mach, alpha, order in CASES:
             mesh = make()
@@ -269,4 +306,4 @@ if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "_c1run":
         c1_run(sys.argv[2], sys.argv[3])
     else:
-        main(set(sys.argv[1:]) or {"unit", "cases", "msweep", "c1"})
+        main(set(sys.argv[1:]) or {"unit", "unit3d", "cases", "msweep", "c1"})
