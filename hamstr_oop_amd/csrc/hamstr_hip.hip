@@ -689,23 +689,31 @@ __device__ __forceinline__ void store_solution(double* __restrict__ r, double* _
 }
 
 // --------------------------------------------------------------------------
-// ADI phase, exact order: one thread per chain runs the reference's block
-// Thomas (blockTridag4) or cyclic variant (blockTridagPeriodic4) and writes
-// the solution back into r (ADI.cpp:226-244).  Scratch rows hold C', F' and
-// the spike a' for the back sweep: element e of row i of chain t lives at
-// scratch[(i*36+e)*stride + t].
+// ADI phase, exact order (ADI.cpp:57-244 with blockTridag4 /
+// blockTridagPeriodic4, triSolvers.cpp:769-1046), in two launches.
+//
+// k_exact_assemble: every row of the group's chains assembled at once
+// (assemble_row, a thread per row, a workgroup per chain) into rows[p*52 + e]
+// (p the row's processed-face slot; e < 52: A, B, C row-major, then F), so
+// that the sequential walk reads each row from one 416-byte block.
+//
+// k_adi_exact16: 16 lanes per chain, four chains per 64-lane wavefront.  Lane
+// (ri, ci) owns entry (ri, ci) of the 4x4 blocks.  Per row it updates its B
+// entry (B -= A C'), its F row (F -= A F') and its spike entry (a = -A a');
+// one LDS exchange gives every lane of the chain the whole B, F and its
+// column of the spike; every lane factors B and solves F (the same
+// operations in each lane), and the lanes of column ci solve column ci of C
+// (and of the spike) -- each entry with the reference's own operation
+// sequence, so the result is the reference's bit for bit.  The rows C', F', a'
+// go to scratch scr[p*36 + e] (C' 0-15, F' 16-19, a' 20-35, row-major) for the
+// back sweep, which every lane of the chain runs on whole rows; one lane per
+// chain writes the solution into r (and dq).
 // --------------------------------------------------------------------------
-// EXACT mode, first pass of an ADI launch group: every row of the group's
-// chains assembled at once (assemble_row, one thread per row, a workgroup
-// per chain) into rows[e * nRows + pb + i] (e < 52: A, B, C, F), so that
-// k_adi_exact's sequential walk reads each row with one round of loads it can
-// issue a row ahead, instead of the dependent loads of the assembly.
 template <typename T>
 __global__ __launch_bounds__(256) void k_exact_assemble(ChainTab ct, Live lv, const int* __restrict__ chains, int nch,
                                                         const T* __restrict__ ff, int nProc, double cfl,
                                                         const double* __restrict__ sigma,
-                                                        const double* __restrict__ r, T* __restrict__ rows,
-                                                        int nRows) {
+                                                        const double* __restrict__ r, T* __restrict__ rows) {
   const int w = blockIdx.x;
   if (w >= nch) return;
   const int ch = chains[w];
@@ -716,205 +724,314 @@ __global__ __launch_bounds__(256) void k_exact_assemble(ChainTab ct, Live lv, co
     T A[16], B[16], C[16], F[4];
     int cell;
     assemble_row<T>(ct, ff, nProc, pb, nf, N, periodic, i, cfl, sigma, r, A, B, C, F, &cell);
-    T* o = rows + pb + i;
+    T* o = rows + (size_t)(pb + i) * 52;
 #pragma unroll
     for (int e = 0; e < 16; e++) {
-      o[(size_t)e * nRows] = A[e];
-      o[(size_t)(16 + e) * nRows] = B[e];
-      o[(size_t)(32 + e) * nRows] = C[e];
+      o[e] = A[e];
+      o[16 + e] = B[e];
+      o[32 + e] = C[e];
     }
 #pragma unroll
-    for (int e = 0; e < 4; e++) o[(size_t)(48 + e) * nRows] = F[e];
+    for (int e = 0; e < 4; e++) o[48 + e] = F[e];
   }
 }
 
+// one lane's share of an assembled row: row ri of A, B[ri][ci], column ci of
+// C, and all of F
 template <typename T>
-__device__ __forceinline__ void load_exact_row(const T* __restrict__ rows, int nRows, int p, T* A, T* B, T* C, T* F) {
+struct ExactRow {
+  T a[4], b, c[4], f[4];
+};
+
+template <typename T>
+__device__ __forceinline__ void load_exact_row(const T* __restrict__ rows, size_t p, int ri, int ci, ExactRow<T>& o) {
+  const T* base = rows + p * 52;
 #pragma unroll
-  for (int e = 0; e < 16; e++) {
-    A[e] = rows[(size_t)e * nRows + p];
-    B[e] = rows[(size_t)(16 + e) * nRows + p];
-    C[e] = rows[(size_t)(32 + e) * nRows + p];
+  for (int l = 0; l < 4; l++) {
+    o.a[l] = base[4 * ri + l];
+    o.c[l] = base[32 + 4 * l + ci];
+    o.f[l] = base[48 + l];
   }
-#pragma unroll
-  for (int e = 0; e < 4; e++) F[e] = rows[(size_t)(48 + e) * nRows + p];
+  o.b = base[16 + 4 * ri + ci];
 }
 
+// one lane's share of a back-sweep row: row ri of C' and of a', and F'[ri]
 template <typename T>
-__global__ __launch_bounds__(64) void k_adi_exact(ChainTab ct, Live lv, const int* __restrict__ chains, int nch,
-                                                  const T* __restrict__ rows, int nRows,
-                                                  double* __restrict__ r, T* __restrict__ scratch, int stride,
-                                                  double* __restrict__ dq, int dqmode) {
-  const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (t >= nch) return;
-  const int ch = chains[t];
-  if (lv.chain && !lv.chain[ch]) return;
-  const int pb = ct.pbase[ch], nf = ct.nf[ch], N = ct.ncell[ch];
-  const bool periodic = ct.periodic[ch] != 0;
-  auto S = [&](int i, int e) -> T& { return scratch[((size_t)i * 36 + e) * stride + t]; };
+struct BackRow {
+  T c[4], a[4], f;
+};
 
-  T Cp[16], Fp[4], ap[16];
-  // row i+1's entries are loaded while row i is eliminated
-  T nA[16], nB[16], nC[16], nF[4];
-  load_exact_row(rows, nRows, pb, nA, nB, nC, nF);
-  for (int i = 0; i < N; i++) {
-    T A[16], B[16], C[16], F[4], a[16];
+template <typename T>
+__device__ __forceinline__ void load_back_row(const T* __restrict__ scr, size_t p, int ri, bool per, BackRow<T>& o) {
+  const T* base = scr + p * 36;
 #pragma unroll
-    for (int e = 0; e < 16; e++) {
-      A[e] = nA[e];
-      B[e] = nB[e];
-      C[e] = nC[e];
+  for (int l = 0; l < 4; l++) {
+    o.c[l] = base[4 * ri + l];
+    o.a[l] = per ? base[20 + 4 * ri + l] : T(0);
+  }
+  o.f = base[16 + ri];
+}
+
+// one lane's share of a row of the bordered elimination: column ci of C' and
+// of a', and all of F'
+template <typename T>
+struct BorderRow {
+  T c[4], a[4], f[4];
+};
+
+template <typename T>
+__device__ __forceinline__ void load_border_row(const T* __restrict__ scr, size_t p, int ci, BorderRow<T>& o) {
+  const T* base = scr + p * 36;
+#pragma unroll
+  for (int l = 0; l < 4; l++) {
+    o.c[l] = base[4 * l + ci];
+    o.a[l] = base[20 + 4 * l + ci];
+    o.f[l] = base[16 + l];
+  }
+}
+
+// Orders the LDS accesses of one wavefront around an exchange.  A wavefront's
+// LDS operations are performed in order, so no wait is needed; the fence and
+// the barrier only keep the compiler from moving them.  Unlike
+// __syncthreads(), it does not wait for outstanding global loads (the rows
+// prefetched ahead), so it is only for workgroups of a single wavefront.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// v[k] for a lane-dependent k, as selects (no indexed register access)
+template <typename T>
+__device__ __forceinline__ T pick4(const T* v, int k) {
+  return k == 0 ? v[0] : (k == 1 ? v[1] : (k == 2 ? v[2] : v[3]));
+}
+
+// The three sequential sweeps below each keep kRing rows in flight: the loop
+// is unrolled kRing times over kRing buffers, and a buffer is refilled with
+// the row kRing ahead right after its row is used (no register copies, which
+// would wait for the newest loads).  The refills are unconditional, with the
+// row clamped into the chain: with a fixed number of loads per step the
+// compiler waits only for the row a step uses, not for every load in flight.
+constexpr int kRing = 4;
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_adi_exact16(ChainTab ct, Live lv, const int* __restrict__ chains, int nch,
+                                                    const T* __restrict__ rows, double* __restrict__ r,
+                                                    T* __restrict__ scr, double* __restrict__ dq, int dqmode) {
+  __shared__ T xb[4][36];  // per chain: B (16), F (4), spike a (16), row-major
+  __shared__ int xn[4];
+  const int lane = threadIdx.x, slot = lane >> 4, q = lane & 15, ri = q & 3, ci = q >> 2;
+  const int w = xcd_block(blockIdx.x, gridDim.x) * 4 + slot;
+  int pb = 0, N = 0;
+  bool per = false;
+  if (w < nch) {
+    const int ch = chains[w];
+    if (!lv.chain || lv.chain[ch]) {
+      pb = ct.pbase[ch];
+      N = ct.ncell[ch];
+      per = ct.periodic[ch] != 0;
     }
-#pragma unroll
-    for (int e = 0; e < 4; e++) F[e] = nF[e];
-    if (i + 1 < N) load_exact_row(rows, nRows, pb + i + 1, nA, nB, nC, nF);
+  }
+  if (q == 0) xn[slot] = N;
+  wave_lds_sync();
+  // every lane runs the longest chain's number of steps (the exchanges below
+  // are wavefront-wide)
+  const int Nmax = max(max(xn[0], xn[1]), max(xn[2], xn[3]));
+  T* X = xb[slot];
+
+  // ---- forward elimination ----
+  T Ccol[4] = {T(0), T(0), T(0), T(0)}, acol[4] = {T(0), T(0), T(0), T(0)}, Fp[4] = {T(0), T(0), T(0), T(0)};
+  auto forward = [&](int i, const ExactRow<T>& cur) __attribute__((always_inline)) {
+    const bool act = i < N;
+    T bv = cur.b, fr = pick4(cur.f, ri), av = T(0);
     if (i != 0) {
-      mat_sub_mul(B, A, Cp);
-      vec_sub_mul(F, A, Fp);
-      if (periodic) {
+      // triSolvers.cpp:790-803 (and the spike a = -A a', :877-891)
 #pragma unroll
-        for (int k = 0; k < 4; k++)
+      for (int l = 0; l < 4; l++) bv -= cur.a[l] * Ccol[l];
 #pragma unroll
-          for (int j = 0; j < 4; j++) {
-            T x = T(0);
+      for (int j = 0; j < 4; j++) fr -= cur.a[j] * Fp[j];
+      if (per) {
 #pragma unroll
-            for (int l = 0; l < 4; l++) x -= A[4 * k + l] * ap[4 * l + j];
-            a[4 * k + j] = x;
-          }
+        for (int l = 0; l < 4; l++) av -= cur.a[l] * acol[l];
       }
-    } else if (periodic) {
-#pragma unroll
-      for (int e = 0; e < 16; e++) a[e] = A[e];
+    } else if (per) {
+      av = pick4(cur.a, ci);
     }
-    const LU4<T> L = lu4(B);
-    lu4_solve(L, F, 1);
-    if (!periodic) {
-      if (i != N - 1) {
+    X[4 * ri + ci] = bv;
+    if (ci == 0) X[16 + ri] = fr;
+    X[20 + 4 * ri + ci] = av;
+    wave_lds_sync();
+    T B[16], F[4], ac[4];
 #pragma unroll
-        for (int j = 0; j < 4; j++) lu4_solve(L, C + j, 4);
-      }
-    } else if (i != N - 2) {
+    for (int e = 0; e < 16; e++) B[e] = X[e];
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        lu4_solve(L, C + j, 4);
-        lu4_solve(L, a + j, 4);
+    for (int l = 0; l < 4; l++) {
+      F[l] = X[16 + l];
+      ac[l] = X[20 + 4 * l + ci];
+    }
+    wave_lds_sync();
+    if (act) {
+      const LU4<T> L = lu4(B);
+      lu4_solve(L, F, 1);
+      T cc[4] = {cur.c[0], cur.c[1], cur.c[2], cur.c[3]};
+      if (!per) {
+        if (i != N - 1) lu4_solve(L, cc, 1);
+      } else if (i != N - 2) {
+        lu4_solve(L, cc, 1);
+        lu4_solve(L, ac, 1);
+      } else {
+#pragma unroll
+        for (int l = 0; l < 4; l++) cc[l] += ac[l];
+        lu4_solve(L, cc, 1);
+#pragma unroll
+        for (int l = 0; l < 4; l++) ac[l] = cc[l];
       }
+      if (i < N - 1) {
+        T* o = scr + (size_t)(pb + i) * 36;
+        o[4 * ri + ci] = pick4(cc, ri);
+        if (ci == 0) o[16 + ri] = pick4(F, ri);
+        if (per) o[20 + 4 * ri + ci] = pick4(ac, ri);
+      }
+#pragma unroll
+      for (int l = 0; l < 4; l++) {
+        Ccol[l] = cc[l];
+        acol[l] = ac[l];
+        Fp[l] = F[l];
+      }
+    }
+  };
+  {
+    const int last = max(N - 1, 0);
+    ExactRow<T> ring[kRing];
+#pragma unroll
+    for (int k = 0; k < kRing; k++) load_exact_row(rows, (size_t)pb + min(k, last), ri, ci, ring[k]);
+    // whole rounds of kRing steps: the steps past a chain's end do nothing
+    // but the exchange, and the unrolled body stays free of branches around
+    // its loads
+    for (int i0 = 0; i0 < Nmax; i0 += kRing) {
+#pragma unroll
+      for (int k = 0; k < kRing; k++) {
+        const int i = i0 + k;
+        forward(i, ring[k]);
+        load_exact_row(rows, (size_t)pb + min(i + kRing, last), ri, ci, ring[k]);
+      }
+    }
+  }
+
+  // ---- bordered last row of the cyclic solve (triSolvers.cpp:966-1029) ----
+  // Lane (ri, ci) carries Bl[ri][ci], Fl[ri] and row ri of Cl; Cl starts as
+  // the normalised C of the last row, whose column ci this lane holds.  Only
+  // cyclic chains take part: their lengths are exchanged like N.
+  if (q == 0) xn[slot] = per ? N : 0;
+  wave_lds_sync();
+  int Mmax = 0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) Mmax = max(Mmax, xn[k] - 1);
+  T blv = (ri == ci) ? T(1.0) : T(0.0), flv = pick4(Fp, ri), clr[4];
+  X[4 * ri + ci] = pick4(Ccol, ri);
+  wave_lds_sync();
+#pragma unroll
+  for (int l = 0; l < 4; l++) clr[l] = X[4 * ri + l];
+  wave_lds_sync();
+  T xl[4] = {Fp[0], Fp[1], Fp[2], Fp[3]};
+  if (Mmax > 0) {
+    const int M = per ? N - 1 : 0;
+    auto border = [&](int i, const BorderRow<T>& o) __attribute__((always_inline)) {
+      const bool act = i < M;
+      T t = T(0);
+      if (act) {
+#pragma unroll
+        for (int l = 0; l < 4; l++) {
+          t -= clr[l] * o.c[l];
+          blv -= clr[l] * o.a[l];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) flv -= clr[j] * o.f[j];
+      }
+      X[4 * ri + ci] = t;
+      wave_lds_sync();
+      if (act) {
+#pragma unroll
+        for (int l = 0; l < 4; l++) clr[l] = X[4 * ri + l];
+      }
+      wave_lds_sync();
+    };
+    const int lastm = max(M - 1, 0);
+    BorderRow<T> ring[kRing];
+#pragma unroll
+    for (int k = 0; k < kRing; k++) load_border_row(scr, (size_t)pb + min(k, lastm), ci, ring[k]);
+    for (int i0 = 0; i0 < Mmax; i0 += kRing) {
+#pragma unroll
+      for (int k = 0; k < kRing; k++) {
+        const int i = i0 + k;
+        border(i, ring[k]);
+        load_border_row(scr, (size_t)pb + min(i + kRing, lastm), ci, ring[k]);
+      }
+    }
+    X[4 * ri + ci] = blv;
+    if (ci == 0) X[16 + ri] = flv;
+    wave_lds_sync();
+    T Bl[16], Fl[4];
+#pragma unroll
+    for (int e = 0; e < 16; e++) Bl[e] = X[e];
+#pragma unroll
+    for (int l = 0; l < 4; l++) Fl[l] = X[16 + l];
+    wave_lds_sync();
+    if (per) {
+      const LU4<T> L = lu4(Bl);
+      lu4_solve(L, Fl, 1);
+#pragma unroll
+      for (int l = 0; l < 4; l++) xl[l] = Fl[l];
+    }
+  }
+
+  // ---- back sweep: lane (ri, ci) computes x[ri] of each row, every lane
+  // of the chain then reads the whole x ----
+  auto put = [&](int i, const T* x) __attribute__((always_inline)) {
+    if (q == 0) store_solution(r, dq, dqmode, ct.pleft[pb + i], x);
+  };
+  if (N > 0) put(N - 1, xl);
+  T x[4] = {xl[0], xl[1], xl[2], xl[3]};
+  // row i = N - 2 - s at step s
+  const int S = max(N - 1, 0);
+  int Smax = 0;
+  if (q == 0) xn[slot] = S;
+  wave_lds_sync();
+#pragma unroll
+  for (int k = 0; k < 4; k++) Smax = max(Smax, xn[k]);
+  auto back = [&](int s, const BackRow<T>& o) __attribute__((always_inline)) {
+    const bool act = s < S;
+    const int i = N - 2 - s;
+    T v = o.f;
+    if (!per || i == N - 2) {
+#pragma unroll
+      for (int j = 0; j < 4; j++) v -= o.c[j] * x[j];
     } else {
 #pragma unroll
-      for (int e = 0; e < 16; e++) C[e] += a[e];
-#pragma unroll
-      for (int j = 0; j < 4; j++) lu4_solve(L, C + j, 4);
-#pragma unroll
-      for (int e = 0; e < 16; e++) a[e] = C[e];
+      for (int j = 0; j < 4; j++) v -= (o.c[j] * x[j] + o.a[j] * xl[j]);
     }
-    if (i < N - 1) {
+    if (ci == 0) X[ri] = v;
+    wave_lds_sync();
+    if (act) {
 #pragma unroll
-      for (int e = 0; e < 16; e++) S(i, e) = C[e];
-#pragma unroll
-      for (int e = 0; e < 4; e++) S(i, 16 + e) = F[e];
-      if (periodic) {
-#pragma unroll
-        for (int e = 0; e < 16; e++) S(i, 20 + e) = a[e];
-      }
+      for (int l = 0; l < 4; l++) x[l] = X[l];
+      put(i, x);
     }
-#pragma unroll
-    for (int e = 0; e < 16; e++) {
-      Cp[e] = C[e];
-      ap[e] = a[e];
-    }
-#pragma unroll
-    for (int e = 0; e < 4; e++) Fp[e] = F[e];
-  }
-
-  auto put = [&](int i, const T* x) {
-    const int cell = ct.pleft[pb + i];
-    store_solution(r, dq, dqmode, cell, x);
+    wave_lds_sync();
   };
-  if (!periodic) {
-    T x[4] = {Fp[0], Fp[1], Fp[2], Fp[3]};
-    put(N - 1, x);
-    for (int i = N - 2; i >= 0; i--) {
-      T c[16], fv[4];
+  {
+    // row N-2-s, clamped into [0, N-2] (row 0 for chains without a back sweep)
+    auto brow = [&](int s) { return (size_t)pb + max(N - 2 - min(s, max(S - 1, 0)), 0); };
+    BackRow<T> ring[kRing];
 #pragma unroll
-      for (int e = 0; e < 16; e++) c[e] = S(i, e);
+    for (int k = 0; k < kRing; k++) load_back_row(scr, brow(k), ri, per, ring[k]);
+    for (int s0 = 0; s0 < Smax; s0 += kRing) {
 #pragma unroll
-      for (int e = 0; e < 4; e++) fv[e] = S(i, 16 + e);
-      vec_sub_mul(fv, c, x);
-#pragma unroll
-      for (int e = 0; e < 4; e++) x[e] = fv[e];
-      put(i, x);
-    }
-  } else {
-    // bordered last row (triSolvers.cpp:966-1029)
-    T Bl[16], Cl[16], Fl[4];
-#pragma unroll
-    for (int e = 0; e < 16; e++) {
-      Bl[e] = (e % 5 == 0) ? T(1.0) : T(0.0);
-      Cl[e] = Cp[e];
-    }
-#pragma unroll
-    for (int e = 0; e < 4; e++) Fl[e] = Fp[e];
-    for (int i = 0; i < N - 1; i++) {
-      T c[16], a[16], fv[4], tt[16];
-#pragma unroll
-      for (int e = 0; e < 16; e++) {
-        c[e] = S(i, e);
-        a[e] = S(i, 20 + e);
+      for (int k = 0; k < kRing; k++) {
+        const int s = s0 + k;
+        back(s, ring[k]);
+        load_back_row(scr, brow(s + kRing), ri, per, ring[k]);
       }
-#pragma unroll
-      for (int e = 0; e < 4; e++) fv[e] = S(i, 16 + e);
-#pragma unroll
-      for (int k = 0; k < 4; k++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-          T x = T(0);
-          T b = Bl[4 * k + j];
-#pragma unroll
-          for (int l = 0; l < 4; l++) {
-            x -= Cl[4 * k + l] * c[4 * l + j];
-            b -= Cl[4 * k + l] * a[4 * l + j];
-          }
-          tt[4 * k + j] = x;
-          Bl[4 * k + j] = b;
-        }
-      vec_sub_mul(Fl, Cl, fv);
-#pragma unroll
-      for (int e = 0; e < 16; e++) Cl[e] = tt[e];
-    }
-    const LU4<T> L = lu4(Bl);
-    lu4_solve(L, Fl, 1);
-    put(N - 1, Fl);
-    T x[4];
-    {
-      T c[16], fv[4];
-#pragma unroll
-      for (int e = 0; e < 16; e++) c[e] = S(N - 2, e);
-#pragma unroll
-      for (int e = 0; e < 4; e++) fv[e] = S(N - 2, 16 + e);
-      vec_sub_mul(fv, c, Fl);
-#pragma unroll
-      for (int e = 0; e < 4; e++) x[e] = fv[e];
-      put(N - 2, x);
-    }
-    for (int i = N - 3; i >= 0; i--) {
-      T c[16], a[16], fv[4];
-#pragma unroll
-      for (int e = 0; e < 16; e++) {
-        c[e] = S(i, e);
-        a[e] = S(i, 20 + e);
-      }
-#pragma unroll
-      for (int e = 0; e < 4; e++) fv[e] = S(i, 16 + e);
-#pragma unroll
-      for (int k = 0; k < 4; k++) {
-        T v = fv[k];
-#pragma unroll
-        for (int j = 0; j < 4; j++) v -= (c[4 * k + j] * x[j] + a[4 * k + j] * Fl[j]);
-        fv[k] = v;
-      }
-#pragma unroll
-      for (int e = 0; e < 4; e++) x[e] = fv[e];
-      put(i, x);
     }
   }
 }
@@ -2656,9 +2773,8 @@ struct hamstr_ctx {
   std::vector<Group> groups;   // in phase order
   int nPhase = 0;
   bool last_phase_all_cells = false;
-  void* scratch = nullptr;
-  size_t scratch_stride = 0;
-  void* exact_rows = nullptr;  // EXACT groups: T[52][nChainProc] assembled rows (k_exact_assemble)
+  void* scratch = nullptr;     // EXACT groups: T[nChainProc][36] rows C', F', a' of the back sweep
+  void* exact_rows = nullptr;  // EXACT groups: T[nChainProc][52] assembled rows (k_exact_assemble)
   double* partial = nullptr;
   int nPart = 0;
   double* npart = nullptr;  // single case: [kGraphMaxIters][2 stages][nPart][2] block partials
@@ -2832,20 +2948,16 @@ int launch_group(hamstr_ctx* x, const Group& g, int grid, int dqmode, int ncr) {
 #undef FASTT
   } else if (x->lhs32()) {
     hipLaunchKernelGGL((k_exact_assemble<float>), dim3(g.n), dim3(256), 0, x->stream, ct, x->live(), chains, g.n,
-                       (const float*)x->ff, x->nProc, x->c.cfl, x->sigma, x->r, (float*)x->exact_rows,
-                       x->nChainProc);
+                       (const float*)x->ff, x->nProc, x->c.cfl, x->sigma, x->r, (float*)x->exact_rows);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL((k_adi_exact<float>), dim3(nblk(g.n, 64)), dim3(64), 0, x->stream, ct, x->live(), chains, g.n,
-                       (const float*)x->exact_rows, x->nChainProc, x->r, (float*)x->scratch, (int)x->scratch_stride,
-                       x->dq, dqmode);
+    hipLaunchKernelGGL((k_adi_exact16<float>), dim3(nblk(g.n, 4)), dim3(64), 0, x->stream, ct, x->live(), chains,
+                       g.n, (const float*)x->exact_rows, x->r, (float*)x->scratch, x->dq, dqmode);
   } else {
     hipLaunchKernelGGL((k_exact_assemble<double>), dim3(g.n), dim3(256), 0, x->stream, ct, x->live(), chains, g.n,
-                       (const double*)x->ff, x->nProc, x->c.cfl, x->sigma, x->r, (double*)x->exact_rows,
-                       x->nChainProc);
+                       (const double*)x->ff, x->nProc, x->c.cfl, x->sigma, x->r, (double*)x->exact_rows);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL((k_adi_exact<double>), dim3(nblk(g.n, 64)), dim3(64), 0, x->stream, ct, x->live(), chains, g.n,
-                       (const double*)x->exact_rows, x->nChainProc, x->r, (double*)x->scratch,
-                       (int)x->scratch_stride, x->dq, dqmode);
+    hipLaunchKernelGGL((k_adi_exact16<double>), dim3(nblk(g.n, 4)), dim3(64), 0, x->stream, ct, x->live(), chains,
+                       g.n, (const double*)x->exact_rows, x->r, (double*)x->scratch, x->dq, dqmode);
   }
   HIPCHK(hipGetLastError());
   return HAMSTR_OK;
@@ -3187,7 +3299,7 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
   const int maxt = (P.lhs == HAMSTR_LHS_FP32) ? 1024 : 512;
   std::vector<Group> groups;
   std::vector<int> group_chains;
-  size_t maxGroupChains = 0, maxRows = 0;
+  size_t maxRows = 0;
   std::vector<int> ch_hbase(nChain, 0);
   long nLong = 0;  // odd rows of the long chains, the rows their scratch holds
   for (int ph = 0; ph < nPhase; ph++) {
@@ -3264,7 +3376,6 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
       }
       for (int i : bucket[b]) group_chains.push_back(i);
       groups.push_back(g);
-      maxGroupChains = std::max(maxGroupChains, (size_t)g.n);
       if (!g.fast) maxRows = std::max(maxRows, (size_t)g.maxN);
     }
   }
@@ -3365,8 +3476,9 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
   chk(hipMalloc(&x->sr, sizeof(double) * nProc), "alloc sr");
   chk(hipMalloc(&x->lflux, sizeof(double) * 4 * nProc), "alloc lflux");
   chk(hipMalloc(&x->ff, tl * 32 * nProc), "alloc ff");
-  x->scratch_stride = std::max<size_t>(maxGroupChains, 1);
-  chk(hipMalloc(&x->scratch, tl * 36 * std::max<size_t>(maxRows, 1) * x->scratch_stride), "alloc scratch");
+  // EXACT groups: the assembled rows (52 per row) and the rows C', F', a' of
+  // the back sweep (36 per row), both indexed by the row's processed-face slot
+  chk(hipMalloc(&x->scratch, tl * 36 * (maxRows > 0 ? (size_t)std::max(nChainProc, 1) : 1)), "alloc scratch");
   if (maxRows > 0) chk(hipMalloc(&x->exact_rows, tl * 52 * (size_t)std::max(nChainProc, 1)), "alloc exact rows");
   chk(hipMalloc(&x->long_scratch, tl * 36 * (size_t)x->nLong), "alloc long-chain scratch");
   x->nPart = nblk(nCell);
