@@ -1,0 +1,105 @@
+"""Configuration 2 at full size against the oracle: the NACA0012 O-grid of
+512 x 256 cells whose outer 128 layers are split into triangles (196,608
+cells), MUSCL, M=0.5, alpha=2, CFL 10, from the freestream.
+
+tests/golden/c2_oracle.npz holds the oracle's own 10,000-step run (written by
+tests/golden/make_c2_oracle.py; the oracle is pinned bit for bit to the
+reference in tests/test_oracle_golden.py, and runs the orphan-face extension
+that a mesh with triangles needs).  The reference's own code cannot compute
+this mesh.
+
+* EXACT is bit-identical to it: q after 1,000 and 4,200 steps (SHA-256 of
+  the whole state, and every 97th cell), and every step's norms to 1e-12.
+* FAST follows the same trajectory to rounding over the first 1,000 steps.
+* The scheme settles into a limit cycle on this mesh: from step 7,000 on,
+  the oracle's nonlinear norm stays between 3.2e-11 and 3.6e-11 and never
+  reaches 1e-11.  FAST settles into the same band with the FP64 and the FP32
+  left-hand side (EXACT is the oracle itself, bit for bit), so the band is
+  the scheme's, not a kernel's.
+"""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.fixture(scope="module")
+def c2():
+    from hamstr_oop_amd import meshgen
+    d = np.load(os.path.join(GOLD, "c2_oracle.npz"))
+    mesh = meshgen.naca_ogrid(int(d["nc"]), int(d["nr"]), tri_layers=int(d["tri_layers"]))
+    return d, mesh
+
+
+@pytest.fixture(scope="module")
+def hip():
+    from hamstr_oop_amd import hip as H
+    H.lib()
+    return H
+
+
+def _solver(hip, d, mesh, lhs, mode):
+    return hip.Solver(mesh, float(d["mach"]), float(d["alpha"]), int(d["order"]), float(d["cfl"]), lhs=lhs,
+                      adi_mode=mode)
+
+
+def _scaled(q, ref):
+    q = np.asarray(q).reshape(-1, 4)
+    ref = np.asarray(ref).reshape(-1, 4)
+    return float((np.abs(q - ref) / np.abs(ref).max(axis=0)).max())
+
+
+def test_config2_exact_bitwise_to_the_oracle(hip, c2):
+    d, mesh = c2
+    print("config 2 EXACT, 4200 steps", flush=True)
+    g = _solver(hip, d, mesh, hip.LHS_FP64, hip.ADI_EXACT)
+    n, l2a, lla = g.step(1000, check_every=50)
+    assert n == 1000
+    q = g.get("q")
+    assert hashlib.sha256(q.tobytes()).hexdigest() == str(d["q_sha256_1000"])
+    assert np.array_equal(q.reshape(-1, 4)[::97], d["q_sub_1000"])
+    n, l2b, llb = g.step(3200, check_every=50)
+    assert n == 3200
+    q = g.get("q")
+    assert hashlib.sha256(q.tobytes()).hexdigest() == str(d["q_sha256_4200"])
+    np.testing.assert_allclose(np.concatenate([l2a, l2b]), d["l2rhs"][:4200], rtol=1e-12, atol=0)
+    np.testing.assert_allclose(np.concatenate([lla, llb]), d["l2lin"][:4200], rtol=1e-12, atol=0)
+    g.close()
+
+
+def test_config2_fast_follows_the_oracle(hip, c2):
+    """FAST over 1,000 steps: fields within 1e-10 of the oracle's (scaled per
+    component), norms to 1e-6 relative."""
+    d, mesh = c2
+    g = _solver(hip, d, mesh, hip.LHS_FP64, hip.ADI_FAST)
+    n, l2, _ = g.step(1000, check_every=50)
+    assert n == 1000
+    diff = _scaled(g.get("q").reshape(-1, 4)[::97], d["q_sub_1000"])
+    print(f"config 2 FAST after 1000 steps: {diff:.2e} from the oracle", flush=True)
+    assert diff <= 1e-10
+    np.testing.assert_allclose(l2, d["l2rhs"][:1000], rtol=1e-6, atol=0)
+    g.close()
+
+
+@pytest.mark.parametrize("lhs", ["fp64", "fp32"])
+def test_config2_limit_cycle_is_the_schemes(hip, c2, lhs):
+    """10,000 steps of FAST: from step 8,000 on the norm stays in the band the
+    oracle's norm keeps from step 7,000 on (3.2e-11 to 3.6e-11, within 3%),
+    neither converging below it nor leaving it."""
+    d, mesh = c2
+    ref = d["l2rhs"][7000:]
+    lo, hi = float(ref.min()), float(ref.max())
+    assert 3.0e-11 < lo and hi < 4.0e-11   # the oracle's own band
+    g = _solver(hip, d, mesh, hip.LHS_FP32 if lhs == "fp32" else hip.LHS_FP64, hip.ADI_FAST)
+    n, l2, _ = g.step(10000, check_every=100)
+    assert n == 10000
+    tail = l2[8000:]
+    print(f"config 2 FAST {lhs}: steps 8000-10000 norm in [{tail.min():.4e}, {tail.max():.4e}], "
+          f"oracle [{lo:.4e}, {hi:.4e}]", flush=True)
+    assert 0.97 * lo <= tail.min() and tail.max() <= 1.03 * hi
+    g.close()
