@@ -22,25 +22,44 @@ HIPCC_FLAGS = [
 ]
 
 
-def build(verbose: bool = False) -> str:
+# The sanitizer build instruments the host code only (table builders, readers,
+# the C ABI's argument checks): ASan and UBSan with ROCm clang's shared
+# runtime, each flag given to the host compilation alone.
+ASAN_DIR = os.path.join(HERE, "_asan")
+SAN_HOST = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+            "-Xarch_host", "-fno-sanitize-recover=undefined", "-Xarch_host", "-fno-omit-frame-pointer",
+            "-shared-libasan"]
+
+
+def build(verbose: bool = False, sanitize: bool = False) -> str:
+    """Builds libhamstr_hip.so and the drop-in driver; with sanitize, the
+    host-sanitized copies in _asan/ instead (used by tests/test_sanitizers.py)."""
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    cmd = [hipcc, *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), *SRCS, "-o", LIB + ".tmp"]
+    out_dir = ASAN_DIR if sanitize else HERE
+    os.makedirs(out_dir, exist_ok=True)
+    lib = os.path.join(out_dir, "libhamstr_hip.so")
+    driver = os.path.join(out_dir, "hamstr_mi355x")
+    cmd = [hipcc, *HIPCC_FLAGS, *(SAN_HOST if sanitize else []), "-I", os.path.join(ROOT, "include"), *SRCS,
+           "-o", lib + ".tmp"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
-    os.replace(LIB + ".tmp", LIB)
+    os.replace(lib + ".tmp", lib)
     # the drop-in driver: plain C++ over the C ABI
-    cxx = os.environ.get("CXX", "g++")
-    cmd = [cxx, "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), DRIVER_SRC, "-o", DRIVER + ".tmp",
-           "-L", HERE, "-lhamstr_hip", "-Wl,-rpath,$ORIGIN"]
+    if sanitize:
+        cxx = [os.environ.get("CLANGXX", "/opt/rocm/llvm/bin/clang++"), "-O1", "-g", "-fno-omit-frame-pointer",
+               "-fsanitize=address,undefined", "-fno-sanitize-recover=undefined", "-shared-libasan"]
+    else:
+        cxx = [os.environ.get("CXX", "g++"), "-O2"]
+    cmd = [*cxx, "-std=c++17", "-I", os.path.join(ROOT, "include"), DRIVER_SRC, "-o", driver + ".tmp",
+           "-L", out_dir, "-lhamstr_hip", "-Wl,-rpath,$ORIGIN"]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
-    os.replace(DRIVER + ".tmp", DRIVER)
-    return LIB
+    os.replace(driver + ".tmp", driver)
+    return lib
 
 
 if __name__ == "__main__":
-    build(verbose=True)
-    print(LIB)
+    print(build(verbose=True, sanitize="--sanitize" in sys.argv))
     sys.exit(0)

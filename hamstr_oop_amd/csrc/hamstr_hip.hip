@@ -3089,9 +3089,6 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
   if (nCell <= 0 || nFace <= 0 || nChain <= 0 || !mesh->faces || !mesh->chainStart || !mesh->chainConn ||
       !mesh->normVec || !mesh->refMtx)
     return fail(HAMSTR_ERR_ARG, "hamstr_create: bad mesh description");
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(HAMSTR_ERR_NODEV, "no HIP device");
-
   const int* faces = mesh->faces;
   const int* fs = mesh->chainStart;
   const int* cc = mesh->chainConn;
@@ -3382,6 +3379,10 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
 
   // scratch rows (long and half-LDS chains) are indexed with int, 36 per row
   if (nLong >= (1L << 31) / 36) return fail(HAMSTR_ERR_MESH, "too many rows in long chains");
+  // every host table above is built before the device is looked for, so that
+  // a machine without one (and its sanitizer builds) runs all of this code
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(HAMSTR_ERR_NODEV, "no HIP device");
   hamstr_ctx* x = new hamstr_ctx();
   x->prm = P;
   x->nCell = nCell;

@@ -15,7 +15,9 @@ from typing import Optional
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libhamstr_hip.so")
+# HAMSTR_HIP_LIB selects another build of the same library (the sanitizer
+# build of the host code, _asan/libhamstr_hip.so, in tests/test_sanitizers.py)
+LIB_PATH = os.environ.get("HAMSTR_HIP_LIB") or os.path.join(HERE, "libhamstr_hip.so")
 
 OK, ERR_ARG, ERR_HIP, ERR_NAN, ERR_MESH, ERR_NODEV = range(6)
 LHS_FP64, LHS_FP32 = 0, 1

@@ -151,7 +151,8 @@ class Oracle(_Unit):
     """The CPU restatement (liboracle.so)."""
 
     def __init__(self):
-        lib = _load(os.path.join(HERE, "liboracle.so"))
+        # HAMSTR_ORACLE_LIB selects another build (the sanitizer build, _asan/)
+        lib = _load(os.environ.get("HAMSTR_ORACLE_LIB") or os.path.join(HERE, "liboracle.so"))
         lib.orc_create.restype = C.c_void_p
         for name in ("orc_destroy", "orc_get", "orc_set", "orc_compute_rhs", "orc_adi",
                      "orc_compute_linear_rhs", "orc_step", "orc_freestream"):
