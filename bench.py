@@ -12,9 +12,11 @@ variant.  A "step" is one stepSolution ADI iteration: computeRHS("implicit")
 
 Prints one JSON line: cell-iterations/s (value), the roofline of the dominant
 kernel from HIP events over the timed region, the time to nonlinear
-L2 <= 1e-10 from freestream, the FP32-LHS variant, the explicit euler / rk3
-steps of the same mesh, the mixed-batch sweep, and the reference's own CPU
-code timed on this host on a bounded sample of the same workload.
+L2 <= 1e-10 from freestream (FAST, EXACT, FP32-LHS), FAST against EXACT beyond
+1e-10, the explicit euler / rk3 steps of the same mesh, the 1024x384 WENO
+transonic case, the mixed-batch sweep with the learned warm start and its
+256-case held-out evaluation, and the CPU restatement timed on this host on a
+bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -32,39 +34,55 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
-def algorithmic_bytes(n_cell, n_proc, phase_faces, b, msweep=1):
+def algorithmic_bytes(n_cell, n_proc, phase_faces, b, msweep=1, phase_cells=None):
     """Bytes per cell-iteration of the iteration's dataflow when every
     materialised array (q, r0, sigma, ff, r, dq, face fluxes) is read and
     written once per phase and stencil data is reused on chip.
 
     phi = processed faces per cell, b = bytes per LHS scalar (8 FP64, 4 FP32).
-    computeRHS + Jacobians: 72 + phi*(24 + 32b); ADI phase k: 72 + 32*b*phi_k;
+    computeRHS + Jacobians: 72 + phi*(24 + 32b); ADI phase k: 72 per cell its
+    chains cover (sigma, r read, r written) + 32*b per face of the phase;
     computeLinearRHS + update: 168 + phi*(8 + 32b); extra sweeps:
     312 + phi*(8 + 64b) each.
     """
     phi = n_proc / n_cell
     stages = {"rhs": 72 + phi * (24 + 32 * b)}
     for k, nf in enumerate(phase_faces):
-        stages[f"adi_phase{k}"] = 72 + 32 * b * nf / n_cell
+        cells = n_cell if phase_cells is None else phase_cells[k]
+        stages[f"adi_phase{k}"] = (72 * cells + 32 * b * nf) / n_cell
     stages["linear"] = 168 + phi * (8 + 32 * b)
     total = sum(stages.values()) + (msweep - 1) * (312 + phi * (8 + 64 * b))
     return total, stages
 
 
 def phase_face_counts(mesh):
-    """Processed faces per ADI phase for an O-grid (loops, then lines), and in
-    all; orphan faces count with the last phase, whose rows take their
-    diagonal blocks."""
+    """Processed faces per ADI phase for an O-grid (loops, then lines), in
+    all, and the cells each phase's chains cover; orphan faces count with the
+    last phase, whose rows take their diagonal blocks."""
     import numpy as np
     start = np.asarray(mesh.chain_start)
     conn = np.asarray(mesh.chain_conn)
-    per = []
+    per, cells = [], []
     for c in range(len(start) - 1):
         f1, f2 = start[c], start[c + 1]
         per.append((f2 - f1) - (1 if conn[f1] == conn[f2 - 1] else 0))
+        cells.append((f2 - f1) - 1)   # a closed chain repeats its first face
     nl = mesh.chains_per_color[0]
     no = int(len(mesh.orphans))
-    return [int(sum(per[:nl])), int(sum(per[nl:])) + no], int(sum(per)) + no
+    return ([int(sum(per[:nl])), int(sum(per[nl:])) + no], int(sum(per)) + no,
+            [int(sum(cells[:nl])), int(sum(cells[nl:]))])
+
+
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or the platform's name."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
 
 
 def measured_traffic(stage, nc, nr, order, lhs, adi, tri=0):
@@ -115,7 +133,7 @@ def cpu_baseline(mesh, mach, alpha, order, cfl, budget_s=12.0, max_iters=30):
     return {
         "value": mesh.n_cell * n / sec, "unit": "cell-iters/s", "cores": 1, "kind": kind,
         "sample": f"{n} iterations (after 1 warm-up) of the same case from freestream; {what}; "
-                  f"{sec:.2f} s wall on {platform.processor() or platform.machine()}, nproc={os.cpu_count()}",
+                  f"{sec:.2f} s wall on {cpu_model()}, nproc={os.cpu_count()}",
     }
 
 
@@ -141,13 +159,15 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-explicit", action="store_true", help="skip the euler / rk3 stepSolution leg")
-    ap.add_argument("--weno", action="store_true", help="also run the 1024x384 WENO FP32-LHS error case")
+    ap.add_argument("--no-weno", action="store_true", help="skip the 1024x384 WENO FP32-LHS case")
     ap.add_argument("--sweep-cases", type=int, default=256)
     ap.add_argument("--train-cases", type=int, default=384)
     ap.add_argument("--train-steps", type=int, default=20000)
     ap.add_argument("--stall-window", type=int, default=2000)
-    ap.add_argument("--heldout", type=int, default=0,
-                    help="also evaluate the warm start on this many held-out 256x128 cases (e.g. 256)")
+    ap.add_argument("--heldout", type=int, default=256,
+                    help="held-out 256x128 cases for the warm start's own evaluation (0: skip)")
+    ap.add_argument("--no-deep", action="store_true",
+                    help="skip the FAST vs EXACT comparison beyond L2 1e-10")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one plain 256 MiB device-to-device copy (counter calibration under rocprofv3)")
     args = ap.parse_args()
@@ -159,7 +179,7 @@ def main():
 
     from hamstr_oop_amd import hip, meshgen
     mesh = meshgen.naca_ogrid(args.nc, args.nr, t=0.12, tri_layers=args.tri_layers)
-    phase_faces, n_proc = phase_face_counts(mesh)
+    phase_faces, n_proc, phase_cells = phase_face_counts(mesh)
     adi_mode = hip.ADI_FAST if args.adi == "fast" else hip.ADI_EXACT
 
     def run(lhs, mode=None, steps=None):
@@ -202,7 +222,7 @@ def main():
 
     n, dt, stages, l2, dt_ev = run(hip.LHS_FP64)
     value = mesh.n_cell * n / dt
-    total_b, stage_b = algorithmic_bytes(mesh.n_cell, n_proc, phase_faces, 8)
+    total_b, stage_b = algorithmic_bytes(mesh.n_cell, n_proc, phase_faces, 8, phase_cells=phase_cells)
     # dominant kernel by event time; its algorithmic bytes per launch (a phase's
     # bytes are split over the kernels of that phase in proportion to time)
     group = {"rhs_faces": "rhs", "rhs_cells": "rhs", "accum_dq": "linear", "lin_faces": "linear",
@@ -262,7 +282,11 @@ def main():
             out["exact_adi_fp64"]["time_to_tol"] = ce
     if not args.no_fp32:
         n32, dt32, st32, _, _ = run(hip.LHS_FP32)
-        out["fp32_lhs"] = {"value": mesh.n_cell * n32 / dt32, "ms_per_step": dt32 / n32 * 1e3,
+        v32 = mesh.n_cell * n32 / dt32
+        b32, _ = algorithmic_bytes(mesh.n_cell, n_proc, phase_faces, 4, phase_cells=phase_cells)
+        out["fp32_lhs"] = {"value": v32, "ms_per_step": dt32 / n32 * 1e3,
+                           "iteration_roofline": {"bytes_per_cell_iter": b32, "achieved_gbs": v32 * b32 / 1e9,
+                                                  "frac": v32 * b32 / 1e9 / HBM_PEAK_GBS},
                            "stage_ms": {k: round(v, 5) for k, v in st32.items()}}
         if not args.no_converge:
             import numpy as np
@@ -270,17 +294,56 @@ def main():
             scale = np.abs(q64.reshape(-1, 4)).max(axis=0)
             c32["max_field_diff_vs_fp64_rel"] = float((np.abs(q32 - q64).reshape(-1, 4) / scale).max())
             out["fp32_lhs"]["time_to_tol"] = c32
+    if args.adi == "fast" and not args.no_exact and not args.no_deep and not args.no_converge:
+        out["fast_vs_exact_deep"] = deep_section(args, mesh)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(mesh, args.mach, args.alpha, args.order, args.cfl, args.cpu_budget)
     if not args.no_explicit:
         out["explicit_schemes"] = explicit_section(args, mesh, n_proc)
-    if args.weno:
+    if not args.no_weno:
         out["weno_transonic_fp32_lhs"] = weno_section(args)
     if args.copy_probe:
         out["copy_probe_bytes"] = copy_probe(256 << 20)
     if not args.no_sweep and args.sweep_cases > 0:
         out["sweep"] = sweep_section(args, with_cpu=not args.no_cpu)
     print(json.dumps(out), flush=True)
+
+
+def deep_section(args, mesh, steps=8000):
+    """FAST against EXACT beyond L2 = 1e-10.
+
+    On the bench mesh the scheme settles into a limit cycle (its norm stays
+    near 3.2e-11 from about step 7,000 on, in the oracle as on the GPU), so
+    both run `steps` iterations and the band of the norm over the last 1,000
+    is reported with the field difference at the end.  The all-quad variant of
+    the same mesh converges to the roundoff floor: there both run to
+    L2 <= 1e-13 and the converged fields are compared."""
+    import numpy as np
+    from hamstr_oop_amd import hip, meshgen
+
+    def two(m, **kw):
+        qs, recs = {}, {}
+        for name, mode in (("fast", hip.ADI_FAST), ("exact", hip.ADI_EXACT)):
+            s = hip.Solver(m, args.mach, args.alpha, args.order, args.cfl, adi_mode=mode)
+            t0 = time.perf_counter()
+            n, l2, _ = s.step(kw["n"], tol=kw.get("tol", 0.0), check_every=50)
+            s.sync()
+            recs[name] = {"iterations": int(n), "seconds": time.perf_counter() - t0, "final_l2": float(l2[-1])}
+            if kw.get("tol", 0.0) == 0.0:
+                tail = l2[-1000:]
+                recs[name]["l2_band_last_1000"] = [float(tail.min()), float(tail.max())]
+            qs[name] = s.get("q").reshape(-1, 4)
+            s.close()
+        scale = np.abs(qs["exact"]).max(axis=0)
+        return recs, float((np.abs(qs["fast"] - qs["exact"]) / scale).max())
+
+    out = {}
+    recs, d = two(mesh, n=steps)
+    out["bench_mesh_limit_cycle"] = {"steps": steps, **recs, "max_field_diff_rel": d}
+    quad = meshgen.naca_ogrid(args.nc, args.nr, t=0.12, tri_layers=0)
+    recs, d = two(quad, n=args.max_iters, tol=1e-13)
+    out["all_quad_to_1e-13"] = {"tol": 1e-13, **recs, "max_field_diff_rel": d, "cells": quad.n_cell}
+    return out
 
 
 def explicit_bytes(n_cell, n_proc, scheme):
@@ -354,7 +417,8 @@ def explicit_cpu_baseline(mesh, mach, alpha, order, cfl, scheme, budget_s=4.0, m
         _, _, _, sec = oc.step_scheme(scheme, n)
         kind, what = "port", "oracle C++ restatement, -O2"
     return {"value": mesh.n_cell * n / sec, "unit": "cell-iters/s", "cores": 1, "kind": kind,
-            "sample": f"{n} {scheme} steps (after 1 warm-up) from freestream; {what}; {sec:.2f} s wall"}
+            "sample": f"{n} {scheme} steps (after 1 warm-up) from freestream; {what}; {sec:.2f} s wall on "
+                      f"{cpu_model()}"}
 
 
 def weno_section(args, nc=1024, nr=412, tri=28, mach=0.8, alpha=1.25):
@@ -535,8 +599,8 @@ def sweep_cpu_baseline(specs, meshes, cold64, iters=40):
     need = float(sum(m.n_cell * (it if st > 0 else 0) for m, it, st in zip(meshes, cold64.iters, cold64.status)))
     return {"value": rate * cores, "unit": "cell-iters/s", "cores": cores, "kind": "port",
             "per_core": rate, "time_to_tol_extrapolated_s": need / (rate * cores),
-            "sample": f"{iters} iterations of {len(picks)} cases (2 per mesh size), one case per thread; "
-                      f"time to tol = sum of cells x FP64 GPU iterations / (per-core rate x {cores})"}
+            "sample": f"{iters} iterations of {len(picks)} cases (2 per mesh size), one case per thread on "
+                      f"{cpu_model()}; time to tol = sum of cells x FP64 GPU iterations / (per-core rate x {cores})"}
 
 
 if __name__ == "__main__":
