@@ -3,7 +3,7 @@
 cells), MUSCL, M=0.5, alpha=2, CFL 10, from the freestream.
 
 tests/golden/c2_oracle.npz holds the oracle's own 10,000-step run (written by
-tests/golden/make_c2_oracle.py; the oracle is pinned bit for bit to the
+tests/golden/make_oracle_runs.py; the oracle is pinned bit for bit to the
 reference in tests/test_oracle_golden.py, and runs the orphan-face extension
 that a mesh with triangles needs).  The reference's own code cannot compute
 this mesh.
