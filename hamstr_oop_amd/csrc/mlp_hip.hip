@@ -1,11 +1,14 @@
 // mlp_hip.hip -- the learned warm start: an MLP (6 -> 256 x 4 SiLU -> 4) that
 // maps (Mach, alpha, thickness, cell centroid x, y, wall distance) to the
-// primitive state (rho, u, v, p).  The three 256x256 layers run on BF16
-// matrix cores (v_mfma_f32_32x32x8_bf16, FP32 accumulation) for the forward
-// pass and both backward products; the 6-wide input and 4-wide output layers
-// run on the vector ALUs in FP32.  Weights are FP32 masters with BF16 copies;
-// training is Adam on mean-squared error.  All reductions are split into fixed
-// partials and summed in a fixed order, so training is deterministic.
+// primitive state (rho, u, v, p).  The three 256x256 layers run on gfx950's
+// BF16 matrix cores (v_mfma_f32_32x32x16_bf16, FP32 accumulation) for the
+// forward pass and both backward products, with 16-byte operand loads (the
+// activations row-major, the weights stored both ways) and, for the weight
+// gradients, batch tiles transposed through LDS; the 6-wide input and 4-wide
+// output layers run on the vector ALUs in FP32.  Weights are FP32 masters with
+// BF16 copies; training is Adam on mean-squared error.  Every reduction over
+// the batch is split into fixed chunks whose partials one kernel sums in a
+// fixed order, so training is deterministic.
 //
 // The network only supplies the initial state of a solve (hamstr_warm_start);
 // the ADI iteration decides convergence, and cells whose prediction is not a
@@ -42,11 +45,22 @@ constexpr int oW3 = oB2 + kH, oB3 = oW3 + kH * kH;
 constexpr int oW4 = oB3 + kH, oB4 = oW4 + kH * kH;
 constexpr int oW5 = oB4 + kH, oB5 = oW5 + kH * kOut;
 constexpr int kParams = oB5 + kOut;  // 200196
-constexpr int kSplit = 64;           // split-K partials for reductions over the batch
+constexpr int kSplit = 16;           // at most this many batch chunks per reduction
 
 typedef unsigned short bf16;
-typedef short v4s __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+typedef unsigned short v8u __attribute__((ext_vector_type(8)));
 typedef float v16f __attribute__((ext_vector_type(16)));
+
+// batch chunks of the reductions: `rows` per chunk (a multiple of the
+// matrix-core K step, 16), `n` chunks, n <= kSplit
+struct Chunks {
+  int rows, n;
+};
+inline Chunks chunks_of(int B) {
+  const int rows = ((B + kSplit - 1) / kSplit + 15) / 16 * 16;
+  return Chunks{rows, (B + rows - 1) / rows};
+}
 
 __device__ __forceinline__ bf16 f2bf(float f) {
   unsigned u = __float_as_uint(f);
@@ -57,54 +71,127 @@ __device__ __forceinline__ float bf2f(bf16 h) { return __uint_as_float(((unsigne
 __device__ __forceinline__ float sigm(float z) { return 1.0f / (1.0f + __expf(-z)); }
 
 // --------------------------------------------------------------------------
-// C[M x 256] (+)= A[M x K] * B[K x 256] on matrix cores.  One wave per 32x32
-// tile, four waves per block covering 32 rows x 128 columns; K in steps of 8.
-// Element (m,k) of A is a[m*sam + k*sak], element (k,n) of B is b[k*sbk + n*sbn].
-// Lane l supplies A[m0 + l%32][k0 + 4*(l/32) + j] and B[k0 + 4*(l/32) + j][n0 + l%32],
-// j = 0..3, and holds D[m0 + 8*(v/4) + 4*(l/32) + v%4][n0 + l%32] in register v.
-// EPI 0: Z = acc + bias -> zout (f32), H = silu(Z) -> hout (bf16)
-// EPI 1: plain store of acc -> zout (f32)
-// EPI 2: split-K partial over blockIdx.z: rows of A in [z*kchunk, ...) -> zout + z*M*256
+// C[M x 256] = A[M x 256] * B on matrix cores, B given as Bt[n][k] (both with
+// k contiguous, so every operand fragment is one 16-byte load).  One 32x32
+// tile per wave; a workgroup of four waves covers 32 rows and 128 columns
+// (grid M/32 x 2); K in steps of 16.  Lane l supplies
+// A[m0 + l%32][k0 + 8*(l/32) + j] and Bt[n0 + l%32][k0 + 8*(l/32) + j],
+// j = 0..7, and holds D[m0 + 8*(v/4) + 4*(l/32) + v%4][n0 + l%32] in v.
+// EPI 0 (forward): Z = acc + bias -> zout, H = silu(Z) -> hout (bf16)
+// EPI 1 (backward): acc is dH of the layer below; dZ = dH silu'(Z) with Z
+//   from zin -> dzout (f32) and dzbout (bf16)
 // --------------------------------------------------------------------------
 template <int EPI>
-__global__ __launch_bounds__(256) void k_gemm_mfma(const bf16* __restrict__ a, long sam, long sak,
-                                                   const bf16* __restrict__ b, long sbk, long sbn, int M, int K,
-                                                   int kchunk, float* __restrict__ zout,
-                                                   const float* __restrict__ bias, bf16* __restrict__ hout) {
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int m0 = blockIdx.x * 32;
-  const int n0 = blockIdx.y * 128 + wave * 32;
-  int kb = 0, ke = K;
-  if (EPI == 2) {
-    kb = blockIdx.z * kchunk;
-    ke = min(K, kb + kchunk);
-  }
-  const int r = lane & 31, half = lane >> 5;
+__global__ __launch_bounds__(256) void k_gemm_rows(const bf16* __restrict__ A, const bf16* __restrict__ Bt,
+                                                   float* __restrict__ zout, const float* __restrict__ bias,
+                                                   bf16* __restrict__ hout, const float* __restrict__ zin,
+                                                   float* __restrict__ dzout, bf16* __restrict__ dzbout) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m0 = blockIdx.x * 32, n0 = blockIdx.y * 128 + wave * 32;
+  const int r = lane & 31, kh = 8 * (lane >> 5);
+  const bf16* arow = A + (long)(m0 + r) * kH + kh;
+  const bf16* brow = Bt + (long)(n0 + r) * kH + kh;
   v16f acc = {};
-  for (int k0 = kb; k0 < ke; k0 += 8) {
-    v4s av, bv;
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int k = k0 + 4 * half + j;
-      av[j] = (short)a[(long)(m0 + r) * sam + (long)k * sak];
-      bv[j] = (short)b[(long)k * sbk + (long)(n0 + r) * sbn];
-    }
-    acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, acc, 0, 0, 0);
+  for (int k0 = 0; k0 < kH; k0 += 16) {
+    const v8bf a = *reinterpret_cast<const v8bf*>(arow + k0);
+    const v8bf b = *reinterpret_cast<const v8bf*>(brow + k0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
   }
-  float* out = zout + (EPI == 2 ? (long)blockIdx.z * M * kH : 0);
 #pragma unroll
   for (int v = 0; v < 16; v++) {
-    const int row = m0 + 8 * (v / 4) + 4 * half + (v % 4);
+    const int row = m0 + 8 * (v / 4) + 4 * (lane >> 5) + (v % 4);
     const int col = n0 + r;
     const long o = (long)row * kH + col;
     if (EPI == 0) {
       const float z = acc[v] + bias[col];
-      out[o] = z;
+      zout[o] = z;
       hout[o] = f2bf(z * sigm(z));
     } else {
-      out[o] = acc[v];
+      const float zz = zin[o];
+      const float sg = sigm(zz);
+      const float g = acc[v] * (sg + zz * sg * (1.0f - sg));
+      dzout[o] = g;
+      dzbout[o] = f2bf(g);
     }
+  }
+}
+
+// --------------------------------------------------------------------------
+// Weight gradient of a hidden layer over one batch chunk, dW = H^T dZ
+// (m = input unit i, n = output unit j, k = batch row), into the chunk's
+// partials part[z][oW + i*256 + j].  A workgroup covers a 64x64 block of dW
+// (four waves of 32x32).  The chunk's rows go through LDS up to 256 at a
+// time, transposed so that every fragment is one 16-byte LDS read: thread t
+// takes the 16-byte segment t/32 of the block's units in rows 2p, 2p+1 (+64m),
+// p = t%32, issues all its loads first and writes each unit's pair of rows as
+// one 4-byte word.  The workgroups of the first row of blocks also sum the
+// chunk's FP32 dZ into the bias partials part[z][oB + j], in a fixed order.
+// --------------------------------------------------------------------------
+constexpr int kDwRows = 256;           // batch rows per LDS stage
+constexpr int kDwPitch = kDwRows + 8;  // LDS row pitch (elements); 16-byte aligned rows
+__global__ __launch_bounds__(256) void k_gemm_dw(const bf16* __restrict__ H, const bf16* __restrict__ dZb,
+                                                 const float* __restrict__ dz, int B, int rows,
+                                                 float* __restrict__ part, int oW, int oB) {
+  __shared__ __attribute__((aligned(16))) bf16 Ht[64][kDwPitch];
+  __shared__ __attribute__((aligned(16))) bf16 Zt[64][kDwPitch];
+  __shared__ float red[4][64];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int i0 = blockIdx.x * 64, j0 = blockIdx.y * 64, z = blockIdx.z;
+  const int r0 = z * rows, r1 = min(B, r0 + rows);
+  const int wi = wave & 1, wj = wave >> 1;
+  const int r = lane & 31, kh = 8 * (lane >> 5);
+  const int seg = t >> 5, pr = t & 31;
+  float* out = part + (long)z * kParams;
+  v16f acc = {};
+  for (int s0 = r0; s0 < r1; s0 += kDwRows) {
+    const int len = min(kDwRows, r1 - s0);  // a multiple of 16
+    v8u hv[8], zv[8];
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+      for (int w = 0; w < 2; w++) {
+        const int q = 64 * m + 2 * pr + w;
+        if (q < len) {
+          const long base = (long)(s0 + q) * kH + seg * 8;
+          hv[2 * m + w] = *reinterpret_cast<const v8u*>(H + base + i0);
+          zv[2 * m + w] = *reinterpret_cast<const v8u*>(dZb + base + j0);
+        }
+      }
+    __syncthreads();  // the previous stage's reads are done
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      const int q = 64 * m + 2 * pr;
+      if (q < len) {
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+          *reinterpret_cast<unsigned*>(&Ht[seg * 8 + e][q]) = (unsigned)hv[2 * m][e] | ((unsigned)hv[2 * m + 1][e] << 16);
+          *reinterpret_cast<unsigned*>(&Zt[seg * 8 + e][q]) = (unsigned)zv[2 * m][e] | ((unsigned)zv[2 * m + 1][e] << 16);
+        }
+      }
+    }
+    __syncthreads();
+    for (int k0 = 0; k0 < len; k0 += 16) {
+      const v8bf a = *reinterpret_cast<const v8bf*>(&Ht[wi * 32 + r][k0 + kh]);
+      const v8bf b = *reinterpret_cast<const v8bf*>(&Zt[wj * 32 + r][k0 + kh]);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 16; v++) {
+    const int i = i0 + wi * 32 + 8 * (v / 4) + 4 * (lane >> 5) + (v % 4);
+    const int j = j0 + wj * 32 + r;
+    out[oW + (long)i * kH + j] = acc[v];
+  }
+  if (blockIdx.x == 0) {
+    // bias: four interleaved partial sums per column, added in a fixed order
+    const int j = j0 + (t & 63), q = t >> 6;
+    float sum = 0.f;
+#pragma unroll 8
+    for (int row = r0 + q; row < r1; row += 4) sum += dz[(long)row * kH + j];
+    red[q][t & 63] = sum;
+    __syncthreads();
+    if (q == 0) out[oB + j] = ((red[0][t] + red[1][t]) + red[2][t]) + red[3][t];
   }
 }
 
@@ -121,33 +208,32 @@ __global__ void k_layer1(const float* __restrict__ x, const float* __restrict__ 
   h[id] = f2bf(s * sigm(s));
 }
 
-// output layer (N = 4): Y = H4 W5 + b5
-__global__ void k_out(const bf16* __restrict__ h, const float* __restrict__ p, int B, float* __restrict__ y) {
-  const int row = blockIdx.x * blockDim.x + threadIdx.x;
-  if (row >= B) return;
-  float s[kOut];
-#pragma unroll
-  for (int j = 0; j < kOut; j++) s[j] = p[oB5 + j];
-  for (int k = 0; k < kH; k++) {
-    const float hv = bf2f(h[(long)row * kH + k]);
-#pragma unroll
-    for (int j = 0; j < kOut; j++) s[j] += hv * p[oW5 + k * kOut + j];
-  }
-#pragma unroll
-  for (int j = 0; j < kOut; j++) y[row * kOut + j] = s[j];
-}
-
-// dY = 2 (Y - T) / (B * 4); per-block loss partials
-__global__ void k_loss(const float* __restrict__ y, const float* __restrict__ t, int B, float* __restrict__ dy,
-                       float* __restrict__ lpart) {
+// output layer (N = 4), one thread per output: Y = H4 W5 + b5, summed over
+// the 256 hidden units in order.  With t: dY = 2 (Y - T) / (B * 4) and the
+// per-block loss partials, summed in a fixed tree.
+__global__ __launch_bounds__(256) void k_out_loss(const bf16* __restrict__ h, const float* __restrict__ p, int B,
+                                                  float* __restrict__ y, const float* __restrict__ t,
+                                                  float* __restrict__ dy, float* __restrict__ lpart) {
   __shared__ float sh[256];
   const int id = blockIdx.x * blockDim.x + threadIdx.x;
   float l = 0.f;
   if (id < B * kOut) {
-    const float d = y[id] - t[id];
-    dy[id] = 2.0f * d / (float)(B * kOut);
-    l = d * d / (float)(B * kOut);
+    const int row = id / kOut, j = id % kOut;
+    float s = p[oB5 + j];
+    const bf16* hr = h + (long)row * kH;
+    for (int k0 = 0; k0 < kH; k0 += 8) {
+      const v8u hv = *reinterpret_cast<const v8u*>(hr + k0);
+#pragma unroll
+      for (int e = 0; e < 8; e++) s += bf2f(hv[e]) * p[oW5 + (k0 + e) * kOut + j];
+    }
+    y[id] = s;
+    if (t) {
+      const float d = s - t[id];
+      dy[id] = 2.0f * d / (float)(B * kOut);
+      l = d * d / (float)(B * kOut);
+    }
   }
+  if (!t) return;  // the same for the whole launch
   sh[threadIdx.x] = l;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
@@ -157,78 +243,100 @@ __global__ void k_loss(const float* __restrict__ y, const float* __restrict__ t,
   if (threadIdx.x == 0) lpart[blockIdx.x] = sh[0];
 }
 
-// dH4 = dY W5^T (f32)
-__global__ void k_out_bwd_dh(const float* __restrict__ dy, const float* __restrict__ p, int B,
-                             float* __restrict__ dh) {
+// dZ4 = (dY W5^T) silu'(Z4), as f32 and bf16
+__global__ void k_out_bwd_dz(const float* __restrict__ dy, const float* __restrict__ p,
+                             const float* __restrict__ z, int B, float* __restrict__ dz,
+                             bf16* __restrict__ dzb) {
   const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (id >= (long)B * kH) return;
   const int row = id / kH, col = id % kH;
-  float s = 0.f;
+  float dh = 0.f;
 #pragma unroll
-  for (int j = 0; j < kOut; j++) s += dy[row * kOut + j] * p[oW5 + col * kOut + j];
-  dh[id] = s;
-}
-
-// dZ = dH * silu'(Z), as f32 and bf16
-__global__ void k_silu_bwd(const float* __restrict__ dh, const float* __restrict__ z, long n,
-                           float* __restrict__ dz, bf16* __restrict__ dzb) {
-  const long id = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= n) return;
+  for (int j = 0; j < kOut; j++) dh += dy[row * kOut + j] * p[oW5 + col * kOut + j];
   const float zz = z[id];
-  const float s = sigm(zz);
-  const float g = dh[id] * (s + zz * s * (1.0f - s));
+  const float sg = sigm(zz);
+  const float g = dh * (sg + zz * sg * (1.0f - sg));
   dz[id] = g;
   dzb[id] = f2bf(g);
 }
 
-// split-K partial column products over the batch for the VALU layers:
-// part[z][i][j] = sum_{rows in chunk z} L[row][i] * R[row][j]; L f32 or bf16
-template <typename TL>
-__global__ void k_outer_partial(const TL* __restrict__ L, int nl, const float* __restrict__ R, int nr, int B,
-                                int chunk, float* __restrict__ part) {
-  const int i = blockIdx.x, z = blockIdx.y;
-  const int r0 = z * chunk, r1 = min(B, r0 + chunk);
-  for (int j = threadIdx.x; j < nr; j += blockDim.x) {
-    float s = 0.f;
-    for (int row = r0; row < r1; row++) {
-      float lv;
-      if constexpr (sizeof(TL) == 2)
-        lv = bf2f(L[(long)row * nl + i]);
-      else
-        lv = L[(long)row * nl + i];
-      s += lv * R[(long)row * nr + j];
-    }
-    part[((long)z * nl + i) * nr + j] = s;
+// W5 and b5 gradient partials of one batch chunk (block z; thread = hidden
+// unit i and row group g = t/256, which takes the chunk's rows g, g+4, ...):
+// part[z][oW5 + 4i + j] = sum_rows H4[row][i] dY[row][j] and
+// part[z][oB5 + j] = sum_rows dY[row][j], the four groups added in order
+__global__ __launch_bounds__(1024) void k_outer_h4(const bf16* __restrict__ h4, const float* __restrict__ dy, int B,
+                                                   int rows, float* __restrict__ part) {
+  __shared__ float red[4][kH][kOut + 1];
+  const int i = threadIdx.x & (kH - 1), g = threadIdx.x >> 8, z = blockIdx.x;
+  const int r0 = z * rows, r1 = min(B, r0 + rows);
+  float s[kOut] = {0.f, 0.f, 0.f, 0.f}, sb = 0.f;
+#pragma unroll 4
+  for (int row = r0 + g; row < r1; row += 4) {
+    const float hv = bf2f(h4[(long)row * kH + i]);
+    const float4 d = *reinterpret_cast<const float4*>(dy + row * kOut);
+    s[0] += hv * d.x;
+    s[1] += hv * d.y;
+    s[2] += hv * d.z;
+    s[3] += hv * d.w;
+    if (i < kOut) sb += (i == 0 ? d.x : i == 1 ? d.y : i == 2 ? d.z : d.w);
   }
+#pragma unroll
+  for (int j = 0; j < kOut; j++) red[g][i][j] = s[j];
+  red[g][i][kOut] = sb;
+  __syncthreads();
+  if (g != 0) return;
+  float* out = part + (long)z * kParams;
+#pragma unroll
+  for (int j = 0; j < kOut; j++) out[oW5 + i * kOut + j] = ((red[0][i][j] + red[1][i][j]) + red[2][i][j]) + red[3][i][j];
+  if (i < kOut) out[oB5 + i] = ((red[0][i][kOut] + red[1][i][kOut]) + red[2][i][kOut]) + red[3][i][kOut];
 }
 
-// column sums (bias gradients) split over the batch
-__global__ void k_colsum_partial(const float* __restrict__ d, int ncol, int B, int chunk,
-                                 float* __restrict__ part) {
-  const int z = blockIdx.y;
-  const int r0 = z * chunk, r1 = min(B, r0 + chunk);
-  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < ncol; j += gridDim.x * blockDim.x) {
-    float s = 0.f;
-    for (int row = r0; row < r1; row++) s += d[(long)row * ncol + j];
-    part[(long)z * ncol + j] = s;
+// W1 and b1 gradient partials of one batch chunk (block z; thread = hidden
+// unit j and row group g): part[z][oW1 + 256k + j] = sum_rows X[row][k] dZ1[row][j]
+// and part[z][oB1 + j] = sum_rows dZ1[row][j], the four groups added in order
+__global__ __launch_bounds__(1024) void k_outer_x(const float* __restrict__ x, const float* __restrict__ dz, int B,
+                                                  int rows, float* __restrict__ part) {
+  __shared__ float red[4][kIn + 1][kH];
+  const int j = threadIdx.x & (kH - 1), g = threadIdx.x >> 8, z = blockIdx.x;
+  const int r0 = z * rows, r1 = min(B, r0 + rows);
+  float s[kIn] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, sb = 0.f;
+#pragma unroll 4
+  for (int row = r0 + g; row < r1; row += 4) {
+    const float d = dz[(long)row * kH + j];
+#pragma unroll
+    for (int k = 0; k < kIn; k++) s[k] += x[row * kIn + k] * d;
+    sb += d;
   }
+#pragma unroll
+  for (int k = 0; k < kIn; k++) red[g][k][j] = s[k];
+  red[g][kIn][j] = sb;
+  __syncthreads();
+  if (g != 0) return;
+  float* out = part + (long)z * kParams;
+#pragma unroll
+  for (int k = 0; k < kIn; k++) out[oW1 + k * kH + j] = ((red[0][k][j] + red[1][k][j]) + red[2][k][j]) + red[3][k][j];
+  out[oB1 + j] = ((red[0][kIn][j] + red[1][kIn][j]) + red[2][kIn][j]) + red[3][kIn][j];
 }
 
-// out[i] = sum_z part[z][i], fixed order
-__global__ void k_reduce(const float* __restrict__ part, int nz, long n, float* __restrict__ out) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// the gradient: g[i] = sum over the nz chunks of part[z][i], in chunk order
+__global__ void k_reduce_all(const float* __restrict__ part, int nz, float* __restrict__ g) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= kParams) return;
   float s = 0.f;
-  for (int z = 0; z < nz; z++) s += part[(long)z * n + i];
-  out[i] = s;
+  for (int z = 0; z < nz; z++) s += part[(long)z * kParams + i];
+  g[i] = s;
 }
 
 // Adam with the step's learning rate (cosine from lr to lr_final over `steps`)
 // and bias corrections computed from the device step counter, so that a
-// captured training step can be replayed unchanged
+// captured training step can be replayed unchanged.  The new W2..W4 also go
+// to their BF16 copies (row-major and transposed) for the next step.
+struct Bf16Copies {
+  bf16 *wb[3], *wtb[3];
+};
 __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
                        float* __restrict__ v, int n, const int* __restrict__ step, long t0, int steps, float lr0,
-                       float lr_final, float b1, float b2, float eps) {
+                       float lr_final, float b1, float b2, float eps, Bf16Copies cp) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int s = *step;
@@ -240,7 +348,17 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
   const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
   m[i] = mi;
   v[i] = vi;
-  p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  const float pn = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  p[i] = pn;
+#pragma unroll
+  for (int l = 0; l < 3; l++) {
+    const int w = i - (oW2 + l * (kH * kH + kH));  // W2, W3, W4 start kH*kH + kH apart
+    if (w >= 0 && w < kH * kH) {
+      const bf16 h = f2bf(pn);
+      cp.wb[l][w] = h;
+      cp.wtb[l][(w % kH) * kH + w / kH] = h;
+    }
+  }
 }
 
 // BF16 copies of a 256x256 weight: w (row-major K x N) and wt (N x K)
@@ -328,8 +446,8 @@ struct hamstr_mlp {
   float *x = nullptr, *t = nullptr, *y = nullptr, *dy = nullptr, *part = nullptr, *lpart = nullptr;
   float *z[kHidden] = {nullptr, nullptr, nullptr, nullptr};
   bf16* h[kHidden] = {nullptr, nullptr, nullptr, nullptr};
-  float *dh = nullptr, *dz = nullptr;
-  bf16* dzb = nullptr;
+  float* dz[2] = {nullptr, nullptr};  // dZ of the current and the next layer down (f32 and bf16)
+  bf16* dzb[2] = {nullptr, nullptr};
   // training set
   float *X = nullptr, *T = nullptr;
   long n = 0;
@@ -354,7 +472,7 @@ int refresh_bf16(hamstr_mlp* M) {
 
 int ensure_cap(hamstr_mlp* M, int B) {
   if (M->cap >= B) return HAMSTR_OK;
-  void* old[] = {M->x, M->t, M->y, M->dy, M->part, M->lpart, M->dh, M->dz, M->dzb,
+  void* old[] = {M->x, M->t, M->y, M->dy, M->part, M->lpart, M->dz[0], M->dz[1], M->dzb[0], M->dzb[1],
                  M->z[0], M->z[1], M->z[2], M->z[3], M->h[0], M->h[1], M->h[2], M->h[3]};
   for (void* q : old)
     if (q) (void)hipFree(q);
@@ -363,11 +481,12 @@ int ensure_cap(hamstr_mlp* M, int B) {
   MCHK(hipMalloc(&M->t, sizeof(float) * B * kOut));
   MCHK(hipMalloc(&M->y, sizeof(float) * B * kOut));
   MCHK(hipMalloc(&M->dy, sizeof(float) * B * kOut));
-  MCHK(hipMalloc(&M->part, sizeof(float) * (long)kSplit * kH * kH));
+  MCHK(hipMalloc(&M->part, sizeof(float) * (long)kSplit * kParams));
   MCHK(hipMalloc(&M->lpart, sizeof(float) * ((B * kOut + 255) / 256)));
-  MCHK(hipMalloc(&M->dh, sizeof(float) * bh));
-  MCHK(hipMalloc(&M->dz, sizeof(float) * bh));
-  MCHK(hipMalloc(&M->dzb, sizeof(bf16) * bh));
+  for (int k = 0; k < 2; k++) {
+    MCHK(hipMalloc(&M->dz[k], sizeof(float) * bh));
+    MCHK(hipMalloc(&M->dzb[k], sizeof(bf16) * bh));
+  }
   for (int l = 0; l < kHidden; l++) {
     MCHK(hipMalloc(&M->z[l], sizeof(float) * bh));
     MCHK(hipMalloc(&M->h[l], sizeof(bf16) * bh));
@@ -376,56 +495,55 @@ int ensure_cap(hamstr_mlp* M, int B) {
   return HAMSTR_OK;
 }
 
-// forward on M->x (B rows, B a multiple of 32) -> M->y
-int forward(hamstr_mlp* M, int B) {
+// the hidden layers on M->x (B rows, B a multiple of 32): Z1..Z4, H1..H4
+int forward_hidden(hamstr_mlp* M, int B) {
   const long bh = (long)B * kH;
   hipLaunchKernelGGL(k_layer1, dim3((bh + 255) / 256), dim3(256), 0, M->stream, M->x, M->p, B, M->z[0], M->h[0]);
   for (int l = 0; l < 3; l++)
-    hipLaunchKernelGGL((k_gemm_mfma<0>), dim3(B / 32, kH / 128), dim3(256), 0, M->stream, M->h[l], (long)kH, 1L,
-                       M->wtb[l], 1L, (long)kH, B, kH, 0, M->z[l + 1], M->p + kHB[l], M->h[l + 1]);
-  hipLaunchKernelGGL(k_out, dim3((B + 255) / 256), dim3(256), 0, M->stream, M->h[3], M->p, B, M->y);
+    hipLaunchKernelGGL((k_gemm_rows<0>), dim3(B / 32, 2), dim3(256), 0, M->stream, M->h[l], M->wtb[l], M->z[l + 1],
+                       M->p + kHB[l], M->h[l + 1], (const float*)nullptr, (float*)nullptr, (bf16*)nullptr);
   MCHK(hipGetLastError());
   return HAMSTR_OK;
 }
 
-// loss and gradient (into M->g) for the batch in M->x, M->t; *loss on the host
-int backward(hamstr_mlp* M, int B, float* loss) {
+// forward on M->x -> M->y
+int forward(hamstr_mlp* M, int B) {
+  int rc = forward_hidden(M, B);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_out_loss, dim3((B * kOut + 255) / 256), dim3(256), 0, M->stream, M->h[3], M->p, B, M->y,
+                     (const float*)nullptr, (float*)nullptr, (float*)nullptr);
+  MCHK(hipGetLastError());
+  return HAMSTR_OK;
+}
+
+// forward, loss and gradient (into M->g) for the batch in M->x, M->t; *loss
+// on the host when asked.  Every reduction over the batch writes per-chunk
+// partials into M->part, which k_reduce_all sums in chunk order.
+int forward_backward(hamstr_mlp* M, int B, float* loss) {
+  int rc = forward_hidden(M, B);
+  if (rc) return rc;
   const long bh = (long)B * kH;
   const int nlb = (B * kOut + 255) / 256;
-  hipLaunchKernelGGL(k_loss, dim3(nlb), dim3(256), 0, M->stream, M->y, M->t, B, M->dy, M->lpart);
-  const int chunk = (B + kSplit - 1) / kSplit;
-  const int chunk8 = (chunk + 7) / 8 * 8;  // matrix-core K steps of 8 rows
-  // output layer: dW5 = H4^T dY, db5 = sum dY, dH4 = dY W5^T
-  hipLaunchKernelGGL((k_outer_partial<bf16>), dim3(kH, kSplit), dim3(64), 0, M->stream, M->h[3], kH, M->dy, kOut, B,
-                     chunk, M->part);
-  hipLaunchKernelGGL(k_reduce, dim3((kH * kOut + 255) / 256), dim3(256), 0, M->stream, M->part, kSplit,
-                     (long)kH * kOut, M->g + oW5);
-  hipLaunchKernelGGL(k_colsum_partial, dim3(1, kSplit), dim3(64), 0, M->stream, M->dy, kOut, B, chunk, M->part);
-  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, M->stream, M->part, kSplit, (long)kOut, M->g + oB5);
-  hipLaunchKernelGGL(k_out_bwd_dh, dim3((bh + 255) / 256), dim3(256), 0, M->stream, M->dy, M->p, B, M->dh);
-  // hidden layers 4, 3, 2 on matrix cores
+  const Chunks c = chunks_of(B);
+  hipLaunchKernelGGL(k_out_loss, dim3(nlb), dim3(256), 0, M->stream, M->h[3], M->p, B, M->y, M->t, M->dy, M->lpart);
+  // output layer: dW5, db5; dZ4 = (dY W5^T) silu'(Z4)
+  hipLaunchKernelGGL(k_outer_h4, dim3(c.n), dim3(1024), 0, M->stream, M->h[3], M->dy, B, c.rows, M->part);
+  hipLaunchKernelGGL(k_out_bwd_dz, dim3((bh + 255) / 256), dim3(256), 0, M->stream, M->dy, M->p, M->z[3], B,
+                     M->dz[0], M->dzb[0]);
+  // hidden layers 4, 3, 2 on matrix cores: dW = H^T dZ (+ db), then the next
+  // layer's dZ = (dZ W^T) silu'(Z) into the other buffer
+  int cur = 0;
   for (int l = 2; l >= 0; l--) {
-    hipLaunchKernelGGL(k_silu_bwd, dim3((bh + 255) / 256), dim3(256), 0, M->stream, M->dh, M->z[l + 1], bh, M->dz,
-                       M->dzb);
-    // dW = H_l^T dZ: A = H_l^T (m = input unit, k = row), B = dZ (k = row, n = output unit)
-    hipLaunchKernelGGL((k_gemm_mfma<2>), dim3(kH / 32, kH / 128, kSplit), dim3(256), 0, M->stream, M->h[l], 1L,
-                       (long)kH, M->dzb, (long)kH, 1L, kH, B, chunk8, M->part, (const float*)nullptr, (bf16*)nullptr);
-    hipLaunchKernelGGL(k_reduce, dim3((kH * kH + 255) / 256), dim3(256), 0, M->stream, M->part, kSplit,
-                       (long)kH * kH, M->g + kHW[l]);
-    hipLaunchKernelGGL(k_colsum_partial, dim3(1, kSplit), dim3(256), 0, M->stream, M->dz, kH, B, chunk, M->part);
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, M->stream, M->part, kSplit, (long)kH, M->g + kHB[l]);
-    // dH_l = dZ W^T: A = dZ (m = row, k = output unit), B(k = output unit, n = input unit) = W[n][k]
-    hipLaunchKernelGGL((k_gemm_mfma<1>), dim3(B / 32, kH / 128), dim3(256), 0, M->stream, M->dzb, (long)kH, 1L,
-                       M->wb[l], 1L, (long)kH, B, kH, 0, M->dh, (const float*)nullptr, (bf16*)nullptr);
+    hipLaunchKernelGGL(k_gemm_dw, dim3(kH / 64, kH / 64, c.n), dim3(256), 0, M->stream, M->h[l], M->dzb[cur],
+                       M->dz[cur], B, c.rows, M->part, kHW[l], kHB[l]);
+    hipLaunchKernelGGL((k_gemm_rows<1>), dim3(B / 32, 2), dim3(256), 0, M->stream, M->dzb[cur], M->wb[l],
+                       (float*)nullptr, (const float*)nullptr, (bf16*)nullptr, M->z[l], M->dz[1 - cur],
+                       M->dzb[1 - cur]);
+    cur = 1 - cur;
   }
-  // layer 1: dZ1 = dH1 silu'(Z1); dW1 = X^T dZ1, db1 = sum dZ1
-  hipLaunchKernelGGL(k_silu_bwd, dim3((bh + 255) / 256), dim3(256), 0, M->stream, M->dh, M->z[0], bh, M->dz, M->dzb);
-  hipLaunchKernelGGL((k_outer_partial<float>), dim3(kIn, kSplit), dim3(256), 0, M->stream, M->x, kIn, M->dz, kH, B,
-                     chunk, M->part);
-  hipLaunchKernelGGL(k_reduce, dim3((kIn * kH + 255) / 256), dim3(256), 0, M->stream, M->part, kSplit,
-                     (long)kIn * kH, M->g + oW1);
-  hipLaunchKernelGGL(k_colsum_partial, dim3(1, kSplit), dim3(256), 0, M->stream, M->dz, kH, B, chunk, M->part);
-  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, M->stream, M->part, kSplit, (long)kH, M->g + oB1);
+  // layer 1: dW1 = X^T dZ1, db1 = sum dZ1; then the whole gradient
+  hipLaunchKernelGGL(k_outer_x, dim3(c.n), dim3(1024), 0, M->stream, M->x, M->dz[cur], B, c.rows, M->part);
+  hipLaunchKernelGGL(k_reduce_all, dim3((kParams + 255) / 256), dim3(256), 0, M->stream, M->part, c.n, M->g);
   MCHK(hipGetLastError());
   if (loss) {
     std::vector<float> lp(nlb);
@@ -485,8 +603,8 @@ int hamstr_mlp_destroy(hamstr_mlp* M) {
   if (!M) return HAMSTR_OK;
   (void)hipStreamSynchronize(M->stream);
   void* ptrs[] = {M->p, M->g, M->m, M->v, M->wb[0], M->wb[1], M->wb[2], M->wtb[0], M->wtb[1], M->wtb[2],
-                  M->x, M->t, M->y, M->dy, M->part, M->lpart, M->dh, M->dz, M->dzb, M->z[0], M->z[1],
-                  M->z[2], M->z[3], M->h[0], M->h[1], M->h[2], M->h[3], M->X, M->T};
+                  M->x, M->t, M->y, M->dy, M->part, M->lpart, M->dz[0], M->dz[1], M->dzb[0], M->dzb[1],
+                  M->z[0], M->z[1], M->z[2], M->z[3], M->h[0], M->h[1], M->h[2], M->h[3], M->X, M->T};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (M->dstep) (void)hipFree(M->dstep);
@@ -542,7 +660,7 @@ int hamstr_mlp_loss_grad(hamstr_mlp* M, int n, const float* x, const float* t, f
   if (rc) return rc;
   MCHK(hipMemcpyAsync(M->x, x, sizeof(float) * n * kIn, hipMemcpyHostToDevice, M->stream));
   MCHK(hipMemcpyAsync(M->t, t, sizeof(float) * n * kOut, hipMemcpyHostToDevice, M->stream));
-  if ((rc = forward(M, n)) || (rc = backward(M, n, loss))) return rc;
+  if ((rc = forward_backward(M, n, loss))) return rc;
   MCHK(hipMemcpyAsync(grad, M->g, sizeof(float) * kParams, hipMemcpyDeviceToHost, M->stream));
   MCHK(hipStreamSynchronize(M->stream));
   return HAMSTR_OK;
@@ -564,8 +682,8 @@ int hamstr_mlp_set_data(hamstr_mlp* M, long n, const float* x, const float* t) {
 
 // Adam on random batches of the training set, learning rate decaying from lr
 // to lr_final on a cosine; the batch losses stay on the device until the end.
-// One training step (gather, forward, backward, Adam, BF16 weight copies,
-// step counter) is captured as a HIP graph and replayed `steps` times.
+// One training step (gather, forward, backward, Adam with the BF16 weight
+// copies, step counter) is captured as a HIP graph and replayed `steps` times.
 int hamstr_mlp_train(hamstr_mlp* M, int steps, int batch, float lr, float lr_final, unsigned seed, float* losses) {
   if (!M || steps < 0 || batch <= 0 || batch % 32 || M->n <= 0 || !(lr > 0) || !(lr_final >= 0))
     return mfail(HAMSTR_ERR_ARG, "bad arguments");
@@ -583,11 +701,10 @@ int hamstr_mlp_train(hamstr_mlp* M, int steps, int batch, float lr, float lr_fin
     hipLaunchKernelGGL(k_gather, dim3((batch + 255) / 256), dim3(256), 0, M->stream, M->X, M->T, M->n, batch, seed,
                        M->dstep, M->x, M->t);
     int r;
-    if ((r = forward(M, batch))) return r;
-    if ((r = backward(M, batch, nullptr))) return r;
+    if ((r = forward_backward(M, batch, nullptr))) return r;
+    const Bf16Copies cp{{M->wb[0], M->wb[1], M->wb[2]}, {M->wtb[0], M->wtb[1], M->wtb[2]}};
     hipLaunchKernelGGL(k_adam, dim3((kParams + 255) / 256), dim3(256), 0, M->stream, M->p, M->g, M->m, M->v, kParams,
-                       M->dstep, t0, steps, lr, lr_final, b1, b2, eps);
-    if ((r = refresh_bf16(M))) return r;
+                       M->dstep, t0, steps, lr, lr_final, b1, b2, eps, cp);
     hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, M->stream, M->lpart, nlb, dloss, M->dstep);
     MCHK(hipGetLastError());
     return HAMSTR_OK;

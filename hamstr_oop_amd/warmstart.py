@@ -232,3 +232,33 @@ def case_samples(case: Case, q: np.ndarray, per_case: int, rng: np.random.Genera
 
 def batch_features(cases: Sequence[Case]) -> np.ndarray:
     return np.concatenate([cell_features(c.mesh, c.mach, c.alpha, c.thickness) for c in cases])
+
+
+def time_training(steps: int = 500, batch: int = 4096, graph: bool = True, n: int = 200000,
+                  seed: int = 1) -> float:
+    """Milliseconds per training step at `batch` on a synthetic data set of n
+    rows (6 normal features, 4 smooth targets); 20 untimed steps first."""
+    import time
+    rng = np.random.default_rng(seed)
+    x = rng.normal(size=(n, 6)).astype(np.float32)
+    t = np.tanh(0.7 * x[:, :4]).astype(np.float32)
+    mlp = MLP(seed=2)
+    mlp.set_data(x, t)
+    mlp.set_graph(graph)
+    mlp.train(20, batch=batch, lr=1e-3, lr_final=1e-4, seed=1)
+    t0 = time.perf_counter()
+    mlp.train(steps, batch=batch, lr=1e-3, lr_final=1e-4, seed=3)
+    ms = (time.perf_counter() - t0) / max(steps, 1) * 1e3
+    mlp.close()
+    return ms
+
+
+if __name__ == "__main__":
+    import argparse
+    ap = argparse.ArgumentParser(description="time the warm-start network's training step")
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--batch", type=int, default=4096)
+    ap.add_argument("--eager", action="store_true", help="launch each step's kernels instead of a graph")
+    a = ap.parse_args()
+    print(f"{a.steps} steps at batch {a.batch}: {time_training(a.steps, a.batch, not a.eager):.3f} ms/step",
+          flush=True)
