@@ -219,23 +219,15 @@ __device__ __forceinline__ void reconstruct_dface(const double (*f)[4], const do
   }
 }
 
-// --------------------------------------------------------------------------
-// rhs_faces: chain gather + reconstruction + Roe flux (+ Jacobians)
-// computeRHS.cpp:53-408
-// --------------------------------------------------------------------------
-// FASTJ: the FAST-mode Jacobians (reciprocals instead of divisions); the flux
-// and spectral radius are the reference's arithmetic either way
-template <int ORDER, typename TL, bool IMPL, bool FASTJ>
-__global__ __launch_bounds__(kBlock) void k_rhs_faces(FaceTab ft, Consts c, Live lv, const double* __restrict__ q,
-                                                      double* __restrict__ flux, double* __restrict__ sr,
-                                                      TL* __restrict__ ff, int* __restrict__ nanflag) {
-  const int p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (p >= c.nProc || (lv.face && !lv.face[p])) return;
-  const int code = ft.pslot[p];
+// The two states of processed face `code` (its pslot code) that its flux and
+// Jacobians take (computeRHS.cpp:53-260): for an orphan face the two cell
+// states (first order), else the states reconstructed along the chain,
+// oriented as the face is
+template <int ORDER>
+__device__ __forceinline__ void face_states(const FaceTab& ft, const double* __restrict__ q, double eps, double qt,
+                                            int code, double* left, double* right) {
   const int s = code & kSlotMask;
-  double left[4], right[4];
   if (code & kOrphanSlot) {
-    // orphan face: first order, the two cell states
     put4(left, load_state(ft.src[s], q, ft.refmtx, ft.qinf));
     put4(right, load_state(ft.src[s + 1], q, ft.refmtx, ft.qinf));
   } else {
@@ -246,20 +238,67 @@ __global__ __launch_bounds__(kBlock) void k_rhs_faces(FaceTab ft, Consts c, Live
 #pragma unroll
     for (int j = lo; j <= hi; j++) put4(f[j], load_state(ft.src[s - 2 + j], q, ft.refmtx, ft.qinf));
     double qln[4], qrn1[4];
-    reconstruct_face<ORDER>(f, c.eps, c.qt, qln, qrn1);
+    reconstruct_face<ORDER>(f, eps, qt, qln, qrn1);
 #pragma unroll
     for (int k = 0; k < 4; k++) {
       left[k] = flip ? qln[k] : qrn1[k];
       right[k] = flip ? qrn1[k] : qln[k];
     }
   }
+}
+
+// Where a FAST ADI kernel takes the face Jacobians from.  order 0: the array
+// ff k_rhs_faces stored.  order 1, 3 or 5: computed again from q at that
+// order (face_jac_q), which the fused FAST iteration does for the chain
+// faces instead of storing and reading back 32 values per face.
+struct JacSrc {
+  FaceTab ft;
+  const double* q;
+  double gamma, qt, eps;
+  int order;
+};
+
+// the FAST-mode Jacobians of processed face p from q: the states and the
+// code k_rhs_faces uses (face_jacobians is opaque to its surroundings), so
+// bit for bit what k_rhs_faces stores
+template <typename TL>
+__device__ __forceinline__ void face_jac_q(const JacSrc& js, int p, TL* lm, TL* rm) {
+  const int code = js.ft.pslot[p];
+  double left[4], right[4];
+  if (js.order == 1)
+    face_states<1>(js.ft, js.q, js.eps, js.qt, code, left, right);
+  else if (js.order == 3)
+    face_states<3>(js.ft, js.q, js.eps, js.qt, code, left, right);
+  else
+    face_states<5>(js.ft, js.q, js.eps, js.qt, code, left, right);
+  const double2 n = *reinterpret_cast<const double2*>(js.ft.nvec + 2 * (size_t)p);
+  face_jacobians<TL, true>(n.x, n.y, left, right, js.gamma, lm, rm);
+}
+
+// --------------------------------------------------------------------------
+// rhs_faces: chain gather + reconstruction + Roe flux (+ Jacobians)
+// computeRHS.cpp:53-408
+// --------------------------------------------------------------------------
+// FASTJ: the FAST-mode Jacobians (reciprocals instead of divisions); the flux
+// and spectral radius are the reference's arithmetic either way
+// ff_from: the Jacobians are computed and stored for processed faces p >=
+// ff_from only (the fused FAST iteration stores none of the chain faces',
+// which its ADI kernels compute again from q)
+template <int ORDER, typename TL, bool IMPL, bool FASTJ>
+__global__ __launch_bounds__(kBlock) void k_rhs_faces(FaceTab ft, Consts c, Live lv, const double* __restrict__ q,
+                                                      double* __restrict__ flux, double* __restrict__ sr,
+                                                      TL* __restrict__ ff, int ff_from, int* __restrict__ nanflag) {
+  const int p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (p >= c.nProc || (lv.face && !lv.face[p])) return;
+  double left[4], right[4];
+  face_states<ORDER>(ft, q, c.eps, c.qt, ft.pslot[p], left, right);
   const double2 n = *reinterpret_cast<const double2*>(ft.nvec + 2 * (size_t)p);
   double fl[4], s_r;
   flux_roe2d(n.x, n.y, left, right, c.gamma, fl, &s_r);
   *reinterpret_cast<double4*>(flux + 4 * (size_t)p) = make_double4(fl[0], fl[1], fl[2], fl[3]);
   sr[p] = s_r;
   if (s_r != s_r) atomicOr(nanflag, 1);
-  if (IMPL) {
+  if (IMPL && p >= ff_from) {
     TL lm[16], rm[16];
     face_jacobians<TL, FASTJ>(n.x, n.y, left, right, c.gamma, lm, rm);
 #pragma unroll
@@ -1607,12 +1646,159 @@ __device__ __forceinline__ void fast_solve_lds(T* lds, const int N, const int NL
 
 #endif  // HAMSTR_LONG_TU
 
+// Row m assembled as assemble_row does and normalised by its diagonal block
+// (an open chain's end rows without the coupling past the chain's ends), for
+// a chain whose rows are all assembled at once by one group of threads, each
+// row taking only its own face m (L_m, R_m) from js: face m+1, which row m
+// also needs, comes through LDS (`xl`, element e of face j at xl[e * XS + j],
+// 16 entries, XS > N; L first, then R), so that no face is read from memory
+// or computed twice.  An open chain's last face (no row owns it) is taken by
+// the last row; computed from q, its blocks wait in xl[16 * XS + e] (L, then
+// R), so the slice must hold 16 * XS + 32 values.  Row m holds A (= R_m scaled) and C
+// (= 0 - L_(m+1) scaled) before the diagonal block is factored.  Orphan
+// diagonal blocks are read from ff.  Every thread of the group calls it (it
+// holds barriers); inactive threads pass active = false.  The entries take
+// the operations assemble_row and normalise_row give them.
+constexpr int kXStride = 385;  // the 384-thread half-LDS kernels' XS (MAXT + 1)
+// JQ: the Jacobians computed from q (js), else read from ff (a compile-time
+// choice: in one function the two paths cost the two-line kernel 160 spilled
+// VGPRs)
+template <typename T, bool JQ>
+__device__ __forceinline__ void fast_row_xchg(const ChainTab& ct, const JacSrc& js, const T* __restrict__ ff,
+                                              int nProc, int pb, int nf, int N, bool periodic, int m, bool active,
+                                              double cfl, const double* __restrict__ sigma,
+                                              const double* __restrict__ r, T* xl, const int XS, T* A, T* C,
+                                              T* F) {
+  const int pm = pb + m;
+  const bool has_next = active && (periodic || m + 1 < nf);
+  const int mn = periodic ? ((m + 1 == N) ? 0 : m + 1) : m + 1;  // the next face, as a row index
+  const bool next_owned = periodic || m + 1 < N;                 // else the chain's last face
+  const int pn = pb + mn;
+  const int rc = (periodic || !active) ? 0 : ct.pright[pm];
+  const int rn = (periodic || !has_next) ? 0 : ct.pright[pn];
+  const bool end = !periodic && has_next && m + 1 == nf - 1 && rn < 0;
+  const bool hasA = periodic || (m > 0 && rc > -1);
+  const bool hasC = periodic || (has_next && !end && rn > -1 && m != N - 1);
+  constexpr bool comp = JQ;
+  T B[16];
+  // L: own face into LDS (computed: R_m too, kept in A), the next face's back
+  if (active) {
+    if (comp && has_next && !next_owned) {
+      // the chain's last face first, while nothing else is live
+      T Ln[16], Rn[16];
+      face_jac_q<T>(js, pn, Ln, Rn);
+#pragma unroll
+      for (int e = 0; e < 16; e++) {
+        xl[16 * XS + e] = Ln[e];
+        xl[16 * XS + 16 + e] = Rn[e];
+      }
+    }
+    if (comp) {
+      face_jac_q<T>(js, pm, B, A);
+    } else {
+#pragma unroll
+      for (int e = 0; e < 16; e++) B[e] = ff[(size_t)e * nProc + pm];
+    }
+#pragma unroll
+    for (int e = 0; e < 16; e++) xl[e * XS + m] = B[e];
+  }
+  __syncthreads();
+  if (has_next) {
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+      const T v = next_owned ? xl[e * XS + mn] : (comp ? xl[16 * XS + e] : ff[(size_t)e * nProc + pn]);
+      C[e] = v;  // L_(m+1)
+    }
+  }
+  __syncthreads();
+  // R: own face into LDS (A, and the wall case of B), the next face's back
+  if (active) {
+    if (!comp) {
+#pragma unroll
+      for (int e = 0; e < 16; e++) A[e] = ff[(size_t)(16 + e) * nProc + pm];
+    }
+#pragma unroll
+    for (int e = 0; e < 16; e++) xl[e * XS + m] = A[e];
+    if (!periodic && rc == -2) {
+      T R[16];
+#pragma unroll
+      for (int e = 0; e < 16; e++) R[e] = A[e];
+      reflect_cols(R, ct.refmtx + 4 * (size_t)ct.pface[pm]);
+#pragma unroll
+      for (int e = 0; e < 16; e++) B[e] += R[e];
+    }
+  }
+  __syncthreads();
+  if (has_next) {
+    T R[16];
+#pragma unroll
+    for (int e = 0; e < 16; e++)
+      R[e] = next_owned ? xl[e * XS + mn] : (comp ? xl[16 * XS + 16 + e] : ff[(size_t)(16 + e) * nProc + pn]);
+    if (periodic) {
+#pragma unroll
+      for (int e = 0; e < 16; e++) B[e] -= R[e];
+    } else if (end) {
+      if (rn == -2) reflect_cols(R, ct.refmtx + 4 * (size_t)ct.pface[pn]);
+#pragma unroll
+      for (int e = 0; e < 16; e++) B[e] += C[e];  // L_(m+1)
+      if (rn == -2) {
+#pragma unroll
+        for (int e = 0; e < 16; e++) B[e] += R[e];
+      }
+    } else if (rn > -1) {
+#pragma unroll
+      for (int e = 0; e < 16; e++) B[e] -= R[e];
+    }
+  }
+  __syncthreads();  // the LDS slice is free again
+  if (!active) return;
+  const int cell = ct.pleft[pm];
+  if (ct.od_chain && ct.od_chain[cell] == pb) {
+    const int e1 = ct.od_off[cell + 1];
+    for (int e = ct.od_off[cell]; e < e1; e++) {
+      const int code = ct.od_ent[e];
+      const int po = code >> 1;
+      if (code & 1) {
+#pragma unroll
+        for (int k = 0; k < 16; k++) B[k] += ff[(size_t)k * nProc + po];
+      } else {
+#pragma unroll
+        for (int k = 0; k < 16; k++) B[k] -= ff[(size_t)(16 + k) * nProc + po];
+      }
+    }
+  }
+  const T dtfac = T(cfl / sigma[cell]);
+#pragma unroll
+  for (int e = 0; e < 16; e++) B[e] *= dtfac;
+  B[0] += T(1.0);
+  B[5] += T(1.0);
+  B[10] += T(1.0);
+  B[15] += T(1.0);
+  const LU4<T> LU = lu4_fast(B);
+  const double4 rv = *reinterpret_cast<const double4*>(r + 4 * (size_t)cell);
+  F[0] = T(rv.x);
+  F[1] = T(rv.y);
+  F[2] = T(rv.z);
+  F[3] = T(rv.w);
+  lu4_solve_fast(LU, F, 1);
+#pragma unroll
+  for (int e = 0; e < 16; e++) {
+    A[e] = (hasA ? A[e] : T(0)) * dtfac;
+    C[e] = (hasC ? T(0) - C[e] : T(0)) * dtfac;
+  }
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    lu4_solve_fast(LU, A + j, 4);
+    lu4_solve_fast(LU, C + j, 4);
+  }
+}
+
 // ADI phase, FAST order, one thread and one LDS row per block row (steps as
 // in fast_solve_lds, written out in place: the shared function costs this
 // kernel about 20 VGPRs and 4% of its time)
-template <typename T, int FINAL>
-__device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const T* __restrict__ ff, int nProc,
-                                               double cfl, const double* __restrict__ sigma,
+template <typename T, int FINAL, bool JQ>
+__device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const JacSrc& js, const T* __restrict__ ff,
+                                               int nProc, double cfl, const double* __restrict__ sigma,
                                                double* __restrict__ r, double* __restrict__ dq, int dqmode,
                                                int ncr) {
 #pragma clang fp contract(fast)
@@ -1624,10 +1810,16 @@ __device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const 
   const bool active = i < N;
   const int S = lds_stride(N);
 
-  // every row, assembled and normalised, goes to LDS.  (Passing the faces
-  // through LDS, as the half-LDS kernels do, made this kernel slower: here
-  // the second read of a face hits in cache.)
-  if (active) {
+  // every row, assembled and normalised, goes to LDS.  Jacobians computed
+  // from q pass each face to the neighbouring row through LDS, so that none
+  // is computed twice; read from ff, both of a row's faces are read directly
+  // (passing them through LDS made this kernel slower: here the second read
+  // of a face hits in cache).
+  if constexpr (JQ) {
+    T a[16], c[16], f[4];
+    fast_row_xchg<T, true>(ct, js, ff, nProc, pb, nf, N, periodic, i, active, cfl, sigma, r, lds, S, a, c, f);
+    if (active) lds_store_row(lds, S, lpad(i), a, c, f);
+  } else if (active) {
     T a[16], c[16], f[4], B[16];
     int cell;
     assemble_row<T>(ct, ff, nProc, pb, nf, N, periodic, i, cfl, sigma, r, a, B, c, f, &cell);
@@ -1962,130 +2154,6 @@ __device__ __forceinline__ void adi_fast_chain_long(ChainTab ct, const int ch, c
   }
 }
 
-// Row m assembled as assemble_row does and normalised by its diagonal block
-// (an open chain's end rows without the coupling past the chain's ends), for
-// a chain whose rows are all assembled at once by one group of threads, each row loading only its own face m (L_m,
-// R_m): face m+1, which row m also needs, comes through LDS (`xl`, element e
-// of face j at xl[e * XS + j], 16 entries, XS > N; L first, then R), so that
-// no face is read from memory twice.  Row m holds A (= R_m scaled) and C (= 0 -
-// L_(m+1) scaled) before the diagonal block is factored.  Every thread of the
-// group calls it (it holds barriers); inactive threads pass active = false.
-// An open chain's last face (no row owns it) is read by the last row.  The
-// entries take the operations assemble_row and normalise_row give them.
-constexpr int kXStride = 385;  // the 384-thread half-LDS kernels' XS (MAXT + 1)
-template <typename T>
-__device__ __forceinline__ void fast_row_xchg(const ChainTab& ct, const T* __restrict__ ff, int nProc, int pb,
-                                              int nf, int N, bool periodic, int m, bool active, double cfl,
-                                              const double* __restrict__ sigma, const double* __restrict__ r,
-                                              T* xl, const int XS, T* A, T* C, T* F) {
-  const int pm = pb + m;
-  const bool has_next = active && (periodic || m + 1 < nf);
-  const int mn = periodic ? ((m + 1 == N) ? 0 : m + 1) : m + 1;  // the next face, as a row index
-  const bool next_owned = periodic || m + 1 < N;                 // else the chain's last face
-  const int pn = pb + mn;
-  const int rc = (periodic || !active) ? 0 : ct.pright[pm];
-  const int rn = (periodic || !has_next) ? 0 : ct.pright[pn];
-  const bool end = !periodic && has_next && m + 1 == nf - 1 && rn < 0;
-  const bool hasA = periodic || (m > 0 && rc > -1);
-  const bool hasC = periodic || (has_next && !end && rn > -1 && m != N - 1);
-  T B[16];
-  // L: own face into LDS, the next face's back
-  if (active) {
-#pragma unroll
-    for (int e = 0; e < 16; e++) {
-      B[e] = ff[(size_t)e * nProc + pm];
-      xl[e * XS + m] = B[e];
-    }
-  }
-  __syncthreads();
-  if (has_next) {
-#pragma unroll
-    for (int e = 0; e < 16; e++) {
-      const T v = next_owned ? xl[e * XS + mn] : ff[(size_t)e * nProc + pn];
-      C[e] = v;  // L_(m+1)
-    }
-  }
-  __syncthreads();
-  // R: own face into LDS (A, and the wall case of B), the next face's back
-  if (active) {
-#pragma unroll
-    for (int e = 0; e < 16; e++) {
-      A[e] = ff[(size_t)(16 + e) * nProc + pm];
-      xl[e * XS + m] = A[e];
-    }
-    if (!periodic && rc == -2) {
-      T R[16];
-#pragma unroll
-      for (int e = 0; e < 16; e++) R[e] = A[e];
-      reflect_cols(R, ct.refmtx + 4 * (size_t)ct.pface[pm]);
-#pragma unroll
-      for (int e = 0; e < 16; e++) B[e] += R[e];
-    }
-  }
-  __syncthreads();
-  if (has_next) {
-    T R[16];
-#pragma unroll
-    for (int e = 0; e < 16; e++) R[e] = next_owned ? xl[e * XS + mn] : ff[(size_t)(16 + e) * nProc + pn];
-    if (periodic) {
-#pragma unroll
-      for (int e = 0; e < 16; e++) B[e] -= R[e];
-    } else if (end) {
-      if (rn == -2) reflect_cols(R, ct.refmtx + 4 * (size_t)ct.pface[pn]);
-#pragma unroll
-      for (int e = 0; e < 16; e++) B[e] += C[e];  // L_(m+1)
-      if (rn == -2) {
-#pragma unroll
-        for (int e = 0; e < 16; e++) B[e] += R[e];
-      }
-    } else if (rn > -1) {
-#pragma unroll
-      for (int e = 0; e < 16; e++) B[e] -= R[e];
-    }
-  }
-  __syncthreads();  // the LDS slice is free again
-  if (!active) return;
-  const int cell = ct.pleft[pm];
-  if (ct.od_chain && ct.od_chain[cell] == pb) {
-    const int e1 = ct.od_off[cell + 1];
-    for (int e = ct.od_off[cell]; e < e1; e++) {
-      const int code = ct.od_ent[e];
-      const int po = code >> 1;
-      if (code & 1) {
-#pragma unroll
-        for (int k = 0; k < 16; k++) B[k] += ff[(size_t)k * nProc + po];
-      } else {
-#pragma unroll
-        for (int k = 0; k < 16; k++) B[k] -= ff[(size_t)(16 + k) * nProc + po];
-      }
-    }
-  }
-  const T dtfac = T(cfl / sigma[cell]);
-#pragma unroll
-  for (int e = 0; e < 16; e++) B[e] *= dtfac;
-  B[0] += T(1.0);
-  B[5] += T(1.0);
-  B[10] += T(1.0);
-  B[15] += T(1.0);
-  const LU4<T> LU = lu4_fast(B);
-  const double4 rv = *reinterpret_cast<const double4*>(r + 4 * (size_t)cell);
-  F[0] = T(rv.x);
-  F[1] = T(rv.y);
-  F[2] = T(rv.z);
-  F[3] = T(rv.w);
-  lu4_solve_fast(LU, F, 1);
-#pragma unroll
-  for (int e = 0; e < 16; e++) {
-    A[e] = (hasA ? A[e] : T(0)) * dtfac;
-    C[e] = (hasC ? T(0) - C[e] : T(0)) * dtfac;
-  }
-#pragma unroll
-  for (int j = 0; j < 4; j++) {
-    lu4_solve_fast(LU, A + j, 4);
-    lu4_solve_fast(LU, C + j, 4);
-  }
-}
-
 // ADI phase, FAST order, one row per thread with half the rows in LDS: thread
 // t < M = ceil(N/2) holds even row 2t, thread t >= M odd row 2(t-M)+1.  The
 // odd rows, which the first cyclic-reduction level eliminates, go to LDS and
@@ -2101,9 +2169,9 @@ __device__ __forceinline__ void fast_row_xchg(const ChainTab& ct, const T* __res
 // closed chains).  Needs N <= the group's threads, open chains or closed
 // chains of an even N >= 4, ncr >= 1, and ceil(NL/2) rows within the fixed
 // LDS stride SM (SM >= lds_stride(ceil(NL/2))).
-template <typename T, int FINAL, int SM, int XS>
-__device__ __forceinline__ void adi_fast_chain_half(ChainTab ct, const int ch, const int NL, T* lds, const int t,
-                                                    const int NT,
+template <typename T, int FINAL, int SM, int XS, bool JQ>
+__device__ __forceinline__ void adi_fast_chain_half(ChainTab ct, const JacSrc& js, const int ch, const int NL, T* lds,
+                                                    const int t, const int NT,
                                                     const T* __restrict__ ff, int nProc, double cfl,
                                                     const double* __restrict__ sigma, double* __restrict__ r,
                                                     double* __restrict__ dq, int dqmode, int ncr,
@@ -2117,7 +2185,7 @@ __device__ __forceinline__ void adi_fast_chain_half(ChainTab ct, const int ch, c
   const bool active = row < N;
   {
     T a[16], c[16], f[4];
-    fast_row_xchg<T>(ct, ff, nProc, pb, nf, N, periodic, row, active, cfl, sigma, r, lds, XS, a, c, f);
+    fast_row_xchg<T, JQ>(ct, js, ff, nProc, pb, nf, N, periodic, row, active, cfl, sigma, r, lds, XS, a, c, f);
     if (active && !even) {
       lds_store_row(lds, SM, lpad(t - M), a, c, f);
       lds_store_row(hrow, hstride, t - M, a, c, f);
@@ -2160,10 +2228,10 @@ __device__ __forceinline__ void adi_fast_chain_half(ChainTab ct, const int ch, c
 // 6-wave workgroups (384-row chains) pack onto a CU: for odd cycles that costs
 // 22 spilled VGPRs and still took a 256-case batch's 384-row loops from 1.34
 // to 0.95 ms.
-template <typename T, int FINAL, int MAXT>
+template <typename T, int FINAL, int MAXT, bool JQ>
 __global__ __launch_bounds__(MAXT)
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && MAXT == 512 ? 4 : 1)))
-void k_adi_fast(ChainTab ct, Live lv, const int* __restrict__ chains, int nch,
+void k_adi_fast(ChainTab ct, Live lv, JacSrc js, const int* __restrict__ chains, int nch,
                                                    const T* __restrict__ ff, int nProc, double cfl,
                                                    const double* __restrict__ sigma, double* __restrict__ r,
                                                    double* __restrict__ dq, int dqmode, int ncr) {
@@ -2171,7 +2239,7 @@ void k_adi_fast(ChainTab ct, Live lv, const int* __restrict__ chains, int nch,
   for (int w = vb; w < nch; w += gridDim.x) {
     const int ch = chains[w];
     if (lv.chain && !lv.chain[ch]) continue;  // the same for the whole workgroup
-    adi_fast_chain<T, FINAL>(ct, ch, ff, nProc, cfl, sigma, r, dq, dqmode, ncr);
+    adi_fast_chain<T, FINAL, JQ>(ct, ch, js, ff, nProc, cfl, sigma, r, dq, dqmode, ncr);
     __syncthreads();
   }
 }
@@ -2199,22 +2267,21 @@ __global__ __launch_bounds__(MAXT) void k_adi_fast_long(ChainTab ct, Live lv, co
 // minimum waves per SIMD the register allocation must leave room for (3: at
 // most 168 VGPRs).  The LDS stride is fixed at lds_stride(ceil(MAXT/2)); the
 // launch provides 36 such element rows.
-template <typename T, int FINAL, int MAXT, int WPE>
+template <typename T, int FINAL, int MAXT, int WPE, bool JQ>
 __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) void k_adi_fast_half(
-    ChainTab ct, Live lv, const int* __restrict__ chains, int nch, const T* __restrict__ ff, int nProc, double cfl,
-    const double* __restrict__ sigma, double* __restrict__ r, double* __restrict__ dq, int dqmode, int ncr,
-    T* __restrict__ hs, int hstride, const int* __restrict__ hbase) {
-  static_assert(16 * (MAXT + 1) <= 36 * lds_stride((MAXT + 1) / 2), "face exchange fits the slice");
+    ChainTab ct, Live lv, JacSrc js, const int* __restrict__ chains, int nch, const T* __restrict__ ff, int nProc,
+    double cfl, const double* __restrict__ sigma, double* __restrict__ r, double* __restrict__ dq, int dqmode,
+    int ncr, T* __restrict__ hs, int hstride, const int* __restrict__ hbase) {
+  static_assert(16 * (MAXT + 1) + 32 <= 36 * lds_stride((MAXT + 1) / 2), "face exchange fits the slice");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* lds = reinterpret_cast<T*>(smem_raw);
   const int vb = xcd_block(blockIdx.x, gridDim.x);
   for (int w = vb; w < nch; w += gridDim.x) {
     const int ch = chains[w];
     if (lv.chain && !lv.chain[ch]) continue;  // the same for the whole workgroup
-    adi_fast_chain_half<T, FINAL, lds_stride((MAXT + 1) / 2), MAXT + 1>(ct, ch, ct.ncell[ch], lds, threadIdx.x,
-                                                                        (int)blockDim.x,
-                                                              ff, nProc, cfl, sigma, r, dq, dqmode, ncr, hs, hstride,
-                                                              hbase);
+    adi_fast_chain_half<T, FINAL, lds_stride((MAXT + 1) / 2), MAXT + 1, JQ>(ct, js, ch, ct.ncell[ch], lds, threadIdx.x,
+                                                                        (int)blockDim.x, ff, nProc, cfl, sigma, r,
+                                                                        dq, dqmode, ncr, hs, hstride, hbase);
     __syncthreads();
   }
 }
@@ -2226,13 +2293,13 @@ __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) voi
 // CU (their waves do not pack evenly onto the 4 SIMDs), so this is how two
 // 384-row chains run on one CU.  Both halves run the reduction levels of the
 // longer chain.
-template <typename T, int MAXH>
+template <typename T, int MAXH, bool JQ>
 __global__ __launch_bounds__(2 * MAXH) __attribute__((amdgpu_waves_per_eu(3))) void k_adi_fast_half2(
-    ChainTab ct, Live lv, const int* __restrict__ chains, int nch, const T* __restrict__ ff, int nProc, double cfl,
-    const double* __restrict__ sigma, double* __restrict__ r, double* __restrict__ dq, int dqmode, int ncr,
-    T* __restrict__ hs, int hstride, const int* __restrict__ hbase) {
+    ChainTab ct, Live lv, JacSrc js, const int* __restrict__ chains, int nch, const T* __restrict__ ff, int nProc,
+    double cfl, const double* __restrict__ sigma, double* __restrict__ r, double* __restrict__ dq, int dqmode,
+    int ncr, T* __restrict__ hs, int hstride, const int* __restrict__ hbase) {
   constexpr int SM = lds_stride((MAXH + 1) / 2);
-  static_assert(16 * (MAXH + 1) <= 36 * SM, "face exchange fits the slice");
+  static_assert(16 * (MAXH + 1) + 32 <= 36 * SM, "face exchange fits the slice");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int sub = threadIdx.x >= MAXH ? 1 : 0;
   T* lds = reinterpret_cast<T*>(smem_raw) + sub * 36 * SM;
@@ -2246,9 +2313,8 @@ __global__ __launch_bounds__(2 * MAXH) __attribute__((amdgpu_waves_per_eu(3))) v
     if (c1 >= 0 && lv.chain && !lv.chain[c1]) c1 = -1;
     const int NL = max(c0 >= 0 ? ct.ncell[c0] : 0, c1 >= 0 ? ct.ncell[c1] : 0);
     if (NL == 0) continue;  // the same for the whole workgroup
-    adi_fast_chain_half<T, kFinalOpen, SM, MAXH + 1>(ct, sub ? c1 : c0, NL, lds, t, MAXH, ff, nProc, cfl, sigma, r, dq,
-                                                     dqmode,
-                                           ncr, hs, hstride, hbase);
+    adi_fast_chain_half<T, kFinalOpen, SM, MAXH + 1, JQ>(ct, js, sub ? c1 : c0, NL, lds, t, MAXH, ff, nProc, cfl, sigma,
+                                                     r, dq, dqmode, ncr, hs, hstride, hbase);
     __syncthreads();
   }
 }
@@ -2703,7 +2769,8 @@ hipError_t hamstr_adi_long_launch(bool fp32, int final_type, unsigned grid, unsi
 // launcher of k_adi_fast_half / k_adi_fast_half2 (FP64, chains of at most
 // kHalfMaxRows rows), in adi_long.hip
 hipError_t hamstr_adi_half_launch(bool fp32, int final_type, int pairs, unsigned grid, unsigned block, size_t lds,
-                                  hipStream_t stream, const void* ct, const void* lv, const int* chains, int nch,
+                                  hipStream_t stream, const void* ct, const void* lv, const void* js,
+                                  const int* chains, int nch,
                                   const void* ff, int nProc, double cfl, const double* sigma, double* r, double* dq,
                                   int dqmode, int ncr, void* hs, int hstride, const int* hbase);
 constexpr int kHalfMaxRows = 384;  // the workgroup bound of k_adi_fast_half, per chain of k_adi_fast_half2
@@ -2771,6 +2838,9 @@ struct hamstr_ctx {
   int nLong = 0;
   int* group_chains = nullptr;
   std::vector<Group> groups;   // in phase order
+  // the fused FAST iteration computes the chain faces' Jacobians again in the
+  // ADI kernels instead of storing them (chosen by hamstr_autotune)
+  bool jac_recompute = false;
   int nPhase = 0;
   bool last_phase_all_cells = false;
   void* scratch = nullptr;     // EXACT groups: T[nChainProc][36] rows C', F', a' of the back sweep
@@ -2819,12 +2889,15 @@ struct hamstr_ctx {
 
 namespace {
 
-int launch_rhs_faces(hamstr_ctx* x, bool implicit) {
+// ff_from: the first processed face whose Jacobians are stored (0: all;
+// nChainProc: the orphan faces only, for ADI kernels that compute the chain
+// faces' Jacobians again)
+int launch_rhs_faces(hamstr_ctx* x, bool implicit, int ff_from = 0) {
   const FaceTab ft = x->ftab();
   dim3 g(nblk(x->nProc)), b(kBlock);
 #define RHSF(O, T, I, F)                                                                                         \
   hipLaunchKernelGGL((k_rhs_faces<O, T, I, F>), g, b, 0, x->stream, ft, x->c, x->live(), x->q, x->flux, x->sr, (T*)x->ff, \
-                     x->nanflag)
+                     ff_from, x->nanflag)
 #define RHSO(T, I, F) \
   if (o == 1) RHSF(1, T, I, F); else if (o == 3) RHSF(3, T, I, F); else RHSF(5, T, I, F)
   const int o = x->prm.order;
@@ -2898,10 +2971,26 @@ int launch_rhs_cells(hamstr_ctx* x, bool adi_init, double* normbase, double* par
   return part ? HAMSTR_OK : launch_norm(x, normbase, 0);
 }
 
-int launch_group(hamstr_ctx* x, const Group& g, int grid, int dqmode, int ncr) {
+// Whether every ADI launch group can compute its chain faces' Jacobians from
+// q: FAST kernels other than the long-chain one, and not FP32 odd cycles,
+// whose one-row kernel, held to 128 VGPRs, would spill some 500 registers
+// computing them.  The fused iteration does so (and stores the orphan faces'
+// Jacobians only) when this holds and the autotuner found it faster
+// (x->jac_recompute).
+bool jac_recompute_possible(const hamstr_ctx* x) {
+  if (x->prm.adi_mode != HAMSTR_ADI_FAST || x->groups.empty()) return false;
+  for (const Group& g : x->groups)
+    if (!g.fast || g.two || (x->lhs32() && g.final == kFinalOdd)) return false;
+  return true;
+}
+
+// recompute: the group's chain-face Jacobians computed again from q (the
+// fused iteration, when the autotuner chose it), else read from ff
+int launch_group(hamstr_ctx* x, const Group& g, int grid, int dqmode, int ncr, bool recompute = false) {
   const ChainTab ct = x->ctab();
   const int* chains = x->group_chains + g.off;
   if (g.fast) {
+    const JacSrc js{x->ftab(), x->q, x->c.gamma, x->c.qt, x->c.eps, recompute ? x->prm.order : 0};
     const dim3 gr(grid), b(g.block);
     // long chains always run the first reduction level (it is what halves
     // them), and so does the half-LDS kernel
@@ -2911,41 +3000,51 @@ int launch_group(hamstr_ctx* x, const Group& g, int grid, int dqmode, int ncr) {
                        (half ? (g.kern == kKernHalf2 ? 2 * (size_t)lds_stride((kHalfMaxRows + 1) / 2)
                                                      : (size_t)lds_stride((half_maxt(g.block) + 1) / 2))
                              : (size_t)lds_stride(g.two ? (g.maxN + 1) / 2 : g.maxN));
-#define FASTT(T, F, M)                                                                                            \
-  hipLaunchKernelGGL((k_adi_fast<T, F, M>), gr, b, lds, x->stream, ct, x->live(), chains, g.n, (const T*)x->ff,   \
-                     x->nProc, x->c.cfl, x->sigma, x->r, x->dq, dqmode, nc)
+#define FASTJ(T, F, M, J)                                                                                          \
+  hipLaunchKernelGGL((k_adi_fast<T, F, M, J>), gr, b, lds, x->stream, ct, x->live(), js, chains, g.n,               \
+                     (const T*)x->ff, x->nProc, x->c.cfl, x->sigma, x->r, x->dq, dqmode, nc)
+#define FASTT(T, F, M)             \
+  do {                             \
+    if (recompute)                 \
+      FASTJ(T, F, M, true);        \
+    else                           \
+      FASTJ(T, F, M, false);       \
+  } while (0)
     if (half) {
       const Live lv = x->live();
       const bool pairs = g.kern == kKernHalf2;
       const int gw = pairs ? std::min(grid, (g.n + 1) / 2) : grid;  // workgroups of two chains
       HIPCHK(hamstr_adi_half_launch(x->lhs32(), g.final, pairs, (unsigned)gw, pairs ? 2u * kHalfMaxRows : (unsigned)g.block, lds,
-                                    x->stream, &ct, &lv, chains, g.n, x->ff, x->nProc, x->c.cfl, x->sigma, x->r,
+                                    x->stream, &ct, &lv, &js, chains, g.n, x->ff, x->nProc, x->c.cfl, x->sigma, x->r,
                                     x->dq, dqmode, nc, x->long_scratch, x->nLong, x->ch_hbase));
     } else if (g.two) {
       const Live lv = x->live();
       HIPCHK(hamstr_adi_long_launch(x->lhs32(), g.final, (unsigned)grid, (unsigned)g.block, lds, x->stream, &ct, &lv,
                                     chains, g.n, x->ff, x->nProc, x->c.cfl, x->sigma, x->r, x->dq, dqmode, nc,
                                     x->long_scratch, x->nLong, x->ch_hbase));
+    } else if (x->lhs32() && g.final == kFinalOdd && recompute) {
+      return fail(HAMSTR_ERR_ARG, "FP32 odd cycles read their Jacobians from ff");
     } else if (x->lhs32() && g.final == kFinalOdd && g.block <= 384 && grid <= x->ncu) {
       // odd cycles in one round of workgroups (one per CU): without the
       // 128-VGPR bound, which would only add spills here
-      FASTT(float, kFinalOdd, 384);
+      FASTJ(float, kFinalOdd, 384, false);
     } else if (x->lhs32() && g.block <= 512) {
       // up to 512 threads: held to 128 VGPRs, so that two 384-row
       // workgroups share a CU
       if (g.final == kFinalOpen) FASTT(float, kFinalOpen, 512);
       else if (g.final == kFinalPair) FASTT(float, kFinalPair, 512);
-      else FASTT(float, kFinalOdd, 512);
+      else FASTJ(float, kFinalOdd, 512, false);
     } else if (x->lhs32()) {
       if (g.final == kFinalOpen) FASTT(float, kFinalOpen, 1024);
       else if (g.final == kFinalPair) FASTT(float, kFinalPair, 1024);
-      else FASTT(float, kFinalOdd, 1024);
+      else FASTJ(float, kFinalOdd, 1024, false);
     } else {
       if (g.final == kFinalOpen) FASTT(double, kFinalOpen, 512);
       else if (g.final == kFinalPair) FASTT(double, kFinalPair, 512);
       else FASTT(double, kFinalOdd, 512);
     }
 #undef FASTT
+#undef FASTJ
   } else if (x->lhs32()) {
     hipLaunchKernelGGL((k_exact_assemble<float>), dim3(g.n), dim3(256), 0, x->stream, ct, x->live(), chains, g.n,
                        (const float*)x->ff, x->nProc, x->c.cfl, x->sigma, x->r, (float*)x->exact_rows);
@@ -2965,11 +3064,12 @@ int launch_group(hamstr_ctx* x, const Group& g, int grid, int dqmode, int ncr) {
 
 // all phases of one sweep; the last phase also accumulates dq when it covers
 // every cell, otherwise a separate kernel does (ADI.cpp:57-248)
-int launch_sweep_phases(hamstr_ctx* x, int sweep, const std::function<int(const char*)>& mark) {
+int launch_sweep_phases(hamstr_ctx* x, int sweep, const std::function<int(const char*)>& mark,
+                        bool recompute = false) {
   for (size_t k = 0; k < x->groups.size(); k++) {
     const Group& g = x->groups[k];
     const bool fuse = (g.phase == x->nPhase - 1) && x->last_phase_all_cells;
-    const int rc = launch_group(x, g, g.grid, fuse ? (sweep == 0 ? 1 : 2) : 0, g.ncr);
+    const int rc = launch_group(x, g, g.grid, fuse ? (sweep == 0 ? 1 : 2) : 0, g.ncr, recompute);
     if (rc) return rc;
     const bool phase_end = (k + 1 == x->groups.size()) || x->groups[k + 1].phase != g.phase;
     if (mark && phase_end &&
@@ -3998,12 +4098,16 @@ int enqueue_iteration(hamstr_ctx* x, double* nb, const std::function<int(const c
 int enqueue_adi_iteration(hamstr_ctx* x, double* nb, const std::function<int(const char*)>& mark, double* p_nl,
                           double* p_lin) {
   int rc;
-  if ((rc = launch_rhs_faces(x, true))) return rc;
+  // FAST: the ADI kernels compute the chain faces' Jacobians again from q
+  // (k_rhs_faces stores the orphan faces' only): 256 B per face neither
+  // written nor read back (FP64)
+  const bool rj = x->jac_recompute && jac_recompute_possible(x);
+  if ((rc = launch_rhs_faces(x, true, rj ? x->nChainProc : 0))) return rc;
   if (mark && mark("rhs_faces")) return HAMSTR_ERR_HIP;
   if ((rc = launch_rhs_cells(x, true, nb, p_nl))) return rc;
   if (mark && mark("rhs_cells")) return HAMSTR_ERR_HIP;
   for (int sw = 0; sw < x->prm.msweep; sw++) {
-    if ((rc = launch_sweep_phases(x, sw, mark))) return rc;
+    if ((rc = launch_sweep_phases(x, sw, mark, rj))) return rc;
     // the Jacobians computed again beat reading them back (FP64: 26.9 ->
     // 24.1 us on the 512x256 mixed mesh; FP32: the same there, 1.14 -> 0.84
     // ms in a 256-case batch)
@@ -4262,6 +4366,22 @@ int hamstr_set_launch(hamstr_ctx* x, int group, int grid, int cr_levels) {
   return HAMSTR_OK;
 }
 
+int hamstr_set_jacobian_recompute(hamstr_ctx* x, int on) {
+  if (!x) return fail(HAMSTR_ERR_ARG, "null context");
+  if (on && !jac_recompute_possible(x))
+    return fail(HAMSTR_ERR_ARG, "the Jacobians can be computed again only by FAST kernels other than the "
+                                "long-chain one, and not for FP32 odd cycles");
+  x->jac_recompute = on != 0;
+  drop_graph(x);
+  return HAMSTR_OK;
+}
+
+int hamstr_jacobian_recompute(const hamstr_ctx* x, int* on) {
+  if (!x || !on) return fail(HAMSTR_ERR_ARG, "null argument");
+  *on = (x->jac_recompute && jac_recompute_possible(x)) ? 1 : 0;
+  return HAMSTR_OK;
+}
+
 int hamstr_set_adi_kernel(hamstr_ctx* x, int group, int kernel) {
   if (!x || group < 0 || group >= (int)x->groups.size()) return fail(HAMSTR_ERR_ARG, "bad launch group");
   Group& g = x->groups[group];
@@ -4397,10 +4517,30 @@ int hamstr_autotune(hamstr_ctx* x, int reps, double* best_ms, int cap, int* ngro
     saved = true;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    int k = 0;
-    for (Group& g : x->groups) {
-      double best = -1;
-      if (g.fast) {
+    // events around `reps` launches of `launch` after one warm-up: ms each
+    auto timed = [&](const std::function<int()>& launch, double* ms_out) -> int {
+      int rc = launch();
+      if (rc) return rc;
+      HIPCHK(hipEventRecord(e0, x->stream));
+      for (int i = 0; i < reps; i++)
+        if ((rc = launch())) return rc;
+      HIPCHK(hipEventRecord(e1, x->stream));
+      HIPCHK(hipEventSynchronize(e1));
+      float ms = 0;
+      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      *ms_out = ms / reps;
+      return HAMSTR_OK;
+    };
+    // the group search with the Jacobians read from ff (rj false) or
+    // computed again from q (rj true): gs gets every group's best shape,
+    // per[k] its time, *sum their total
+    auto tune_groups = [&](bool rj, std::vector<Group>& gs, std::vector<double>& per, double* sum) -> int {
+      per.assign(gs.size(), -1.0);
+      *sum = 0;
+      for (size_t k = 0; k < gs.size(); k++) {
+        Group& g = gs[k];
+        if (!g.fast) continue;
+        double best = -1;
         std::vector<int> cand = {g.n};
         for (int w : {8, 4, 2, 1})
           if (w * cu < g.n) cand.push_back(w * cu);
@@ -4414,16 +4554,9 @@ int hamstr_autotune(hamstr_ctx* x, int reps, double* best_ms, int cap, int* ngro
           double kbest = -1;
           int kgrid = cand[0];
           auto trial = [&](int grid, int ncr) -> int {
-            int rc = launch_group(x, g, grid, 0, ncr);  // warm-up
+            double t = 0;
+            const int rc = timed([&]() { return launch_group(x, g, grid, 0, ncr, rj); }, &t);
             if (rc) return rc;
-            HIPCHK(hipEventRecord(e0, x->stream));
-            for (int i = 0; i < reps; i++)
-              if ((rc = launch_group(x, g, grid, 0, ncr))) return rc;
-            HIPCHK(hipEventRecord(e1, x->stream));
-            HIPCHK(hipEventSynchronize(e1));
-            float ms = 0;
-            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-            const double t = ms / reps;
             if (kbest < 0 || t < kbest) {
               kbest = t;
               kgrid = grid;
@@ -4447,10 +4580,38 @@ int hamstr_autotune(hamstr_ctx* x, int reps, double* best_ms, int cap, int* ngro
         g.kern = best_kern;
         g.grid = best_grid;
         g.ncr = best_ncr;
+        per[k] = best;
+        *sum += best;
       }
-      if (best_ms && k < cap) best_ms[k] = best;
-      k++;
+      return HAMSTR_OK;
+    };
+    int rc;
+    std::vector<Group> gs = x->groups;
+    std::vector<double> per;
+    double sum = 0;
+    if ((rc = tune_groups(false, gs, per, &sum))) return rc;
+    bool rj = false;
+    if (jac_recompute_possible(x)) {
+      // the fused iteration either stores every face's Jacobians and the ADI
+      // kernels read them, or stores the orphan faces' only and the ADI
+      // kernels compute the chain faces' again: the faster total wins
+      std::vector<Group> gq = x->groups;
+      std::vector<double> perq;
+      double sumq = 0, t_all = 0, t_orph = 0;
+      if ((rc = tune_groups(true, gq, perq, &sumq))) return rc;
+      if ((rc = timed([&]() { return launch_rhs_faces(x, true, 0); }, &t_all))) return rc;
+      if ((rc = timed([&]() { return launch_rhs_faces(x, true, x->nChainProc); }, &t_orph))) return rc;
+      if (sumq + t_orph < sum + t_all) {
+        rj = true;
+        gs.swap(gq);
+        per.swap(perq);
+      }
+      // every face's Jacobians stored again, as hamstr_compute_rhs leaves them
+      if ((rc = launch_rhs_faces(x, true, 0))) return rc;
     }
+    x->groups = gs;
+    x->jac_recompute = rj;
+    for (size_t k = 0; k < per.size() && best_ms && (int)k < cap; k++) best_ms[k] = per[k];
     return HAMSTR_OK;
   };
   const int rc = search();

@@ -45,6 +45,7 @@ EXPORTS = [
     "hamstr_update_solution", "hamstr_set_scheme", "hamstr_get_scheme",
     "hamstr_write_solution", "hamstr_read_solution", "hamstr_block_tridag4_noalloc",
     "hamstr_flux_roe3d", "hamstr_jacobian_roe3d", "hamstr_block_tridag5", "hamstr_block_tridag_periodic5",
+    "hamstr_set_jacobian_recompute", "hamstr_jacobian_recompute",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -478,6 +479,18 @@ def set_adi_kernel(solver: Solver, group: int, kernel: int):
     """ADI kernel of a FAST launch group: KERNEL_ROW, or KERNEL_HALF /
     KERNEL_HALF2 (open chains) where the group's chains fit them."""
     _chk(lib().hamstr_set_adi_kernel(solver.h, int(group), int(kernel)))
+
+
+def set_jacobian_recompute(solver: Solver, on: bool):
+    """Fused FAST iteration: compute the chain faces' Jacobians again in the
+    ADI kernels (on) instead of storing and reading them (off, the default)."""
+    _chk(lib().hamstr_set_jacobian_recompute(solver.h, int(bool(on))))
+
+
+def jacobian_recompute(solver: Solver) -> bool:
+    on = C.c_int()
+    _chk(lib().hamstr_jacobian_recompute(solver.h, C.byref(on)))
+    return bool(on.value)
 
 
 def launch_groups(solver: Solver):
