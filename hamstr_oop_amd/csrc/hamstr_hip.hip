@@ -1422,9 +1422,9 @@ __device__ __forceinline__ void row_reduce_lds(T* A, T* C, T* F, const T* src, i
 // of different lengths side by side meet the same barriers: the levels an
 // N-row system does not need leave its rows without neighbours, which only
 // renormalises them by the identity.  QUAD: run the parallel cyclic
-// reduction by quads of lanes where the threads allow it (it pays where the
-// SIMDs would otherwise idle; the two-chain and two-row kernels keep them
-// busy, and it would cost them registers).
+// reduction by quads of lanes where the threads allow it (the half-LDS
+// kernels: their 96-member PCR levels otherwise keep 1.5 waves of a chain
+// busy; the two-row long-chain kernel has no threads to spare).
 template <typename T, int FINAL, int SFIX = 0, bool QUAD = true>
 __device__ __forceinline__ void fast_solve_lds(T* lds, const int N, const int NL, const int ncr, const int i,
                                                const int NT) {
@@ -2161,8 +2161,9 @@ __device__ __forceinline__ void adi_fast_chain_long(ChainTab ct, const int ch, c
 // hbase[ch]); each even row absorbs its two odd neighbours in registers and
 // replaces them in LDS, as a system of M rows that fast_solve_lds finishes
 // with ncr-1 more levels; the odd rows then back-substitute from scratch.
-// Half the LDS of adi_fast_chain, in 164-166 VGPRs.  The arithmetic is that
-// of adi_fast_chain with ncr levels.
+// Half the LDS of adi_fast_chain, at most 168 VGPRs.  The parallel cyclic
+// reduction runs by quads of lanes.  The arithmetic is that of
+// adi_fast_chain with ncr levels.
 // Arguments: t, the thread's index in the group of NT threads solving chain
 // ch (ch < 0: none; that group only meets the barriers), its LDS slice `lds`,
 // and NL >= N, the row count the reduction levels run for (NL == N for
@@ -2201,7 +2202,7 @@ __device__ __forceinline__ void adi_fast_chain_half(ChainTab ct, const JacSrc& j
     if (active && even) lds_store_row(lds, SM, lpad(t), a, c, f);
   }
   __syncthreads();
-  fast_solve_lds<T, FINAL, SM, false>(lds, M, (NL + 1) / 2, ncr - 1, t, NT);
+  fast_solve_lds<T, FINAL, SM, true>(lds, M, (NL + 1) / 2, ncr - 1, t, NT);
   if (active) {
     T x[4];
     if (even) {
