@@ -282,23 +282,25 @@ __device__ __forceinline__ void face_jac_q(const JacSrc& js, int p, TL* lm, TL* 
 // FASTJ: the FAST-mode Jacobians (reciprocals instead of divisions); the flux
 // and spectral radius are the reference's arithmetic either way
 // ff_from: the Jacobians are computed and stored for processed faces p >=
-// ff_from only (the fused FAST iteration stores none of the chain faces',
-// which its ADI kernels compute again from q)
+// ff_from and the last faces of open chains only (the fused FAST iteration
+// can leave the other chain faces' to its ADI kernels, which compute them
+// again from q)
 template <int ORDER, typename TL, bool IMPL, bool FASTJ>
 __global__ __launch_bounds__(kBlock) void k_rhs_faces(FaceTab ft, Consts c, Live lv, const double* __restrict__ q,
                                                       double* __restrict__ flux, double* __restrict__ sr,
                                                       TL* __restrict__ ff, int ff_from, int* __restrict__ nanflag) {
   const int p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (p >= c.nProc || (lv.face && !lv.face[p])) return;
+  const int code = ft.pslot[p];
   double left[4], right[4];
-  face_states<ORDER>(ft, q, c.eps, c.qt, ft.pslot[p], left, right);
+  face_states<ORDER>(ft, q, c.eps, c.qt, code, left, right);
   const double2 n = *reinterpret_cast<const double2*>(ft.nvec + 2 * (size_t)p);
   double fl[4], s_r;
   flux_roe2d(n.x, n.y, left, right, c.gamma, fl, &s_r);
   *reinterpret_cast<double4*>(flux + 4 * (size_t)p) = make_double4(fl[0], fl[1], fl[2], fl[3]);
   sr[p] = s_r;
   if (s_r != s_r) atomicOr(nanflag, 1);
-  if (IMPL && p >= ff_from) {
+  if (IMPL && (p >= ff_from || ((code >> 30) & 1))) {
     TL lm[16], rm[16];
     face_jacobians<TL, FASTJ>(n.x, n.y, left, right, c.gamma, lm, rm);
 #pragma unroll
@@ -1652,9 +1654,9 @@ __device__ __forceinline__ void fast_solve_lds(T* lds, const int N, const int NL
 // row taking only its own face m (L_m, R_m) from js: face m+1, which row m
 // also needs, comes through LDS (`xl`, element e of face j at xl[e * XS + j],
 // 16 entries, XS > N; L first, then R), so that no face is read from memory
-// or computed twice.  An open chain's last face (no row owns it) is taken by
-// the last row; computed from q, its blocks wait in xl[16 * XS + e] (L, then
-// R), so the slice must hold 16 * XS + 32 values.  Row m holds A (= R_m scaled) and C
+// or computed twice.  An open chain's last face (no row owns it) is read from
+// ff by the last row (k_rhs_faces stores it in either case: a second face to
+// compute would hold every row's barrier for the last row).  Row m holds A (= R_m scaled) and C
 // (= 0 - L_(m+1) scaled) before the diagonal block is factored.  Orphan
 // diagonal blocks are read from ff.  Every thread of the group calls it (it
 // holds barriers); inactive threads pass active = false.  The entries take
@@ -1683,16 +1685,6 @@ __device__ __forceinline__ void fast_row_xchg(const ChainTab& ct, const JacSrc& 
   T B[16];
   // L: own face into LDS (computed: R_m too, kept in A), the next face's back
   if (active) {
-    if (comp && has_next && !next_owned) {
-      // the chain's last face first, while nothing else is live
-      T Ln[16], Rn[16];
-      face_jac_q<T>(js, pn, Ln, Rn);
-#pragma unroll
-      for (int e = 0; e < 16; e++) {
-        xl[16 * XS + e] = Ln[e];
-        xl[16 * XS + 16 + e] = Rn[e];
-      }
-    }
     if (comp) {
       face_jac_q<T>(js, pm, B, A);
     } else {
@@ -1706,7 +1698,7 @@ __device__ __forceinline__ void fast_row_xchg(const ChainTab& ct, const JacSrc& 
   if (has_next) {
 #pragma unroll
     for (int e = 0; e < 16; e++) {
-      const T v = next_owned ? xl[e * XS + mn] : (comp ? xl[16 * XS + e] : ff[(size_t)e * nProc + pn]);
+      const T v = next_owned ? xl[e * XS + mn] : ff[(size_t)e * nProc + pn];
       C[e] = v;  // L_(m+1)
     }
   }
@@ -1733,7 +1725,7 @@ __device__ __forceinline__ void fast_row_xchg(const ChainTab& ct, const JacSrc& 
     T R[16];
 #pragma unroll
     for (int e = 0; e < 16; e++)
-      R[e] = next_owned ? xl[e * XS + mn] : (comp ? xl[16 * XS + 16 + e] : ff[(size_t)(16 + e) * nProc + pn]);
+      R[e] = next_owned ? xl[e * XS + mn] : ff[(size_t)(16 + e) * nProc + pn];
     if (periodic) {
 #pragma unroll
       for (int e = 0; e < 16; e++) B[e] -= R[e];
@@ -2273,7 +2265,7 @@ __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) voi
     ChainTab ct, Live lv, JacSrc js, const int* __restrict__ chains, int nch, const T* __restrict__ ff, int nProc,
     double cfl, const double* __restrict__ sigma, double* __restrict__ r, double* __restrict__ dq, int dqmode,
     int ncr, T* __restrict__ hs, int hstride, const int* __restrict__ hbase) {
-  static_assert(16 * (MAXT + 1) + 32 <= 36 * lds_stride((MAXT + 1) / 2), "face exchange fits the slice");
+  static_assert(16 * (MAXT + 1) <= 36 * lds_stride((MAXT + 1) / 2), "face exchange fits the slice");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* lds = reinterpret_cast<T*>(smem_raw);
   const int vb = xcd_block(blockIdx.x, gridDim.x);
@@ -2300,7 +2292,7 @@ __global__ __launch_bounds__(2 * MAXH) __attribute__((amdgpu_waves_per_eu(3))) v
     double cfl, const double* __restrict__ sigma, double* __restrict__ r, double* __restrict__ dq, int dqmode,
     int ncr, T* __restrict__ hs, int hstride, const int* __restrict__ hbase) {
   constexpr int SM = lds_stride((MAXH + 1) / 2);
-  static_assert(16 * (MAXH + 1) + 32 <= 36 * SM, "face exchange fits the slice");
+  static_assert(16 * (MAXH + 1) <= 36 * SM, "face exchange fits the slice");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int sub = threadIdx.x >= MAXH ? 1 : 0;
   T* lds = reinterpret_cast<T*>(smem_raw) + sub * 36 * SM;
