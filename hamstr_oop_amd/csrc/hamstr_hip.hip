@@ -492,17 +492,21 @@ __device__ __forceinline__ void cell_lin_sum(const int* __restrict__ ent, int e0
 // rhs_cells: r and sigma summed in the reference's order, norms, and (in the
 // fused iteration) ADI's scaling r *= CFL/sigma, r0 = r (ADI.cpp:40-49)
 // --------------------------------------------------------------------------
+// Cells [c0, c1) (c0 a multiple of kBlock), block partial c0/kBlock + vb
+// for the cells of virtual block vb: a launch split at c0 leaves every
+// partial as one launch over all cells writes it.
 template <bool ADI_INIT>
-__global__ __launch_bounds__(kBlock) void k_rhs_cells(Consts c, Live lv, const int* __restrict__ off,
+__global__ __launch_bounds__(kBlock) void k_rhs_cells(Consts c, Live lv, int c0, const int* __restrict__ off,
                                                       const int* __restrict__ ent,
                                                       const double* __restrict__ flux,
                                                       const double* __restrict__ sr, double* __restrict__ r,
                                                       double* __restrict__ r0, double* __restrict__ sigma,
-                                                      double* __restrict__ partial, double* __restrict__ rrho) {
+                                                      double* __restrict__ partial, double* __restrict__ rrho,
+                                                      int c1) {
   const int vb = xcd_block(blockIdx.x, gridDim.x);
-  const int i = vb * blockDim.x + threadIdx.x;
+  const int i = c0 + vb * blockDim.x + threadIdx.x;
   double sq = 0.0, mx = 0.0;
-  if (i < c.nCell && (!lv.cell || lv.cell[i])) {
+  if (i < c1 && (!lv.cell || lv.cell[i])) {
     double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, sg = 0.0;
     cell_flux_sum(ent, off[i], off[i + 1], flux, sr, a0, a1, a2, a3, sg);
     sq = a0 * a0;
@@ -518,7 +522,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_cells(Consts c, Live lv, const i
       *reinterpret_cast<double4*>(r + 4 * (size_t)i) = make_double4(a0, a1, a2, a3);
     }
   }
-  block_norm_partial(sq, mx, partial, vb);
+  block_norm_partial(sq, mx, partial, c0 / kBlock + vb);
 }
 
 // --------------------------------------------------------------------------
@@ -2221,15 +2225,85 @@ __device__ __forceinline__ void adi_fast_chain_half(ChainTab ct, const JacSrc& j
 // 6-wave workgroups (384-row chains) pack onto a CU: for odd cycles that costs
 // 22 spilled VGPRs and still took a 256-case batch's 384-row loops from 1.34
 // to 0.95 ms.
+// Work for workgroups beyond the chains' in the same launch: the residual
+// sums of cells [c0, c1) (k_rhs_cells<true>'s, in blocks of kBlock cells with
+// the same block partials), done by workgroups gl.. of a launch whose first gl
+// workgroups walk the chains.  c1 <= c0: none.
+struct CellWork {
+  const int *off, *ent;
+  const double *flux, *sr;
+  double *r0, *partial;
+  int c0, c1, gl;
+};
+
+// the blocks of cw's cells, workgroup wi of nw taking blockDim/kBlock
+// blocks per pass (blockDim a multiple of kBlock); every thread calls it
+__device__ __forceinline__ void rhs_cells_by_blocks(const CellWork& cw, const Consts& c, Live lv, double* __restrict__ r,
+                                                 double* __restrict__ sigma, int wi, int nw) {
+  __shared__ double wsq[1024 / 64], wmx[1024 / 64];
+  const int t = threadIdx.x;
+  const int nsub = blockDim.x / kBlock;
+  const int sub = t / kBlock;
+  const int nb = (cw.c1 - cw.c0 + kBlock - 1) / kBlock;
+  for (int b0 = wi * nsub; b0 < nb; b0 += nw * nsub) {
+    const int blk = b0 + sub;
+    const int i = cw.c0 + blk * kBlock + (t % kBlock);
+    double sq = 0.0, mx = 0.0;
+    if (blk < nb && i < cw.c1 && (!lv.cell || lv.cell[i])) {
+      double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, sg = 0.0;
+      cell_flux_sum(cw.ent, cw.off[i], cw.off[i + 1], cw.flux, cw.sr, a0, a1, a2, a3, sg);
+      sq = a0 * a0;
+      mx = fabs(a0);
+      sigma[i] = sg;
+      const double dtfac = c.cfl / sg;
+      const double4 v = make_double4(a0 * dtfac, a1 * dtfac, a2 * dtfac, a3 * dtfac);
+      *reinterpret_cast<double4*>(r + 4 * (size_t)i) = v;
+      *reinterpret_cast<double4*>(cw.r0 + 4 * (size_t)i) = v;
+    }
+    // block_norm_partial's reduction, per block of kBlock threads
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      sq += __shfl_xor(sq, o);
+      mx = fmax(mx, __shfl_xor(mx, o));
+    }
+    if ((t & 63) == 0) {
+      wsq[t >> 6] = sq;
+      wmx[t >> 6] = mx;
+    }
+    __syncthreads();
+    if (t % kBlock == 0 && blk < nb) {
+      const int w0 = t >> 6;
+      double a = wsq[w0], b = wmx[w0];
+#pragma unroll
+      for (int w = 1; w < kBlock / 64; w++) {
+        a += wsq[w0 + w];
+        b = fmax(b, wmx[w0 + w]);
+      }
+      const int pb = cw.c0 / kBlock + blk;
+      cw.partial[2 * pb] = a;
+      cw.partial[2 * pb + 1] = b;
+    }
+    __syncthreads();
+  }
+}
+
 template <typename T, int FINAL, int MAXT, bool JQ>
 __global__ __launch_bounds__(MAXT)
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && MAXT == 512 ? 4 : 1)))
 void k_adi_fast(ChainTab ct, Live lv, JacSrc js, const int* __restrict__ chains, int nch,
                                                    const T* __restrict__ ff, int nProc, double cfl,
                                                    const double* __restrict__ sigma, double* __restrict__ r,
-                                                   double* __restrict__ dq, int dqmode, int ncr) {
-  const int vb = xcd_block(blockIdx.x, gridDim.x);
-  for (int w = vb; w < nch; w += gridDim.x) {
+                                                   double* __restrict__ dq, int dqmode, int ncr, CellWork cw,
+                                                   Consts cc) {
+  // the chains' workgroups come first in the launch, so that the hardware
+  // deals them to CUs of their own before any cell workgroup
+  const int gl = cw.c1 > cw.c0 ? cw.gl : (int)gridDim.x;
+  if ((int)blockIdx.x >= gl) {
+    rhs_cells_by_blocks(cw, cc, lv, r, const_cast<double*>(sigma), blockIdx.x - gl, gridDim.x - gl);
+    return;
+  }
+  const int vb = xcd_block(blockIdx.x, gl);
+  for (int w = vb; w < nch; w += gl) {
     const int ch = chains[w];
     if (lv.chain && !lv.chain[ch]) continue;  // the same for the whole workgroup
     adi_fast_chain<T, FINAL, JQ>(ct, ch, js, ff, nProc, cfl, sigma, r, dq, dqmode, ncr);
@@ -2836,6 +2910,9 @@ struct hamstr_ctx {
   bool jac_recompute = false;
   int nPhase = 0;
   bool last_phase_all_cells = false;
+  // single case: ADI phase 0 covers exactly the cells [0, split_cells), whole
+  // kBlock blocks, and later phases follow (else 0)
+  int split_cells = 0;
   void* scratch = nullptr;     // EXACT groups: T[nChainProc][36] rows C', F', a' of the back sweep
   void* exact_rows = nullptr;  // EXACT groups: T[nChainProc][52] assembled rows (k_exact_assemble)
   double* partial = nullptr;
@@ -2952,16 +3029,45 @@ int launch_norm(hamstr_ctx* x, double* base, int slot) {
 // part == nullptr: block partials to x->partial and the norms at once into
 // normbase; otherwise block partials to part, reduced later (k_norm_slots /
 // k_case_norms) by the caller
+// k_rhs_cells over cells [c0, c1) (c0 a multiple of kBlock), block partials to pp
+int launch_rhs_cells_range(hamstr_ctx* x, bool adi_init, double* pp, int c0, int c1) {
+  if (c1 <= c0) return HAMSTR_OK;
+  const dim3 g(nblk(c1 - c0)), b(kBlock);
+  if (adi_init)
+    hipLaunchKernelGGL((k_rhs_cells<true>), g, b, 0, x->stream, x->c, x->live(), c0, x->cell_off, x->cell_ent,
+                       x->flux, x->sr, x->r, x->r0, x->sigma, pp, x->rrho, c1);
+  else
+    hipLaunchKernelGGL((k_rhs_cells<false>), g, b, 0, x->stream, x->c, x->live(), c0, x->cell_off, x->cell_ent,
+                       x->flux, x->sr, x->r, x->r0, x->sigma, pp, x->rrho, c1);
+  HIPCHK(hipGetLastError());
+  return HAMSTR_OK;
+}
+
 int launch_rhs_cells(hamstr_ctx* x, bool adi_init, double* normbase, double* part = nullptr) {
   double* pp = part ? part : x->partial;
-  if (adi_init)
-    hipLaunchKernelGGL((k_rhs_cells<true>), dim3(nblk(x->nCell)), dim3(kBlock), 0, x->stream, x->c, x->live(), x->cell_off,
-                       x->cell_ent, x->flux, x->sr, x->r, x->r0, x->sigma, pp, x->rrho);
-  else
-    hipLaunchKernelGGL((k_rhs_cells<false>), dim3(nblk(x->nCell)), dim3(kBlock), 0, x->stream, x->c, x->live(),
-                       x->cell_off, x->cell_ent, x->flux, x->sr, x->r, x->r0, x->sigma, pp, x->rrho);
-  HIPCHK(hipGetLastError());
+  const int rc = launch_rhs_cells_range(x, adi_init, pp, 0, x->nCell);
+  if (rc) return rc;
   return part ? HAMSTR_OK : launch_norm(x, normbase, 0);
+}
+
+// Workgroups the fused iteration adds to ADI phase 0's launch for the residual
+// sums of the cells phase 0 does not cover (0: none).  Only for a single case
+// whose phase 0 is one group of the one-row kernel over the leading cells
+// [0, split_cells) on fewer workgroups than the device has CUs: the loops of
+// an O-grid then leave CUs idle (one 512-row loop per CU, 128 loops).
+int cells_overlap_extra(const hamstr_ctx* x) {
+  if (x->split_cells <= 0 || x->nCase != 1) return 0;
+  const Group* g0 = nullptr;
+  int n0 = 0;
+  for (const Group& g : x->groups)
+    if (g.phase == 0) {
+      g0 = &g;
+      n0++;
+    }
+  if (n0 != 1 || !g0->fast || g0->two || g0->kern == kKernHalf || g0->kern == kKernHalf2) return 0;
+  if (g0->block % kBlock != 0 || g0->block > 1024) return 0;
+  const int used = std::min(g0->grid, g0->n);
+  return used < x->ncu ? x->ncu - used : 0;
 }
 
 // Whether every ADI launch group can compute its chain faces' Jacobians from
@@ -2979,23 +3085,35 @@ bool jac_recompute_possible(const hamstr_ctx* x) {
 
 // recompute: the group's chain-face Jacobians computed again from q (the
 // fused iteration, when the autotuner chose it), else read from ff
-int launch_group(hamstr_ctx* x, const Group& g, int grid, int dqmode, int ncr, bool recompute = false) {
+// cells: k_rhs_cells<true>'s work on cells [c0, c1), done in the same launch
+// by `extra` workgroups after the chains' (the one-row kernel only; null or
+// extra == 0: none)
+int launch_group(hamstr_ctx* x, const Group& g, int grid, int dqmode, int ncr, bool recompute = false,
+                 const CellWork* cells = nullptr, int extra = 0) {
   const ChainTab ct = x->ctab();
   const int* chains = x->group_chains + g.off;
+  CellWork cw{};
+  if (cells && extra > 0) {
+    cw = *cells;
+    cw.gl = grid;
+  } else {
+    extra = 0;
+  }
   if (g.fast) {
     const JacSrc js{x->ftab(), x->q, x->c.gamma, x->c.qt, x->c.eps, recompute ? x->prm.order : 0};
     const dim3 gr(grid), b(g.block);
     // long chains always run the first reduction level (it is what halves
     // them), and so does the half-LDS kernel
     const bool half = g.kern == kKernHalf || g.kern == kKernHalf2;
+    if (extra > 0 && (half || g.two)) return fail(HAMSTR_ERR_ARG, "cell work rides in the one-row kernel only");
     const int nc = (g.two || half) ? std::max(ncr, 1) : ncr;
     const size_t lds = (x->lhs32() ? sizeof(float) : sizeof(double)) * 36 *
                        (half ? (g.kern == kKernHalf2 ? 2 * (size_t)lds_stride((kHalfMaxRows + 1) / 2)
                                                      : (size_t)lds_stride((half_maxt(g.block) + 1) / 2))
                              : (size_t)lds_stride(g.two ? (g.maxN + 1) / 2 : g.maxN));
 #define FASTJ(T, F, M, J)                                                                                          \
-  hipLaunchKernelGGL((k_adi_fast<T, F, M, J>), gr, b, lds, x->stream, ct, x->live(), js, chains, g.n,               \
-                     (const T*)x->ff, x->nProc, x->c.cfl, x->sigma, x->r, x->dq, dqmode, nc)
+  hipLaunchKernelGGL((k_adi_fast<T, F, M, J>), dim3(grid + extra), b, lds, x->stream, ct, x->live(), js, chains,    \
+                     g.n, (const T*)x->ff, x->nProc, x->c.cfl, x->sigma, x->r, x->dq, dqmode, nc, cw, x->c)
 #define FASTT(T, F, M)             \
   do {                             \
     if (recompute)                 \
@@ -3057,12 +3175,14 @@ int launch_group(hamstr_ctx* x, const Group& g, int grid, int dqmode, int ncr, b
 
 // all phases of one sweep; the last phase also accumulates dq when it covers
 // every cell, otherwise a separate kernel does (ADI.cpp:57-248)
+// cells / extra: residual sums added to phase 0's launch (launch_group)
 int launch_sweep_phases(hamstr_ctx* x, int sweep, const std::function<int(const char*)>& mark,
-                        bool recompute = false) {
+                        bool recompute = false, const CellWork* cells = nullptr, int extra = 0) {
   for (size_t k = 0; k < x->groups.size(); k++) {
     const Group& g = x->groups[k];
     const bool fuse = (g.phase == x->nPhase - 1) && x->last_phase_all_cells;
-    const int rc = launch_group(x, g, g.grid, fuse ? (sweep == 0 ? 1 : 2) : 0, g.ncr, recompute);
+    const int rc = launch_group(x, g, g.grid, fuse ? (sweep == 0 ? 1 : 2) : 0, g.ncr, recompute,
+                                g.phase == 0 ? cells : nullptr, g.phase == 0 ? extra : 0);
     if (rc) return rc;
     const bool phase_end = (k + 1 == x->groups.size()) || x->groups[k + 1].phase != g.phase;
     if (mark && phase_end &&
@@ -3496,6 +3616,24 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
     for (int i = 0; i < nChain; i++)
       if (phase[i] == nPhase - 1) cover += ch_ncell[i];
     x->last_phase_all_cells = (cover == nCell);
+  }
+  if (nCase == 1 && nPhase >= 2) {
+    // are phase 0's cells exactly [0, P) with P a multiple of kBlock?
+    std::vector<char> in0(nCell, 0);
+    long n0 = 0;
+    int top = -1;
+    for (int i = 0; i < nChain; i++) {
+      if (phase[i] != 0) continue;
+      for (int k = 0; k < ch_ncell[i]; k++) {
+        const int cl = pleft[ch_pbase[i] + k];
+        if (!in0[cl]) {
+          in0[cl] = 1;
+          n0++;
+          top = std::max(top, cl);
+        }
+      }
+    }
+    if (n0 > 0 && top + 1 == n0 && n0 % kBlock == 0 && n0 < nCell) x->split_cells = (int)n0;
   }
   x->qinf_h.assign(4 * (size_t)nCase, 0.0);
   for (int k = 0; k < nCase; k++) freestream_state(cases[k].mach, cases[k].alpha, &x->qinf_h[4 * k]);
@@ -4097,10 +4235,21 @@ int enqueue_adi_iteration(hamstr_ctx* x, double* nb, const std::function<int(con
   const bool rj = x->jac_recompute && jac_recompute_possible(x);
   if ((rc = launch_rhs_faces(x, true, rj ? x->nChainProc : 0))) return rc;
   if (mark && mark("rhs_faces")) return HAMSTR_ERR_HIP;
-  if ((rc = launch_rhs_cells(x, true, nb, p_nl))) return rc;
+  // phase 0 over the leading cells on part of the device: the other cells'
+  // residual sums ride in phase 0's launch, on the CUs its chains leave idle
+  // (phase 0's rows read only their own cells' r and sigma; the block
+  // partials are those of one k_rhs_cells launch)
+  const int extra = p_nl ? cells_overlap_extra(x) : 0;
+  CellWork cw{};
+  if (extra > 0) {
+    if ((rc = launch_rhs_cells_range(x, true, p_nl, 0, x->split_cells))) return rc;
+    cw = CellWork{x->cell_off, x->cell_ent, x->flux, x->sr, x->r0, p_nl, x->split_cells, x->nCell, 0};
+  } else if ((rc = launch_rhs_cells(x, true, nb, p_nl))) {
+    return rc;
+  }
   if (mark && mark("rhs_cells")) return HAMSTR_ERR_HIP;
   for (int sw = 0; sw < x->prm.msweep; sw++) {
-    if ((rc = launch_sweep_phases(x, sw, mark, rj))) return rc;
+    if ((rc = launch_sweep_phases(x, sw, mark, rj, sw == 0 ? &cw : nullptr, sw == 0 ? extra : 0))) return rc;
     // the Jacobians computed again beat reading them back (FP64: 26.9 ->
     // 24.1 us on the 512x256 mixed mesh; FP32: the same there, 1.14 -> 0.84
     // ms in a 256-case batch)
