@@ -248,29 +248,41 @@ __device__ __forceinline__ void face_states(const FaceTab& ft, const double* __r
 }
 
 // Where a FAST ADI kernel takes the face Jacobians from.  order 0: the array
-// ff k_rhs_faces stored.  order 1, 3 or 5: computed again from q at that
-// order (face_jac_q), which the fused FAST iteration does for the chain
-// faces instead of storing and reading back 32 values per face.
+// ff k_rhs_faces stored.  Otherwise computed again (face_jac_q), which the
+// fused FAST iteration can do for the chain faces instead of storing and
+// reading back 32 values per face: from the face states k_rhs_faces stored
+// (fst non-null: [8][nProc], left then right), or from q at `order` (1, 3
+// or 5: the stencil gathered and reconstructed again).
 struct JacSrc {
   FaceTab ft;
   const double* q;
   double gamma, qt, eps;
   int order;
+  const double* fst;
+  int nProc;
 };
 
-// the FAST-mode Jacobians of processed face p from q: the states and the
-// code k_rhs_faces uses (face_jacobians is opaque to its surroundings), so
-// bit for bit what k_rhs_faces stores
+// the FAST-mode Jacobians of processed face p: the states and the code
+// k_rhs_faces uses (face_jacobians is opaque to its surroundings), so bit
+// for bit what k_rhs_faces stores
 template <typename TL>
 __device__ __forceinline__ void face_jac_q(const JacSrc& js, int p, TL* lm, TL* rm) {
-  const int code = js.ft.pslot[p];
   double left[4], right[4];
-  if (js.order == 1)
-    face_states<1>(js.ft, js.q, js.eps, js.qt, code, left, right);
-  else if (js.order == 3)
-    face_states<3>(js.ft, js.q, js.eps, js.qt, code, left, right);
-  else
-    face_states<5>(js.ft, js.q, js.eps, js.qt, code, left, right);
+  if (js.fst) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      left[k] = js.fst[(size_t)k * js.nProc + p];
+      right[k] = js.fst[(size_t)(4 + k) * js.nProc + p];
+    }
+  } else {
+    const int code = js.ft.pslot[p];
+    if (js.order == 1)
+      face_states<1>(js.ft, js.q, js.eps, js.qt, code, left, right);
+    else if (js.order == 3)
+      face_states<3>(js.ft, js.q, js.eps, js.qt, code, left, right);
+    else
+      face_states<5>(js.ft, js.q, js.eps, js.qt, code, left, right);
+  }
   const double2 n = *reinterpret_cast<const double2*>(js.ft.nvec + 2 * (size_t)p);
   face_jacobians<TL, true>(n.x, n.y, left, right, js.gamma, lm, rm);
 }
@@ -284,11 +296,13 @@ __device__ __forceinline__ void face_jac_q(const JacSrc& js, int p, TL* lm, TL* 
 // ff_from: the Jacobians are computed and stored for processed faces p >=
 // ff_from and the last faces of open chains only (the fused FAST iteration
 // can leave the other chain faces' to its ADI kernels, which compute them
-// again from q)
+// again); fst non-null: those other faces' two states are stored instead
+// ([8][nProc], left then right) for the ADI kernels to compute them from
 template <int ORDER, typename TL, bool IMPL, bool FASTJ>
 __global__ __launch_bounds__(kBlock) void k_rhs_faces(FaceTab ft, Consts c, Live lv, const double* __restrict__ q,
                                                       double* __restrict__ flux, double* __restrict__ sr,
-                                                      TL* __restrict__ ff, int ff_from, int* __restrict__ nanflag) {
+                                                      TL* __restrict__ ff, int ff_from, double* __restrict__ fst,
+                                                      int* __restrict__ nanflag) {
   const int p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (p >= c.nProc || (lv.face && !lv.face[p])) return;
   const int code = ft.pslot[p];
@@ -300,7 +314,15 @@ __global__ __launch_bounds__(kBlock) void k_rhs_faces(FaceTab ft, Consts c, Live
   *reinterpret_cast<double4*>(flux + 4 * (size_t)p) = make_double4(fl[0], fl[1], fl[2], fl[3]);
   sr[p] = s_r;
   if (s_r != s_r) atomicOr(nanflag, 1);
-  if (IMPL && (p >= ff_from || ((code >> 30) & 1))) {
+  const bool store_jac = p >= ff_from || ((code >> 30) & 1);
+  if (IMPL && fst && !store_jac) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      fst[(size_t)k * c.nProc + p] = left[k];
+      fst[(size_t)(4 + k) * c.nProc + p] = right[k];
+    }
+  }
+  if (IMPL && store_jac) {
     TL lm[16], rm[16];
     face_jacobians<TL, FASTJ>(n.x, n.y, left, right, c.gamma, lm, rm);
 #pragma unroll
@@ -2905,9 +2927,10 @@ struct hamstr_ctx {
   int nLong = 0;
   int* group_chains = nullptr;
   std::vector<Group> groups;   // in phase order
-  // the fused FAST iteration computes the chain faces' Jacobians again in the
-  // ADI kernels instead of storing them (chosen by hamstr_autotune)
-  bool jac_recompute = false;
+  // where the fused FAST iteration's ADI kernels take the chain faces'
+  // Jacobians from (HAMSTR_JAC_*; chosen by hamstr_autotune)
+  int jac_src = HAMSTR_JAC_STORED;
+  double* fst = nullptr;  // HAMSTR_JAC_FROM_STATES: [8][nProc] face states
   int nPhase = 0;
   bool last_phase_all_cells = false;
   // single case: ADI phase 0 covers exactly the cells [0, split_cells), whole
@@ -2962,12 +2985,12 @@ namespace {
 // ff_from: the first processed face whose Jacobians are stored (0: all;
 // nChainProc: the orphan faces only, for ADI kernels that compute the chain
 // faces' Jacobians again)
-int launch_rhs_faces(hamstr_ctx* x, bool implicit, int ff_from = 0) {
+int launch_rhs_faces(hamstr_ctx* x, bool implicit, int ff_from = 0, double* fst = nullptr) {
   const FaceTab ft = x->ftab();
   dim3 g(nblk(x->nProc)), b(kBlock);
 #define RHSF(O, T, I, F)                                                                                         \
   hipLaunchKernelGGL((k_rhs_faces<O, T, I, F>), g, b, 0, x->stream, ft, x->c, x->live(), x->q, x->flux, x->sr, (T*)x->ff, \
-                     ff_from, x->nanflag)
+                     ff_from, fst, x->nanflag)
 #define RHSO(T, I, F) \
   if (o == 1) RHSF(1, T, I, F); else if (o == 3) RHSF(3, T, I, F); else RHSF(5, T, I, F)
   const int o = x->prm.order;
@@ -3074,8 +3097,13 @@ int cells_overlap_extra(const hamstr_ctx* x) {
 // q: FAST kernels other than the long-chain one, and not FP32 odd cycles,
 // whose one-row kernel, held to 128 VGPRs, would spill some 500 registers
 // computing them.  The fused iteration does so (and stores the orphan faces'
-// Jacobians only) when this holds and the autotuner found it faster
-// (x->jac_recompute).
+// Jacobians only) when this holds and x->jac_src asks for it.
+bool jac_recompute_possible(const hamstr_ctx* x);
+// the source the fused iteration uses now: the chosen one where possible
+int jac_source(const hamstr_ctx* x) {
+  return jac_recompute_possible(x) ? x->jac_src : HAMSTR_JAC_STORED;
+}
+
 bool jac_recompute_possible(const hamstr_ctx* x) {
   if (x->prm.adi_mode != HAMSTR_ADI_FAST || x->groups.empty()) return false;
   for (const Group& g : x->groups)
@@ -3083,13 +3111,14 @@ bool jac_recompute_possible(const hamstr_ctx* x) {
   return true;
 }
 
-// recompute: the group's chain-face Jacobians computed again from q (the
-// fused iteration, when the autotuner chose it), else read from ff
+// jsrc: where the group's chain-face Jacobians come from (HAMSTR_JAC_*; the
+// fused iteration's choice, else read from ff)
 // cells: k_rhs_cells<true>'s work on cells [c0, c1), done in the same launch
 // by `extra` workgroups after the chains' (the one-row kernel only; null or
 // extra == 0: none)
-int launch_group(hamstr_ctx* x, const Group& g, int grid, int dqmode, int ncr, bool recompute = false,
+int launch_group(hamstr_ctx* x, const Group& g, int grid, int dqmode, int ncr, int jsrc = HAMSTR_JAC_STORED,
                  const CellWork* cells = nullptr, int extra = 0) {
+  const bool recompute = jsrc != HAMSTR_JAC_STORED;
   const ChainTab ct = x->ctab();
   const int* chains = x->group_chains + g.off;
   CellWork cw{};
@@ -3100,7 +3129,8 @@ int launch_group(hamstr_ctx* x, const Group& g, int grid, int dqmode, int ncr, b
     extra = 0;
   }
   if (g.fast) {
-    const JacSrc js{x->ftab(), x->q, x->c.gamma, x->c.qt, x->c.eps, recompute ? x->prm.order : 0};
+    const JacSrc js{x->ftab(), x->q, x->c.gamma, x->c.qt, x->c.eps, recompute ? x->prm.order : 0,
+                    jsrc == HAMSTR_JAC_FROM_STATES ? x->fst : nullptr, x->nProc};
     const dim3 gr(grid), b(g.block);
     // long chains always run the first reduction level (it is what halves
     // them), and so does the half-LDS kernel
@@ -3177,11 +3207,11 @@ int launch_group(hamstr_ctx* x, const Group& g, int grid, int dqmode, int ncr, b
 // every cell, otherwise a separate kernel does (ADI.cpp:57-248)
 // cells / extra: residual sums added to phase 0's launch (launch_group)
 int launch_sweep_phases(hamstr_ctx* x, int sweep, const std::function<int(const char*)>& mark,
-                        bool recompute = false, const CellWork* cells = nullptr, int extra = 0) {
+                        int jsrc = HAMSTR_JAC_STORED, const CellWork* cells = nullptr, int extra = 0) {
   for (size_t k = 0; k < x->groups.size(); k++) {
     const Group& g = x->groups[k];
     const bool fuse = (g.phase == x->nPhase - 1) && x->last_phase_all_cells;
-    const int rc = launch_group(x, g, g.grid, fuse ? (sweep == 0 ? 1 : 2) : 0, g.ncr, recompute,
+    const int rc = launch_group(x, g, g.grid, fuse ? (sweep == 0 ? 1 : 2) : 0, g.ncr, jsrc,
                                 g.phase == 0 ? cells : nullptr, g.phase == 0 ? extra : 0);
     if (rc) return rc;
     const bool phase_end = (k + 1 == x->groups.size()) || x->groups[k + 1].phase != g.phase;
@@ -3252,7 +3282,7 @@ void free_ctx(hamstr_ctx* x) {
                   x->cell_off, x->cell_ent, x->ch_pbase, x->ch_nf, x->ch_ncell, x->ch_periodic, x->group_chains,
                   x->scratch, x->partial, x->norms, x->hist, x->nanflag, x->rrho, x->face_on, x->cell_on,
                   x->chain_on, x->ghist, x->npart, x->keep_norms, x->od_chain, x->od_off, x->od_ent,
-                  x->ch_hbase, x->long_scratch, x->exact_rows};
+                  x->ch_hbase, x->long_scratch, x->exact_rows, x->fst};
   if (x->gexec) (void)hipGraphExecDestroy(x->gexec);
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -4229,11 +4259,13 @@ int enqueue_iteration(hamstr_ctx* x, double* nb, const std::function<int(const c
 int enqueue_adi_iteration(hamstr_ctx* x, double* nb, const std::function<int(const char*)>& mark, double* p_nl,
                           double* p_lin) {
   int rc;
-  // FAST: the ADI kernels compute the chain faces' Jacobians again from q
-  // (k_rhs_faces stores the orphan faces' only): 256 B per face neither
-  // written nor read back (FP64)
-  const bool rj = x->jac_recompute && jac_recompute_possible(x);
-  if ((rc = launch_rhs_faces(x, true, rj ? x->nChainProc : 0))) return rc;
+  // FAST: the ADI kernels may compute the chain faces' Jacobians again, from
+  // the face states k_rhs_faces stores instead (64 B per face, FP64 256 B)
+  // or from q; k_rhs_faces then stores the orphan faces' Jacobians only
+  const int rj = jac_source(x);
+  if ((rc = launch_rhs_faces(x, true, rj != HAMSTR_JAC_STORED ? x->nChainProc : 0,
+                             rj == HAMSTR_JAC_FROM_STATES ? x->fst : nullptr)))
+    return rc;
   if (mark && mark("rhs_faces")) return HAMSTR_ERR_HIP;
   // phase 0 over the leading cells on part of the device: the other cells'
   // residual sums ride in phase 0's launch, on the CUs its chains leave idle
@@ -4508,19 +4540,31 @@ int hamstr_set_launch(hamstr_ctx* x, int group, int grid, int cr_levels) {
   return HAMSTR_OK;
 }
 
-int hamstr_set_jacobian_recompute(hamstr_ctx* x, int on) {
+// the face-state array of HAMSTR_JAC_FROM_STATES, allocated on first use
+int ensure_face_states(hamstr_ctx* x) {
+  if (!x->fst) HIPCHK(hipMalloc(&x->fst, sizeof(double) * 8 * (size_t)x->nProc));
+  return HAMSTR_OK;
+}
+
+int hamstr_set_jacobian_source(hamstr_ctx* x, int src) {
   if (!x) return fail(HAMSTR_ERR_ARG, "null context");
-  if (on && !jac_recompute_possible(x))
+  if (src != HAMSTR_JAC_STORED && src != HAMSTR_JAC_FROM_Q && src != HAMSTR_JAC_FROM_STATES)
+    return fail(HAMSTR_ERR_ARG, "unknown Jacobian source");
+  if (src != HAMSTR_JAC_STORED && !jac_recompute_possible(x))
     return fail(HAMSTR_ERR_ARG, "the Jacobians can be computed again only by FAST kernels other than the "
                                 "long-chain one, and not for FP32 odd cycles");
-  x->jac_recompute = on != 0;
+  if (src == HAMSTR_JAC_FROM_STATES) {
+    const int rc = ensure_face_states(x);
+    if (rc) return rc;
+  }
+  x->jac_src = src;
   drop_graph(x);
   return HAMSTR_OK;
 }
 
-int hamstr_jacobian_recompute(const hamstr_ctx* x, int* on) {
-  if (!x || !on) return fail(HAMSTR_ERR_ARG, "null argument");
-  *on = (x->jac_recompute && jac_recompute_possible(x)) ? 1 : 0;
+int hamstr_jacobian_source(const hamstr_ctx* x, int* src) {
+  if (!x || !src) return fail(HAMSTR_ERR_ARG, "null argument");
+  *src = jac_source(x);
   return HAMSTR_OK;
 }
 
@@ -4676,7 +4720,7 @@ int hamstr_autotune(hamstr_ctx* x, int reps, double* best_ms, int cap, int* ngro
     // the group search with the Jacobians read from ff (rj false) or
     // computed again from q (rj true): gs gets every group's best shape,
     // per[k] its time, *sum their total
-    auto tune_groups = [&](bool rj, std::vector<Group>& gs, std::vector<double>& per, double* sum) -> int {
+    auto tune_groups = [&](int rj, std::vector<Group>& gs, std::vector<double>& per, double* sum) -> int {
       per.assign(gs.size(), -1.0);
       *sum = 0;
       for (size_t k = 0; k < gs.size(); k++) {
@@ -4731,28 +4775,36 @@ int hamstr_autotune(hamstr_ctx* x, int reps, double* best_ms, int cap, int* ngro
     std::vector<Group> gs = x->groups;
     std::vector<double> per;
     double sum = 0;
-    if ((rc = tune_groups(false, gs, per, &sum))) return rc;
-    bool rj = false;
+    int rj = HAMSTR_JAC_STORED;
+    if ((rc = tune_groups(HAMSTR_JAC_STORED, gs, per, &sum))) return rc;
     if (jac_recompute_possible(x)) {
-      // the fused iteration either stores every face's Jacobians and the ADI
-      // kernels read them, or stores the orphan faces' only and the ADI
-      // kernels compute the chain faces' again: the faster total wins
-      std::vector<Group> gq = x->groups;
-      std::vector<double> perq;
-      double sumq = 0, t_all = 0, t_orph = 0;
-      if ((rc = tune_groups(true, gq, perq, &sumq))) return rc;
-      if ((rc = timed([&]() { return launch_rhs_faces(x, true, 0); }, &t_all))) return rc;
-      if ((rc = timed([&]() { return launch_rhs_faces(x, true, x->nChainProc); }, &t_orph))) return rc;
-      if (sumq + t_orph < sum + t_all) {
-        rj = true;
-        gs.swap(gq);
-        per.swap(perq);
+      // the fused iteration stores every face's Jacobians and the ADI kernels
+      // read them, or it stores the orphan faces' only and the ADI kernels
+      // compute the chain faces' again, from stored face states or from q:
+      // the fastest total of k_rhs_faces and the groups wins
+      if ((rc = ensure_face_states(x))) return rc;
+      double t_rhs = 0;
+      if ((rc = timed([&]() { return launch_rhs_faces(x, true, 0); }, &t_rhs))) return rc;
+      double best = sum + t_rhs;
+      for (int src : {HAMSTR_JAC_FROM_STATES, HAMSTR_JAC_FROM_Q}) {
+        double* fs = src == HAMSTR_JAC_FROM_STATES ? x->fst : nullptr;
+        if ((rc = timed([&]() { return launch_rhs_faces(x, true, x->nChainProc, fs); }, &t_rhs))) return rc;
+        std::vector<Group> gq = x->groups;
+        std::vector<double> perq;
+        double sumq = 0;
+        if ((rc = tune_groups(src, gq, perq, &sumq))) return rc;
+        if (sumq + t_rhs < best) {
+          best = sumq + t_rhs;
+          rj = src;
+          gs.swap(gq);
+          per.swap(perq);
+        }
       }
       // every face's Jacobians stored again, as hamstr_compute_rhs leaves them
       if ((rc = launch_rhs_faces(x, true, 0))) return rc;
     }
     x->groups = gs;
-    x->jac_recompute = rj;
+    x->jac_src = rj;
     for (size_t k = 0; k < per.size() && best_ms && (int)k < cap; k++) best_ms[k] = per[k];
     return HAMSTR_OK;
   };

@@ -45,7 +45,7 @@ EXPORTS = [
     "hamstr_update_solution", "hamstr_set_scheme", "hamstr_get_scheme",
     "hamstr_write_solution", "hamstr_read_solution", "hamstr_block_tridag4_noalloc",
     "hamstr_flux_roe3d", "hamstr_jacobian_roe3d", "hamstr_block_tridag5", "hamstr_block_tridag_periodic5",
-    "hamstr_set_jacobian_recompute", "hamstr_jacobian_recompute",
+    "hamstr_set_jacobian_source", "hamstr_jacobian_source",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -481,16 +481,20 @@ def set_adi_kernel(solver: Solver, group: int, kernel: int):
     _chk(lib().hamstr_set_adi_kernel(solver.h, int(group), int(kernel)))
 
 
-def set_jacobian_recompute(solver: Solver, on: bool):
-    """Fused FAST iteration: compute the chain faces' Jacobians again in the
-    ADI kernels (on) instead of storing and reading them (off, the default)."""
-    _chk(lib().hamstr_set_jacobian_recompute(solver.h, int(bool(on))))
+JAC_STORED, JAC_FROM_Q, JAC_FROM_STATES = 0, 1, 2
 
 
-def jacobian_recompute(solver: Solver) -> bool:
-    on = C.c_int()
-    _chk(lib().hamstr_jacobian_recompute(solver.h, C.byref(on)))
-    return bool(on.value)
+def set_jacobian_source(solver: Solver, source: int):
+    """Fused FAST iteration: the ADI kernels read the chain faces' stored
+    Jacobians (JAC_STORED, the default) or compute them again from stored face
+    states (JAC_FROM_STATES) or from q (JAC_FROM_Q)."""
+    _chk(lib().hamstr_set_jacobian_source(solver.h, int(source)))
+
+
+def jacobian_source(solver: Solver) -> int:
+    src = C.c_int()
+    _chk(lib().hamstr_jacobian_source(solver.h, C.byref(src)))
+    return src.value
 
 
 def launch_groups(solver: Solver):

@@ -192,16 +192,23 @@ int hamstr_set_launch(hamstr_ctx *ctx, int group, int grid, int cr_levels);
 int hamstr_set_adi_kernel(hamstr_ctx *ctx, int group, int kernel);
 /* Where the fused FAST iteration's ADI kernels take the chain faces' frozen-
  * Roe Jacobians from (no reference counterpart: the reference always stores
- * them, computeRHS.cpp:362-369, and ADI reads them, ADI.cpp:96-97).  off (the
- * default): k_rhs_faces stores every face's 32 entries and the ADI kernels
- * read them; on: it stores the orphan faces' only and the ADI kernels compute
- * the chain faces' again from q, bit for bit the same values.  hamstr_autotune
- * picks the faster.  On is refused (HAMSTR_ERR_ARG) where a group's kernel
- * cannot compute them (EXACT, chains longer than a workgroup, FP32 odd
- * cycles).  The API's own hamstr_compute_rhs / hamstr_adi always store and
- * read every face's. */
-int hamstr_set_jacobian_recompute(hamstr_ctx *ctx, int on);
-int hamstr_jacobian_recompute(const hamstr_ctx *ctx, int *on);
+ * them, computeRHS.cpp:362-369, and ADI reads them, ADI.cpp:96-97).
+ * HAMSTR_JAC_STORED (the default): k_rhs_faces stores every face's 32
+ * entries and the ADI kernels read them.  HAMSTR_JAC_FROM_STATES: it stores
+ * each chain face's two reconstructed states (8 values) and the ADI kernels
+ * compute the Jacobians from them.  HAMSTR_JAC_FROM_Q: the ADI kernels
+ * gather and reconstruct the states again from q.  The computed blocks are
+ * the stored ones bit for bit; orphan faces and the last faces of open
+ * chains keep stored Jacobians.  hamstr_autotune picks the fastest.  The
+ * computed sources are refused (HAMSTR_ERR_ARG) where a group's kernel cannot
+ * compute them (EXACT, chains longer than a workgroup, FP32 odd cycles).
+ * The API's own hamstr_compute_rhs / hamstr_adi always store and read every
+ * face's. */
+#define HAMSTR_JAC_STORED 0
+#define HAMSTR_JAC_FROM_Q 1
+#define HAMSTR_JAC_FROM_STATES 2
+int hamstr_set_jacobian_source(hamstr_ctx *ctx, int source);
+int hamstr_jacobian_source(const hamstr_ctx *ctx, int *source);
 
 /* SOLVER::initFlow (initFlow.cpp:12-93): q := freestream. */
 int hamstr_init_flow(hamstr_ctx *ctx);

@@ -1,10 +1,11 @@
 """The fused FAST iteration with the chain faces' Jacobians computed again in
-the ADI kernels (hamstr_set_jacobian_recompute on: k_rhs_faces stores the
-orphan faces' Jacobians only) against the same iteration storing every face's
-Jacobians and reading them back (off): face_jacobians is opaque to the code
-around it, so the recomputed blocks are the stored ones bit for bit, and the
-states after a run of iterations are equal (up to the sign of zero entries,
-which numpy's comparison ignores).  Covered: orders 1, 3 and 5, FP64 and FP32
+the ADI kernels (hamstr_set_jacobian_source: from the face states k_rhs_faces
+stores, or from q; k_rhs_faces then stores the Jacobians of orphan faces and
+chain-end faces only) against the same iteration storing every face's
+Jacobians and reading them back (HAMSTR_JAC_STORED): face_jacobians is opaque
+to the code around it, so the recomputed blocks are the stored ones bit for
+bit, and the states after a run of iterations are equal (up to the sign of
+zero entries, which numpy's comparison ignores).  Covered: orders 1, 3 and 5, FP64 and FP32
 left-hand sides, the one-row kernel (loops with two-row and odd final cycles,
 open lines), both half-LDS kernels, triangulated meshes with orphan faces,
 msweep 2 and a batch.  The reference always stores them (computeRHS.cpp:362-369)
@@ -43,26 +44,30 @@ def _pair(hip, mesh, order, lhs, msweep=1):
     return (hip.Solver(mesh, 0.5, 2.0, order, 10.0, **kw), hip.Solver(mesh, 0.5, 2.0, order, 10.0, **kw))
 
 
+SOURCES = {"states": 2, "q": 1}
+
+
+@pytest.mark.parametrize("src", list(SOURCES))
 @pytest.mark.parametrize("lhs", ["fp64", "fp32"])
 @pytest.mark.parametrize("order", [1, 3, 5])
 @pytest.mark.parametrize("name", list(MESHES))
-def test_recomputed_jacobians_give_the_same_iteration(hip, name, order, lhs):
+def test_recomputed_jacobians_give_the_same_iteration(hip, name, order, lhs, src):
     from hamstr_oop_amd import meshgen
     mesh = MESHES[name](meshgen)
     L = hip.LHS_FP32 if lhs == "fp32" else hip.LHS_FP64
-    print(f"recomputed Jacobians {name} order={order} {lhs}", flush=True)
+    print(f"recomputed Jacobians {name} order={order} {lhs} from {src}", flush=True)
     a, b = _pair(hip, mesh, order, L)
-    assert not hip.jacobian_recompute(a)
+    assert hip.jacobian_source(a) == hip.JAC_STORED
     odd32 = lhs == "fp32" and any(g["final"] == 2 for g in hip.launch_groups(b))
     if odd32:
         # FP32 odd cycles read their Jacobians from ff
         with pytest.raises(hip.HamstrError):
-            hip.set_jacobian_recompute(b, True)
+            hip.set_jacobian_source(b, SOURCES[src])
         a.close()
         b.close()
         return
-    hip.set_jacobian_recompute(b, True)
-    assert hip.jacobian_recompute(b)
+    hip.set_jacobian_source(b, SOURCES[src])
+    assert hip.jacobian_source(b) == SOURCES[src]
     na, l2a, lla = a.step(12, check_every=4)
     nb, l2b, llb = b.step(12, check_every=4)
     assert na == nb == 12
@@ -79,9 +84,10 @@ def test_half_lds_kernels_and_msweep(hip):
     one-row kernel) and two sweeps per iteration."""
     from hamstr_oop_amd import meshgen
     mesh = meshgen.naca_ogrid(48, 200, tri_layers=64)
-    for kernel in (hip.KERNEL_ROW, hip.KERNEL_HALF, hip.KERNEL_HALF2):
+    for kernel, src in ((hip.KERNEL_ROW, hip.JAC_FROM_STATES), (hip.KERNEL_HALF, hip.JAC_FROM_STATES),
+                        (hip.KERNEL_HALF2, hip.JAC_FROM_STATES), (hip.KERNEL_HALF2, hip.JAC_FROM_Q)):
         for msweep in (1, 2):
-            print(f"recomputed Jacobians kernel={kernel} msweep={msweep}", flush=True)
+            print(f"recomputed Jacobians kernel={kernel} source={src} msweep={msweep}", flush=True)
             a, b = _pair(hip, mesh, 3, hip.LHS_FP64, msweep)
             for s in (a, b):
                 for k, g in enumerate(hip.launch_groups(s)):
@@ -92,7 +98,7 @@ def test_half_lds_kernels_and_msweep(hip):
                         hip.set_adi_kernel(s, k, kk)
                     except hip.HamstrError:
                         assert kk != hip.KERNEL_ROW  # odd cycles keep the one-row kernel
-            hip.set_jacobian_recompute(b, True)
+            hip.set_jacobian_source(b, src)
             a.step(8, check_every=8)
             b.step(8, check_every=8)
             _eq(a.get("q"), b.get("q"), f"q, kernel {kernel}, msweep {msweep}")
@@ -109,32 +115,36 @@ def test_batch_and_autotuner(hip):
              (meshgen.naca_ogrid(64, 16), 0.5, -2.0)]
     ref = hip.Batch(cases, 3, 10.0, adi_mode=hip.ADI_FAST)
     on = hip.Batch(cases, 3, 10.0, adi_mode=hip.ADI_FAST)
+    onq = hip.Batch(cases, 3, 10.0, adi_mode=hip.ADI_FAST)
     tuned = hip.Batch(cases, 3, 10.0, adi_mode=hip.ADI_FAST)
     tuned.computeRHS("implicit")
     hip.autotune(tuned)
-    print(f"autotuner chose recompute={hip.jacobian_recompute(tuned)}", flush=True)
+    print(f"autotuner chose Jacobian source {hip.jacobian_source(tuned)}", flush=True)
     # the tuned kernels, grids and depths on the other two (the depth changes
     # the rounding of the block solves)
     for k, g in enumerate(hip.launch_groups(tuned)):
-        for b in (ref, on):
+        for b in (ref, on, onq):
             if g["kernel"] in (hip.KERNEL_ROW, hip.KERNEL_HALF, hip.KERNEL_HALF2):
                 hip.set_adi_kernel(b, k, g["kernel"])
             hip.set_launch(b, k, g["grid"], g["cr_levels"])
-    hip.set_jacobian_recompute(on, True)
-    for b in (ref, on, tuned):
+    hip.set_jacobian_source(on, hip.JAC_FROM_STATES)
+    hip.set_jacobian_source(onq, hip.JAC_FROM_Q)
+    for b in (ref, on, onq, tuned):
         b.batch_step(20, check_every=10)
     for k in range(len(cases)):
-        _eq(ref.case_q(k), on.case_q(k), f"case {k} recomputed")
+        _eq(ref.case_q(k), on.case_q(k), f"case {k} from states")
+        _eq(ref.case_q(k), onq.case_q(k), f"case {k} from q")
         _eq(ref.case_q(k), tuned.case_q(k), f"case {k} autotuned")
-    for b in (ref, on, tuned):
+    for b in (ref, on, onq, tuned):
         b.close()
 
 
 def test_refused_where_a_kernel_cannot_compute_them(hip):
     from hamstr_oop_amd import meshgen
     s = hip.Solver(meshgen.naca_ogrid(40, 16, tri_layers=6), 0.5, 2.0, 3, 10.0, adi_mode=hip.ADI_EXACT)
-    with pytest.raises(hip.HamstrError):
-        hip.set_jacobian_recompute(s, True)
-    hip.set_jacobian_recompute(s, False)
-    assert not hip.jacobian_recompute(s)
+    for src in (hip.JAC_FROM_Q, hip.JAC_FROM_STATES, 7):
+        with pytest.raises(hip.HamstrError):
+            hip.set_jacobian_source(s, src)
+    hip.set_jacobian_source(s, hip.JAC_STORED)
+    assert hip.jacobian_source(s) == hip.JAC_STORED
     s.close()
