@@ -39,12 +39,16 @@ def _runtime():
     return rts[-1]
 
 
+def _stale(target, sources):
+    return not os.path.exists(target) or any(os.path.getmtime(f) > os.path.getmtime(target) for f in sources)
+
+
 @pytest.fixture(scope="module")
 def san_env():
     rt = _runtime()
-    if not os.path.exists(ORACLE_ASAN):
-        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "asan"], check=True)
-    if not (os.path.exists(HIP_ASAN) and os.path.exists(DRIVER_ASAN)):
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "asan"], check=True)  # incremental
+    srcs = glob.glob(os.path.join(ROOT, "hamstr_oop_amd", "csrc", "*")) + glob.glob(os.path.join(ROOT, "include", "*.h"))
+    if _stale(HIP_ASAN, srcs) or _stale(DRIVER_ASAN, srcs):
         sys.path.insert(0, ROOT)
         from hamstr_oop_amd import build
         build.build(sanitize=True)
