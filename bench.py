@@ -468,9 +468,12 @@ def weno_section(args, nc=1024, nr=412, tri=28, mach=0.8, alpha=1.25):
     res["workload"] = f"NACA0012 O-grid {nc}x{nr - tri} quads + {tri} triangulated far-field layers " \
                       f"({mesh.n_cell} cells, {len(mesh.orphans)} orphan faces), order 5 (WENO), M={mach}, " \
                       f"alpha={alpha}, CFL={args.cfl}"
-    # The FP32 left-hand side's error where a fixed point exists: the same mesh
-    # and scheme at M=0.5 (no shock) converges to the roundoff floor; both
-    # precisions run to L2 <= 1e-12 and the fields are compared.
+    # The FP32 left-hand side's error where the iteration converges: the same
+    # mesh and scheme at M=0.5 (no shock).  Both precisions run to L2 <= 1e-12,
+    # WENO's noise floor here (about 5e-13), and the fields are compared; the
+    # FP64 field's own change over 1000 more steps at that floor is the scale
+    # to read the difference against (WENO's weights keep switching near the
+    # leading edge, moving a few cells there by ~1e-3).
     sub, qsub = {"mach": 0.5, "tol": 1e-12}, {}
     for name, lhs in (("fp64", hip.LHS_FP64), ("fp32_lhs", hip.LHS_FP32)):
         s = hip.Solver(mesh, 0.5, alpha, 5, args.cfl, lhs=lhs, adi_mode=hip.ADI_FAST)
@@ -480,10 +483,18 @@ def weno_section(args, nc=1024, nr=412, tri=28, mach=0.8, alpha=1.25):
         s.sync()
         sub[name] = {"iterations": int(n), "seconds": time.perf_counter() - t0, "final_l2": float(l2[-1]),
                      "converged": bool(l2[-1] <= 1e-12)}
-        qsub[name] = s.get("q").reshape(-1, 4)
+        qsub[name] = s.get("q").reshape(-1, 4).copy()
+        if name == "fp64":
+            s.step(1000, check_every=100)
+            sc = np.abs(qsub[name]).max(axis=0)
+            dself = (np.abs(s.get("q").reshape(-1, 4) - qsub[name]) / sc).max(axis=1)
+            sub[name]["self_change_next_1000_steps_rel"] = {"max": float(dself.max()),
+                                                            "median": float(np.median(dself))}
         s.close()
     scale = np.abs(qsub["fp64"]).max(axis=0)
-    sub["fp32_lhs"]["max_field_diff_vs_fp64_rel"] = float((np.abs(qsub["fp32_lhs"] - qsub["fp64"]) / scale).max())
+    d = (np.abs(qsub["fp32_lhs"] - qsub["fp64"]) / scale).max(axis=1)
+    sub["fp32_lhs"]["max_field_diff_vs_fp64_rel"] = float(d.max())
+    sub["fp32_lhs"]["median_field_diff_vs_fp64_rel"] = float(np.median(d))
     res["subsonic_fixed_point"] = sub
     return res
 
