@@ -172,12 +172,18 @@ def main():
                     help="skip the FAST vs EXACT comparison beyond L2 1e-10")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one plain 256 MiB device-to-device copy (counter calibration under rocprofv3)")
+    ap.add_argument("--weno-profile", type=int, default=0, metavar="N",
+                    help="only the WENO transonic case: N iterations per left-hand side, with stage times "
+                         "(a short command for rocprofv3)")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU only; multi-GPU runs are not part of it "
                                    "(cases would shard whole meshes per GPU)", "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.weno_profile > 0:
+        print(json.dumps(weno_profile(args, args.weno_profile)), flush=True)
+        return
 
     from hamstr_oop_amd import hip, meshgen
     mesh = meshgen.naca_ogrid(args.nc, args.nr, t=0.12, tri_layers=args.tri_layers)
@@ -497,6 +503,31 @@ def weno_section(args, nc=1024, nr=412, tri=28, mach=0.8, alpha=1.25):
     sub["fp32_lhs"]["median_field_diff_vs_fp64_rel"] = float(np.median(d))
     res["subsonic_fixed_point"] = sub
     return res
+
+
+def weno_profile(args, steps, nc=1024, nr=412, tri=28, mach=0.8, alpha=1.25):
+    """The WENO transonic case of weno_section, `steps` iterations per
+    left-hand side from the freestream (graph replay, then the same count with
+    events between kernels for stage times)."""
+    from hamstr_oop_amd import hip, meshgen
+    mesh = meshgen.naca_ogrid(nc, nr, t=0.12, tri_layers=tri)
+    out = {"workload": f"NACA0012 O-grid {nc}x{nr - tri} quads + {tri} triangulated far-field layers "
+                       f"({mesh.n_cell} cells), order 5 (WENO), M={mach}, alpha={alpha}, CFL={args.cfl}",
+           "steps": steps}
+    for name, lhs in (("fp64", hip.LHS_FP64), ("fp32_lhs", hip.LHS_FP32)):
+        s = hip.Solver(mesh, mach, alpha, 5, args.cfl, lhs=lhs, adi_mode=hip.ADI_FAST)
+        s.prepare_graph(steps)
+        s.sync()
+        t0 = time.perf_counter()
+        s.step(steps, check_every=steps)
+        s.sync()
+        dt = time.perf_counter() - t0
+        s.set_timing(True)
+        s.step(steps, check_every=steps)
+        out[name] = {"ms_per_step": dt / steps * 1e3, "cell_iters_per_s": mesh.n_cell * steps / dt,
+                     "stage_ms": {k: round(v, 5) for k, v in s.last_timings().items()}}
+        s.close()
+    return out
 
 
 def copy_probe(nbytes):

@@ -4256,13 +4256,15 @@ int enqueue_iteration(hamstr_ctx* x, double* nb, const std::function<int(const c
 }
 
 // stepSolution's ADI branch (march.cpp:112-117) without the batch norms
-int enqueue_adi_iteration(hamstr_ctx* x, double* nb, const std::function<int(const char*)>& mark, double* p_nl,
-                          double* p_lin) {
+// computeRHS("implicit") and the ADI phases of the first sweep, the chain
+// faces' Jacobians taken from source rj (HAMSTR_JAC_*): the part of the
+// fused iteration the Jacobian source changes (the autotuner times it)
+int enqueue_rhs_sweep0(hamstr_ctx* x, double* nb, const std::function<int(const char*)>& mark, double* p_nl,
+                       int rj) {
   int rc;
   // FAST: the ADI kernels may compute the chain faces' Jacobians again, from
   // the face states k_rhs_faces stores instead (64 B per face, FP64 256 B)
   // or from q; k_rhs_faces then stores the orphan faces' Jacobians only
-  const int rj = jac_source(x);
   if ((rc = launch_rhs_faces(x, true, rj != HAMSTR_JAC_STORED ? x->nChainProc : 0,
                              rj == HAMSTR_JAC_FROM_STATES ? x->fst : nullptr)))
     return rc;
@@ -4280,8 +4282,19 @@ int enqueue_adi_iteration(hamstr_ctx* x, double* nb, const std::function<int(con
     return rc;
   }
   if (mark && mark("rhs_cells")) return HAMSTR_ERR_HIP;
+  return launch_sweep_phases(x, 0, mark, rj, &cw, extra);
+}
+
+int enqueue_adi_iteration(hamstr_ctx* x, double* nb, const std::function<int(const char*)>& mark, double* p_nl,
+                          double* p_lin) {
+  int rc;
+  const int rj = jac_source(x);
   for (int sw = 0; sw < x->prm.msweep; sw++) {
-    if ((rc = launch_sweep_phases(x, sw, mark, rj, sw == 0 ? &cw : nullptr, sw == 0 ? extra : 0))) return rc;
+    if (sw == 0) {
+      if ((rc = enqueue_rhs_sweep0(x, nb, mark, p_nl, rj))) return rc;
+    } else if ((rc = launch_sweep_phases(x, sw, mark, rj))) {
+      return rc;
+    }
     // the Jacobians computed again beat reading them back (FP64: 26.9 ->
     // 24.1 us on the 512x256 mixed mesh; FP32: the same there, 1.14 -> 0.84
     // ms in a 256-case batch)
@@ -4690,16 +4703,18 @@ int hamstr_autotune(hamstr_ctx* x, int reps, double* best_ms, int cap, int* ngro
   HIPCHK(hipGetDeviceProperties(&prop, x->device));
   const int cu = std::max(prop.multiProcessorCount, 1);
   const size_t c4 = sizeof(double) * 4 * x->nCell;
-  double *r_save = nullptr, *dq_save = nullptr;
-  bool saved = false;  // r and dq copied out: restore them
+  double *r_save = nullptr, *dq_save = nullptr, *r0_save = nullptr;
+  bool saved = false;  // r, r0 and dq copied out: restore them
   hipEvent_t e0 = nullptr, e1 = nullptr;
   // the trials; a failure stops them, and r, dq and the buffers are still
   // restored and released below
   auto search = [&]() -> int {
     HIPCHK(hipMalloc(&r_save, c4));
     HIPCHK(hipMalloc(&dq_save, c4));
+    HIPCHK(hipMalloc(&r0_save, c4));
     HIPCHK(hipMemcpyAsync(r_save, x->r, c4, hipMemcpyDeviceToDevice, x->stream));
     HIPCHK(hipMemcpyAsync(dq_save, x->dq, c4, hipMemcpyDeviceToDevice, x->stream));
+    HIPCHK(hipMemcpyAsync(r0_save, x->r0, c4, hipMemcpyDeviceToDevice, x->stream));
     saved = true;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
@@ -4780,21 +4795,29 @@ int hamstr_autotune(hamstr_ctx* x, int reps, double* best_ms, int cap, int* ngro
     if (jac_recompute_possible(x)) {
       // the fused iteration stores every face's Jacobians and the ADI kernels
       // read them, or it stores the orphan faces' only and the ADI kernels
-      // compute the chain faces' again, from stored face states or from q:
-      // the fastest total of k_rhs_faces and the groups wins
+      // compute the chain faces' again, from stored face states or from q.
+      // Each source's groups are tuned for it; then computeRHS and the first
+      // sweep run as the fused iteration runs them (kernel times measured
+      // one by one leave out how each kernel finds the caches), and the
+      // fastest wins.  r, r0 and dq are restored below.
       if ((rc = ensure_face_states(x))) return rc;
-      double t_rhs = 0;
-      if ((rc = timed([&]() { return launch_rhs_faces(x, true, 0); }, &t_rhs))) return rc;
-      double best = sum + t_rhs;
+      const std::vector<Group> orig = x->groups;
+      auto seq = [&](int src, const std::vector<Group>& g, double* ms) -> int {
+        x->groups = g;
+        const int rc2 = timed([&]() { return enqueue_rhs_sweep0(x, nullptr, nullptr, x->partial, src); }, ms);
+        x->groups = orig;
+        return rc2;
+      };
+      double best = 0;
+      if ((rc = seq(HAMSTR_JAC_STORED, gs, &best))) return rc;
       for (int src : {HAMSTR_JAC_FROM_STATES, HAMSTR_JAC_FROM_Q}) {
-        double* fs = src == HAMSTR_JAC_FROM_STATES ? x->fst : nullptr;
-        if ((rc = timed([&]() { return launch_rhs_faces(x, true, x->nChainProc, fs); }, &t_rhs))) return rc;
-        std::vector<Group> gq = x->groups;
+        std::vector<Group> gq = orig;
         std::vector<double> perq;
-        double sumq = 0;
+        double sumq = 0, t = 0;
         if ((rc = tune_groups(src, gq, perq, &sumq))) return rc;
-        if (sumq + t_rhs < best) {
-          best = sumq + t_rhs;
+        if ((rc = seq(src, gq, &t))) return rc;
+        if (t < best) {
+          best = t;
           rj = src;
           gs.swap(gq);
           per.swap(perq);
@@ -4812,10 +4835,12 @@ int hamstr_autotune(hamstr_ctx* x, int reps, double* best_ms, int cap, int* ngro
   if (saved) {
     (void)hipMemcpyAsync(x->r, r_save, c4, hipMemcpyDeviceToDevice, x->stream);
     (void)hipMemcpyAsync(x->dq, dq_save, c4, hipMemcpyDeviceToDevice, x->stream);
+    (void)hipMemcpyAsync(x->r0, r0_save, c4, hipMemcpyDeviceToDevice, x->stream);
   }
   const hipError_t se = hipStreamSynchronize(x->stream);
   if (r_save) (void)hipFree(r_save);
   if (dq_save) (void)hipFree(dq_save);
+  if (r0_save) (void)hipFree(r0_save);
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   if (rc) return rc;
