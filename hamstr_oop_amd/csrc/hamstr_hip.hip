@@ -1307,6 +1307,106 @@ __device__ __forceinline__ void pcr_quad_row(const T* lds, int S, int sO, bool h
   lu4_solve_fast(L, Cc, 1);
 }
 
+// Reduction depths of a FAST launch in one int: the cyclic-reduction levels in
+// the low byte; above it, one more than the parallel-cyclic-reduction levels
+// an open chain runs before its Thomas finish (0: none, PCR to the end).
+constexpr __host__ __device__ __forceinline__ int depth_code(int ncr, int tpcr) { return ncr | ((tpcr + 1) << 8); }
+constexpr __host__ __device__ __forceinline__ int cr_levels(int code) { return code & 0xff; }
+constexpr __host__ __device__ __forceinline__ int pcr_cap(int code) { return (code >> 8) - 1; }
+constexpr int kMaxThomasPcr = 4;  // at most 16 subsystems: one wave's quads hold them all
+
+// Thomas finish of an open chain after parallel cyclic reduction has run to
+// stride dd over the members at multiples of d: the rows r_j + k*dd
+// (r_j = j*d, j < dd/d) form dd/d independent block-tridiagonal systems of
+// normalised rows x_k + a_k x_(k-1) + c_k x_(k+1) = f_k, with a = 0 on the
+// first row of each and c = 0 on its last (their couplings fell past the
+// chain's ends).  Quad j (lane q) runs the block Thomas algorithm on system j:
+// lane q carries column q of gamma_k = D_k^-1 c_k, D_k = I - a_k gamma_(k-1),
+// and all of phi_k = D_k^-1 (f_k - a_k phi_(k-1)) (D_0 = I), written over
+// column q of c_k and f_k[q]; then x_k = phi_k - gamma_k x_(k+1), lane q
+// computing x_k[q] into the F slot.  The whole quad must call it.
+template <typename T>
+__device__ __forceinline__ void thomas_quad_open(T* lds, int S, int N, int d, int dd, int j, int q) {
+#pragma clang fp contract(fast)
+  const int r0 = j * d;
+  if (r0 >= N) return;
+  const int n = (N - 1 - r0) / dd + 1;
+  T G[4], P[4];
+#pragma unroll
+  for (int m = 0; m < 4; m++) {
+    G[m] = lds[(16 + 4 * m + q) * S + lpad(r0)];
+    P[m] = lds[(32 + m) * S + lpad(r0)];
+  }
+  for (int k = 1; k < n; k++) {
+    const int s = lpad(r0 + k * dd);
+    T a[16], cq[4], f[4], Dq[4];
+#pragma unroll
+    for (int e = 0; e < 16; e++) a[e] = lds[e * S + s];
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      cq[m] = lds[(16 + 4 * m + q) * S + s];
+      f[m] = lds[(32 + m) * S + s];
+      Dq[m] = (m == q) ? T(1) : T(0);
+    }
+#pragma unroll
+    for (int l = 0; l < 4; l++)
+#pragma unroll
+      for (int m = 0; m < 4; m++) {
+        Dq[m] -= a[4 * m + l] * G[l];  // D[:, q]
+        f[m] -= a[4 * m + l] * P[l];
+      }
+    T D[16];
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      D[4 * m + 0] = quad_get<0>(Dq[m]);
+      D[4 * m + 1] = quad_get<1>(Dq[m]);
+      D[4 * m + 2] = quad_get<2>(Dq[m]);
+      D[4 * m + 3] = quad_get<3>(Dq[m]);
+    }
+    const LU4<T> L = lu4_fast(D);
+    lu4_solve_fast(L, f, 1);
+    lu4_solve_fast(L, cq, 1);
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      lds[(16 + 4 * m + q) * S + s] = cq[m];
+      G[m] = cq[m];
+      P[m] = f[m];
+    }
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+      if (q == m) lds[(32 + m) * S + s] = f[m];
+  }
+  // the quad's lanes wrote the gamma columns the back sweep reads by rows: a
+  // wavefront's LDS operations complete in order; this keeps the compiler
+  // from moving them
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  T x[4] = {P[0], P[1], P[2], P[3]};  // x_(n-1) = phi_(n-1), already in its F slot
+  for (int k = n - 2; k >= 0; k--) {
+    const int s = lpad(r0 + k * dd);
+    T xq = lds[(32 + q) * S + s];
+#pragma unroll
+    for (int l = 0; l < 4; l++) xq -= lds[(16 + 4 * q + l) * S + s] * x[l];
+    lds[(32 + q) * S + s] = xq;
+    x[0] = quad_get<0>(xq);
+    x[1] = quad_get<1>(xq);
+    x[2] = quad_get<2>(xq);
+    x[3] = quad_get<3>(xq);
+  }
+}
+
+// The Thomas finish by the quads of the first tw waves of a chain's group of
+// threads (i: the thread's index in the group): system j on wave j % tw,
+// lanes 4*(j / tw) to 4*(j / tw) + 3, so that tw waves (on tw SIMDs) share the
+// sequential work.  Needs dd/d <= 16*tw.
+template <typename T>
+__device__ __forceinline__ void thomas_finish_open(T* lds, int S, int N, int d, int dd, int i, int tw) {
+  const int w = i >> 6, lane = i & 63;
+  if (w >= tw) return;
+  const int j = (lane >> 2) * tw + w;
+  if (j < dd / d) thomas_quad_open(lds, S, N, d, dd, j, lane & 3);
+}
+
 // the quad's results into row slot sO: column q of A' and C', and F'[q]
 template <typename T>
 __device__ __forceinline__ void pcr_quad_store(T* lds, int S, int sO, int q, const T* Ac, const T* Cc, const T* F) {
@@ -1454,8 +1554,8 @@ __device__ __forceinline__ void row_reduce_lds(T* A, T* C, T* F, const T* src, i
 // kernels: their 96-member PCR levels otherwise keep 1.5 waves of a chain
 // busy; the two-row long-chain kernel has no threads to spare).
 template <typename T, int FINAL, int SFIX = 0, bool QUAD = true>
-__device__ __forceinline__ void fast_solve_lds(T* lds, const int N, const int NL, const int ncr, const int i,
-                                               const int NT) {
+__device__ __forceinline__ void fast_solve_lds(T* lds, const int N, const int NL, const int ncr, const int tpcr,
+                                               const int i, const int NT, const int tw) {
 #pragma clang fp contract(fast)
   const bool periodic = FINAL != kFinalOpen;
   const int S = SFIX > 0 ? SFIX : lds_stride(N);
@@ -1494,9 +1594,12 @@ __device__ __forceinline__ void fast_solve_lds(T* lds, const int N, const int NL
   const int mrow = i * d;
   T a[16], c[16], f[4];
   int dd = d;
+  // open chains with a Thomas finish stop after tpcr levels
+  const int pcap = (FINAL == kFinalOpen && tpcr >= 0) ? tpcr : (1 << 30);
+  int npl = 0;
   if (QUAD && 4 * ((NL + d - 1) / d) <= NT) {
     const int g = i >> 2, q = i & 3, grow = g * d;
-    while (periodic ? (n % 2 == 0 && n >= 4) : (dd < NL)) {
+    while ((periodic ? (n % 2 == 0 && n >= 4) : (dd < NL)) && npl < pcap) {
       T Ac[4], Cc[4], F[4];
       if (g < nmem) {
         const int iL = periodic ? (grow - dd + N) % N : grow - dd;
@@ -1508,10 +1611,11 @@ __device__ __forceinline__ void fast_solve_lds(T* lds, const int N, const int NL
       __syncthreads();
       dd *= 2;
       n /= 2;
+      npl++;
     }
   }
   if (member) lds_load_row(lds, S, lpad(mrow), a, c, f);
-  while (periodic ? (n % 2 == 0 && n >= 4) : (dd < NL)) {
+  while ((periodic ? (n % 2 == 0 && n >= 4) : (dd < NL)) && npl < pcap) {
     if (member) {
       const int iL = periodic ? (mrow - dd + N) % N : mrow - dd;
       const int iR = periodic ? (mrow + dd) % N : mrow + dd;
@@ -1522,11 +1626,17 @@ __device__ __forceinline__ void fast_solve_lds(T* lds, const int N, const int NL
     __syncthreads();
     dd *= 2;
     n /= 2;
+    npl++;
   }
 
   // (3) the members' final systems; their solutions go to their F slots
   if (FINAL == kFinalOpen) {
-    // decoupled: x = f, already in the F slot
+    // decoupled (x = f, already in the F slot), or after tpcr levels the
+    // subsystems at stride dd, by the Thomas finish
+    if (dd < NL) {
+      thomas_finish_open(lds, S, N, d, dd, i, tw);
+      __syncthreads();
+    }
   } else if (FINAL == kFinalPair) {
     // rows i and j = i + N/2 couple only to each other:
     // x_i = (I - P Q)^-1 (f_i - P f_j), P = A^_i + C^_i, Q = A^_j + C^_j
@@ -1864,7 +1974,8 @@ __device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const 
   int d = 1;  // stride between the rows still in the system
   int n = N;  // cycle length of the remaining system (closed chains)
   int ncr_done = 0;
-  while (ncr_done < ncr && (periodic ? (n % 2 == 0 && n >= 4) : (d < N))) {
+  const int ncrl = cr_levels(ncr);
+  while (ncr_done < ncrl && (periodic ? (n % 2 == 0 && n >= 4) : (d < N))) {
     const int nsurv = (N + 2 * d - 1) / (2 * d);
     if (i < nsurv) {
       const int row = i * 2 * d;
@@ -1907,9 +2018,12 @@ __device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const 
   const int mrow = i * d;
   T a[16], c[16], f[4];
   int dd = d;
+  // open chains with a Thomas finish stop after pcr_cap(ncr) levels
+  const int pcap = (FINAL == kFinalOpen && pcr_cap(ncr) >= 0) ? pcr_cap(ncr) : (1 << 30);
+  int npl = 0;
   if ((FINAL != kFinalOdd || sizeof(T) == sizeof(float)) && 4 * nmem <= (int)blockDim.x) {
     const int g = i >> 2, q = i & 3, grow = g * d;
-    while (periodic ? (n % 2 == 0 && n >= 4) : (dd < N)) {
+    while ((periodic ? (n % 2 == 0 && n >= 4) : (dd < N)) && npl < pcap) {
       T Ac[4], Cc[4], F[4];
       if (g < nmem) {
         const int iL = periodic ? (grow - dd + N) % N : grow - dd;
@@ -1921,10 +2035,11 @@ __device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const 
       __syncthreads();
       dd *= 2;
       n /= 2;
+      npl++;
     }
   }
   if (member) lds_load_row(lds, S, lpad(mrow), a, c, f);
-  while (periodic ? (n % 2 == 0 && n >= 4) : (dd < N)) {
+  while ((periodic ? (n % 2 == 0 && n >= 4) : (dd < N)) && npl < pcap) {
     if (member) {
       const int iL = periodic ? (mrow - dd + N) % N : mrow - dd;
       const int iR = periodic ? (mrow + dd) % N : mrow + dd;
@@ -1957,11 +2072,17 @@ __device__ __forceinline__ void adi_fast_chain(ChainTab ct, const int ch, const 
     __syncthreads();
     dd *= 2;
     n /= 2;
+    npl++;
   }
 
   // (3) the members' final systems; their solutions go to their F slots
   if (FINAL == kFinalOpen) {
-    // decoupled: x = f, already in the F slot
+    // decoupled (x = f, already in the F slot), or after pcr_cap(ncr) levels
+    // the subsystems at stride dd, by the Thomas finish on up to four waves
+    if (dd < N) {
+      thomas_finish_open(lds, S, N, d, dd, i, max(1, min(4, (int)blockDim.x >> 6)));
+      __syncthreads();
+    }
   } else if (FINAL == kFinalPair) {
     // rows i and j = i + N/2 couple only to each other:
     // x_i = (I - P Q)^-1 (f_i - P f_j), P = A^_i + C^_i, Q = A^_j + C^_j
@@ -2154,7 +2275,7 @@ __device__ __forceinline__ void adi_fast_chain_long(ChainTab ct, const int ch, c
     lds_store_row(lds, SM, lpad(t), a, c, f);
   }
   __syncthreads();
-  fast_solve_lds<T, FINAL, 0, false>(lds, M, M, ncr - 1, t, (int)blockDim.x);
+  fast_solve_lds<T, FINAL, 0, false>(lds, M, M, cr_levels(ncr) - 1, -1, t, (int)blockDim.x, 1);
   if (2 * t + 1 < N) {
     // x = F^ - A^ x_2t - C^ x_2t+2, with the even rows' x in the F slots
     const int row = 2 * t + 1;
@@ -2190,7 +2311,7 @@ __device__ __forceinline__ void adi_fast_chain_long(ChainTab ct, const int ch, c
 // LDS stride SM (SM >= lds_stride(ceil(NL/2))).
 template <typename T, int FINAL, int SM, int XS, bool JQ>
 __device__ __forceinline__ void adi_fast_chain_half(ChainTab ct, const JacSrc& js, const int ch, const int NL, T* lds,
-                                                    const int t, const int NT,
+                                                    const int t, const int NT, const int tw,
                                                     const T* __restrict__ ff, int nProc, double cfl,
                                                     const double* __restrict__ sigma, double* __restrict__ r,
                                                     double* __restrict__ dq, int dqmode, int ncr,
@@ -2220,7 +2341,7 @@ __device__ __forceinline__ void adi_fast_chain_half(ChainTab ct, const JacSrc& j
     if (active && even) lds_store_row(lds, SM, lpad(t), a, c, f);
   }
   __syncthreads();
-  fast_solve_lds<T, FINAL, SM, true>(lds, M, (NL + 1) / 2, ncr - 1, t, NT);
+  fast_solve_lds<T, FINAL, SM, true>(lds, M, (NL + 1) / 2, cr_levels(ncr) - 1, pcr_cap(ncr), t, NT, tw);
   if (active) {
     T x[4];
     if (even) {
@@ -2369,7 +2490,8 @@ __global__ __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(WPE))) voi
     const int ch = chains[w];
     if (lv.chain && !lv.chain[ch]) continue;  // the same for the whole workgroup
     adi_fast_chain_half<T, FINAL, lds_stride((MAXT + 1) / 2), MAXT + 1, JQ>(ct, js, ch, ct.ncell[ch], lds, threadIdx.x,
-                                                                        (int)blockDim.x, ff, nProc, cfl, sigma, r,
+                                                                        (int)blockDim.x, min(4, (int)blockDim.x >> 6),
+                                                                        ff, nProc, cfl, sigma, r,
                                                                         dq, dqmode, ncr, hs, hstride, hbase);
     __syncthreads();
   }
@@ -2402,7 +2524,8 @@ __global__ __launch_bounds__(2 * MAXH) __attribute__((amdgpu_waves_per_eu(3))) v
     if (c1 >= 0 && lv.chain && !lv.chain[c1]) c1 = -1;
     const int NL = max(c0 >= 0 ? ct.ncell[c0] : 0, c1 >= 0 ? ct.ncell[c1] : 0);
     if (NL == 0) continue;  // the same for the whole workgroup
-    adi_fast_chain_half<T, kFinalOpen, SM, MAXH + 1, JQ>(ct, js, sub ? c1 : c0, NL, lds, t, MAXH, ff, nProc, cfl, sigma,
+    // the Thomas finish on two waves per chain: waves 0-1 and 6-7, four SIMDs
+    adi_fast_chain_half<T, kFinalOpen, SM, MAXH + 1, JQ>(ct, js, sub ? c1 : c0, NL, lds, t, MAXH, 2, ff, nProc, cfl, sigma,
                                                      r, dq, dqmode, ncr, hs, hstride, hbase);
     __syncthreads();
   }
@@ -2536,6 +2659,205 @@ __global__ __launch_bounds__(kBlock) void k_lin_cells(Consts c, Live lv, const i
     }
   }
   block_norm_partial(sq, mx, partial, vb);
+}
+
+// --------------------------------------------------------------------------
+// The end of one ADI iteration fused with the start of the next (a single
+// case, msweep = 1): computeLinearRHS of iteration n shares its face gather
+// with computeRHS("implicit") of iteration n + 1, which reads the state
+// q + dq that iteration n's update produces.  Each value is computed by the
+// operations of the kernels it replaces (k_lin_faces with the Jacobians
+// computed again, k_rhs_faces; k_lin_cells with the update, k_rhs_cells), so
+// the iterations are bit-identical to the unfused ones.
+// --------------------------------------------------------------------------
+
+// load_state of the updated state q + dq (the sum formed per raw cell, as
+// k_lin_cells<true> forms it, before any wall reflection)
+__device__ __forceinline__ double4 load_state_sum(const int2 e, const double* __restrict__ q,
+                                                  const double* __restrict__ dq,
+                                                  const double* __restrict__ refmtx,
+                                                  const double4* __restrict__ qinf) {
+  if (e.x < 0) return qinf[e.y];
+  const double4 a = *reinterpret_cast<const double4*>(q + 4 * (size_t)e.x);
+  const double4 b = *reinterpret_cast<const double4*>(dq + 4 * (size_t)e.x);
+  double4 v = make_double4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+  if (e.y >= 0) {
+    const double4 R = *reinterpret_cast<const double4*>(refmtx + 4 * (size_t)e.y);
+    const double u = v.y, w = v.z;
+    v.y = (u * R.x + w * R.y);
+    v.z = (u * R.z + w * R.w);
+  }
+  return v;
+}
+
+// face_states of the updated state q + dq
+template <int ORDER>
+__device__ __forceinline__ void face_states_sum(const FaceTab& ft, const double* __restrict__ q,
+                                                const double* __restrict__ dq, double eps, double qt, int code,
+                                                double* left, double* right) {
+  const int s = code & kSlotMask;
+  if (code & kOrphanSlot) {
+    put4(left, load_state_sum(ft.src[s], q, dq, ft.refmtx, ft.qinf));
+    put4(right, load_state_sum(ft.src[s + 1], q, dq, ft.refmtx, ft.qinf));
+  } else {
+    const bool flip = (code >> 30) & 1;
+    constexpr int lo = (ORDER == 5) ? 0 : (ORDER == 3 ? 1 : 2);
+    constexpr int hi = (ORDER == 5) ? 5 : (ORDER == 3 ? 4 : 3);
+    double f[6][4];
+#pragma unroll
+    for (int j = lo; j <= hi; j++) put4(f[j], load_state_sum(ft.src[s - 2 + j], q, dq, ft.refmtx, ft.qinf));
+    double qln[4], qrn1[4];
+    reconstruct_face<ORDER>(f, eps, qt, qln, qrn1);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      left[k] = flip ? qln[k] : qrn1[k];
+      right[k] = flip ? qrn1[k] : qln[k];
+    }
+  }
+}
+
+// k_lin_faces<ORDER, TL, FASTJ, true> on (q, dq), then k_rhs_faces<ORDER, TL,
+// true, FASTJ> on q + dq (ff_from, fst as there)
+template <int ORDER, typename TL, bool FASTJ>
+__global__ __launch_bounds__(kBlock) void k_lin_rhs_faces(FaceTab ft, Consts c, const double* __restrict__ q,
+                                                          const double* __restrict__ dq,
+                                                          double* __restrict__ lflux, double* __restrict__ flux,
+                                                          double* __restrict__ sr, TL* __restrict__ ff, int ff_from,
+                                                          double* __restrict__ fst, int* __restrict__ nanflag) {
+  const int p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (p >= c.nProc) return;
+  const int code = ft.pslot[p];
+  const int s = code & kSlotMask;
+  const double2 n = *reinterpret_cast<const double2*>(ft.nvec + 2 * (size_t)p);
+  {
+    double dl[4], dr[4], left[4], right[4];
+    if (code & kOrphanSlot) {
+      put4(dl, load_dstate(ft.src[s], dq, ft.refmtx));
+      put4(dr, load_dstate(ft.src[s + 1], dq, ft.refmtx));
+      put4(left, load_state(ft.src[s], q, ft.refmtx, ft.qinf));
+      put4(right, load_state(ft.src[s + 1], q, ft.refmtx, ft.qinf));
+    } else {
+      const bool flip = (code >> 30) & 1;
+      constexpr int lo = (ORDER == 5) ? 0 : (ORDER == 3 ? 1 : 2);
+      constexpr int hi = (ORDER == 5) ? 5 : (ORDER == 3 ? 4 : 3);
+      double f[6][4], d[6][4];
+#pragma unroll
+      for (int j = lo; j <= hi; j++) {
+        const int2 e = ft.src[s - 2 + j];
+        put4(f[j], load_state(e, q, ft.refmtx, ft.qinf));
+        put4(d[j], load_dstate(e, dq, ft.refmtx));
+      }
+      double dln[4], drn1[4];
+      reconstruct_dface<ORDER, FASTJ>(f, d, c.eps, c.qt, dln, drn1);
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        dl[k] = flip ? dln[k] : drn1[k];
+        dr[k] = flip ? drn1[k] : dln[k];
+      }
+      // opaque copies, as in k_lin_faces
+      double fb[6][4], qln[4], qrn1[4];
+#pragma unroll
+      for (int j = lo; j <= hi; j++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          fb[j][k] = f[j][k];
+          asm volatile("" : "+v"(fb[j][k]));
+        }
+      reconstruct_face<ORDER>(fb, c.eps, c.qt, qln, qrn1);
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        left[k] = flip ? qln[k] : qrn1[k];
+        right[k] = flip ? qrn1[k] : qln[k];
+      }
+    }
+    TL lmat[16], rmat[16];
+    face_jacobians<TL, FASTJ>(n.x, n.y, left, right, c.gamma, lmat, rmat);
+    double out[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      double acc = 0;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const double lm = double(lmat[4 * j + k]);
+        const double rm = double(rmat[4 * j + k]);
+        acc += ((lm * dl[k]) + (rm * dr[k]));
+      }
+      out[j] = acc;
+    }
+    *reinterpret_cast<double4*>(lflux + 4 * (size_t)p) = make_double4(out[0], out[1], out[2], out[3]);
+  }
+  double left[4], right[4];
+  face_states_sum<ORDER>(ft, q, dq, c.eps, c.qt, code, left, right);
+  double fl[4], s_r;
+  flux_roe2d(n.x, n.y, left, right, c.gamma, fl, &s_r);
+  *reinterpret_cast<double4*>(flux + 4 * (size_t)p) = make_double4(fl[0], fl[1], fl[2], fl[3]);
+  sr[p] = s_r;
+  if (s_r != s_r) atomicOr(nanflag, 1);
+  const bool store_jac = p >= ff_from || ((code >> 30) & 1);
+  if (fst && !store_jac) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      fst[(size_t)k * c.nProc + p] = left[k];
+      fst[(size_t)(4 + k) * c.nProc + p] = right[k];
+    }
+  }
+  if (store_jac) {
+    TL lm[16], rm[16];
+    face_jacobians<TL, FASTJ>(n.x, n.y, left, right, c.gamma, lm, rm);
+#pragma unroll
+    for (int e = 0; e < 16; e++) {
+      ff[(size_t)e * c.nProc + p] = lm[e];
+      ff[(size_t)(16 + e) * c.nProc + p] = rm[e];
+    }
+  }
+}
+
+// k_lin_cells<true> over every cell (its block partials to p_lin; the linear
+// residual itself is not kept: the next computeRHS overwrites r), then
+// k_rhs_cells<true> over the cells of the first nb_rhs blocks (block partials
+// to p_nl)
+__global__ __launch_bounds__(kBlock) void k_lin_rhs_cells(Consts c, const int* __restrict__ off,
+                                                          const int* __restrict__ ent,
+                                                          const double* __restrict__ lflux,
+                                                          const double* __restrict__ flux,
+                                                          const double* __restrict__ sr, double* __restrict__ r0,
+                                                          const double* __restrict__ dq, double* __restrict__ sigma,
+                                                          double* __restrict__ r, double* __restrict__ q,
+                                                          double* __restrict__ p_lin, double* __restrict__ p_nl,
+                                                          int nb_rhs) {
+  const int vb = xcd_block(blockIdx.x, gridDim.x);
+  const int i = vb * blockDim.x + threadIdx.x;
+  const bool in = i < c.nCell;
+  double sq = 0.0, mx = 0.0;
+  if (in) {
+    const double4 a = *reinterpret_cast<const double4*>(r0 + 4 * (size_t)i);
+    const double4 b = *reinterpret_cast<const double4*>(dq + 4 * (size_t)i);
+    double a0 = (a.x - b.x), a1 = (a.y - b.y), a2 = (a.z - b.z), a3 = (a.w - b.w);
+    const double dtfac = c.cfl / sigma[i];
+    cell_lin_sum(ent, off[i], off[i + 1], lflux, dtfac, a0, a1, a2, a3);
+    sq = a0 * a0;
+    mx = fabs(a0);
+    double4 qv = *reinterpret_cast<double4*>(q + 4 * (size_t)i);
+    qv.x += b.x; qv.y += b.y; qv.z += b.z; qv.w += b.w;
+    *reinterpret_cast<double4*>(q + 4 * (size_t)i) = qv;
+  }
+  block_norm_partial(sq, mx, p_lin, vb);
+  if (vb >= nb_rhs) return;  // the same for the whole block
+  sq = 0.0;
+  mx = 0.0;
+  if (in) {
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, sg = 0.0;
+    cell_flux_sum(ent, off[i], off[i + 1], flux, sr, a0, a1, a2, a3, sg);
+    sq = a0 * a0;
+    mx = fabs(a0);
+    sigma[i] = sg;
+    const double dtfac = c.cfl / sg;
+    const double4 v = make_double4(a0 * dtfac, a1 * dtfac, a2 * dtfac, a3 * dtfac);
+    *reinterpret_cast<double4*>(r + 4 * (size_t)i) = v;
+    *reinterpret_cast<double4*>(r0 + 4 * (size_t)i) = v;
+  }
+  __syncthreads();  // block_norm_partial's LDS is reused
+  block_norm_partial(sq, mx, p_nl, vb);
 }
 
 __global__ void k_update(int n4, Live lv, double* __restrict__ q, const double* __restrict__ dq) {
@@ -2885,6 +3207,7 @@ struct Group {
   int half_ok = 0;     // FAST, FP64: every chain fits adi_fast_chain_half (scratch rows reserved)
   int half2_ok = 0;    // FAST, open chains that fit k_adi_fast_half2 (FP64 or FP32; scratch rows reserved)
   int kern = 0;        // FAST one-row kernel: kKernRow, kKernHalf or (open chains) kKernHalf2
+  int tpcr = -1;       // FAST open chains (not kKernTwo): PCR levels before the Thomas finish (-1: none; autotuned)
 };
 
 struct hamstr_ctx {
@@ -3136,7 +3459,8 @@ int launch_group(hamstr_ctx* x, const Group& g, int grid, int dqmode, int ncr, i
     // them), and so does the half-LDS kernel
     const bool half = g.kern == kKernHalf || g.kern == kKernHalf2;
     if (extra > 0 && (half || g.two)) return fail(HAMSTR_ERR_ARG, "cell work rides in the one-row kernel only");
-    const int nc = (g.two || half) ? std::max(ncr, 1) : ncr;
+    const int nc = depth_code((g.two || half) ? std::max(ncr, 1) : ncr,
+                              (g.final == kFinalOpen && !g.two) ? g.tpcr : -1);
     const size_t lds = (x->lhs32() ? sizeof(float) : sizeof(double)) * 36 *
                        (half ? (g.kern == kKernHalf2 ? 2 * (size_t)lds_stride((kHalfMaxRows + 1) / 2)
                                                      : (size_t)lds_stride((half_maxt(g.block) + 1) / 2))
@@ -4197,7 +4521,8 @@ void drop_graph(hamstr_ctx* x) {
 }
 
 int enqueue_adi_iteration(hamstr_ctx* x, double* nb, const std::function<int(const char*)>& mark, double* p_nl,
-                          double* p_lin);
+                          double* p_lin, bool fuse_in, double* p_nl_next);
+bool can_fuse(const hamstr_ctx* x);
 
 // stepSolution's euler and rk3 branches (march.cpp:73-108): per stage one
 // face launch and one cell launch that also applies updateSolution.  The
@@ -4236,14 +4561,18 @@ int enqueue_explicit(hamstr_ctx* x, const std::function<int(const char*)>& mark,
 // the norms' block partials go to iteration slot `slot` of x->npart (reduced
 // per run of iterations by enqueue_block); batches: per-case norms of both
 // stages to nb[nCase][4] at the end of the iteration.
-int enqueue_iteration(hamstr_ctx* x, double* nb, const std::function<int(const char*)>& mark, int slot) {
+// fuse_in / fuse_out: the iteration's computeRHS is fused into the previous
+// iteration's tail / its tail into the next iteration's computeRHS (can_fuse)
+int enqueue_iteration(hamstr_ctx* x, double* nb, const std::function<int(const char*)>& mark, int slot,
+                      bool fuse_in = false, bool fuse_out = false) {
   int rc;
   const bool one = x->nCase == 1;
   double* p_nl = one ? x->npart + (size_t)(2 * slot) * x->nPart * 2 : x->partial;
   double* p_lin = one ? x->npart + (size_t)(2 * slot + 1) * x->nPart * 2 : x->partial;
+  double* p_next = (one && fuse_out) ? x->npart + (size_t)(2 * (slot + 1)) * x->nPart * 2 : nullptr;
   if (x->scheme != HAMSTR_SCHEME_ADI) {
     if ((rc = enqueue_explicit(x, mark, one ? p_nl : nullptr, one ? p_lin : nullptr))) return rc;
-  } else if ((rc = enqueue_adi_iteration(x, nb, mark, p_nl, p_lin))) {
+  } else if ((rc = enqueue_adi_iteration(x, nb, mark, p_nl, p_lin, fuse_in, p_next))) {
     return rc;
   }
   if (!one) {
@@ -4259,16 +4588,20 @@ int enqueue_iteration(hamstr_ctx* x, double* nb, const std::function<int(const c
 // computeRHS("implicit") and the ADI phases of the first sweep, the chain
 // faces' Jacobians taken from source rj (HAMSTR_JAC_*): the part of the
 // fused iteration the Jacobian source changes (the autotuner times it)
+// rhs_done: the fused tail of the previous iteration (launch_lin_rhs) has
+// done computeRHS already, except the residual sums phase 0's launch carries
 int enqueue_rhs_sweep0(hamstr_ctx* x, double* nb, const std::function<int(const char*)>& mark, double* p_nl,
-                       int rj) {
+                       int rj, bool rhs_done = false) {
   int rc;
   // FAST: the ADI kernels may compute the chain faces' Jacobians again, from
   // the face states k_rhs_faces stores instead (64 B per face, FP64 256 B)
   // or from q; k_rhs_faces then stores the orphan faces' Jacobians only
-  if ((rc = launch_rhs_faces(x, true, rj != HAMSTR_JAC_STORED ? x->nChainProc : 0,
-                             rj == HAMSTR_JAC_FROM_STATES ? x->fst : nullptr)))
-    return rc;
-  if (mark && mark("rhs_faces")) return HAMSTR_ERR_HIP;
+  if (!rhs_done) {
+    if ((rc = launch_rhs_faces(x, true, rj != HAMSTR_JAC_STORED ? x->nChainProc : 0,
+                               rj == HAMSTR_JAC_FROM_STATES ? x->fst : nullptr)))
+      return rc;
+    if (mark && mark("rhs_faces")) return HAMSTR_ERR_HIP;
+  }
   // phase 0 over the leading cells on part of the device: the other cells'
   // residual sums ride in phase 0's launch, on the CUs its chains leave idle
   // (phase 0's rows read only their own cells' r and sigma; the block
@@ -4276,24 +4609,73 @@ int enqueue_rhs_sweep0(hamstr_ctx* x, double* nb, const std::function<int(const 
   const int extra = p_nl ? cells_overlap_extra(x) : 0;
   CellWork cw{};
   if (extra > 0) {
-    if ((rc = launch_rhs_cells_range(x, true, p_nl, 0, x->split_cells))) return rc;
+    if (!rhs_done && (rc = launch_rhs_cells_range(x, true, p_nl, 0, x->split_cells))) return rc;
     cw = CellWork{x->cell_off, x->cell_ent, x->flux, x->sr, x->r0, p_nl, x->split_cells, x->nCell, 0};
-  } else if ((rc = launch_rhs_cells(x, true, nb, p_nl))) {
+  } else if (!rhs_done && (rc = launch_rhs_cells(x, true, nb, p_nl))) {
     return rc;
   }
-  if (mark && mark("rhs_cells")) return HAMSTR_ERR_HIP;
+  if (!rhs_done && mark && mark("rhs_cells")) return HAMSTR_ERR_HIP;
   return launch_sweep_phases(x, 0, mark, rj, &cw, extra);
 }
 
+// Iterations of a captured block run fused across their boundary (launch_lin_rhs)
+// for a single live case with one sweep: every value the same, bit for bit.
+bool can_fuse(const hamstr_ctx* x) {
+  return x->nCase == 1 && x->prm.msweep == 1 && x->scheme == HAMSTR_SCHEME_ADI && !x->retire && !x->any_retired &&
+         x->npart != nullptr && x->rrho == nullptr;
+}
+
+// computeLinearRHS of this iteration with the update, fused with computeRHS
+// ("implicit") of the next: block partials of this iteration's linear norm to
+// p_lin, of the next one's residual norm to p_nl_next (for the cells phase 0's
+// launch does not sum)
+int launch_lin_rhs(hamstr_ctx* x, int rj, double* p_lin, double* p_nl_next,
+                   const std::function<int(const char*)>& mark) {
+  const FaceTab ft = x->ftab();
+  const int ff_from = rj != HAMSTR_JAC_STORED ? x->nChainProc : 0;
+  double* fst = rj == HAMSTR_JAC_FROM_STATES ? x->fst : nullptr;
+  const dim3 g(nblk(x->nProc)), b(kBlock);
+#define LRF(O, T, F)                                                                                         \
+  hipLaunchKernelGGL((k_lin_rhs_faces<O, T, F>), g, b, 0, x->stream, ft, x->c, x->q, x->dq, x->lflux, x->flux, \
+                     x->sr, (T*)x->ff, ff_from, fst, x->nanflag)
+#define LRO(T, F) \
+  if (o == 1) { LRF(1, T, F); } else if (o == 3) { LRF(3, T, F); } else { LRF(5, T, F); }
+  const int o = x->prm.order;
+  const bool fj = x->prm.adi_mode == HAMSTR_ADI_FAST;
+  if (x->lhs32()) {
+    if (fj) { LRO(float, true); } else { LRO(float, false); }
+  } else {
+    if (fj) { LRO(double, true); } else { LRO(double, false); }
+  }
+#undef LRO
+#undef LRF
+  HIPCHK(hipGetLastError());
+  if (mark && mark("lin_rhs_faces")) return HAMSTR_ERR_HIP;
+  const int nb_rhs = cells_overlap_extra(x) > 0 ? x->split_cells / kBlock : nblk(x->nCell);
+  hipLaunchKernelGGL(k_lin_rhs_cells, dim3(nblk(x->nCell)), dim3(kBlock), 0, x->stream, x->c, x->cell_off,
+                     x->cell_ent, x->lflux, x->flux, x->sr, x->r0, x->dq, x->sigma, x->r, x->q, p_lin, p_nl_next,
+                     nb_rhs);
+  HIPCHK(hipGetLastError());
+  if (mark && mark("lin_rhs_cells")) return HAMSTR_ERR_HIP;
+  return HAMSTR_OK;
+}
+
+// fuse_in: this iteration's computeRHS was fused into the previous one's tail;
+// p_nl_next non-null: fuse this iteration's tail with the next's computeRHS
+// (can_fuse contexts only)
 int enqueue_adi_iteration(hamstr_ctx* x, double* nb, const std::function<int(const char*)>& mark, double* p_nl,
-                          double* p_lin) {
+                          double* p_lin, bool fuse_in, double* p_nl_next) {
   int rc;
   const int rj = jac_source(x);
   for (int sw = 0; sw < x->prm.msweep; sw++) {
     if (sw == 0) {
-      if ((rc = enqueue_rhs_sweep0(x, nb, mark, p_nl, rj))) return rc;
+      if ((rc = enqueue_rhs_sweep0(x, nb, mark, p_nl, rj, fuse_in))) return rc;
     } else if ((rc = launch_sweep_phases(x, sw, mark, rj))) {
       return rc;
+    }
+    if (p_nl_next && sw == x->prm.msweep - 1) {
+      if ((rc = launch_lin_rhs(x, rj, p_lin, p_nl_next, mark))) return rc;
+      continue;
     }
     // the Jacobians computed again beat reading them back (FP64: 26.9 ->
     // 24.1 us on the 512x256 mixed mesh; FP32: the same there, 1.14 -> 0.84
@@ -4309,8 +4691,9 @@ int enqueue_adi_iteration(hamstr_ctx* x, double* nb, const std::function<int(con
 // `count` iterations whose norms go to base[count][nCase][4]
 int enqueue_block(hamstr_ctx* x, int count, double* base, const std::function<int(const char*)>& mark) {
   const int nc = x->nCase;
+  const bool fuse = count > 1 && can_fuse(x);
   for (int s = 0; s < count; s++) {
-    const int rc = enqueue_iteration(x, base + 4 * (size_t)nc * s, mark, s);
+    const int rc = enqueue_iteration(x, base + 4 * (size_t)nc * s, mark, s, fuse && s > 0, fuse && s + 1 < count);
     if (rc) return rc;
   }
   if (nc == 1) {
@@ -4553,6 +4936,23 @@ int hamstr_set_launch(hamstr_ctx* x, int group, int grid, int cr_levels) {
   return HAMSTR_OK;
 }
 
+int hamstr_set_thomas(hamstr_ctx* x, int group, int pcr_levels) {
+  if (!x || group < 0 || group >= (int)x->groups.size()) return fail(HAMSTR_ERR_ARG, "bad launch group");
+  Group& g = x->groups[group];
+  if (!g.fast || g.two || g.final != kFinalOpen)
+    return fail(HAMSTR_ERR_ARG, "a Thomas finish needs a FAST group of open chains that fit a workgroup");
+  if (pcr_levels < -1 || pcr_levels > kMaxThomasPcr) return fail(HAMSTR_ERR_ARG, "bad PCR levels");
+  g.tpcr = pcr_levels;
+  drop_graph(x);
+  return HAMSTR_OK;
+}
+
+int hamstr_thomas(const hamstr_ctx* x, int group, int* pcr_levels) {
+  if (!x || !pcr_levels || group < 0 || group >= (int)x->groups.size()) return fail(HAMSTR_ERR_ARG, "bad launch group");
+  *pcr_levels = x->groups[group].tpcr;
+  return HAMSTR_OK;
+}
+
 // the face-state array of HAMSTR_JAC_FROM_STATES, allocated on first use
 int ensure_face_states(hamstr_ctx* x) {
   if (!x->fst) HIPCHK(hipMalloc(&x->fst, sizeof(double) * 8 * (size_t)x->nProc));
@@ -4781,6 +5181,32 @@ int hamstr_autotune(hamstr_ctx* x, int reps, double* best_ms, int cap, int* ngro
         g.kern = best_kern;
         g.grid = best_grid;
         g.ncr = best_ncr;
+        if (g.final == kFinalOpen && !g.two) {
+          // open chains: the PCR levels before the Thomas finish, at the best
+          // kernel, grid and depth, then the depths again at the best level
+          const int t0 = g.tpcr;
+          int best_t = t0;
+          auto ttrial = [&](int tp, int ncr) -> int {
+            double t = 0;
+            g.tpcr = tp;
+            const int rc = timed([&]() { return launch_group(x, g, best_grid, 0, ncr, rj); }, &t);
+            if (rc) return rc;
+            if (t < best) {
+              best = t;
+              best_t = tp;
+              best_ncr = ncr;
+            }
+            return HAMSTR_OK;
+          };
+          int rc;
+          for (int tp : {-1, 2, 3, 4})
+            if (tp != t0 && tp <= kMaxThomasPcr && (rc = ttrial(tp, best_ncr))) return rc;
+          const int n1 = best_ncr;
+          for (int ncr = g.kern == kKernRow ? 0 : 1; ncr <= kMaxCR; ncr++)
+            if (ncr != n1 && (rc = ttrial(best_t, ncr))) return rc;
+          g.tpcr = best_t;
+          g.ncr = best_ncr;
+        }
         per[k] = best;
         *sum += best;
       }

@@ -45,7 +45,7 @@ EXPORTS = [
     "hamstr_update_solution", "hamstr_set_scheme", "hamstr_get_scheme",
     "hamstr_write_solution", "hamstr_read_solution", "hamstr_block_tridag4_noalloc",
     "hamstr_flux_roe3d", "hamstr_jacobian_roe3d", "hamstr_block_tridag5", "hamstr_block_tridag_periodic5",
-    "hamstr_set_jacobian_source", "hamstr_jacobian_source",
+    "hamstr_set_jacobian_source", "hamstr_jacobian_source", "hamstr_set_thomas", "hamstr_thomas",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -475,6 +475,19 @@ def set_launch(solver: Solver, group: int, grid: int, cr_levels: int):
     _chk(lib().hamstr_set_launch(solver.h, int(group), int(grid), int(cr_levels)))
 
 
+def set_thomas(solver: Solver, group: int, pcr_levels: int):
+    """PCR levels (0..4) before the block-Thomas finish of a FAST launch group
+    of open chains; -1: parallel cyclic reduction to the end."""
+    _chk(lib().hamstr_set_thomas(solver.h, int(group), int(pcr_levels)))
+
+
+def thomas_levels(solver: Solver, group: int) -> int:
+    """The group's PCR levels before the Thomas finish (-1: none)."""
+    v = C.c_int()
+    _chk(lib().hamstr_thomas(solver.h, int(group), C.byref(v)))
+    return v.value
+
+
 def set_adi_kernel(solver: Solver, group: int, kernel: int):
     """ADI kernel of a FAST launch group: KERNEL_ROW, or KERNEL_HALF /
     KERNEL_HALF2 (open chains) where the group's chains fit them."""
@@ -499,12 +512,16 @@ def jacobian_source(solver: Solver) -> int:
 
 def launch_groups(solver: Solver):
     """Launch groups as dicts: phase, kernel (KERNEL_*; "fast" is true for the
-    FAST kernels), final solve, chains, max_rows, grid, block, cr_levels."""
+    FAST kernels), final solve, chains, max_rows, grid, block, cr_levels, and
+    thomas (PCR levels before the Thomas finish of open chains, -1: none)."""
     out = np.zeros(8 * 64, np.int32)
     n = C.c_int()
     _chk(lib().hamstr_launch_groups(solver.h, _i(out), 64, C.byref(n)))
     keys = ("phase", "kernel", "final", "chains", "max_rows", "grid", "block", "cr_levels")
     groups = [dict(zip(keys, map(int, out[8 * k:8 * k + 8]))) for k in range(n.value)]
-    for g in groups:
+    for k, g in enumerate(groups):
         g["fast"] = int(g["kernel"] != KERNEL_EXACT)
+        v = C.c_int()
+        _chk(lib().hamstr_thomas(solver.h, k, C.byref(v)))
+        g["thomas"] = v.value
     return groups

@@ -181,6 +181,17 @@ int hamstr_launch_groups(const hamstr_ctx *ctx, int *out8, int cap, int *n);
 /* Sets FAST launch group `group`'s persistent grid and its cyclic-reduction
  * levels (0..3) by hand, as the autotuner would. */
 int hamstr_set_launch(hamstr_ctx *ctx, int group, int grid, int cr_levels);
+/* FAST launch groups of open chains (not HAMSTR_KERNEL_TWO): the parallel
+ * cyclic reduction after the cyclic-reduction levels stops after pcr_levels
+ * levels (0..4), and the block Thomas algorithm finishes the 2^pcr_levels
+ * interleaved subsystems left, one quad of lanes per subsystem (-1, the
+ * default: reduce until every row is decoupled).  The solution equals the
+ * other orderings up to rounding; every kernel gives the same bits for the
+ * same setting.  hamstr_autotune tries -1, 2, 3 and 4.  No reference
+ * counterpart: the reference solves each chain with one sequential block
+ * Thomas sweep (triSolvers.cpp:769-847). */
+int hamstr_set_thomas(hamstr_ctx *ctx, int group, int pcr_levels);
+int hamstr_thomas(const hamstr_ctx *ctx, int group, int *pcr_levels);
 /* Chooses a FAST launch group's kernel by hand: HAMSTR_KERNEL_ROW, or
  * HAMSTR_KERNEL_HALF where the group's chains fit it (FP64 left-hand side, at
  * most 384 rows, open chains or closed chains whose final solve is two-row

@@ -120,13 +120,15 @@ def test_batch_and_autotuner(hip):
     tuned.computeRHS("implicit")
     hip.autotune(tuned)
     print(f"autotuner chose Jacobian source {hip.jacobian_source(tuned)}", flush=True)
-    # the tuned kernels, grids and depths on the other two (the depth changes
+    # the tuned kernels, grids and depths on the other three (the depths change
     # the rounding of the block solves)
     for k, g in enumerate(hip.launch_groups(tuned)):
         for b in (ref, on, onq):
             if g["kernel"] in (hip.KERNEL_ROW, hip.KERNEL_HALF, hip.KERNEL_HALF2):
                 hip.set_adi_kernel(b, k, g["kernel"])
             hip.set_launch(b, k, g["grid"], g["cr_levels"])
+            if g["thomas"] >= 0:
+                hip.set_thomas(b, k, g["thomas"])
     hip.set_jacobian_source(on, hip.JAC_FROM_STATES)
     hip.set_jacobian_source(onq, hip.JAC_FROM_Q)
     for b in (ref, on, onq, tuned):
