@@ -45,14 +45,17 @@ def _pair(hip, mesh, order, lhs, mode, jsrc, steps, mach=0.5, alpha=2.0):
     return out
 
 
+@pytest.mark.parametrize("nc,nr", [(64, 24), (128, 40)])
 @pytest.mark.parametrize("order", [1, 3, 5])
 @pytest.mark.parametrize("lhs,mode", [("fp64", "fast"), ("fp32", "fast"), ("fp64", "exact")])
-def test_fused_block_equals_single_iterations(hip, order, lhs, mode):
+def test_fused_block_equals_single_iterations(hip, order, lhs, mode, nc, nr):
     """O-grid with triangulated outer layers (orphan faces; phase 0's launch
-    carries the other cells' residual sums), 24 iterations in one block."""
+    carries the other cells' residual sums), 24 iterations in one block; the
+    loops' cells fill 4 blocks of the cell kernel (all blocks then sum the
+    residuals too) or 16 (a multiple of 8: those blocks lead the launch)."""
     from hamstr_oop_amd import meshgen
-    mesh = meshgen.naca_ogrid(64, 24, tri_layers=8)
-    print(f"fused order={order} {lhs} {mode}", flush=True)
+    mesh = meshgen.naca_ogrid(nc, nr, tri_layers=8)
+    print(f"fused {nc}x{nr} order={order} {lhs} {mode}", flush=True)
     lm = hip.LHS_FP64 if lhs == "fp64" else hip.LHS_FP32
     am = hip.ADI_FAST if mode == "fast" else hip.ADI_EXACT
     (qa, sa, ra, l2a, lla, xa), (qb, sb, rb, l2b, llb, xb) = _pair(hip, mesh, order, lm, am, None, 24)
