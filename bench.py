@@ -86,19 +86,24 @@ def cpu_model():
     return platform.processor() or platform.machine()
 
 
-def measured_traffic(stage, nc, nr, order, lhs, adi, tri=0):
+JAC_NAMES = ("stored", "q", "states")   # HAMSTR_JAC_STORED, _FROM_Q, _FROM_STATES
+
+
+def measured_traffic(stage, nc, nr, order, lhs, adi, tri=0, jsrc=0):
     """HBM bytes per launch of the stage's kernel from the committed rocprofv3
-    counter summary of this workload (FETCH_SIZE and WRITE_SIZE scaled by a
-    plain copy of known size), or None for another workload."""
+    counter summary of this workload and Jacobian source (FETCH_SIZE and
+    WRITE_SIZE scaled by a plain copy of known size), or None for another
+    workload or source."""
     tag = f"_tri{tri}" if tri else ""
-    path = os.path.join(ROOT, "profiles", f"traffic_{lhs}_naca{nc}x{nr}{tag}_o{order}.json")
+    src = JAC_NAMES[jsrc]
+    path = os.path.join(ROOT, "profiles", f"traffic_{lhs}_naca{nc}x{nr}{tag}_o{order}_{src}.json")
     if not os.path.exists(path):
         return None
     with open(path) as fh:
         t = json.load(fh)
     w = t.get("workload", {})
-    if (w.get("nc"), w.get("nr"), w.get("order"), w.get("lhs"), w.get("adi"), w.get("tri_layers", 0)) != \
-            (nc, nr, order, lhs, adi, tri):
+    if (w.get("nc"), w.get("nr"), w.get("order"), w.get("lhs"), w.get("adi"), w.get("tri_layers", 0),
+            w.get("jacobian_source", "stored")) != (nc, nr, order, lhs, adi, tri, src):
         return None
     st = t["stages"].get(stage)
     return None if st is None else st["traffic_bytes"]
@@ -290,7 +295,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(dom, args.nc, args.nr, args.order, "fp64", args.adi,
-                                                 args.tri_layers),
+                                                 args.tri_layers, tune_s[1]),
                      "algorithmic_bytes_per_launch": dom_bytes, "avg_launch_us": stages[dom] * 1e3},
         "iteration_roofline": {"bytes_per_cell_iter": total_b,
                                "achieved_gbs": value * total_b / 1e9,
