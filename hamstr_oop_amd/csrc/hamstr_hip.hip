@@ -2512,7 +2512,9 @@ __global__ __launch_bounds__(2 * MAXH) __attribute__((amdgpu_waves_per_eu(3))) v
   constexpr int SM = lds_stride((MAXH + 1) / 2);
   static_assert(16 * (MAXH + 1) <= 36 * SM, "face exchange fits the slice");
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  const int sub = threadIdx.x >= MAXH ? 1 : 0;
+  // which chain of the pair: the same for a whole wavefront (MAXH is a multiple
+  // of 64), so kept in scalar registers, as is everything derived from it
+  const int sub = __builtin_amdgcn_readfirstlane(threadIdx.x >= MAXH ? 1 : 0);
   T* lds = reinterpret_cast<T*>(smem_raw) + sub * 36 * SM;
   const int t = threadIdx.x - sub * MAXH;
   const int npair = (nch + 1) / 2;
@@ -2525,7 +2527,8 @@ __global__ __launch_bounds__(2 * MAXH) __attribute__((amdgpu_waves_per_eu(3))) v
     const int NL = max(c0 >= 0 ? ct.ncell[c0] : 0, c1 >= 0 ? ct.ncell[c1] : 0);
     if (NL == 0) continue;  // the same for the whole workgroup
     // the Thomas finish on two waves per chain: waves 0-1 and 6-7, four SIMDs
-    adi_fast_chain_half<T, kFinalOpen, SM, MAXH + 1, JQ>(ct, js, sub ? c1 : c0, NL, lds, t, MAXH, 2, ff, nProc, cfl, sigma,
+    adi_fast_chain_half<T, kFinalOpen, SM, MAXH + 1, JQ>(ct, js, __builtin_amdgcn_readfirstlane(sub ? c1 : c0), NL, lds,
+                                                     t, MAXH, 2, ff, nProc, cfl, sigma,
                                                      r, dq, dqmode, ncr, hs, hstride, hbase);
     __syncthreads();
   }
