@@ -1241,9 +1241,13 @@ __device__ __forceinline__ void pcr_quad_row(const T* lds, int S, int sO, bool h
   sR = hasR ? sR : 0;
   T A[16], C[16];
   lds_load_row(lds, S, sO, A, C, F);
+  // the identity column from an opaque copy of q: hoisted out of the levels'
+  // loop, the four selects would hold 8 VGPRs for the whole kernel
+  int qo = q;
+  asm volatile("" : "+v"(qo));
   T m[4];
 #pragma unroll
-  for (int k = 0; k < 4; k++) m[k] = (k == q) ? T(1) : T(0);
+  for (int k = 0; k < 4; k++) m[k] = (k == qo) ? T(1) : T(0);
 #pragma unroll
   for (int l = 0; l < 4; l++) {
     const T v = lds[(16 + 4 * l + q) * S + sL];  // C_L[l][q]
@@ -1342,11 +1346,13 @@ __device__ __forceinline__ void thomas_quad_open(T* lds, int S, int N, int d, in
     T a[16], cq[4], f[4], Dq[4];
 #pragma unroll
     for (int e = 0; e < 16; e++) a[e] = lds[e * S + s];
+    int qo = q;  // opaque, as in pcr_quad_row
+    asm volatile("" : "+v"(qo));
 #pragma unroll
     for (int m = 0; m < 4; m++) {
       cq[m] = lds[(16 + 4 * m + q) * S + s];
       f[m] = lds[(32 + m) * S + s];
-      Dq[m] = (m == q) ? T(1) : T(0);
+      Dq[m] = (m == qo) ? T(1) : T(0);
     }
 #pragma unroll
     for (int l = 0; l < 4; l++)
