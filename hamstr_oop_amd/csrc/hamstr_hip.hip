@@ -2728,13 +2728,13 @@ __device__ __forceinline__ void face_states_sum(const FaceTab& ft, const double*
 // k_lin_faces<ORDER, TL, FASTJ, true> on (q, dq), then k_rhs_faces<ORDER, TL,
 // true, FASTJ> on q + dq (ff_from, fst as there)
 template <int ORDER, typename TL, bool FASTJ>
-__global__ __launch_bounds__(kBlock) void k_lin_rhs_faces(FaceTab ft, Consts c, const double* __restrict__ q,
+__global__ __launch_bounds__(kBlock) void k_lin_rhs_faces(FaceTab ft, Consts c, Live lv, const double* __restrict__ q,
                                                           const double* __restrict__ dq,
                                                           double* __restrict__ lflux, double* __restrict__ flux,
                                                           double* __restrict__ sr, TL* __restrict__ ff, int ff_from,
                                                           double* __restrict__ fst, int* __restrict__ nanflag) {
   const int p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (p >= c.nProc) return;
+  if (p >= c.nProc || (lv.face && !lv.face[p])) return;
   const int code = ft.pslot[p];
   const int s = code & kSlotMask;
   const double2 n = *reinterpret_cast<const double2*>(ft.nvec + 2 * (size_t)p);
@@ -2821,14 +2821,14 @@ __global__ __launch_bounds__(kBlock) void k_lin_rhs_faces(FaceTab ft, Consts c, 
   }
 }
 
-// k_lin_cells<true> over every cell (its block partials to p_lin; the linear
-// residual itself is not kept: the next computeRHS overwrites r), then
-// k_rhs_cells<true> over the cells of the first nb_rhs blocks (block partials
-// to p_nl).  The launch's first nb_rhs workgroups take those blocks, spread
+// k_lin_cells<true> over every live cell (its block partials to p_lin, and in
+// a batch its density residual to rrho[nCell + i]; the linear residual itself
+// is not kept: the next computeRHS overwrites r), then k_rhs_cells<true> over
+// the cells of the first nb_rhs blocks (block partials to p_nl, rrho[i]).  The launch's first nb_rhs workgroups take those blocks, spread
 // over the XCDs like a launch of their own (nb_rhs a multiple of 8), so that
 // every XCD gets its share of the blocks with both sums (all of them on three
 // XCDs took the kernel from 12.9 to 16.4 us on the bench mesh).
-__global__ __launch_bounds__(kBlock) void k_lin_rhs_cells(Consts c, const int* __restrict__ off,
+__global__ __launch_bounds__(kBlock) void k_lin_rhs_cells(Consts c, Live lv, const int* __restrict__ off,
                                                           const int* __restrict__ ent,
                                                           const double* __restrict__ lflux,
                                                           const double* __restrict__ flux,
@@ -2836,11 +2836,11 @@ __global__ __launch_bounds__(kBlock) void k_lin_rhs_cells(Consts c, const int* _
                                                           const double* __restrict__ dq, double* __restrict__ sigma,
                                                           double* __restrict__ r, double* __restrict__ q,
                                                           double* __restrict__ p_lin, double* __restrict__ p_nl,
-                                                          int nb_rhs) {
+                                                          double* __restrict__ rrho, int nb_rhs) {
   const int bx = blockIdx.x;
   const int vb = bx < nb_rhs ? xcd_block(bx, nb_rhs) : nb_rhs + xcd_block(bx - nb_rhs, gridDim.x - nb_rhs);
   const int i = vb * blockDim.x + threadIdx.x;
-  const bool in = i < c.nCell;
+  const bool in = i < c.nCell && (!lv.cell || lv.cell[i]);
   double sq = 0.0, mx = 0.0;
   if (in) {
     const double4 a = *reinterpret_cast<const double4*>(r0 + 4 * (size_t)i);
@@ -2850,6 +2850,7 @@ __global__ __launch_bounds__(kBlock) void k_lin_rhs_cells(Consts c, const int* _
     cell_lin_sum(ent, off[i], off[i + 1], lflux, dtfac, a0, a1, a2, a3);
     sq = a0 * a0;
     mx = fabs(a0);
+    if (rrho) rrho[c.nCell + i] = a0;
     double4 qv = *reinterpret_cast<double4*>(q + 4 * (size_t)i);
     qv.x += b.x; qv.y += b.y; qv.z += b.z; qv.w += b.w;
     *reinterpret_cast<double4*>(q + 4 * (size_t)i) = qv;
@@ -2863,6 +2864,7 @@ __global__ __launch_bounds__(kBlock) void k_lin_rhs_cells(Consts c, const int* _
     cell_flux_sum(ent, off[i], off[i + 1], flux, sr, a0, a1, a2, a3, sg);
     sq = a0 * a0;
     mx = fabs(a0);
+    if (rrho) rrho[i] = a0;
     sigma[i] = sg;
     const double dtfac = c.cfl / sg;
     const double4 v = make_double4(a0 * dtfac, a1 * dtfac, a2 * dtfac, a3 * dtfac);
@@ -4534,7 +4536,7 @@ void drop_graph(hamstr_ctx* x) {
 }
 
 int enqueue_adi_iteration(hamstr_ctx* x, double* nb, const std::function<int(const char*)>& mark, double* p_nl,
-                          double* p_lin, bool fuse_in, double* p_nl_next);
+                          double* p_lin, bool fuse_in, double* p_nl_next, double* nl_norm_now);
 bool can_fuse(const hamstr_ctx* x);
 
 // stepSolution's euler and rk3 branches (march.cpp:73-108): per stage one
@@ -4575,23 +4577,33 @@ int enqueue_explicit(hamstr_ctx* x, const std::function<int(const char*)>& mark,
 // per run of iterations by enqueue_block); batches: per-case norms of both
 // stages to nb[nCase][4] at the end of the iteration.
 // fuse_in / fuse_out: the iteration's computeRHS is fused into the previous
-// iteration's tail / its tail into the next iteration's computeRHS (can_fuse)
+// iteration's tail / its tail into the next iteration's computeRHS (can_fuse;
+// nb_next: the next iteration's norms).  A batch's per-case norms are reduced
+// from rrho, which the fused tail overwrites with the next iteration's
+// residuals: each stage's norms are taken as soon as it is complete.
 int enqueue_iteration(hamstr_ctx* x, double* nb, const std::function<int(const char*)>& mark, int slot,
-                      bool fuse_in = false, bool fuse_out = false) {
+                      bool fuse_in = false, bool fuse_out = false, double* nb_next = nullptr) {
   int rc;
   const bool one = x->nCase == 1;
   double* p_nl = one ? x->npart + (size_t)(2 * slot) * x->nPart * 2 : x->partial;
   double* p_lin = one ? x->npart + (size_t)(2 * slot + 1) * x->nPart * 2 : x->partial;
-  double* p_next = (one && fuse_out) ? x->npart + (size_t)(2 * (slot + 1)) * x->nPart * 2 : nullptr;
+  double* p_next = fuse_out ? (one ? x->npart + (size_t)(2 * (slot + 1)) * x->nPart * 2 : x->partial) : nullptr;
+  double* nl_now = (!one && fuse_out && !fuse_in) ? nb : nullptr;
   if (x->scheme != HAMSTR_SCHEME_ADI) {
     if ((rc = enqueue_explicit(x, mark, one ? p_nl : nullptr, one ? p_lin : nullptr))) return rc;
-  } else if ((rc = enqueue_adi_iteration(x, nb, mark, p_nl, p_lin, fuse_in, p_next))) {
+  } else if ((rc = enqueue_adi_iteration(x, nb, mark, p_nl, p_lin, fuse_in, p_next, nl_now))) {
     return rc;
   }
   if (!one) {
-    hipLaunchKernelGGL(k_case_norms, dim3(x->nCase, 2), dim3(kRedBlock), 0, x->stream, x->rrho, x->nCell,
-                       x->case_off, 0, nb, x->live(), x->keep_norms);
-    HIPCHK(hipGetLastError());
+    if (fuse_out) {
+      if ((rc = launch_norm(x, nb, 2)) || (rc = launch_norm(x, nb_next, 0))) return rc;
+    } else if (fuse_in) {
+      if ((rc = launch_norm(x, nb, 2))) return rc;
+    } else {
+      hipLaunchKernelGGL(k_case_norms, dim3(x->nCase, 2), dim3(kRedBlock), 0, x->stream, x->rrho, x->nCell,
+                         x->case_off, 0, nb, x->live(), x->keep_norms);
+      HIPCHK(hipGetLastError());
+    }
     if (mark && mark("norms")) return HAMSTR_ERR_HIP;
   }
   return HAMSTR_OK;
@@ -4603,8 +4615,10 @@ int enqueue_iteration(hamstr_ctx* x, double* nb, const std::function<int(const c
 // fused iteration the Jacobian source changes (the autotuner times it)
 // rhs_done: the fused tail of the previous iteration (launch_lin_rhs) has
 // done computeRHS already, except the residual sums phase 0's launch carries
+// nl_norm_now (a batch whose residual norm is read before the fused tail
+// overwrites it): the per-case norms of this computeRHS go there at once
 int enqueue_rhs_sweep0(hamstr_ctx* x, double* nb, const std::function<int(const char*)>& mark, double* p_nl,
-                       int rj, bool rhs_done = false) {
+                       int rj, bool rhs_done = false, double* nl_norm_now = nullptr) {
   int rc;
   // FAST: the ADI kernels may compute the chain faces' Jacobians again, from
   // the face states k_rhs_faces stores instead (64 B per face, FP64 256 B)
@@ -4628,14 +4642,17 @@ int enqueue_rhs_sweep0(hamstr_ctx* x, double* nb, const std::function<int(const 
     return rc;
   }
   if (!rhs_done && mark && mark("rhs_cells")) return HAMSTR_ERR_HIP;
+  if (!rhs_done && nl_norm_now && (rc = launch_norm(x, nl_norm_now, 0))) return rc;
   return launch_sweep_phases(x, 0, mark, rj, &cw, extra);
 }
 
 // Iterations of a captured block run fused across their boundary (launch_lin_rhs)
 // for a single live case with one sweep: every value the same, bit for bit.
 bool can_fuse(const hamstr_ctx* x) {
-  return x->nCase == 1 && x->prm.msweep == 1 && x->scheme == HAMSTR_SCHEME_ADI && !x->retire && !x->any_retired &&
-         x->npart != nullptr && x->rrho == nullptr;
+  if (x->prm.msweep != 1 || x->scheme != HAMSTR_SCHEME_ADI) return false;
+  // a single case reduces block partials per iteration slot, a batch its
+  // per-cell density residuals per case
+  return x->nCase == 1 ? (x->npart != nullptr && x->rrho == nullptr) : x->rrho != nullptr;
 }
 
 // computeLinearRHS of this iteration with the update, fused with computeRHS
@@ -4645,12 +4662,13 @@ bool can_fuse(const hamstr_ctx* x) {
 int launch_lin_rhs(hamstr_ctx* x, int rj, double* p_lin, double* p_nl_next,
                    const std::function<int(const char*)>& mark) {
   const FaceTab ft = x->ftab();
+  const Live lv = x->live();
   const int ff_from = rj != HAMSTR_JAC_STORED ? x->nChainProc : 0;
   double* fst = rj == HAMSTR_JAC_FROM_STATES ? x->fst : nullptr;
   const dim3 g(nblk(x->nProc)), b(kBlock);
 #define LRF(O, T, F)                                                                                         \
-  hipLaunchKernelGGL((k_lin_rhs_faces<O, T, F>), g, b, 0, x->stream, ft, x->c, x->q, x->dq, x->lflux, x->flux, \
-                     x->sr, (T*)x->ff, ff_from, fst, x->nanflag)
+  hipLaunchKernelGGL((k_lin_rhs_faces<O, T, F>), g, b, 0, x->stream, ft, x->c, lv, x->q, x->dq, x->lflux,      \
+                     x->flux, x->sr, (T*)x->ff, ff_from, fst, x->nanflag)
 #define LRO(T, F) \
   if (o == 1) { LRF(1, T, F); } else if (o == 3) { LRF(3, T, F); } else { LRF(5, T, F); }
   const int o = x->prm.order;
@@ -4668,9 +4686,9 @@ int launch_lin_rhs(hamstr_ctx* x, int rj, double* p_lin, double* p_nl_next,
   // else all of them)
   int nb_rhs = cells_overlap_extra(x) > 0 ? x->split_cells / kBlock : nblk(x->nCell);
   if (nb_rhs % kXcd != 0) nb_rhs = nblk(x->nCell);
-  hipLaunchKernelGGL(k_lin_rhs_cells, dim3(nblk(x->nCell)), dim3(kBlock), 0, x->stream, x->c, x->cell_off,
+  hipLaunchKernelGGL(k_lin_rhs_cells, dim3(nblk(x->nCell)), dim3(kBlock), 0, x->stream, x->c, lv, x->cell_off,
                      x->cell_ent, x->lflux, x->flux, x->sr, x->r0, x->dq, x->sigma, x->r, x->q, p_lin, p_nl_next,
-                     nb_rhs);
+                     x->rrho, nb_rhs);
   HIPCHK(hipGetLastError());
   if (mark && mark("lin_rhs_cells")) return HAMSTR_ERR_HIP;
   return HAMSTR_OK;
@@ -4678,14 +4696,14 @@ int launch_lin_rhs(hamstr_ctx* x, int rj, double* p_lin, double* p_nl_next,
 
 // fuse_in: this iteration's computeRHS was fused into the previous one's tail;
 // p_nl_next non-null: fuse this iteration's tail with the next's computeRHS
-// (can_fuse contexts only)
+// (can_fuse contexts only); nl_norm_now: as in enqueue_rhs_sweep0
 int enqueue_adi_iteration(hamstr_ctx* x, double* nb, const std::function<int(const char*)>& mark, double* p_nl,
-                          double* p_lin, bool fuse_in, double* p_nl_next) {
+                          double* p_lin, bool fuse_in, double* p_nl_next, double* nl_norm_now) {
   int rc;
   const int rj = jac_source(x);
   for (int sw = 0; sw < x->prm.msweep; sw++) {
     if (sw == 0) {
-      if ((rc = enqueue_rhs_sweep0(x, nb, mark, p_nl, rj, fuse_in))) return rc;
+      if ((rc = enqueue_rhs_sweep0(x, nb, mark, p_nl, rj, fuse_in, nl_norm_now))) return rc;
     } else if ((rc = launch_sweep_phases(x, sw, mark, rj))) {
       return rc;
     }
@@ -4709,7 +4727,8 @@ int enqueue_block(hamstr_ctx* x, int count, double* base, const std::function<in
   const int nc = x->nCase;
   const bool fuse = count > 1 && can_fuse(x);
   for (int s = 0; s < count; s++) {
-    const int rc = enqueue_iteration(x, base + 4 * (size_t)nc * s, mark, s, fuse && s > 0, fuse && s + 1 < count);
+    const int rc = enqueue_iteration(x, base + 4 * (size_t)nc * s, mark, s, fuse && s > 0, fuse && s + 1 < count,
+                                     base + 4 * (size_t)nc * (s + 1));
     if (rc) return rc;
   }
   if (nc == 1) {

@@ -1,7 +1,8 @@
 """Iterations fused across their boundary: inside a captured block of
 iterations, computeLinearRHS of one iteration (with the update q += dq) and
 computeRHS("implicit") of the next run as one face kernel and one cell kernel
-(k_lin_rhs_faces, k_lin_rhs_cells) for a single case with one sweep.  Each
+(k_lin_rhs_faces, k_lin_rhs_cells) for a single case or a batch with one
+sweep.  Each
 value is computed by the same operations as unfused, so a run replaying
 graphs (fused) equals the same run launched one iteration at a time
 (unfused) bit for bit: the state, the residual of the last computeRHS, the
@@ -109,8 +110,61 @@ def test_fused_exact_block_matches_the_oracle(hip):
     g.close()
 
 
+def _batch_cases():
+    from hamstr_oop_amd import meshgen
+    return [(meshgen.circle_ogrid(32, 12), 0.3, 0.0), (meshgen.naca_ogrid(40, 16, tri_layers=6), 0.6, 1.0),
+            (meshgen.naca_ogrid(64, 16), 0.5, -2.0), (meshgen.naca_ogrid(48, 20, t=0.10), 0.7, 3.0)]
+
+
+@pytest.mark.parametrize("lhs", ["fp64", "fp32"])
+def test_fused_batch_equals_single_iterations(hip, lhs):
+    """A batch of uneven cases: blocks of 10 iterations replayed from graphs
+    (fused) against the same iterations launched one at a time, to a tolerance
+    some cases reach: states, iterations to tolerance and final per-case norms
+    bit for bit (each stage's per-case norms taken before the fused tail
+    overwrites its residuals)."""
+    lm = hip.LHS_FP64 if lhs == "fp64" else hip.LHS_FP32
+    out = []
+    for graph in (True, False):
+        b = hip.Batch(_batch_cases(), 3, 10.0, lhs=lm, adi_mode=hip.ADI_FAST)
+        b.set_graph(graph)
+        n, it, nrm = b.batch_step(200, tol=1e-6, check_every=10)
+        out.append((n, it.copy(), nrm.copy(), [b.case_q(k).copy() for k in range(b.n_case)]))
+        b.close()
+    (na, ia, ra, qa), (nb, ib, rb, qb) = out
+    assert na == nb
+    _eq(ia, ib, "iterations to tolerance")
+    assert (ia > 0).any()  # some case reaches the tolerance
+    _eq(ra, rb, "final norms")
+    for k in range(len(qa)):
+        _eq(qa[k], qb[k], f"case {k}")
+
+
+def test_fused_batch_with_retirement(hip):
+    """Retired cases drop out of the fused kernels as of every other one: a
+    batch retiring converged and stalled cases, graphs against one iteration
+    at a time."""
+    out = []
+    for graph in (True, False):
+        b = hip.Batch(_batch_cases(), 3, 10.0, adi_mode=hip.ADI_FAST)
+        b.set_graph(graph)
+        b.set_retire(True, 40)
+        n, it, nrm = b.batch_step(300, tol=1e-7, check_every=10)
+        st, its = b.case_status()
+        out.append((n, it.copy(), nrm.copy(), st.copy(), its.copy(), [b.case_q(k).copy() for k in range(b.n_case)]))
+        b.close()
+    (na, ia, ra, sa, ta, qa), (nb, ib, rb, sb, tb, qb) = out
+    assert na == nb
+    _eq(ia, ib, "iterations to tolerance")
+    _eq(sa, sb, "case status")
+    _eq(ta, tb, "retirement iterations")
+    _eq(ra, rb, "final norms")
+    for k in range(len(qa)):
+        _eq(qa[k], qb[k], f"case {k}")
+
+
 def test_batches_and_sweeps_stay_unfused(hip):
-    """Batches and msweep > 1 run unfused (equal results either way)."""
+    """msweep > 1 runs unfused (equal results either way)."""
     from hamstr_oop_amd import meshgen
     mesh = meshgen.circle_ogrid(32, 16)
     out = []
