@@ -14,6 +14,15 @@ computeRHS.cpp:268-271): a case whose norm has not halved within stall_window
 iterations is called stalled, in every run alike, and counted separately.  A
 few cases blow up from the freestream at CFL 10 (a NaN sound speed, where the
 reference exits): they count as diverged.
+
+Most of those limit cycles belong to the iteration at CFL 10, not to the
+discrete equations: continued at CFL 5, nearly every stalled case converges
+within a few hundred iterations, to the same fixed point (the CFL number
+scales the implicit operator and the pseudo-time step, not the residual).
+Every run therefore ends with the same CFL back-off (BACKOFF_CFL): cases
+still stalled continue from the state they reached, and diverged ones restart
+from their initial state, at CFL 5 and then 2.5, plain FP64 ADI each time,
+under the same stall rule.  A case converged in any pass counts as converged.
 """
 from __future__ import annotations
 
@@ -27,6 +36,9 @@ from . import hip, meshgen
 
 SIZES = ((128, 64), (256, 128), (384, 192))
 SEED = 20261203
+# CFL numbers of the passes that continue stalled cases and restart diverged
+# ones (after the run at its own CFL); () turns the back-off off
+BACKOFF_CFL = (5.0, 2.5)
 
 
 @dataclass
@@ -83,61 +95,82 @@ class RunResult:
 
 def run_sequential(specs: Sequence[CaseSpec], meshes, tol: float, max_iters: int, stall_window: int,
                    lhs: int = hip.LHS_FP64, check_every: int = 50, order: int = 3, cfl: float = 10.0,
-                   progress=None) -> RunResult:
+                   progress=None, backoff: Sequence[float] = BACKOFF_CFL) -> RunResult:
     """One single-case context per case, freestream start; the clock covers the
-    iterations only (context set-up excluded)."""
+    iterations only (context set-up excluded).  A case that stalls or
+    diverges goes through the back-off passes (a context of its own per CFL
+    number, FP64 left-hand side), their iterations added to its own."""
     iters = np.full(len(specs), -1, np.int64)
     status = np.zeros(len(specs), np.int32)
     secs = np.zeros(len(specs))
     total = 0.0
     cell_iters = 0.0
     for k, (c, mesh) in enumerate(zip(specs, meshes)):
-        s = hip.Solver(mesh, c.mach, c.alpha, order, cfl, lhs=lhs, adi_mode=hip.ADI_FAST)
-        s.set_retire(True, stall_window)
-        s.prepare_graph(check_every)   # capture outside the clock, like context creation
-        s.sync()
-        t0 = time.perf_counter()
-        n, _, _ = s.step(max_iters, tol=tol, check_every=check_every)
-        s.sync()
-        secs[k] = time.perf_counter() - t0
+        done = 0
+        q = None
+        for p, cf in enumerate((cfl,) + tuple(backoff)):
+            if p > 0 and status[k] not in (2, 3):
+                break
+            s = hip.Solver(mesh, c.mach, c.alpha, order, cf, lhs=lhs if p == 0 else hip.LHS_FP64,
+                           adi_mode=hip.ADI_FAST)
+            s.set_retire(True, stall_window)
+            s.prepare_graph(check_every)   # capture outside the clock, like context creation
+            s.sync()
+            t0 = time.perf_counter()
+            if q is not None:
+                s.set("q", q)   # a stalled case continues from its state
+            n, _, _ = s.step(max_iters, tol=tol, check_every=check_every)
+            s.sync()
+            secs[k] += time.perf_counter() - t0
+            st, it = s.case_status()
+            ran = int(it[0]) if st[0] > 0 else n   # iterations until the case retired
+            status[k] = st[0]
+            iters[k] = done + ran
+            done += ran
+            cell_iters += mesh.n_cell * ran
+            q = s.get("q") if st[0] == 2 else None
+            s.close()
         total += secs[k]
-        st, it = s.case_status()
-        status[k], iters[k] = st[0], it[0]
-        cell_iters += mesh.n_cell * n
-        s.close()
         if progress and (k + 1) % 32 == 0:
             progress(f"sequential: {k + 1}/{len(specs)} cases, {total:.1f} s")
     return RunResult(total, iters, status, cell_iters, case_seconds=secs)
 
 
 class BatchOutcome:
-    """Final states of a batched run, per case (from the FP64 fallback context
-    for the cases it re-ran)."""
+    """Final states of a batched run, per case: from the last of the run's
+    contexts (the first pass, the FP64 fallback, the back-off passes) that
+    ran the case."""
 
-    def __init__(self, main, sub=None, redo=()):
-        self.main, self.sub = main, sub
-        self.where = {k: j for j, k in enumerate(redo)}
+    def __init__(self, main):
+        self.main = main
+        self.layers = []   # (context, {case: index in that context}), in pass order
+
+    def add(self, ctx, idx):
+        self.layers.append((ctx, {k: j for j, k in enumerate(idx)}))
 
     def case_q(self, k: int) -> np.ndarray:
-        if self.sub is not None and k in self.where:
-            return self.sub.case_q(self.where[k])
+        for ctx, where in reversed(self.layers):
+            if k in where:
+                return ctx.case_q(where[k])
         return self.main.case_q(k)
 
     def close(self):
-        for b in (self.main, self.sub):
-            if b is not None:
-                b.close()
+        for b in [self.main] + [ctx for ctx, _ in self.layers]:
+            b.close()
 
 
 def run_batched(specs: Sequence[CaseSpec], meshes, tol: float, max_iters: int, stall_window: int,
                 lhs: int = hip.LHS_FP32, check_every: int = 50, order: int = 3, cfl: float = 10.0,
-                init=None, autotune: bool = True, fallback: bool = True):
+                init=None, autotune: bool = True, fallback: bool = True,
+                backoff: Sequence[float] = BACKOFF_CFL):
     """All cases in one batched context.  init(batch, case_indices) sets the
     initial state (a warm start); it and the autotuner are timed with the
     iterations.  With an FP32 LHS, cases that stall or diverge are re-run with
     the FP64 LHS (the plain ADI iteration is the fallback, and decides convergence): a
     stalled case from the state it reached, a diverged one from its initial
-    state.  Every pass applies the stall rule of the sequential runs (a shorter
+    state.  Then (with fallback) the cases still stalled or diverged go through
+    the back-off passes the same way, one batched FP64 context per CFL number.
+    Every pass applies the stall rule of the sequential runs (a shorter
     window for the FP32 pass hands cases that are merely on a plateau to the
     FP64 pass and costs more than it saves).
     Returns (RunResult, BatchOutcome, info)."""
@@ -177,32 +210,46 @@ def run_batched(specs: Sequence[CaseSpec], meshes, tol: float, max_iters: int, s
     cell_iters = float((ncell * np.where(st > 0, it, max_iters)).sum())
     info = {"stalled_first_pass": int((st == 2).sum()), "diverged_first_pass": int((st == 3).sum()),
             "fallback_cases": 0}
-    sub, redo = None, []
-    if fallback and lhs == hip.LHS_FP32:
-        redo = [k for k in range(n) if st[k] in (2, 3)]
-        if redo:
-            sub = hip.Batch([(meshes[k], specs[k].mach, specs[k].alpha) for k in redo], order=order, cfl=cfl,
-                            lhs=hip.LHS_FP64, adi_mode=hip.ADI_FAST)
-            if init is not None:
-                init(sub, redo)
-            for j, k in enumerate(redo):
-                if st[k] == 2:
-                    sub.set_case_q(j, b.case_q(k))
-            sub.set_retire(True, stall_window)
-            stepped(sub, redo, max_iters)
-            st2, it2 = sub.case_status()
-            for j, k in enumerate(redo):
-                run2 = it2[j] if st2[j] > 0 else max_iters
-                status[k] = st2[j]
-                iters[k] = it[k] + run2
-                cell_iters += ncell[k] * run2
+    outcome = BatchOutcome(b)
+    passes = []
+    if fallback:
+        if lhs == hip.LHS_FP32:
+            passes.append(("fallback", cfl))
+        passes += [(f"backoff_cfl{cf:g}", cf) for cf in backoff]
+    for name, cf in passes:
+        redo = [k for k in range(n) if status[k] in (2, 3)]
+        if not redo:
+            break
+        # a stalled case continues from the state it reached (in the last
+        # context that ran it), a diverged one restarts from its initial state
+        prev = {k: outcome.case_q(k) for k in redo if status[k] == 2}
+        sub = hip.Batch([(meshes[k], specs[k].mach, specs[k].alpha) for k in redo], order=order, cfl=cf,
+                        lhs=hip.LHS_FP64, adi_mode=hip.ADI_FAST)
+        outcome.add(sub, redo)   # closed with the outcome
+        if init is not None:
+            init(sub, redo)
+        for j, k in enumerate(redo):
+            if k in prev:
+                sub.set_case_q(j, prev[k])
+        sub.set_retire(True, stall_window)
+        stepped(sub, redo, max_iters)
+        st2, it2 = sub.case_status()
+        for j, k in enumerate(redo):
+            run2 = it2[j] if st2[j] > 0 else max_iters
+            status[k] = st2[j]
+            iters[k] += run2
+            cell_iters += ncell[k] * run2
+        if name == "fallback":
             info["fallback_cases"] = len(redo)
             info["fallback_converged"] = int((st2 == 1).sum())
+        else:
+            info[name + "_cases"] = len(redo)
+            info[name + "_converged"] = int((st2 == 1).sum())
     b.sync()
     sec = time.perf_counter() - t0
     info["first_pass_s"] = round(t_first, 3)
     info["fallback_s"] = round(sec - t_first, 3)
-    return RunResult(sec, iters, status, cell_iters, conv_time=conv_time), BatchOutcome(b, sub, redo), info
+    return RunResult(sec, iters, status, cell_iters, conv_time=conv_time), outcome, info
 
 
 def train_network(n_train: int, seed: int = SEED + 1, size=(256, 128), tol: float = 1e-11, max_iters: int = 20000,
