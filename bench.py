@@ -109,6 +109,66 @@ def measured_traffic(stage, nc, nr, order, lhs, adi, tri=0, jsrc=0):
     return None if st is None else st["traffic_bytes"]
 
 
+def counter_summary(args, fetch_dir, write_dir, trace_dir, timed=20):
+    """The traffic summary measured_traffic reads, from three rocprofv3 runs of
+    this command with the Jacobian source fixed (--jacobians) and
+    --copy-probe: `--pmc FETCH_SIZE`, `--pmc WRITE_SIZE` and `--kernel-trace`
+    (each `-f csv -o run`).  The copy of 256 MiB scales each counter to bytes.
+    Per stage: the eager region's last `timed` dispatches of its kernel
+    (`stages`, the region `roofline` and `stage_ms` come from), and the
+    timed graph region's (`graph_stages`): the `timed` ADI dispatches before
+    those and the fused kernels' last timed - 1.  Writes
+    profiles/traffic_*.json and returns its path."""
+    import csv
+    import re
+
+    def short(name):
+        return re.split(r"\(", name.replace("(anonymous namespace)::", "").replace("void ", ""))[0].strip()
+
+    def load(d, value):
+        out = {}
+        f = os.path.join(d, "run_counter_collection.csv" if value else "run_kernel_trace.csv")
+        with open(f) as fh:
+            for r in csv.DictReader(fh):
+                v = float(r["Counter_Value"]) if value else \
+                    (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+                out.setdefault(short(r["Kernel_Name"]), []).append((int(r.get("Dispatch_Id", 0)), v))
+        return {k: [v for _, v in sorted(x)] for k, x in out.items()}
+
+    fetch, write, trace = load(fetch_dir, True), load(write_dir, True), load(trace_dir, False)
+    copy = "__amd_rocclr_copyBuffer"
+    fk, wk = max(fetch[copy]), max(write[copy])
+    fs, ws = (256 << 10) / fk, (256 << 10) / wk
+    src = args.jacobians
+    jq = "true" if src in ("states", "q") else "false"
+    o, t = args.order, "double"
+    eager = {"adi_phase1": f"k_adi_fast_half2<{t}, 384, {jq}>", "adi_phase0": f"k_adi_fast<{t}, 1, 512, {jq}>",
+             "lin_faces": f"k_lin_faces<{o}, {t}, true, true>", "rhs_faces": f"k_rhs_faces<{o}, {t}, true, true>",
+             "lin_cells": "k_lin_cells<true>", "rhs_cells": "k_rhs_cells<true>", "norms": "k_norm_slots"}
+    graph = {"adi_phase1": (eager["adi_phase1"], 2 * timed, timed), "adi_phase0": (eager["adi_phase0"], 2 * timed, timed),
+             "lin_rhs_faces": (f"k_lin_rhs_faces<{o}, {t}, true>", timed - 1, 0),
+             "lin_rhs_cells": ("k_lin_rhs_cells", timed - 1, 0)}
+
+    def stage(name, a, b):
+        sl = slice(-a, -b if b else None)
+        f, w, d = fetch[name][sl], write[name][sl], trace[name][sl]
+        fm, wm = sum(f) / len(f), sum(w) / len(w)
+        return {"kernels": [name], "avg_us": sum(d) / len(d), "traffic_bytes": (fm * fs + wm * ws) * 1024,
+                "fetch_kib": fm, "write_kib": wm}
+
+    res = {"workload": {"nc": args.nc, "nr": args.nr, "tri_layers": args.tri_layers, "order": args.order,
+                        "lhs": "fp64", "adi": args.adi, "jacobian_source": src},
+           "timed_iterations": timed, "fetch_scale": fs, "write_scale": ws,
+           "calibration": {"copy_bytes": 256 << 20, "fetch_kib": fk, "write_kib": wk},
+           "stages": {k: stage(n, timed, 0) for k, n in eager.items() if n in fetch},
+           "graph_stages": {k: stage(n, a, b) for k, (n, a, b) in graph.items() if n in fetch}}
+    tag = f"_tri{args.tri_layers}" if args.tri_layers else ""
+    path = os.path.join(ROOT, "profiles", f"traffic_fp64_naca{args.nc}x{args.nr}{tag}_o{args.order}_{src}.json")
+    with open(path, "w") as fh:
+        json.dump(res, fh, indent=1)
+    return path
+
+
 def cpu_baseline(mesh, mach, alpha, order, cfl, budget_s=12.0, max_iters=30):
     """The reference's own code (oracle/_ref, compiled from its sources) on a
     bounded sample: `n` iterations of the same case from freestream, 1 core.
@@ -180,11 +240,17 @@ def main():
                     help="skip the FAST vs EXACT comparison beyond L2 1e-10")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one plain 256 MiB device-to-device copy (counter calibration under rocprofv3)")
+    ap.add_argument("--counter-summary", nargs=3, metavar=("FETCH_DIR", "WRITE_DIR", "TRACE_DIR"),
+                    help="write profiles/traffic_*.json for this workload and --jacobians source from the csv "
+                         "output of three rocprofv3 runs of the same command (no GPU needed)")
     ap.add_argument("--weno-profile", type=int, default=0, metavar="N",
                     help="only the WENO transonic case: N iterations per left-hand side, with stage times "
                          "(a short command for rocprofv3)")
     args = ap.parse_args()
 
+    if args.counter_summary:
+        print(counter_summary(args, *args.counter_summary), flush=True)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this benchmark runs on one GPU only; multi-GPU runs are not part of it "
                                    "(cases would shard whole meshes per GPU)", "n_gpus": args.gpus}))

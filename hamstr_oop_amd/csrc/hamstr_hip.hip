@@ -1163,6 +1163,18 @@ __device__ __forceinline__ void normalise_row(const T* B, T* a, T* c, T* f) {
 constexpr __host__ __device__ __forceinline__ int lpad(int i) { return i + (i >> 5); }
 constexpr __host__ __device__ __forceinline__ int lds_stride(int N) { return N + (N >> 5) + 1; }
 
+// LDS of a gfx950 workgroup (160 KiB)
+constexpr int kLdsBytes = 160 * 1024;
+// How many odd rows per chain k_adi_fast_half2 keeps in the LDS its two
+// slices leave free (36 entries each, stride one more): FP64 84 of at most
+// 192, FP32 all of them.  The rest go through global scratch.
+template <typename T, int MAXH>
+constexpr __host__ __device__ int half2_odd_lds() {
+  constexpr int free_ent = kLdsBytes / (int)sizeof(T) - 2 * 36 * lds_stride((MAXH + 1) / 2);
+  constexpr int ks = free_ent / (2 * 36);
+  return ks - 1 < MAXH / 2 ? ks - 1 : MAXH / 2;
+}
+
 template <typename T>
 __device__ __forceinline__ void lds_load_row(const T* lds, int N, int i, T* a, T* c, T* f) {
 #pragma unroll
@@ -2314,14 +2326,17 @@ __device__ __forceinline__ void adi_fast_chain_long(ChainTab ct, const int ch, c
 // and NL >= N, the row count the reduction levels run for (NL == N for
 // closed chains).  Needs N <= the group's threads, open chains or closed
 // chains of an even N >= 4, ncr >= 1, and ceil(NL/2) rows within the fixed
-// LDS stride SM (SM >= lds_stride(ceil(NL/2))).
-template <typename T, int FINAL, int SM, int XS, bool JQ>
+// LDS stride SM (SM >= lds_stride(ceil(NL/2))).  With KL > 0 the first KL
+// odd rows stay in LDS instead, in `ol` (element-major, stride KS >= KL),
+// and only the others go through global scratch.
+template <typename T, int FINAL, int SM, int XS, bool JQ, int KL = 0, int KS = 1>
 __device__ __forceinline__ void adi_fast_chain_half(ChainTab ct, const JacSrc& js, const int ch, const int NL, T* lds,
                                                     const int t, const int NT, const int tw,
                                                     const T* __restrict__ ff, int nProc, double cfl,
                                                     const double* __restrict__ sigma, double* __restrict__ r,
                                                     double* __restrict__ dq, int dqmode, int ncr,
-                                                    T* __restrict__ hs, int hstride, const int* __restrict__ hbase) {
+                                                    T* __restrict__ hs, int hstride, const int* __restrict__ hbase,
+                                                    T* ol = nullptr) {
   const int pb = ch >= 0 ? ct.pbase[ch] : 0, nf = ch >= 0 ? ct.nf[ch] : 0, N = ch >= 0 ? ct.ncell[ch] : 0;
   const bool periodic = FINAL != kFinalOpen;
   const int M = (N + 1) / 2;
@@ -2334,7 +2349,10 @@ __device__ __forceinline__ void adi_fast_chain_half(ChainTab ct, const JacSrc& j
     fast_row_xchg<T, JQ>(ct, js, ff, nProc, pb, nf, N, periodic, row, active, cfl, sigma, r, lds, XS, a, c, f);
     if (active && !even) {
       lds_store_row(lds, SM, lpad(t - M), a, c, f);
-      lds_store_row(hrow, hstride, t - M, a, c, f);
+      if (KL > 0 && t - M < KL)
+        lds_store_row(ol, KS, t - M, a, c, f);
+      else
+        lds_store_row(hrow, hstride, t - M, a, c, f);
     }
     __syncthreads();
     if (active && even) {
@@ -2357,7 +2375,10 @@ __device__ __forceinline__ void adi_fast_chain_half(ChainTab ct, const JacSrc& j
       // x = F^ - A^ x_(row-1) - C^ x_(row+1), the even rows' x in the F slots
       const int k = t - M;
       T ra[16], rc[16];
-      lds_load_row(hrow, hstride, k, ra, rc, x);
+      if (KL > 0 && k < KL)
+        lds_load_row(ol, KS, k, ra, rc, x);
+      else
+        lds_load_row(hrow, hstride, k, ra, rc, x);
       mv4_acc_lds(x, ra, lds, SM, lpad(k));
       if (periodic || row + 1 < N) mv4_acc_lds(x, rc, lds, SM, lpad(((row + 1) % N) >> 1));
     }
@@ -2516,12 +2537,16 @@ __global__ __launch_bounds__(2 * MAXH) __attribute__((amdgpu_waves_per_eu(3))) v
     double cfl, const double* __restrict__ sigma, double* __restrict__ r, double* __restrict__ dq, int dqmode,
     int ncr, T* __restrict__ hs, int hstride, const int* __restrict__ hbase) {
   constexpr int SM = lds_stride((MAXH + 1) / 2);
+  constexpr int KL = half2_odd_lds<T, MAXH>(), KS = KL + 1;
   static_assert(16 * (MAXH + 1) <= 36 * SM, "face exchange fits the slice");
+  static_assert(36 * (2 * SM + 2 * KS) * (int)sizeof(T) <= kLdsBytes, "slices and odd rows fit the LDS");
   extern __shared__ __align__(16) unsigned char smem_raw[];
   // which chain of the pair: the same for a whole wavefront (MAXH is a multiple
   // of 64), so kept in scalar registers, as is everything derived from it
   const int sub = __builtin_amdgcn_readfirstlane(threadIdx.x >= MAXH ? 1 : 0);
   T* lds = reinterpret_cast<T*>(smem_raw) + sub * 36 * SM;
+  // the pair's odd rows kept in LDS, after both slices
+  T* ol = reinterpret_cast<T*>(smem_raw) + 2 * 36 * SM + sub * 36 * KS;
   const int t = threadIdx.x - sub * MAXH;
   const int npair = (nch + 1) / 2;
   const int vb = xcd_block(blockIdx.x, gridDim.x);
@@ -2533,9 +2558,9 @@ __global__ __launch_bounds__(2 * MAXH) __attribute__((amdgpu_waves_per_eu(3))) v
     const int NL = max(c0 >= 0 ? ct.ncell[c0] : 0, c1 >= 0 ? ct.ncell[c1] : 0);
     if (NL == 0) continue;  // the same for the whole workgroup
     // the Thomas finish on two waves per chain: waves 0-1 and 6-7, four SIMDs
-    adi_fast_chain_half<T, kFinalOpen, SM, MAXH + 1, JQ>(ct, js, __builtin_amdgcn_readfirstlane(sub ? c1 : c0), NL, lds,
-                                                     t, MAXH, 2, ff, nProc, cfl, sigma,
-                                                     r, dq, dqmode, ncr, hs, hstride, hbase);
+    adi_fast_chain_half<T, kFinalOpen, SM, MAXH + 1, JQ, KL, KS>(ct, js, __builtin_amdgcn_readfirstlane(sub ? c1 : c0),
+                                                                 NL, lds, t, MAXH, 2, ff, nProc, cfl, sigma, r, dq,
+                                                                 dqmode, ncr, hs, hstride, hbase, ol);
     __syncthreads();
   }
 }
@@ -3476,8 +3501,10 @@ int launch_group(hamstr_ctx* x, const Group& g, int grid, int dqmode, int ncr, i
     if (extra > 0 && (half || g.two)) return fail(HAMSTR_ERR_ARG, "cell work rides in the one-row kernel only");
     const int nc = depth_code((g.two || half) ? std::max(ncr, 1) : ncr,
                               (g.final == kFinalOpen && !g.two) ? g.tpcr : -1);
+    // half2: two slices, then each chain's odd rows kept in LDS
+    const size_t kodd = x->lhs32() ? half2_odd_lds<float, kHalfMaxRows>() : half2_odd_lds<double, kHalfMaxRows>();
     const size_t lds = (x->lhs32() ? sizeof(float) : sizeof(double)) * 36 *
-                       (half ? (g.kern == kKernHalf2 ? 2 * (size_t)lds_stride((kHalfMaxRows + 1) / 2)
+                       (half ? (g.kern == kKernHalf2 ? 2 * (size_t)lds_stride((kHalfMaxRows + 1) / 2) + 2 * (kodd + 1)
                                                      : (size_t)lds_stride((half_maxt(g.block) + 1) / 2))
                              : (size_t)lds_stride(g.two ? (g.maxN + 1) / 2 : g.maxN));
 #define FASTJ(T, F, M, J)                                                                                          \
