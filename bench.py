@@ -86,7 +86,7 @@ def cpu_model():
     return platform.processor() or platform.machine()
 
 
-JAC_NAMES = ("stored", "q", "states")   # HAMSTR_JAC_STORED, _FROM_Q, _FROM_STATES
+JAC_NAMES = ("stored", "q", "states", "states_lines")   # HAMSTR_JAC_STORED, _FROM_Q, _FROM_STATES, _STATES_LINES
 
 
 def measured_traffic(stage, nc, nr, order, lhs, adi, tri=0, jsrc=0):
@@ -220,7 +220,7 @@ def main():
     ap.add_argument("--max-iters", type=int, default=20000)
     ap.add_argument("--no-converge", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default launch shapes (FAST)")
-    ap.add_argument("--jacobians", choices=["auto", "stored", "states", "q"], default="auto",
+    ap.add_argument("--jacobians", choices=["auto", "stored", "states", "q", "states_lines"], default="auto",
                     help="where the FAST ADI kernels take the chain faces' Jacobians (auto: the autotuner's "
                          "choice; a fixed source keeps profiled runs on the same kernels)")
     ap.add_argument("--no-fp32", action="store_true")
@@ -305,7 +305,7 @@ def main():
         hip.autotune(s)
         if args.jacobians != "auto":
             hip.set_jacobian_source(s, {"stored": hip.JAC_STORED, "states": hip.JAC_FROM_STATES,
-                                        "q": hip.JAC_FROM_Q}[args.jacobians])
+                                        "q": hip.JAC_FROM_Q, "states_lines": hip.JAC_STATES_LINES}[args.jacobians])
         s.sync()
         return time.perf_counter() - t0
 
@@ -370,7 +370,8 @@ def main():
         "ms_per_step_eager_with_events": dt_ev / n * 1e3,
         "l2_after_timed_steps": l2,
         "autotune_seconds": tune_s[0],
-        "jacobian_source": ["stored", "from q", "from stored face states"][tune_s[1]],
+        "jacobian_source": ["stored", "from q", "from stored face states",
+                            "stored for phase 0, from stored face states for the others"][tune_s[1]],
     }
     if not args.no_converge:
         c64, q64 = converge(hip.LHS_FP64)

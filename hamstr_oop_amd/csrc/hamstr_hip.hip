@@ -294,15 +294,16 @@ __device__ __forceinline__ void face_jac_q(const JacSrc& js, int p, TL* lm, TL* 
 // FASTJ: the FAST-mode Jacobians (reciprocals instead of divisions); the flux
 // and spectral radius are the reference's arithmetic either way
 // ff_from: the Jacobians are computed and stored for processed faces p >=
-// ff_from and the last faces of open chains only (the fused FAST iteration
-// can leave the other chain faces' to its ADI kernels, which compute them
-// again); fst non-null: those other faces' two states are stored instead
+// ff_from or p < jac_to and the last faces of open chains only (the fused
+// FAST iteration can leave the other chain faces' to its ADI kernels, which
+// compute them again); fst non-null: those other faces' two states are
+// stored instead
 // ([8][nProc], left then right) for the ADI kernels to compute them from
 template <int ORDER, typename TL, bool IMPL, bool FASTJ>
 __global__ __launch_bounds__(kBlock) void k_rhs_faces(FaceTab ft, Consts c, Live lv, const double* __restrict__ q,
                                                       double* __restrict__ flux, double* __restrict__ sr,
                                                       TL* __restrict__ ff, int ff_from, double* __restrict__ fst,
-                                                      int* __restrict__ nanflag) {
+                                                      int* __restrict__ nanflag, int jac_to = 0) {
   const int p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (p >= c.nProc || (lv.face && !lv.face[p])) return;
   const int code = ft.pslot[p];
@@ -314,7 +315,7 @@ __global__ __launch_bounds__(kBlock) void k_rhs_faces(FaceTab ft, Consts c, Live
   *reinterpret_cast<double4*>(flux + 4 * (size_t)p) = make_double4(fl[0], fl[1], fl[2], fl[3]);
   sr[p] = s_r;
   if (s_r != s_r) atomicOr(nanflag, 1);
-  const bool store_jac = p >= ff_from || ((code >> 30) & 1);
+  const bool store_jac = p >= ff_from || p < jac_to || ((code >> 30) & 1);
   if (IMPL && fst && !store_jac) {
 #pragma unroll
     for (int k = 0; k < 4; k++) {
@@ -2757,7 +2758,7 @@ __global__ __launch_bounds__(kBlock) void k_lin_rhs_faces(FaceTab ft, Consts c, 
                                                           const double* __restrict__ dq,
                                                           double* __restrict__ lflux, double* __restrict__ flux,
                                                           double* __restrict__ sr, TL* __restrict__ ff, int ff_from,
-                                                          double* __restrict__ fst, int* __restrict__ nanflag) {
+                                                          double* __restrict__ fst, int* __restrict__ nanflag, int jac_to) {
   const int p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (p >= c.nProc || (lv.face && !lv.face[p])) return;
   const int code = ft.pslot[p];
@@ -2827,7 +2828,7 @@ __global__ __launch_bounds__(kBlock) void k_lin_rhs_faces(FaceTab ft, Consts c, 
   *reinterpret_cast<double4*>(flux + 4 * (size_t)p) = make_double4(fl[0], fl[1], fl[2], fl[3]);
   sr[p] = s_r;
   if (s_r != s_r) atomicOr(nanflag, 1);
-  const bool store_jac = p >= ff_from || ((code >> 30) & 1);
+  const bool store_jac = p >= ff_from || p < jac_to || ((code >> 30) & 1);
   if (fst && !store_jac) {
 #pragma unroll
     for (int k = 0; k < 4; k++) {
@@ -3293,7 +3294,10 @@ struct hamstr_ctx {
   // where the fused FAST iteration's ADI kernels take the chain faces'
   // Jacobians from (HAMSTR_JAC_*; chosen by hamstr_autotune)
   int jac_src = HAMSTR_JAC_STORED;
-  double* fst = nullptr;  // HAMSTR_JAC_FROM_STATES: [8][nProc] face states
+  double* fst = nullptr;  // HAMSTR_JAC_FROM_STATES / _STATES_LINES: [8][nProc] face states
+  // HAMSTR_JAC_STATES_LINES: the processed faces below jac_to are phase 0's
+  // (its chains come first); 0: that source is not available
+  int jac_to = 0;
   int nPhase = 0;
   bool last_phase_all_cells = false;
   // single case: ADI phase 0 covers exactly the cells [0, split_cells), whole
@@ -3347,13 +3351,13 @@ namespace {
 
 // ff_from: the first processed face whose Jacobians are stored (0: all;
 // nChainProc: the orphan faces only, for ADI kernels that compute the chain
-// faces' Jacobians again)
-int launch_rhs_faces(hamstr_ctx* x, bool implicit, int ff_from = 0, double* fst = nullptr) {
+// faces' Jacobians again); the faces below jac_to are stored as well
+int launch_rhs_faces(hamstr_ctx* x, bool implicit, int ff_from = 0, double* fst = nullptr, int jac_to = 0) {
   const FaceTab ft = x->ftab();
   dim3 g(nblk(x->nProc)), b(kBlock);
 #define RHSF(O, T, I, F)                                                                                         \
   hipLaunchKernelGGL((k_rhs_faces<O, T, I, F>), g, b, 0, x->stream, ft, x->c, x->live(), x->q, x->flux, x->sr, (T*)x->ff, \
-                     ff_from, fst, x->nanflag)
+                     ff_from, fst, x->nanflag, jac_to)
 #define RHSO(T, I, F) \
   if (o == 1) RHSF(1, T, I, F); else if (o == 3) RHSF(3, T, I, F); else RHSF(5, T, I, F)
   const int o = x->prm.order;
@@ -3467,6 +3471,9 @@ int jac_source(const hamstr_ctx* x) {
   return jac_recompute_possible(x) ? x->jac_src : HAMSTR_JAC_STORED;
 }
 
+// whether the face kernels store face states for source rj
+bool jac_states(int rj) { return rj == HAMSTR_JAC_FROM_STATES || rj == HAMSTR_JAC_STATES_LINES; }
+
 bool jac_recompute_possible(const hamstr_ctx* x) {
   if (x->prm.adi_mode != HAMSTR_ADI_FAST || x->groups.empty()) return false;
   for (const Group& g : x->groups)
@@ -3481,6 +3488,9 @@ bool jac_recompute_possible(const hamstr_ctx* x) {
 // extra == 0: none)
 int launch_group(hamstr_ctx* x, const Group& g, int grid, int dqmode, int ncr, int jsrc = HAMSTR_JAC_STORED,
                  const CellWork* cells = nullptr, int extra = 0) {
+  // HAMSTR_JAC_STATES_LINES: phase 0 reads its stored Jacobians, the other
+  // phases compute theirs from the stored face states
+  if (jsrc == HAMSTR_JAC_STATES_LINES) jsrc = g.phase == 0 ? HAMSTR_JAC_STORED : HAMSTR_JAC_FROM_STATES;
   const bool recompute = jsrc != HAMSTR_JAC_STORED;
   const ChainTab ct = x->ctab();
   const int* chains = x->group_chains + g.off;
@@ -4004,6 +4014,18 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
   x->nGhost = nGhost;
   x->nCase = nCase;
   x->nPhase = nPhase;
+  {
+    // HAMSTR_JAC_STATES_LINES needs phase 0's chains first: their processed
+    // faces are then [0, jac_to)
+    int p0_end = 0, rest_start = nChainProc;
+    for (int i = 0; i < nChain; i++) {
+      if (phase[i] == 0)
+        p0_end = std::max(p0_end, ch_pbase[i] + ch_nf[i]);
+      else
+        rest_start = std::min(rest_start, ch_pbase[i]);
+    }
+    x->jac_to = (nPhase >= 2 && p0_end <= rest_start) ? p0_end : 0;
+  }
   x->groups = groups;
   x->proc_face = pface;
   x->case_off_h = case_off;
@@ -4652,7 +4674,8 @@ int enqueue_rhs_sweep0(hamstr_ctx* x, double* nb, const std::function<int(const 
   // or from q; k_rhs_faces then stores the orphan faces' Jacobians only
   if (!rhs_done) {
     if ((rc = launch_rhs_faces(x, true, rj != HAMSTR_JAC_STORED ? x->nChainProc : 0,
-                               rj == HAMSTR_JAC_FROM_STATES ? x->fst : nullptr)))
+                               jac_states(rj) ? x->fst : nullptr,
+                               rj == HAMSTR_JAC_STATES_LINES ? x->jac_to : 0)))
       return rc;
     if (mark && mark("rhs_faces")) return HAMSTR_ERR_HIP;
   }
@@ -4691,11 +4714,12 @@ int launch_lin_rhs(hamstr_ctx* x, int rj, double* p_lin, double* p_nl_next,
   const FaceTab ft = x->ftab();
   const Live lv = x->live();
   const int ff_from = rj != HAMSTR_JAC_STORED ? x->nChainProc : 0;
-  double* fst = rj == HAMSTR_JAC_FROM_STATES ? x->fst : nullptr;
+  double* fst = jac_states(rj) ? x->fst : nullptr;
+  const int jac_to = rj == HAMSTR_JAC_STATES_LINES ? x->jac_to : 0;
   const dim3 g(nblk(x->nProc)), b(kBlock);
 #define LRF(O, T, F)                                                                                         \
   hipLaunchKernelGGL((k_lin_rhs_faces<O, T, F>), g, b, 0, x->stream, ft, x->c, lv, x->q, x->dq, x->lflux,      \
-                     x->flux, x->sr, (T*)x->ff, ff_from, fst, x->nanflag)
+                     x->flux, x->sr, (T*)x->ff, ff_from, fst, x->nanflag, jac_to)
 #define LRO(T, F) \
   if (o == 1) { LRF(1, T, F); } else if (o == 3) { LRF(3, T, F); } else { LRF(5, T, F); }
   const int o = x->prm.order;
@@ -5026,12 +5050,15 @@ int ensure_face_states(hamstr_ctx* x) {
 
 int hamstr_set_jacobian_source(hamstr_ctx* x, int src) {
   if (!x) return fail(HAMSTR_ERR_ARG, "null context");
-  if (src != HAMSTR_JAC_STORED && src != HAMSTR_JAC_FROM_Q && src != HAMSTR_JAC_FROM_STATES)
+  if (src != HAMSTR_JAC_STORED && src != HAMSTR_JAC_FROM_Q && src != HAMSTR_JAC_FROM_STATES &&
+      src != HAMSTR_JAC_STATES_LINES)
     return fail(HAMSTR_ERR_ARG, "unknown Jacobian source");
+  if (src == HAMSTR_JAC_STATES_LINES && x->jac_to <= 0)
+    return fail(HAMSTR_ERR_ARG, "phase 0's chains do not come first, or there is one phase only");
   if (src != HAMSTR_JAC_STORED && !jac_recompute_possible(x))
     return fail(HAMSTR_ERR_ARG, "the Jacobians can be computed again only by FAST kernels other than the "
                                 "long-chain one, and not for FP32 odd cycles");
-  if (src == HAMSTR_JAC_FROM_STATES) {
+  if (jac_states(src)) {
     const int rc = ensure_face_states(x);
     if (rc) return rc;
   }
@@ -5168,8 +5195,12 @@ int hamstr_autotune(hamstr_ctx* x, int reps, double* best_ms, int cap, int* ngro
   HIPCHK(hipGetDeviceProperties(&prop, x->device));
   const int cu = std::max(prop.multiProcessorCount, 1);
   const size_t c4 = sizeof(double) * 4 * x->nCell;
-  double *r_save = nullptr, *dq_save = nullptr, *r0_save = nullptr;
-  bool saved = false;  // r, r0 and dq copied out: restore them
+  double *r_save = nullptr, *dq_save = nullptr, *r0_save = nullptr, *q_save = nullptr, *sg_save = nullptr;
+  int* nan_save = nullptr;
+  // r, r0, dq, q, sigma and the NaN flag copied out: restore them (the
+  // trials of the fused iteration step the state, and a case that blows up
+  // within them must not leave the flag set)
+  bool saved = false;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   // the trials; a failure stops them, and r, dq and the buffers are still
   // restored and released below
@@ -5177,9 +5208,15 @@ int hamstr_autotune(hamstr_ctx* x, int reps, double* best_ms, int cap, int* ngro
     HIPCHK(hipMalloc(&r_save, c4));
     HIPCHK(hipMalloc(&dq_save, c4));
     HIPCHK(hipMalloc(&r0_save, c4));
+    HIPCHK(hipMalloc(&q_save, c4));
+    HIPCHK(hipMalloc(&sg_save, c4 / 4));
+    HIPCHK(hipMalloc(&nan_save, sizeof(int)));
     HIPCHK(hipMemcpyAsync(r_save, x->r, c4, hipMemcpyDeviceToDevice, x->stream));
     HIPCHK(hipMemcpyAsync(dq_save, x->dq, c4, hipMemcpyDeviceToDevice, x->stream));
     HIPCHK(hipMemcpyAsync(r0_save, x->r0, c4, hipMemcpyDeviceToDevice, x->stream));
+    HIPCHK(hipMemcpyAsync(q_save, x->q, c4, hipMemcpyDeviceToDevice, x->stream));
+    HIPCHK(hipMemcpyAsync(sg_save, x->sigma, c4 / 4, hipMemcpyDeviceToDevice, x->stream));
+    HIPCHK(hipMemcpyAsync(nan_save, x->nanflag, sizeof(int), hipMemcpyDeviceToDevice, x->stream));
     saved = true;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
@@ -5286,22 +5323,40 @@ int hamstr_autotune(hamstr_ctx* x, int reps, double* best_ms, int cap, int* ngro
     if (jac_recompute_possible(x)) {
       // the fused iteration stores every face's Jacobians and the ADI kernels
       // read them, or it stores the orphan faces' only and the ADI kernels
-      // compute the chain faces' again, from stored face states or from q.
-      // Each source's groups are tuned for it; then computeRHS and the first
-      // sweep run as the fused iteration runs them (kernel times measured
-      // one by one leave out how each kernel finds the caches), and the
-      // fastest wins.  r, r0 and dq are restored below.
+      // compute the chain faces' again, from stored face states or from q
+      // (or phase 0 reads stored ones and the other phases use states).
+      // Each source's groups are tuned for it; then the iteration runs as a
+      // captured block runs it (kernel times measured one by one leave out
+      // how each kernel finds the caches), and the fastest wins: with fused
+      // iterations their steady state, computeLinearRHS and the update fused
+      // with the next computeRHS, then the first sweep; else computeRHS and
+      // the first sweep.  r, r0, dq, q and sigma are restored below.
       if ((rc = ensure_face_states(x))) return rc;
       const std::vector<Group> orig = x->groups;
+      const bool fused = can_fuse(x);
       auto seq = [&](int src, const std::vector<Group>& g, double* ms) -> int {
         x->groups = g;
-        const int rc2 = timed([&]() { return enqueue_rhs_sweep0(x, nullptr, nullptr, x->partial, src); }, ms);
+        int rc2;
+        if (fused) {
+          // the source's stored arrays and a valid dq first, then the steady state
+          rc2 = enqueue_rhs_sweep0(x, nullptr, nullptr, x->partial, src);
+          if (!rc2)
+            rc2 = timed(
+                [&]() {
+                  const int rc3 = launch_lin_rhs(x, src, x->partial, x->partial, nullptr);
+                  return rc3 ? rc3 : enqueue_rhs_sweep0(x, nullptr, nullptr, x->partial, src, true);
+                },
+                ms);
+        } else {
+          rc2 = timed([&]() { return enqueue_rhs_sweep0(x, nullptr, nullptr, x->partial, src); }, ms);
+        }
         x->groups = orig;
         return rc2;
       };
       double best = 0;
       if ((rc = seq(HAMSTR_JAC_STORED, gs, &best))) return rc;
-      for (int src : {HAMSTR_JAC_FROM_STATES, HAMSTR_JAC_FROM_Q}) {
+      for (int src : {HAMSTR_JAC_FROM_STATES, HAMSTR_JAC_FROM_Q, HAMSTR_JAC_STATES_LINES}) {
+        if (src == HAMSTR_JAC_STATES_LINES && x->jac_to <= 0) continue;
         std::vector<Group> gq = orig;
         std::vector<double> perq;
         double sumq = 0, t = 0;
@@ -5314,7 +5369,9 @@ int hamstr_autotune(hamstr_ctx* x, int reps, double* best_ms, int cap, int* ngro
           per.swap(perq);
         }
       }
-      // every face's Jacobians stored again, as hamstr_compute_rhs leaves them
+      // the state the trials stepped restored, and every face's Jacobians
+      // stored again for it, as hamstr_compute_rhs leaves them
+      HIPCHK(hipMemcpyAsync(x->q, q_save, c4, hipMemcpyDeviceToDevice, x->stream));
       if ((rc = launch_rhs_faces(x, true, 0))) return rc;
     }
     x->groups = gs;
@@ -5327,11 +5384,17 @@ int hamstr_autotune(hamstr_ctx* x, int reps, double* best_ms, int cap, int* ngro
     (void)hipMemcpyAsync(x->r, r_save, c4, hipMemcpyDeviceToDevice, x->stream);
     (void)hipMemcpyAsync(x->dq, dq_save, c4, hipMemcpyDeviceToDevice, x->stream);
     (void)hipMemcpyAsync(x->r0, r0_save, c4, hipMemcpyDeviceToDevice, x->stream);
+    (void)hipMemcpyAsync(x->q, q_save, c4, hipMemcpyDeviceToDevice, x->stream);
+    (void)hipMemcpyAsync(x->sigma, sg_save, c4 / 4, hipMemcpyDeviceToDevice, x->stream);
+    (void)hipMemcpyAsync(x->nanflag, nan_save, sizeof(int), hipMemcpyDeviceToDevice, x->stream);
   }
   const hipError_t se = hipStreamSynchronize(x->stream);
   if (r_save) (void)hipFree(r_save);
   if (dq_save) (void)hipFree(dq_save);
   if (r0_save) (void)hipFree(r0_save);
+  if (q_save) (void)hipFree(q_save);
+  if (sg_save) (void)hipFree(sg_save);
+  if (nan_save) (void)hipFree(nan_save);
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   if (rc) return rc;

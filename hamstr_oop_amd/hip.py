@@ -494,13 +494,15 @@ def set_adi_kernel(solver: Solver, group: int, kernel: int):
     _chk(lib().hamstr_set_adi_kernel(solver.h, int(group), int(kernel)))
 
 
-JAC_STORED, JAC_FROM_Q, JAC_FROM_STATES = 0, 1, 2
+JAC_STORED, JAC_FROM_Q, JAC_FROM_STATES, JAC_STATES_LINES = 0, 1, 2, 3
 
 
 def set_jacobian_source(solver: Solver, source: int):
     """Fused FAST iteration: the ADI kernels read the chain faces' stored
     Jacobians (JAC_STORED, the default) or compute them again from stored face
-    states (JAC_FROM_STATES) or from q (JAC_FROM_Q)."""
+    states (JAC_FROM_STATES) or from q (JAC_FROM_Q); JAC_STATES_LINES: phase 0
+    reads stored Jacobians and the other phases use stored states (where phase
+    0's chains come first)."""
     _chk(lib().hamstr_set_jacobian_source(solver.h, int(source)))
 
 

@@ -340,7 +340,9 @@ def time_batch_iteration(n_cases: int = 256, lhs: int = hip.LHS_FP32, warmup: in
         b.set_timing(False)
         return {"cases": n_cases, "cells": int(b.cell_off[-1]), "lhs": "fp32" if lhs == hip.LHS_FP32 else "fp64",
                 "ms_per_iter": round(ms, 4), "stage_ms": {k: round(v, 4) for k, v in stages.items()},
-                "jacobian_source": ["stored", "from q", "from stored face states"][hip.jacobian_source(b)],
+                "jacobian_source": ["stored", "from q", "from stored face states",
+                                    "stored for phase 0, from stored face states for the others"][
+                                        hip.jacobian_source(b)],
                 "groups": [{k: g[k] for k in ("phase", "kernel", "final", "chains", "max_rows", "grid", "cr_levels")}
                            for g in groups]}
     finally:

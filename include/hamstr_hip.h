@@ -208,7 +208,10 @@ int hamstr_set_adi_kernel(hamstr_ctx *ctx, int group, int kernel);
  * entries and the ADI kernels read them.  HAMSTR_JAC_FROM_STATES: it stores
  * each chain face's two reconstructed states (8 values) and the ADI kernels
  * compute the Jacobians from them.  HAMSTR_JAC_FROM_Q: the ADI kernels
- * gather and reconstruct the states again from q.  The computed blocks are
+ * gather and reconstruct the states again from q.  HAMSTR_JAC_STATES_LINES:
+ * phase 0's chain faces keep stored Jacobians and the other phases' use
+ * stored states (where phase 0's chains come first, e.g. the loops of an
+ * O-grid).  The computed blocks are
  * the stored ones bit for bit; orphan faces and the last faces of open
  * chains keep stored Jacobians.  hamstr_autotune picks the fastest.  The
  * computed sources are refused (HAMSTR_ERR_ARG) where a group's kernel cannot
@@ -218,6 +221,7 @@ int hamstr_set_adi_kernel(hamstr_ctx *ctx, int group, int kernel);
 #define HAMSTR_JAC_STORED 0
 #define HAMSTR_JAC_FROM_Q 1
 #define HAMSTR_JAC_FROM_STATES 2
+#define HAMSTR_JAC_STATES_LINES 3
 int hamstr_set_jacobian_source(hamstr_ctx *ctx, int source);
 int hamstr_jacobian_source(const hamstr_ctx *ctx, int *source);
 

@@ -44,7 +44,7 @@ def _pair(hip, mesh, order, lhs, msweep=1):
     return (hip.Solver(mesh, 0.5, 2.0, order, 10.0, **kw), hip.Solver(mesh, 0.5, 2.0, order, 10.0, **kw))
 
 
-SOURCES = {"states": 2, "q": 1}
+SOURCES = {"states": 2, "q": 1, "states_lines": 3}
 
 
 @pytest.mark.parametrize("src", list(SOURCES))
@@ -150,3 +150,44 @@ def test_refused_where_a_kernel_cannot_compute_them(hip):
     hip.set_jacobian_source(s, hip.JAC_STORED)
     assert hip.jacobian_source(s) == hip.JAC_STORED
     s.close()
+
+
+@pytest.mark.parametrize("batch", [False, True])
+def test_autotuner_restores_the_state(hip, batch):
+    """The autotuner times the fused iteration's steady state, which steps the
+    solution: q, r, r0, dq, sigma and (single case) the stored Jacobians come
+    back as they were (bitwise), and the next steps equal those of an untuned
+    context given the tuned launch shapes and source."""
+    from hamstr_oop_amd import meshgen
+    cases = [(meshgen.naca_ogrid(64, 24, tri_layers=8), 0.5, 2.0), (meshgen.circle_ogrid(32, 12), 0.3, 0.0)]
+    if batch:
+        mk = lambda: hip.Batch(cases, 3, 10.0, adi_mode=hip.ADI_FAST)  # noqa: E731
+    else:
+        mk = lambda: hip.Solver(cases[0][0], cases[0][1], cases[0][2], 3, 10.0, adi_mode=hip.ADI_FAST)  # noqa: E731
+    a, ref = mk(), mk()
+    for s in (a, ref):
+        s.step(5, check_every=5)
+        s.computeRHS("implicit")
+    fields = ("q", "r", "r0", "dq", "sigma") + (() if batch else ("ff",))   # a batch has no per-face read-back
+    before = {f: a.get(f).copy() for f in fields}
+    print(f"autotuner state restore, batch={batch}", flush=True)
+    hip.autotune(a)
+    for f, v in before.items():
+        _eq(a.get(f), v, f"{f} after the autotuner")
+    for k, g in enumerate(hip.launch_groups(a)):
+        if g["kernel"] in (hip.KERNEL_ROW, hip.KERNEL_HALF, hip.KERNEL_HALF2):
+            hip.set_adi_kernel(ref, k, g["kernel"])
+        if g["fast"]:
+            hip.set_launch(ref, k, g["grid"], g["cr_levels"])
+        if g["thomas"] >= 0:
+            hip.set_thomas(ref, k, g["thomas"])
+    if hip.jacobian_source(a) != hip.JAC_STORED:
+        hip.set_jacobian_source(ref, hip.jacobian_source(a))
+    for s in (a, ref):
+        if batch:
+            s.batch_step(12, check_every=6)
+        else:
+            s.step(12, check_every=6)
+    _eq(a.get("q"), ref.get("q"), "q 12 steps after tuning")
+    a.close()
+    ref.close()
