@@ -140,9 +140,12 @@ def counter_summary(args, fetch_dir, write_dir, trace_dir, timed=20):
     fk, wk = max(fetch[copy]), max(write[copy])
     fs, ws = (256 << 10) / fk, (256 << 10) / wk
     src = args.jacobians
-    jq = "true" if src in ("states", "q") else "false"
+    # the ADI kernels compute the Jacobians (JQ) for a computed source; with
+    # states_lines phase 0 reads stored ones
+    jq1 = "true" if src in ("states", "q", "states_lines") else "false"
+    jq0 = "true" if src in ("states", "q") else "false"
     o, t = args.order, "double"
-    eager = {"adi_phase1": f"k_adi_fast_half2<{t}, 384, {jq}>", "adi_phase0": f"k_adi_fast<{t}, 1, 512, {jq}>",
+    eager = {"adi_phase1": f"k_adi_fast_half2<{t}, 384, {jq1}>", "adi_phase0": f"k_adi_fast<{t}, 1, 512, {jq0}>",
              "lin_faces": f"k_lin_faces<{o}, {t}, true, true>", "rhs_faces": f"k_rhs_faces<{o}, {t}, true, true>",
              "lin_cells": "k_lin_cells<true>", "rhs_cells": "k_rhs_cells<true>", "norms": "k_norm_slots"}
     graph = {"adi_phase1": (eager["adi_phase1"], 2 * timed, timed), "adi_phase0": (eager["adi_phase0"], 2 * timed, timed),
