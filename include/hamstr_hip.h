@@ -158,8 +158,10 @@ long hamstr_iterations(const hamstr_ctx *ctx);
  * persistent grid sizes (all chains; 8, 4, 2, 1 workgroups per CU), 0 to 3
  * levels of cyclic reduction before the parallel cyclic reduction and, where
  * the group's chains fit it, both one-row-per-thread kernels (all rows in
- * LDS, or half of them: HAMSTR_KERNEL_*), on the current state; keeps the
- * fastest and restores r and dq (on an error as well).  Per kernel it times
+ * LDS, or half of them: HAMSTR_KERNEL_*), on the current state, and where
+ * the Jacobians can be computed again the source (HAMSTR_JAC_*) by timing
+ * the iteration as a captured block runs it; keeps the fastest and restores
+ * q, r, r0, dq, sigma and the NaN flag (on an error as well).  Per kernel it times
  * every grid at the group's current depth, then every other depth at the
  * best grid.  best_ms[k] (k < cap) gets group k's best time (-1 for groups
  * with nothing to tune). */
