@@ -92,6 +92,7 @@ struct FaceTab {
   const double* nvec;    // [nProc][2]
   const double* refmtx;  // [nFace][4]
   const double4* qinf;   // [nCase] freestream state per flow case
+  int nslot;             // slots in src
 };
 
 // Retired cases of a batch: one byte per processed face, cell and chain, zero
@@ -2751,26 +2752,70 @@ __device__ __forceinline__ void face_states_sum(const FaceTab& ft, const double*
   }
 }
 
+// slots a face's stencil reads, relative to its own slot s: s-2+j for j in
+// [lo, hi] of reconstruct_face's table (orphan faces: s and s+1)
+constexpr __host__ __device__ int stencil_lo(int order) { return order == 5 ? -2 : (order == 3 ? -1 : 0); }
+constexpr __host__ __device__ int stencil_hi(int order) { return order == 5 ? 3 : (order == 3 ? 2 : 1); }
+// LDS capacity, in slots, of the staged fused face kernel
+constexpr int kStageSlots = 520;
+
 // k_lin_faces<ORDER, TL, FASTJ, true> on (q, dq), then k_rhs_faces<ORDER, TL,
-// true, FASTJ> on q + dq (ff_from, fst as there)
-template <int ORDER, typename TL, bool FASTJ>
+// true, FASTJ> on q + dq (ff_from, fst as there).
+// STAGED: the workgroup's stencil rows are staged in LDS first.  Slots grow
+// with the processed face (chains in order, orphan faces after them), so the
+// stencils of a workgroup's 256 faces lie in one range of slots; each slot's
+// load_state, load_dstate and load_state_sum is formed once into LDS and the
+// faces read them there instead of each gathering them from global memory.
+// The host launches it only where every workgroup's range fits kStageSlots.
+
+template <int ORDER, typename TL, bool FASTJ, bool STAGED>
 __global__ __launch_bounds__(kBlock) void k_lin_rhs_faces(FaceTab ft, Consts c, Live lv, const double* __restrict__ q,
                                                           const double* __restrict__ dq,
                                                           double* __restrict__ lflux, double* __restrict__ flux,
                                                           double* __restrict__ sr, TL* __restrict__ ff, int ff_from,
                                                           double* __restrict__ fst, int* __restrict__ nanflag, int jac_to) {
-  const int p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int pf = xcd_block(blockIdx.x, gridDim.x) * blockDim.x;
+  const int p = pf + threadIdx.x;
+  // per staged slot: load_state (0-3), load_dstate (4-7), load_state_sum (8-11)
+  __shared__ double stg[STAGED ? 12 : 1][STAGED ? kStageSlots : 1];
+  int S0 = 0;
+  if constexpr (STAGED) {
+    const int pl = min(pf + (int)blockDim.x - 1, c.nProc - 1);
+    S0 = max((ft.pslot[pf] & kSlotMask) + stencil_lo(ORDER), 0);
+    const int S1 = min((ft.pslot[pl] & kSlotMask) + stencil_hi(ORDER), ft.nslot - 1);
+    for (int i = threadIdx.x; i <= S1 - S0; i += blockDim.x) {
+      const int2 e = ft.src[S0 + i];
+      const double4 a = load_state(e, q, ft.refmtx, ft.qinf);
+      const double4 b = load_dstate(e, dq, ft.refmtx);
+      const double4 u = load_state_sum(e, q, dq, ft.refmtx, ft.qinf);
+      stg[0][i] = a.x; stg[1][i] = a.y; stg[2][i] = a.z; stg[3][i] = a.w;
+      stg[4][i] = b.x; stg[5][i] = b.y; stg[6][i] = b.z; stg[7][i] = b.w;
+      stg[8][i] = u.x; stg[9][i] = u.y; stg[10][i] = u.z; stg[11][i] = u.w;
+    }
+    __syncthreads();
+  }
   if (p >= c.nProc || (lv.face && !lv.face[p])) return;
   const int code = ft.pslot[p];
   const int s = code & kSlotMask;
+  // slot row k of quantity w (0 state, 1 dstate, 2 state sum), staged or gathered
+  auto row = [&](int slot, int w, double* o) __attribute__((always_inline)) {
+    if constexpr (STAGED) {
+#pragma unroll
+      for (int k = 0; k < 4; k++) o[k] = stg[4 * w + k][slot - S0];
+    } else {
+      const int2 e = ft.src[slot];
+      put4(o, w == 0 ? load_state(e, q, ft.refmtx, ft.qinf)
+                     : (w == 1 ? load_dstate(e, dq, ft.refmtx) : load_state_sum(e, q, dq, ft.refmtx, ft.qinf)));
+    }
+  };
   const double2 n = *reinterpret_cast<const double2*>(ft.nvec + 2 * (size_t)p);
   {
     double dl[4], dr[4], left[4], right[4];
     if (code & kOrphanSlot) {
-      put4(dl, load_dstate(ft.src[s], dq, ft.refmtx));
-      put4(dr, load_dstate(ft.src[s + 1], dq, ft.refmtx));
-      put4(left, load_state(ft.src[s], q, ft.refmtx, ft.qinf));
-      put4(right, load_state(ft.src[s + 1], q, ft.refmtx, ft.qinf));
+      row(s, 1, dl);
+      row(s + 1, 1, dr);
+      row(s, 0, left);
+      row(s + 1, 0, right);
     } else {
       const bool flip = (code >> 30) & 1;
       constexpr int lo = (ORDER == 5) ? 0 : (ORDER == 3 ? 1 : 2);
@@ -2778,9 +2823,8 @@ __global__ __launch_bounds__(kBlock) void k_lin_rhs_faces(FaceTab ft, Consts c, 
       double f[6][4], d[6][4];
 #pragma unroll
       for (int j = lo; j <= hi; j++) {
-        const int2 e = ft.src[s - 2 + j];
-        put4(f[j], load_state(e, q, ft.refmtx, ft.qinf));
-        put4(d[j], load_dstate(e, dq, ft.refmtx));
+        row(s - 2 + j, 0, f[j]);
+        row(s - 2 + j, 1, d[j]);
       }
       double dln[4], drn1[4];
       reconstruct_dface<ORDER, FASTJ>(f, d, c.eps, c.qt, dln, drn1);
@@ -2822,7 +2866,29 @@ __global__ __launch_bounds__(kBlock) void k_lin_rhs_faces(FaceTab ft, Consts c, 
     *reinterpret_cast<double4*>(lflux + 4 * (size_t)p) = make_double4(out[0], out[1], out[2], out[3]);
   }
   double left[4], right[4];
-  face_states_sum<ORDER>(ft, q, dq, c.eps, c.qt, code, left, right);
+  if constexpr (STAGED) {
+    // face_states_sum on the staged rows
+    if (code & kOrphanSlot) {
+      row(s, 2, left);
+      row(s + 1, 2, right);
+    } else {
+      const bool flip = (code >> 30) & 1;
+      constexpr int lo = (ORDER == 5) ? 0 : (ORDER == 3 ? 1 : 2);
+      constexpr int hi = (ORDER == 5) ? 5 : (ORDER == 3 ? 4 : 3);
+      double f[6][4];
+#pragma unroll
+      for (int j = lo; j <= hi; j++) row(s - 2 + j, 2, f[j]);
+      double qln[4], qrn1[4];
+      reconstruct_face<ORDER>(f, c.eps, c.qt, qln, qrn1);
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        left[k] = flip ? qln[k] : qrn1[k];
+        right[k] = flip ? qrn1[k] : qln[k];
+      }
+    }
+  } else {
+    face_states_sum<ORDER>(ft, q, dq, c.eps, c.qt, code, left, right);
+  }
   double fl[4], s_r;
   flux_roe2d(n.x, n.y, left, right, c.gamma, fl, &s_r);
   *reinterpret_cast<double4*>(flux + 4 * (size_t)p) = make_double4(fl[0], fl[1], fl[2], fl[3]);
@@ -3268,6 +3334,9 @@ struct hamstr_ctx {
   void* ff = nullptr;
   int* pslot = nullptr;
   int2* src = nullptr;
+  // whether every workgroup's stencil slots fit kStageSlots, and whether the
+  // fused face kernel stages them in LDS (by default where they fit)
+  bool stage_fits = false, stage_faces = false;
   double *nvec = nullptr, *refmtx = nullptr;
   double4* qinf = nullptr;
   std::vector<double> qinf_h;  // [nCase][4]
@@ -3339,7 +3408,7 @@ struct hamstr_ctx {
   Live live() const {
     return (any_retired || retire) ? Live{face_on, cell_on, chain_on} : Live{nullptr, nullptr, nullptr};
   }
-  FaceTab ftab() const { return FaceTab{pslot, src, nvec, refmtx, qinf}; }
+  FaceTab ftab() const { return FaceTab{pslot, src, nvec, refmtx, qinf, nSlot}; }
   ChainTab ctab() const {
     return ChainTab{ch_pbase, ch_nf, ch_ncell, ch_periodic, pleft, pright, pface, refmtx,
                     od_chain, od_off, od_ent};
@@ -4011,6 +4080,25 @@ int create_internal(const hamstr_mesh_desc* mesh, const hamstr_params& P, const 
   x->nChainProc = nChainProc;
   x->nLong = (int)std::max<long>(nLong, 1);
   x->nSlot = nSlot;
+  {
+    // the staged fused face kernel's LDS range per workgroup, as that kernel
+    // forms it (first face's slot + lo .. last face's slot + hi), must hold
+    // every live stencil of its faces
+    const int lo = stencil_lo(P.order), hi = stencil_hi(P.order);
+    bool fits = nProc > 0;
+    for (int b0 = 0; fits && b0 < nProc; b0 += kBlock) {
+      const int b1 = std::min(b0 + kBlock, nProc) - 1;
+      const int S0 = std::max((pslot[b0] & kSlotMask) + lo, 0);
+      const int S1 = std::min((pslot[b1] & kSlotMask) + hi, nSlot - 1);
+      if (S1 - S0 + 1 > kStageSlots) fits = false;
+      for (int p = b0; fits && p <= b1; p++) {
+        const int sl = pslot[p] & kSlotMask;
+        const bool orphan = pslot[p] & kOrphanSlot;
+        if ((orphan ? sl : sl + lo) < S0 || (orphan ? sl + 1 : sl + hi) > S1) fits = false;
+      }
+    }
+    x->stage_fits = x->stage_faces = fits;
+  }
   x->nGhost = nGhost;
   x->nCase = nCase;
   x->nPhase = nPhase;
@@ -4718,8 +4806,12 @@ int launch_lin_rhs(hamstr_ctx* x, int rj, double* p_lin, double* p_nl_next,
   const int jac_to = rj == HAMSTR_JAC_STATES_LINES ? x->jac_to : 0;
   const dim3 g(nblk(x->nProc)), b(kBlock);
 #define LRF(O, T, F)                                                                                         \
-  hipLaunchKernelGGL((k_lin_rhs_faces<O, T, F>), g, b, 0, x->stream, ft, x->c, lv, x->q, x->dq, x->lflux,      \
-                     x->flux, x->sr, (T*)x->ff, ff_from, fst, x->nanflag, jac_to)
+  if (x->stage_faces)                                                                                          \
+    hipLaunchKernelGGL((k_lin_rhs_faces<O, T, F, true>), g, b, 0, x->stream, ft, x->c, lv, x->q, x->dq,        \
+                       x->lflux, x->flux, x->sr, (T*)x->ff, ff_from, fst, x->nanflag, jac_to);                 \
+  else                                                                                                         \
+    hipLaunchKernelGGL((k_lin_rhs_faces<O, T, F, false>), g, b, 0, x->stream, ft, x->c, lv, x->q, x->dq,       \
+                       x->lflux, x->flux, x->sr, (T*)x->ff, ff_from, fst, x->nanflag, jac_to)
 #define LRO(T, F) \
   if (o == 1) { LRF(1, T, F); } else if (o == 3) { LRF(3, T, F); } else { LRF(5, T, F); }
   const int o = x->prm.order;
@@ -5070,6 +5162,21 @@ int hamstr_set_jacobian_source(hamstr_ctx* x, int src) {
 int hamstr_jacobian_source(const hamstr_ctx* x, int* src) {
   if (!x || !src) return fail(HAMSTR_ERR_ARG, "null argument");
   *src = jac_source(x);
+  return HAMSTR_OK;
+}
+
+int hamstr_set_face_staging(hamstr_ctx* x, int on) {
+  if (!x) return fail(HAMSTR_ERR_ARG, "null context");
+  if (on && !x->stage_fits)
+    return fail(HAMSTR_ERR_ARG, "a workgroup's stencil slots exceed the staging capacity on this mesh");
+  x->stage_faces = on != 0;
+  drop_graph(x);
+  return HAMSTR_OK;
+}
+
+int hamstr_face_staging(const hamstr_ctx* x, int* on) {
+  if (!x || !on) return fail(HAMSTR_ERR_ARG, "null argument");
+  *on = x->stage_faces ? 1 : 0;
   return HAMSTR_OK;
 }
 

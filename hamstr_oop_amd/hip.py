@@ -46,6 +46,7 @@ EXPORTS = [
     "hamstr_write_solution", "hamstr_read_solution", "hamstr_block_tridag4_noalloc",
     "hamstr_flux_roe3d", "hamstr_jacobian_roe3d", "hamstr_block_tridag5", "hamstr_block_tridag_periodic5",
     "hamstr_set_jacobian_source", "hamstr_jacobian_source", "hamstr_set_thomas", "hamstr_thomas",
+    "hamstr_set_face_staging", "hamstr_face_staging",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -510,6 +511,19 @@ def jacobian_source(solver: Solver) -> int:
     src = C.c_int()
     _chk(lib().hamstr_jacobian_source(solver.h, C.byref(src)))
     return src.value
+
+
+def set_face_staging(solver: Solver, on: bool):
+    """Fused face kernel: stage each workgroup's stencil rows in LDS (the
+    default where they fit; HamstrError where they do not) or gather them
+    directly.  The same results either way."""
+    _chk(lib().hamstr_set_face_staging(solver.h, int(bool(on))))
+
+
+def face_staging(solver: Solver) -> bool:
+    on = C.c_int()
+    _chk(lib().hamstr_face_staging(solver.h, C.byref(on)))
+    return bool(on.value)
 
 
 def launch_groups(solver: Solver):

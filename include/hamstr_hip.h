@@ -226,6 +226,17 @@ int hamstr_set_adi_kernel(hamstr_ctx *ctx, int group, int kernel);
 #define HAMSTR_JAC_STATES_LINES 3
 int hamstr_set_jacobian_source(hamstr_ctx *ctx, int source);
 int hamstr_jacobian_source(const hamstr_ctx *ctx, int *source);
+/* Whether the fused iteration's face kernel stages each workgroup's stencil
+ * rows in LDS (no reference counterpart: a launch choice for the
+ * reconstruction of computeRHS.cpp:53-298 and computeLinearRHS.cpp:53-298).
+ * A workgroup of 256 processed faces reads the consecutive chain slots from
+ * its first face's stencil to its last one's; each slot's state, increment
+ * and updated state are formed once into LDS instead of gathered by every
+ * face that reads them.  On by default where every workgroup's range fits 520
+ * slots (chains of a few cells at order 5 may not: then 1 is refused with
+ * HAMSTR_ERR_ARG); 0 gathers directly.  The results are the same bit for bit. */
+int hamstr_set_face_staging(hamstr_ctx *ctx, int on);
+int hamstr_face_staging(const hamstr_ctx *ctx, int *on);
 
 /* SOLVER::initFlow (initFlow.cpp:12-93): q := freestream. */
 int hamstr_init_flow(hamstr_ctx *ctx);

@@ -64,6 +64,8 @@ def test_bad_arguments_are_rejected():
     assert L.hamstr_mlp_forward(None, 1, None, None) == hip.ERR_ARG
     assert L.hamstr_warm_start(None, None, None, None, None) == hip.ERR_ARG
     assert L.hamstr_write_solution(None, b"x", 0) == hip.ERR_ARG
+    assert L.hamstr_set_face_staging(None, 1) == hip.ERR_ARG
+    assert L.hamstr_face_staging(None, None) == hip.ERR_ARG
     assert L.hamstr_read_solution(None, b"x", None) == hip.ERR_ARG
     assert L.hamstr_set_scheme(None, 1) == hip.ERR_ARG
     assert L.hamstr_get_scheme(None) == -1

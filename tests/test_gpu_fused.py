@@ -175,3 +175,69 @@ def test_batches_and_sweeps_stay_unfused(hip):
         out.append(s.get("q"))
         s.close()
     _eq(out[0], out[1], "msweep 2")
+
+
+def _run(hip, mesh, order, lhs, staged, steps=24, graph=True):
+    s = hip.Solver(mesh, 0.5, 2.0, order, 10.0, lhs=lhs, adi_mode=hip.ADI_FAST)
+    if staged is not None:
+        hip.set_face_staging(s, staged)
+    s.set_graph(graph)
+    n, l2r, l2l = s.step(steps, check_every=steps)
+    assert n == steps
+    out = (s.get("q"), s.get("r"), s.get("sigma"), s.get("r0"), l2r, l2l)
+    s.close()
+    return out
+
+
+@pytest.mark.parametrize("order", [1, 3, 5])
+@pytest.mark.parametrize("lhs", ["fp64", "fp32"])
+def test_staged_faces_equal_gathered(hip, order, lhs):
+    """The fused face kernel with each workgroup's stencil rows staged in LDS
+    (the default on an O-grid) and with every face gathering its own give
+    the same iterations bit for bit."""
+    from hamstr_oop_amd import meshgen
+    mesh = meshgen.naca_ogrid(64, 24, tri_layers=8)
+    lm = hip.LHS_FP64 if lhs == "fp64" else hip.LHS_FP32
+    s = hip.Solver(mesh, 0.5, 2.0, order, 10.0, lhs=lm, adi_mode=hip.ADI_FAST)
+    assert hip.face_staging(s)
+    s.close()
+    print(f"staged/gathered order={order} {lhs}", flush=True)
+    a = _run(hip, mesh, order, lm, True)
+    b = _run(hip, mesh, order, lm, False)
+    for x, y, what in zip(a, b, ("q", "r", "sigma", "r0", "L2 history", "linear L2 history")):
+        _eq(np.asarray(x), np.asarray(y), what)
+
+
+def test_staged_faces_equal_gathered_batch(hip):
+    from hamstr_oop_amd import meshgen
+    cases = [(meshgen.naca_ogrid(32, 12), 0.8, 1.25), (meshgen.naca_ogrid(64, 16, t=0.14, tri_layers=4), 0.6, -3.0)]
+    qs = []
+    for staged in (True, False):
+        b = hip.Batch(cases, order=3, cfl=10.0, adi_mode=hip.ADI_FAST)
+        hip.set_face_staging(b, staged)
+        n, _, norms = b.batch_step(16, check_every=16)
+        assert n == 16
+        qs.append((b.get("q"), norms))
+        b.close()
+    _eq(qs[0][0], qs[1][0], "q")
+    _eq(qs[0][1], qs[1][1], "per-case norms")
+
+
+def test_staging_refused_where_the_slots_do_not_fit(hip):
+    """Radial lines of 3 cells at order 5: a workgroup of 256 line faces
+    spans 576 slots (64 lines of 9), beyond the 520 the kernel stages; the
+    fused iteration then gathers, still equal to the unfused one."""
+    from hamstr_oop_amd import meshgen
+    mesh = meshgen.naca_ogrid(128, 3)
+    s = hip.Solver(mesh, 0.5, 2.0, 5, 10.0, adi_mode=hip.ADI_FAST)
+    assert not hip.face_staging(s)
+    with pytest.raises(hip.HamstrError):
+        hip.set_face_staging(s, True)
+    s.close()
+    s = hip.Solver(mesh, 0.5, 2.0, 3, 10.0, adi_mode=hip.ADI_FAST)
+    assert hip.face_staging(s)
+    s.close()
+    a = _run(hip, mesh, 5, hip.LHS_FP64, None, steps=16, graph=True)
+    b = _run(hip, mesh, 5, hip.LHS_FP64, None, steps=16, graph=False)
+    for x, y, what in zip(a, b, ("q", "r", "sigma", "r0", "L2 history", "linear L2 history")):
+        _eq(np.asarray(x), np.asarray(y), what)
