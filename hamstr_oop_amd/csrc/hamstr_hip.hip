@@ -220,24 +220,44 @@ __device__ __forceinline__ void reconstruct_dface(const double (*f)[4], const do
   }
 }
 
+// slots a face's stencil reads, relative to its own slot s: s-2+j for j in
+// [lo, hi] of reconstruct_face's table (orphan faces: s and s+1)
+constexpr __host__ __device__ int stencil_lo(int order) { return order == 5 ? -2 : (order == 3 ? -1 : 0); }
+constexpr __host__ __device__ int stencil_hi(int order) { return order == 5 ? 3 : (order == 3 ? 2 : 1); }
+// LDS capacity, in slots, of the staged face kernels
+constexpr int kStageSlots = 520;
+
+// Staged face kernels: slots grow with the processed face (chains in order,
+// orphan faces after them), so the stencils of a workgroup's faces lie in one
+// range of slots, from its first face's stencil to its last one's.  Each
+// slot's rows (load_state, load_dstate, ...) are formed once into LDS and the
+// faces read them there instead of each gathering them from global memory.
+// The host launches them only where every workgroup's range fits kStageSlots.
+template <int ORDER>
+__device__ __forceinline__ int2 stage_range(const FaceTab& ft, int nProc, int pf) {
+  const int pl = min(pf + kBlock - 1, nProc - 1);
+  return make_int2(max((ft.pslot[pf] & kSlotMask) + stencil_lo(ORDER), 0),
+                   min((ft.pslot[pl] & kSlotMask) + stencil_hi(ORDER), ft.nslot - 1));
+}
+
 // The two states of processed face `code` (its pslot code) that its flux and
 // Jacobians take (computeRHS.cpp:53-260): for an orphan face the two cell
 // states (first order), else the states reconstructed along the chain,
-// oriented as the face is
-template <int ORDER>
-__device__ __forceinline__ void face_states(const FaceTab& ft, const double* __restrict__ q, double eps, double qt,
-                                            int code, double* left, double* right) {
+// oriented as the face is; get(slot, out4) gives a slot's state
+template <int ORDER, typename Get>
+__device__ __forceinline__ void face_states_from(Get get, double eps, double qt, int code, double* left,
+                                                 double* right) {
   const int s = code & kSlotMask;
   if (code & kOrphanSlot) {
-    put4(left, load_state(ft.src[s], q, ft.refmtx, ft.qinf));
-    put4(right, load_state(ft.src[s + 1], q, ft.refmtx, ft.qinf));
+    get(s, left);
+    get(s + 1, right);
   } else {
     const bool flip = (code >> 30) & 1;
     constexpr int lo = (ORDER == 5) ? 0 : (ORDER == 3 ? 1 : 2);
     constexpr int hi = (ORDER == 5) ? 5 : (ORDER == 3 ? 4 : 3);
     double f[6][4];
 #pragma unroll
-    for (int j = lo; j <= hi; j++) put4(f[j], load_state(ft.src[s - 2 + j], q, ft.refmtx, ft.qinf));
+    for (int j = lo; j <= hi; j++) get(s - 2 + j, f[j]);
     double qln[4], qrn1[4];
     reconstruct_face<ORDER>(f, eps, qt, qln, qrn1);
 #pragma unroll
@@ -246,6 +266,17 @@ __device__ __forceinline__ void face_states(const FaceTab& ft, const double* __r
       right[k] = flip ? qrn1[k] : qln[k];
     }
   }
+}
+
+// face_states_from with the states gathered from q
+template <int ORDER>
+__device__ __forceinline__ void face_states(const FaceTab& ft, const double* __restrict__ q, double eps, double qt,
+                                            int code, double* left, double* right) {
+  face_states_from<ORDER>(
+      [&](int slot, double* o) __attribute__((always_inline)) {
+        put4(o, load_state(ft.src[slot], q, ft.refmtx, ft.qinf));
+      },
+      eps, qt, code, left, right);
 }
 
 // Where a FAST ADI kernel takes the face Jacobians from.  order 0: the array
@@ -300,16 +331,37 @@ __device__ __forceinline__ void face_jac_q(const JacSrc& js, int p, TL* lm, TL* 
 // compute them again); fst non-null: those other faces' two states are
 // stored instead
 // ([8][nProc], left then right) for the ADI kernels to compute them from
-template <int ORDER, typename TL, bool IMPL, bool FASTJ>
+// STAGED: the workgroup's load_state rows staged in LDS (stage_range)
+template <int ORDER, typename TL, bool IMPL, bool FASTJ, bool STAGED>
 __global__ __launch_bounds__(kBlock) void k_rhs_faces(FaceTab ft, Consts c, Live lv, const double* __restrict__ q,
                                                       double* __restrict__ flux, double* __restrict__ sr,
                                                       TL* __restrict__ ff, int ff_from, double* __restrict__ fst,
                                                       int* __restrict__ nanflag, int jac_to = 0) {
-  const int p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int pf = xcd_block(blockIdx.x, gridDim.x) * blockDim.x;
+  const int p = pf + threadIdx.x;
+  __shared__ double stg[STAGED ? 4 : 1][STAGED ? kStageSlots : 1];
+  int S0 = 0;
+  if constexpr (STAGED) {
+    const int2 R = stage_range<ORDER>(ft, c.nProc, pf);
+    S0 = R.x;
+    for (int i = threadIdx.x; i <= R.y - R.x; i += blockDim.x) {
+      const double4 a = load_state(ft.src[S0 + i], q, ft.refmtx, ft.qinf);
+      stg[0][i] = a.x; stg[1][i] = a.y; stg[2][i] = a.z; stg[3][i] = a.w;
+    }
+    __syncthreads();
+  }
   if (p >= c.nProc || (lv.face && !lv.face[p])) return;
   const int code = ft.pslot[p];
   double left[4], right[4];
-  face_states<ORDER>(ft, q, c.eps, c.qt, code, left, right);
+  if constexpr (STAGED)
+    face_states_from<ORDER>(
+        [&](int slot, double* o) __attribute__((always_inline)) {
+#pragma unroll
+          for (int k = 0; k < 4; k++) o[k] = stg[k][slot - S0];
+        },
+        c.eps, c.qt, code, left, right);
+  else
+    face_states<ORDER>(ft, q, c.eps, c.qt, code, left, right);
   const double2 n = *reinterpret_cast<const double2*>(ft.nvec + 2 * (size_t)p);
   double fl[4], s_r;
   flux_roe2d(n.x, n.y, left, right, c.gamma, fl, &s_r);
@@ -2576,23 +2628,55 @@ __global__ __launch_bounds__(2 * MAXH) __attribute__((amdgpu_waves_per_eu(3))) v
 // the states k_rhs_faces used (the same q), instead of read back from ff: 1.1
 // kflop per face in place of 32 loads.  Only for the fused iteration, where ff
 // is k_rhs_faces' output for the current q.
-template <int ORDER, typename TL, bool FASTJ, bool RECOMP>
+// STAGED: the workgroup's load_state (where read) and load_dstate rows staged
+// in LDS (stage_range)
+template <int ORDER, typename TL, bool FASTJ, bool RECOMP, bool STAGED>
 __global__ __launch_bounds__(kBlock) void k_lin_faces(FaceTab ft, Consts c, Live lv, const double* __restrict__ q,
                                                       const double* __restrict__ dq, const TL* __restrict__ ff,
                                                       double* __restrict__ lflux) {
-  const int p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  constexpr bool QROWS = ORDER != 1 || RECOMP;  // the stencil's states are read
+  const int pf = xcd_block(blockIdx.x, gridDim.x) * blockDim.x;
+  const int p = pf + threadIdx.x;
+  // rows 0-3: load_dstate; 4-7: load_state
+  __shared__ double stg[STAGED ? (QROWS ? 8 : 4) : 1][STAGED ? kStageSlots : 1];
+  int S0 = 0;
+  if constexpr (STAGED) {
+    const int2 R = stage_range<ORDER>(ft, c.nProc, pf);
+    S0 = R.x;
+    for (int i = threadIdx.x; i <= R.y - R.x; i += blockDim.x) {
+      const int2 e = ft.src[S0 + i];
+      const double4 b = load_dstate(e, dq, ft.refmtx);
+      stg[0][i] = b.x; stg[1][i] = b.y; stg[2][i] = b.z; stg[3][i] = b.w;
+      if constexpr (QROWS) {
+        const double4 a = load_state(e, q, ft.refmtx, ft.qinf);
+        stg[QROWS ? 4 : 0][i] = a.x; stg[QROWS ? 5 : 1][i] = a.y;
+        stg[QROWS ? 6 : 2][i] = a.z; stg[QROWS ? 7 : 3][i] = a.w;
+      }
+    }
+    __syncthreads();
+  }
   if (p >= c.nProc || (lv.face && !lv.face[p])) return;
   const int code = ft.pslot[p];
   const int s = code & kSlotMask;
+  // a slot's load_dstate (w 0) or load_state (w 1), staged or gathered
+  auto row = [&](int slot, int w, double* o) __attribute__((always_inline)) {
+    if constexpr (STAGED) {
+#pragma unroll
+      for (int k = 0; k < 4; k++) o[k] = stg[(QROWS && w) ? 4 + k : k][slot - S0];
+    } else {
+      const int2 e = ft.src[slot];
+      put4(o, w ? load_state(e, q, ft.refmtx, ft.qinf) : load_dstate(e, dq, ft.refmtx));
+    }
+  };
   double dl[4], dr[4];
   double left[4], right[4];  // the face states (RECOMP)
   if (code & kOrphanSlot) {
     // orphan face: the full product of the two cells' dq
-    put4(dl, load_dstate(ft.src[s], dq, ft.refmtx));
-    put4(dr, load_dstate(ft.src[s + 1], dq, ft.refmtx));
+    row(s, 0, dl);
+    row(s + 1, 0, dr);
     if (RECOMP) {
-      put4(left, load_state(ft.src[s], q, ft.refmtx, ft.qinf));
-      put4(right, load_state(ft.src[s + 1], q, ft.refmtx, ft.qinf));
+      row(s, 1, left);
+      row(s + 1, 1, right);
     }
   } else {
     const bool flip = (code >> 30) & 1;
@@ -2601,9 +2685,8 @@ __global__ __launch_bounds__(kBlock) void k_lin_faces(FaceTab ft, Consts c, Live
     double f[6][4], d[6][4];
 #pragma unroll
     for (int j = lo; j <= hi; j++) {
-      const int2 e = ft.src[s - 2 + j];
-      if (ORDER != 1 || RECOMP) put4(f[j], load_state(e, q, ft.refmtx, ft.qinf));
-      put4(d[j], load_dstate(e, dq, ft.refmtx));
+      if (QROWS) row(s - 2 + j, 1, f[j]);
+      row(s - 2 + j, 0, d[j]);
     }
     double dln[4], drn1[4];
     reconstruct_dface<ORDER, FASTJ>(f, d, c.eps, c.qt, dln, drn1);
@@ -2752,21 +2835,10 @@ __device__ __forceinline__ void face_states_sum(const FaceTab& ft, const double*
   }
 }
 
-// slots a face's stencil reads, relative to its own slot s: s-2+j for j in
-// [lo, hi] of reconstruct_face's table (orphan faces: s and s+1)
-constexpr __host__ __device__ int stencil_lo(int order) { return order == 5 ? -2 : (order == 3 ? -1 : 0); }
-constexpr __host__ __device__ int stencil_hi(int order) { return order == 5 ? 3 : (order == 3 ? 2 : 1); }
-// LDS capacity, in slots, of the staged fused face kernel
-constexpr int kStageSlots = 520;
-
 // k_lin_faces<ORDER, TL, FASTJ, true> on (q, dq), then k_rhs_faces<ORDER, TL,
 // true, FASTJ> on q + dq (ff_from, fst as there).
-// STAGED: the workgroup's stencil rows are staged in LDS first.  Slots grow
-// with the processed face (chains in order, orphan faces after them), so the
-// stencils of a workgroup's 256 faces lie in one range of slots; each slot's
-// load_state, load_dstate and load_state_sum is formed once into LDS and the
-// faces read them there instead of each gathering them from global memory.
-// The host launches it only where every workgroup's range fits kStageSlots.
+// STAGED: the workgroup's load_state, load_dstate and load_state_sum rows
+// staged in LDS (stage_range)
 
 template <int ORDER, typename TL, bool FASTJ, bool STAGED>
 __global__ __launch_bounds__(kBlock) void k_lin_rhs_faces(FaceTab ft, Consts c, Live lv, const double* __restrict__ q,
@@ -2780,10 +2852,9 @@ __global__ __launch_bounds__(kBlock) void k_lin_rhs_faces(FaceTab ft, Consts c, 
   __shared__ double stg[STAGED ? 12 : 1][STAGED ? kStageSlots : 1];
   int S0 = 0;
   if constexpr (STAGED) {
-    const int pl = min(pf + (int)blockDim.x - 1, c.nProc - 1);
-    S0 = max((ft.pslot[pf] & kSlotMask) + stencil_lo(ORDER), 0);
-    const int S1 = min((ft.pslot[pl] & kSlotMask) + stencil_hi(ORDER), ft.nslot - 1);
-    for (int i = threadIdx.x; i <= S1 - S0; i += blockDim.x) {
+    const int2 R = stage_range<ORDER>(ft, c.nProc, pf);
+    S0 = R.x;
+    for (int i = threadIdx.x; i <= R.y - R.x; i += blockDim.x) {
       const int2 e = ft.src[S0 + i];
       const double4 a = load_state(e, q, ft.refmtx, ft.qinf);
       const double4 b = load_dstate(e, dq, ft.refmtx);
@@ -2868,24 +2939,8 @@ __global__ __launch_bounds__(kBlock) void k_lin_rhs_faces(FaceTab ft, Consts c, 
   double left[4], right[4];
   if constexpr (STAGED) {
     // face_states_sum on the staged rows
-    if (code & kOrphanSlot) {
-      row(s, 2, left);
-      row(s + 1, 2, right);
-    } else {
-      const bool flip = (code >> 30) & 1;
-      constexpr int lo = (ORDER == 5) ? 0 : (ORDER == 3 ? 1 : 2);
-      constexpr int hi = (ORDER == 5) ? 5 : (ORDER == 3 ? 4 : 3);
-      double f[6][4];
-#pragma unroll
-      for (int j = lo; j <= hi; j++) row(s - 2 + j, 2, f[j]);
-      double qln[4], qrn1[4];
-      reconstruct_face<ORDER>(f, c.eps, c.qt, qln, qrn1);
-#pragma unroll
-      for (int k = 0; k < 4; k++) {
-        left[k] = flip ? qln[k] : qrn1[k];
-        right[k] = flip ? qrn1[k] : qln[k];
-      }
-    }
+    face_states_from<ORDER>([&](int slot, double* o) __attribute__((always_inline)) { row(slot, 2, o); }, c.eps,
+                            c.qt, code, left, right);
   } else {
     face_states_sum<ORDER>(ft, q, dq, c.eps, c.qt, code, left, right);
   }
@@ -3425,20 +3480,24 @@ int launch_rhs_faces(hamstr_ctx* x, bool implicit, int ff_from = 0, double* fst 
   const FaceTab ft = x->ftab();
   dim3 g(nblk(x->nProc)), b(kBlock);
 #define RHSF(O, T, I, F)                                                                                         \
-  hipLaunchKernelGGL((k_rhs_faces<O, T, I, F>), g, b, 0, x->stream, ft, x->c, x->live(), x->q, x->flux, x->sr, (T*)x->ff, \
-                     ff_from, fst, x->nanflag, jac_to)
+  if (x->stage_faces)                                                                                          \
+    hipLaunchKernelGGL((k_rhs_faces<O, T, I, F, true>), g, b, 0, x->stream, ft, x->c, x->live(), x->q, x->flux, \
+                       x->sr, (T*)x->ff, ff_from, fst, x->nanflag, jac_to);                                     \
+  else                                                                                                         \
+    hipLaunchKernelGGL((k_rhs_faces<O, T, I, F, false>), g, b, 0, x->stream, ft, x->c, x->live(), x->q, x->flux,\
+                       x->sr, (T*)x->ff, ff_from, fst, x->nanflag, jac_to)
 #define RHSO(T, I, F) \
-  if (o == 1) RHSF(1, T, I, F); else if (o == 3) RHSF(3, T, I, F); else RHSF(5, T, I, F)
+  if (o == 1) { RHSF(1, T, I, F); } else if (o == 3) { RHSF(3, T, I, F); } else { RHSF(5, T, I, F); }
   const int o = x->prm.order;
   const bool fj = x->prm.adi_mode == HAMSTR_ADI_FAST;
   if (x->lhs32()) {
-    if (!implicit) RHSO(float, false, false);
-    else if (fj) RHSO(float, true, true);
-    else RHSO(float, true, false);
+    if (!implicit) { RHSO(float, false, false); }
+    else if (fj) { RHSO(float, true, true); }
+    else { RHSO(float, true, false); }
   } else {
-    if (!implicit) RHSO(double, false, false);
-    else if (fj) RHSO(double, true, true);
-    else RHSO(double, true, false);
+    if (!implicit) { RHSO(double, false, false); }
+    else if (fj) { RHSO(double, true, true); }
+    else { RHSO(double, true, false); }
   }
 #undef RHSO
 #undef RHSF
@@ -3451,13 +3510,19 @@ int launch_rhs_faces(hamstr_ctx* x, bool implicit, int ff_from = 0, double* fst 
 int launch_lin_faces(hamstr_ctx* x, bool recompute = false) {
   const FaceTab ft = x->ftab();
   dim3 g(nblk(x->nProc)), b(kBlock);
-#define LINF(O, T, F)                                                                                            \
+#define LINF2(O, T, F, S)                                                                                        \
   if (recompute)                                                                                                 \
-    hipLaunchKernelGGL((k_lin_faces<O, T, F, true>), g, b, 0, x->stream, ft, x->c, x->live(), x->q, x->dq,       \
+    hipLaunchKernelGGL((k_lin_faces<O, T, F, true, S>), g, b, 0, x->stream, ft, x->c, x->live(), x->q, x->dq,    \
                        (const T*)x->ff, x->lflux);                                                               \
   else                                                                                                           \
-    hipLaunchKernelGGL((k_lin_faces<O, T, F, false>), g, b, 0, x->stream, ft, x->c, x->live(), x->q, x->dq,      \
+    hipLaunchKernelGGL((k_lin_faces<O, T, F, false, S>), g, b, 0, x->stream, ft, x->c, x->live(), x->q, x->dq,   \
                        (const T*)x->ff, x->lflux)
+#define LINF(O, T, F)          \
+  if (x->stage_faces) {        \
+    LINF2(O, T, F, true);      \
+  } else {                     \
+    LINF2(O, T, F, false);     \
+  }
 #define LINO(T, F) \
   if (o == 1) { LINF(1, T, F); } else if (o == 3) { LINF(3, T, F); } else { LINF(5, T, F); }
   const int o = x->prm.order;
@@ -3469,6 +3534,7 @@ int launch_lin_faces(hamstr_ctx* x, bool recompute = false) {
   }
 #undef LINO
 #undef LINF
+#undef LINF2
   HIPCHK(hipGetLastError());
   return HAMSTR_OK;
 }
@@ -5443,19 +5509,28 @@ int hamstr_autotune(hamstr_ctx* x, int reps, double* best_ms, int cap, int* ngro
       const bool fused = can_fuse(x);
       auto seq = [&](int src, const std::vector<Group>& g, double* ms) -> int {
         x->groups = g;
-        int rc2;
+        int rc2 = 0;
         if (fused) {
           // the source's stored arrays and a valid dq first, then the steady state
           rc2 = enqueue_rhs_sweep0(x, nullptr, nullptr, x->partial, src);
-          if (!rc2)
+          // the least of five timings: sources within a few percent of each
+          // other are told apart by their best, not by one noisy sample
+          for (int k = 0; k < 5 && !rc2; k++) {
+            double t = 0;
             rc2 = timed(
                 [&]() {
                   const int rc3 = launch_lin_rhs(x, src, x->partial, x->partial, nullptr);
                   return rc3 ? rc3 : enqueue_rhs_sweep0(x, nullptr, nullptr, x->partial, src, true);
                 },
-                ms);
+                &t);
+            if (!rc2) *ms = k == 0 ? t : std::min(*ms, t);
+          }
         } else {
-          rc2 = timed([&]() { return enqueue_rhs_sweep0(x, nullptr, nullptr, x->partial, src); }, ms);
+          for (int k = 0; k < 5 && !rc2; k++) {
+            double t = 0;
+            rc2 = timed([&]() { return enqueue_rhs_sweep0(x, nullptr, nullptr, x->partial, src); }, &t);
+            if (!rc2) *ms = k == 0 ? t : std::min(*ms, t);
+          }
         }
         x->groups = orig;
         return rc2;

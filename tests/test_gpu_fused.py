@@ -189,10 +189,12 @@ def _run(hip, mesh, order, lhs, staged, steps=24, graph=True):
     return out
 
 
+@pytest.mark.parametrize("graph", [True, False])
 @pytest.mark.parametrize("order", [1, 3, 5])
 @pytest.mark.parametrize("lhs", ["fp64", "fp32"])
-def test_staged_faces_equal_gathered(hip, order, lhs):
-    """The fused face kernel with each workgroup's stencil rows staged in LDS
+def test_staged_faces_equal_gathered(hip, order, lhs, graph):
+    """The face kernels (fused in graphs, k_rhs_faces and k_lin_faces one
+    iteration at a time) with each workgroup's stencil rows staged in LDS
     (the default on an O-grid) and with every face gathering its own give
     the same iterations bit for bit."""
     from hamstr_oop_amd import meshgen
@@ -201,9 +203,9 @@ def test_staged_faces_equal_gathered(hip, order, lhs):
     s = hip.Solver(mesh, 0.5, 2.0, order, 10.0, lhs=lm, adi_mode=hip.ADI_FAST)
     assert hip.face_staging(s)
     s.close()
-    print(f"staged/gathered order={order} {lhs}", flush=True)
-    a = _run(hip, mesh, order, lm, True)
-    b = _run(hip, mesh, order, lm, False)
+    print(f"staged/gathered order={order} {lhs} graph={graph}", flush=True)
+    a = _run(hip, mesh, order, lm, True, graph=graph)
+    b = _run(hip, mesh, order, lm, False, graph=graph)
     for x, y, what in zip(a, b, ("q", "r", "sigma", "r0", "L2 history", "linear L2 history")):
         _eq(np.asarray(x), np.asarray(y), what)
 
