@@ -144,12 +144,13 @@ def counter_summary(args, fetch_dir, write_dir, trace_dir, timed=20):
     # states_lines phase 0 reads stored ones
     jq1 = "true" if src in ("states", "q", "states_lines") else "false"
     jq0 = "true" if src in ("states", "q") else "false"
-    o, t = args.order, "double"
+    o, t = args.order, "double"  # the face kernels staged (the default on this mesh)
     eager = {"adi_phase1": f"k_adi_fast_half2<{t}, 384, {jq1}>", "adi_phase0": f"k_adi_fast<{t}, 1, 512, {jq0}>",
-             "lin_faces": f"k_lin_faces<{o}, {t}, true, true>", "rhs_faces": f"k_rhs_faces<{o}, {t}, true, true>",
+             "lin_faces": f"k_lin_faces<{o}, {t}, true, true, true>",
+             "rhs_faces": f"k_rhs_faces<{o}, {t}, true, true, true>",
              "lin_cells": "k_lin_cells<true>", "rhs_cells": "k_rhs_cells<true>", "norms": "k_norm_slots"}
     graph = {"adi_phase1": (eager["adi_phase1"], 2 * timed, timed), "adi_phase0": (eager["adi_phase0"], 2 * timed, timed),
-             "lin_rhs_faces": (f"k_lin_rhs_faces<{o}, {t}, true>", timed - 1, 0),
+             "lin_rhs_faces": (f"k_lin_rhs_faces<{o}, {t}, true, true>", timed - 1, 0),
              "lin_rhs_cells": ("k_lin_rhs_cells", timed - 1, 0)}
 
     def stage(name, a, b):

@@ -342,6 +342,9 @@ __global__ __launch_bounds__(kBlock) void k_rhs_faces(FaceTab ft, Consts c, Live
   __shared__ double stg[STAGED ? 4 : 1][STAGED ? kStageSlots : 1];
   int S0 = 0;
   if constexpr (STAGED) {
+    // a workgroup whose faces are all retired (converged cases of a batch)
+    // stages nothing
+    if (!__syncthreads_or(p < c.nProc && (!lv.face || lv.face[p]))) return;
     const int2 R = stage_range<ORDER>(ft, c.nProc, pf);
     S0 = R.x;
     for (int i = threadIdx.x; i <= R.y - R.x; i += blockDim.x) {
@@ -2641,6 +2644,9 @@ __global__ __launch_bounds__(kBlock) void k_lin_faces(FaceTab ft, Consts c, Live
   __shared__ double stg[STAGED ? (QROWS ? 8 : 4) : 1][STAGED ? kStageSlots : 1];
   int S0 = 0;
   if constexpr (STAGED) {
+    // a workgroup whose faces are all retired (converged cases of a batch)
+    // stages nothing
+    if (!__syncthreads_or(p < c.nProc && (!lv.face || lv.face[p]))) return;
     const int2 R = stage_range<ORDER>(ft, c.nProc, pf);
     S0 = R.x;
     for (int i = threadIdx.x; i <= R.y - R.x; i += blockDim.x) {
@@ -2852,6 +2858,9 @@ __global__ __launch_bounds__(kBlock) void k_lin_rhs_faces(FaceTab ft, Consts c, 
   __shared__ double stg[STAGED ? 12 : 1][STAGED ? kStageSlots : 1];
   int S0 = 0;
   if constexpr (STAGED) {
+    // a workgroup whose faces are all retired (converged cases of a batch)
+    // stages nothing
+    if (!__syncthreads_or(p < c.nProc && (!lv.face || lv.face[p]))) return;
     const int2 R = stage_range<ORDER>(ft, c.nProc, pf);
     S0 = R.x;
     for (int i = threadIdx.x; i <= R.y - R.x; i += blockDim.x) {
