@@ -243,3 +243,22 @@ def test_staging_refused_where_the_slots_do_not_fit(hip):
     b = _run(hip, mesh, 5, hip.LHS_FP64, None, steps=16, graph=False)
     for x, y, what in zip(a, b, ("q", "r", "sigma", "r0", "L2 history", "linear L2 history")):
         _eq(np.asarray(x), np.asarray(y), what)
+
+
+def test_staged_faces_with_retired_cases(hip):
+    """Cases of a batch that converge retire: workgroups whose faces all
+    belong to them stage nothing and exit, the others stage their whole slot
+    range; every case's iterations and state equal the gathered run's."""
+    from hamstr_oop_amd import meshgen
+    cases = [(meshgen.naca_ogrid(32, 12), 0.8, 1.25), (meshgen.naca_ogrid(64, 16, t=0.14), 0.6, -3.0),
+             (meshgen.naca_ogrid(48, 16, tri_layers=4), 0.3, 1.0)]
+    out = []
+    for staged in (True, False):
+        b = hip.Batch(cases, order=3, cfl=10.0, adi_mode=hip.ADI_FAST)
+        hip.set_face_staging(b, staged)
+        n, iters, _ = b.batch_step(3000, tol=1e-9, check_every=20)
+        out.append((iters.copy(), b.get("q")))
+        b.close()
+    assert (out[0][0] > 0).all() and len(set(out[0][0].tolist())) > 1, out[0][0]
+    _eq(out[0][0], out[1][0], "iterations to tolerance")
+    _eq(out[0][1], out[1][1], "q")
