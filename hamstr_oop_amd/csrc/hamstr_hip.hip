@@ -224,7 +224,10 @@ __device__ __forceinline__ void reconstruct_dface(const double (*f)[4], const do
 // [lo, hi] of reconstruct_face's table (orphan faces: s and s+1)
 constexpr __host__ __device__ int stencil_lo(int order) { return order == 5 ? -2 : (order == 3 ? -1 : 0); }
 constexpr __host__ __device__ int stencil_hi(int order) { return order == 5 ? 3 : (order == 3 ? 2 : 1); }
-// LDS capacity, in slots, of the staged face kernels
+// LDS capacity, in slots, of the staged face kernels: 256 orphan faces (two
+// slots each) and their halo take 515; 256 faces on chains of a few dozen
+// cells or more take fewer.  The fused kernel holds 12 FP64 rows per slot
+// (49,920 B), as many workgroups per CU as its 154 VGPRs allow.
 constexpr int kStageSlots = 520;
 
 // Staged face kernels: slots grow with the processed face (chains in order,
